@@ -1,0 +1,51 @@
+"""Mie-series backscatter of a PEC sphere (analytic oracle).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product path.
+
+Restates mie_sphere.py:42-101, 178-183, 231-235 in the e^{+jwt} convention
+(outgoing Riccati-Hankel xi_n = psi_n - j zeta_n):
+    A_n = -psi_n'(x)/xi_n'(x)  (TM: U' = 0),  B_n = -psi_n(x)/xi_n(x)  (TE: U = 0)
+    sigma = lambda^2/(4 pi) |sum_{n>=1} (-1)^n (2n+1)(A_n - B_n)|^2
+Riccati functions are built from scipy's spherical Bessel routines.
+"""
+
+import math
+
+import numpy as np
+from scipy import special
+
+C0 = 299_792_458.0
+
+
+def _nmax(x: float, pad: int = 12) -> int:
+    """Wiscombe-style truncation (mie_sphere.py:74-77)."""
+    x = abs(x)
+    return max(6, int(math.ceil(x + 4.05 * x ** (1.0 / 3.0))) + pad)
+
+
+def _riccati(nmax: int, x: float):
+    n = np.arange(nmax + 1)
+    j = special.spherical_jn(n, x)
+    y = special.spherical_yn(n, x)
+    dj = special.spherical_jn(n, x, derivative=True)
+    dy = special.spherical_yn(n, x, derivative=True)
+    psi, dpsi = x * j, j + x * dj
+    zeta, dzeta = x * y, y + x * dy
+    return psi, dpsi, psi - 1j * zeta, dpsi - 1j * dzeta
+
+
+def pec_sphere_coeffs(x: float, nmax: int):
+    psi, dpsi, xi, dxi = _riccati(nmax, x)
+    return -dpsi / dxi, -psi / xi
+
+
+def sigma_pec_sphere(radius_m: float, freq_hz: float) -> float:
+    """Monostatic RCS (m^2) of a PEC sphere."""
+    k = 2.0 * math.pi * freq_hz / C0
+    x = k * radius_m
+    a, b = pec_sphere_coeffs(x, _nmax(x))
+    n = np.arange(1, len(a))
+    s = np.sum(((-1.0) ** n) * (2 * n + 1) * (a[1:] - b[1:]))
+    lam = 2.0 * math.pi / k
+    return float(lam ** 2 / (4.0 * math.pi) * abs(s) ** 2)
