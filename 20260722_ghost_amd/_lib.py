@@ -1,0 +1,218 @@
+"""ctypes binding of libghost_bor.so (the gfx950 BoR-MoM hot path).
+
+The library is built in-tree next to this file (``__graft_entry__.build()``).
+There is no CPU fallback: if the shared object is missing or cannot be
+loaded, every call raises GhostLibraryError.
+"""
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libghost_bor.so")
+
+GHOST_FP64 = 0
+GHOST_MIXED = 1
+GHOST_UNMASKED = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ci = ctypes.c_int
+_cd = ctypes.c_double
+_vp = ctypes.c_void_p
+
+# (name, restype, argtypes) of every symbol include/ghost_bor.h declares
+SIGNATURES = [
+    ("ghost_last_error", ctypes.c_char_p, []),
+    ("sample_g", None, [_ci, _ci, _ci, _dp, _dp, _dp, _dp, _cd, _dp, _dp]),
+    ("sample_mfie", None, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
+    ("sample_ibc", None, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
+    ("ghost_sample_g", _ci, [_ci, _ci, _ci, _dp, _dp, _dp, _dp, _cd, _dp, _dp]),
+    ("ghost_sample_mfie", _ci, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
+    ("ghost_sample_ibc", _ci, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
+    ("ghost_bor_create", _ci, [ctypes.POINTER(_vp), _ci, _ci]),
+    ("ghost_bor_destroy", _ci, [_vp]),
+    ("ghost_bor_set_residual", _ci, [_vp, _ci]),
+    ("ghost_bor_add_problem", _ci, [_vp, _dp, _ci, _cd, _ci, _ci, _ci, _cd, _cd, _dp, _ci, _ci, _ip]),
+    ("ghost_bor_build", _ci, [_vp]),
+    ("ghost_bor_fill", _ci, [_vp]),
+    ("ghost_bor_solve", _ci, [_vp, _ci]),
+    ("ghost_bor_farfield", _ci, [_vp]),
+    ("ghost_bor_run", _ci, [_vp, _ci]),
+    ("ghost_bor_sync", _ci, [_vp]),
+    ("ghost_bor_get_dF", _ci, [_vp, _ci, _dp]),
+    ("ghost_bor_residual", _ci, [_vp, _dp]),
+    ("ghost_bor_system_info", _ci, [_vp, _ci, _ci, _ip, _ip, _ip, _ip]),
+    ("ghost_bor_num_systems", _ci, [_vp, _ci, _ip, _ip]),
+    ("ghost_bor_get_system", _ci, [_vp, _ci, _ci, _ci, _dp, _dp]),
+    ("ghost_bor_info", _ci, [_vp, _ip]),
+    ("ghost_bor_timings", _ci, [_vp, _fp]),
+    ("ghost_bor_set_profile", _ci, [_vp, _ci]),
+    ("ghost_bor_kernel_count", _ci, []),
+    ("ghost_bor_kernel_name", ctypes.c_char_p, [_ci]),
+    ("ghost_bor_kernel_times", _ci, [_vp, _dp, _ip, _dp]),
+]
+
+
+class GhostLibraryError(RuntimeError):
+    """libghost_bor is missing, failed to load, or a call returned an error."""
+
+
+_LIB = None
+_LOCK = threading.Lock()
+
+
+def load(path: str = LIB_PATH):
+    """Load (once) and return the ctypes library; raises if it is absent."""
+    global _LIB
+    with _LOCK:
+        if _LIB is not None:
+            return _LIB
+        if not os.path.exists(path):
+            raise GhostLibraryError(
+                f"{path} not found: build it first with __graft_entry__.build() "
+                "(there is no CPU fallback for the BoR hot path)")
+        try:
+            lib = ctypes.CDLL(path)
+        except OSError as exc:
+            raise GhostLibraryError(f"cannot load {path}: {exc}") from exc
+        for name, res, args in SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+        return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().ghost_last_error().decode(errors="replace")
+        if rc == 4:
+            raise MemoryError(msg)
+        if rc == 1:
+            raise ValueError(msg)
+        raise GhostLibraryError(f"libghost_bor error {rc}: {msg}")
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+class Handle:
+    """Owner of one ghost_bor handle (a batch of problems on the current device)."""
+
+    def __init__(self, precision: int = GHOST_FP64, nb: int = 64, residual: bool = True):
+        self.lib = load()
+        self._h = _vp()
+        check(self.lib.ghost_bor_create(ctypes.byref(self._h), int(precision), int(nb)))
+        check(self.lib.ghost_bor_set_residual(self._h, 1 if residual else 0))
+        self.precision = precision
+        self.problems = []
+
+    def close(self):
+        if self._h:
+            self.lib.ghost_bor_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_problem(self, nodes, k, m_max, n_xi_efie, n_xi_bracket, c_efie, c_mfie, thetas_deg,
+                    unmasked: bool = False) -> int:
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        th = np.ascontiguousarray(np.atleast_1d(thetas_deg), dtype=np.float64)
+        idx = ctypes.c_int(-1)
+        check(self.lib.ghost_bor_add_problem(self._h, dptr(nodes), int(nodes.shape[0]), float(k), int(m_max),
+                                             int(n_xi_efie), int(n_xi_bracket), float(c_efie), float(c_mfie),
+                                             dptr(th), int(th.size), GHOST_UNMASKED if unmasked else 0,
+                                             ctypes.byref(idx)))
+        self.problems.append((int(m_max), int(th.size)))
+        return idx.value
+
+    def build(self):
+        check(self.lib.ghost_bor_build(self._h))
+
+    def fill(self):
+        check(self.lib.ghost_bor_fill(self._h))
+
+    def solve(self, refine_iters: int = 0):
+        check(self.lib.ghost_bor_solve(self._h, int(refine_iters)))
+
+    def farfield(self):
+        check(self.lib.ghost_bor_farfield(self._h))
+
+    def run(self, refine_iters: int = 0):
+        check(self.lib.ghost_bor_run(self._h, int(refine_iters)))
+
+    def sync(self):
+        check(self.lib.ghost_bor_sync(self._h))
+
+    def dF(self, prob: int) -> np.ndarray:
+        """complex [2 m_max + 1, 2 (VV, HH), n_theta]"""
+        m_max, nth = self.problems[prob]
+        out = np.zeros((2 * m_max + 1, 2, nth, 2), dtype=np.float64)
+        check(self.lib.ghost_bor_get_dF(self._h, int(prob), dptr(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def residuals(self) -> np.ndarray:
+        n = sum(2 * m + 1 for m, _ in self.problems)
+        out = np.zeros(n, dtype=np.float64)
+        check(self.lib.ghost_bor_residual(self._h, dptr(out)))
+        return out
+
+    def system_info(self, prob: int, m: int):
+        n, nt, t0, f0 = (ctypes.c_int() for _ in range(4))
+        check(self.lib.ghost_bor_system_info(self._h, int(prob), int(m), ctypes.byref(n), ctypes.byref(nt),
+                                             ctypes.byref(t0), ctypes.byref(f0)))
+        return n.value, nt.value, t0.value, f0.value
+
+    def system(self, prob: int, m: int):
+        """(A [n, n], B [n, nrhs]) as assembled; call after fill and before solve."""
+        n = self.system_info(prob, m)[0]
+        nrhs = 2 * self.problems[prob][1]
+        A = np.zeros((n, n, 2))
+        B = np.zeros((n, nrhs, 2))
+        check(self.lib.ghost_bor_get_system(self._h, int(prob), int(m), 0, dptr(A), dptr(B)))
+        return A[..., 0] + 1j * A[..., 1], B[..., 0] + 1j * B[..., 1]
+
+    def solution(self, prob: int, m: int) -> np.ndarray:
+        n = self.system_info(prob, m)[0]
+        nrhs = 2 * self.problems[prob][1]
+        X = np.zeros((n, nrhs, 2))
+        check(self.lib.ghost_bor_get_system(self._h, int(prob), int(m), 1, dptr(X), None))
+        return X[..., 0] + 1j * X[..., 1]
+
+    def zero_pivots(self) -> np.ndarray:
+        n = sum(2 * m + 1 for m, _ in self.problems)
+        out = np.zeros(n, dtype=np.int32)
+        check(self.lib.ghost_bor_info(self._h, iptr(out)))
+        return out
+
+    def set_profile(self, on: bool = True):
+        check(self.lib.ghost_bor_set_profile(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        '''{family: (ms, launches, algorithmic flops)} since set_profile.'''
+        n = self.lib.ghost_bor_kernel_count()
+        ms = np.zeros(n)
+        cnt = np.zeros(n, dtype=np.int32)
+        fl = np.zeros(n)
+        check(self.lib.ghost_bor_kernel_times(self._h, dptr(ms), iptr(cnt), dptr(fl)))
+        return {self.lib.ghost_bor_kernel_name(i).decode(): (float(ms[i]), int(cnt[i]), float(fl[i]))
+                for i in range(n)}
+
+    def timings(self):
+        out = np.zeros(3, dtype=np.float32)
+        check(self.lib.ghost_bor_timings(self._h, out.ctypes.data_as(_fp)))
+        return {"fill_ms": float(out[0]), "solve_ms": float(out[1]), "farfield_ms": float(out[2])}

@@ -1,0 +1,1450 @@
+// Host side of libghost_bor: C ABI, problem set-up, device buffers and the
+// launch schedule of the BoR-MoM hot path (fill -> batched LU/solve -> far field).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fill.hip"
+#include "excite.hip"
+#include "lu.hip"
+#include "../../include/ghost_bor.h"
+
+using namespace ghost;
+
+// ─────────────────────────────────────────────────────────────────────────────
+// errors
+// ─────────────────────────────────────────────────────────────────────────────
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess)                                                           \
+      return fail(GHOST_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                              \
+  } while (0)
+
+#define LAUNCHCHK()                                                                  \
+  do {                                                                               \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess)                                                            \
+      return fail(GHOST_ERR_HIP, "kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                               \
+  } while (0)
+
+extern "C" const char* ghost_last_error(void) { return g_err.c_str(); }
+
+// ─────────────────────────────────────────────────────────────────────────────
+// quadrature rules
+// ─────────────────────────────────────────────────────────────────────────────
+
+// Gauss-Legendre on [-1, 1] by Newton iteration on P_n, ascending nodes.
+static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int j = 2; j <= n; ++j) {
+        const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+        p0 = p1;
+        p1 = p2;
+      }
+      double pn = n == 1 ? z : p1;
+      double pm = n == 1 ? 1.0 : p0;
+      if (n == 1) { pn = z; pm = 1.0; }
+      pp = n * (z * pn - pm) / (z * z - 1.0);
+      const double dz = pn / pp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    // recompute derivative at the converged node
+    double p0 = 1.0, p1 = z;
+    for (int j = 2; j <= n; ++j) {
+      const double p2 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p0) / j;
+      p0 = p1;
+      p1 = p2;
+    }
+    pp = n * (z * p1 - p0) / (z * z - 1.0);
+    const double wt = 2.0 / ((1.0 - z * z) * pp * pp);
+    x[n - 1 - i] = z;
+    x[i] = -z;
+    w[i] = wt;
+    w[n - 1 - i] = wt;
+  }
+  if (n & 1) x[n / 2] = 0.0;
+}
+
+// Graded cells of [0,1]^2 toward s = s' (kind 0) or a shared corner (kind 1:
+// s = 1, s' = 0 for f = e + 1; kind 2: s = 0, s' = 1 for f = e - 1)
+// (bor_solver.py:54-79), depth 4.
+static void graded_cells(int kind, double a0, double a1, double b0, double b1, int lev,
+                         std::vector<double>& cells) {
+  bool hit;
+  if (kind == 0) hit = !(a1 <= b0 || b1 <= a0);
+  else {
+    const double ca = kind == 1 ? 1.0 : 0.0, cb = kind == 1 ? 0.0 : 1.0;
+    hit = a0 <= ca && ca <= a1 && b0 <= cb && cb <= b1;
+  }
+  if (lev >= 4 || !hit) {
+    cells.push_back(a0); cells.push_back(a1); cells.push_back(b0); cells.push_back(b1);
+    return;
+  }
+  const double am = 0.5 * (a0 + a1), bm = 0.5 * (b0 + b1);
+  graded_cells(kind, a0, am, b0, bm, lev + 1, cells);
+  graded_cells(kind, a0, am, bm, b1, lev + 1, cells);
+  graded_cells(kind, am, a1, b0, bm, lev + 1, cells);
+  graded_cells(kind, am, a1, bm, b1, lev + 1, cells);
+}
+
+// Tensor Gauss points on each cell: order 4 on the test axis, 5 on the source
+// axis (bor_solver.py:85-110).
+static void cell_points(int kind, std::vector<double>& s, std::vector<double>& sp, std::vector<double>& w) {
+  std::vector<double> cells;
+  graded_cells(kind, 0.0, 1.0, 0.0, 1.0, 0, cells);
+  std::vector<double> x4, w4, x5, w5;
+  gauss_legendre(4, x4, w4);
+  gauss_legendre(5, x5, w5);
+  for (size_t c = 0; c < cells.size(); c += 4) {
+    const double a0 = cells[c], a1 = cells[c + 1], b0 = cells[c + 2], b1 = cells[c + 3];
+    const double ha = a1 - a0, hb = b1 - b0;
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 5; ++j) {
+        s.push_back(a0 + 0.5 * (x4[i] + 1.0) * ha);
+        sp.push_back(b0 + 0.5 * (x5[j] + 1.0) * hb);
+        w.push_back((0.5 * w4[i] * ha) * (0.5 * w5[j] * hb));
+      }
+  }
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// state
+// ─────────────────────────────────────────────────────────────────────────────
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct Problem {
+  int Nn = 0, Ne = 0, P = 0, M = 0, nE = 0, nB = 0, ntheta = 0, nrhs = 0, nmodes = 0;
+  double k = 0, cE = 0, cM = 0;
+  std::vector<double> nodes, theta;
+  bool axis0 = false, axis1 = false;
+  int sys0 = 0;
+  size_t dF_off = 0;             // first element of this problem's far-field block
+  // host geometry
+  std::vector<double> pts;  // 9 arrays of P
+  std::vector<double> L, trho, tz;
+  int npp = 0;
+  std::vector<NearPairInfo> pairs;
+  std::vector<int2> batches;
+  std::vector<double> tail_u, tail_w;
+  // device
+  double* d_pts = nullptr;
+  double* d_nodes = nullptr;
+  double* d_elem = nullptr;      // L, trho, tz
+  double* d_tabs = nullptr;      // sin2E | cosE | cxB | sxB | cosB | sinB
+  double* d_theta = nullptr;
+  double* d_J = nullptr;
+  double* d_cells = nullptr;     // s | sp | w
+  double* d_core = nullptr;      // u | w
+  double* d_tail = nullptr;      // u | w
+  NearPairInfo* d_pairs = nullptr;
+  int2* d_batches = nullptr;
+  double2* d_NP = nullptr;
+  double2* d_NB = nullptr;
+  void* d_zfar = nullptr;
+  int cell_off[3] = {0, 0, 0}, cell_cnt[3] = {0, 0, 0};
+  size_t tab_off[6] = {0, 0, 0, 0, 0, 0};
+};
+
+struct Launch {  // one batched kernel launch of the solve schedule
+  int kind;      // see enum below
+  int ops;       // index into the op arrays
+  int nops;      // number of systems (grid.y)
+  int gx;        // grid.x
+  int arg;       // extra (RPT for panels)
+  double flops = 0.0;  // algorithmic real flops of the launch
+};
+
+// kernel families timed by the optional per-launch HIP-event profile
+enum {
+  K_ZERO, K_NEAR_MODAL, K_NEAR_BLOCKS, K_FAR_FILL, K_ASSEMBLE, K_BESSEL, K_RHS, K_PANEL, K_SWAP,
+  K_TRI_INV, K_GEMM, K_GEMM64, K_VEC, K_FARFIELD, K_COUNT
+};
+static const char* kKernelNames[K_COUNT] = {"zero", "near_modal", "near_blocks", "far_fill", "assemble",
+                                            "bessel", "rhs", "lu_panel", "lu_swap", "tri_inv", "gemm",
+                                            "gemm_fp64_residual", "vec", "farfield"};
+
+enum { L_PANEL, L_SWAP, L_TRIL, L_TRIU, L_GEMM_AUG, L_GEMM_AUG_W, L_GEMM_Y, L_GEMM_Y_W, L_GEMM_Y_LU };
+
+struct ghost_bor {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = GHOST_FP64;
+  int nb = 64;
+  int residual = 1;
+  bool built = false;
+  std::vector<Problem> probs;
+  std::vector<SysInfo> sys;
+  SysInfo* d_sys = nullptr;
+  size_t a_elems = 0, x_elems = 0, a64_elems = 0;
+  int piv_elems = 0;
+  int max_blocks = 0;
+  void* d_A = nullptr;          // augmented systems, FP64 or FP32 complex
+  double2* d_A64 = nullptr;     // FP64 copies (residual / refinement)
+  double2* d_B64 = nullptr;
+  double2* d_X64 = nullptr;
+  double2* d_R64 = nullptr;
+  float2* d_Y32 = nullptr;
+  int* d_piv = nullptr;
+  int* d_perm = nullptr;
+  int* d_info = nullptr;
+  void* d_W = nullptr;          // I - T^{-1} per (system, block, lower/upper)
+  double2* d_dF = nullptr;
+  double2* d_partial = nullptr;
+  double* d_red = nullptr;
+  int ff_chunk = 64;
+  size_t dF_elems = 0, partial_elems = 0;
+  // schedule
+  std::vector<PanelOp> h_panel;
+  std::vector<SwapOp> h_swap;
+  std::vector<TrsmOp> h_trsm;
+  std::vector<GemmOp> h_gemm;
+  std::vector<VecOp> h_vec;
+  PanelOp* d_panel = nullptr;
+  SwapOp* d_swap = nullptr;
+  TrsmOp* d_trsm = nullptr;
+  GemmOp* d_gemm = nullptr;
+  VecOp* d_vec = nullptr;
+  std::vector<Launch> lu_sched, back_sched, fwdY_sched, backY_sched;
+  int triu_ops = 0, triu_n = 0;
+  int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
+  int vec_all = 0, vec_aug = 0;          // VecOps over all systems: x-blocks / augmented RHS -> x-blocks
+  int rpt = 1;
+  std::vector<double> res_norm;      // per system ||AX - B||_F / ||B||_F
+  std::vector<double> refine_hist;   // max over systems of max|R|/max|B| per iteration
+  hipEvent_t ev[8];
+  float t_ms[8] = {0};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int refine_iters = 0;
+  std::vector<DevBuf> allocs;
+  // optional per-launch profile
+  bool profile = false;
+  std::vector<hipEvent_t> pev;
+  struct Rec { int kind; double flops; int e0; };
+  std::vector<Rec> recs;
+};
+
+static void prof_begin(ghost_bor* h) {
+  if (!h->profile) return;
+  const size_t need = 2 * (h->recs.size() + 1);
+  while (h->pev.size() < need) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) { h->profile = false; return; }
+    h->pev.push_back(e);
+  }
+  hipEventRecord(h->pev[2 * h->recs.size()], h->stream);
+}
+
+static void prof_end(ghost_bor* h, int kind, double flops) {
+  if (!h->profile) return;
+  const int i = (int)h->recs.size();
+  hipEventRecord(h->pev[2 * i + 1], h->stream);
+  h->recs.push_back(ghost_bor::Rec{kind, flops, 2 * i});
+}
+
+template <typename P> static int dalloc(ghost_bor* h, P** out, size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess)
+    return fail(GHOST_ERR_OOM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  h->allocs.push_back({p, bytes});
+  *out = (P*)p;
+  return 0;
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// C ABI: lifecycle
+// ─────────────────────────────────────────────────────────────────────────────
+
+extern "C" int ghost_bor_create(ghost_bor** out, int precision, int nb) {
+  if (!out) return fail(GHOST_ERR_ARG, "null output handle");
+  if (precision != GHOST_FP64 && precision != GHOST_MIXED)
+    return fail(GHOST_ERR_ARG, "precision must be GHOST_FP64 or GHOST_MIXED");
+  if (nb <= 0) nb = 64;
+  if (nb % 8 != 0 || nb > 64) return fail(GHOST_ERR_ARG, "block size must be a multiple of 8 and <= 64");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  ghost_bor* h = new ghost_bor();
+  h->device = dev;
+  h->precision = precision;
+  h->nb = nb;
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete h; return fail(GHOST_ERR_HIP, "stream create: %s", hipGetErrorString(e)); }
+  for (int i = 0; i < 8; ++i) {
+    e = hipEventCreate(&h->ev[i]);
+    if (e != hipSuccess) { delete h; return fail(GHOST_ERR_HIP, "event create: %s", hipGetErrorString(e)); }
+  }
+  *out = h;
+  return 0;
+}
+
+extern "C" int ghost_bor_destroy(ghost_bor* h) {
+  if (!h) return 0;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  for (auto e : h->pev) hipEventDestroy(e);
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->graph) hipGraphDestroy(h->graph);
+  for (auto& a : h->allocs) hipFree(a.p);
+  for (int i = 0; i < 8; ++i) hipEventDestroy(h->ev[i]);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+extern "C" int ghost_bor_set_residual(ghost_bor* h, int on) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "set_residual must precede build");
+  h->residual = on ? 1 : 0;
+  return 0;
+}
+
+extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, double k, int m_max,
+                                     int n_xi_efie, int n_xi_bracket, double c_efie, double c_mfie,
+                                     const double* thetas_deg, int n_theta, int flags, int* idx) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "problems must be added before build");
+  if (!nodes || Nn < 3) return fail(GHOST_ERR_ARG, "need at least 3 generatrix nodes");
+  if (!(k > 0)) return fail(GHOST_ERR_ARG, "wavenumber must be positive (real k only)");
+  if (m_max < 0 || m_max > 126) return fail(GHOST_ERR_ARG, "m_max must lie in [0, 126]");
+  auto pow2 = [](int v) { return v >= 4 && (v & (v - 1)) == 0; };
+  if (!pow2(n_xi_efie) || !pow2(n_xi_bracket))
+    return fail(GHOST_ERR_ARG, "xi grid sizes must be powers of two >= 4");
+  if (!thetas_deg || n_theta < 1) return fail(GHOST_ERR_ARG, "need at least one aspect angle");
+  Problem p;
+  p.Nn = Nn;
+  p.Ne = Nn - 1;
+  p.P = 4 * p.Ne;
+  p.k = k;
+  p.M = m_max;
+  p.nmodes = 2 * m_max + 1;
+  p.nE = n_xi_efie;
+  p.nB = n_xi_bracket;
+  p.cE = c_efie;
+  p.cM = c_mfie;
+  p.ntheta = n_theta;
+  p.nrhs = 2 * n_theta;
+  p.nodes.assign(nodes, nodes + 2 * Nn);
+  double rmax = 0;
+  for (int i = 0; i < Nn; ++i) {
+    if (p.nodes[2 * i] < -1e-12) return fail(GHOST_ERR_ARG, "generatrix rho must be >= 0");
+    p.nodes[2 * i] = std::max(p.nodes[2 * i], 0.0);
+    rmax = std::max(rmax, p.nodes[2 * i]);
+  }
+  for (int e = 0; e < p.Ne; ++e) {
+    const double dr = p.nodes[2 * e + 2] - p.nodes[2 * e], dz = p.nodes[2 * e + 3] - p.nodes[2 * e + 1];
+    const double L = std::hypot(dr, dz);
+    if (!(L > 0)) return fail(GHOST_ERR_ARG, "generatrix has a zero-length element (%d)", e);
+    p.L.push_back(L);
+    p.trho.push_back(dr / L);
+    p.tz.push_back(dz / L);
+  }
+  const double tol = 1e-12 * std::max(1.0, rmax);  // bor_kernels.py:79-80
+  p.axis0 = p.nodes[0] <= tol;
+  p.axis1 = p.nodes[2 * (Nn - 1)] <= tol;
+  for (int i = 0; i < n_theta; ++i) p.theta.push_back(thetas_deg[i] * M_PI / 180.0);
+  p.sys0 = (int)h->sys.size();
+  // systems m = -M..M
+  for (int m = -m_max; m <= m_max; ++m) {
+    SysInfo s;
+    memset(&s, 0, sizeof(s));
+    s.prob = (int)h->probs.size();
+    s.m = m;
+    // active bases (bor_solver.py:611-622); GHOST_UNMASKED keeps all 2 Nn
+    const bool all = (flags & GHOST_UNMASKED) != 0;
+    const bool t_first = all || (p.axis0 ? (std::abs(m) == 1) : false);
+    const bool t_last = all || (p.axis1 ? (std::abs(m) == 1) : false);
+    const bool f_first = all || !p.axis0, f_last = all || !p.axis1;
+    s.t0 = t_first ? 0 : 1;
+    s.nt = Nn - (t_first ? 0 : 1) - (t_last ? 0 : 1);
+    s.f0 = f_first ? 0 : 1;
+    const int nf = Nn - (f_first ? 0 : 1) - (f_last ? 0 : 1);
+    s.n = s.nt + nf;
+    s.nrhs = p.nrhs;
+    h->sys.push_back(s);
+  }
+  if (idx) *idx = (int)h->probs.size();
+  h->probs.push_back(std::move(p));
+  return 0;
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// build: host tables, device buffers, solve schedule
+// ─────────────────────────────────────────────────────────────────────────────
+
+static int prepare_problem(ghost_bor* h, Problem& p) {
+  const int P = p.P, Ne = p.Ne;
+  // ── Gauss points, order 4 (bor_kernels.py:103-125) ──
+  std::vector<double> x4, w4;
+  gauss_legendre(4, x4, w4);
+  p.pts.assign(9 * (size_t)P, 0.0);
+  double* rho = &p.pts[0 * P]; double* z = &p.pts[1 * P]; double* tr = &p.pts[2 * P];
+  double* tzp = &p.pts[3 * P]; double* w = &p.pts[4 * P]; double* T0 = &p.pts[5 * P];
+  double* T1 = &p.pts[6 * P]; double* D0 = &p.pts[7 * P]; double* D1 = &p.pts[8 * P];
+  for (int e = 0; e < Ne; ++e) {
+    const double r0 = p.nodes[2 * e], z0 = p.nodes[2 * e + 1];
+    const double dr = p.nodes[2 * e + 2] - r0, dz = p.nodes[2 * e + 3] - z0;
+    for (int g = 0; g < 4; ++g) {
+      const int i = 4 * e + g;
+      const double s = 0.5 * (x4[g] + 1.0);
+      rho[i] = r0 + s * dr;
+      z[i] = z0 + s * dz;
+      tr[i] = p.trho[e];
+      tzp[i] = p.tz[e];
+      w[i] = 0.5 * w4[g] * p.L[e];
+      T0[i] = 1.0 - s;
+      T1[i] = s;
+      D0[i] = (dr * (1.0 - s) - rho[i]) / p.L[e];
+      D1[i] = (dr * s + rho[i]) / p.L[e];
+    }
+  }
+  // ── near pairs: cells, tail orders (bor_kernels.py:238-240) ──
+  std::vector<double> cs[3], csp[3], cw[3];
+  for (int kd = 0; kd < 3; ++kd) cell_points(kd, cs[kd], csp[kd], cw[kd]);
+  std::vector<double> cell_all;
+  int off = 0;
+  for (int kd = 0; kd < 3; ++kd) {
+    p.cell_off[kd] = off;
+    p.cell_cnt[kd] = (int)cs[kd].size();
+    off += p.cell_cnt[kd];
+  }
+  std::map<int, int> tail_of;   // n_tail -> offset
+  p.pairs.assign(3 * (size_t)Ne, NearPairInfo{-1, -1, 0, 0, 0, 0});
+  int pp = 0;
+  for (int e = 0; e < Ne; ++e)
+    for (int d = 0; d < 3; ++d) {
+      const int f = e + d - 1;
+      if (f < 0 || f >= Ne) continue;
+      const int kind = (f == e) ? 0 : (f == e + 1 ? 1 : 2);
+      double osc = 0.0;
+      const double r0e = p.nodes[2 * e], dre = p.nodes[2 * e + 2] - r0e;
+      const double r0f = p.nodes[2 * f], drf = p.nodes[2 * f + 2] - r0f;
+      for (size_t q = 0; q < cs[kind].size(); ++q) {
+        const double rp = r0e + cs[kind][q] * dre, rq = r0f + csp[kind][q] * drf;
+        const double rr4 = 4.0 * rp * rq;
+        const double a = std::sqrt(std::max(rr4, 1e-300));
+        osc = std::max(osc, std::fabs(p.k) * a);
+      }
+      const double o = osc / M_PI + (p.M + 2);
+      const int nt = (int)std::min(1024.0, std::max(64.0, std::ceil(4.0 * o)));
+      NearPairInfo pi;
+      pi.e = e;
+      pi.f = f;
+      pi.kind = kind;
+      pi.pp_off = pp;
+      pi.n_tail = nt;
+      if (!tail_of.count(nt)) {
+        tail_of[nt] = (int)p.tail_u.size();
+        std::vector<double> xt, wt;
+        gauss_legendre(nt, xt, wt);
+        for (int i = 0; i < nt; ++i) { p.tail_u.push_back(0.5 * (xt[i] + 1.0)); p.tail_w.push_back(0.5 * wt[i]); }
+      }
+      pi.tail_off = tail_of[nt];
+      p.pairs[3 * e + d] = pi;
+      for (int b = 0; b < p.cell_cnt[kind]; b += 32) p.batches.push_back(make_int2(3 * e + d, b));
+      pp += p.cell_cnt[kind];
+    }
+  p.npp = pp;
+
+  // ── device buffers ──
+  int rc;
+  if ((rc = dalloc(h, &p.d_pts, p.pts.size() * 8))) return rc;
+  if ((rc = dalloc(h, &p.d_nodes, p.nodes.size() * 8))) return rc;
+  if ((rc = dalloc(h, &p.d_elem, 3 * (size_t)Ne * 8))) return rc;
+  std::vector<double> elem(3 * (size_t)Ne);
+  for (int e = 0; e < Ne; ++e) { elem[e] = p.L[e]; elem[Ne + e] = p.trho[e]; elem[2 * Ne + e] = p.tz[e]; }
+  // far tables with exact values at 0, pi/2, pi, 3pi/2
+  std::vector<double> tabs;
+  auto cos_exact = [](long long i, int N) {
+    i %= N;
+    if (i == 0) return 1.0;
+    if (2 * i == N) return -1.0;
+    if (4 * i == N || 4 * i == 3 * (long long)N) return 0.0;
+    return std::cos(2.0 * M_PI * (double)i / N);
+  };
+  auto sin_exact = [](long long i, int N) {
+    i %= N;
+    if (i == 0 || 2 * i == N) return 0.0;
+    if (4 * i == N) return 1.0;
+    if (4 * i == 3 * (long long)N) return -1.0;
+    return std::sin(2.0 * M_PI * (double)i / N);
+  };
+  p.tab_off[0] = tabs.size();
+  for (int j = 0; j <= p.nE / 2; ++j) { const double s = std::sin(M_PI * j / p.nE); tabs.push_back(j == p.nE / 2 ? 1.0 : s * s); }
+  p.tab_off[1] = tabs.size();
+  for (int i = 0; i < p.nE; ++i) tabs.push_back(cos_exact(i, p.nE));
+  p.tab_off[2] = tabs.size();
+  for (int j = 0; j <= p.nB / 2; ++j) tabs.push_back(cos_exact(j, p.nB));
+  p.tab_off[3] = tabs.size();
+  for (int j = 0; j <= p.nB / 2; ++j) tabs.push_back(sin_exact(j, p.nB));
+  p.tab_off[4] = tabs.size();
+  for (int i = 0; i < p.nB; ++i) tabs.push_back(cos_exact(i, p.nB));
+  p.tab_off[5] = tabs.size();
+  for (int i = 0; i < p.nB; ++i) tabs.push_back(sin_exact(i, p.nB));
+  if ((rc = dalloc(h, &p.d_tabs, tabs.size() * 8))) return rc;
+  if ((rc = dalloc(h, &p.d_theta, p.theta.size() * 8))) return rc;
+  if ((rc = dalloc(h, &p.d_J, (size_t)(p.M + 2) * P * p.ntheta * 8))) return rc;
+  std::vector<double> cells;
+  for (int kd = 0; kd < 3; ++kd) cells.insert(cells.end(), cs[kd].begin(), cs[kd].end());
+  for (int kd = 0; kd < 3; ++kd) cells.insert(cells.end(), csp[kd].begin(), csp[kd].end());
+  for (int kd = 0; kd < 3; ++kd) cells.insert(cells.end(), cw[kd].begin(), cw[kd].end());
+  if ((rc = dalloc(h, &p.d_cells, cells.size() * 8))) return rc;
+  std::vector<double> x48, w48, core;
+  gauss_legendre(48, x48, w48);
+  for (int i = 0; i < 48; ++i) core.push_back(0.5 * (x48[i] + 1.0));
+  for (int i = 0; i < 48; ++i) core.push_back(0.5 * w48[i]);
+  if ((rc = dalloc(h, &p.d_core, core.size() * 8))) return rc;
+  std::vector<double> tail(p.tail_u);
+  tail.insert(tail.end(), p.tail_w.begin(), p.tail_w.end());
+  if ((rc = dalloc(h, &p.d_tail, tail.size() * 8))) return rc;
+  if ((rc = dalloc(h, &p.d_pairs, p.pairs.size() * sizeof(NearPairInfo)))) return rc;
+  if ((rc = dalloc(h, &p.d_batches, p.batches.size() * sizeof(int2)))) return rc;
+  if ((rc = dalloc(h, &p.d_NP, (size_t)p.npp * (p.M + 2) * 5 * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &p.d_NB, (size_t)3 * Ne * 16 * p.nmodes * sizeof(double2)))) return rc;
+  const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.nmodes;
+  const size_t zsz = h->precision == GHOST_FP64 ? sizeof(double2) : sizeof(float2);
+  if ((rc = dalloc(h, &p.d_zfar, zel * zsz))) return rc;
+
+  HIPCHK(hipMemcpy(p.d_pts, p.pts.data(), p.pts.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_nodes, p.nodes.data(), p.nodes.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_elem, elem.data(), elem.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_tabs, tabs.data(), tabs.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_theta, p.theta.data(), p.theta.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_cells, cells.data(), cells.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_core, core.data(), core.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_tail, tail.data(), tail.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_pairs, p.pairs.data(), p.pairs.size() * sizeof(NearPairInfo), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_batches, p.batches.data(), p.batches.size() * sizeof(int2), hipMemcpyHostToDevice));
+  p.tail_u.clear();  // keep sizes only
+  return 0;
+}
+
+static double gemm_flops(const std::vector<GemmOp>& ops, int first, int n, bool triangular) {
+  double f = 0.0;
+  for (int i = first; i < first + n; ++i) {
+    const GemmOp& g = ops[i];
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0) continue;
+    // complex multiply-add = 8 real flops; a triangular block costs half of a full one
+    f += (triangular ? 4.0 : 8.0) * (double)g.M * g.N * g.K;
+  }
+  return f;
+}
+
+static void set_flops(ghost_bor* h, std::vector<Launch>& sched) {
+  for (Launch& L : sched) {
+    if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W) L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, true);
+    else if (L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU) L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, false);
+  }
+}
+
+static int build_schedule(ghost_bor* h) {
+  const int nb = h->nb;
+  const int nsys = (int)h->sys.size();
+  int maxn = 0;
+  for (auto& s : h->sys) maxn = std::max(maxn, s.n);
+  h->rpt = (maxn + 1023) / 1024;
+  if (h->rpt > 4) return fail(GHOST_ERR_ARG, "system size %d exceeds the panel limit 4096", maxn);
+  const int nblk_max = (maxn + nb - 1) / nb;
+  h->max_blocks = nblk_max;
+  const bool mixed = h->precision == GHOST_MIXED;
+  auto wl = [&](int s, int b) { return ((long long)s * nblk_max + b) * 2 * 64 * 64; };
+  auto wu = [&](int s, int b) { return ((long long)s * nblk_max + b) * 2 * 64 * 64 + 64 * 64; };
+  auto tiles = [](int M, int N) { return (M > 0 && N > 0) ? ((M + 63) / 64) * ((N + 63) / 64) : 0; };
+
+  // ── LU steps on the augmented systems ──
+  for (int b = 0; b < nblk_max; ++b) {
+    const int k = b * nb;
+    Launch lp{L_PANEL, (int)h->h_panel.size(), nsys, nsys, h->rpt};
+    Launch ls{L_SWAP, (int)h->h_swap.size(), nsys, 0, 0};
+    Launch lt{L_TRIL, (int)h->h_trsm.size(), nsys, nsys, 0};
+    Launch g1{L_GEMM_AUG_W, (int)h->h_gemm.size(), nsys, 0, 0};
+    for (int si = 0; si < nsys; ++si) {
+      const SysInfo& s = h->sys[si];
+      const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+      const int ncols = s.n + s.nrhs;
+      h->h_panel.push_back(PanelOp{s.a_off, s.n, s.ld, k, w, 0});
+      SwapOp so{s.a_off, s.ld, k, w, k, k + w, ncols, 0};
+      h->h_swap.push_back(so);
+      ls.gx = std::max(ls.gx, (k + (ncols - k - w) + 255) / 256);
+      h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wl(si, b), s.ld, w});
+      // X = L11^{-1} X on rows [k, k+w), cols [k+w, ncols)
+      GemmOp go{wl(si, b), s.a_off + (long long)k * s.ld + k + w, s.a_off + (long long)k * s.ld + k + w,
+                64, s.ld, s.ld, w, ncols - k - w, w};
+      if (w == 0) go.M = 0;
+      h->h_gemm.push_back(go);
+      g1.gx = std::max(g1.gx, tiles(go.M, go.N));
+    }
+    Launch g2{L_GEMM_AUG, (int)h->h_gemm.size(), nsys, 0, 0};
+    for (int si = 0; si < nsys; ++si) {
+      const SysInfo& s = h->sys[si];
+      const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+      const int ncols = s.n + s.nrhs;
+      GemmOp go{s.a_off + (long long)(k + w) * s.ld + k, s.a_off + (long long)k * s.ld + k + w,
+                s.a_off + (long long)(k + w) * s.ld + k + w, s.ld, s.ld, s.ld, s.n - k - w, ncols - k - w, w};
+      if (w == 0) go.M = 0;
+      h->h_gemm.push_back(go);
+      g2.gx = std::max(g2.gx, tiles(go.M, go.N));
+    }
+    h->lu_sched.push_back(lp);
+    h->lu_sched.push_back(ls);
+    h->lu_sched.push_back(lt);
+    h->lu_sched.push_back(g1);
+    h->lu_sched.push_back(g2);
+  }
+  // ── upper diagonal-block inverses (all blocks, one launch) ──
+  h->triu_ops = (int)h->h_trsm.size();
+  for (int si = 0; si < nsys; ++si) {
+    const SysInfo& s = h->sys[si];
+    for (int b = 0; b < nblk_max; ++b) {
+      const int k = b * nb;
+      const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+      h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wu(si, b), s.ld, w});
+    }
+  }
+  h->triu_n = nsys * nblk_max;
+  // ── back substitution on the RHS columns of the augmented systems ──
+  for (int st = 0; st < nblk_max; ++st) {
+    Launch g1{L_GEMM_AUG_W, (int)h->h_gemm.size(), nsys, 0, 0};
+    std::vector<GemmOp> upd;
+    for (int si = 0; si < nsys; ++si) {
+      const SysInfo& s = h->sys[si];
+      const int nbs = (s.n + nb - 1) / nb;
+      const int b = nbs - 1 - st;
+      GemmOp go{0, 0, 0, 64, s.ld, s.ld, 0, 0, 0};
+      GemmOp gu{0, 0, 0, s.ld, s.ld, s.ld, 0, 0, 0};
+      if (b >= 0) {
+        const int kb = b * nb, w = std::min(nb, s.n - kb);
+        go = GemmOp{wu(si, b), s.a_off + (long long)kb * s.ld + s.n, s.a_off + (long long)kb * s.ld + s.n,
+                    64, s.ld, s.ld, w, s.nrhs, w};
+        gu = GemmOp{s.a_off + kb, s.a_off + (long long)kb * s.ld + s.n, s.a_off + s.n, s.ld, s.ld, s.ld,
+                    kb, s.nrhs, w};
+      }
+      h->h_gemm.push_back(go);
+      g1.gx = std::max(g1.gx, tiles(go.M, go.N));
+      upd.push_back(gu);
+    }
+    Launch g2{L_GEMM_AUG, (int)h->h_gemm.size(), nsys, 0, 0};
+    for (auto& gu : upd) { h->h_gemm.push_back(gu); g2.gx = std::max(g2.gx, tiles(gu.M, gu.N)); }
+    h->back_sched.push_back(g1);
+    h->back_sched.push_back(g2);
+  }
+  // ── VecOps over all systems (copies, casts, norms) ──
+  h->vec_all = (int)h->h_vec.size();
+  for (int si = 0; si < nsys; ++si) {
+    const SysInfo& s = h->sys[si];
+    h->h_vec.push_back(VecOp{s.x_off, s.x_off, s.nrhs, s.nrhs, s.n, s.nrhs, 0});
+  }
+  h->vec_aug = (int)h->h_vec.size();
+  for (int si = 0; si < nsys; ++si) {
+    const SysInfo& s = h->sys[si];
+    h->h_vec.push_back(VecOp{s.a_off + s.n, s.x_off, s.ld, s.nrhs, s.n, s.nrhs, 0});
+  }
+  // ── residual / refinement GEMM: R = B - A64 X (X in X64 or in the augmented columns) ──
+  h->res_gemm = (int)h->h_gemm.size();
+  for (int si = 0; si < nsys; ++si) {
+    const SysInfo& s = h->sys[si];
+    h->h_gemm.push_back(GemmOp{s.a64_off, s.x_off, s.x_off, s.n, s.nrhs, s.nrhs, s.n, s.nrhs, s.n});
+  }
+  h->res_gemm_aug = (int)h->h_gemm.size();
+  for (int si = 0; si < nsys; ++si) {
+    const SysInfo& s = h->sys[si];
+    h->h_gemm.push_back(GemmOp{s.a64_off, s.a_off + s.n, s.x_off, s.n, s.ld, s.nrhs, s.n, s.nrhs, s.n});
+  }
+  if (mixed) {
+    // forward / backward solves on a separate FP32 RHS block Y (n x nrhs)
+    for (int b = 0; b < nblk_max; ++b) {
+      const int k = b * nb;
+      Launch g1{L_GEMM_Y_W, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        GemmOp go{wl(si, b), s.x_off + (long long)k * s.nrhs, s.x_off + (long long)k * s.nrhs, 64, s.nrhs,
+                  s.nrhs, w, s.nrhs, w};
+        h->h_gemm.push_back(go);
+        g1.gx = std::max(g1.gx, tiles(go.M, go.N));
+      }
+      Launch g2{L_GEMM_Y_LU, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        GemmOp go{s.a_off + (long long)(k + w) * s.ld + k, s.x_off + (long long)k * s.nrhs,
+                  s.x_off + (long long)(k + w) * s.nrhs, s.ld, s.nrhs, s.nrhs, s.n - k - w, s.nrhs, w};
+        if (w == 0) go.M = 0;
+        h->h_gemm.push_back(go);
+        g2.gx = std::max(g2.gx, tiles(go.M, go.N));
+      }
+      h->fwdY_sched.push_back(g1);
+      h->fwdY_sched.push_back(g2);
+    }
+    for (int st = 0; st < nblk_max; ++st) {
+      Launch g1{L_GEMM_Y_W, (int)h->h_gemm.size(), nsys, 0, 0};
+      std::vector<GemmOp> upd;
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int nbs = (s.n + nb - 1) / nb;
+        const int b = nbs - 1 - st;
+        GemmOp go{0, 0, 0, 64, s.nrhs, s.nrhs, 0, 0, 0};
+        GemmOp gu{0, 0, 0, s.ld, s.nrhs, s.nrhs, 0, 0, 0};
+        if (b >= 0) {
+          const int kb = b * nb, w = std::min(nb, s.n - kb);
+          go = GemmOp{wu(si, b), s.x_off + (long long)kb * s.nrhs, s.x_off + (long long)kb * s.nrhs, 64,
+                      s.nrhs, s.nrhs, w, s.nrhs, w};
+          gu = GemmOp{s.a_off + kb, s.x_off + (long long)kb * s.nrhs, s.x_off, s.ld, s.nrhs, s.nrhs, kb,
+                      s.nrhs, w};
+        }
+        h->h_gemm.push_back(go);
+        g1.gx = std::max(g1.gx, tiles(go.M, go.N));
+        upd.push_back(gu);
+      }
+      Launch g2{L_GEMM_Y_LU, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (auto& gu : upd) { h->h_gemm.push_back(gu); g2.gx = std::max(g2.gx, tiles(gu.M, gu.N)); }
+      h->backY_sched.push_back(g1);
+      h->backY_sched.push_back(g2);
+    }
+  }
+  set_flops(h, h->lu_sched);
+  set_flops(h, h->back_sched);
+  set_flops(h, h->fwdY_sched);
+  set_flops(h, h->backY_sched);
+  // pivot offsets
+  int poff = 0;
+  std::vector<int> pofs(nsys);
+  for (int si = 0; si < nsys; ++si) { pofs[si] = poff; poff += h->sys[si].n; }
+  for (size_t i = 0; i < h->h_panel.size(); ++i) h->h_panel[i].piv = pofs[i % nsys];
+  for (size_t i = 0; i < h->h_swap.size(); ++i) h->h_swap[i].piv = pofs[i % nsys];
+  for (size_t i = 0; i < h->h_vec.size(); ++i) h->h_vec[i].piv = pofs[i % nsys];
+  h->piv_elems = poff;
+  int rc;
+  if ((rc = dalloc(h, &h->d_panel, h->h_panel.size() * sizeof(PanelOp)))) return rc;
+  if ((rc = dalloc(h, &h->d_swap, h->h_swap.size() * sizeof(SwapOp)))) return rc;
+  if ((rc = dalloc(h, &h->d_trsm, h->h_trsm.size() * sizeof(TrsmOp)))) return rc;
+  if ((rc = dalloc(h, &h->d_gemm, h->h_gemm.size() * sizeof(GemmOp)))) return rc;
+  if ((rc = dalloc(h, &h->d_vec, h->h_vec.size() * sizeof(VecOp)))) return rc;
+  HIPCHK(hipMemcpy(h->d_panel, h->h_panel.data(), h->h_panel.size() * sizeof(PanelOp), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_swap, h->h_swap.data(), h->h_swap.size() * sizeof(SwapOp), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_trsm, h->h_trsm.data(), h->h_trsm.size() * sizeof(TrsmOp), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_gemm, h->h_gemm.data(), h->h_gemm.size() * sizeof(GemmOp), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->d_vec, h->h_vec.data(), h->h_vec.size() * sizeof(VecOp), hipMemcpyHostToDevice));
+  return 0;
+}
+
+extern "C" int ghost_bor_build(ghost_bor* h) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "already built");
+  if (h->probs.empty()) return fail(GHOST_ERR_STATE, "no problems added");
+  HIPCHK(hipSetDevice(h->device));
+  const bool mixed = h->precision == GHOST_MIXED;
+  // system offsets
+  size_t a = 0, x = 0, a64 = 0;
+  for (auto& s : h->sys) {
+    s.ld = ((s.n + s.nrhs + 15) / 16) * 16;
+    s.a_off = (long long)a;
+    s.x_off = (long long)x;
+    s.a64_off = (long long)a64;
+    a += (size_t)s.n * s.ld;
+    x += (size_t)s.n * s.nrhs;
+    a64 += (size_t)s.n * s.n;
+  }
+  h->a_elems = a;
+  h->x_elems = x;
+  h->a64_elems = a64;
+  int rc;
+  for (auto& p : h->probs)
+    if ((rc = prepare_problem(h, p))) return rc;
+  if ((rc = build_schedule(h))) return rc;
+  const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
+  if ((rc = dalloc(h, &h->d_sys, h->sys.size() * sizeof(SysInfo)))) return rc;
+  HIPCHK(hipMemcpy(h->d_sys, h->sys.data(), h->sys.size() * sizeof(SysInfo), hipMemcpyHostToDevice));
+  if ((rc = dalloc(h, &h->d_A, h->a_elems * csz))) return rc;
+  if (mixed || h->residual) {
+    if ((rc = dalloc(h, &h->d_A64, h->a64_elems * sizeof(double2)))) return rc;
+    if ((rc = dalloc(h, &h->d_B64, h->x_elems * sizeof(double2)))) return rc;
+    if ((rc = dalloc(h, &h->d_R64, h->x_elems * sizeof(double2)))) return rc;
+  }
+  if (mixed) {
+    if ((rc = dalloc(h, &h->d_X64, h->x_elems * sizeof(double2)))) return rc;
+    if ((rc = dalloc(h, &h->d_Y32, h->x_elems * sizeof(float2)))) return rc;
+  }
+  if ((rc = dalloc(h, &h->d_piv, (size_t)h->piv_elems * sizeof(int)))) return rc;
+  if ((rc = dalloc(h, &h->d_perm, (size_t)h->piv_elems * sizeof(int)))) return rc;
+  if ((rc = dalloc(h, &h->d_info, h->sys.size() * sizeof(int)))) return rc;
+  if ((rc = dalloc(h, &h->d_W, h->sys.size() * (size_t)h->max_blocks * 2 * 64 * 64 * csz))) return rc;
+  h->dF_elems = 0;
+  h->partial_elems = 0;
+  for (auto& p : h->probs) {
+    p.dF_off = h->dF_elems;
+    h->dF_elems += (size_t)p.nmodes * p.nrhs;
+    const size_t nch = (p.P + h->ff_chunk - 1) / h->ff_chunk;
+    h->partial_elems = std::max(h->partial_elems, nch * p.nmodes * p.nrhs);
+  }
+  if ((rc = dalloc(h, &h->d_dF, h->dF_elems * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_partial, h->partial_elems * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_red, 2 * h->sys.size() * sizeof(double)))) return rc;
+  h->built = true;
+  return 0;
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// phases
+// ─────────────────────────────────────────────────────────────────────────────
+
+template <typename R> static int far_fill(ghost_bor* h, Problem& p) {
+  FarArgs fa;
+  const int P = p.P;
+  fa.pts = PointSet{p.d_pts, p.d_pts + P, p.d_pts + 2 * P, p.d_pts + 3 * P, p.d_pts + 4 * P,
+                    p.d_pts + 5 * P, p.d_pts + 6 * P, p.d_pts + 7 * P, p.d_pts + 8 * P};
+  fa.sin2E = p.d_tabs + p.tab_off[0];
+  fa.cosE = p.d_tabs + p.tab_off[1];
+  fa.cxB = p.d_tabs + p.tab_off[2];
+  fa.sxB = p.d_tabs + p.tab_off[3];
+  fa.cosB = p.d_tabs + p.tab_off[4];
+  fa.sinB = p.d_tabs + p.tab_off[5];
+  fa.Ne = p.Ne;
+  fa.Nn = p.Nn;
+  fa.M = p.M;
+  fa.nE = p.nE;
+  fa.nB = p.nB;
+  fa.k = p.k;
+  // EFIE coefficient cE * j k eta0 2 pi ; far MFIE coefficient -2 pi cM
+  fa.ce_re = 0.0;
+  fa.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
+  fa.cm = -2.0 * M_PI * p.cM;
+  fa.Z = p.d_zfar;
+  const int NT = (p.M + 2 + 15) / 16;
+  for (int pe = 0; pe < 2; ++pe)
+    for (int pf = 0; pf < 2; ++pf) {
+      const int nep = (p.Ne - pe + 1) / 2, nfp = (p.Ne - pf + 1) / 2;
+      if (nep <= 0 || nfp <= 0) continue;
+      fa.pe = pe;
+      fa.pf = pf;
+      fa.nfp = nfp;
+      fa.npairs = nep * nfp;
+      const int grid = (fa.npairs + 1) / 2;
+      prof_begin(h);
+      switch (NT) {
+#define FARCASE(n) case n: hipLaunchKernelGGL((k_far_fill<R, n>), dim3(grid), dim3(128), 0, h->stream, fa); break;
+        FARCASE(1) FARCASE(2) FARCASE(3) FARCASE(4) FARCASE(5) FARCASE(6) FARCASE(7) FARCASE(8)
+#undef FARCASE
+        default: return fail(GHOST_ERR_ARG, "m_max too large for the far fill");
+      }
+      prof_end(h, K_FAR_FILL, 0.0);
+      LAUNCHCHK();
+    }
+  return 0;
+}
+
+static int fill_problem(ghost_bor* h, Problem& p) {
+  const bool mixed = h->precision == GHOST_MIXED;
+  const int P = p.P;
+  const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.nmodes;
+  const size_t zero_n = mixed ? (zel + 1) / 2 : zel;  // in double2 units
+  prof_begin(h);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (zero_n + 255) / 256)), dim3(256), 0,
+                     h->stream, (double2*)p.d_zfar, zero_n);
+  prof_end(h, K_ZERO, 0.0);
+
+  LAUNCHCHK();
+  // near stage 1 + 2 (FP64)
+  NearArgs na;
+  na.nodes = p.d_nodes;
+  na.L = p.d_elem;
+  na.trho = p.d_elem + p.Ne;
+  na.tz = p.d_elem + 2 * p.Ne;
+  const int ncell = p.cell_off[2] + p.cell_cnt[2];
+  na.cell_s = p.d_cells;
+  na.cell_sp = p.d_cells + ncell;
+  na.cell_w = p.d_cells + 2 * ncell;
+  for (int i = 0; i < 3; ++i) { na.cell_off[i] = p.cell_off[i]; na.cell_cnt[i] = p.cell_cnt[i]; }
+  na.core_u = p.d_core;
+  na.core_w = p.d_core + 48;
+  na.tail_u = p.d_tail;
+  na.tail_w = p.d_tail + p.tail_w.size();
+  na.pairs = p.d_pairs;
+  na.batches = p.d_batches;
+  na.Ne = p.Ne;
+  na.M = p.M;
+  na.nord = p.M + 2;
+  na.k = p.k;
+  na.NP = p.d_NP;
+  if (!p.batches.empty()) {
+    const int C = (na.nord + 7) / 8;
+    const dim3 g((unsigned)p.batches.size());
+    prof_begin(h);
+    switch (C) {
+#define NEARCASE(c) case c: hipLaunchKernelGGL((k_near_modal<c>), g, dim3(256), 0, h->stream, na); break;
+      NEARCASE(1) NEARCASE(2) NEARCASE(3) NEARCASE(4) NEARCASE(5) NEARCASE(6) NEARCASE(7) NEARCASE(8)
+      NEARCASE(9) NEARCASE(10) NEARCASE(11) NEARCASE(12) NEARCASE(13) NEARCASE(14) NEARCASE(15) NEARCASE(16)
+#undef NEARCASE
+      default: return fail(GHOST_ERR_ARG, "m_max too large for the near quadrature");
+    }
+    prof_end(h, K_NEAR_MODAL, 0.0);
+    LAUNCHCHK();
+  }
+  NearBlockArgs nb;
+  nb.na = na;
+  nb.gw = p.d_pts + 4 * P;
+  nb.grho = p.d_pts;
+  nb.gT0 = p.d_pts + 5 * P;
+  nb.gT1 = p.d_pts + 6 * P;
+  nb.ce_re = 0.0;
+  nb.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
+  nb.cM = p.cM;
+  nb.NB = p.d_NB;
+  const int nthr = ((p.M + 1 + 63) / 64) * 64;
+  if (nthr > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near blocks");
+  prof_begin(h);
+  hipLaunchKernelGGL(k_near_blocks, dim3(3 * p.Ne), dim3(nthr), 0, h->stream, nb);
+  prof_end(h, K_NEAR_BLOCKS, 0.0);
+
+  LAUNCHCHK();
+  // far
+  int rc = mixed ? far_fill<float>(h, p) : far_fill<double>(h, p);
+  if (rc) return rc;
+  // assemble
+  AsmArgs aa;
+  aa.Zfar = p.d_zfar;
+  aa.NB = p.d_NB;
+  aa.sys = h->d_sys + p.sys0;
+  aa.Nn = p.Nn;
+  aa.Ne = p.Ne;
+  aa.M = p.M;
+  aa.A = h->d_A;
+  aa.A64 = h->d_A64;
+  dim3 g(2 * p.Nn, (2 * p.Nn + 31) / 32, (p.nmodes + 31) / 32);
+  prof_begin(h);
+  if (mixed) hipLaunchKernelGGL((k_assemble<float, float>), g, dim3(256), 0, h->stream, aa);
+  else hipLaunchKernelGGL((k_assemble<double, double>), g, dim3(256), 0, h->stream, aa);
+  prof_end(h, K_ASSEMBLE, 0.0);
+  LAUNCHCHK();
+  // excitation
+  ExciteArgs ea;
+  ea.pts = PointSet{p.d_pts, p.d_pts + P, p.d_pts + 2 * P, p.d_pts + 3 * P, p.d_pts + 4 * P,
+                    p.d_pts + 5 * P, p.d_pts + 6 * P, p.d_pts + 7 * P, p.d_pts + 8 * P};
+  ea.P = P;
+  ea.Nn = p.Nn;
+  ea.M = p.M;
+  ea.ntheta = p.ntheta;
+  ea.k = p.k;
+  ea.cE = p.cE;
+  ea.cM = p.cM;
+  ea.theta = p.d_theta;
+  ea.J = p.d_J;
+  const size_t nbes = (size_t)P * p.ntheta;
+  prof_begin(h);
+  hipLaunchKernelGGL(k_bessel, dim3((unsigned)std::min<size_t>(65535, (nbes + 127) / 128)), dim3(128), 0,
+                     h->stream, ea);
+  prof_end(h, K_BESSEL, 0.0);
+
+  LAUNCHCHK();
+  int maxn = 0;
+  for (int i = 0; i < p.nmodes; ++i) maxn = std::max(maxn, h->sys[p.sys0 + i].n);
+  dim3 gr((p.nrhs + 255) / 256, maxn, p.nmodes);
+  prof_begin(h);
+  if (mixed) hipLaunchKernelGGL((k_rhs<float>), gr, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0, h->d_A, h->d_B64, 0);
+  else hipLaunchKernelGGL((k_rhs<double>), gr, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0, h->d_A, h->d_B64, 0);
+  prof_end(h, K_RHS, 0.0);
+  LAUNCHCHK();
+  return 0;
+}
+
+static int run_fill(ghost_bor* h) {
+  for (auto& p : h->probs) {
+    int rc = fill_problem(h, p);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launch>& sched, void* factor,
+                                           void* rhs) {
+  for (const Launch& L : sched) {
+    prof_begin(h);
+    switch (L.kind) {
+      case L_PANEL:
+        switch (L.arg) {
+          case 1: hipLaunchKernelGGL((k_lu_panel<T, 1>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          case 2: hipLaunchKernelGGL((k_lu_panel<T, 2>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          case 3: hipLaunchKernelGGL((k_lu_panel<T, 3>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          default: hipLaunchKernelGGL((k_lu_panel<T, 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+        }
+        break;
+      case L_SWAP:
+        if (L.gx > 0) hipLaunchKernelGGL((k_lu_swap<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_swap + L.ops, factor, h->d_piv);
+        break;
+      case L_TRIL:
+        hipLaunchKernelGGL((k_tri_inv<T, false>), dim3(L.nops), dim3(64), 0, h->stream, h->d_trsm + L.ops, factor, h->d_W);
+        break;
+      case L_GEMM_AUG_W:
+        if (L.gx > 0) hipLaunchKernelGGL((k_gemm_update<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_gemm + L.ops, h->d_W, factor, factor);
+        break;
+      case L_GEMM_AUG:
+        if (L.gx > 0) hipLaunchKernelGGL((k_gemm_update<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_gemm + L.ops, factor, factor, factor);
+        break;
+      case L_GEMM_Y_W:
+        if (L.gx > 0) hipLaunchKernelGGL((k_gemm_update<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_gemm + L.ops, h->d_W, rhs, rhs);
+        break;
+      case L_GEMM_Y_LU:
+        if (L.gx > 0) hipLaunchKernelGGL((k_gemm_update<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_gemm + L.ops, factor, rhs, rhs);
+        break;
+      default: break;
+    }
+    {
+      int kind = K_GEMM;
+      if (L.kind == L_PANEL) kind = K_PANEL;
+      else if (L.kind == L_SWAP) kind = K_SWAP;
+      else if (L.kind == L_TRIL || L.kind == L_TRIU) kind = K_TRI_INV;
+      prof_end(h, kind, L.flops);
+    }
+    LAUNCHCHK();
+  }
+  return 0;
+}
+
+template <typename T> static int run_lu(ghost_bor* h) {
+  const int nsys = (int)h->sys.size();
+  hipLaunchKernelGGL(k_zero_int, dim3((nsys + 255) / 256), dim3(256), 0, h->stream, h->d_info, nsys);
+  LAUNCHCHK();
+  int rc = run_sched<T>(h, h->lu_sched, h->d_A, nullptr);
+  if (rc) return rc;
+  prof_begin(h);
+  hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(h->triu_n), dim3(64), 0, h->stream, h->d_trsm + h->triu_ops,
+                     h->d_A, h->d_W);
+  prof_end(h, K_TRI_INV, 0.0);
+
+  LAUNCHCHK();
+  return run_sched<T>(h, h->back_sched, h->d_A, nullptr);
+}
+
+// R64 = B64 - A64 X with X in X64 (x_in_aug = 0) or in the FP64 augmented columns
+static int residual_gemm(ghost_bor* h, int x_in_aug) {
+  const int nsys = (int)h->sys.size();
+  size_t maxn = 0, maxr = 0;
+  int gx = 0;
+  for (auto& s : h->sys) {
+    maxn = std::max(maxn, (size_t)s.n);
+    maxr = std::max(maxr, (size_t)s.nrhs);
+    gx = std::max(gx, ((s.n + 63) / 64) * ((s.nrhs + 63) / 64));
+  }
+  dim3 gv((unsigned)((maxr + 255) / 256), (unsigned)maxn, nsys);
+  hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_B64, h->d_R64);
+  LAUNCHCHK();
+  prof_begin(h);
+  if (x_in_aug)
+    hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
+                       h->d_gemm + h->res_gemm_aug, h->d_A64, h->d_A, h->d_R64);
+  else
+    hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
+                       h->d_gemm + h->res_gemm, h->d_A64, h->d_X64, h->d_R64);
+  prof_end(h, K_GEMM64, gemm_flops(h->h_gemm, x_in_aug ? h->res_gemm_aug : h->res_gemm, nsys, false));
+  LAUNCHCHK();
+  return 0;
+}
+
+extern "C" int ghost_bor_fill(ghost_bor* h) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipEventRecord(h->ev[0], h->stream));
+  int rc = run_fill(h);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(h->ev[1], h->stream));
+  return 0;
+}
+
+static int frob2(ghost_bor* h, const double2* buf, std::vector<double>& out) {
+  // sum |z|^2 per system over an n x nrhs block laid out by x_off
+  const int nsys = (int)h->sys.size();
+  out.assign(nsys, 0.0);
+  std::vector<double2> host(h->x_elems);
+  HIPCHK(hipMemcpyAsync(host.data(), buf, h->x_elems * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int si = 0; si < nsys; ++si) {
+    const SysInfo& s = h->sys[si];
+    double acc = 0;
+    for (size_t i = 0; i < (size_t)s.n * s.nrhs; ++i) {
+      const double2 z = host[s.x_off + i];
+      acc += z.x * z.x + z.y * z.y;
+    }
+    out[si] = acc;
+  }
+  return 0;
+}
+
+extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  HIPCHK(hipSetDevice(h->device));
+  const int nsys = (int)h->sys.size();
+  const bool mixed = h->precision == GHOST_MIXED;
+  HIPCHK(hipEventRecord(h->ev[2], h->stream));
+  int rc;
+  h->refine_hist.clear();
+  if (!mixed) {
+    if ((rc = run_lu<double>(h))) return rc;
+  } else {
+    if ((rc = run_lu<float>(h))) return rc;
+    size_t maxn = 0, maxr = 0;
+    for (auto& s : h->sys) { maxn = std::max(maxn, (size_t)s.n); maxr = std::max(maxr, (size_t)s.nrhs); }
+    dim3 gv((unsigned)((maxr + 255) / 256), (unsigned)maxn, nsys);
+    // X64 = (double) X32 from the augmented columns
+    hipLaunchKernelGGL(k_widen, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_aug, (const float2*)h->d_A, h->d_X64, 0);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_perm_from_pivots, dim3((nsys + 63) / 64), dim3(64), 0, h->stream, h->d_vec + h->vec_all,
+                       nsys, h->d_piv, h->d_perm);
+    LAUNCHCHK();
+    for (int it = 0; it < refine_iters; ++it) {
+      if ((rc = residual_gemm(h, 0))) return rc;      // R = B - A X in FP64
+      hipLaunchKernelGGL(k_permute_cast, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_perm, h->d_R64, h->d_Y32);
+      LAUNCHCHK();
+      if ((rc = run_sched<float>(h, h->fwdY_sched, h->d_A, h->d_Y32))) return rc;
+      if ((rc = run_sched<float>(h, h->backY_sched, h->d_A, h->d_Y32))) return rc;
+      hipLaunchKernelGGL(k_widen, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_Y32, h->d_X64, 1);
+      LAUNCHCHK();
+    }
+  }
+  HIPCHK(hipEventRecord(h->ev[3], h->stream));
+  h->refine_iters = refine_iters;
+  return 0;
+}
+
+extern "C" int ghost_bor_farfield(ghost_bor* h) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  HIPCHK(hipSetDevice(h->device));
+  const bool mixed = h->precision == GHOST_MIXED;
+  HIPCHK(hipEventRecord(h->ev[4], h->stream));
+  const int nsys = (int)h->sys.size();
+  for (auto& p : h->probs) {
+    ExciteArgs ea;
+    const int P = p.P;
+    ea.pts = PointSet{p.d_pts, p.d_pts + P, p.d_pts + 2 * P, p.d_pts + 3 * P, p.d_pts + 4 * P,
+                      p.d_pts + 5 * P, p.d_pts + 6 * P, p.d_pts + 7 * P, p.d_pts + 8 * P};
+    ea.P = P;
+    ea.Nn = p.Nn;
+    ea.M = p.M;
+    ea.ntheta = p.ntheta;
+    ea.k = p.k;
+    ea.cE = p.cE;
+    ea.cM = p.cM;
+    ea.theta = p.d_theta;
+    ea.J = p.d_J;
+    const int nch = (P + h->ff_chunk - 1) / h->ff_chunk;
+    dim3 g((p.nrhs + 255) / 256, p.nmodes, nch);
+    // partial layout [chunk][system-in-problem][nrhs], reduced into this problem's dF block
+    prof_begin(h);
+    if (mixed)
+      hipLaunchKernelGGL((k_farfield_partial<double>), g, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0,
+                         (const void*)h->d_X64, 0, h->ff_chunk, h->d_partial, p.nmodes);
+    else
+      hipLaunchKernelGGL((k_farfield_partial<double>), g, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0,
+                         (const void*)h->d_A, 1, h->ff_chunk, h->d_partial, p.nmodes);
+    prof_end(h, K_FARFIELD, 0.0);
+    LAUNCHCHK();
+    const int tot = p.nmodes * p.nrhs;
+    hipLaunchKernelGGL(k_farfield_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream, h->d_partial, nch,
+                       p.nmodes, p.nrhs, p.k, h->d_dF + p.dF_off);
+    LAUNCHCHK();
+  }
+  (void)nsys;
+  HIPCHK(hipEventRecord(h->ev[5], h->stream));
+  return 0;
+}
+
+extern "C" int ghost_bor_residual(ghost_bor* h, double* out) {
+  // ||A X - B||_F / ||B||_F per system (bor_solver.py:754-756); FP64 A and B copies
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (!h->d_A64) return fail(GHOST_ERR_STATE, "residual copies were disabled at build time");
+  HIPCHK(hipSetDevice(h->device));
+  const bool mixed = h->precision == GHOST_MIXED;
+  int rc = residual_gemm(h, mixed ? 0 : 1);
+  if (rc) return rc;
+  std::vector<double> rn, bn;
+  if ((rc = frob2(h, h->d_R64, rn))) return rc;
+  if ((rc = frob2(h, h->d_B64, bn))) return rc;
+  for (size_t i = 0; i < rn.size(); ++i) out[i] = bn[i] > 0 ? std::sqrt(rn[i] / bn[i]) : 0.0;
+  return 0;
+}
+
+extern "C" int ghost_bor_run(ghost_bor* h, int refine_iters) {
+  int rc = ghost_bor_fill(h);
+  if (rc) return rc;
+  if ((rc = ghost_bor_solve(h, refine_iters))) return rc;
+  return ghost_bor_farfield(h);
+}
+
+extern "C" int ghost_bor_set_profile(ghost_bor* h, int on) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  h->profile = on != 0;
+  h->recs.clear();
+  return 0;
+}
+
+// Per kernel family since the last ghost_bor_set_profile: summed ms, launch
+// count and algorithmic flops (GEMM families), each out array of length
+// ghost_bor_kernel_count().
+extern "C" int ghost_bor_kernel_count(void) { return K_COUNT; }
+
+extern "C" const char* ghost_bor_kernel_name(int kind) {
+  return (kind >= 0 && kind < K_COUNT) ? kKernelNames[kind] : "";
+}
+
+extern "C" int ghost_bor_kernel_times(ghost_bor* h, double* ms, int* count, double* flops) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int k = 0; k < K_COUNT; ++k) { ms[k] = 0.0; count[k] = 0; flops[k] = 0.0; }
+  for (const auto& r : h->recs) {
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, h->pev[r.e0], h->pev[r.e0 + 1]));
+    ms[r.kind] += t;
+    count[r.kind] += 1;
+    flops[r.kind] += r.flops;
+  }
+  return 0;
+}
+
+extern "C" int ghost_bor_sync(ghost_bor* h) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+extern "C" int ghost_bor_timings(ghost_bor* h, float* ms3) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipEventElapsedTime(&ms3[0], h->ev[0], h->ev[1]));
+  HIPCHK(hipEventElapsedTime(&ms3[1], h->ev[2], h->ev[3]));
+  HIPCHK(hipEventElapsedTime(&ms3[2], h->ev[4], h->ev[5]));
+  return 0;
+}
+
+extern "C" int ghost_bor_num_systems(ghost_bor* h, int prob, int* nmodes, int* m_max) {
+  if (!h || prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index");
+  *nmodes = h->probs[prob].nmodes;
+  *m_max = h->probs[prob].M;
+  return 0;
+}
+
+extern "C" int ghost_bor_system_info(ghost_bor* h, int prob, int m, int* n, int* nt, int* t0, int* f0) {
+  if (!h || prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index");
+  const Problem& p = h->probs[prob];
+  if (m < -p.M || m > p.M) return fail(GHOST_ERR_ARG, "mode %d outside [-%d, %d]", m, p.M, p.M);
+  const SysInfo& s = h->sys[p.sys0 + m + p.M];
+  *n = s.n; *nt = s.nt; *t0 = s.t0; *f0 = s.f0;
+  return 0;
+}
+
+extern "C" int ghost_bor_get_dF(ghost_bor* h, int prob, double* out) {
+  // out: complex [2M+1][2][n_theta] interleaved (mode index m + M, pol VV = 0, HH = 1)
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index");
+  const Problem& p = h->probs[prob];
+  std::vector<double2> buf((size_t)p.nmodes * p.nrhs);
+  HIPCHK(hipMemcpyAsync(buf.data(), h->d_dF + p.dF_off, buf.size() * sizeof(double2),
+                        hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int mi = 0; mi < p.nmodes; ++mi)
+    for (int it = 0; it < p.ntheta; ++it)
+      for (int pol = 0; pol < 2; ++pol) {
+        const double2 v = buf[(size_t)mi * p.nrhs + 2 * it + pol];
+        const size_t o = (((size_t)mi * 2 + pol) * p.ntheta + it) * 2;
+        out[o] = v.x;
+        out[o + 1] = v.y;
+      }
+  return 0;
+}
+
+// Copy one mode system out: A (n x n), B (n x nrhs), X (n x nrhs), complex interleaved, row-major.
+// which: 0 = A and B as assembled (call after fill, before solve), 1 = X (after solve).
+extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, double* out1, double* out2) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index");
+  const Problem& p = h->probs[prob];
+  if (m < -p.M || m > p.M) return fail(GHOST_ERR_ARG, "mode outside range");
+  const SysInfo& s = h->sys[p.sys0 + m + p.M];
+  const bool mixed = h->precision == GHOST_MIXED;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
+  std::vector<char> rowbuf((size_t)s.ld * csz);
+  auto get = [&](int r, int c) -> double2 {
+    if (mixed) { float2 v = ((float2*)rowbuf.data())[c]; return make_double2(v.x, v.y); }
+    return ((double2*)rowbuf.data())[c];
+  };
+  if (which == 0) {
+    for (int r = 0; r < s.n; ++r) {
+      HIPCHK(hipMemcpy(rowbuf.data(), (char*)h->d_A + ((size_t)s.a_off + (size_t)r * s.ld) * csz, (size_t)s.ld * csz,
+                       hipMemcpyDeviceToHost));
+      for (int c = 0; c < s.n; ++c) { double2 v = get(r, c); out1[2 * ((size_t)r * s.n + c)] = v.x; out1[2 * ((size_t)r * s.n + c) + 1] = v.y; }
+      if (out2)
+        for (int c = 0; c < s.nrhs; ++c) { double2 v = get(r, s.n + c); out2[2 * ((size_t)r * s.nrhs + c)] = v.x; out2[2 * ((size_t)r * s.nrhs + c) + 1] = v.y; }
+    }
+    if (mixed && h->d_A64) {   // the FP64 copies are exact; prefer them
+      HIPCHK(hipMemcpy(out1, h->d_A64 + s.a64_off, (size_t)s.n * s.n * sizeof(double2), hipMemcpyDeviceToHost));
+      if (out2) HIPCHK(hipMemcpy(out2, h->d_B64 + s.x_off, (size_t)s.n * s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
+    }
+  } else {
+    if (mixed) {
+      HIPCHK(hipMemcpy(out1, h->d_X64 + s.x_off, (size_t)s.n * s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
+    } else {
+      for (int r = 0; r < s.n; ++r) {
+        HIPCHK(hipMemcpy(rowbuf.data(), (char*)h->d_A + ((size_t)s.a_off + (size_t)r * s.ld) * csz, (size_t)s.ld * csz,
+                         hipMemcpyDeviceToHost));
+        for (int c = 0; c < s.nrhs; ++c) { double2 v = get(r, s.n + c); out1[2 * ((size_t)r * s.nrhs + c)] = v.x; out1[2 * ((size_t)r * s.nrhs + c) + 1] = v.y; }
+      }
+    }
+  }
+  return 0;
+}
+
+extern "C" int ghost_bor_info(ghost_bor* h, int* out) {
+  // per-system count of zero pivots met by the panel factorisation
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  HIPCHK(hipMemcpy(out, h->d_info, h->sys.size() * sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// compat samplers: the bor_stream_kernel.c entry points, HIP-backed
+// (host pointers in and out, synchronous, reentrant)
+// ─────────────────────────────────────────────────────────────────────────────
+
+struct ScopedBufs {
+  std::vector<void*> p;
+  hipStream_t s = nullptr;
+  ~ScopedBufs() {
+    if (s) hipStreamSynchronize(s);
+    for (void* q : p) hipFree(q);
+    if (s) hipStreamDestroy(s);
+  }
+  template <typename T> T* up(const T* src, size_t n) {
+    void* d = nullptr;
+    if (hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    p.push_back(d);
+    if (src && hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess) return nullptr;
+    return (T*)d;
+  }
+};
+
+extern "C" int ghost_sample_g(int nr, int np_, int nxi, const double* rho_p, const double* z_p,
+                              const double* rho_q, const double* z_q, double k, const double* sin2_tab,
+                              double* out) {
+  if (nr <= 0 || np_ <= 0 || nxi <= 0) return 0;
+  ScopedBufs b;
+  HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
+  const double *a = b.up(rho_p, nr), *bz = b.up(z_p, nr), *c = b.up(rho_q, np_), *d = b.up(z_q, np_);
+  const double* t = b.up(sin2_tab, nxi);
+  const size_t tot = (size_t)nr * np_ * nxi;
+  double2* o = b.up((const double2*)nullptr, tot);
+  if (!a || !bz || !c || !d || !t || !o) return fail(GHOST_ERR_OOM, "sample_g: device allocation failed");
+  hipLaunchKernelGGL(k_sample_g, dim3((unsigned)std::min<size_t>(65535, (tot + 255) / 256)), dim3(256), 0, b.s,
+                     nr, np_, nxi, a, bz, c, d, k, t, o);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(out, o, tot * sizeof(double2), hipMemcpyDeviceToHost, b.s));
+  HIPCHK(hipStreamSynchronize(b.s));
+  return 0;
+}
+
+static int sample_brackets(bool ibc, int nr, int np_, int nxi, const double* rho_p, const double* z_p,
+                           const double* tr_p, const double* tz_p, const double* rho_q, const double* z_q,
+                           const double* tr_q, const double* tz_q, double k, const double* cx_tab,
+                           const double* sx_tab, double* o_tt, double* o_tf, double* o_ft, double* o_ff) {
+  if (nr <= 0 || np_ <= 0 || nxi <= 0) return 0;
+  ScopedBufs b;
+  HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
+  const double *rp = b.up(rho_p, nr), *zp = b.up(z_p, nr), *trp = b.up(tr_p, nr), *tzp = b.up(tz_p, nr);
+  const double *rq = b.up(rho_q, np_), *zq = b.up(z_q, np_), *trq = b.up(tr_q, np_), *tzq = b.up(tz_q, np_);
+  const double *cx = b.up(cx_tab, nxi), *sx = b.up(sx_tab, nxi);
+  const size_t tot = (size_t)nr * np_ * nxi;
+  double2* o[4];
+  for (int i = 0; i < 4; ++i) o[i] = b.up((const double2*)nullptr, tot);
+  if (!rp || !zp || !trp || !tzp || !rq || !zq || !trq || !tzq || !cx || !sx || !o[0] || !o[1] || !o[2] || !o[3])
+    return fail(GHOST_ERR_OOM, "sample brackets: device allocation failed");
+  const dim3 g((unsigned)std::min<size_t>(65535, (tot + 255) / 256));
+  if (ibc)
+    hipLaunchKernelGGL(k_sample_brackets<true>, g, dim3(256), 0, b.s, nr, np_, nxi, rp, zp, trp, tzp, rq, zq, trq,
+                       tzq, k, cx, sx, o[0], o[1], o[2], o[3]);
+  else
+    hipLaunchKernelGGL(k_sample_brackets<false>, g, dim3(256), 0, b.s, nr, np_, nxi, rp, zp, trp, tzp, rq, zq, trq,
+                       tzq, k, cx, sx, o[0], o[1], o[2], o[3]);
+  LAUNCHCHK();
+  double* outs[4] = {o_tt, o_tf, o_ft, o_ff};
+  for (int i = 0; i < 4; ++i)
+    HIPCHK(hipMemcpyAsync(outs[i], o[i], tot * sizeof(double2), hipMemcpyDeviceToHost, b.s));
+  HIPCHK(hipStreamSynchronize(b.s));
+  return 0;
+}
+
+extern "C" int ghost_sample_mfie(int nr, int np_, int nxi, const double* rho_p, const double* z_p,
+                                 const double* tr_p, const double* tz_p, const double* rho_q,
+                                 const double* z_q, const double* tr_q, const double* tz_q, double k,
+                                 const double* cx_tab, const double* sx_tab, double* o_tt, double* o_tf,
+                                 double* o_ft, double* o_ff) {
+  return sample_brackets(false, nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab,
+                         o_tt, o_tf, o_ft, o_ff);
+}
+
+extern "C" int ghost_sample_ibc(int nr, int np_, int nxi, const double* rho_p, const double* z_p,
+                                const double* tr_p, const double* tz_p, const double* rho_q,
+                                const double* z_q, const double* tr_q, const double* tz_q, double k,
+                                const double* cx_tab, const double* sx_tab, double* o_tt, double* o_tf,
+                                double* o_ft, double* o_ff) {
+  return sample_brackets(true, nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab,
+                         o_tt, o_tf, o_ft, o_ff);
+}
+
+// void-returning names with the exact bor_stream_kernel.c signatures
+// (bor_stream_kernel.c:26-29, 65-72, 108-115): errors surface through
+// ghost_last_error() because the reference ABI has no error channel.
+extern "C" void sample_g(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* rho_q,
+                         const double* z_q, double k, const double* sin2_tab, double* out) {
+  ghost_sample_g(nr, np_, nxi, rho_p, z_p, rho_q, z_q, k, sin2_tab, out);
+}
+
+extern "C" void sample_mfie(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* tr_p,
+                            const double* tz_p, const double* rho_q, const double* z_q, const double* tr_q,
+                            const double* tz_q, double k, const double* cx_tab, const double* sx_tab,
+                            double* o_tt, double* o_tf, double* o_ft, double* o_ff) {
+  ghost_sample_mfie(nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab, o_tt, o_tf,
+                    o_ft, o_ff);
+}
+
+extern "C" void sample_ibc(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* tr_p,
+                           const double* tz_p, const double* rho_q, const double* z_q, const double* tr_q,
+                           const double* tz_q, double k, const double* cx_tab, const double* sx_tab,
+                           double* o_tt, double* o_tf, double* o_ft, double* o_ff) {
+  ghost_sample_ibc(nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab, o_tt, o_tf,
+                   o_ft, o_ff);
+}

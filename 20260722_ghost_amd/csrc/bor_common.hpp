@@ -1,0 +1,85 @@
+// Shared device helpers for the BoR-MoM HIP library (gfx950 / CDNA4).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#define GH_PI 3.14159265358979323846
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <typename R> struct CT;
+template <> struct CT<double> { typedef double2 c; typedef d4v v4; };
+template <> struct CT<float> { typedef float2 c; typedef f4v v4; };
+
+// One 16x16x4 MFMA step.  Lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// the accumulator holds D[4*(l/16)+r][l%16] in element r.
+__device__ __forceinline__ d4v mfma4(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4v mfma4(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Row of the 16x16 result held by lane group kk = l/16 in accumulator element r
+// (measured on gfx950: FP32 4*kk + r, FP64 kk + 4*r); the column is l%16.
+template <typename R> __device__ __forceinline__ constexpr int mrow(int kk, int r);
+template <> __device__ __forceinline__ constexpr int mrow<float>(int kk, int r) { return 4 * kk + r; }
+template <> __device__ __forceinline__ constexpr int mrow<double>(int kk, int r) { return kk + 4 * r; }
+
+__device__ __forceinline__ void sincos_r(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ void sincos_r(float x, float* s, float* c) { sincosf(x, s, c); }
+
+// Row-rotate a 64-bit value within each 16-lane DPP row: lane c receives
+// the value of lane (c - N) mod 16 of the same row.
+template <int N> __device__ __forceinline__ double row_ror(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x120 + N, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x120 + N, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int N> __device__ __forceinline__ float row_ror(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + N, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// Per-Gauss-point generatrix data, element-major (4 points per element).
+struct PointSet {
+  const double* rho;
+  const double* z;
+  const double* trho;
+  const double* tz;
+  const double* w;    // Gauss weight x element length
+  const double* T0;
+  const double* T1;
+  const double* D0;   // d(rho T0)/dt
+  const double* D1;   // d(rho T1)/dt
+};
+
+// One assembled mode system inside a batch.
+struct SysInfo {
+  int prob;      // owning problem (body x frequency)
+  int m;         // azimuthal mode
+  int n;         // masked system size
+  int nt;        // active J_t bases (rows [0, nt))
+  int t0;        // first active J_t node
+  int f0;        // first active J_phi node (rows [nt, n))
+  int ld;        // leading dimension of the augmented row-major system
+  int nrhs;      // right-hand sides (2 x n_theta)
+  long long a_off;   // element offset of the augmented system [n x ld]
+  long long x_off;   // element offset of the n x nrhs solution / RHS copy
+  long long a64_off; // element offset of the FP64 n x n copy (mixed precision)
+};

@@ -1,0 +1,264 @@
+// Plane-wave excitation and monostatic far field for every mode at once.
+//
+// Reference algorithm: bor_solver.py:625-645 (EFIE rhs), 437-461 (MFIE rhs),
+// 659-708 (far field; note the (j_{m+1}, j_{m-1}) order there), 940-950
+// (CFIE mix).  The Bessel functions J_n(k rho sin th), n = 0..M+1, that the
+// reference obtains per call from scipy.special.jv are produced here for all
+// orders of one (point, angle) by Miller's backward recurrence normalised with
+// J_0 + 2 sum J_2k = 1 (power series below x = 1).
+#include "bor_common.hpp"
+
+namespace ghost {
+
+struct ExciteArgs {
+  PointSet pts;
+  int P, Nn, M, ntheta;
+  double k;
+  double cE, cM;          // V = cE V_E + cM V_H
+  const double* theta;    // [ntheta] radians
+  double* J;              // [M+2][P][ntheta] Bessel table
+};
+
+// J_n(x), n = 0..nmax written to out[n * stride].
+__device__ void bessel_orders(double x, int nmax, double* out, size_t stride) {
+  if (x < 1e-300) {
+    out[0] = 1.0;
+    for (int n = 1; n <= nmax; ++n) out[n * stride] = 0.0;
+    return;
+  }
+  if (x < 1.0) {
+    // J_n(x) = sum_k (-1)^k (x/2)^(2k+n) / (k! (n+k)!)
+    const double h = 0.5 * x, h2 = h * h;
+    double lead = 1.0;  // (x/2)^n / n!
+    for (int n = 0; n <= nmax; ++n) {
+      if (n > 0) lead *= h / n;
+      double term = lead, sum = lead;
+      for (int kk = 1; kk < 40; ++kk) {
+        term *= -h2 / ((double)kk * (double)(n + kk));
+        sum += term;
+        if (fabs(term) < 1e-18 * fabs(sum)) break;
+      }
+      out[n * stride] = sum;
+    }
+    return;
+  }
+  // Miller: start well above max(nmax, x); rescale by 1e-250 when values grow.
+  // Pass 1 finds the normalisation and the number of rescales; pass 2 repeats
+  // the identical recurrence and stores each order in the final scale.
+  const double top = fmax((double)nmax, x);
+  int N = (int)(top + 20.0 + sqrt(40.0 * top));
+  N += N & 1;
+  const double BIG = 1e250, SMALL = 1e-250;
+  int nres = 0;
+  double norm = 0.0;
+  {
+    double fp = 0.0, f = 1e-280;
+    for (int n = N; n >= 1; --n) {
+      const double fm = (2.0 * n / x) * f - fp;
+      fp = f;
+      f = fm;
+      if ((n - 1) > 0 && ((n - 1) & 1) == 0) norm += 2.0 * f;
+      if (fabs(f) > BIG) { f *= SMALL; fp *= SMALL; norm *= SMALL; ++nres; }
+    }
+    norm += f;  // f is now f_0
+  }
+  {
+    double fp = 0.0, f = 1e-280;
+    int r = 0;
+    for (int n = N; n >= 1; --n) {
+      const double fm = (2.0 * n / x) * f - fp;
+      fp = f;
+      f = fm;
+      if (fabs(f) > BIG) { f *= SMALL; fp *= SMALL; ++r; }
+      const int o = n - 1;
+      if (o <= nmax) {
+        double v = f / norm;
+        for (int q = r; q < nres; ++q) v *= SMALL;
+        out[o * stride] = v;
+      }
+    }
+  }
+}
+
+__global__ void k_bessel(ExciteArgs a) {
+  const size_t tot = (size_t)a.P * a.ntheta;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int it = (int)(t % a.ntheta);
+    const int p = (int)(t / a.ntheta);
+    const double x = a.k * a.pts.rho[p] * sin(a.theta[it]);
+    bessel_orders(fabs(x), a.M + 1, a.J + (size_t)p * a.ntheta + it, (size_t)a.P * a.ntheta);
+  }
+}
+
+// j_n = i^n J_n(u) with n >= 0 (j_{-n} = j_n for integer n)
+__device__ __forceinline__ double2 jn_val(const double* J, int n, size_t stride) {
+  const double v = J[(size_t)n * stride];
+  switch (n & 3) {
+    case 0: return make_double2(v, 0.0);
+    case 1: return make_double2(0.0, v);
+    case 2: return make_double2(-v, 0.0);
+    default: return make_double2(0.0, -v);
+  }
+}
+
+// RHS columns theta-major then (VV, HH), written into the augmented system
+// columns [n, n + nrhs) (bor_solver.py:750-752).
+template <typename Ra>
+__global__ __launch_bounds__(256) void k_rhs(ExciteArgs a, const SysInfo* sys, void* Abase,
+                                             double2* B64, int sys0) {
+  typedef typename CT<Ra>::c Ca;
+  const SysInfo s = sys[blockIdx.z];
+  const int r = blockIdx.y;
+  if (r >= s.n) return;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= s.nrhs) return;
+  const int it = col >> 1, pol = col & 1;
+  const bool phi = r >= s.nt;
+  const int node = phi ? s.f0 + (r - s.nt) : s.t0 + r;
+  const int am = abs(s.m);
+  const double th = a.theta[it];
+  double st, ct;
+  sincos(th, &st, &ct);
+  const size_t stride = (size_t)a.P * a.ntheta;
+  double2 acc = make_double2(0.0, 0.0);
+  const PointSet& P = a.pts;
+  const int Ne = a.Nn - 1;
+  for (int side = 0; side < 2; ++side) {
+    const int e = node - 1 + side;  // element e with this node as T1 (side 0) or T0 (side 1)
+    if (e < 0 || e >= Ne) continue;
+    for (int gq = 0; gq < 4; ++gq) {
+      const int p = 4 * e + gq;
+      const double T = side == 0 ? P.T1[p] : P.T0[p];
+      const double* Jp = a.J + (size_t)p * a.ntheta + it;
+      const double2 jm = jn_val(Jp, am, stride);
+      const double2 jm1 = jn_val(Jp, abs(s.m - 1), stride);
+      const double2 jp1 = jn_val(Jp, abs(s.m + 1), stride);
+      // Ic = pi (j_{m-1} + j_{m+1}),  Is = -j pi (j_{m-1} - j_{m+1}),  I1 = 2 pi j_m
+      const double2 Ic = make_double2(GH_PI * (jm1.x + jp1.x), GH_PI * (jm1.y + jp1.y));
+      const double2 Is = make_double2(GH_PI * (jm1.y - jp1.y), -GH_PI * (jm1.x - jp1.x));
+      const double2 I1 = make_double2(2.0 * GH_PI * jm.x, 2.0 * GH_PI * jm.y);
+      const double tr = P.trho[p], tz = P.tz[p];
+      double2 eE, eH;
+      if (pol == 0) {  // VV
+        if (!phi) {
+          eE = make_double2(ct * tr * Ic.x - st * tz * I1.x, ct * tr * Ic.y - st * tz * I1.y);
+          eH = make_double2(Ic.x / 376.730313668, Ic.y / 376.730313668);
+        } else {
+          eE = make_double2(-ct * Is.x, -ct * Is.y);
+          eH = make_double2(-tr * Is.x / 376.730313668, -tr * Is.y / 376.730313668);
+        }
+      } else {         // HH
+        if (!phi) {
+          eE = make_double2(tr * Is.x, tr * Is.y);
+          eH = make_double2(ct * Is.x / 376.730313668, ct * Is.y / 376.730313668);
+        } else {
+          eE = Ic;
+          eH = make_double2((ct * tr * Ic.x - st * tz * I1.x) / 376.730313668,
+                            (ct * tr * Ic.y - st * tz * I1.y) / 376.730313668);
+        }
+      }
+      const double2 ev = make_double2(a.cE * eE.x + a.cM * eH.x, a.cE * eE.y + a.cM * eH.y);
+      double sp, cp;
+      sincos(a.k * ct * P.z[p], &sp, &cp);  // e^{j k cos(th) z}
+      const double wgt = T * P.w[p] * P.rho[p];
+      acc.x += wgt * (ev.x * cp - ev.y * sp);
+      acc.y += wgt * (ev.x * sp + ev.y * cp);
+    }
+  }
+  Ca* A = (Ca*)Abase + s.a_off;
+  Ca o; o.x = (Ra)acc.x; o.y = (Ra)acc.y;
+  A[(size_t)r * s.ld + s.n + col] = o;
+  if (B64) B64[s.x_off + (size_t)r * s.nrhs + col] = acc;
+}
+
+// Partial far-field sums over a chunk of Gauss points for every (system, column).
+// X is read from rows of an n x nrhs block at element offset x_off with leading
+// dimension ldx (the augmented system or a separate solution buffer).
+template <typename Rx>
+__global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const SysInfo* sys,
+                                                          const void* Xbase, int x_in_aug, int chunk,
+                                                          double2* partial, int nsys) {
+  typedef typename CT<Rx>::c Cx;
+  const int si = blockIdx.y;
+  const SysInfo s = sys[si];
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= s.nrhs) return;
+  const int pc = blockIdx.z;
+  const int p0 = pc * chunk, p1 = min(a.P, p0 + chunk);
+  const int it = col >> 1, pol = col & 1;
+  const double th = a.theta[it];
+  double st, ct;
+  sincos(th, &st, &ct);
+  const size_t stride = (size_t)a.P * a.ntheta;
+  const Cx* X;
+  size_t ldx;
+  if (x_in_aug) { X = (const Cx*)Xbase + s.a_off + s.n; ldx = s.ld; }
+  else { X = (const Cx*)Xbase + s.x_off; ldx = s.nrhs; }
+  const int nf = s.n - s.nt;
+  const PointSet& P = a.pts;
+  auto xt = [&](int node) -> double2 {
+    const int r = node - s.t0;
+    if (r < 0 || r >= s.nt) return make_double2(0.0, 0.0);
+    const Cx v = X[(size_t)r * ldx + col];
+    return make_double2((double)v.x, (double)v.y);
+  };
+  auto xf = [&](int node) -> double2 {
+    const int r = node - s.f0;
+    if (r < 0 || r >= nf) return make_double2(0.0, 0.0);
+    const Cx v = X[(size_t)(s.nt + r) * ldx + col];
+    return make_double2((double)v.x, (double)v.y);
+  };
+  double2 acc = make_double2(0.0, 0.0);
+  int e_prev = -1;
+  double2 t0v = make_double2(0, 0), t1v = t0v, f0v = t0v, f1v = t0v;
+  for (int p = p0; p < p1; ++p) {
+    const int e = p >> 2;
+    if (e != e_prev) { t0v = xt(e); t1v = xt(e + 1); f0v = xf(e); f1v = xf(e + 1); e_prev = e; }
+    const double T0 = P.T0[p], T1 = P.T1[p];
+    const double2 Jt = make_double2(T0 * t0v.x + T1 * t1v.x, T0 * t0v.y + T1 * t1v.y);
+    const double2 Jf = make_double2(T0 * f0v.x + T1 * f1v.x, T0 * f0v.y + T1 * f1v.y);
+    const double* Jp = a.J + (size_t)p * a.ntheta + it;
+    const double2 jm = jn_val(Jp, abs(s.m), stride);
+    const double2 jm1 = jn_val(Jp, abs(s.m - 1), stride);
+    const double2 jp1 = jn_val(Jp, abs(s.m + 1), stride);
+    // Icos = pi (j_{m+1} + j_{m-1}),  Isin = -j pi (j_{m+1} - j_{m-1}),  I1 = 2 pi j_m
+    const double2 Icos = make_double2(GH_PI * (jp1.x + jm1.x), GH_PI * (jp1.y + jm1.y));
+    const double2 Isin = make_double2(GH_PI * (jp1.y - jm1.y), -GH_PI * (jp1.x - jm1.x));
+    const double2 I1 = make_double2(2.0 * GH_PI * jm.x, 2.0 * GH_PI * jm.y);
+    const double tr = P.trho[p], tz = P.tz[p];
+    double2 term;
+    if (pol == 0) {
+      // Jt (ct tr Icos - st tz I1) - Jf ct Isin
+      const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
+      term = make_double2(Jt.x * u.x - Jt.y * u.y - ct * (Jf.x * Isin.x - Jf.y * Isin.y),
+                          Jt.x * u.y + Jt.y * u.x - ct * (Jf.x * Isin.y + Jf.y * Isin.x));
+    } else {
+      // Jt tr Isin + Jf Icos
+      term = make_double2(tr * (Jt.x * Isin.x - Jt.y * Isin.y) + Jf.x * Icos.x - Jf.y * Icos.y,
+                          tr * (Jt.x * Isin.y + Jt.y * Isin.x) + Jf.x * Icos.y + Jf.y * Icos.x);
+    }
+    double sp, cp;
+    sincos(a.k * ct * P.z[p], &sp, &cp);
+    const double wgt = P.w[p] * P.rho[p];
+    acc.x += wgt * (term.x * cp - term.y * sp);
+    acc.y += wgt * (term.x * sp + term.y * cp);
+  }
+  partial[((size_t)pc * nsys + si) * s.nrhs + col] = acc;
+}
+
+// Sum the chunk partials in chunk order and apply -j k eta0 / (4 pi).
+__global__ void k_farfield_reduce(const double2* partial, int nchunk, int nsys, int nrhs, double k,
+                                  double2* dF) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsys * nrhs) return;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int c = 0; c < nchunk; ++c) {
+    const double2 v = partial[(size_t)c * nsys * nrhs + t];
+    acc.x += v.x; acc.y += v.y;
+  }
+  const double pref = k * 376.730313668 / (4.0 * GH_PI);  // -j * pref
+  dF[t] = make_double2(pref * acc.y, -pref * acc.x);
+}
+
+}  // namespace ghost

@@ -1,0 +1,774 @@
+// Modal impedance fill for the BoR-MoM hot path on gfx950.
+//
+// Far element pairs (|e-f| >= 2): one wavefront per element pair samples the
+// azimuthal integrands on the uniform xi grid, folds +-xi (the Green's
+// function and the MFIE brackets have definite parity in xi), and turns the
+// folded samples into every azimuthal order at once with a cosine/sine
+// transform run as a 16x16x4 MFMA contraction against a twiddle table.  The
+// 16 point pairs of the element pair are contracted to the 2x2 nodal blocks
+// in registers (DPP/shuffle reductions) and written as complex Z entries for
+// modes +m and -m into a mode-innermost buffer Zfar[2Nn][2Nn][2M+1].
+//
+// Reference algorithm: bor_kernels.py:132-167 (G_m by FFT), 273-331 (MFIE
+// brackets), bor_streaming.py:320-414 (nodal contraction, 9 EFIE order
+// primitives, per-mode recombination), bor_solver.py:130-154, 293-345,
+// 399-435 (mode blocks), bor_stream_kernel.c:26-106 (integrands).
+//
+// Near element pairs (|e-f| <= 1): graded-cell quadrature with a sinh core
+// and an oscillation-scaled Gauss tail per point pair (bor_kernels.py:170-257,
+// 360-431; bor_solver.py:54-110, 293-345, 399-435), in FP64.
+#include "bor_common.hpp"
+
+namespace ghost {
+
+// ─────────────────────────────────────────────────────────────────────────────
+// far fill
+// ─────────────────────────────────────────────────────────────────────────────
+
+struct FarArgs {
+  PointSet pts;
+  const double* sin2E;   // [nE/2+1]  sin^2(pi j / nE)
+  const double* cosE;    // [nE]      cos(2 pi i / nE)
+  const double* cxB;     // [nB/2+1]  cos(2 pi j / nB)
+  const double* sxB;     // [nB/2+1]  sin(2 pi j / nB)
+  const double* cosB;    // [nB]
+  const double* sinB;    // [nB]
+  int Ne, Nn, M, nE, nB, pe, pf, nfp, npairs;
+  double k, ce_re, ce_im, cm;
+  void* Z;               // Zfar [2Nn][2Nn][2M+1] complex<R>
+};
+
+// Sum 32 per-lane values over the 16 lanes of a DPP row and scatter the
+// result: lane r (= lane & 15) receives the sums of values r and 16 + r.
+template <typename R>
+__device__ __forceinline__ void row_reduce_scatter32(const R (&v)[32], int lane, R& out_lo, R& out_hi) {
+  R a16[16];
+  {
+    const bool b = (lane >> 3) & 1;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+      if (i & 8) continue;
+      const R lo = v[i], hi = v[i | 8];
+      const R keep = b ? hi : lo, send = b ? lo : hi;
+      a16[(i & 7) | ((i >> 4) << 3)] = keep + __shfl_xor(send, 8);
+    }
+  }
+  R a8[8];
+  {
+    const bool b = (lane >> 2) & 1;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i & 4) continue;
+      const R lo = a16[i], hi = a16[i | 4];
+      const R keep = b ? hi : lo, send = b ? lo : hi;
+      a8[(i & 3) | ((i >> 3) << 2)] = keep + __shfl_xor(send, 4);
+    }
+  }
+  R a4[4];
+  {
+    const bool b = (lane >> 1) & 1;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i & 2) continue;
+      const R lo = a8[i], hi = a8[i | 2];
+      const R keep = b ? hi : lo, send = b ? lo : hi;
+      a4[(i & 1) | ((i >> 2) << 1)] = keep + __shfl_xor(send, 2);
+    }
+  }
+  const bool b = lane & 1;
+  {
+    const R keep = b ? a4[1] : a4[0], send = b ? a4[0] : a4[1];
+    out_lo = keep + __shfl_xor(send, 1);
+  }
+  {
+    const R keep = b ? a4[3] : a4[2], send = b ? a4[2] : a4[3];
+    out_hi = keep + __shfl_xor(send, 1);
+  }
+}
+
+// One wavefront per far element pair (e, f); 2 wavefronts per block.
+// Phase 1: EFIE samples -> G_n on MFMA.  Phase 2: per-order EFIE blocks,
+// reduced over the test points and stashed in LDS.  Phase 3: MFIE brackets
+// contracted to nodal rows -> modes on MFMA.  Phase 4: combine, RMW Zfar.
+template <typename R, int NT>
+__global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
+  typedef typename CT<R>::c C2;
+  typedef typename CT<R>::v4 V4;
+  __shared__ R stash[2][NT][8][64];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int wid = blockIdx.x * 2 + wl;
+  if (wid >= a.npairs) return;  // wave-uniform exit (no block barriers below)
+  const int e = a.pe + 2 * (wid / a.nfp);
+  const int f = a.pf + 2 * (wid % a.nfp);
+  if (e >= a.Ne || f >= a.Ne || abs(e - f) <= 1) return;
+
+  const int c16 = lane & 15, kk = lane >> 4;
+  const R k = (R)a.k;
+  const PointSet& P = a.pts;
+
+  // operand lane: the point pair (g, h) whose result lands in lane group
+  // kk = g, accumulator element r = h (row mrow(g, h) = c16)
+  const bool f64 = sizeof(R) == 8;
+  const int g_in = f64 ? (c16 & 3) : (c16 >> 2), h_in = f64 ? (c16 >> 2) : (c16 & 3);
+  const int ip = 4 * e + g_in, iq = 4 * f + h_in;
+  const R rp = (R)P.rho[ip], zp = (R)P.z[ip], rq = (R)P.rho[iq], zq = (R)P.z[iq];
+  const R dz = zp - zq, dr = rp - rq;
+  const R d2 = dr * dr + dz * dz, rr4 = (R)4 * rp * rq;
+  const int uvsub = kk >> 1, asel = kk & 1;
+
+  // ── phase 1: G_n = sum_j c_j (2pi/N) g(2 pi j/N) cos(2 pi n j / N) ──
+  {
+    V4 accR[NT], accI[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) { accR[t] = V4{0, 0, 0, 0}; accI[t] = V4{0, 0, 0, 0}; }
+    {
+      const int hE = a.nE >> 1, mskE = a.nE - 1;
+      const R sc = (R)1 / ((R)2 * (R)a.nE);  // (2 pi / N) / (4 pi)
+      for (int j0 = 0; j0 <= hE; j0 += 4) {
+        const int j = j0 + kk;
+        const int jc = j < hE ? j : hE;
+        const R cw = (j == 0 || j == hE) ? (R)1 : (j > hE ? (R)0 : (R)2);
+        R Rd = sqrt(d2 + rr4 * (R)a.sin2E[jc]);
+        Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
+        R s, c;
+        sincos_r(k * Rd, &s, &c);
+        const R amp = cw * sc / Rd;
+        const R ar = amp * c, ai = -amp * s;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const R bv = (R)a.cosE[((16 * t + c16) * j) & mskE];
+          accR[t] = mfma4(ar, bv, accR[t]);
+          accI[t] = mfma4(ai, bv, accI[t]);
+        }
+      }
+    }
+
+    // ── phase 2: EFIE blocks per order; lane holds G[g = kk][h = r][n = 16 t + c16] ──
+    const int ig = 4 * e + kk;
+    const R trp = (R)P.trho[ig], tzp = (R)P.tz[ig];
+    const R wrp = (R)(P.w[ig] * P.rho[ig]);
+    const R Twa[2] = {(R)P.T0[ig] * wrp, (R)P.T1[ig] * wrp};
+    const R Swa[2] = {(R)(P.T0[ig] * P.w[ig]), (R)(P.T1[ig] * P.w[ig])};
+    const R Dwa[2] = {(R)(P.D0[ig] * P.w[ig]), (R)(P.D1[ig] * P.w[ig])};
+    const R trq = (R)P.trho[4 * f], tzq = (R)P.tz[4 * f];
+    const R ik2 = (R)1 / (k * k);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const R am = (R)(16 * t + c16);
+      R z[32];  // [blk(4)][a(2)][b(2)][reim]: blk 0 tt, 1 tf, 2 ft, 3 ff
+#pragma unroll
+      for (int i = 0; i < 32; ++i) z[i] = (R)0;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const int iq2 = 4 * f + h;
+        const R wr = (R)(P.w[iq2] * P.rho[iq2]);
+        const R Twb[2] = {(R)P.T0[iq2] * wr, (R)P.T1[iq2] * wr};
+        const R Swb[2] = {(R)(P.T0[iq2] * P.w[iq2]), (R)(P.T1[iq2] * P.w[iq2])};
+        const R Dwb[2] = {(R)(P.D0[iq2] * P.w[iq2]), (R)(P.D1[iq2] * P.w[iq2])};
+        const R Gr = accR[t][h], Gi = accI[t][h];
+        // neighbour orders by row rotation: lane c receives lane c-1 (ror 1) / c+1 (ror 15)
+        R Pr = row_ror<15>(accR[t][h]), Pi = row_ror<15>(accI[t][h]);  // G_{n+1}
+        R Mr = row_ror<1>(accR[t][h]), Mi = row_ror<1>(accI[t][h]);    // G_{n-1}
+        if (t + 1 < NT) {
+          const R xr = row_ror<15>(accR[(t + 1) % NT][h]), xi = row_ror<15>(accI[(t + 1) % NT][h]);
+          if (c16 == 15) { Pr = xr; Pi = xi; }
+        }
+        if (t > 0) {
+          const R xr = row_ror<1>(accR[(t + NT - 1) % NT][h]), xi = row_ror<1>(accI[(t + NT - 1) % NT][h]);
+          if (c16 == 0) { Mr = xr; Mi = xi; }
+        } else if (c16 == 0) {
+          Mr = Pr; Mi = Pi;  // G_{|-1|} = G_1
+        }
+        const R Gcr = (R)0.5 * (Mr + Pr), Gci = (R)0.5 * (Mi + Pi);
+        const R Gsr = (R)0.5 * (Mi - Pi), Gsi = (R)-0.5 * (Mr - Pr);  // (M - P)/(2j)
+        const R ktr = trp * trq * Gcr + tzp * tzq * Gr, kti = trp * trq * Gci + tzp * tzq * Gi;
+#pragma unroll
+        for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+          for (int bi = 0; bi < 2; ++bi) {
+            const int ab = 2 * ai + bi;
+            const R TT = Twa[ai] * Twb[bi];
+            const R DD = Dwa[ai] * Dwb[bi] * ik2;
+            const R DS = Dwa[ai] * Swb[bi] * am * ik2;
+            const R SD = Swa[ai] * Dwb[bi] * am * ik2;
+            const R SS = Swa[ai] * Swb[bi] * am * am * ik2;
+            z[0 * 8 + 2 * ab] += TT * ktr - DD * Gr;            // tt: TT kt - DD G
+            z[0 * 8 + 2 * ab + 1] += TT * kti - DD * Gi;
+            z[1 * 8 + 2 * ab] += TT * trp * Gsr + DS * Gi;      // tf: TT trp Gs - j DS G
+            z[1 * 8 + 2 * ab + 1] += TT * trp * Gsi - DS * Gr;
+            z[2 * 8 + 2 * ab] += -TT * trq * Gsr - SD * Gi;     // ft: -TT trq Gs + j SD G
+            z[2 * 8 + 2 * ab + 1] += -TT * trq * Gsi + SD * Gr;
+            z[3 * 8 + 2 * ab] += TT * Gcr - SS * Gr;            // ff: TT Gc - SS G
+            z[3 * 8 + 2 * ab + 1] += TT * Gci - SS * Gi;
+          }
+      }
+      // reduce over g (lanes c16 + 16 kk): stage 1 keeps {tt, tf} (uvsub 0) or {ff, ft}
+      R y16[16];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const R lo = z[0 * 8 + i], hi = z[3 * 8 + i];   // tt <-> ff
+        const R keep = uvsub ? hi : lo, send = uvsub ? lo : hi;
+        y16[i] = keep + __shfl_xor(send, 32);
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const R lo = z[1 * 8 + i], hi = z[2 * 8 + i];   // tf <-> ft
+        const R keep = uvsub ? hi : lo, send = uvsub ? lo : hi;
+        y16[8 + i] = keep + __shfl_xor(send, 32);
+      }
+      // stage 2 keeps a = asel; stash (even, odd) x (b, reim)
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const R lo = y16[8 * q + i], hi = y16[8 * q + 4 + i];
+          const R keep = asel ? hi : lo, send = asel ? lo : hi;
+          stash[wl][t][4 * q + i][lane] = keep + __shfl_xor(send, 16);
+        }
+    }
+  }
+
+  // ── phase 3: MFIE brackets, contracted to nodal (a, b) rows before the transform ──
+  V4 accE[NT], accO[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) { accE[t] = V4{0, 0, 0, 0}; accO[t] = V4{0, 0, 0, 0}; }
+  {
+    const R trp = (R)P.trho[ip], tzp = (R)P.tz[ip], trq = (R)P.trho[iq], tzq = (R)P.tz[iq];
+    const R wp = (R)(P.w[ip] * P.rho[ip]), wq = (R)(P.w[iq] * P.rho[iq]);
+    const R Wa0 = (R)P.T0[ip] * wp, Wa1 = (R)P.T1[ip] * wp;
+    const R Wb0 = (R)P.T0[iq] * wq, Wb1 = (R)P.T1[iq] * wq;
+    const R Wab[4] = {Wa0 * Wb0, Wa0 * Wb1, Wa1 * Wb0, Wa1 * Wb1};
+    const int hB = a.nB >> 1, mskB = a.nB - 1;
+    const R scB = (R)1 / ((R)2 * (R)a.nB);  // (2 pi/N)/(4 pi)
+    for (int j0 = 0; j0 <= hB; j0 += 4) {
+      const int j = j0 + kk;
+      const int jc = j < hB ? j : hB;
+      const R cw = (j == 0 || j == hB) ? (R)1 : (j > hB ? (R)0 : (R)2);
+      const R cx = (R)a.cxB[jc], sx = (R)a.sxB[jc];
+      const R Rx = rp - rq * cx, Ry = rq * sx;
+      R Rd = sqrt(Rx * Rx + Ry * Ry + dz * dz);
+      Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
+      R s, c;
+      sincos_r(k * Rd, &s, &c);
+      const R inv = (R)1 / Rd;
+      const R pre = cw * scB * inv * inv * inv;
+      const R kr = k * Rd;
+      const R pr = pre * (c + kr * s), pim = pre * (kr * c - s);
+      const R WtR = trp * Rx + tzp * dz, nR = -tzp * Rx + trp * dz;
+      const R n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
+      const R Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
+      const R btt = -(WtR * n_t - Wt_t * nR);
+      const R btf = -(WtR * n_f - Wt_f * nR);
+      const R bft = -(Ry * n_t - Wf_t * nR);
+      const R bff = -(Ry * n_f - cx * nR);
+      // value 16*parity + mrow(a + 2 uvsub, reim + 2 b): after the row
+      // reduction the transform leaves (uvsub, a) in lane group kk = a + 2 uvsub
+      // and (b, reim) in accumulator element r = reim + 2 b.
+      // parity 0: uvsub 0 tt, 1 ff;  parity 1: uvsub 0 tf, 1 ft.
+      R v[32];
+#pragma unroll
+      for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+          const R W = Wab[2 * ai + bi];
+          v[mrow<R>(ai, 2 * bi)] = W * btt * pr;          v[mrow<R>(ai, 2 * bi + 1)] = W * btt * pim;
+          v[mrow<R>(ai + 2, 2 * bi)] = W * bff * pr;      v[mrow<R>(ai + 2, 2 * bi + 1)] = W * bff * pim;
+          v[16 + mrow<R>(ai, 2 * bi)] = W * btf * pr;     v[16 + mrow<R>(ai, 2 * bi + 1)] = W * btf * pim;
+          v[16 + mrow<R>(ai + 2, 2 * bi)] = W * bft * pr; v[16 + mrow<R>(ai + 2, 2 * bi + 1)] = W * bft * pim;
+        }
+      R ve, vo;
+      row_reduce_scatter32<R>(v, lane, ve, vo);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const int idx = ((16 * t + c16) * j) & mskB;
+        accE[t] = mfma4(ve, (R)a.cosB[idx], accE[t]);
+        accO[t] = mfma4(vo, (R)a.sinB[idx], accO[t]);
+      }
+    }
+  }
+
+  // ── phase 4: Z = ce * EFIE + cm * K for modes +n and -n ──
+  const R ce_re = (R)a.ce_re, ce_im = (R)a.ce_im, cm = (R)a.cm;
+  const int nmodes = 2 * a.M + 1;
+  const size_t ncol = 2 * (size_t)a.Nn;
+  C2* Z = (C2*)a.Z;
+  const int rowE = (uvsub ? a.Nn : 0) + e + asel;   // tt / ff, and tf / ft rows
+  const int colE = (uvsub ? a.Nn : 0) + f;          // tt / ff column (+ b)
+  const int colO = (uvsub ? 0 : a.Nn) + f;          // tf / ft column (+ b)
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int n = 16 * t + c16;
+    if (n > a.M) continue;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+      const R er = stash[wl][t][2 * bi][lane], ei = stash[wl][t][2 * bi + 1][lane];
+      const R orr = stash[wl][t][4 + 2 * bi][lane], oi = stash[wl][t][4 + 2 * bi + 1][lane];
+      const R Ker = accE[t][2 * bi], Kei = accE[t][2 * bi + 1];
+      const R Kor = accO[t][2 * bi + 1], Koi = -accO[t][2 * bi];  // -j (re + j im)
+      const R zer = ce_re * er - ce_im * ei + cm * Ker;
+      const R zei = ce_re * ei + ce_im * er + cm * Kei;
+      const R zor = ce_re * orr - ce_im * oi + cm * Kor;
+      const R zoi = ce_re * oi + ce_im * orr + cm * Koi;
+      C2* pe_ = Z + ((size_t)rowE * ncol + colE + bi) * nmodes + a.M;
+      C2* po_ = Z + ((size_t)rowE * ncol + colO + bi) * nmodes + a.M;
+      C2 v;
+      v = pe_[n]; v.x += zer; v.y += zei; pe_[n] = v;
+      v = po_[n]; v.x += zor; v.y += zoi; po_[n] = v;
+      if (n > 0) {
+        v = pe_[-n]; v.x += zer; v.y += zei; pe_[-n] = v;
+        v = po_[-n]; v.x -= zor; v.y -= zoi; po_[-n] = v;
+      }
+    }
+  }
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// near element pairs, stage 1: modal values per cell point pair (FP64)
+// ─────────────────────────────────────────────────────────────────────────────
+
+struct NearPairInfo {
+  int e, f, kind;   // kind 0 diag, 1 corner10 (f = e+1), 2 corner01 (f = e-1)
+  int pp_off;       // first point pair in the NP buffer
+  int n_tail;       // Gauss order of the [xi0, pi] tail
+  int tail_off;     // offset of that rule in the tail tables
+};
+
+struct NearArgs {
+  const double* nodes;    // [Nn][2]
+  const double* L;        // [Ne]
+  const double* trho;     // [Ne]
+  const double* tz;       // [Ne]
+  const double* cell_s;   // concatenated cell points of the 3 kinds
+  const double* cell_sp;
+  const double* cell_w;
+  int cell_off[3], cell_cnt[3];
+  const double* core_u;   // [48] Gauss nodes on [0,1]
+  const double* core_w;
+  const double* tail_u;   // concatenated tail rules on [0,1]
+  const double* tail_w;
+  const NearPairInfo* pairs;
+  const int2* batches;    // (pair slot, first cell point) per work-group
+  int Ne, M, nord;        // nord = M + 2 orders stored per point pair
+  double k;
+  double2* NP;            // [pp][nord][5] : G, Ktt, Ktf, Kft, Kff
+};
+
+__device__ __forceinline__ void cell_geom(const NearArgs& a, int e, double s, double& rho, double& z) {
+  const double r0 = a.nodes[2 * e], z0 = a.nodes[2 * e + 1];
+  const double r1 = a.nodes[2 * e + 2], z1 = a.nodes[2 * e + 3];
+  rho = r0 + s * (r1 - r0);
+  z = z0 + s * (z1 - z0);
+}
+
+// 256 threads = 32 point pairs x 8 order chunks of C orders.  Node values are
+// evaluated once per (point pair, node) and shared through LDS in batches of 8.
+template <int C>
+__global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
+  const int2 bt = a.batches[blockIdx.x];
+  const NearPairInfo pr = a.pairs[bt.x];
+  const int slot = threadIdx.x >> 3, cs = threadIdx.x & 7;
+  const int q = bt.y + slot;
+  const int cnt = a.cell_cnt[pr.kind];
+  const bool valid = q < cnt;
+  const int qi = a.cell_off[pr.kind] + (valid ? q : 0);
+  const double s = a.cell_s[qi], sp = a.cell_sp[qi];
+  double rp, zp, rq, zq;
+  cell_geom(a, pr.e, s, rp, zp);
+  cell_geom(a, pr.f, sp, rq, zq);
+  const double trp = a.trho[pr.e], tzp = a.tz[pr.e], trq = a.trho[pr.f], tzq = a.tz[pr.f];
+  const double k = a.k;
+  const double d2 = (rp - rq) * (rp - rq) + (zp - zq) * (zp - zq);
+  const double rr4 = 4.0 * rp * rq;
+  const bool axis = rr4 <= 1e-30;
+  const double d = sqrt(d2 > 1e-300 ? d2 : 1e-300);
+  const double aa = sqrt(rr4 > 1e-300 ? rr4 : 1e-300);
+  const double s0 = axis ? 1.0 : fmin(0.25, 20.0 * d / aa);
+  const double vmax = asinh((aa / d) * s0);
+  const double xi0 = 2.0 * asin(fmin(s0, 1.0));
+  const double span = GH_PI - xi0;
+  const int nnodes = 48 + pr.n_tail;
+  const int n0 = cs * C;
+
+  __shared__ double nv[32][8][12];
+  __shared__ double st[32][8][16];   // e^{j n0 xi} per chunk
+
+  double2 accG[C], accK[4][C];
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    accG[i] = make_double2(0, 0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) accK[u][i] = make_double2(0, 0);
+  }
+
+  for (int v0 = 0; v0 < nnodes; v0 += 8) {
+    // ── node value for node v0 + cs of this point pair ──
+    const int vn = v0 + cs;
+    double* out = nv[slot][cs];
+    if (valid && vn < nnodes) {
+      double xi, wq, Rd;
+      if (vn < 48) {
+        const double vv = a.core_u[vn] * vmax, wv = a.core_w[vn] * vmax;
+        double sv = (d / aa) * sinh(vv);
+        sv = sv < 1.0 ? sv : 1.0;
+        xi = 2.0 * asin(sv);
+        wq = wv * 2.0 * (d / aa) * cosh(vv) / sqrt(fmax(1.0 - sv * sv, 1e-15));
+        Rd = sqrt(d * d + (aa * sv) * (aa * sv));
+      } else {
+        const int ti = pr.tail_off + (vn - 48);
+        xi = xi0 + a.tail_u[ti] * span;
+        wq = a.tail_w[ti] * span;
+        const double sh = sin(0.5 * xi);
+        Rd = sqrt(d * d + (aa * sh) * (aa * sh));
+      }
+      Rd = Rd > 1e-300 ? Rd : 1e-300;
+      double sk, ck, sx, cx;
+      sincos(k * Rd, &sk, &ck);
+      sincos(xi, &sx, &cx);
+      const double inv = 1.0 / Rd;
+      // EFIE: 2 g w  (axis pairs use the closed form instead)
+      const double ge = axis ? 0.0 : 2.0 * wq * inv / (4.0 * GH_PI);
+      out[0] = ge * ck;
+      out[1] = -ge * sk;
+      // brackets: 2 w p(R) b_uv(xi)
+      const double pre = 2.0 * wq * inv * inv * inv / (4.0 * GH_PI);
+      const double kr = k * Rd;
+      const double pr_ = pre * (ck + kr * sk), pi_ = pre * (kr * ck - sk);
+      const double Rx = rp - rq * cx, Ry = rq * sx, Rz = zp - zq;
+      const double WtR = trp * Rx + tzp * Rz, nR = -tzp * Rx + trp * Rz;
+      const double n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
+      const double Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
+      const double btt = -(WtR * n_t - Wt_t * nR), btf = -(WtR * n_f - Wt_f * nR);
+      const double bft = -(Ry * n_t - Wf_t * nR), bff = -(Ry * n_f - cx * nR);
+      out[2] = btt * pr_; out[3] = btt * pi_;
+      out[4] = btf * pr_; out[5] = btf * pi_;
+      out[6] = bft * pr_; out[7] = bft * pi_;
+      out[8] = bff * pr_; out[9] = bff * pi_;
+      out[10] = cx; out[11] = sx;
+      // chunk starts e^{j c C xi}, c = 0..7
+      double sC, cC;
+      sincos((double)C * xi, &sC, &cC);
+      double zr = 1.0, zi = 0.0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        st[slot][cs][2 * c] = zr;
+        st[slot][cs][2 * c + 1] = zi;
+        const double t = zr * cC - zi * sC;
+        zi = zr * sC + zi * cC;
+        zr = t;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 12; ++i) out[i] = 0.0;
+      out[10] = 1.0;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) st[slot][cs][c] = 0.0;
+    }
+    __syncthreads();
+    // ── accumulate this thread's orders over the 8 nodes ──
+    if (valid) {
+      for (int u = 0; u < 8; ++u) {
+        const double* x = nv[slot][u];
+        const double c1 = x[10], s1 = x[11];
+        double cr = st[slot][u][2 * cs], ci = st[slot][u][2 * cs + 1];
+#pragma unroll
+        for (int i = 0; i < C; ++i) {
+          // EFIE and even brackets project on cos(n xi), odd brackets on sin(n xi)
+          accG[i].x += x[0] * cr;  accG[i].y += x[1] * cr;
+          accK[0][i].x += x[2] * cr; accK[0][i].y += x[3] * cr;
+          accK[1][i].x += x[4] * ci; accK[1][i].y += x[5] * ci;
+          accK[2][i].x += x[6] * ci; accK[2][i].y += x[7] * ci;
+          accK[3][i].x += x[8] * cr; accK[3][i].y += x[9] * cr;
+          const double t = cr * c1 - ci * s1;
+          ci = cr * s1 + ci * c1;
+          cr = t;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!valid) return;
+  double2* out = a.NP + ((size_t)(pr.pp_off + q) * a.nord) * 5;
+#pragma unroll
+  for (int i = 0; i < C; ++i) {
+    const int n = n0 + i;
+    if (n >= a.nord) break;
+    double2 G = accG[i];
+    if (axis) {
+      // closed form: only order 0, 2 pi g(d)
+      if (n == 0) {
+        const double R0 = sqrt(d2);
+        const double Rc = R0 > 1e-300 ? R0 : 1e-300;
+        double sk, ck;
+        sincos(k * R0, &sk, &ck);
+        G = make_double2(2.0 * GH_PI * ck / (4.0 * GH_PI * Rc), -2.0 * GH_PI * sk / (4.0 * GH_PI * Rc));
+      } else {
+        G = make_double2(0.0, 0.0);
+      }
+    }
+    double2* o = out + (size_t)n * 5;
+    o[0] = G;
+    o[1] = accK[0][i];
+    o[2] = make_double2(accK[1][i].y, -accK[1][i].x);  // -j * sum
+    o[3] = make_double2(accK[2][i].y, -accK[2][i].x);
+    o[4] = accK[3][i];
+  }
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// near element pairs, stage 2: 2x2 nodal blocks per mode (FP64)
+// ─────────────────────────────────────────────────────────────────────────────
+
+struct NearBlockArgs {
+  NearArgs na;
+  const double* gw;      // Gauss weights x L per point [P] (mass term)
+  const double* grho;    // [P]
+  const double* gT0;
+  const double* gT1;
+  double ce_re, ce_im;   // EFIE coefficient
+  double cM;             // MFIE coefficient
+  double2* NB;           // [3 Ne][16][2M+1]
+};
+
+// one work-group per near pair slot, one thread per |m|
+__global__ __launch_bounds__(128) void k_near_blocks(NearBlockArgs b) {
+  const NearArgs& a = b.na;
+  const int slot = blockIdx.x;
+  const NearPairInfo pr = a.pairs[slot];
+  if (pr.e < 0) return;
+  const int am = threadIdx.x;
+  if (am > a.M) return;
+  const int e = pr.e, f = pr.f;
+  const double trp = a.trho[e], tzp = a.tz[e], trq = a.trho[f], tzq = a.tz[f];
+  const double Le = a.L[e], Lf = a.L[f];
+  const double dre = a.nodes[2 * e + 2] - a.nodes[2 * e];
+  const double drf = a.nodes[2 * f + 2] - a.nodes[2 * f];
+  const double k = a.k, ik2 = 1.0 / (k * k);
+  const int cnt = a.cell_cnt[pr.kind], off = a.cell_off[pr.kind];
+  double2 E[16], K[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) { E[i] = make_double2(0, 0); K[i] = make_double2(0, 0); }
+  const int om1 = am > 0 ? am - 1 : 1;
+  for (int q = 0; q < cnt; ++q) {
+    const double s = a.cell_s[off + q], sp = a.cell_sp[off + q];
+    const double W = a.cell_w[off + q] * Le * Lf;
+    double rp, zp, rq, zq;
+    cell_geom(a, e, s, rp, zp);
+    cell_geom(a, f, sp, rq, zq);
+    const double Tp[2] = {1.0 - s, s}, Tq[2] = {1.0 - sp, sp};
+    const double Dp[2] = {(dre * (1.0 - s) - rp) / Le, (dre * s + rp) / Le};
+    const double Dq[2] = {(drf * (1.0 - sp) - rq) / Lf, (drf * sp + rq) / Lf};
+    const double rr = rp * rq * W;
+    const double2* v = a.NP + (size_t)(pr.pp_off + q) * a.nord * 5;
+    const double2 G = v[am * 5], Gm = v[om1 * 5], Gp = v[(am + 1) * 5];
+    const double2 Gc = make_double2(0.5 * (Gm.x + Gp.x), 0.5 * (Gm.y + Gp.y));
+    const double2 Gs = make_double2(0.5 * (Gm.y - Gp.y), -0.5 * (Gm.x - Gp.x));
+    const double2 ktt = make_double2(rr * (trp * trq * Gc.x + tzp * tzq * G.x),
+                                     rr * (trp * trq * Gc.y + tzp * tzq * G.y));
+    const double2 ktf = make_double2(rr * trp * Gs.x, rr * trp * Gs.y);
+    const double2 kft = make_double2(-rr * trq * Gs.x, -rr * trq * Gs.y);
+    const double2 kff = make_double2(rr * Gc.x, rr * Gc.y);
+    const double2 ksc = make_double2(W * G.x, W * G.y);
+    const double m = (double)am;
+#pragma unroll
+    for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+      for (int bi = 0; bi < 2; ++bi) {
+        const int ab = 2 * ai + bi;
+        const double TT = Tp[ai] * Tq[bi];
+        const double DD = Dp[ai] * Dq[bi] * ik2, DT = Dp[ai] * Tq[bi] * m * ik2;
+        const double TD = Tp[ai] * Dq[bi] * m * ik2, SS = TT * m * m * ik2;
+        E[0 + ab].x += TT * ktt.x - DD * ksc.x;   E[0 + ab].y += TT * ktt.y - DD * ksc.y;
+        E[4 + ab].x += TT * ktf.x + DT * ksc.y;   E[4 + ab].y += TT * ktf.y - DT * ksc.x;
+        E[8 + ab].x += TT * kft.x - TD * ksc.y;   E[8 + ab].y += TT * kft.y + TD * ksc.x;
+        E[12 + ab].x += TT * kff.x - SS * ksc.x;  E[12 + ab].y += TT * kff.y - SS * ksc.y;
+#pragma unroll
+        for (int uv = 0; uv < 4; ++uv) {
+          const double2 Kv = v[am * 5 + 1 + uv];
+          const double wgt = 2.0 * GH_PI * TT * rr;
+          K[4 * uv + ab].x += wgt * Kv.x;
+          K[4 * uv + ab].y += wgt * Kv.y;
+        }
+      }
+  }
+  // mass term 2 pi int rho T_a T_b dt on the element itself (diagonal pairs)
+  double Mab[4] = {0, 0, 0, 0};
+  if (e == f) {
+    for (int gq = 0; gq < 4; ++gq) {
+      const int ip = 4 * e + gq;
+      const double wr = b.gw[ip] * b.grho[ip];
+      const double T[2] = {b.gT0[ip], b.gT1[ip]};
+      for (int ai = 0; ai < 2; ++ai)
+        for (int bi = 0; bi < 2; ++bi) Mab[2 * ai + bi] += 2.0 * GH_PI * wr * T[ai] * T[bi];
+    }
+  }
+  const int nmodes = 2 * a.M + 1;
+  double2* out = b.NB + (size_t)slot * 16 * nmodes;
+  for (int blk = 0; blk < 4; ++blk)
+    for (int ab = 0; ab < 4; ++ab) {
+      const double2 Ev = E[4 * blk + ab], Kv = K[4 * blk + ab];
+      double zr = b.ce_re * Ev.x - b.ce_im * Ev.y - b.cM * Kv.x;
+      double zi = b.ce_re * Ev.y + b.ce_im * Ev.x - b.cM * Kv.y;
+      if (blk == 0 || blk == 3) zr += 0.5 * b.cM * Mab[ab];
+      double2* o = out + (size_t)(4 * blk + ab) * nmodes + a.M;
+      o[am] = make_double2(zr, zi);
+      if (am > 0) {
+        const bool odd = (blk == 1 || blk == 2);
+        const double sg = odd ? -1.0 : 1.0;
+        double zr2 = sg * (b.ce_re * Ev.x - b.ce_im * Ev.y - b.cM * Kv.x);
+        double zi2 = sg * (b.ce_re * Ev.y + b.ce_im * Ev.x - b.cM * Kv.y);
+        if (!odd) zr2 += 0.5 * b.cM * Mab[ab];
+        o[-am] = make_double2(zr2, zi2);
+      }
+    }
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// assembly: Zfar + near band -> masked row-major mode systems
+// ─────────────────────────────────────────────────────────────────────────────
+
+struct AsmArgs {
+  const void* Zfar;          // [2Nn][2Nn][nmodes] complex<Rf>
+  const double2* NB;         // [3 Ne][16][nmodes]
+  const SysInfo* sys;        // systems of this problem, index = m + M
+  int Nn, Ne, M;
+  void* A;                   // augmented systems (complex<Ra>), base pointer
+  double2* A64;              // optional FP64 copy of the n x n block (nullptr if unused)
+};
+
+template <typename Rf, typename Ra>
+__global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
+  typedef typename CT<Rf>::c Cf;
+  typedef typename CT<Ra>::c Ca;
+  __shared__ double2 tile[32][33];
+  const int i_row = blockIdx.x;            // node-level row in [0, 2Nn)
+  const int j0 = blockIdx.y * 32;          // node-level column chunk
+  const int m0 = blockIdx.z * 32;          // mode chunk (index into [0, nmodes))
+  const int nmodes = 2 * a.M + 1;
+  const size_t ncol = 2 * (size_t)a.Nn;
+  const Cf* Z = (const Cf*)a.Zfar;
+  // load [32 cols][32 modes], modes fastest
+  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
+    const int jj = t >> 5, mm = t & 31;
+    const int j = j0 + jj, mi = m0 + mm;
+    double2 v = make_double2(0, 0);
+    if (j < (int)ncol && mi < nmodes) {
+      const Cf z = Z[((size_t)i_row * ncol + j) * nmodes + mi];
+      v = make_double2((double)z.x, (double)z.y);
+    }
+    tile[jj][mm] = v;
+  }
+  __syncthreads();
+  const int bi = i_row >= a.Nn ? 1 : 0;
+  const int ii = i_row - bi * a.Nn;
+  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
+    const int mm = t >> 5, jj = t & 31;
+    const int j = j0 + jj, mi = m0 + mm;
+    if (j >= (int)ncol || mi >= nmodes) continue;
+    const SysInfo s = a.sys[mi];
+    const int nf = s.n - s.nt;
+    int r;
+    if (bi == 0) { r = ii - s.t0; if (r < 0 || r >= s.nt) continue; }
+    else { r = ii - s.f0; if (r < 0 || r >= nf) continue; r += s.nt; }
+    const int bj = j >= a.Nn ? 1 : 0;
+    const int jj2 = j - bj * a.Nn;
+    int c;
+    if (bj == 0) { c = jj2 - s.t0; if (c < 0 || c >= s.nt) continue; }
+    else { c = jj2 - s.f0; if (c < 0 || c >= nf) continue; c += s.nt; }
+    double2 v = tile[jj][mm];
+    if (abs(ii - jj2) <= 2) {
+      const int blk = 2 * bi + bj;
+#pragma unroll
+      for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb) {
+          const int e = ii - ai, f = jj2 - bb;
+          if (e < 0 || f < 0 || e >= a.Ne || f >= a.Ne || abs(e - f) > 1) continue;
+          const int slot = 3 * e + (f - e + 1);
+          const double2 w = a.NB[((size_t)slot * 16 + 4 * blk + 2 * ai + bb) * nmodes + mi];
+          v.x += w.x; v.y += w.y;
+        }
+    }
+    Ca* A = (Ca*)a.A + s.a_off;
+    Ca o; o.x = (Ra)v.x; o.y = (Ra)v.y;
+    A[(size_t)r * s.ld + c] = o;
+    if (a.A64) a.A64[s.a64_off + (size_t)r * s.n + c] = v;
+  }
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// compat samplers (bor_stream_kernel.c signatures; host entry points wrap these)
+// ─────────────────────────────────────────────────────────────────────────────
+
+__global__ void k_sample_g(int nr, int np_, int nxi, const double* rho_p, const double* z_p,
+                           const double* rho_q, const double* z_q, double k,
+                           const double* sin2, double2* out) {
+  const size_t tot = (size_t)nr * np_ * nxi;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int l = (int)(t % nxi);
+    const size_t ij = t / nxi;
+    const int j = (int)(ij % np_), i = (int)(ij / np_);
+    const double dr = rho_p[i] - rho_q[j], dz = z_p[i] - z_q[j];
+    double R = sqrt(dr * dr + dz * dz + 4.0 * rho_p[i] * rho_q[j] * sin2[l]);
+    if (R < 1e-30) R = 1e-30;
+    const double amp = 1.0 / (4.0 * GH_PI * R);
+    double s, c;
+    sincos(k * R, &s, &c);
+    out[t] = make_double2(amp * c, -amp * s);
+  }
+}
+
+template <bool IBC>
+__global__ void k_sample_brackets(int nr, int np_, int nxi, const double* rho_p, const double* z_p,
+                                  const double* tr_p, const double* tz_p, const double* rho_q,
+                                  const double* z_q, const double* tr_q, const double* tz_q, double k,
+                                  const double* cx_t, const double* sx_t, double2* o_tt, double2* o_tf,
+                                  double2* o_ft, double2* o_ff) {
+  const size_t tot = (size_t)nr * np_ * nxi;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const int l = (int)(t % nxi);
+    const size_t ij = t / nxi;
+    const int j = (int)(ij % np_), i = (int)(ij / np_);
+    const double cx = cx_t[l], sx = sx_t[l];
+    const double Rx = rho_p[i] - rho_q[j] * cx, Ry = rho_q[j] * sx, Rz = z_p[i] - z_q[j];
+    double R = sqrt(Rx * Rx + Ry * Ry + Rz * Rz);
+    if (R < 1e-30) R = 1e-30;
+    const double pre = 1.0 / (4.0 * GH_PI * R * R * R);
+    double s, c;
+    sincos(k * R, &s, &c);
+    const double kr = k * R;
+    const double pr = pre * (c + kr * s), pim = pre * (kr * c - s);
+    const double trp = tr_p[i], tzp = tz_p[i], trq = tr_q[j], tzq = tz_q[j];
+    double ftt, ftf, fft, fff;
+    if (!IBC) {
+      const double WtR = trp * Rx + tzp * Rz, nR = -tzp * Rx + trp * Rz;
+      const double n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
+      const double Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
+      ftt = -(WtR * n_t - Wt_t * nR);
+      ftf = -(WtR * n_f - Wt_f * nR);
+      fft = -(Ry * n_t - Wf_t * nR);
+      fff = -(Ry * n_f - cx * nR);
+    } else {
+      const double Wt_n = -trp * tzq * cx + tzp * trq, Wf_n = tzq * sx;
+      const double D = rho_p[i] * cx - rho_q[j];
+      const double R_t = trq * D + tzq * Rz, R_f = rho_p[i] * sx, R_n = -tzq * D + trq * Rz;
+      const double Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
+      ftt = Wt_n * R_t - Wt_t * R_n;
+      ftf = Wt_n * R_f - Wt_f * R_n;
+      fft = Wf_n * R_t - Wf_t * R_n;
+      fff = Wf_n * R_f - cx * R_n;
+    }
+    o_tt[t] = make_double2(ftt * pr, ftt * pim);
+    o_tf[t] = make_double2(ftf * pr, ftf * pim);
+    o_ft[t] = make_double2(fft * pr, fft * pim);
+    o_ff[t] = make_double2(fff * pr, fff * pim);
+  }
+}
+
+__global__ void k_zero(double2* p, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    p[t] = make_double2(0.0, 0.0);
+}
+
+}  // namespace ghost

@@ -1,0 +1,512 @@
+// Batched blocked complex LU with partial pivoting, triangular solves and the
+// complex MFMA GEMM that carries the trailing updates, for a batch of
+// row-major systems of uneven size (one per azimuthal mode and frequency).
+//
+// Replaces numpy.linalg.solve (LAPACK zgesv) at bor_solver.py:753: pivot
+// choice by max |Re| + |Im| as LAPACK's izamax, row interchanges applied to
+// whole rows, unit-lower L and upper U stored in place.  Right-hand sides are
+// appended as extra columns so the trailing updates also perform the forward
+// substitution; back substitution runs blocked (diagonal TRSM + GEMM).
+#include "bor_common.hpp"
+
+namespace ghost {
+
+struct GemmOp {      // C[M x N] -= A[M x K] B[K x N], complex, row-major
+  long long a, b, c; // element offsets from the respective base pointers
+  int lda, ldb, ldc;
+  int M, N, K;
+};
+
+struct PanelOp {     // factor rows [k, n) x cols [k, k + w)
+  long long a;       // system base (element offset)
+  int n, ld, k, w;
+  int piv;           // offset of this system's pivot array
+};
+
+struct SwapOp {      // apply pivots k..k+w-1 to cols [0, c_lo) and [c_hi, ncols)
+  long long a;
+  int ld, k, w, c_lo, c_hi, ncols, piv;
+};
+
+struct TrsmOp {      // diagonal block T (w x w) of a factor and the slot of its W = I - T^{-1}
+  long long t;       // element offset of T's (0, 0) in the factor buffer
+  long long wo;      // element offset of the 64 x 64 W slot
+  int ldt, w;
+};
+
+// ─────────────────────────────────────────────────────────────────────────────
+// complex GEMM update on 16x16x4 MFMA: 64 x 64 tile per 256-thread block
+// ─────────────────────────────────────────────────────────────────────────────
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, const void* Abase,
+                                                     const void* Bbase, void* Cbase) {
+  typedef typename CT<T>::c C;
+  typedef typename CT<T>::v4 V4;
+  const GemmOp op = ops[blockIdx.y];
+  if (op.M <= 0 || op.N <= 0 || op.K <= 0) return;
+  const int tn = (op.N + 63) >> 6;
+  const int tiles = ((op.M + 63) >> 6) * tn;
+  if ((int)blockIdx.x >= tiles) return;
+  const int m0 = (blockIdx.x / tn) * 64, n0 = (blockIdx.x % tn) * 64;
+  const C* A = (const C*)Abase + op.a;
+  const C* B = (const C*)Bbase + op.b;
+  C* Cm = (C*)Cbase + op.c;
+
+  __shared__ T Ar[16][64 + 1], Ai[16][64 + 1];   // [k][row]
+  __shared__ T Br[16][64 + 1], Bi[16][64 + 1];   // [k][col]
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wy = wv >> 1, wx = wv & 1;
+  const int c16 = lane & 15, kk = lane >> 4;
+  V4 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) { accR[i][j] = V4{0, 0, 0, 0}; accI[i][j] = V4{0, 0, 0, 0}; }
+
+  for (int k0 = 0; k0 < op.K; k0 += 16) {
+    // A tile: 64 rows x 16 k, thread -> (row, 4 consecutive k)
+    {
+      const int row = tid >> 2, kq = (tid & 3) * 4;
+      const int gr = m0 + row;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int gk = k0 + kq + q;
+        C v;
+        v.x = 0; v.y = 0;
+        if (gr < op.M && gk < op.K) v = A[(size_t)gr * op.lda + gk];
+        Ar[kq + q][row] = v.x;
+        Ai[kq + q][row] = v.y;
+      }
+    }
+    // B tile: 16 k x 64 cols, thread -> (k, 4 consecutive cols)
+    {
+      const int kr = tid >> 4, cq = (tid & 15) * 4;
+      const int gk = k0 + kr;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int gc = n0 + cq + q;
+        C v;
+        v.x = 0; v.y = 0;
+        if (gk < op.K && gc < op.N) v = B[(size_t)gk * op.ldb + gc];
+        Br[kr][cq + q] = v.x;
+        Bi[kr][cq + q] = v.y;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int kx = 4 * ks + kk;
+      T ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Ar[kx][wy * 32 + i * 16 + c16];
+        ai[i] = Ai[kx][wy * 32 + i * 16 + c16];
+        nai[i] = -ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        br[j] = Br[kx][wx * 32 + j * 16 + c16];
+        bi[j] = Bi[kx][wx * 32 + j * 16 + c16];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = mfma4(ar[i], br[j], accR[i][j]);
+          accR[i][j] = mfma4(nai[i], bi[j], accR[i][j]);
+          accI[i][j] = mfma4(ar[i], bi[j], accI[i][j]);
+          accI[i][j] = mfma4(ai[i], br[j], accI[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+  // C -= acc
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = m0 + wy * 32 + i * 16 + mrow<T>(kk, r);
+        const int gc = n0 + wx * 32 + j * 16 + c16;
+        if (gr < op.M && gc < op.N) {
+          C* p = Cm + (size_t)gr * op.ldc + gc;
+          C v = *p;
+          v.x -= accR[i][j][r];
+          v.y -= accI[i][j][r];
+          *p = v;
+        }
+      }
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// panel factorisation: one 1024-thread block per system, 8-column sub-panels
+// held in registers (rows r0 + tid + 1024 i), pivots by max |Re| + |Im|
+// ─────────────────────────────────────────────────────────────────────────────
+
+template <typename T> __device__ __forceinline__ typename CT<T>::c cdiv1(typename CT<T>::c z) {
+  // 1 / z with scaling (Smith's method)
+  typename CT<T>::c r;
+  if (fabs(z.x) >= fabs(z.y)) {
+    const T q = z.y / z.x, d = z.x + z.y * q;
+    r.x = (T)1 / d; r.y = -q / d;
+  } else {
+    const T q = z.x / z.y, d = z.y + z.x * q;
+    r.x = q / d; r.y = (T)-1 / d;
+  }
+  return r;
+}
+
+template <typename T, int RPT>
+__global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* base, int* pivots, int* info) {
+  typedef typename CT<T>::c C;
+  const PanelOp op = ops[blockIdx.x];
+  if (op.w <= 0) return;
+  C* A = (C*)base + op.a;
+  int* ipiv = pivots + op.piv;
+  const int n = op.n, ld = op.ld, k = op.k, w = op.w;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+
+  __shared__ T red_v[16];
+  __shared__ int red_i[16];
+  __shared__ int piv_row;
+  __shared__ C prow[8];
+  __shared__ C srow[8];
+  __shared__ C L11[8][8];
+  __shared__ C U12[8][64];
+  __shared__ int piv_s[8];
+
+  for (int j0 = 0; j0 < w; j0 += 8) {
+    const int ws = min(8, w - j0);
+    const int c0 = k + j0;
+    C v[RPT][8];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int row = c0 + tid + 1024 * i;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        C z;
+        z.x = 0; z.y = 0;
+        if (row < n && c < ws) z = A[(size_t)row * ld + c0 + c];
+        v[i][c] = z;
+      }
+    }
+    for (int jj = 0; jj < ws; ++jj) {
+      const int col = c0 + jj;
+      // ── pivot search ──
+      T best = (T)-1;
+      int bidx = 0x7fffffff;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int row = c0 + tid + 1024 * i;
+        if (row >= col && row < n) {
+          C z;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) if (c == jj) z = v[i][c];
+          const T mag = fabs(z.x) + fabs(z.y);
+          if (mag > best || (mag == best && row < bidx)) { best = mag; bidx = row; }
+        }
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const T ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bidx, off);
+        if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+      }
+      if (lane == 0) { red_v[wv] = best; red_i[wv] = bidx; }
+      __syncthreads();
+      if (tid == 0) {
+        T b = red_v[0];
+        int bi = red_i[0];
+        for (int q = 1; q < 16; ++q)
+          if (red_v[q] > b || (red_v[q] == b && red_i[q] < bi)) { b = red_v[q]; bi = red_i[q]; }
+        if (!(b > (T)0)) { bi = col; if (info) atomicAdd(info + blockIdx.x, 1); }
+        piv_row = bi;
+        ipiv[col] = bi;
+        piv_s[jj] = bi;
+      }
+      __syncthreads();
+      const int pr = piv_row;
+      // ── interchange rows col and pr inside the sub-panel ──
+      const int oc = col - c0, op_ = pr - c0;
+      if ((oc & 1023) == tid) {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+          if (i == (oc >> 10))
+#pragma unroll
+            for (int c = 0; c < 8; ++c) srow[c] = v[i][c];
+      }
+      if ((op_ & 1023) == tid) {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+          if (i == (op_ >> 10))
+#pragma unroll
+            for (int c = 0; c < 8; ++c) prow[c] = v[i][c];
+      }
+      __syncthreads();
+      if (pr != col) {
+        if ((oc & 1023) == tid) {
+#pragma unroll
+          for (int i = 0; i < RPT; ++i)
+            if (i == (oc >> 10))
+#pragma unroll
+              for (int c = 0; c < 8; ++c) v[i][c] = prow[c];
+        }
+        if ((op_ & 1023) == tid) {
+#pragma unroll
+          for (int i = 0; i < RPT; ++i)
+            if (i == (op_ >> 10))
+#pragma unroll
+              for (int c = 0; c < 8; ++c) v[i][c] = srow[c];
+        }
+      }
+      // ── scale the column, rank-1 update of the sub-panel ──
+      C pv;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) if (c == jj) pv = prow[c];
+      const bool ok = (pv.x != (T)0 || pv.y != (T)0);
+      const C inv = ok ? cdiv1<T>(pv) : pv;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int row = c0 + tid + 1024 * i;
+        if (row > col && row < n && ok) {
+          C l;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) if (c == jj) l = v[i][c];
+          const C lv = cmul(l, inv);
+#pragma unroll
+          for (int c = 0; c < 8; ++c) {
+            if (c == jj) v[i][c] = lv;
+            else if (c > jj && c < ws) v[i][c] = csub(v[i][c], cmul(lv, prow[c]));
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // ── write the sub-panel back; keep L11 ──
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int row = c0 + tid + 1024 * i;
+      if (row < n)
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+          if (c < ws) A[(size_t)row * ld + c0 + c] = v[i][c];
+    }
+    if (tid < ws) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) L11[tid][c] = v[0][c];
+    }
+    __syncthreads();
+    // ── interchanges on the other panel columns [k, c0) and [c0 + ws, k + w) ──
+    const int nleft = c0 - k, nright = k + w - (c0 + ws);
+    if (tid < nleft + nright) {
+      const int cc = tid < nleft ? k + tid : c0 + ws + (tid - nleft);
+      for (int jj = 0; jj < ws; ++jj) {
+        const int r1 = c0 + jj, r2 = piv_s[jj];
+        if (r1 != r2) {
+          C* p1 = A + (size_t)r1 * ld + cc;
+          C* p2 = A + (size_t)r2 * ld + cc;
+          const C t1 = *p1;
+          *p1 = *p2;
+          *p2 = t1;
+        }
+      }
+    }
+    __syncthreads();
+    if (nright > 0) {
+      // U12 = L11^{-1} A12 (unit lower), one thread per right column
+      if (tid < nright) {
+        const int cc = c0 + ws + tid;
+        C x[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          if (r < ws) {
+            C z = A[(size_t)(c0 + r) * ld + cc];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              if (q < r) z = csub(z, cmul(L11[r][q], x[q]));
+            x[r] = z;
+            A[(size_t)(c0 + r) * ld + cc] = z;
+            U12[r][tid] = z;
+          }
+        }
+      }
+      __syncthreads();
+      // rank-ws update of rows [c0 + ws, n) on the right columns
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int row = c0 + tid + 1024 * i;
+        if (row >= c0 + ws && row < n) {
+          C* p = A + (size_t)row * ld + c0 + ws;
+          for (int c = 0; c < nright; ++c) {
+            C acc = p[c];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+              if (q < ws) acc = csub(acc, cmul(v[i][q], U12[q][c]));
+            p[c] = acc;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// row interchanges of one panel on the columns outside it
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_swap(const SwapOp* ops, void* base, const int* pivots) {
+  typedef typename CT<T>::c C;
+  const SwapOp op = ops[blockIdx.y];
+  if (op.w <= 0) return;
+  const int nout = op.c_lo + (op.ncols - op.c_hi);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nout) return;
+  const int cc = t < op.c_lo ? t : op.c_hi + (t - op.c_lo);
+  C* A = (C*)base + op.a;
+  const int* ipiv = pivots + op.piv;
+  for (int jj = 0; jj < op.w; ++jj) {
+    const int r1 = op.k + jj, r2 = ipiv[op.k + jj];
+    if (r1 != r2) {
+      C* p1 = A + (size_t)r1 * op.ld + cc;
+      C* p2 = A + (size_t)r2 * op.ld + cc;
+      const C a1 = *p1;
+      *p1 = *p2;
+      *p2 = a1;
+    }
+  }
+}
+
+// W = I - T^{-1} for the w x w diagonal block T (w <= 64) of the factor:
+// unit lower (UPPER = false) or upper (UPPER = true).  A following
+// k_gemm_update with A = W and B = C = X then leaves X = T^{-1} X: the whole
+// B column slice is read before any C element of that slice is written.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(64) void k_tri_inv(const TrsmOp* ops, const void* tbase, void* wbuf) {
+  typedef typename CT<T>::c C;
+  const TrsmOp op = ops[blockIdx.x];
+  C* W = (C*)wbuf + op.wo;
+  if (op.w <= 0) return;
+  __shared__ C Tm[64][65];
+  __shared__ C Iv[64][65];
+  const C* Tp = (const C*)tbase + op.t;
+  const int w = op.w;
+  for (int i = threadIdx.x; i < w * w; i += 64) {
+    const int r = i / w, c = i % w;
+    Tm[r][c] = Tp[(size_t)r * op.ldt + c];
+  }
+  __syncthreads();
+  const int j = threadIdx.x;   // column of the inverse
+  if (j < w) {
+    if (!UPPER) {
+      for (int r = 0; r < w; ++r) {
+        C z;
+        z.x = (r == j) ? (T)1 : (T)0;
+        z.y = 0;
+        if (r > j)
+          for (int q = j; q < r; ++q) z = csub(z, cmul(Tm[r][q], Iv[q][j]));
+        Iv[r][j] = z;
+      }
+    } else {
+      for (int r = w - 1; r >= 0; --r) {
+        C z;
+        z.x = (r == j) ? (T)1 : (T)0;
+        z.y = 0;
+        if (r < j)
+          for (int q = r + 1; q <= j; ++q) z = csub(z, cmul(Tm[r][q], Iv[q][j]));
+        Iv[r][j] = (r <= j) ? cmul(z, cdiv1<T>(Tm[r][r])) : z;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += 64) {
+    const int r = i >> 6, c = i & 63;
+    C z;
+    z.x = 0; z.y = 0;
+    if (r < w && c < w) {
+      z = Iv[r][c];
+      z.x = -z.x; z.y = -z.y;
+      if (r == c) z.x += (T)1;
+    }
+    W[i] = z;
+  }
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// helpers for the mixed-precision refinement
+// ─────────────────────────────────────────────────────────────────────────────
+
+struct VecOp {       // an n x N block: src at s (lds), dst at d (ldd)
+  long long s, d;
+  int lds, ldd, n, N, piv;
+};
+
+// dst32[i] = (float) src64[perm[i]] with perm the row order of the factor
+__global__ void k_permute_cast(const VecOp* ops, const int* perm, const double2* src, float2* dst) {
+  const VecOp op = ops[blockIdx.z];
+  const int r = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= op.n || c >= op.N) return;
+  const int pr = perm[op.piv + r];
+  const double2 v = src[op.s + (size_t)pr * op.lds + c];
+  dst[op.d + (size_t)r * op.ldd + c] = make_float2((float)v.x, (float)v.y);
+}
+
+// dst64 (+)= (double) src32 ; with add = 0 a plain widening copy
+__global__ void k_widen(const VecOp* ops, const float2* src, double2* dst, int add) {
+  const VecOp op = ops[blockIdx.z];
+  const int r = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= op.n || c >= op.N) return;
+  const float2 v = src[op.s + (size_t)r * op.lds + c];
+  double2* p = dst + op.d + (size_t)r * op.ldd + c;
+  if (add) { double2 o = *p; o.x += (double)v.x; o.y += (double)v.y; *p = o; }
+  else *p = make_double2((double)v.x, (double)v.y);
+}
+
+__global__ void k_zero_int(int* p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0;
+}
+
+// dst64 = src64 (copy of an n x N block)
+__global__ void k_copy64(const VecOp* ops, const double2* src, double2* dst) {
+  const VecOp op = ops[blockIdx.z];
+  const int r = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= op.n || c >= op.N) return;
+  dst[op.d + (size_t)r * op.ldd + c] = src[op.s + (size_t)r * op.lds + c];
+}
+
+// permutation of the factor rows from LAPACK-style interchanges (one thread per system)
+__global__ void k_perm_from_pivots(const VecOp* ops, int nsys, const int* pivots, int* perm) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  const VecOp op = ops[s];
+  int* p = perm + op.piv;
+  const int* ip = pivots + op.piv;
+  for (int i = 0; i < op.n; ++i) p[i] = i;
+  for (int i = 0; i < op.n; ++i) {
+    const int j = ip[i];
+    if (j != i) { const int t = p[i]; p[i] = p[j]; p[j] = t; }
+  }
+}
+
+// per-system max |v| over an n x N block (for the refinement stop test)
+__global__ void k_block_maxabs(const VecOp* ops, const double2* v, double* out) {
+  const VecOp op = ops[blockIdx.x];
+  __shared__ double red[256];
+  double m = 0.0;
+  for (size_t i = threadIdx.x; i < (size_t)op.n * op.N; i += 256) {
+    const int r = (int)(i / op.N), c = (int)(i % op.N);
+    const double2 z = v[op.d + (size_t)r * op.ldd + c];
+    m = fmax(m, fabs(z.x) + fabs(z.y));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+}  // namespace ghost

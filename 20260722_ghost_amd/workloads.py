@@ -1,0 +1,110 @@
+"""Deterministic synthetic generatrices for the benchmark configurations.
+
+All bodies are closed (both ends on the axis) and traversed from the +z end,
+the reference's generatrix convention (bor_kernels.py:38-47).
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+from scipy.interpolate import CubicSpline
+
+C0 = 299_792_458.0
+
+
+def _resample(profile: np.ndarray, n_elems: int) -> np.ndarray:
+    """Uniform arc-length resampling of a dense (rho, z) polyline to n_elems elements."""
+    seg = np.hypot(*np.diff(profile, axis=0).T)
+    s = np.concatenate([[0.0], np.cumsum(seg)])
+    t = np.linspace(0.0, s[-1], n_elems + 1)
+    out = np.column_stack([np.interp(t, s, profile[:, 0]), np.interp(t, s, profile[:, 1])])
+    out[0, 0] = 0.0
+    out[-1, 0] = 0.0
+    return out
+
+
+def sphere(radius: float = 0.5, n_elems: int = 120) -> np.ndarray:
+    """Pole-to-pole semicircle (bor_solver.py:2179-2182)."""
+    th = np.linspace(0.0, math.pi, n_elems + 1)
+    return np.column_stack([radius * np.sin(th), radius * np.cos(th)])
+
+
+def ogive_cylinder_cone(n_elems: int = 1200, wavelength: float = 1.0, seed: int = 20260722) -> np.ndarray:
+    """Rocket-like body: tangent-ogive nose, cylinder, conical boat-tail and a
+    flat base; length 10 wavelengths, max radius 1 wavelength.  A seeded cubic
+    spline of small radial offsets (<= 0.02 wavelength) perturbs the profile;
+    nodes are uniform in arc length."""
+    rng = np.random.default_rng(seed)
+    lam = wavelength
+    L, R = 10.0 * lam, 1.0 * lam
+    Ln = rng.uniform(2.5, 3.5) * lam       # nose
+    La = rng.uniform(1.5, 2.5) * lam       # boat-tail
+    Rb = rng.uniform(0.35, 0.55) * R       # base radius
+    Lc = L - Ln - La
+    z_tip = 0.5 * L
+    x = np.linspace(0.0, L, 40001)         # axial distance from the tip
+    rho = np.empty_like(x)
+    Ro = (R * R + Ln * Ln) / (2.0 * R)
+    nose = x <= Ln
+    rho[nose] = np.sqrt(np.maximum(Ro * Ro - (Ln - x[nose]) ** 2, 0.0)) + R - Ro
+    cyl = (x > Ln) & (x <= Ln + Lc)
+    rho[cyl] = R
+    aft = x > Ln + Lc
+    rho[aft] = R - (R - Rb) * (x[aft] - Ln - Lc) / La
+    # seeded smooth ripple, zero at the tip and at the base
+    knots = np.linspace(0.0, L, 9)
+    offs = rng.uniform(-0.02, 0.02, 9) * lam
+    offs[0] = offs[-1] = 0.0
+    ripple = CubicSpline(knots, offs, bc_type="clamped")(x)
+    rho = np.maximum(rho + ripple * np.clip(rho / (0.2 * R), 0.0, 1.0), 0.0)
+    rho[0] = 0.0
+    side = np.column_stack([rho, z_tip - x])
+    base = np.column_stack([np.linspace(rho[-1], 0.0, 200)[1:], np.full(199, z_tip - L)])
+    return _resample(np.vstack([side, base]), n_elems)
+
+
+def random_smooth(length: float = 1.0, rmin: float = 0.05, rmax: float = 0.2, n_elems: int = 200,
+                  seed: int = 7) -> np.ndarray:
+    """Seeded smooth body: rho(z) spline in [rmin, rmax] with rounded ends."""
+    rng = np.random.default_rng(seed)
+    z = np.linspace(0.0, length, 4001)
+    knots = np.linspace(0.0, length, 7)
+    vals = rng.uniform(rmin, rmax, 7)
+    core = CubicSpline(knots, vals)(z)
+    core = np.clip(core, rmin, rmax)
+    env = np.sqrt(np.clip(1.0 - ((2.0 * z / length) - 1.0) ** 8, 0.0, 1.0))
+    rho = core * env
+    rho[0] = rho[-1] = 0.0
+    prof = np.column_stack([rho, 0.5 * length - z])
+    return _resample(prof, n_elems)
+
+
+def cone_sphere(n_elems: int = 600, half_angle_deg: float = 15.0, radius: float = 1.0) -> np.ndarray:
+    """Cone tip at +z, flank tangent to a sphere cap of the given radius, pole at -z;
+    nodes uniform in arc length."""
+    al = math.radians(half_angle_deg)
+    tp = math.pi / 2 - al
+    zt, rt = radius * math.cos(tp), radius * math.sin(tp)
+    tip = zt + rt / math.tan(al)
+    cone = np.column_stack([np.linspace(0.0, rt, 2001), np.linspace(tip, zt, 2001)])
+    th = np.linspace(tp, math.pi, 4001)[1:]
+    sph = np.column_stack([radius * np.sin(th), radius * np.cos(th)])
+    return _resample(np.vstack([cone, sph]), n_elems)
+
+
+def config(index: int):
+    """(name, nodes, freq_hz, thetas_deg, m_max) of a benchmark configuration."""
+    if index == 1:
+        return ("pec-sphere-a0.5m-300MHz-120seg", sphere(0.5, 120), 3e8,
+                np.arange(0.0, 181.0, 1.0), 12)
+    if index == 2:
+        lam = 1.0
+        return ("ogive-cylinder-cone-10lambda-1200seg", ogive_cylinder_cone(1200, lam), C0 / lam,
+                np.arange(0.0, 180.25, 0.5), 70)
+    if index == 4:
+        lam = 1.0
+        return ("cone-sphere-15deg-600seg", cone_sphere(600, 15.0, lam), C0 / lam,
+                np.arange(0.0, 181.0, 2.0), 40)
+    raise ValueError(f"no configuration {index}")

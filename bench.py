@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the BoR monostatic hot path on one MI355X.
+
+Workload: the ogive-cylinder-cone body (10 wavelengths long, 1 wavelength
+max radius, 1200 elements), |m| <= 70 (141 mode systems of size ~2400),
+361 aspects x 2 polarisations at one frequency.  One step = fill + batched
+LU/solve (+ FP64 refinement) + far-field sum of all 141 x 361 solves, with
+the geometry already on the device.  The headline path is the mixed
+precision one (FP32 far fill and LU, FP64 near band, residual refinement and
+far field); the plain FP64 path runs once beside it to give the dB error.
+
+Prints one JSON line (the driver's contract).
+"""
+
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
+
+
+def pkg(m):
+    return importlib.import_module("20260722_ghost_amd." + m)
+
+
+def db(x):
+    return 10.0 * np.log10(np.maximum(np.abs(x) ** 2 * 4 * math.pi, 1e-300))
+
+
+def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profile=True):
+    L = pkg("_lib")
+    bs = pkg("bor_solver")
+    k = 2 * math.pi * freq / bs.C0
+    _, ne, nbk, streaming = bs.discretisation(nodes, k, thetas, m_max, "cfie")
+    h = L.Handle(L.GHOST_MIXED if precision == "mixed" else L.GHOST_FP64, nb=64, residual=False)
+    h.add_problem(nodes, k, m_max, ne, nbk, 0.5, 0.5 * bs.ETA0, thetas)
+    h.build()
+    for _ in range(warmup):
+        h.run(refine)
+    h.sync()
+    import torch  # device synchronisation on both sides of the timed region
+    torch.cuda.synchronize()
+    if profile:
+        h.set_profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.run(refine)
+    h.sync()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ktimes = h.kernel_times() if profile else {}
+    phase = h.timings()
+    dF = h.dF(0)
+    h.close()
+    return dt, dF, ktimes, phase, (ne, nbk)
+
+
+def cpu_baseline(nodes, freq, thetas, m_max, budget_s=25.0):
+    """Oracle (NumPy/SciPy + OpenBLAS) on a bounded sample of the same sweep:
+    far fill of a few test-element rows, near blocks of a few element pairs,
+    LU + solve of one mode system and its far-field projections, each scaled
+    to the full sweep."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import bor_oracle as bo
+    body = bo.Body(nodes)
+    k = 2 * math.pi * freq / bo.C0
+    osc = bo.osc_for(body, k, streaming=True)
+    ne, nbk = bo.n_xi_efie(osc, m_max), bo.n_xi_bracket(osc, m_max)
+    Ne = body.Ne
+    rows = 2
+    # far fill of a 2-element test-row tile: samples, FFT over xi, near zeroing
+    t0 = time.perf_counter()
+    g = body.g
+    e0 = Ne // 2
+    sl = slice(e0 * 4, (e0 + rows) * 4)
+    xe, xb = bo.xi_grid(ne), bo.xi_grid(nbk)
+    P_ = lambda v: v[sl][:, None]
+    Q_ = lambda v: v[None, :]
+    G = bo.green_orders_fft(bo.green_samples(P_(g.rho), P_(g.z), Q_(g.rho), Q_(g.z), k, xe), m_max + 2)
+    bo._zero_near_rows(G, e0, rows, Ne, 4)
+    Fs = bo.bracket_samples(P_(g.rho), P_(g.z), P_(g.trho), P_(g.tz), Q_(g.rho), Q_(g.z), Q_(g.trho), Q_(g.tz), k, xb)
+    Ks = [bo.bracket_modes_fft(F, m_max) for F in Fs]
+    t_far = (time.perf_counter() - t0) * Ne / rows
+    # near: a few element pairs
+    t0 = time.perf_counter()
+    npairs = 6
+    for (e, f) in [(e0, e0), (e0, e0 + 1), (e0 + 1, e0)] * (npairs // 3):
+        s, sp, w = bo.cell_points(bo.near_kind(e, f))
+        p, q = body.element_points(e, s), body.element_points(f, sp)
+        bo.green_orders_near(p["rho"], p["z"], q["rho"], q["z"], k, m_max)
+        bo.bracket_modes_near(p["rho"], p["z"], p["trho"], p["tz"], q["rho"], q["z"], q["trho"], q["tz"], k, m_max)
+    t_near = (time.perf_counter() - t0) * (3 * Ne - 2) / npairs
+    # one mode: LU + solve of a random complex system of the same size, RHS and far field
+    n = 2 * body.Nn - 4
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + n * np.eye(n)
+    B = rng.standard_normal((n, 2 * len(thetas))) + 1j * rng.standard_normal((n, 2 * len(thetas)))
+    t0 = time.perf_counter()
+    X = np.linalg.solve(A, B)
+    t_solve = (time.perf_counter() - t0) * (2 * m_max + 1)
+    t0 = time.perf_counter()
+    nth = 12
+    for th in thetas[:nth]:
+        bo.rhs_efie(body, k, 3, th, "VV")
+        bo.rhs_mfie(body, k, 3, th, "VV")
+        sol = np.zeros(2 * body.Nn, dtype=complex)
+        bo.farfield(body, k, 3, sol, th)
+    t_exc = (time.perf_counter() - t0) * (2 * m_max + 1) * (2 * len(thetas)) / nth
+    total = t_far + t_near + t_solve + t_exc
+    solves = (2 * m_max + 1) * len(thetas)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    return {
+        "value": solves / total,
+        "unit": "solves/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"FP64 oracle timed on {rows} of {Ne} far test rows, {npairs} of {3 * Ne - 2} near pairs, "
+                   f"1 of {2 * m_max + 1} mode LU+solves (n={n}, nrhs={2 * len(thetas)}) and {nth} of "
+                   f"{len(thetas)} aspects of RHS/far field, each scaled to the full sweep; "
+                   f"est. total {total:.0f} s (far {t_far:.0f}, near {t_near:.0f}, solve {t_solve:.0f}, "
+                   f"rhs/far-field {t_exc:.0f}); OpenBLAS threads = {cores}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X only: multi-GPU runs are not part of this build "
+              "(see DESIGN.md for how the sweep would shard).", file=sys.stderr)
+        sys.exit(2)
+    wl = pkg("workloads")
+    name, nodes, freq, thetas, m_max = wl.config(args.config)
+    solves = (2 * m_max + 1) * len(thetas)
+
+    dt_mx, dF_mx, kt_mx, ph_mx, nxi = run_gpu(nodes, freq, thetas, m_max, "mixed", args.steps, args.warmup,
+                                               args.refine)
+    F_mx = pkg("bor_solver").truncate_modes(dF_mx, m_max, 0.0)[0]
+    err_db = None
+    fp64_rate = None
+    kt_64 = {}
+    if not args.no_fp64:
+        dt_64, dF_64, kt_64, ph_64, _ = run_gpu(nodes, freq, thetas, m_max, "fp64", 1, 0, 0)
+        F_64 = pkg("bor_solver").truncate_modes(dF_64, m_max, 0.0)[0]
+        s64, smx = db(F_64), db(F_mx)
+        peak = np.max(s64)
+        err_db = {
+            "max_abs_db": float(np.max(np.abs(smx - s64))),
+            "max_abs_db_above_peak_minus_40db": float(np.max(np.abs(smx - s64)[s64 > peak - 40.0])),
+        }
+        fp64_rate = solves / dt_64
+
+    # dominant kernel of the headline path, roofline from live HIP-event timing
+    fam, (ms, cnt, fl) = max(kt_mx.items(), key=lambda kv: kv[1][0])
+    gemm_ms, gemm_n, gemm_fl = kt_mx.get("gemm", (0.0, 0, 0.0))
+    roof = None
+    if gemm_n > 0 and gemm_ms > 0:
+        achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12
+        roof = {"kernel": "k_gemm_update<float> (complex FP32 MFMA 16x16x4, LU/solve updates)",
+                "bound": "mfma", "achieved": round(achieved, 3), "peak": MI355X_PEAK["fp32_mfma_tflops"],
+                "unit": "TFLOP/s", "frac": round(achieved / MI355X_PEAK["fp32_mfma_tflops"], 4),
+                "traffic": None, "launches": gemm_n, "avg_launch_ms": round(gemm_ms / gemm_n, 4),
+                "flops_per_launch": gemm_fl / gemm_n}
+    out = {
+        "metric": "BOR monostatic solves/s (mode x freq x angle) at <=0.01 dB vs FP64; MFMA/HBM %",
+        "value": round(solves / dt_mx, 2),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt_mx * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic (seeded generatrix)",
+        "config": {"workload": name, "modes": 2 * m_max + 1, "m_max": m_max, "aspects": len(thetas),
+                   "polarizations": 2, "frequencies": 1, "n_elems": int(len(nodes) - 1),
+                   "n_xi": list(nxi), "formulation": "cfie(alpha=0.5)", "precision": "mixed",
+                   "refine_iters": args.refine, "solves_per_step": solves},
+        "fp64_path_solves_per_s": round(fp64_rate, 2) if fp64_rate else None,
+        "speedup_vs_fp64_path": round((solves / dt_mx) / fp64_rate, 3) if fp64_rate else None,
+        "err_vs_fp64": err_db,
+        "dominant_kernel": {"family": fam, "ms_per_step": round(ms / args.steps, 3), "launches_per_step": cnt // max(1, args.steps)},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in kt_mx.items() if v[1]},
+        "fp64_kernel_ms": {k: round(v[0], 3) for k, v in kt_64.items() if v[1]},
+        "phase_ms_last_step": ph_mx,
+        "roofline": roof,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(nodes, freq, thetas, m_max),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,131 @@
+/*
+ * libghost_bor: MI355X (gfx950) body-of-revolution MoM hot path.
+ *
+ * C ABI only (plain pointers, sizes and ints; no torch types).  Every entry
+ * point returns 0 on success or a GHOST_ERR_* code, with the message
+ * available from ghost_last_error() (thread-local), except the three
+ * void-returning compat samplers whose signatures are fixed by the reference.
+ *
+ * Complex arrays are interleaved (re, im) doubles, row-major.
+ */
+#ifndef GHOST_BOR_H
+#define GHOST_BOR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GHOST_FP64 0  /* FP64 fill + FP64 LU */
+#define GHOST_MIXED 1 /* FP32 far fill + FP32 LU, FP64 near band, residual, refinement, far field */
+
+#define GHOST_UNMASKED 1 /* add_problem flag: keep all 2 Nn bases (no axis / open-end mask) */
+
+#define GHOST_ERR_ARG 1
+#define GHOST_ERR_STATE 2
+#define GHOST_ERR_HIP 3
+#define GHOST_ERR_OOM 4
+
+typedef struct ghost_bor ghost_bor;
+
+/* Message of the last failing call on this thread. */
+const char* ghost_last_error(void);
+
+/* ── compat samplers ─────────────────────────────────────────────────────
+ * Replace the ctypes-bound natives of bor_stream_kernel.c (loaded by
+ * bor_streaming.py:50-76, called at bor_streaming.py:289-293, 328-331):
+ * identical names, argument order and meaning; host pointers in/out,
+ * synchronous, reentrant; outputs are caller-allocated complex128
+ * [nr, np_, nxi] viewed as interleaved doubles; R is clamped at 1e-30.
+ * The void forms keep the reference ABI (no error channel); the ghost_
+ * forms return an error code. */
+void sample_g(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* rho_q,
+              const double* z_q, double k, const double* sin2_tab,
+              double* out); /* bor_stream_kernel.c:26-48 */
+void sample_mfie(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* tr_p,
+                 const double* tz_p, const double* rho_q, const double* z_q, const double* tr_q,
+                 const double* tz_q, double k, const double* cx_tab, const double* sx_tab, double* o_tt,
+                 double* o_tf, double* o_ft, double* o_ff); /* bor_stream_kernel.c:65-106 */
+void sample_ibc(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* tr_p,
+                const double* tz_p, const double* rho_q, const double* z_q, const double* tr_q,
+                const double* tz_q, double k, const double* cx_tab, const double* sx_tab, double* o_tt,
+                double* o_tf, double* o_ft, double* o_ff); /* bor_stream_kernel.c:108-150 */
+int ghost_sample_g(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* rho_q,
+                   const double* z_q, double k, const double* sin2_tab, double* out);
+int ghost_sample_mfie(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* tr_p,
+                      const double* tz_p, const double* rho_q, const double* z_q, const double* tr_q,
+                      const double* tz_q, double k, const double* cx_tab, const double* sx_tab,
+                      double* o_tt, double* o_tf, double* o_ft, double* o_ff);
+int ghost_sample_ibc(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* tr_p,
+                     const double* tz_p, const double* rho_q, const double* z_q, const double* tr_q,
+                     const double* tz_q, double k, const double* cx_tab, const double* sx_tab,
+                     double* o_tt, double* o_tf, double* o_ft, double* o_ff);
+
+/* ── device-resident mode sweep ──────────────────────────────────────────
+ * One handle holds a batch of problems (generatrix x frequency), each with
+ * its 2 m_max + 1 mode systems, on the device current at creation and on a
+ * stream of its own.  Replaces the per-frequency body of solve_bor
+ * (bor_solver.py:820-1004): BorPecSolver's fill (293-345, 399-435), the
+ * excitation (625-645, 437-461), _mode_sweep's np.linalg.solve (744-758)
+ * and farfield_mode (659-708).  Calls on one handle must not overlap. */
+
+/* precision: GHOST_FP64 or GHOST_MIXED; nb: LU block size (multiple of 8, <= 64; 0 = 64). */
+int ghost_bor_create(ghost_bor** out, int precision, int nb);
+int ghost_bor_destroy(ghost_bor* h);
+
+/* Keep FP64 copies of A and B for ghost_bor_residual (default on; before build). */
+int ghost_bor_set_residual(ghost_bor* h, int on);
+
+/* Add one problem.  nodes: [Nn][2] (rho, z), traversed +z end first
+ * (bor_kernels.py:38-81).  k: real wavenumber (2 pi f / c0).  m_max: modes
+ * -m_max..m_max.  n_xi_efie / n_xi_bracket: azimuth FFT sizes (powers of two;
+ * bor_kernels.py:149-152, 315-319).  Z = c_efie Z_EFIE + c_mfie Z_MFIE and
+ * V = c_efie V_E + c_mfie V_H (bor_solver.py:929-946: CFIE c_efie = alpha,
+ * c_mfie = (1 - alpha) eta0).  thetas_deg: monostatic aspects from +z.
+ * flags: 0 or GHOST_UNMASKED.  *idx receives the problem index. */
+int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, double k, int m_max, int n_xi_efie,
+                          int n_xi_bracket, double c_efie, double c_mfie, const double* thetas_deg,
+                          int n_theta, int flags, int* idx);
+
+/* Geometry tables, device buffers and the launch schedule. */
+int ghost_bor_build(ghost_bor* h);
+
+/* Fill: near band, far blocks, masked mode systems and RHS columns. */
+int ghost_bor_fill(ghost_bor* h);
+/* Batched LU + solve of every mode system; GHOST_MIXED adds refine_iters
+ * FP64-residual refinement sweeps. */
+int ghost_bor_solve(ghost_bor* h, int refine_iters);
+/* Monostatic far-field contribution of every mode and column. */
+int ghost_bor_farfield(ghost_bor* h);
+/* fill + solve + farfield. */
+int ghost_bor_run(ghost_bor* h, int refine_iters);
+int ghost_bor_sync(ghost_bor* h);
+
+/* dF of one problem: complex [2 m_max + 1][2 (VV, HH)][n_theta], mode index m + m_max. */
+int ghost_bor_get_dF(ghost_bor* h, int prob, double* out);
+/* ||A X - B||_F / ||B||_F for every system (bor_solver.py:754-756), out[sum of modes]. */
+int ghost_bor_residual(ghost_bor* h, double* out);
+/* Mode-system layout of (prob, m): size n, active J_t count nt and first active t / phi nodes. */
+int ghost_bor_system_info(ghost_bor* h, int prob, int m, int* n, int* nt, int* t0, int* f0);
+int ghost_bor_num_systems(ghost_bor* h, int prob, int* nmodes, int* m_max);
+/* Copy a system out (complex, row-major): which = 0 -> A [n x n] and B [n x nrhs]
+ * (after fill, before solve); which = 1 -> X [n x nrhs] (after solve). */
+int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, double* out1, double* out2);
+/* Zero-pivot count per system from the last factorisation. */
+int ghost_bor_info(ghost_bor* h, int* out);
+/* Milliseconds of the last fill, solve and far-field phases (HIP events on the handle's stream). */
+int ghost_bor_timings(ghost_bor* h, float* ms3);
+
+/* Optional per-launch profile: with on != 0 every later launch is bracketed by
+ * HIP events on the handle's stream (and the record is cleared). */
+int ghost_bor_set_profile(ghost_bor* h, int on);
+/* Number and names of the timed kernel families. */
+int ghost_bor_kernel_count(void);
+const char* ghost_bor_kernel_name(int kind);
+/* Per family since set_profile: summed ms, launch count, algorithmic real flops. */
+int ghost_bor_kernel_times(ghost_bor* h, double* ms, int* count, double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GHOST_BOR_H */

@@ -1,0 +1,160 @@
+"""HIP path vs the reference's golden vectors and the FP64 oracle (run on an MI355X)."""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+import bor_oracle as bo
+import mie
+from conftest import pkg
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = ["sphere12_cfie", "sphere12_efie", "conesphere16_cfie", "sphere10_stream"]
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name + ".npz"))
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def db(s):
+    return 10.0 * np.log10(np.asarray(s, dtype=float))
+
+
+def make_handle(nodes, freq_hz, thetas, m_max, alpha, streaming, precision=0, unmasked=False, nb=64):
+    L = pkg("_lib")
+    bs = pkg("bor_solver")
+    pts = np.asarray(nodes, dtype=float)
+    k = 2 * math.pi * freq_hz / bs.C0
+    form = "cfie" if 0 < alpha < 1 else ("efie" if alpha == 1 else "mfie")
+    _, ne, nbk, _ = bs.discretisation(pts, k, np.asarray(thetas), m_max, form,
+                                      "streaming" if streaming else "tables")
+    h = L.Handle(precision, nb=nb)
+    h.add_problem(pts, k, m_max, ne, nbk, alpha, (1 - alpha) * bs.ETA0, thetas, unmasked=unmasked)
+    h.build()
+    return h
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_systems_currents_and_farfield_match_reference(name):
+    d = load(name)
+    mm = int(d["m_max"])
+    print(f"[case] {name} m_max={mm}", flush=True)
+    h = make_handle(d["nodes"], float(d["freq_hz"]), d["thetas"], mm, float(d["alpha"]), "stream" in name)
+    h.fill()
+    h.sync()
+    for m in range(-mm, mm + 1):
+        A, B = h.system(0, m)
+        assert A.shape == d[f"A_{m}"].shape
+        assert rel(A, d[f"A_{m}"]) < 1e-11, (m, rel(A, d[f"A_{m}"]))
+        assert rel(B, d[f"B_{m}"]) < 1e-12, (m, rel(B, d[f"B_{m}"]))
+    h.solve()
+    h.farfield()
+    h.sync()
+    dF = h.dF(0)
+    for m in range(-mm, mm + 1):
+        X = h.solution(0, m)
+        assert rel(X, d[f"X_{m}"]) < 1e-9, (m, rel(X, d[f"X_{m}"]))
+        assert rel(dF[m + mm], d[f"dF_{m}"]) < 1e-9, (m, rel(dF[m + mm], d[f"dF_{m}"]))
+    res = h.residuals()
+    assert np.max(res) < 1e-13
+    assert not np.any(h.zero_pivots())
+    h.close()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_solve_bor_matches_reference(name, bs):
+    d = load(name)
+    form = "cfie" if float(d["alpha"]) < 1 else "efie"
+    out = bs.solve_bor(d["nodes"], float(d["freq_hz"]), d["thetas"], formulation=form,
+                       n_modes=int(d["m_max"]), mode_tol=1e-6,
+                       assembly="streaming" if "stream" in name else "auto")
+    assert out["modes_used"] == int(d["modes_used"])
+    assert np.max(np.abs(db(out["sigma_vv"]) - db(d["sigma_vv"]))) < 1e-7
+    assert np.max(np.abs(db(out["sigma_hh"]) - db(d["sigma_hh"]))) < 1e-7
+    assert rel(np.array(out["amp_vv"]), d["amp_vv"]) < 1e-9
+    assert out["linear_residual"] < 1e-13
+
+
+def test_unmasked_blocks_match_oracle():
+    nodes = bo.sphere_nodes(0.5, 14)
+    f, mm = 3.5e8, 4
+    sysm = bo.ModeSystem(nodes, f, mm, "cfie", 0.5, streaming=False)
+    h = make_handle(nodes, f, [0.0, 90.0], mm, 0.5, False, unmasked=True)
+    h.fill()
+    h.sync()
+    for m in range(-mm, mm + 1):
+        A, _ = h.system(0, m)
+        ref = 0.5 * sysm.efie_full(m) + 0.5 * bo.ETA0 * sysm.mfie_full(m)
+        assert rel(A, ref) < 1e-11, (m, rel(A, ref))
+    h.close()
+
+
+def test_larger_sphere_against_oracle_and_mie(bs):
+    nodes = bo.sphere_nodes(0.5, 48)
+    thetas = np.array([0.0, 37.0, 90.0, 143.0, 180.0])
+    f = 3e8
+    ref = bo.solve_bor(nodes, f, thetas, formulation="cfie", n_modes=9, mode_tol=0.0, keep=True)
+    out = bs.solve_bor(nodes, f, thetas, formulation="cfie", n_modes=9, mode_tol=0.0)
+    assert np.max(np.abs(db(out["sigma_vv"]) - db(ref["sigma_vv"]))) < 1e-8
+    assert np.max(np.abs(db(out["sigma_hh"]) - db(ref["sigma_hh"]))) < 1e-8
+    s_mie = db(mie.sigma_pec_sphere(0.5, f))
+    assert np.max(np.abs(db(out["sigma_vv"]) - s_mie)) < 0.1
+
+
+def test_mixed_precision_within_0p01_db(bs):
+    nodes = bo.sphere_nodes(0.5, 40)
+    thetas = np.linspace(0, 180, 19)
+    a = bs.solve_bor(nodes, 3e8, thetas, formulation="cfie", n_modes=9, mode_tol=0.0)
+    b = bs.solve_bor(nodes, 3e8, thetas, formulation="cfie", n_modes=9, mode_tol=0.0,
+                     precision="mixed", refine_iters=2)
+    assert np.max(np.abs(db(a["sigma_vv"]) - db(b["sigma_vv"]))) < 0.01
+    assert np.max(np.abs(db(a["sigma_hh"]) - db(b["sigma_hh"]))) < 0.01
+    assert b["linear_residual"] < 1e-10
+
+
+def test_fill_is_deterministic():
+    nodes = bo.sphere_nodes(0.5, 20)
+    h = make_handle(nodes, 3e8, [10.0, 80.0], 5, 0.5, False)
+    h.fill()
+    h.sync()
+    A1, _ = h.system(0, 2)
+    h.fill()
+    h.sync()
+    A2, _ = h.system(0, 2)
+    assert np.array_equal(A1, A2)
+    h.close()
+
+
+def test_compat_samplers_match_oracle():
+    L = pkg("_lib")
+    lib = L.load()
+    rng = np.random.default_rng(3)
+    nr, np_, nxi = 3, 5, 16
+    rp, zp, rq, zq = rng.uniform(0.1, 1, nr), rng.uniform(-1, 1, nr), rng.uniform(0.1, 1, np_), rng.uniform(-1, 1, np_)
+    k = 7.3
+    xi = bo.xi_grid(nxi)
+    sin2 = np.sin(0.5 * xi) ** 2
+    out = np.zeros((nr, np_, nxi), dtype=np.complex128)
+    lib.sample_g(nr, np_, nxi, L.dptr(rp), L.dptr(zp), L.dptr(rq), L.dptr(zq), k, L.dptr(sin2),
+                 out.ctypes.data_as(L._dp))
+    ref = bo.green_samples(rp[:, None], zp[:, None], rq[None, :], zq[None, :], k, xi, r_min=1e-30)
+    assert rel(out, ref) < 1e-13
+    trp, tzp = rng.uniform(-1, 1, nr), rng.uniform(-1, 1, nr)
+    trq, tzq = rng.uniform(-1, 1, np_), rng.uniform(-1, 1, np_)
+    cx, sx = np.cos(xi), np.sin(xi)
+    for fn, refn in ((lib.sample_mfie, bo.bracket_samples), (lib.sample_ibc, bo.ibc_bracket_samples)):
+        outs = [np.zeros((nr, np_, nxi), dtype=np.complex128) for _ in range(4)]
+        fn(nr, np_, nxi, L.dptr(rp), L.dptr(zp), L.dptr(trp), L.dptr(tzp), L.dptr(rq), L.dptr(zq),
+           L.dptr(trq), L.dptr(tzq), k, L.dptr(cx), L.dptr(sx), *[o.ctypes.data_as(L._dp) for o in outs])
+        refs = refn(rp[:, None], zp[:, None], trp[:, None], tzp[:, None], rq[None, :], zq[None, :],
+                    trq[None, :], tzq[None, :], k, xi, r_min=1e-30)
+        for o, r in zip(outs, refs):
+            assert rel(o, r) < 1e-12
