@@ -51,6 +51,9 @@ SIGNATURES = [
     ("ghost_bor_get_system", _ci, [_vp, _ci, _ci, _ci, _dp, _dp]),
     ("ghost_bor_info", _ci, [_vp, _ip]),
     ("ghost_bor_timings", _ci, [_vp, _fp]),
+    ("ghost_bor_capture", _ci, [_vp, _ci]),
+    ("ghost_bor_launch", _ci, [_vp]),
+    ("ghost_bor_graph_ms", _ci, [_vp, _fp]),
     ("ghost_bor_set_profile", _ci, [_vp, _ci]),
     ("ghost_bor_kernel_count", _ci, []),
     ("ghost_bor_kernel_name", ctypes.c_char_p, [_ci]),
@@ -198,6 +201,17 @@ class Handle:
         out = np.zeros(n, dtype=np.int32)
         check(self.lib.ghost_bor_info(self._h, iptr(out)))
         return out
+
+    def capture(self, refine_iters: int = 0):
+        check(self.lib.ghost_bor_capture(self._h, int(refine_iters)))
+
+    def launch(self):
+        check(self.lib.ghost_bor_launch(self._h))
+
+    def graph_ms(self) -> float:
+        out = np.zeros(1, dtype=np.float32)
+        check(self.lib.ghost_bor_graph_ms(self._h, out.ctypes.data_as(_fp)))
+        return float(out[0])
 
     def set_profile(self, on: bool = True):
         check(self.lib.ghost_bor_set_profile(self._h, 1 if on else 0))

@@ -144,6 +144,96 @@ def _coefficients(form: str, cfie_alpha: float):
 
 
 # ─────────────────────────────────────────────────────────────────────────────
+# batched sweeps
+# ─────────────────────────────────────────────────────────────────────────────
+
+def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, prec, refine_iters, times):
+    F, used = truncate_modes(dF, m_max, mode_tol)
+    out = {
+        "theta_deg": thetas.tolist(),
+        "sigma_vv": (4.0 * math.pi * np.abs(F[0]) ** 2).tolist(),
+        "sigma_hh": (4.0 * math.pi * np.abs(F[1]) ** 2).tolist(),
+        "amp_vv": F[0].tolist(),
+        "amp_hh": F[1].tolist(),
+        "modes_used": used,
+        "n_unknowns": int(2 * len(pts)),
+        "formulation": form,
+        "assembly": "streaming" if streaming else "tables",
+        "table_precision": "single" if prec == "mixed" else "double",
+        "stream_mode_block": None,
+        "stream_sweeps": 0,
+        "warnings": [],
+        "precision": prec,
+        "refine_iters": refine_iters if prec == "mixed" else 0,
+        "n_xi": (ne, nbk),
+        "m_max": m_max,
+        "timings_ms": times,
+    }
+    if res is not None:
+        # max over the modes the truncated sweep used (bor_solver.py:754-756, 784)
+        ms = np.arange(-m_max, m_max + 1)
+        out["linear_residual"] = float(np.max(res[np.abs(ms) <= used]))
+    else:
+        out["linear_residual"] = float("nan")
+    return out
+
+
+def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_iters: int = 2,
+                    residual: bool = True, nb: int = 64, mode_tol: float = 1e-6,
+                    graph: bool = False, check_abort: Optional[Callable] = None) -> List[Dict]:
+    """Solve several (generatrix, frequency) problems as ONE batch: every mode
+    system of every problem goes through the same batched fill, LU/solve and
+    far-field launches (systems of uneven size share each launch).  Each job
+    is a dict with keys points, freq_hz, thetas_deg and optionally
+    formulation ("cfie"), cfie_alpha (0.5), n_modes, assembly.  Returns one
+    solve_bor-style dict per job.  graph=True captures the batch in a
+    hipGraph and replays it."""
+    prec = str(precision).strip().lower()
+    if prec not in _PRECISION:
+        raise ValueError("precision must be 'fp64' or 'mixed'.")
+    specs = []
+    for j in jobs:
+        form = str(j.get("formulation", "cfie")).strip().lower()
+        if form not in ("efie", "mfie", "cfie"):
+            raise ValueError("formulation must be 'efie', 'mfie' or 'cfie'.")
+        pts = _validate_nodes(j["points"])
+        thetas = np.atleast_1d(np.asarray(j["thetas_deg"], dtype=float))
+        k = 2.0 * math.pi * float(j["freq_hz"]) / C0
+        m_max, ne, nbk, streaming = discretisation(pts, k, thetas, j.get("n_modes"), form,
+                                                   j.get("assembly", "auto"))
+        cE, cM = _coefficients(form, j.get("cfie_alpha", 0.5))
+        specs.append((pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM))
+    if check_abort is not None:
+        check_abort()
+    h = _lib.Handle(_PRECISION[prec], nb=nb, residual=residual)
+    try:
+        for (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM) in specs:
+            h.add_problem(pts, k, m_max, ne, nbk, cE, cM, thetas)
+        h.build()
+        it = refine_iters if prec == "mixed" else 0
+        if graph:
+            h.capture(it)
+            h.launch()
+        else:
+            h.run(it)
+        dFs = [h.dF(i) for i in range(len(specs))]
+        res = h.residuals() if residual else None
+        times = h.timings() if not graph else {"graph_ms": h.graph_ms()}
+    finally:
+        h.close()
+    if check_abort is not None:
+        check_abort()
+    outs, off = [], 0
+    for i, (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM) in enumerate(specs):
+        nm = 2 * m_max + 1
+        r = res[off:off + nm] if res is not None else None
+        off += nm
+        outs.append(_result(pts, thetas, form, m_max, ne, nbk, streaming, dFs[i], r, mode_tol, prec,
+                            refine_iters, times))
+    return outs
+
+
+# ─────────────────────────────────────────────────────────────────────────────
 # solve_bor
 # ─────────────────────────────────────────────────────────────────────────────
 
@@ -174,57 +264,13 @@ def solve_bor(points, freq_hz: float, thetas_deg, formulation: str = "efie",
     tp = str(table_precision).strip().lower()
     if tp not in ("auto", "single", "double"):
         raise ValueError("table_precision must be 'auto', 'single', or 'double'.")
-    prec = str(precision).strip().lower()
-    if prec not in _PRECISION:
-        raise ValueError("precision must be 'fp64' or 'mixed'.")
-    pts = _validate_nodes(points)
-    thetas = np.atleast_1d(np.asarray(thetas_deg, dtype=float))
-    k = 2.0 * math.pi * freq_hz / C0
-    m_max, ne, nbk, streaming = discretisation(pts, k, thetas, n_modes, form, assembly)
-    cE, cM = _coefficients(form, cfie_alpha)
-    if check_abort is not None:
-        check_abort()
-
-    h = _lib.Handle(_PRECISION[prec], nb=nb, residual=residual)
-    try:
-        prob = h.add_problem(pts, k, m_max, ne, nbk, cE, cM, thetas)
-        h.build()
-        h.run(refine_iters if prec == "mixed" else 0)
-        dF = h.dF(prob)
-        res = h.residuals() if residual else None
-        times = h.timings()
-    finally:
-        h.close()
-    if check_abort is not None:
-        check_abort()
-    F, used = truncate_modes(dF, m_max, mode_tol)
+    out = solve_bor_batch([{"points": points, "freq_hz": freq_hz, "thetas_deg": thetas_deg,
+                            "formulation": form, "cfie_alpha": cfie_alpha, "n_modes": n_modes,
+                            "assembly": assembly}],
+                          precision=precision, refine_iters=refine_iters, residual=residual, nb=nb,
+                          mode_tol=mode_tol, check_abort=check_abort)[0]
     if progress is not None:
-        progress(used, m_max)
-    out = {
-        "theta_deg": thetas.tolist(),
-        "sigma_vv": (4.0 * math.pi * np.abs(F[0]) ** 2).tolist(),
-        "sigma_hh": (4.0 * math.pi * np.abs(F[1]) ** 2).tolist(),
-        "amp_vv": F[0].tolist(),
-        "amp_hh": F[1].tolist(),
-        "modes_used": used,
-        "n_unknowns": int(2 * len(pts)),
-        "formulation": form,
-        "assembly": "streaming" if streaming else "tables",
-        "table_precision": "single" if prec == "mixed" else "double",
-        "stream_mode_block": None,
-        "stream_sweeps": 0,
-        "warnings": [],
-        "precision": prec,
-        "refine_iters": refine_iters if prec == "mixed" else 0,
-        "n_xi": (ne, nbk),
-        "timings_ms": times,
-    }
-    if res is not None:
-        # max over the modes the truncated sweep used (bor_solver.py:754-756, 784)
-        ms = np.arange(-m_max, m_max + 1)
-        out["linear_residual"] = float(np.max(res[np.abs(ms) <= used]))
-    else:
-        out["linear_residual"] = float("nan")
+        progress(out["modes_used"], out["m_max"])
     return out
 
 

@@ -254,6 +254,7 @@ struct ghost_bor {
   std::vector<DevBuf> allocs;
   // optional per-launch profile
   bool profile = false;
+  bool capturing = false;
   std::vector<hipEvent_t> pev;
   struct Rec { int kind; double flops; int e0; };
   std::vector<Rec> recs;
@@ -1079,10 +1080,10 @@ static int residual_gemm(ghost_bor* h, int x_in_aug) {
 extern "C" int ghost_bor_fill(ghost_bor* h) {
   if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
   HIPCHK(hipSetDevice(h->device));
-  HIPCHK(hipEventRecord(h->ev[0], h->stream));
+  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[0], h->stream));
   int rc = run_fill(h);
   if (rc) return rc;
-  HIPCHK(hipEventRecord(h->ev[1], h->stream));
+  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[1], h->stream));
   return 0;
 }
 
@@ -1110,7 +1111,7 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
   HIPCHK(hipSetDevice(h->device));
   const int nsys = (int)h->sys.size();
   const bool mixed = h->precision == GHOST_MIXED;
-  HIPCHK(hipEventRecord(h->ev[2], h->stream));
+  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[2], h->stream));
   int rc;
   h->refine_hist.clear();
   if (!mixed) {
@@ -1136,7 +1137,7 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
       LAUNCHCHK();
     }
   }
-  HIPCHK(hipEventRecord(h->ev[3], h->stream));
+  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[3], h->stream));
   h->refine_iters = refine_iters;
   return 0;
 }
@@ -1145,7 +1146,7 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
   if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
   HIPCHK(hipSetDevice(h->device));
   const bool mixed = h->precision == GHOST_MIXED;
-  HIPCHK(hipEventRecord(h->ev[4], h->stream));
+  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[4], h->stream));
   const int nsys = (int)h->sys.size();
   for (auto& p : h->probs) {
     ExciteArgs ea;
@@ -1179,7 +1180,7 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
     LAUNCHCHK();
   }
   (void)nsys;
-  HIPCHK(hipEventRecord(h->ev[5], h->stream));
+  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[5], h->stream));
   return 0;
 }
 
@@ -1232,6 +1233,50 @@ extern "C" int ghost_bor_kernel_times(ghost_bor* h, double* ms, int* count, doub
     count[r.kind] += 1;
     flops[r.kind] += r.flops;
   }
+  return 0;
+}
+
+// Capture fill + solve + far field of the whole batch into one hipGraph
+// (kernels only: no allocations, memsets or host synchronisation inside).
+extern "C" int ghost_bor_capture(ghost_bor* h, int refine_iters) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  const bool prof = h->profile;
+  h->profile = false;
+  h->capturing = true;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  int rc = run_fill(h);
+  if (!rc) rc = ghost_bor_solve(h, refine_iters);
+  if (!rc) rc = ghost_bor_farfield(h);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  h->capturing = false;
+  h->profile = prof;
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return fail(GHOST_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+  h->graph = g;
+  HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+// Replay the captured graph on the handle's stream (asynchronous).
+extern "C" int ghost_bor_launch(ghost_bor* h) {
+  if (!h || !h->gexec) return fail(GHOST_ERR_STATE, "capture first");
+  HIPCHK(hipSetDevice(h->device));
+  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[0], h->stream));
+  HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[6], h->stream));
+  return 0;
+}
+
+// Milliseconds of the last graph replay.
+extern "C" int ghost_bor_graph_ms(ghost_bor* h, float* ms) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipEventElapsedTime(ms, h->ev[0], h->ev[6]));
   return 0;
 }
 

@@ -100,6 +100,14 @@ int ghost_bor_farfield(ghost_bor* h);
 int ghost_bor_run(ghost_bor* h, int refine_iters);
 int ghost_bor_sync(ghost_bor* h);
 
+/* Capture fill + solve + far field of the whole batch into one hipGraph, and
+ * replay it (asynchronously) on the handle's stream; graph_ms gives the last
+ * replay's time.  Together they replace the reference's CPU mode-sweep
+ * driver loop (bor_solver.py:774-796) for a whole batch of problems. */
+int ghost_bor_capture(ghost_bor* h, int refine_iters);
+int ghost_bor_launch(ghost_bor* h);
+int ghost_bor_graph_ms(ghost_bor* h, float* ms);
+
 /* dF of one problem: complex [2 m_max + 1][2 (VV, HH)][n_theta], mode index m + m_max. */
 int ghost_bor_get_dF(ghost_bor* h, int prob, double* out);
 /* ||A X - B||_F / ||B||_F for every system (bor_solver.py:754-756), out[sum of modes]. */

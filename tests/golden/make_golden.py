@@ -37,6 +37,19 @@ def cone_sphere_nodes(n_cone: int, n_sph: int, r: float = 0.3, half_deg: float =
     return np.vstack([cone, sph])
 
 
+def dispatch_snapshot():
+    """A 0.4 m PEC sphere drawn as two TYPE 2 segments (+z pole -> equator -> -z pole),
+    10 primitives each, with 2 elements per primitive (N = 2)."""
+    def arc(t0, t1, n):
+        th = np.linspace(t0, t1, n + 1)
+        x, y = 0.4 * np.sin(th), 0.4 * np.cos(th)
+        return [{"x1": x[i], "y1": y[i], "x2": x[i + 1], "y2": y[i + 1]} for i in range(n)]
+    return {"title": "sphere", "segments": [
+        {"name": "north", "seg_type": 2, "properties": [2, 2, 0, 0, 0], "point_pairs": arc(0.0, math.pi / 2, 10)},
+        {"name": "south", "seg_type": 2, "properties": [2, 2, 0, 0, 0], "point_pairs": arc(math.pi / 2, math.pi, 10)},
+    ], "ibcs": [], "dielectrics": []}
+
+
 def main(ref_dir: str):
     sys.path.insert(0, ref_dir)
     import bor_kernels as bk
@@ -111,6 +124,22 @@ def main(ref_dir: str):
         data["modes_used"] = out["modes_used"]
         data["linear_residual"] = out["linear_residual"]
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **data)
+
+    # ── dispatch: snapshot -> chains -> mesh -> samples (bor_dispatch.py:109-251, 335-702) ──
+    import bor_dispatch as bd
+    snap = dispatch_snapshot()
+    out = bd.solve_monostatic_rcs_bor(snap, [0.3], [0.0, 60.0, 90.0, 180.0], "VV",
+                                      geometry_units="meters", cfie_alpha=0.5, n_modes=3,
+                                      workers=1, table_precision="double")
+    chains = bd._chains_from_snapshot(snap, 1.0)
+    diag = max(float(np.ptp(np.vstack([c.pts for c in chains]), axis=0).max()), 1e-9)
+    ordered = bd._stitch_generatrix(chains, "TYPE 2", max(1e-12, 1e-9 * diag))
+    pts, _, _ = bd._mesh_generatrix(ordered, bk.C0 / 0.3e9, 50000, 1e-6 * diag)
+    np.savez_compressed(os.path.join(HERE, "dispatch_sphere.npz"), nodes=pts,
+                        rcs_db=np.array([s["rcs_db"] for s in out["samples"]]),
+                        amp=np.array([s["rcs_amp_real"] + 1j * s["rcs_amp_imag"] for s in out["samples"]]),
+                        theta=np.array([s["theta_inc_deg"] for s in out["samples"]]),
+                        modes_used=out["metadata"]["per_frequency"][0]["modes_used"])
 
     # ── Mie anchors (analytic oracle, mie_sphere.py:231-235) ──
     ka = np.array([0.05, 0.5, 1.0, math.pi, 5.0, 10.0])

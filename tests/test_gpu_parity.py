@@ -158,3 +158,49 @@ def test_compat_samplers_match_oracle():
                     trq[None, :], tzq[None, :], k, xi, r_min=1e-30)
         for o, r in zip(outs, refs):
             assert rel(o, r) < 1e-12
+
+
+def test_dispatch_matches_reference():
+    import sys as _s
+    _s.path.insert(0, GOLD)
+    from make_golden import dispatch_snapshot
+    bd = pkg("bor_dispatch")
+    d = load("dispatch_sphere")
+    out = bd.solve_monostatic_rcs_bor(dispatch_snapshot(), [0.3], [0.0, 60.0, 90.0, 180.0], "VV",
+                                      geometry_units="meters", cfie_alpha=0.5, n_modes=3, workers=1)
+    got = np.array([s["rcs_db"] for s in out["samples"]])
+    assert np.max(np.abs(got - d["rcs_db"])) < 1e-7
+    amp = np.array([s["rcs_amp_real"] + 1j * s["rcs_amp_imag"] for s in out["samples"]])
+    assert rel(amp, d["amp"]) < 1e-9
+    assert out["metadata"]["per_frequency"][0]["modes_used"] == int(d["modes_used"])
+
+
+def test_batched_frequencies_match_single_solves(bs):
+    nodes = bo.sphere_nodes(0.5, 30)
+    jobs = [{"points": nodes, "freq_hz": f, "thetas_deg": [0.0, 45.0, 90.0], "n_modes": 6}
+            for f in (2e8, 3e8, 4.5e8)]
+    jobs.append({"points": bo.sphere_nodes(0.3, 18), "freq_hz": 5e8, "thetas_deg": [10.0, 170.0], "n_modes": 4})
+    batch = bs.solve_bor_batch(jobs, mode_tol=0.0)
+    graph = bs.solve_bor_batch(jobs, mode_tol=0.0, graph=True)
+    for j, b, g in zip(jobs, batch, graph):
+        one = bs.solve_bor(j["points"], j["freq_hz"], j["thetas_deg"], formulation="cfie",
+                           n_modes=j["n_modes"], mode_tol=0.0)
+        assert rel(np.array(b["amp_vv"]), np.array(one["amp_vv"])) < 1e-12
+        assert rel(np.array(g["amp_hh"]), np.array(one["amp_hh"])) < 1e-12
+
+
+def test_graph_replays_match_eager():
+    nodes = bo.sphere_nodes(0.5, 24)
+    h = make_handle(nodes, 3e8, [0.0, 90.0, 135.0], 6, 0.5, False)
+    h.run()
+    h.sync()
+    ref = h.dF(0)
+    h.capture()
+    for rep in range(3):
+        h.launch()
+        h.sync()
+        got = h.dF(0)
+        assert np.array_equal(got, ref), rep
+        h.run()          # eager work between replays
+        h.sync()
+    h.close()
