@@ -44,6 +44,7 @@ SIGNATURES = [
     ("ghost_bor_farfield", _ci, [_vp]),
     ("ghost_bor_run", _ci, [_vp, _ci]),
     ("ghost_bor_sync", _ci, [_vp]),
+    ("ghost_device_synchronize", _ci, []),
     ("ghost_bor_get_dF", _ci, [_vp, _ci, _dp]),
     ("ghost_bor_residual", _ci, [_vp, _dp]),
     ("ghost_bor_system_info", _ci, [_vp, _ci, _ci, _ip, _ip, _ip, _ip]),
@@ -59,6 +60,10 @@ SIGNATURES = [
     ("ghost_bor_kernel_name", ctypes.c_char_p, [_ci]),
     ("ghost_bor_kernel_times", _ci, [_vp, _dp, _ip, _dp]),
 ]
+
+
+def device_synchronize() -> None:
+    check(load().ghost_device_synchronize())
 
 
 class GhostLibraryError(RuntimeError):

@@ -998,11 +998,12 @@ template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launc
     prof_begin(h);
     switch (L.kind) {
       case L_PANEL:
+        // register sub-panel: 8 columns, 4 when FP64 rows need 2+ registers sets
         switch (L.arg) {
-          case 1: hipLaunchKernelGGL((k_lu_panel<T, 1>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
-          case 2: hipLaunchKernelGGL((k_lu_panel<T, 2>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
-          case 3: hipLaunchKernelGGL((k_lu_panel<T, 3>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
-          default: hipLaunchKernelGGL((k_lu_panel<T, 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          case 1: hipLaunchKernelGGL((k_lu_panel<T, 1, 8>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          case 2: hipLaunchKernelGGL((k_lu_panel<T, 2, sizeof(T) == 8 ? 4 : 8>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          case 3: hipLaunchKernelGGL((k_lu_panel<T, 3, sizeof(T) == 8 ? 4 : 8>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          default: hipLaunchKernelGGL((k_lu_panel<T, 4, 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
         }
         break;
       case L_SWAP:
@@ -1277,6 +1278,12 @@ extern "C" int ghost_bor_graph_ms(ghost_bor* h, float* ms) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipEventElapsedTime(ms, h->ev[0], h->ev[6]));
+  return 0;
+}
+
+// Device-wide barrier on the current device (every stream, every handle).
+extern "C" int ghost_device_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
   return 0;
 }
 

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, const vo
 }
 
 // ─────────────────────────────────────────────────────────────────────────────
-// panel factorisation: one 1024-thread block per system, 8-column sub-panels
+// panel factorisation: one 1024-thread block per system, SW-column sub-panels
 // held in registers (rows r0 + tid + 1024 i), pivots by max |Re| + |Im|
 // ─────────────────────────────────────────────────────────────────────────────
 
@@ -159,7 +159,7 @@ template <typename T> __device__ __forceinline__ typename CT<T>::c cdiv1(typenam
   return r;
 }
 
-template <typename T, int RPT>
+template <typename T, int RPT, int SW>
 __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* base, int* pivots, int* info) {
   typedef typename CT<T>::c C;
   const PanelOp op = ops[blockIdx.x];
@@ -172,21 +172,21 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
   __shared__ T red_v[16];
   __shared__ int red_i[16];
   __shared__ int piv_row;
-  __shared__ C prow[8];
-  __shared__ C srow[8];
-  __shared__ C L11[8][8];
-  __shared__ C U12[8][64];
-  __shared__ int piv_s[8];
+  __shared__ C prow[SW];
+  __shared__ C srow[SW];
+  __shared__ C L11[SW][SW];
+  __shared__ C U12[SW][64];
+  __shared__ int piv_s[SW];
 
-  for (int j0 = 0; j0 < w; j0 += 8) {
-    const int ws = min(8, w - j0);
+  for (int j0 = 0; j0 < w; j0 += SW) {
+    const int ws = min(SW, w - j0);
     const int c0 = k + j0;
-    C v[RPT][8];
+    C v[RPT][SW];
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int row = c0 + tid + 1024 * i;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
+      for (int c = 0; c < SW; ++c) {
         C z;
         z.x = 0; z.y = 0;
         if (row < n && c < ws) z = A[(size_t)row * ld + c0 + c];
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
         if (row >= col && row < n) {
           C z;
 #pragma unroll
-          for (int c = 0; c < 8; ++c) if (c == jj) z = v[i][c];
+          for (int c = 0; c < SW; ++c) if (c == jj) z = v[i][c];
           const T mag = fabs(z.x) + fabs(z.y);
           if (mag > best || (mag == best && row < bidx)) { best = mag; bidx = row; }
         }
@@ -236,14 +236,14 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
         for (int i = 0; i < RPT; ++i)
           if (i == (oc >> 10))
 #pragma unroll
-            for (int c = 0; c < 8; ++c) srow[c] = v[i][c];
+            for (int c = 0; c < SW; ++c) srow[c] = v[i][c];
       }
       if ((op_ & 1023) == tid) {
 #pragma unroll
         for (int i = 0; i < RPT; ++i)
           if (i == (op_ >> 10))
 #pragma unroll
-            for (int c = 0; c < 8; ++c) prow[c] = v[i][c];
+            for (int c = 0; c < SW; ++c) prow[c] = v[i][c];
       }
       __syncthreads();
       if (pr != col) {
@@ -252,20 +252,20 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
           for (int i = 0; i < RPT; ++i)
             if (i == (oc >> 10))
 #pragma unroll
-              for (int c = 0; c < 8; ++c) v[i][c] = prow[c];
+              for (int c = 0; c < SW; ++c) v[i][c] = prow[c];
         }
         if ((op_ & 1023) == tid) {
 #pragma unroll
           for (int i = 0; i < RPT; ++i)
             if (i == (op_ >> 10))
 #pragma unroll
-              for (int c = 0; c < 8; ++c) v[i][c] = srow[c];
+              for (int c = 0; c < SW; ++c) v[i][c] = srow[c];
         }
       }
       // ── scale the column, rank-1 update of the sub-panel ──
       C pv;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) if (c == jj) pv = prow[c];
+      for (int c = 0; c < SW; ++c) if (c == jj) pv = prow[c];
       const bool ok = (pv.x != (T)0 || pv.y != (T)0);
       const C inv = ok ? cdiv1<T>(pv) : pv;
 #pragma unroll
@@ -274,10 +274,10 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
         if (row > col && row < n && ok) {
           C l;
 #pragma unroll
-          for (int c = 0; c < 8; ++c) if (c == jj) l = v[i][c];
+          for (int c = 0; c < SW; ++c) if (c == jj) l = v[i][c];
           const C lv = cmul(l, inv);
 #pragma unroll
-          for (int c = 0; c < 8; ++c) {
+          for (int c = 0; c < SW; ++c) {
             if (c == jj) v[i][c] = lv;
             else if (c > jj && c < ws) v[i][c] = csub(v[i][c], cmul(lv, prow[c]));
           }
@@ -291,12 +291,12 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
       const int row = c0 + tid + 1024 * i;
       if (row < n)
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
+        for (int c = 0; c < SW; ++c)
           if (c < ws) A[(size_t)row * ld + c0 + c] = v[i][c];
     }
     if (tid < ws) {
 #pragma unroll
-      for (int c = 0; c < 8; ++c) L11[tid][c] = v[0][c];
+      for (int c = 0; c < SW; ++c) L11[tid][c] = v[0][c];
     }
     __syncthreads();
     // ── interchanges on the other panel columns [k, c0) and [c0 + ws, k + w) ──
@@ -319,13 +319,13 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
       // U12 = L11^{-1} A12 (unit lower), one thread per right column
       if (tid < nright) {
         const int cc = c0 + ws + tid;
-        C x[8];
+        C x[SW];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < SW; ++r) {
           if (r < ws) {
             C z = A[(size_t)(c0 + r) * ld + cc];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
+            for (int q = 0; q < SW; ++q)
               if (q < r) z = csub(z, cmul(L11[r][q], x[q]));
             x[r] = z;
             A[(size_t)(c0 + r) * ld + cc] = z;
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
           for (int c = 0; c < nright; ++c) {
             C acc = p[c];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
+            for (int q = 0; q < SW; ++q)
               if (q < ws) acc = csub(acc, cmul(v[i][q], U12[q][c]));
             p[c] = acc;
           }

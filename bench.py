@@ -36,7 +36,12 @@ def db(x):
     return 10.0 * np.log10(np.maximum(np.abs(x) ** 2 * 4 * math.pi, 1e-300))
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profile=True):
+    log(f"{precision}: build (m_max={m_max}, n_elems={len(nodes) - 1}, aspects={len(thetas)})")
     L = pkg("_lib")
     bs = pkg("bor_solver")
     k = 2 * math.pi * freq / bs.C0
@@ -44,19 +49,22 @@ def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profil
     h = L.Handle(L.GHOST_MIXED if precision == "mixed" else L.GHOST_FP64, nb=64, residual=False)
     h.add_problem(nodes, k, m_max, ne, nbk, 0.5, 0.5 * bs.ETA0, thetas)
     h.build()
-    for _ in range(warmup):
+    for i in range(warmup):
         h.run(refine)
-    h.sync()
-    import torch  # device synchronisation on both sides of the timed region
-    torch.cuda.synchronize()
+        h.sync()
+        log(f"{precision}: warmup {i + 1}/{warmup} done")
+    # device-wide barrier on both sides of the timed region (libghost_bor owns the
+    # device work; torch's own HIP runtime is not loaded into this process)
+    L.device_synchronize()
     if profile:
         h.set_profile(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         h.run(refine)
     h.sync()
-    torch.cuda.synchronize()
+    L.device_synchronize()
     dt = (time.perf_counter() - t0) / steps
+    log(f"{precision}: {steps} timed steps, {dt * 1e3:.1f} ms/step")
     ktimes = h.kernel_times() if profile else {}
     phase = h.timings()
     dF = h.dF(0)
@@ -71,6 +79,7 @@ def cpu_baseline(nodes, freq, thetas, m_max, budget_s=25.0):
     to the full sweep."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import bor_oracle as bo
+    log("cpu baseline: oracle sample")
     body = bo.Body(nodes)
     k = 2 * math.pi * freq / bo.C0
     osc = bo.osc_for(body, k, streaming=True)

@@ -99,6 +99,8 @@ int ghost_bor_farfield(ghost_bor* h);
 /* fill + solve + farfield. */
 int ghost_bor_run(ghost_bor* h, int refine_iters);
 int ghost_bor_sync(ghost_bor* h);
+/* Device-wide barrier (hipDeviceSynchronize) on the current device. */
+int ghost_device_synchronize(void);
 
 /* Capture fill + solve + far field of the whole batch into one hipGraph, and
  * replay it (asynchronously) on the handle's stream; graph_ms gives the last
