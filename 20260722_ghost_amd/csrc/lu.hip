@@ -38,7 +38,8 @@ struct TrsmOp {      // diagonal block T (w x w) of a factor and the slot of its
 // complex GEMM update on 16x16x4 MFMA: 64 x 64 tile per 256-thread block
 // ─────────────────────────────────────────────────────────────────────────────
 
-template <typename T>
+// TA / TB: the operand is stored transposed (A as K x M with lda, B as N x K with ldb).
+template <typename T, bool TA = false, bool TB = false>
 __global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, const void* Abase,
                                                      const void* Bbase, void* Cbase) {
   typedef typename CT<T>::c C;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, const vo
         const int gk = k0 + kq + q;
         C v;
         v.x = 0; v.y = 0;
-        if (gr < op.M && gk < op.K) v = A[(size_t)gr * op.lda + gk];
+        if (gr < op.M && gk < op.K) v = TA ? A[(size_t)gk * op.lda + gr] : A[(size_t)gr * op.lda + gk];
         Ar[kq + q][row] = v.x;
         Ai[kq + q][row] = v.y;
       }
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, const vo
         const int gc = n0 + cq + q;
         C v;
         v.x = 0; v.y = 0;
-        if (gk < op.K && gc < op.N) v = B[(size_t)gk * op.ldb + gc];
+        if (gk < op.K && gc < op.N) v = TB ? B[(size_t)gc * op.ldb + gk] : B[(size_t)gk * op.ldb + gc];
         Br[kr][cq + q] = v.x;
         Bi[kr][cq + q] = v.y;
       }
