@@ -42,6 +42,9 @@ SIGNATURES = [
     ("ghost_bor_fill", _ci, [_vp]),
     ("ghost_bor_solve", _ci, [_vp, _ci]),
     ("ghost_bor_farfield", _ci, [_vp]),
+    ("ghost_bor_set_gemm_variant", _ci, [_vp, _ci]),
+    ("ghost_bor_set_probe_tol", _ci, [_vp, _cd]),
+    ("ghost_bor_refine_history", _ci, [_vp, _dp, _ip]),
     ("ghost_bor_run", _ci, [_vp, _ci]),
     ("ghost_bor_sync", _ci, [_vp]),
     ("ghost_device_synchronize", _ci, []),
@@ -156,6 +159,18 @@ class Handle:
 
     def solve(self, refine_iters: int = 0):
         check(self.lib.ghost_bor_solve(self._h, int(refine_iters)))
+
+    def set_gemm_variant(self, v: int):
+        check(self.lib.ghost_bor_set_gemm_variant(self._h, int(v)))
+
+    def set_probe_tol(self, tol: float):
+        check(self.lib.ghost_bor_set_probe_tol(self._h, float(tol)))
+
+    def refine_history(self):
+        out = np.zeros(64)
+        n = ctypes.c_int(64)
+        check(self.lib.ghost_bor_refine_history(self._h, dptr(out), ctypes.byref(n)))
+        return out[:min(n.value, 64)].tolist()
 
     def farfield(self):
         check(self.lib.ghost_bor_farfield(self._h))

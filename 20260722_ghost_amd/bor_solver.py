@@ -10,8 +10,9 @@ contributions in the same order, so ``modes_used`` and the summed amplitudes
 follow the reference's rule.
 
 Extra keyword arguments (not in the reference): ``precision`` ("fp64" or
-"mixed"), ``refine_iters`` (mixed precision FP64 refinement sweeps),
-``residual`` (compute linear_residual, default True).
+"mixed"), ``refine_iters`` (mixed precision FP64 refinement sweeps; negative
+= adaptive, up to |refine_iters| sweeps while a probe residual exceeds
+1e-6), ``residual`` (compute linear_residual, default True).
 """
 
 from __future__ import annotations
@@ -147,7 +148,8 @@ def _coefficients(form: str, cfie_alpha: float):
 # batched sweeps
 # ─────────────────────────────────────────────────────────────────────────────
 
-def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, prec, refine_iters, times):
+def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, prec, refine_iters, times,
+            probe=None):
     F, used = truncate_modes(dF, m_max, mode_tol)
     out = {
         "theta_deg": thetas.tolist(),
@@ -165,6 +167,7 @@ def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, pre
         "warnings": [],
         "precision": prec,
         "refine_iters": refine_iters if prec == "mixed" else 0,
+        "probe_residuals": probe,
         "n_xi": (ne, nbk),
         "m_max": m_max,
         "timings_ms": times,
@@ -178,7 +181,7 @@ def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, pre
     return out
 
 
-def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_iters: int = 2,
+def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_iters: int = -3,
                     residual: bool = True, nb: int = 64, mode_tol: float = 1e-6,
                     graph: bool = False, check_abort: Optional[Callable] = None) -> List[Dict]:
     """Solve several (generatrix, frequency) problems as ONE batch: every mode
@@ -212,11 +215,12 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
         h.build()
         it = refine_iters if prec == "mixed" else 0
         if graph:
-            h.capture(it)
+            h.capture(abs(it))
             h.launch()
         else:
             h.run(it)
         dFs = [h.dF(i) for i in range(len(specs))]
+        probe = h.refine_history() if prec == "mixed" else []
         res = h.residuals() if residual else None
         times = h.timings() if not graph else {"graph_ms": h.graph_ms()}
     finally:
@@ -229,7 +233,7 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
         r = res[off:off + nm] if res is not None else None
         off += nm
         outs.append(_result(pts, thetas, form, m_max, ne, nbk, streaming, dFs[i], r, mode_tol, prec,
-                            refine_iters, times))
+                            refine_iters, times, probe))
     return outs
 
 
@@ -244,7 +248,7 @@ def solve_bor(points, freq_hz: float, thetas_deg, formulation: str = "efie",
               check_abort: Optional[Callable] = None,
               table_precision: str = "auto", assembly: str = "auto",
               stream_budget_gb: float = 8.0, *, precision: str = "fp64",
-              refine_iters: int = 2, residual: bool = True, nb: int = 64) -> Dict:
+              refine_iters: int = -3, residual: bool = True, nb: int = 64) -> Dict:
     """Monostatic RCS of a PEC body of revolution at aspects thetas_deg (from +z).
 
     Mirrors bor_solver.solve_bor (bor_solver.py:820-1004) for zs = None.

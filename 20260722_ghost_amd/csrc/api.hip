@@ -187,6 +187,7 @@ struct Launch {  // one batched kernel launch of the solve schedule
   int gx;        // grid.x
   int arg;       // extra (RPT for panels)
   double flops = 0.0;  // algorithmic real flops of the launch
+  int mM = 0, mN = 0;  // largest GEMM M and N over the launch's systems
 };
 
 // kernel families timed by the optional per-launch HIP-event profile
@@ -244,13 +245,23 @@ struct ghost_bor {
   int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
   int vec_all = 0, vec_aug = 0;          // VecOps over all systems: x-blocks / augmented RHS -> x-blocks
   int rpt = 1;
+  // probe residual (adaptive refinement): two fixed random combinations of the columns
+  int probe_ops = -1;                // 3 GemmOps per system: BS = B S, XS = X S, RP = BS - A XS
+  double2* d_S = nullptr;            // [nrhs_max][2]
+  double2* d_BS = nullptr;           // per system n x 2 (offset p_off = sum n * 2)
+  double2* d_XS = nullptr;
+  std::vector<long long> p_off;
+  std::vector<double> probe_hist;    // max over systems of ||RP|| / ||BS|| before each sweep
   std::vector<double> res_norm;      // per system ||AX - B||_F / ||B||_F
   std::vector<double> refine_hist;   // max over systems of max|R|/max|B| per iteration
   hipEvent_t ev[8];
+  unsigned ev_recorded = 0;          // bit i: phase i (fill, solve, far field) has recorded its event pair
   float t_ms[8] = {0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int refine_iters = 0;
+  double probe_tol = 1e-6;
+  int gemm_variant = 1;              // see gemm_launch (1 measured fastest for n ~ 2400, both precisions)
   std::vector<DevBuf> allocs;
   // optional per-launch profile
   bool profile = false;
@@ -569,6 +580,11 @@ static double gemm_flops(const std::vector<GemmOp>& ops, int first, int n, bool 
 
 static void set_flops(ghost_bor* h, std::vector<Launch>& sched) {
   for (Launch& L : sched) {
+    if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W || L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU)
+      for (int i = L.ops; i < L.ops + L.nops; ++i) {
+        L.mM = std::max(L.mM, h->h_gemm[i].M);
+        L.mN = std::max(L.mN, h->h_gemm[i].N);
+      }
     if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W) L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, true);
     else if (L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU) L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, false);
   }
@@ -688,6 +704,23 @@ static int build_schedule(ghost_bor* h) {
     h->h_gemm.push_back(GemmOp{s.a64_off, s.a_off + s.n, s.x_off, s.n, s.ld, s.nrhs, s.n, s.nrhs, s.n});
   }
   if (mixed) {
+    // probe: BS = B64 S (K = nrhs), XS = X64 S, then BS -= A64 XS (K = n)
+    h->probe_ops = (int)h->h_gemm.size();
+    long long po = 0;
+    h->p_off.assign(nsys, 0);
+    for (int si = 0; si < nsys; ++si) { h->p_off[si] = po; po += 2LL * h->sys[si].n; }
+    for (int si = 0; si < nsys; ++si) {   // C = 0 - B (-S): written into BS as +B S (see ghost_bor_probe)
+      const SysInfo& s = h->sys[si];
+      h->h_gemm.push_back(GemmOp{s.x_off, 0, h->p_off[si], s.nrhs, 2, 2, s.n, 2, s.nrhs});
+    }
+    for (int si = 0; si < nsys; ++si) {
+      const SysInfo& s = h->sys[si];
+      h->h_gemm.push_back(GemmOp{s.x_off, 0, h->p_off[si], s.nrhs, 2, 2, s.n, 2, s.nrhs});
+    }
+    for (int si = 0; si < nsys; ++si) {
+      const SysInfo& s = h->sys[si];
+      h->h_gemm.push_back(GemmOp{s.a64_off, h->p_off[si], h->p_off[si], s.n, 2, 2, s.n, 2, s.n});
+    }
     // forward / backward solves on a separate FP32 RHS block Y (n x nrhs)
     for (int b = 0; b < nblk_max; ++b) {
       const int k = b * nb;
@@ -801,6 +834,20 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
   if (mixed) {
     if ((rc = dalloc(h, &h->d_X64, h->x_elems * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_Y32, h->x_elems * sizeof(float2)))) return rc;
+    size_t pe = 0, nrmax = 0;
+    for (auto& s : h->sys) { pe += 2 * (size_t)s.n; nrmax = std::max(nrmax, (size_t)s.nrhs); }
+    if ((rc = dalloc(h, &h->d_S, nrmax * 2 * sizeof(double2)))) return rc;
+    if ((rc = dalloc(h, &h->d_BS, pe * sizeof(double2)))) return rc;
+    if ((rc = dalloc(h, &h->d_XS, pe * sizeof(double2)))) return rc;
+    // S: fixed pseudo-random unit-modulus entries, -1 * S so that k_gemm_update's C -= A B adds B S
+    std::vector<double2> S(nrmax * 2);
+    unsigned long long st = 0x9E3779B97F4A7C15ULL;
+    for (auto& z : S) {
+      st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+      const double ph = 2.0 * M_PI * (double)(st >> 11) / 9007199254740992.0;
+      z = make_double2(-std::cos(ph), -std::sin(ph));
+    }
+    HIPCHK(hipMemcpy(h->d_S, S.data(), S.size() * sizeof(double2), hipMemcpyHostToDevice));
   }
   if ((rc = dalloc(h, &h->d_piv, (size_t)h->piv_elems * sizeof(int)))) return rc;
   if ((rc = dalloc(h, &h->d_perm, (size_t)h->piv_elems * sizeof(int)))) return rc;
@@ -848,6 +895,7 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p) {
   fa.cm = -2.0 * M_PI * p.cM;
   fa.Z = p.d_zfar;
   const int NT = (p.M + 2 + 15) / 16;
+  if (p.nE > 4096 || p.nB > 4096) return fail(GHOST_ERR_ARG, "azimuth grid above 4096 samples");
   for (int pe = 0; pe < 2; ++pe)
     for (int pf = 0; pf < 2; ++pf) {
       const int nep = (p.Ne - pe + 1) / 2, nfp = (p.Ne - pf + 1) / 2;
@@ -857,9 +905,10 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p) {
       fa.nfp = nfp;
       fa.npairs = nep * nfp;
       const int grid = (fa.npairs + 1) / 2;
+      const size_t tab_bytes = sizeof(R) * (size_t)(p.nE / 2 + 1 + p.nE + 2 * (p.nB / 2 + 1) + 2 * p.nB);
       prof_begin(h);
       switch (NT) {
-#define FARCASE(n) case n: hipLaunchKernelGGL((k_far_fill<R, n>), dim3(grid), dim3(128), 0, h->stream, fa); break;
+#define FARCASE(n) case n: hipLaunchKernelGGL((k_far_fill<R, n>), dim3(grid), dim3(128), tab_bytes, h->stream, fa); break;
         FARCASE(1) FARCASE(2) FARCASE(3) FARCASE(4) FARCASE(5) FARCASE(6) FARCASE(7) FARCASE(8)
 #undef FARCASE
         default: return fail(GHOST_ERR_ARG, "m_max too large for the far fill");
@@ -992,6 +1041,23 @@ static int run_fill(ghost_bor* h) {
   return 0;
 }
 
+// Batched complex GEMM C -= A B with the handle's tile variant:
+// 0: 64 x 64 single-buffered; 1: 64 x 64, 2: 64 x 128, 3: 128 x 128 (FP32) /
+// 64 x 128 (FP64), double-buffered.  gx = 0 computes the grid from mM, mN.
+template <typename T> static void gemm_launch(ghost_bor* h, const GemmOp* ops, int nops, int mM, int mN,
+                                              const void* A, const void* B, void* C) {
+  if (mM <= 0 || mN <= 0) return;
+  int v = h->gemm_variant;
+  if (v == 3 && sizeof(T) == 8) v = 2;
+  auto grid = [&](int bm, int bn) { return dim3((unsigned)(((mM + bm - 1) / bm) * ((mN + bn - 1) / bn)), nops); };
+  switch (v) {
+    case 0: hipLaunchKernelGGL((k_gemm_update<T>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
+    case 1: hipLaunchKernelGGL((k_gemm2<T, 2, 2>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
+    case 2: hipLaunchKernelGGL((k_gemm2<T, 2, 4>), grid(64, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
+    default: hipLaunchKernelGGL((k_gemm2<T, 4, 4>), grid(128, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
+  }
+}
+
 template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launch>& sched, void* factor,
                                            void* rhs) {
   for (const Launch& L : sched) {
@@ -1013,16 +1079,16 @@ template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launc
         hipLaunchKernelGGL((k_tri_inv<T, false>), dim3(L.nops), dim3(64), 0, h->stream, h->d_trsm + L.ops, factor, h->d_W);
         break;
       case L_GEMM_AUG_W:
-        if (L.gx > 0) hipLaunchKernelGGL((k_gemm_update<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_gemm + L.ops, h->d_W, factor, factor);
+        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, h->d_W, factor, factor);
         break;
       case L_GEMM_AUG:
-        if (L.gx > 0) hipLaunchKernelGGL((k_gemm_update<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_gemm + L.ops, factor, factor, factor);
+        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, factor, factor, factor);
         break;
       case L_GEMM_Y_W:
-        if (L.gx > 0) hipLaunchKernelGGL((k_gemm_update<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_gemm + L.ops, h->d_W, rhs, rhs);
+        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, h->d_W, rhs, rhs);
         break;
       case L_GEMM_Y_LU:
-        if (L.gx > 0) hipLaunchKernelGGL((k_gemm_update<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_gemm + L.ops, factor, rhs, rhs);
+        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, factor, rhs, rhs);
         break;
       default: break;
     }
@@ -1067,12 +1133,13 @@ static int residual_gemm(ghost_bor* h, int x_in_aug) {
   hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_B64, h->d_R64);
   LAUNCHCHK();
   prof_begin(h);
+  int mM = 0, mN = 0;
+  for (auto& s : h->sys) { mM = std::max(mM, s.n); mN = std::max(mN, s.nrhs); }
+  (void)gx;
   if (x_in_aug)
-    hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
-                       h->d_gemm + h->res_gemm_aug, h->d_A64, h->d_A, h->d_R64);
+    gemm_launch<double>(h, h->d_gemm + h->res_gemm_aug, nsys, mM, mN, h->d_A64, h->d_A, h->d_R64);
   else
-    hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
-                       h->d_gemm + h->res_gemm, h->d_A64, h->d_X64, h->d_R64);
+    gemm_launch<double>(h, h->d_gemm + h->res_gemm, nsys, mM, mN, h->d_A64, h->d_X64, h->d_R64);
   prof_end(h, K_GEMM64, gemm_flops(h->h_gemm, x_in_aug ? h->res_gemm_aug : h->res_gemm, nsys, false));
   LAUNCHCHK();
   return 0;
@@ -1084,7 +1151,10 @@ extern "C" int ghost_bor_fill(ghost_bor* h) {
   if (!h->capturing) HIPCHK(hipEventRecord(h->ev[0], h->stream));
   int rc = run_fill(h);
   if (rc) return rc;
-  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[1], h->stream));
+  if (!h->capturing) {
+    HIPCHK(hipEventRecord(h->ev[1], h->stream));
+    h->ev_recorded |= 1u;
+  }
   return 0;
 }
 
@@ -1103,6 +1173,50 @@ static int frob2(ghost_bor* h, const double2* buf, std::vector<double>& out) {
       acc += z.x * z.x + z.y * z.y;
     }
     out[si] = acc;
+  }
+  return 0;
+}
+
+// eta[s] = ||B S - A64 X S|| / ||B S|| for the current X64 (mixed precision only).
+static int probe_residual(ghost_bor* h, std::vector<double>& eta) {
+  const int nsys = (int)h->sys.size();
+  size_t pe = 0;
+  for (auto& s : h->sys) pe += 2 * (size_t)s.n;
+  int gx1 = 0, gx2 = 0;
+  for (auto& s : h->sys) { gx1 = std::max(gx1, (s.n + 63) / 64); gx2 = gx1; }
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (pe + 255) / 256)), dim3(256), 0, h->stream,
+                     h->d_BS, pe);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (pe + 255) / 256)), dim3(256), 0, h->stream,
+                     h->d_XS, pe);
+  LAUNCHCHK();
+  prof_begin(h);
+  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx1, nsys), dim3(256), 0, h->stream,
+                     h->d_gemm + h->probe_ops, h->d_B64, h->d_S, h->d_BS);
+  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx1, nsys), dim3(256), 0, h->stream,
+                     h->d_gemm + h->probe_ops + nsys, h->d_X64, h->d_S, h->d_XS);
+  // RP = BS - A64 XS, in place in BS
+  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx2, nsys), dim3(256), 0, h->stream,
+                     h->d_gemm + h->probe_ops + 2 * nsys, h->d_A64, h->d_XS, h->d_BS);
+  prof_end(h, K_GEMM64, 0.0);
+  LAUNCHCHK();
+  // ||B S|| needs BS before the update: recompute it into XS (XS is free now)
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (pe + 255) / 256)), dim3(256), 0, h->stream,
+                     h->d_XS, pe);
+  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx1, nsys), dim3(256), 0, h->stream,
+                     h->d_gemm + h->probe_ops, h->d_B64, h->d_S, h->d_XS);
+  LAUNCHCHK();
+  std::vector<double2> rp(pe), bs(pe);
+  HIPCHK(hipMemcpyAsync(rp.data(), h->d_BS, pe * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(bs.data(), h->d_XS, pe * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  eta.assign(nsys, 0.0);
+  for (int si = 0; si < nsys; ++si) {
+    double r2 = 0, b2 = 0;
+    for (long long i = h->p_off[si]; i < h->p_off[si] + 2LL * h->sys[si].n; ++i) {
+      r2 += rp[i].x * rp[i].x + rp[i].y * rp[i].y;
+      b2 += bs[i].x * bs[i].x + bs[i].y * bs[i].y;
+    }
+    eta[si] = b2 > 0 ? std::sqrt(r2 / b2) : 0.0;
   }
   return 0;
 }
@@ -1128,7 +1242,20 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
     hipLaunchKernelGGL(k_perm_from_pivots, dim3((nsys + 63) / 64), dim3(64), 0, h->stream, h->d_vec + h->vec_all,
                        nsys, h->d_piv, h->d_perm);
     LAUNCHCHK();
-    for (int it = 0; it < refine_iters; ++it) {
+    // refine_iters >= 0: that many FP64-residual sweeps; < 0: adaptive, up to
+    // -refine_iters sweeps while the probe residual exceeds h->probe_tol
+    const bool adaptive = refine_iters < 0;
+    const int max_it = adaptive ? -refine_iters : refine_iters;
+    h->probe_hist.clear();
+    if (adaptive && h->capturing) return fail(GHOST_ERR_STATE, "adaptive refinement cannot be captured");
+    for (int it = 0; it < max_it; ++it) {
+      if (adaptive) {
+        std::vector<double> eta;
+        if ((rc = probe_residual(h, eta))) return rc;
+        const double mx = *std::max_element(eta.begin(), eta.end());
+        h->probe_hist.push_back(mx);
+        if (mx <= h->probe_tol) break;
+      }
       if ((rc = residual_gemm(h, 0))) return rc;      // R = B - A X in FP64
       hipLaunchKernelGGL(k_permute_cast, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_perm, h->d_R64, h->d_Y32);
       LAUNCHCHK();
@@ -1138,8 +1265,33 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
       LAUNCHCHK();
     }
   }
-  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[3], h->stream));
+  if (!h->capturing) {
+    HIPCHK(hipEventRecord(h->ev[3], h->stream));
+    h->ev_recorded |= 2u;
+  }
   h->refine_iters = refine_iters;
+  return 0;
+}
+
+extern "C" int ghost_bor_set_gemm_variant(ghost_bor* h, int variant) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (variant < 0 || variant > 3) return fail(GHOST_ERR_ARG, "GEMM variant must be 0..3");
+  h->gemm_variant = variant;
+  return 0;
+}
+
+extern "C" int ghost_bor_set_probe_tol(ghost_bor* h, double tol) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  h->probe_tol = tol;
+  return 0;
+}
+
+// Probe residuals measured by the last adaptive solve (one per probe), *n in/out.
+extern "C" int ghost_bor_refine_history(ghost_bor* h, double* out, int* n) {
+  if (!h || !n) return fail(GHOST_ERR_ARG, "null argument");
+  const int k = std::min(*n, (int)h->probe_hist.size());
+  for (int i = 0; i < k; ++i) out[i] = h->probe_hist[i];
+  *n = (int)h->probe_hist.size();
   return 0;
 }
 
@@ -1181,7 +1333,10 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
     LAUNCHCHK();
   }
   (void)nsys;
-  if (!h->capturing) HIPCHK(hipEventRecord(h->ev[5], h->stream));
+  if (!h->capturing) {
+    HIPCHK(hipEventRecord(h->ev[5], h->stream));
+    h->ev_recorded |= 4u;
+  }
   return 0;
 }
 
@@ -1297,9 +1452,11 @@ extern "C" int ghost_bor_sync(ghost_bor* h) {
 extern "C" int ghost_bor_timings(ghost_bor* h, float* ms3) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
   HIPCHK(hipStreamSynchronize(h->stream));
-  HIPCHK(hipEventElapsedTime(&ms3[0], h->ev[0], h->ev[1]));
-  HIPCHK(hipEventElapsedTime(&ms3[1], h->ev[2], h->ev[3]));
-  HIPCHK(hipEventElapsedTime(&ms3[2], h->ev[4], h->ev[5]));
+  // a phase that has not run since build reports -1
+  for (int i = 0; i < 3; ++i) {
+    ms3[i] = -1.0f;
+    if (h->ev_recorded & (1u << i)) HIPCHK(hipEventElapsedTime(&ms3[i], h->ev[2 * i], h->ev[2 * i + 1]));
+  }
   return 0;
 }
 

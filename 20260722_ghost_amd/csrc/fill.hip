@@ -40,6 +40,16 @@ struct FarArgs {
 
 // Sum 32 per-lane values over the 16 lanes of a DPP row and scatter the
 // result: lane r (= lane & 15) receives the sums of values r and 16 + r.
+// Butterfly reduce-scatter whose stage partners are DPP involutions that flip
+// one lane bit: row_mirror (bit 3), row_half_mirror (bit 2), quad_perm
+// [3,2,1,0] (bit 1) and [1,0,3,2] (bit 0).  At each stage a lane keeps the
+// half of its values whose index bit equals its own lane bit.
+template <int CTRL, typename R>
+__device__ __forceinline__ R rs_step(R lo, R hi, bool b) {
+  const R keep = b ? hi : lo, send = b ? lo : hi;
+  return keep + dpp_mov<CTRL>(send);
+}
+
 template <typename R>
 __device__ __forceinline__ void row_reduce_scatter32(const R (&v)[32], int lane, R& out_lo, R& out_hi) {
   R a16[16];
@@ -48,9 +58,7 @@ __device__ __forceinline__ void row_reduce_scatter32(const R (&v)[32], int lane,
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
       if (i & 8) continue;
-      const R lo = v[i], hi = v[i | 8];
-      const R keep = b ? hi : lo, send = b ? lo : hi;
-      a16[(i & 7) | ((i >> 4) << 3)] = keep + __shfl_xor(send, 8);
+      a16[(i & 7) | ((i >> 4) << 3)] = rs_step<0x140>(v[i], v[i | 8], b);
     }
   }
   R a8[8];
@@ -59,9 +67,7 @@ __device__ __forceinline__ void row_reduce_scatter32(const R (&v)[32], int lane,
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       if (i & 4) continue;
-      const R lo = a16[i], hi = a16[i | 4];
-      const R keep = b ? hi : lo, send = b ? lo : hi;
-      a8[(i & 3) | ((i >> 3) << 2)] = keep + __shfl_xor(send, 4);
+      a8[(i & 3) | ((i >> 3) << 2)] = rs_step<0x141>(a16[i], a16[i | 4], b);
     }
   }
   R a4[4];
@@ -70,20 +76,12 @@ __device__ __forceinline__ void row_reduce_scatter32(const R (&v)[32], int lane,
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       if (i & 2) continue;
-      const R lo = a8[i], hi = a8[i | 2];
-      const R keep = b ? hi : lo, send = b ? lo : hi;
-      a4[(i & 1) | ((i >> 2) << 1)] = keep + __shfl_xor(send, 2);
+      a4[(i & 1) | ((i >> 2) << 1)] = rs_step<0x1B>(a8[i], a8[i | 2], b);
     }
   }
   const bool b = lane & 1;
-  {
-    const R keep = b ? a4[1] : a4[0], send = b ? a4[0] : a4[1];
-    out_lo = keep + __shfl_xor(send, 1);
-  }
-  {
-    const R keep = b ? a4[3] : a4[2], send = b ? a4[2] : a4[3];
-    out_hi = keep + __shfl_xor(send, 1);
-  }
+  out_lo = rs_step<0xB1>(a4[0], a4[1], b);
+  out_hi = rs_step<0xB1>(a4[2], a4[3], b);
 }
 
 // One wavefront per far element pair (e, f); 2 wavefronts per block.
@@ -95,6 +93,21 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
   typedef typename CT<R>::c C2;
   typedef typename CT<R>::v4 V4;
   __shared__ R stash[2][NT][8][64];
+  extern __shared__ char dyn_smem[];
+  // azimuth tables in LDS, in the working precision:
+  // sin2E [nE/2+1] | cosE [nE] | cxB [nB/2+1] | sxB [nB/2+1] | cosB [nB] | sinB [nB]
+  R* tab = (R*)dyn_smem;
+  R* t_sin2E = tab;
+  R* t_cosE = t_sin2E + (a.nE / 2 + 1);
+  R* t_cxB = t_cosE + a.nE;
+  R* t_sxB = t_cxB + (a.nB / 2 + 1);
+  R* t_cosB = t_sxB + (a.nB / 2 + 1);
+  R* t_sinB = t_cosB + a.nB;
+  for (int i = threadIdx.x; i <= a.nE / 2; i += blockDim.x) t_sin2E[i] = (R)a.sin2E[i];
+  for (int i = threadIdx.x; i < a.nE; i += blockDim.x) t_cosE[i] = (R)a.cosE[i];
+  for (int i = threadIdx.x; i <= a.nB / 2; i += blockDim.x) { t_cxB[i] = (R)a.cxB[i]; t_sxB[i] = (R)a.sxB[i]; }
+  for (int i = threadIdx.x; i < a.nB; i += blockDim.x) { t_cosB[i] = (R)a.cosB[i]; t_sinB[i] = (R)a.sinB[i]; }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const int wid = blockIdx.x * 2 + wl;
   if (wid >= a.npairs) return;  // wave-uniform exit (no block barriers below)
@@ -128,7 +141,7 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
         const int j = j0 + kk;
         const int jc = j < hE ? j : hE;
         const R cw = (j == 0 || j == hE) ? (R)1 : (j > hE ? (R)0 : (R)2);
-        R Rd = sqrt(d2 + rr4 * (R)a.sin2E[jc]);
+        R Rd = sqrt(d2 + rr4 * t_sin2E[jc]);
         Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
         R s, c;
         sincos_r(k * Rd, &s, &c);
@@ -136,7 +149,7 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
         const R ar = amp * c, ai = -amp * s;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          const R bv = (R)a.cosE[((16 * t + c16) * j) & mskE];
+          const R bv = t_cosE[((16 * t + c16) * j) & mskE];
           accR[t] = mfma4(ar, bv, accR[t]);
           accI[t] = mfma4(ai, bv, accI[t]);
         }
@@ -244,7 +257,7 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
       const int j = j0 + kk;
       const int jc = j < hB ? j : hB;
       const R cw = (j == 0 || j == hB) ? (R)1 : (j > hB ? (R)0 : (R)2);
-      const R cx = (R)a.cxB[jc], sx = (R)a.sxB[jc];
+      const R cx = t_cxB[jc], sx = t_sxB[jc];
       const R Rx = rp - rq * cx, Ry = rq * sx;
       R Rd = sqrt(Rx * Rx + Ry * Ry + dz * dz);
       Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
@@ -281,8 +294,8 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const int idx = ((16 * t + c16) * j) & mskB;
-        accE[t] = mfma4(ve, (R)a.cosB[idx], accE[t]);
-        accO[t] = mfma4(vo, (R)a.sinB[idx], accO[t]);
+        accE[t] = mfma4(ve, t_cosB[idx], accE[t]);
+        accO[t] = mfma4(vo, t_sinB[idx], accO[t]);
       }
     }
   }

@@ -142,6 +142,133 @@ __global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, const vo
       }
 }
 
+// Block tile (32 WM) x (32 WN) complex, 2 x 2 wavefronts of WM x WN MFMA
+// tiles each; K in chunks of 16 staged through two LDS buffers with the next
+// chunk's global loads in flight while the current one feeds the MFMAs.
+template <typename T, int WM, int WN, bool TA = false, bool TB = false>
+__global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, const void* Abase, const void* Bbase,
+                                               void* Cbase) {
+  typedef typename CT<T>::c C;
+  typedef typename CT<T>::v4 V4;
+  constexpr int BM = 32 * WM, BN = 32 * WN, KC = 16;
+  constexpr int AP = BM * KC / 256, BP = BN * KC / 256;   // complex elements per thread
+  const GemmOp op = ops[blockIdx.y];
+  if (op.M <= 0 || op.N <= 0 || op.K <= 0) return;
+  const int tn = (op.N + BN - 1) / BN;
+  const int tiles = ((op.M + BM - 1) / BM) * tn;
+  if ((int)blockIdx.x >= tiles) return;
+  const int m0 = (blockIdx.x / tn) * BM, n0 = (blockIdx.x % tn) * BN;
+  const C* A = (const C*)Abase + op.a;
+  const C* B = (const C*)Bbase + op.b;
+  C* Cm = (C*)Cbase + op.c;
+
+  __shared__ T As[2][2][KC][BM + 1];   // [buffer][re, im][k][row]
+  __shared__ T Bs[2][2][KC][BN + 1];   // [buffer][re, im][k][col]
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wy = wv >> 1, wx = wv & 1;
+  const int c16 = lane & 15, kk = lane >> 4;
+
+  // global -> register assignment
+  //   A (row-major): thread = (row, AP consecutive k); TA: (k, AP consecutive rows)
+  //   B (row-major): thread = (k, BP consecutive cols); TB: (col, BP consecutive k)
+  const int a_r = TA ? (tid % (BM / AP)) * AP : tid / (KC / AP);
+  const int a_k = TA ? tid / (BM / AP) : (tid % (KC / AP)) * AP;
+  const int b_k = TB ? (tid % (KC / BP)) * BP : tid / (BN / BP);
+  const int b_c = TB ? tid / (KC / BP) : (tid % (BN / BP)) * BP;
+  C ra[AP], rb[BP];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < AP; ++q) {
+      const int gr = m0 + (TA ? a_r + q : a_r), gk = k0 + (TA ? a_k : a_k + q);
+      C v;
+      v.x = 0; v.y = 0;
+      if (gr < op.M && gk < op.K) v = TA ? A[(size_t)gk * op.lda + gr] : A[(size_t)gr * op.lda + gk];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < BP; ++q) {
+      const int gk = k0 + (TB ? b_k + q : b_k), gc = n0 + (TB ? b_c : b_c + q);
+      C v;
+      v.x = 0; v.y = 0;
+      if (gk < op.K && gc < op.N) v = TB ? B[(size_t)gc * op.ldb + gk] : B[(size_t)gk * op.ldb + gc];
+      rb[q] = v;
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < AP; ++q) {
+      const int r = TA ? a_r + q : a_r, k = TA ? a_k : a_k + q;
+      As[buf][0][k][r] = ra[q].x;
+      As[buf][1][k][r] = ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < BP; ++q) {
+      const int k = TB ? b_k + q : b_k, c = TB ? b_c : b_c + q;
+      Bs[buf][0][k][c] = rb[q].x;
+      Bs[buf][1][k][c] = rb[q].y;
+    }
+  };
+
+  V4 accR[WM][WN], accI[WM][WN];
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j) { accR[i][j] = V4{0, 0, 0, 0}; accI[i][j] = V4{0, 0, 0, 0}; }
+
+  const int nk = (op.K + KC - 1) / KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nk) gload((kc + 1) * KC);
+#pragma unroll
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int kx = 4 * ks + kk;
+      T ar[WM], ai[WM], nai[WM], br[WN], bi[WN];
+#pragma unroll
+      for (int i = 0; i < WM; ++i) {
+        ar[i] = As[buf][0][kx][wy * 16 * WM + i * 16 + c16];
+        ai[i] = As[buf][1][kx][wy * 16 * WM + i * 16 + c16];
+        nai[i] = -ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < WN; ++j) {
+        br[j] = Bs[buf][0][kx][wx * 16 * WN + j * 16 + c16];
+        bi[j] = Bs[buf][1][kx][wx * 16 * WN + j * 16 + c16];
+      }
+#pragma unroll
+      for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+          accR[i][j] = mfma4(ar[i], br[j], accR[i][j]);
+          accR[i][j] = mfma4(nai[i], bi[j], accR[i][j]);
+          accI[i][j] = mfma4(ar[i], bi[j], accI[i][j]);
+          accI[i][j] = mfma4(ai[i], br[j], accI[i][j]);
+        }
+    }
+    if (kc + 1 < nk) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < WM; ++i)
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = m0 + wy * 16 * WM + i * 16 + mrow<T>(kk, r);
+        const int gc = n0 + wx * 16 * WN + j * 16 + c16;
+        if (gr < op.M && gc < op.N) {
+          C* p = Cm + (size_t)gr * op.ldc + gc;
+          C v = *p;
+          v.x -= accR[i][j][r];
+          v.y -= accI[i][j][r];
+          *p = v;
+        }
+      }
+}
+
 // ─────────────────────────────────────────────────────────────────────────────
 // panel factorisation: one 1024-thread block per system, SW-column sub-panels
 // held in registers (rows r0 + tid + 1024 i), pivots by max |Re| + |Im|

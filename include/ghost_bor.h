@@ -91,9 +91,20 @@ int ghost_bor_build(ghost_bor* h);
 
 /* Fill: near band, far blocks, masked mode systems and RHS columns. */
 int ghost_bor_fill(ghost_bor* h);
-/* Batched LU + solve of every mode system; GHOST_MIXED adds refine_iters
- * FP64-residual refinement sweeps. */
+/* Batched LU + solve of every mode system.  GHOST_MIXED: refine_iters >= 0
+ * runs that many FP64-residual refinement sweeps; refine_iters < 0 is
+ * adaptive: before each of up to -refine_iters sweeps a probe residual
+ * ||B s - A X s|| / ||B s|| (two fixed random column combinations s, FP64)
+ * is measured and the sweeps stop once it is <= the probe tolerance. */
 int ghost_bor_solve(ghost_bor* h, int refine_iters);
+/* Tile variant of the batched complex MFMA GEMM (trailing updates, solves,
+ * residuals): 0 = 64x64 single-buffered, 1 = 64x64, 2 = 64x128, 3 = 128x128
+ * (FP32; 64x128 for FP64) with double-buffered K chunks.  Default 1. */
+int ghost_bor_set_gemm_variant(ghost_bor* h, int variant);
+/* Probe tolerance of the adaptive refinement (default 1e-6). */
+int ghost_bor_set_probe_tol(ghost_bor* h, double tol);
+/* Probe residuals of the last adaptive solve; *n: capacity in, count out. */
+int ghost_bor_refine_history(ghost_bor* h, double* out, int* n);
 /* Monostatic far-field contribution of every mode and column. */
 int ghost_bor_farfield(ghost_bor* h);
 /* fill + solve + farfield. */
@@ -122,7 +133,8 @@ int ghost_bor_num_systems(ghost_bor* h, int prob, int* nmodes, int* m_max);
 int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, double* out1, double* out2);
 /* Zero-pivot count per system from the last factorisation. */
 int ghost_bor_info(ghost_bor* h, int* out);
-/* Milliseconds of the last fill, solve and far-field phases (HIP events on the handle's stream). */
+/* Milliseconds of the last fill, solve and far-field phases (HIP events on the
+ * handle's stream); -1 for a phase that has not run eagerly since build. */
 int ghost_bor_timings(ghost_bor* h, float* ms3);
 
 /* Optional per-launch profile: with on != 0 every later launch is bracketed by

@@ -28,7 +28,8 @@ def db(s):
     return 10.0 * np.log10(np.asarray(s, dtype=float))
 
 
-def make_handle(nodes, freq_hz, thetas, m_max, alpha, streaming, precision=0, unmasked=False, nb=64):
+def make_handle(nodes, freq_hz, thetas, m_max, alpha, streaming, precision=0, unmasked=False, nb=64,
+                variant=None):
     L = pkg("_lib")
     bs = pkg("bor_solver")
     pts = np.asarray(nodes, dtype=float)
@@ -37,6 +38,8 @@ def make_handle(nodes, freq_hz, thetas, m_max, alpha, streaming, precision=0, un
     _, ne, nbk, _ = bs.discretisation(pts, k, np.asarray(thetas), m_max, form,
                                       "streaming" if streaming else "tables")
     h = L.Handle(precision, nb=nb)
+    if variant is not None:
+        h.set_gemm_variant(variant)
     h.add_problem(pts, k, m_max, ne, nbk, alpha, (1 - alpha) * bs.ETA0, thetas, unmasked=unmasked)
     h.build()
     return h
@@ -83,6 +86,28 @@ def test_solve_bor_matches_reference(name, bs):
     assert out["linear_residual"] < 1e-13
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("nb", [16, 32, 64])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lu_tile_variants_and_block_sizes(variant, nb, precision):
+    """Every GEMM tile variant and LU block size gives the reference's currents:
+    FP64 to 1e-9; mixed (FP32 far fill and LU, two FP64 refinement sweeps)
+    to 1e-6, the FP32 rounding of the filled matrix times its conditioning."""
+    d = load("conesphere16_cfie")
+    mm = int(d["m_max"])
+    thetas = d["thetas"]
+    print(f"[case] variant={variant} nb={nb} precision={precision}", flush=True)
+    h = make_handle(d["nodes"], float(d["freq_hz"]), thetas, mm, float(d["alpha"]), False,
+                    precision=precision, nb=nb, variant=variant)
+    h.fill()
+    h.solve(2 if precision == 1 else 0)
+    h.sync()
+    for m in range(-mm, mm + 1):
+        X = h.solution(0, m)
+        assert rel(X, d[f"X_{m}"]) < (1e-6 if precision == 1 else 1e-9), (m, rel(X, d[f"X_{m}"]))
+    h.close()
+
+
 def test_unmasked_blocks_match_oracle():
     nodes = bo.sphere_nodes(0.5, 14)
     f, mm = 3.5e8, 4
@@ -118,6 +143,20 @@ def test_mixed_precision_within_0p01_db(bs):
     assert np.max(np.abs(db(a["sigma_vv"]) - db(b["sigma_vv"]))) < 0.01
     assert np.max(np.abs(db(a["sigma_hh"]) - db(b["sigma_hh"]))) < 0.01
     assert b["linear_residual"] < 1e-10
+
+
+def test_adaptive_refinement_stops_on_probe(bs):
+    nodes = bo.sphere_nodes(0.5, 40)
+    thetas = np.linspace(0, 180, 13)
+    a = bs.solve_bor(nodes, 3e8, thetas, formulation="cfie", n_modes=8, mode_tol=0.0)
+    b = bs.solve_bor(nodes, 3e8, thetas, formulation="cfie", n_modes=8, mode_tol=0.0,
+                     precision="mixed", refine_iters=-3)
+    hist = b["probe_residuals"]
+    assert 1 <= len(hist) <= 3
+    # the sweep count follows the probe: stop as soon as it is below 1e-6
+    assert all(h > 1e-6 for h in hist[:-1])
+    assert np.max(np.abs(db(a["sigma_vv"]) - db(b["sigma_vv"]))) < 0.01
+    assert np.max(np.abs(db(a["sigma_hh"]) - db(b["sigma_hh"]))) < 0.01
 
 
 def test_fill_is_deterministic():
