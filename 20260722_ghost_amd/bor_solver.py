@@ -11,8 +11,9 @@ follow the reference's rule.
 
 Extra keyword arguments (not in the reference): ``precision`` ("fp64" or
 "mixed"), ``refine_iters`` (mixed precision FP64 refinement sweeps; negative
-= adaptive, up to |refine_iters| sweeps while a probe residual exceeds
-1e-6), ``residual`` (compute linear_residual, default True).
+= adaptive, up to |refine_iters| sweeps while the estimated relative error
+of the currents exceeds 1e-5), ``residual`` (compute linear_residual,
+default True).
 """
 
 from __future__ import annotations
@@ -167,7 +168,7 @@ def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, pre
         "warnings": [],
         "precision": prec,
         "refine_iters": refine_iters if prec == "mixed" else 0,
-        "probe_residuals": probe,
+        "probe_errors": probe,
         "n_xi": (ne, nbk),
         "m_max": m_max,
         "timings_ms": times,

@@ -240,27 +240,33 @@ struct ghost_bor {
   TrsmOp* d_trsm = nullptr;
   GemmOp* d_gemm = nullptr;
   VecOp* d_vec = nullptr;
-  std::vector<Launch> lu_sched, back_sched, fwdY_sched, backY_sched;
+  std::vector<Launch> lu_sched, back_sched, fwdY_sched, backY_sched, fwdP_sched, backP_sched;
   int triu_ops = 0, triu_n = 0;
   int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
   int vec_all = 0, vec_aug = 0;          // VecOps over all systems: x-blocks / augmented RHS -> x-blocks
+  int vec_toaug = 0, vec_probe = 0;      // x-blocks -> augmented RHS; the n x 2 probe blocks
+  // mixed precision: power-of-two column scales of B (bscale) and of each
+  // refinement residual (rscale), per system at c_off; probe scales at 2 si
+  double* d_bscale = nullptr;
+  double* d_rscale = nullptr;
+  double* d_pscale = nullptr;
+  float2* d_P32 = nullptr;               // FP32 probe corrections, n x 2 per system at p_off
   int rpt = 1;
-  // probe residual (adaptive refinement): two fixed random combinations of the columns
+  // error probe (adaptive refinement): two fixed random combinations of the columns
   int probe_ops = -1;                // 3 GemmOps per system: BS = B S, XS = X S, RP = BS - A XS
   double2* d_S = nullptr;            // [nrhs_max][2]
   double2* d_BS = nullptr;           // per system n x 2 (offset p_off = sum n * 2)
   double2* d_XS = nullptr;
   std::vector<long long> p_off;
-  std::vector<double> probe_hist;    // max over systems of ||RP|| / ||BS|| before each sweep
+  std::vector<double> probe_hist;    // max over systems of ||A32^-1 RP|| / ||XS|| before each sweep
   std::vector<double> res_norm;      // per system ||AX - B||_F / ||B||_F
-  std::vector<double> refine_hist;   // max over systems of max|R|/max|B| per iteration
   hipEvent_t ev[8];
   unsigned ev_recorded = 0;          // bit i: phase i (fill, solve, far field) has recorded its event pair
   float t_ms[8] = {0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int refine_iters = 0;
-  double probe_tol = 1e-6;
+  double probe_tol = 1e-5;
   int gemm_variant = 1;              // see gemm_launch (1 measured fastest for n ~ 2400, both precisions)
   std::vector<DevBuf> allocs;
   // optional per-launch profile
@@ -603,6 +609,63 @@ static int build_schedule(ghost_bor* h) {
   auto wl = [&](int s, int b) { return ((long long)s * nblk_max + b) * 2 * 64 * 64; };
   auto wu = [&](int s, int b) { return ((long long)s * nblk_max + b) * 2 * 64 * 64 + 64 * 64; };
   auto tiles = [](int M, int N) { return (M > 0 && N > 0) ? ((M + 63) / 64) * ((N + 63) / 64) : 0; };
+  // blocked forward (unit-lower L, diagonal blocks through W = I - L11^-1) and
+  // backward (U, through the upper inverses) substitution on an RHS block of
+  // nc(si) columns at element offset off(si), row-major with ld = nc(si)
+  auto add_tri_solves = [&](std::vector<Launch>& fwd, std::vector<Launch>& back, auto off, auto nc) {
+    for (int b = 0; b < nblk_max; ++b) {
+      const int k = b * nb;
+      Launch g1{L_GEMM_Y_W, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        const long long o = off(si);
+        const int c = nc(si);
+        GemmOp go{wl(si, b), o + (long long)k * c, o + (long long)k * c, 64, c, c, w, c, w};
+        h->h_gemm.push_back(go);
+        g1.gx = std::max(g1.gx, tiles(go.M, go.N));
+      }
+      Launch g2{L_GEMM_Y_LU, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        const long long o = off(si);
+        const int c = nc(si);
+        GemmOp go{s.a_off + (long long)(k + w) * s.ld + k, o + (long long)k * c, o + (long long)(k + w) * c,
+                  s.ld, c, c, s.n - k - w, c, w};
+        if (w == 0) go.M = 0;
+        h->h_gemm.push_back(go);
+        g2.gx = std::max(g2.gx, tiles(go.M, go.N));
+      }
+      fwd.push_back(g1);
+      fwd.push_back(g2);
+    }
+    for (int st = 0; st < nblk_max; ++st) {
+      Launch g1{L_GEMM_Y_W, (int)h->h_gemm.size(), nsys, 0, 0};
+      std::vector<GemmOp> upd;
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int nbs = (s.n + nb - 1) / nb;
+        const int b = nbs - 1 - st;
+        const long long o = off(si);
+        const int c = nc(si);
+        GemmOp go{0, 0, 0, 64, c, c, 0, 0, 0};
+        GemmOp gu{0, 0, 0, s.ld, c, c, 0, 0, 0};
+        if (b >= 0) {
+          const int kb = b * nb, w = std::min(nb, s.n - kb);
+          go = GemmOp{wu(si, b), o + (long long)kb * c, o + (long long)kb * c, 64, c, c, w, c, w};
+          gu = GemmOp{s.a_off + kb, o + (long long)kb * c, o, s.ld, c, c, kb, c, w};
+        }
+        h->h_gemm.push_back(go);
+        g1.gx = std::max(g1.gx, tiles(go.M, go.N));
+        upd.push_back(gu);
+      }
+      Launch g2{L_GEMM_Y_LU, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (auto& gu : upd) { h->h_gemm.push_back(gu); g2.gx = std::max(g2.gx, tiles(gu.M, gu.N)); }
+      back.push_back(g1);
+      back.push_back(g2);
+    }
+  };
 
   // ── LU steps on the augmented systems ──
   for (int b = 0; b < nblk_max; ++b) {
@@ -682,15 +745,22 @@ static int build_schedule(ghost_bor* h) {
     h->back_sched.push_back(g2);
   }
   // ── VecOps over all systems (copies, casts, norms) ──
+  std::vector<int> c_off(nsys);
+  for (int si = 0, co = 0; si < nsys; ++si) { c_off[si] = co; co += h->sys[si].nrhs; }
   h->vec_all = (int)h->h_vec.size();
   for (int si = 0; si < nsys; ++si) {
     const SysInfo& s = h->sys[si];
-    h->h_vec.push_back(VecOp{s.x_off, s.x_off, s.nrhs, s.nrhs, s.n, s.nrhs, 0});
+    h->h_vec.push_back(VecOp{s.x_off, s.x_off, s.nrhs, s.nrhs, s.n, s.nrhs, 0, c_off[si]});
   }
   h->vec_aug = (int)h->h_vec.size();
   for (int si = 0; si < nsys; ++si) {
     const SysInfo& s = h->sys[si];
-    h->h_vec.push_back(VecOp{s.a_off + s.n, s.x_off, s.ld, s.nrhs, s.n, s.nrhs, 0});
+    h->h_vec.push_back(VecOp{s.a_off + s.n, s.x_off, s.ld, s.nrhs, s.n, s.nrhs, 0, c_off[si]});
+  }
+  h->vec_toaug = (int)h->h_vec.size();
+  for (int si = 0; si < nsys; ++si) {
+    const SysInfo& s = h->sys[si];
+    h->h_vec.push_back(VecOp{s.x_off, s.a_off + s.n, s.nrhs, s.ld, s.n, s.nrhs, 0, c_off[si]});
   }
   // ── residual / refinement GEMM: R = B - A64 X (X in X64 or in the augmented columns) ──
   h->res_gemm = (int)h->h_gemm.size();
@@ -709,6 +779,9 @@ static int build_schedule(ghost_bor* h) {
     long long po = 0;
     h->p_off.assign(nsys, 0);
     for (int si = 0; si < nsys; ++si) { h->p_off[si] = po; po += 2LL * h->sys[si].n; }
+    h->vec_probe = (int)h->h_vec.size();
+    for (int si = 0; si < nsys; ++si)
+      h->h_vec.push_back(VecOp{h->p_off[si], h->p_off[si], 2, 2, h->sys[si].n, 2, 0, 2 * si});
     for (int si = 0; si < nsys; ++si) {   // C = 0 - B (-S): written into BS as +B S (see ghost_bor_probe)
       const SysInfo& s = h->sys[si];
       h->h_gemm.push_back(GemmOp{s.x_off, 0, h->p_off[si], s.nrhs, 2, 2, s.n, 2, s.nrhs});
@@ -721,61 +794,19 @@ static int build_schedule(ghost_bor* h) {
       const SysInfo& s = h->sys[si];
       h->h_gemm.push_back(GemmOp{s.a64_off, h->p_off[si], h->p_off[si], s.n, 2, 2, s.n, 2, s.n});
     }
-    // forward / backward solves on a separate FP32 RHS block Y (n x nrhs)
-    for (int b = 0; b < nblk_max; ++b) {
-      const int k = b * nb;
-      Launch g1{L_GEMM_Y_W, (int)h->h_gemm.size(), nsys, 0, 0};
-      for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
-        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
-        GemmOp go{wl(si, b), s.x_off + (long long)k * s.nrhs, s.x_off + (long long)k * s.nrhs, 64, s.nrhs,
-                  s.nrhs, w, s.nrhs, w};
-        h->h_gemm.push_back(go);
-        g1.gx = std::max(g1.gx, tiles(go.M, go.N));
-      }
-      Launch g2{L_GEMM_Y_LU, (int)h->h_gemm.size(), nsys, 0, 0};
-      for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
-        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
-        GemmOp go{s.a_off + (long long)(k + w) * s.ld + k, s.x_off + (long long)k * s.nrhs,
-                  s.x_off + (long long)(k + w) * s.nrhs, s.ld, s.nrhs, s.nrhs, s.n - k - w, s.nrhs, w};
-        if (w == 0) go.M = 0;
-        h->h_gemm.push_back(go);
-        g2.gx = std::max(g2.gx, tiles(go.M, go.N));
-      }
-      h->fwdY_sched.push_back(g1);
-      h->fwdY_sched.push_back(g2);
-    }
-    for (int st = 0; st < nblk_max; ++st) {
-      Launch g1{L_GEMM_Y_W, (int)h->h_gemm.size(), nsys, 0, 0};
-      std::vector<GemmOp> upd;
-      for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
-        const int nbs = (s.n + nb - 1) / nb;
-        const int b = nbs - 1 - st;
-        GemmOp go{0, 0, 0, 64, s.nrhs, s.nrhs, 0, 0, 0};
-        GemmOp gu{0, 0, 0, s.ld, s.nrhs, s.nrhs, 0, 0, 0};
-        if (b >= 0) {
-          const int kb = b * nb, w = std::min(nb, s.n - kb);
-          go = GemmOp{wu(si, b), s.x_off + (long long)kb * s.nrhs, s.x_off + (long long)kb * s.nrhs, 64,
-                      s.nrhs, s.nrhs, w, s.nrhs, w};
-          gu = GemmOp{s.a_off + kb, s.x_off + (long long)kb * s.nrhs, s.x_off, s.ld, s.nrhs, s.nrhs, kb,
-                      s.nrhs, w};
-        }
-        h->h_gemm.push_back(go);
-        g1.gx = std::max(g1.gx, tiles(go.M, go.N));
-        upd.push_back(gu);
-      }
-      Launch g2{L_GEMM_Y_LU, (int)h->h_gemm.size(), nsys, 0, 0};
-      for (auto& gu : upd) { h->h_gemm.push_back(gu); g2.gx = std::max(g2.gx, tiles(gu.M, gu.N)); }
-      h->backY_sched.push_back(g1);
-      h->backY_sched.push_back(g2);
-    }
+    // forward / backward solves with the FP32 factors on separate RHS blocks:
+    // Y (n x nrhs at x_off, the refinement corrections) and P (n x 2 at p_off, the probe)
+    add_tri_solves(h->fwdY_sched, h->backY_sched, [&](int si) { return h->sys[si].x_off; },
+                   [&](int si) { return h->sys[si].nrhs; });
+    add_tri_solves(h->fwdP_sched, h->backP_sched, [&](int si) { return h->p_off[si]; },
+                   [&](int si) { (void)si; return 2; });
   }
   set_flops(h, h->lu_sched);
   set_flops(h, h->back_sched);
   set_flops(h, h->fwdY_sched);
   set_flops(h, h->backY_sched);
+  set_flops(h, h->fwdP_sched);
+  set_flops(h, h->backP_sched);
   // pivot offsets
   int poff = 0;
   std::vector<int> pofs(nsys);
@@ -839,6 +870,12 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
     if ((rc = dalloc(h, &h->d_S, nrmax * 2 * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_BS, pe * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_XS, pe * sizeof(double2)))) return rc;
+    if ((rc = dalloc(h, &h->d_P32, pe * sizeof(float2)))) return rc;
+    size_t ncol = 0;
+    for (auto& s : h->sys) ncol += (size_t)s.nrhs;
+    if ((rc = dalloc(h, &h->d_bscale, ncol * sizeof(double)))) return rc;
+    if ((rc = dalloc(h, &h->d_rscale, ncol * sizeof(double)))) return rc;
+    if ((rc = dalloc(h, &h->d_pscale, 2 * h->sys.size() * sizeof(double)))) return rc;
     // S: fixed pseudo-random unit-modulus entries, -1 * S so that k_gemm_update's C -= A B adds B S
     std::vector<double2> S(nrmax * 2);
     unsigned long long st = 0x9E3779B97F4A7C15ULL;
@@ -1026,7 +1063,8 @@ static int fill_problem(ghost_bor* h, Problem& p) {
   for (int i = 0; i < p.nmodes; ++i) maxn = std::max(maxn, h->sys[p.sys0 + i].n);
   dim3 gr((p.nrhs + 255) / 256, maxn, p.nmodes);
   prof_begin(h);
-  if (mixed) hipLaunchKernelGGL((k_rhs<float>), gr, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0, h->d_A, h->d_B64, 0);
+  // mixed: B in FP64 only; the solve writes the column-scaled FP32 copy into the augmented columns
+  if (mixed) hipLaunchKernelGGL((k_rhs<double>), gr, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0, nullptr, h->d_B64, 0);
   else hipLaunchKernelGGL((k_rhs<double>), gr, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0, h->d_A, h->d_B64, 0);
   prof_end(h, K_RHS, 0.0);
   LAUNCHCHK();
@@ -1177,46 +1215,54 @@ static int frob2(ghost_bor* h, const double2* buf, std::vector<double>& out) {
   return 0;
 }
 
-// eta[s] = ||B S - A64 X S|| / ||B S|| for the current X64 (mixed precision only).
-static int probe_residual(ghost_bor* h, std::vector<double>& eta) {
+// Error probe of the current X64 (mixed precision only): with S two fixed
+// random column combinations, P = B S - A64 X S in FP64 and E = A32^-1 P with
+// the FP32 factors, eta[s] = ||E|| / ||X S|| estimates the relative error of
+// system s's currents (the correction the next refinement sweep would make).
+static int probe_error(ghost_bor* h, std::vector<double>& eta) {
   const int nsys = (int)h->sys.size();
   size_t pe = 0;
   for (auto& s : h->sys) pe += 2 * (size_t)s.n;
-  int gx1 = 0, gx2 = 0;
-  for (auto& s : h->sys) { gx1 = std::max(gx1, (s.n + 63) / 64); gx2 = gx1; }
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (pe + 255) / 256)), dim3(256), 0, h->stream,
-                     h->d_BS, pe);
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (pe + 255) / 256)), dim3(256), 0, h->stream,
-                     h->d_XS, pe);
+  int gx = 0, maxn = 0;
+  for (auto& s : h->sys) { gx = std::max(gx, (s.n + 63) / 64); maxn = std::max(maxn, s.n); }
+  const dim3 gz((unsigned)std::min<size_t>(65535, (pe + 255) / 256));
+  hipLaunchKernelGGL(k_zero, gz, dim3(256), 0, h->stream, h->d_BS, pe);
+  hipLaunchKernelGGL(k_zero, gz, dim3(256), 0, h->stream, h->d_XS, pe);
   LAUNCHCHK();
   prof_begin(h);
-  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx1, nsys), dim3(256), 0, h->stream,
+  // S is stored negated, so C -= A (-S) accumulates +A S
+  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
                      h->d_gemm + h->probe_ops, h->d_B64, h->d_S, h->d_BS);
-  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx1, nsys), dim3(256), 0, h->stream,
+  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
                      h->d_gemm + h->probe_ops + nsys, h->d_X64, h->d_S, h->d_XS);
-  // RP = BS - A64 XS, in place in BS
-  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx2, nsys), dim3(256), 0, h->stream,
+  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
                      h->d_gemm + h->probe_ops + 2 * nsys, h->d_A64, h->d_XS, h->d_BS);
   prof_end(h, K_GEMM64, 0.0);
   LAUNCHCHK();
-  // ||B S|| needs BS before the update: recompute it into XS (XS is free now)
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (pe + 255) / 256)), dim3(256), 0, h->stream,
-                     h->d_XS, pe);
-  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx1, nsys), dim3(256), 0, h->stream,
-                     h->d_gemm + h->probe_ops, h->d_B64, h->d_S, h->d_XS);
+  hipLaunchKernelGGL(k_col_scale, dim3(1, 1, nsys), dim3(64, 4), 0, h->stream, h->d_vec + h->vec_probe,
+                     h->d_BS, h->d_pscale);
+  hipLaunchKernelGGL(k_scale_cast, dim3(1, maxn, nsys), dim3(64), 0, h->stream, h->d_vec + h->vec_probe,
+                     h->d_perm, h->d_BS, h->d_pscale, h->d_P32);
   LAUNCHCHK();
-  std::vector<double2> rp(pe), bs(pe);
-  HIPCHK(hipMemcpyAsync(rp.data(), h->d_BS, pe * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(bs.data(), h->d_XS, pe * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  int rc;
+  if ((rc = run_sched<float>(h, h->fwdP_sched, h->d_A, h->d_P32))) return rc;
+  if ((rc = run_sched<float>(h, h->backP_sched, h->d_A, h->d_P32))) return rc;
+  std::vector<float2> e(pe);
+  std::vector<double2> xs(pe);
+  std::vector<double> sc(2 * (size_t)nsys);
+  HIPCHK(hipMemcpyAsync(e.data(), h->d_P32, pe * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(xs.data(), h->d_XS, pe * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipMemcpyAsync(sc.data(), h->d_pscale, sc.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   eta.assign(nsys, 0.0);
   for (int si = 0; si < nsys; ++si) {
-    double r2 = 0, b2 = 0;
-    for (long long i = h->p_off[si]; i < h->p_off[si] + 2LL * h->sys[si].n; ++i) {
-      r2 += rp[i].x * rp[i].x + rp[i].y * rp[i].y;
-      b2 += bs[i].x * bs[i].x + bs[i].y * bs[i].y;
+    double e2 = 0, x2 = 0;
+    for (long long i = h->p_off[si], j = 0; i < h->p_off[si] + 2LL * h->sys[si].n; ++i, ++j) {
+      const double f = 1.0 / sc[2 * si + (j & 1)];
+      e2 += ((double)e[i].x * e[i].x + (double)e[i].y * e[i].y) * f * f;
+      x2 += xs[i].x * xs[i].x + xs[i].y * xs[i].y;
     }
-    eta[si] = b2 > 0 ? std::sqrt(r2 / b2) : 0.0;
+    eta[si] = x2 > 0 ? std::sqrt(e2 / x2) : (e2 > 0 ? INFINITY : 0.0);
   }
   return 0;
 }
@@ -1228,22 +1274,29 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
   const bool mixed = h->precision == GHOST_MIXED;
   if (!h->capturing) HIPCHK(hipEventRecord(h->ev[2], h->stream));
   int rc;
-  h->refine_hist.clear();
   if (!mixed) {
     if ((rc = run_lu<double>(h))) return rc;
   } else {
-    if ((rc = run_lu<float>(h))) return rc;
     size_t maxn = 0, maxr = 0;
     for (auto& s : h->sys) { maxn = std::max(maxn, (size_t)s.n); maxr = std::max(maxr, (size_t)s.nrhs); }
     dim3 gv((unsigned)((maxr + 255) / 256), (unsigned)maxn, nsys);
-    // X64 = (double) X32 from the augmented columns
-    hipLaunchKernelGGL(k_widen, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_aug, (const float2*)h->d_A, h->d_X64, 0);
+    dim3 gs((unsigned)((maxr + 63) / 64), 1, nsys);
+    // FP32 right-hand sides = B64 with power-of-two column scales (no underflow
+    // for the high orders), into the augmented columns of the factor
+    hipLaunchKernelGGL(k_col_scale, gs, dim3(64, 4), 0, h->stream, h->d_vec + h->vec_all, h->d_B64, h->d_bscale);
+    hipLaunchKernelGGL(k_scale_cast, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_toaug, nullptr, h->d_B64,
+                       h->d_bscale, (float2*)h->d_A);
+    LAUNCHCHK();
+    if ((rc = run_lu<float>(h))) return rc;
+    // X64 = X32 / scale from the augmented columns
+    hipLaunchKernelGGL(k_widen_scaled, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_aug, (const float2*)h->d_A,
+                       h->d_bscale, h->d_X64, 0);
     LAUNCHCHK();
     hipLaunchKernelGGL(k_perm_from_pivots, dim3((nsys + 63) / 64), dim3(64), 0, h->stream, h->d_vec + h->vec_all,
                        nsys, h->d_piv, h->d_perm);
     LAUNCHCHK();
     // refine_iters >= 0: that many FP64-residual sweeps; < 0: adaptive, up to
-    // -refine_iters sweeps while the probe residual exceeds h->probe_tol
+    // -refine_iters sweeps while the error probe exceeds h->probe_tol
     const bool adaptive = refine_iters < 0;
     const int max_it = adaptive ? -refine_iters : refine_iters;
     h->probe_hist.clear();
@@ -1251,17 +1304,20 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
     for (int it = 0; it < max_it; ++it) {
       if (adaptive) {
         std::vector<double> eta;
-        if ((rc = probe_residual(h, eta))) return rc;
+        if ((rc = probe_error(h, eta))) return rc;
         const double mx = *std::max_element(eta.begin(), eta.end());
         h->probe_hist.push_back(mx);
         if (mx <= h->probe_tol) break;
       }
       if ((rc = residual_gemm(h, 0))) return rc;      // R = B - A X in FP64
-      hipLaunchKernelGGL(k_permute_cast, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_perm, h->d_R64, h->d_Y32);
+      hipLaunchKernelGGL(k_col_scale, gs, dim3(64, 4), 0, h->stream, h->d_vec + h->vec_all, h->d_R64, h->d_rscale);
+      hipLaunchKernelGGL(k_scale_cast, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_perm, h->d_R64,
+                         h->d_rscale, h->d_Y32);
       LAUNCHCHK();
       if ((rc = run_sched<float>(h, h->fwdY_sched, h->d_A, h->d_Y32))) return rc;
       if ((rc = run_sched<float>(h, h->backY_sched, h->d_A, h->d_Y32))) return rc;
-      hipLaunchKernelGGL(k_widen, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_Y32, h->d_X64, 1);
+      hipLaunchKernelGGL(k_widen_scaled, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_Y32, h->d_rscale,
+                         h->d_X64, 1);
       LAUNCHCHK();
     }
   }

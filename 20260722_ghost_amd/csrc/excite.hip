@@ -103,7 +103,8 @@ __device__ __forceinline__ double2 jn_val(const double* J, int n, size_t stride)
 }
 
 // RHS columns theta-major then (VV, HH), written into the augmented system
-// columns [n, n + nrhs) (bor_solver.py:750-752).
+// columns [n, n + nrhs) (bor_solver.py:750-752) when Abase is set, and into
+// the FP64 copy B64 when that is set.
 template <typename Ra>
 __global__ __launch_bounds__(256) void k_rhs(ExciteArgs a, const SysInfo* sys, void* Abase,
                                              double2* B64, int sys0) {
@@ -166,9 +167,11 @@ __global__ __launch_bounds__(256) void k_rhs(ExciteArgs a, const SysInfo* sys, v
       acc.y += wgt * (ev.x * sp + ev.y * cp);
     }
   }
-  Ca* A = (Ca*)Abase + s.a_off;
-  Ca o; o.x = (Ra)acc.x; o.y = (Ra)acc.y;
-  A[(size_t)r * s.ld + s.n + col] = o;
+  if (Abase) {
+    Ca* A = (Ca*)Abase + s.a_off;
+    Ca o; o.x = (Ra)acc.x; o.y = (Ra)acc.y;
+    A[(size_t)r * s.ld + s.n + col] = o;
+  }
   if (B64) B64[s.x_off + (size_t)r * s.nrhs + col] = acc;
 }
 

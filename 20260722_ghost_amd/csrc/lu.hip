@@ -568,27 +568,60 @@ __global__ __launch_bounds__(64) void k_tri_inv(const TrsmOp* ops, const void* t
 struct VecOp {       // an n x N block: src at s (lds), dst at d (ldd)
   long long s, d;
   int lds, ldd, n, N, piv;
+  int sc;            // offset of this block's N column scales
 };
 
-// dst32[i] = (float) src64[perm[i]] with perm the row order of the factor
-__global__ void k_permute_cast(const VecOp* ops, const int* perm, const double2* src, float2* dst) {
+// Power-of-two column scales of an n x N FP64 block (src at s, lds):
+// scale[c] = 2^-e with max_r |v[r][c]| in [2^(e-1), 2^e) (1 for a zero
+// column), so that the scaled column has max modulus in [1/2, 1) and survives
+// the cast to FP32 whatever its magnitude (right-hand sides of high azimuthal
+// orders fall to 1e-60 and below).  64 columns x 4 row groups per block.
+__global__ __launch_bounds__(256) void k_col_scale(const VecOp* ops, const double2* src, double* scale) {
+  const VecOp op = ops[blockIdx.z];
+  const int c = blockIdx.x * 64 + threadIdx.x, rg = threadIdx.y;
+  __shared__ double red[4][64];
+  double m = 0.0;
+  if (c < op.N)
+    for (int r = rg; r < op.n; r += 4) {
+      const double2 v = src[op.s + (size_t)r * op.lds + c];
+      m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+    }
+  red[rg][threadIdx.x] = m;
+  __syncthreads();
+  if (rg == 0 && c < op.N) {
+    m = fmax(fmax(red[0][threadIdx.x], red[1][threadIdx.x]), fmax(red[2][threadIdx.x], red[3][threadIdx.x]));
+    // exponent clamped so that the scale stays a finite double (a column of
+    // denormals, such as an excitation that vanishes to rounding, keeps
+    // 2^1000 and stays tiny rather than overflowing)
+    int e = 0;
+    if (m > 0.0) frexp(m, &e);
+    e = max(-1000, min(1000, e));
+    scale[op.sc + c] = ldexp(1.0, -e);
+  }
+}
+
+// dst32[r][c] = (float) (src64[perm ? perm[r] : r][c] * scale[c])
+__global__ void k_scale_cast(const VecOp* ops, const int* perm, const double2* src, const double* scale,
+                             float2* dst) {
   const VecOp op = ops[blockIdx.z];
   const int r = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= op.n || c >= op.N) return;
-  const int pr = perm[op.piv + r];
+  const int pr = perm ? perm[op.piv + r] : r;
   const double2 v = src[op.s + (size_t)pr * op.lds + c];
-  dst[op.d + (size_t)r * op.ldd + c] = make_float2((float)v.x, (float)v.y);
+  const double f = scale[op.sc + c];
+  dst[op.d + (size_t)r * op.ldd + c] = make_float2((float)(v.x * f), (float)(v.y * f));
 }
 
-// dst64 (+)= (double) src32 ; with add = 0 a plain widening copy
-__global__ void k_widen(const VecOp* ops, const float2* src, double2* dst, int add) {
+// dst64 (+)= (double) src32 / scale[c]
+__global__ void k_widen_scaled(const VecOp* ops, const float2* src, const double* scale, double2* dst, int add) {
   const VecOp op = ops[blockIdx.z];
   const int r = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= op.n || c >= op.N) return;
   const float2 v = src[op.s + (size_t)r * op.lds + c];
+  const double f = 1.0 / scale[op.sc + c];
   double2* p = dst + op.d + (size_t)r * op.ldd + c;
-  if (add) { double2 o = *p; o.x += (double)v.x; o.y += (double)v.y; *p = o; }
-  else *p = make_double2((double)v.x, (double)v.y);
+  if (add) { double2 o = *p; o.x += (double)v.x * f; o.y += (double)v.y * f; *p = o; }
+  else *p = make_double2((double)v.x * f, (double)v.y * f);
 }
 
 __global__ void k_zero_int(int* p, int n) {
@@ -616,25 +649,6 @@ __global__ void k_perm_from_pivots(const VecOp* ops, int nsys, const int* pivots
     const int j = ip[i];
     if (j != i) { const int t = p[i]; p[i] = p[j]; p[j] = t; }
   }
-}
-
-// per-system max |v| over an n x N block (for the refinement stop test)
-__global__ void k_block_maxabs(const VecOp* ops, const double2* v, double* out) {
-  const VecOp op = ops[blockIdx.x];
-  __shared__ double red[256];
-  double m = 0.0;
-  for (size_t i = threadIdx.x; i < (size_t)op.n * op.N; i += 256) {
-    const int r = (int)(i / op.N), c = (int)(i % op.N);
-    const double2 z = v[op.d + (size_t)r * op.ldd + c];
-    m = fmax(m, fabs(z.x) + fabs(z.y));
-  }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
 }  // namespace ghost

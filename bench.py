@@ -68,7 +68,7 @@ def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profil
     ktimes = h.kernel_times() if profile else {}
     phase = h.timings()
     if precision == "mixed":
-        phase["probe_residuals"] = h.refine_history()
+        phase["probe_errors"] = h.refine_history()
     dF = h.dF(0)
     h.close()
     return dt, dF, ktimes, phase, (ne, nbk)
@@ -151,7 +151,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--refine", type=int, default=-2,
-                    help="FP64 refinement sweeps; negative = adaptive (probe residual), up to |n|")
+                    help="FP64 refinement sweeps; negative = adaptive (error probe), up to |n|")
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")

@@ -93,17 +93,21 @@ int ghost_bor_build(ghost_bor* h);
 int ghost_bor_fill(ghost_bor* h);
 /* Batched LU + solve of every mode system.  GHOST_MIXED: refine_iters >= 0
  * runs that many FP64-residual refinement sweeps; refine_iters < 0 is
- * adaptive: before each of up to -refine_iters sweeps a probe residual
- * ||B s - A X s|| / ||B s|| (two fixed random column combinations s, FP64)
- * is measured and the sweeps stop once it is <= the probe tolerance. */
+ * adaptive: before each of up to -refine_iters sweeps an error probe
+ * ||A32^-1 (B s - A X s)|| / ||X s|| (two fixed random column combinations
+ * s; residual in FP64, correction with the FP32 factors), the estimated
+ * relative error of each system's currents, is measured and the sweeps stop
+ * once its maximum over systems is <= the probe tolerance.  The FP32
+ * right-hand sides carry power-of-two column scales, so orders whose
+ * excitation lies below the FP32 range are solved too. */
 int ghost_bor_solve(ghost_bor* h, int refine_iters);
 /* Tile variant of the batched complex MFMA GEMM (trailing updates, solves,
  * residuals): 0 = 64x64 single-buffered, 1 = 64x64, 2 = 64x128, 3 = 128x128
  * (FP32; 64x128 for FP64) with double-buffered K chunks.  Default 1. */
 int ghost_bor_set_gemm_variant(ghost_bor* h, int variant);
-/* Probe tolerance of the adaptive refinement (default 1e-6). */
+/* Probe tolerance of the adaptive refinement (default 1e-5). */
 int ghost_bor_set_probe_tol(ghost_bor* h, double tol);
-/* Probe residuals of the last adaptive solve; *n: capacity in, count out. */
+/* Probe errors of the last adaptive solve, one per probe; *n: capacity in, count out. */
 int ghost_bor_refine_history(ghost_bor* h, double* out, int* n);
 /* Monostatic far-field contribution of every mode and column. */
 int ghost_bor_farfield(ghost_bor* h);
