@@ -62,6 +62,7 @@ SIGNATURES = [
     ("ghost_bor_kernel_count", _ci, []),
     ("ghost_bor_kernel_name", ctypes.c_char_p, [_ci]),
     ("ghost_bor_kernel_times", _ci, [_vp, _dp, _ip, _dp]),
+    ("ghost_bor_adjoint", _ci, [_vp, _dp, _cd, _dp]),
 ]
 
 
@@ -127,6 +128,7 @@ class Handle:
         check(self.lib.ghost_bor_set_residual(self._h, 1 if residual else 0))
         self.precision = precision
         self.problems = []
+        self.nodes_per_problem = []
 
     def close(self):
         if self._h:
@@ -149,6 +151,7 @@ class Handle:
                                              dptr(th), int(th.size), GHOST_UNMASKED if unmasked else 0,
                                              ctypes.byref(idx)))
         self.problems.append((int(m_max), int(th.size)))
+        self.nodes_per_problem.append(int(nodes.shape[0]))
         return idx.value
 
     def build(self):
@@ -187,6 +190,21 @@ class Handle:
         out = np.zeros((2 * m_max + 1, 2, nth, 2), dtype=np.float64)
         check(self.lib.ghost_bor_get_dF(self._h, int(prob), dptr(out)))
         return out[..., 0] + 1j * out[..., 1]
+
+    def adjoint(self, acoef, step: float) -> np.ndarray:
+        """Gradient w.r.t. the radial coordinate of every node of every problem
+        of dJ = Re sum_c a_c dF_c, with acoef [problems][n_rhs] complex in
+        column order 2 * theta_index + (0 VV, 1 HH) and F the mode-summed
+        far-field amplitude.  Needs a solved FP64 handle; afterwards the
+        systems hold a perturbed fill (fill again before the next solve)."""
+        a = np.ascontiguousarray(np.asarray(acoef, dtype=np.complex128).ravel())
+        ncol = sum(2 * nth for _, nth in self.problems)
+        if a.size != ncol:
+            raise ValueError(f"acoef must have {ncol} entries, got {a.size}")
+        ab = np.ascontiguousarray(np.column_stack([a.real, a.imag]).ravel())
+        out = np.zeros(sum(self.nodes_per_problem), dtype=np.float64)
+        check(self.lib.ghost_bor_adjoint(self._h, dptr(ab), float(step), dptr(out)))
+        return out
 
     def residuals(self) -> np.ndarray:
         n = sum(2 * m + 1 for m, _ in self.problems)

@@ -15,6 +15,7 @@
 #include "fill.hip"
 #include "excite.hip"
 #include "lu.hip"
+#include "adjoint.hip"
 #include "../../include/ghost_bor.h"
 
 using namespace ghost;
@@ -176,6 +177,7 @@ struct Problem {
   double2* d_NP = nullptr;
   double2* d_NB = nullptr;
   void* d_zfar = nullptr;
+  double* d_alt = nullptr;       // adjoint: perturbed pts | nodes | elem
   int cell_off[3] = {0, 0, 0}, cell_cnt[3] = {0, 0, 0};
   size_t tab_off[6] = {0, 0, 0, 0, 0, 0};
 };
@@ -199,7 +201,8 @@ static const char* kKernelNames[K_COUNT] = {"zero", "near_modal", "near_blocks",
                                             "bessel", "rhs", "lu_panel", "lu_swap", "tri_inv", "gemm",
                                             "gemm_fp64_residual", "vec", "farfield"};
 
-enum { L_PANEL, L_SWAP, L_TRIL, L_TRIU, L_GEMM_AUG, L_GEMM_AUG_W, L_GEMM_Y, L_GEMM_Y_W, L_GEMM_Y_LU };
+enum { L_PANEL, L_SWAP, L_TRIL, L_TRIU, L_GEMM_AUG, L_GEMM_AUG_W, L_GEMM_Y, L_GEMM_Y_W, L_GEMM_Y_LU,
+       L_GEMM_TW, L_GEMM_TLU };   // transposed-operand solves (adjoint): W^T and factor-block^T
 
 struct ghost_bor {
   int device = 0;
@@ -208,6 +211,7 @@ struct ghost_bor {
   int nb = 64;
   int residual = 1;
   bool built = false;
+  bool solved = false;               // the augmented columns hold the solution of the current fill
   std::vector<Problem> probs;
   std::vector<SysInfo> sys;
   SysInfo* d_sys = nullptr;
@@ -241,6 +245,19 @@ struct ghost_bor {
   GemmOp* d_gemm = nullptr;
   VecOp* d_vec = nullptr;
   std::vector<Launch> lu_sched, back_sched, fwdY_sched, backY_sched, fwdP_sched, backP_sched;
+  // adjoint (FP64 handles): A^T lambda = G through the LU factors, Q = -Lambda X^T
+  std::vector<Launch> adjT_fwd, adjT_back;
+  int adj_q_ops = -1;
+  bool adj_alloc = false;
+  double2* d_Xs = nullptr;           // forward solution, x layout
+  double2* d_lam = nullptr;          // G, then z = P lambda (x layout)
+  double2* d_lam2 = nullptr;         // lambda
+  double2* d_Q = nullptr;            // [-Lambda X^T | Lambda] in the augmented layout (FP64)
+  double2* d_rs = nullptr;           // row or column sums, per system at row_off
+  double2* d_acoef = nullptr;        // a' per problem column
+  double2* d_ffpart = nullptr;       // per-element far-field partial sums of one problem
+  double2* d_ffs = nullptr;          // per-element contraction with a'
+  int* d_rowoff = nullptr;
   int triu_ops = 0, triu_n = 0;
   int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
   int vec_all = 0, vec_aug = 0;          // VecOps over all systems: x-blocks / augmented RHS -> x-blocks
@@ -426,32 +443,45 @@ extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, 
 // build: host tables, device buffers, solve schedule
 // ─────────────────────────────────────────────────────────────────────────────
 
-static int prepare_problem(ghost_bor* h, Problem& p) {
-  const int P = p.P, Ne = p.Ne;
-  // ── Gauss points, order 4 (bor_kernels.py:103-125) ──
+// Gauss points, order 4 (bor_kernels.py:103-125), of a generatrix with Ne
+// elements: pts = rho | z | t_rho | t_z | w (x element length) | T0 | T1 |
+// D0 | D1 (d(rho T)/dt), each [4 Ne]; elem = L | t_rho | t_z, each [Ne].
+static void geometry_tables(const double* nodes, int Ne, std::vector<double>& pts, std::vector<double>& elem) {
   std::vector<double> x4, w4;
   gauss_legendre(4, x4, w4);
-  p.pts.assign(9 * (size_t)P, 0.0);
-  double* rho = &p.pts[0 * P]; double* z = &p.pts[1 * P]; double* tr = &p.pts[2 * P];
-  double* tzp = &p.pts[3 * P]; double* w = &p.pts[4 * P]; double* T0 = &p.pts[5 * P];
-  double* T1 = &p.pts[6 * P]; double* D0 = &p.pts[7 * P]; double* D1 = &p.pts[8 * P];
+  const int P = 4 * Ne;
+  pts.assign(9 * (size_t)P, 0.0);
+  elem.assign(3 * (size_t)Ne, 0.0);
+  double* rho = &pts[0 * P]; double* z = &pts[1 * P]; double* tr = &pts[2 * P];
+  double* tzp = &pts[3 * P]; double* w = &pts[4 * P]; double* T0 = &pts[5 * P];
+  double* T1 = &pts[6 * P]; double* D0 = &pts[7 * P]; double* D1 = &pts[8 * P];
   for (int e = 0; e < Ne; ++e) {
-    const double r0 = p.nodes[2 * e], z0 = p.nodes[2 * e + 1];
-    const double dr = p.nodes[2 * e + 2] - r0, dz = p.nodes[2 * e + 3] - z0;
+    const double r0 = nodes[2 * e], z0 = nodes[2 * e + 1];
+    const double dr = nodes[2 * e + 2] - r0, dz = nodes[2 * e + 3] - z0;
+    const double L = std::hypot(dr, dz);
+    elem[e] = L;
+    elem[Ne + e] = dr / L;
+    elem[2 * Ne + e] = dz / L;
     for (int g = 0; g < 4; ++g) {
       const int i = 4 * e + g;
       const double s = 0.5 * (x4[g] + 1.0);
       rho[i] = r0 + s * dr;
       z[i] = z0 + s * dz;
-      tr[i] = p.trho[e];
-      tzp[i] = p.tz[e];
-      w[i] = 0.5 * w4[g] * p.L[e];
+      tr[i] = dr / L;
+      tzp[i] = dz / L;
+      w[i] = 0.5 * w4[g] * L;
       T0[i] = 1.0 - s;
       T1[i] = s;
-      D0[i] = (dr * (1.0 - s) - rho[i]) / p.L[e];
-      D1[i] = (dr * s + rho[i]) / p.L[e];
+      D0[i] = (dr * (1.0 - s) - rho[i]) / L;
+      D1[i] = (dr * s + rho[i]) / L;
     }
   }
+}
+
+static int prepare_problem(ghost_bor* h, Problem& p) {
+  const int P = p.P, Ne = p.Ne;
+  std::vector<double> elem;
+  geometry_tables(p.nodes.data(), Ne, p.pts, elem);
   // ── near pairs: cells, tail orders (bor_kernels.py:238-240) ──
   std::vector<double> cs[3], csp[3], cw[3];
   for (int kd = 0; kd < 3; ++kd) cell_points(kd, cs[kd], csp[kd], cw[kd]);
@@ -505,8 +535,6 @@ static int prepare_problem(ghost_bor* h, Problem& p) {
   if ((rc = dalloc(h, &p.d_pts, p.pts.size() * 8))) return rc;
   if ((rc = dalloc(h, &p.d_nodes, p.nodes.size() * 8))) return rc;
   if ((rc = dalloc(h, &p.d_elem, 3 * (size_t)Ne * 8))) return rc;
-  std::vector<double> elem(3 * (size_t)Ne);
-  for (int e = 0; e < Ne; ++e) { elem[e] = p.L[e]; elem[Ne + e] = p.trho[e]; elem[2 * Ne + e] = p.tz[e]; }
   // far tables with exact values at 0, pi/2, pi, 3pi/2
   std::vector<double> tabs;
   auto cos_exact = [](long long i, int N) {
@@ -586,13 +614,16 @@ static double gemm_flops(const std::vector<GemmOp>& ops, int first, int n, bool 
 
 static void set_flops(ghost_bor* h, std::vector<Launch>& sched) {
   for (Launch& L : sched) {
-    if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W || L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU)
+    if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W || L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU ||
+        L.kind == L_GEMM_TW || L.kind == L_GEMM_TLU)
       for (int i = L.ops; i < L.ops + L.nops; ++i) {
         L.mM = std::max(L.mM, h->h_gemm[i].M);
         L.mN = std::max(L.mN, h->h_gemm[i].N);
       }
-    if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W) L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, true);
-    else if (L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU) L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, false);
+    if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W || L.kind == L_GEMM_TW)
+      L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, true);
+    else if (L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU || L.kind == L_GEMM_TLU)
+      L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, false);
   }
 }
 
@@ -801,8 +832,68 @@ static int build_schedule(ghost_bor* h) {
     add_tri_solves(h->fwdP_sched, h->backP_sched, [&](int si) { return h->p_off[si]; },
                    [&](int si) { (void)si; return 2; });
   }
+  if (!mixed) {
+    // transposed solves on the lambda block (n x nrhs at x_off): with P A = L U,
+    // A^T lambda = G is U^T y = G (forward, diagonal blocks (U11^-1)^T = I - W_u^T),
+    // then L^T z = y (backward, I - W_l^T), then lambda = P^T z
+    for (int b = 0; b < nblk_max; ++b) {
+      const int k = b * nb;
+      Launch g1{L_GEMM_TW, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        const long long o = s.x_off;
+        const int c = s.nrhs;
+        h->h_gemm.push_back(GemmOp{wu(si, b), o + (long long)k * c, o + (long long)k * c, 64, c, c, w, c, w});
+      }
+      Launch g2{L_GEMM_TLU, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        const long long o = s.x_off;
+        const int c = s.nrhs;
+        GemmOp go{s.a_off + (long long)k * s.ld + k + w, o + (long long)k * c, o + (long long)(k + w) * c, s.ld, c,
+                  c, s.n - k - w, c, w};
+        if (w == 0) go.M = 0;
+        h->h_gemm.push_back(go);
+      }
+      h->adjT_fwd.push_back(g1);
+      h->adjT_fwd.push_back(g2);
+    }
+    for (int st = 0; st < nblk_max; ++st) {
+      Launch g1{L_GEMM_TW, (int)h->h_gemm.size(), nsys, 0, 0};
+      std::vector<GemmOp> upd;
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int b = (s.n + nb - 1) / nb - 1 - st;
+        const long long o = s.x_off;
+        const int c = s.nrhs;
+        GemmOp go{0, 0, 0, 64, c, c, 0, 0, 0};
+        GemmOp gu{0, 0, 0, s.ld, c, c, 0, 0, 0};
+        if (b >= 0) {
+          const int kb = b * nb, w = std::min(nb, s.n - kb);
+          go = GemmOp{wl(si, b), o + (long long)kb * c, o + (long long)kb * c, 64, c, c, w, c, w};
+          gu = GemmOp{s.a_off + (long long)kb * s.ld, o + (long long)kb * c, o, s.ld, c, c, kb, c, w};
+        }
+        h->h_gemm.push_back(go);
+        upd.push_back(gu);
+      }
+      Launch g2{L_GEMM_TLU, (int)h->h_gemm.size(), nsys, 0, 0};
+      for (auto& gu : upd) h->h_gemm.push_back(gu);
+      h->adjT_back.push_back(g1);
+      h->adjT_back.push_back(g2);
+    }
+    // Q[:, :n] = -Lambda X^T (X stored n x nrhs, used transposed)
+    h->adj_q_ops = (int)h->h_gemm.size();
+    for (int si = 0; si < nsys; ++si) {
+      const SysInfo& s = h->sys[si];
+      h->h_gemm.push_back(GemmOp{s.x_off, s.x_off, s.a_off, s.nrhs, s.nrhs, s.ld, s.n, s.n, s.nrhs});
+    }
+  }
   set_flops(h, h->lu_sched);
   set_flops(h, h->back_sched);
+  set_flops(h, h->adjT_fwd);
+  set_flops(h, h->adjT_back);
   set_flops(h, h->fwdY_sched);
   set_flops(h, h->backY_sched);
   set_flops(h, h->fwdP_sched);
@@ -909,11 +1000,26 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
 // phases
 // ─────────────────────────────────────────────────────────────────────────────
 
-template <typename R> static int far_fill(ghost_bor* h, Problem& p) {
+// Device geometry of one side of the operator: Gauss point tables (9 arrays of
+// P), nodes [Nn][2] and per-element L | t_rho | t_z.  Test and source sides
+// are the problem's own tables except in the adjoint's perturbed fills.
+struct Geom {
+  const double* pts;
+  const double* nodes;
+  const double* elem;
+};
+
+static PointSet point_set(const double* pts, int P) {
+  return PointSet{pts, pts + P, pts + 2 * P, pts + 3 * P, pts + 4 * P, pts + 5 * P, pts + 6 * P, pts + 7 * P,
+                  pts + 8 * P};
+}
+
+static Geom base_geom(const Problem& p) { return Geom{p.d_pts, p.d_nodes, p.d_elem}; }
+
+template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg) {
   FarArgs fa;
-  const int P = p.P;
-  fa.pts = PointSet{p.d_pts, p.d_pts + P, p.d_pts + 2 * P, p.d_pts + 3 * P, p.d_pts + 4 * P,
-                    p.d_pts + 5 * P, p.d_pts + 6 * P, p.d_pts + 7 * P, p.d_pts + 8 * P};
+  fa.pts = point_set(tg.pts, p.P);
+  fa.spts = point_set(sg.pts, p.P);
   fa.sin2E = p.d_tabs + p.tab_off[0];
   fa.cosE = p.d_tabs + p.tab_off[1];
   fa.cxB = p.d_tabs + p.tab_off[2];
@@ -956,7 +1062,22 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p) {
   return 0;
 }
 
-static int fill_problem(ghost_bor* h, Problem& p) {
+static ExciteArgs excite_args(const Problem& p, const Geom& tg) {
+  ExciteArgs ea;
+  ea.pts = point_set(tg.pts, p.P);
+  ea.P = p.P;
+  ea.Nn = p.Nn;
+  ea.M = p.M;
+  ea.ntheta = p.ntheta;
+  ea.k = p.k;
+  ea.cE = p.cE;
+  ea.cM = p.cM;
+  ea.theta = p.d_theta;
+  ea.J = p.d_J;
+  return ea;
+}
+
+static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg) {
   const bool mixed = h->precision == GHOST_MIXED;
   const int P = p.P;
   const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.nmodes;
@@ -969,10 +1090,14 @@ static int fill_problem(ghost_bor* h, Problem& p) {
   LAUNCHCHK();
   // near stage 1 + 2 (FP64)
   NearArgs na;
-  na.nodes = p.d_nodes;
-  na.L = p.d_elem;
-  na.trho = p.d_elem + p.Ne;
-  na.tz = p.d_elem + 2 * p.Ne;
+  na.nodes = tg.nodes;
+  na.L = tg.elem;
+  na.trho = tg.elem + p.Ne;
+  na.tz = tg.elem + 2 * p.Ne;
+  na.snodes = sg.nodes;
+  na.sL = sg.elem;
+  na.strho = sg.elem + p.Ne;
+  na.stz = sg.elem + 2 * p.Ne;
   const int ncell = p.cell_off[2] + p.cell_cnt[2];
   na.cell_s = p.d_cells;
   na.cell_sp = p.d_cells + ncell;
@@ -1005,10 +1130,10 @@ static int fill_problem(ghost_bor* h, Problem& p) {
   }
   NearBlockArgs nb;
   nb.na = na;
-  nb.gw = p.d_pts + 4 * P;
-  nb.grho = p.d_pts;
-  nb.gT0 = p.d_pts + 5 * P;
-  nb.gT1 = p.d_pts + 6 * P;
+  nb.gw = tg.pts + 4 * P;
+  nb.grho = tg.pts;
+  nb.gT0 = tg.pts + 5 * P;
+  nb.gT1 = tg.pts + 6 * P;
   nb.ce_re = 0.0;
   nb.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
   nb.cM = p.cM;
@@ -1021,7 +1146,7 @@ static int fill_problem(ghost_bor* h, Problem& p) {
 
   LAUNCHCHK();
   // far
-  int rc = mixed ? far_fill<float>(h, p) : far_fill<double>(h, p);
+  int rc = mixed ? far_fill<float>(h, p, tg, sg) : far_fill<double>(h, p, tg, sg);
   if (rc) return rc;
   // assemble
   AsmArgs aa;
@@ -1040,18 +1165,7 @@ static int fill_problem(ghost_bor* h, Problem& p) {
   prof_end(h, K_ASSEMBLE, 0.0);
   LAUNCHCHK();
   // excitation
-  ExciteArgs ea;
-  ea.pts = PointSet{p.d_pts, p.d_pts + P, p.d_pts + 2 * P, p.d_pts + 3 * P, p.d_pts + 4 * P,
-                    p.d_pts + 5 * P, p.d_pts + 6 * P, p.d_pts + 7 * P, p.d_pts + 8 * P};
-  ea.P = P;
-  ea.Nn = p.Nn;
-  ea.M = p.M;
-  ea.ntheta = p.ntheta;
-  ea.k = p.k;
-  ea.cE = p.cE;
-  ea.cM = p.cM;
-  ea.theta = p.d_theta;
-  ea.J = p.d_J;
+  ExciteArgs ea = excite_args(p, tg);
   const size_t nbes = (size_t)P * p.ntheta;
   prof_begin(h);
   hipLaunchKernelGGL(k_bessel, dim3((unsigned)std::min<size_t>(65535, (nbes + 127) / 128)), dim3(128), 0,
@@ -1073,7 +1187,7 @@ static int fill_problem(ghost_bor* h, Problem& p) {
 
 static int run_fill(ghost_bor* h) {
   for (auto& p : h->probs) {
-    int rc = fill_problem(h, p);
+    int rc = fill_problem(h, p, base_geom(p), base_geom(p));
     if (rc) return rc;
   }
   return 0;
@@ -1082,17 +1196,18 @@ static int run_fill(ghost_bor* h) {
 // Batched complex GEMM C -= A B with the handle's tile variant:
 // 0: 64 x 64 single-buffered; 1: 64 x 64, 2: 64 x 128, 3: 128 x 128 (FP32) /
 // 64 x 128 (FP64), double-buffered.  gx = 0 computes the grid from mM, mN.
-template <typename T> static void gemm_launch(ghost_bor* h, const GemmOp* ops, int nops, int mM, int mN,
-                                              const void* A, const void* B, void* C) {
+template <typename T, bool TA = false, bool TB = false>
+static void gemm_launch(ghost_bor* h, const GemmOp* ops, int nops, int mM, int mN, const void* A, const void* B,
+                        void* C) {
   if (mM <= 0 || mN <= 0) return;
   int v = h->gemm_variant;
   if (v == 3 && sizeof(T) == 8) v = 2;
   auto grid = [&](int bm, int bn) { return dim3((unsigned)(((mM + bm - 1) / bm) * ((mN + bn - 1) / bn)), nops); };
   switch (v) {
-    case 0: hipLaunchKernelGGL((k_gemm_update<T>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
-    case 1: hipLaunchKernelGGL((k_gemm2<T, 2, 2>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
-    case 2: hipLaunchKernelGGL((k_gemm2<T, 2, 4>), grid(64, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
-    default: hipLaunchKernelGGL((k_gemm2<T, 4, 4>), grid(128, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
+    case 0: hipLaunchKernelGGL((k_gemm_update<T, TA, TB>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
+    case 1: hipLaunchKernelGGL((k_gemm2<T, 2, 2, TA, TB>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
+    case 2: hipLaunchKernelGGL((k_gemm2<T, 2, 4, TA, TB>), grid(64, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
+    default: hipLaunchKernelGGL((k_gemm2<T, 4, 4, TA, TB>), grid(128, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
   }
 }
 
@@ -1127,6 +1242,12 @@ template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launc
         break;
       case L_GEMM_Y_LU:
         gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, factor, rhs, rhs);
+        break;
+      case L_GEMM_TW:
+        gemm_launch<T, true, false>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, h->d_W, rhs, rhs);
+        break;
+      case L_GEMM_TLU:
+        gemm_launch<T, true, false>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, factor, rhs, rhs);
         break;
       default: break;
     }
@@ -1186,6 +1307,7 @@ static int residual_gemm(ghost_bor* h, int x_in_aug) {
 extern "C" int ghost_bor_fill(ghost_bor* h) {
   if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
   HIPCHK(hipSetDevice(h->device));
+  h->solved = false;
   if (!h->capturing) HIPCHK(hipEventRecord(h->ev[0], h->stream));
   int rc = run_fill(h);
   if (rc) return rc;
@@ -1325,6 +1447,7 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
     HIPCHK(hipEventRecord(h->ev[3], h->stream));
     h->ev_recorded |= 2u;
   }
+  h->solved = true;
   h->refine_iters = refine_iters;
   return 0;
 }
@@ -1358,19 +1481,8 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
   if (!h->capturing) HIPCHK(hipEventRecord(h->ev[4], h->stream));
   const int nsys = (int)h->sys.size();
   for (auto& p : h->probs) {
-    ExciteArgs ea;
+    ExciteArgs ea = excite_args(p, base_geom(p));
     const int P = p.P;
-    ea.pts = PointSet{p.d_pts, p.d_pts + P, p.d_pts + 2 * P, p.d_pts + 3 * P, p.d_pts + 4 * P,
-                      p.d_pts + 5 * P, p.d_pts + 6 * P, p.d_pts + 7 * P, p.d_pts + 8 * P};
-    ea.P = P;
-    ea.Nn = p.Nn;
-    ea.M = p.M;
-    ea.ntheta = p.ntheta;
-    ea.k = p.k;
-    ea.cE = p.cE;
-    ea.cM = p.cM;
-    ea.theta = p.d_theta;
-    ea.J = p.d_J;
     const int nch = (P + h->ff_chunk - 1) / h->ff_chunk;
     dim3 g((p.nrhs + 255) / 256, p.nmodes, nch);
     // partial layout [chunk][system-in-problem][nrhs], reduced into this problem's dF block
@@ -1481,6 +1593,7 @@ extern "C" int ghost_bor_launch(ghost_bor* h) {
   if (!h->capturing) HIPCHK(hipEventRecord(h->ev[0], h->stream));
   HIPCHK(hipGraphLaunch(h->gexec, h->stream));
   if (!h->capturing) HIPCHK(hipEventRecord(h->ev[6], h->stream));
+  h->solved = true;
   return 0;
 }
 
@@ -1712,4 +1825,194 @@ extern "C" void sample_ibc(int nr, int np_, int nxi, const double* rho_p, const 
                            double* o_tt, double* o_tf, double* o_ft, double* o_ff) {
   ghost_sample_ibc(nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab, o_tt, o_tf,
                    o_ft, o_ff);
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// adjoint: gradient of a real far-field objective w.r.t. node radial offsets
+// ─────────────────────────────────────────────────────────────────────────────
+
+static int adjoint_alloc(ghost_bor* h) {
+  if (h->adj_alloc) return 0;
+  int rc;
+  size_t nrow = 0, ncol = 0, ffmax = 0, nemax = 0;
+  for (auto& s : h->sys) nrow += (size_t)s.n;
+  for (auto& p : h->probs) {
+    ncol += (size_t)p.nrhs;
+    ffmax = std::max(ffmax, (size_t)p.Ne * p.nmodes * p.nrhs);
+    nemax = std::max(nemax, (size_t)p.Ne);
+  }
+  if ((rc = dalloc(h, &h->d_Xs, h->x_elems * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_lam, h->x_elems * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_lam2, h->x_elems * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_Q, h->a_elems * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_rs, nrow * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_acoef, ncol * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_ffpart, ffmax * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_ffs, nemax * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &h->d_rowoff, h->sys.size() * sizeof(int)))) return rc;
+  std::vector<int> ro(h->sys.size());
+  for (size_t i = 0, o = 0; i < h->sys.size(); ++i) { ro[i] = (int)o; o += (size_t)h->sys[i].n; }
+  HIPCHK(hipMemcpy(h->d_rowoff, ro.data(), ro.size() * sizeof(int), hipMemcpyHostToDevice));
+  for (auto& p : h->probs)
+    if ((rc = dalloc(h, &p.d_alt, (9 * (size_t)p.P + 2 * (size_t)p.Nn + 3 * (size_t)p.Ne) * 8))) return rc;
+  h->adj_alloc = true;
+  return 0;
+}
+
+// Node q of a problem is perturbed in class r when q = r (mod 3) and it lies
+// off the axis by more than twice the step.
+static bool adj_free(const Problem& p, int q, int r, double step) {
+  return q >= 0 && q < p.Nn && q % 3 == r && p.nodes[2 * q] > 2.0 * step;
+}
+
+extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad) {
+  if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
+  if (!h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (h->precision != GHOST_FP64) return fail(GHOST_ERR_STATE, "the adjoint needs an FP64 handle");
+  if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
+  if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
+  HIPCHK(hipSetDevice(h->device));
+  // half-width per problem: given, or 1e-6 of the shortest element (test and
+  // source sides move separately, so nearly coincident near-band point pairs,
+  // a few thousandths of an element apart, set the curvature scale)
+  std::vector<double> hstep(h->probs.size(), step);
+  for (size_t pi = 0; pi < h->probs.size(); ++pi)
+    if (step == 0.0) {
+      const Problem& p = h->probs[pi];
+      double lmin = INFINITY;
+      for (int e = 0; e < p.Ne; ++e)
+        lmin = std::min(lmin, std::hypot(p.nodes[2 * e + 2] - p.nodes[2 * e], p.nodes[2 * e + 3] - p.nodes[2 * e + 1]));
+      hstep[pi] = 1e-6 * lmin;
+    }
+  int rc;
+  if ((rc = adjoint_alloc(h))) return rc;
+  const int nsys = (int)h->sys.size();
+  const bool prof = h->profile;
+  h->profile = false;
+  size_t maxn = 0, maxr = 0, nrow = 0;
+  int mM = 0, mN = 0;
+  for (auto& s : h->sys) {
+    maxn = std::max(maxn, (size_t)s.n);
+    maxr = std::max(maxr, (size_t)s.nrhs);
+    nrow += (size_t)s.n;
+    mM = std::max(mM, s.n);
+    mN = std::max(mN, s.n);
+  }
+  const dim3 gv((unsigned)((maxr + 255) / 256), (unsigned)maxn, nsys);
+  // forward solution out of the augmented columns; the factor's row order
+  hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_aug, (const double2*)h->d_A, h->d_Xs);
+  hipLaunchKernelGGL(k_perm_from_pivots, dim3((nsys + 63) / 64), dim3(64), 0, h->stream, h->d_vec + h->vec_all, nsys,
+                     h->d_piv, h->d_perm);
+  LAUNCHCHK();
+  // a' = a (-j k eta0 / 4 pi) per problem column; G = a' dS/dX
+  std::vector<size_t> coff(h->probs.size());
+  std::vector<double2> ac;
+  for (size_t pi = 0; pi < h->probs.size(); ++pi) {
+    const Problem& p = h->probs[pi];
+    coff[pi] = ac.size();
+    const double pr = -p.k * 376.730313668 / (4.0 * M_PI);   // pref = j pr
+    for (int c = 0; c < p.nrhs; ++c) {
+      const double ar = acoef[2 * (coff[pi] + c)], ai = acoef[2 * (coff[pi] + c) + 1];
+      ac.push_back(make_double2(-ai * pr, ar * pr));       // (ar + j ai)(j pr)
+    }
+  }
+  HIPCHK(hipMemcpyAsync(h->d_acoef, ac.data(), ac.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  for (size_t pi = 0; pi < h->probs.size(); ++pi) {
+    Problem& p = h->probs[pi];
+    int mn = 0;
+    for (int i = 0; i < p.nmodes; ++i) mn = std::max(mn, h->sys[p.sys0 + i].n);
+    hipLaunchKernelGGL(k_ffvec, dim3((p.nrhs + 255) / 256, mn, p.nmodes), dim3(256), 0, h->stream,
+                       excite_args(p, base_geom(p)), h->d_sys + p.sys0, h->d_acoef + coff[pi], h->d_lam);
+    LAUNCHCHK();
+  }
+  // lambda = A^-T G through the LU factors
+  if ((rc = run_sched<double>(h, h->adjT_fwd, h->d_A, h->d_lam))) return rc;
+  if ((rc = run_sched<double>(h, h->adjT_back, h->d_A, h->d_lam))) return rc;
+  hipLaunchKernelGGL(k_unpermute, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_perm, h->d_lam, h->d_lam2);
+  LAUNCHCHK();
+  // Q = [-Lambda X^T | Lambda]
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (h->a_elems + 255) / 256)), dim3(256), 0,
+                     h->stream, h->d_Q, h->a_elems);
+  gemm_launch<double, false, true>(h, h->d_gemm + h->adj_q_ops, nsys, mM, mN, h->d_lam2, h->d_Xs, h->d_Q);
+  hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_toaug, h->d_lam2, h->d_Q);
+  LAUNCHCHK();
+  // perturbed fills: side 0 moves the test geometry (rows, right-hand sides and
+  // far-field projection), side 1 the source geometry (columns); nodes of one
+  // residue class mod 3 move together, whose row / column triples are disjoint
+  size_t goff_tot = 0;
+  std::vector<size_t> goff(h->probs.size());
+  for (size_t pi = 0; pi < h->probs.size(); ++pi) { goff[pi] = goff_tot; goff_tot += (size_t)h->probs[pi].Nn; }
+  std::vector<double> g(goff_tot, 0.0);
+  std::vector<double2> rs_plus(nrow), rs(nrow);
+  std::vector<std::vector<double2>> ff_plus(h->probs.size()), ff(h->probs.size());
+  for (int side = 0; side < 2; ++side)
+    for (int r = 0; r < 3; ++r)
+      for (int sg = 0; sg < 2; ++sg) {
+        const double sign = sg == 0 ? 1.0 : -1.0;
+        for (size_t pi = 0; pi < h->probs.size(); ++pi) {
+          Problem& p = h->probs[pi];
+          std::vector<double> nodes(p.nodes);
+          for (int q = 0; q < p.Nn; ++q)
+            if (adj_free(p, q, r, hstep[pi])) nodes[2 * q] += sign * hstep[pi];
+          std::vector<double> pts, elem;
+          geometry_tables(nodes.data(), p.Ne, pts, elem);
+          double* alt = p.d_alt;
+          HIPCHK(hipMemcpyAsync(alt, pts.data(), pts.size() * 8, hipMemcpyHostToDevice, h->stream));
+          HIPCHK(hipMemcpyAsync(alt + pts.size(), nodes.data(), nodes.size() * 8, hipMemcpyHostToDevice, h->stream));
+          HIPCHK(hipMemcpyAsync(alt + pts.size() + nodes.size(), elem.data(), elem.size() * 8, hipMemcpyHostToDevice,
+                                h->stream));
+          const Geom ag{alt, alt + pts.size(), alt + pts.size() + nodes.size()};
+          if ((rc = fill_problem(h, p, side == 0 ? ag : base_geom(p), side == 1 ? ag : base_geom(p)))) return rc;
+          if (side == 0) {
+            // explicit far-field term: X fixed, per-element partial sums on the moved points
+            hipLaunchKernelGGL((k_farfield_partial<double>), dim3((p.nrhs + 255) / 256, p.nmodes, p.Ne), dim3(256), 0,
+                               h->stream, excite_args(p, ag), h->d_sys + p.sys0, (const void*)h->d_Xs, 0, 4,
+                               h->d_ffpart, p.nmodes);
+            hipLaunchKernelGGL(k_adj_ffsum, dim3(p.Ne), dim3(256), 0, h->stream, h->d_ffpart, h->d_acoef + coff[pi],
+                               p.nmodes, p.nrhs, h->d_ffs);
+            LAUNCHCHK();
+            ff[pi].resize(p.Ne);
+            HIPCHK(hipMemcpyAsync(ff[pi].data(), h->d_ffs, p.Ne * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+          }
+        }
+        if (side == 0)
+          hipLaunchKernelGGL(k_adj_rowsum, dim3((unsigned)maxn, nsys), dim3(256), 0, h->stream, h->d_sys, h->d_rowoff,
+                             h->d_Q, (const double2*)h->d_A, h->d_rs);
+        else
+          hipLaunchKernelGGL(k_adj_colsum, dim3((unsigned)((maxn + 63) / 64), 1, nsys), dim3(64, 4), 0, h->stream,
+                             h->d_sys, h->d_rowoff, h->d_Q, (const double2*)h->d_A, h->d_rs);
+        LAUNCHCHK();
+        HIPCHK(hipMemcpyAsync(rs.data(), h->d_rs, nrow * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (sg == 0) {
+          rs_plus.swap(rs);
+          ff_plus.swap(ff);
+          continue;
+        }
+        // central differences, binned to the moved node of each row / column / element
+        size_t o = 0;
+        for (int si = 0; si < nsys; ++si) {
+          const SysInfo& s = h->sys[si];
+          const Problem& p = h->probs[s.prob];
+          const double inv2h = 0.5 / hstep[s.prob];
+          for (int i = 0; i < s.n; ++i) {
+            const int node = i < s.nt ? s.t0 + i : s.f0 + (i - s.nt);
+            for (int q = node - 1; q <= node + 1; ++q)
+              if (adj_free(p, q, r, hstep[s.prob])) g[goff[s.prob] + q] += (rs_plus[o + i].x - rs[o + i].x) * inv2h;
+          }
+          o += (size_t)s.n;
+        }
+        if (side == 0)
+          for (size_t pi = 0; pi < h->probs.size(); ++pi) {
+            const Problem& p = h->probs[pi];
+            const double inv2h = 0.5 / hstep[pi];
+            for (int e = 0; e < p.Ne; ++e)
+              for (int q = e; q <= e + 1; ++q)
+                if (adj_free(p, q, r, hstep[pi])) g[goff[pi] + q] += (ff_plus[pi][e].x - ff[pi][e].x) * inv2h;
+          }
+      }
+  std::copy(g.begin(), g.end(), grad);
+  h->solved = false;   // the systems now hold the last perturbed fill
+  h->profile = prof;
+  return 0;
 }

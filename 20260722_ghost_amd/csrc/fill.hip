@@ -26,7 +26,8 @@ namespace ghost {
 // ─────────────────────────────────────────────────────────────────────────────
 
 struct FarArgs {
-  PointSet pts;
+  PointSet pts;          // test points (rows)
+  PointSet spts;         // source points (columns); the same set except in the adjoint's perturbed fills
   const double* sin2E;   // [nE/2+1]  sin^2(pi j / nE)
   const double* cosE;    // [nE]      cos(2 pi i / nE)
   const double* cxB;     // [nB/2+1]  cos(2 pi j / nB)
@@ -118,13 +119,14 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
   const int c16 = lane & 15, kk = lane >> 4;
   const R k = (R)a.k;
   const PointSet& P = a.pts;
+  const PointSet& Q = a.spts;
 
   // operand lane: the point pair (g, h) whose result lands in lane group
   // kk = g, accumulator element r = h (row mrow(g, h) = c16)
   const bool f64 = sizeof(R) == 8;
   const int g_in = f64 ? (c16 & 3) : (c16 >> 2), h_in = f64 ? (c16 >> 2) : (c16 & 3);
   const int ip = 4 * e + g_in, iq = 4 * f + h_in;
-  const R rp = (R)P.rho[ip], zp = (R)P.z[ip], rq = (R)P.rho[iq], zq = (R)P.z[iq];
+  const R rp = (R)P.rho[ip], zp = (R)P.z[ip], rq = (R)Q.rho[iq], zq = (R)Q.z[iq];
   const R dz = zp - zq, dr = rp - rq;
   const R d2 = dr * dr + dz * dz, rr4 = (R)4 * rp * rq;
   const int uvsub = kk >> 1, asel = kk & 1;
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
     const R Twa[2] = {(R)P.T0[ig] * wrp, (R)P.T1[ig] * wrp};
     const R Swa[2] = {(R)(P.T0[ig] * P.w[ig]), (R)(P.T1[ig] * P.w[ig])};
     const R Dwa[2] = {(R)(P.D0[ig] * P.w[ig]), (R)(P.D1[ig] * P.w[ig])};
-    const R trq = (R)P.trho[4 * f], tzq = (R)P.tz[4 * f];
+    const R trq = (R)Q.trho[4 * f], tzq = (R)Q.tz[4 * f];
     const R ik2 = (R)1 / (k * k);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -174,10 +176,10 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
         const int iq2 = 4 * f + h;
-        const R wr = (R)(P.w[iq2] * P.rho[iq2]);
-        const R Twb[2] = {(R)P.T0[iq2] * wr, (R)P.T1[iq2] * wr};
-        const R Swb[2] = {(R)(P.T0[iq2] * P.w[iq2]), (R)(P.T1[iq2] * P.w[iq2])};
-        const R Dwb[2] = {(R)(P.D0[iq2] * P.w[iq2]), (R)(P.D1[iq2] * P.w[iq2])};
+        const R wr = (R)(Q.w[iq2] * Q.rho[iq2]);
+        const R Twb[2] = {(R)Q.T0[iq2] * wr, (R)Q.T1[iq2] * wr};
+        const R Swb[2] = {(R)(Q.T0[iq2] * Q.w[iq2]), (R)(Q.T1[iq2] * Q.w[iq2])};
+        const R Dwb[2] = {(R)(Q.D0[iq2] * Q.w[iq2]), (R)(Q.D1[iq2] * Q.w[iq2])};
         const R Gr = accR[t][h], Gi = accI[t][h];
         // neighbour orders by row rotation: lane c receives lane c-1 (ror 1) / c+1 (ror 15)
         R Pr = row_ror<15>(accR[t][h]), Pi = row_ror<15>(accI[t][h]);  // G_{n+1}
@@ -246,10 +248,10 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
 #pragma unroll
   for (int t = 0; t < NT; ++t) { accE[t] = V4{0, 0, 0, 0}; accO[t] = V4{0, 0, 0, 0}; }
   {
-    const R trp = (R)P.trho[ip], tzp = (R)P.tz[ip], trq = (R)P.trho[iq], tzq = (R)P.tz[iq];
-    const R wp = (R)(P.w[ip] * P.rho[ip]), wq = (R)(P.w[iq] * P.rho[iq]);
+    const R trp = (R)P.trho[ip], tzp = (R)P.tz[ip], trq = (R)Q.trho[iq], tzq = (R)Q.tz[iq];
+    const R wp = (R)(P.w[ip] * P.rho[ip]), wq = (R)(Q.w[iq] * Q.rho[iq]);
     const R Wa0 = (R)P.T0[ip] * wp, Wa1 = (R)P.T1[ip] * wp;
-    const R Wb0 = (R)P.T0[iq] * wq, Wb1 = (R)P.T1[iq] * wq;
+    const R Wb0 = (R)Q.T0[iq] * wq, Wb1 = (R)Q.T1[iq] * wq;
     const R Wab[4] = {Wa0 * Wb0, Wa0 * Wb1, Wa1 * Wb0, Wa1 * Wb1};
     const int hB = a.nB >> 1, mskB = a.nB - 1;
     const R scB = (R)1 / ((R)2 * (R)a.nB);  // (2 pi/N)/(4 pi)
@@ -347,10 +349,14 @@ struct NearPairInfo {
 };
 
 struct NearArgs {
-  const double* nodes;    // [Nn][2]
+  const double* nodes;    // [Nn][2]  test element geometry
   const double* L;        // [Ne]
   const double* trho;     // [Ne]
   const double* tz;       // [Ne]
+  const double* snodes;   // source element geometry (the same arrays except in the
+  const double* sL;       // adjoint's perturbed fills)
+  const double* strho;
+  const double* stz;
   const double* cell_s;   // concatenated cell points of the 3 kinds
   const double* cell_sp;
   const double* cell_w;
@@ -366,9 +372,9 @@ struct NearArgs {
   double2* NP;            // [pp][nord][5] : G, Ktt, Ktf, Kft, Kff
 };
 
-__device__ __forceinline__ void cell_geom(const NearArgs& a, int e, double s, double& rho, double& z) {
-  const double r0 = a.nodes[2 * e], z0 = a.nodes[2 * e + 1];
-  const double r1 = a.nodes[2 * e + 2], z1 = a.nodes[2 * e + 3];
+__device__ __forceinline__ void cell_geom(const double* nodes, int e, double s, double& rho, double& z) {
+  const double r0 = nodes[2 * e], z0 = nodes[2 * e + 1];
+  const double r1 = nodes[2 * e + 2], z1 = nodes[2 * e + 3];
   rho = r0 + s * (r1 - r0);
   z = z0 + s * (z1 - z0);
 }
@@ -386,9 +392,9 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
   const int qi = a.cell_off[pr.kind] + (valid ? q : 0);
   const double s = a.cell_s[qi], sp = a.cell_sp[qi];
   double rp, zp, rq, zq;
-  cell_geom(a, pr.e, s, rp, zp);
-  cell_geom(a, pr.f, sp, rq, zq);
-  const double trp = a.trho[pr.e], tzp = a.tz[pr.e], trq = a.trho[pr.f], tzq = a.tz[pr.f];
+  cell_geom(a.nodes, pr.e, s, rp, zp);
+  cell_geom(a.snodes, pr.f, sp, rq, zq);
+  const double trp = a.trho[pr.e], tzp = a.tz[pr.e], trq = a.strho[pr.f], tzq = a.stz[pr.f];
   const double k = a.k;
   const double d2 = (rp - rq) * (rp - rq) + (zp - zq) * (zp - zq);
   const double rr4 = 4.0 * rp * rq;
@@ -551,10 +557,10 @@ __global__ __launch_bounds__(128) void k_near_blocks(NearBlockArgs b) {
   const int am = threadIdx.x;
   if (am > a.M) return;
   const int e = pr.e, f = pr.f;
-  const double trp = a.trho[e], tzp = a.tz[e], trq = a.trho[f], tzq = a.tz[f];
-  const double Le = a.L[e], Lf = a.L[f];
+  const double trp = a.trho[e], tzp = a.tz[e], trq = a.strho[f], tzq = a.stz[f];
+  const double Le = a.L[e], Lf = a.sL[f];
   const double dre = a.nodes[2 * e + 2] - a.nodes[2 * e];
-  const double drf = a.nodes[2 * f + 2] - a.nodes[2 * f];
+  const double drf = a.snodes[2 * f + 2] - a.snodes[2 * f];
   const double k = a.k, ik2 = 1.0 / (k * k);
   const int cnt = a.cell_cnt[pr.kind], off = a.cell_off[pr.kind];
   double2 E[16], K[16];
@@ -565,8 +571,8 @@ __global__ __launch_bounds__(128) void k_near_blocks(NearBlockArgs b) {
     const double s = a.cell_s[off + q], sp = a.cell_sp[off + q];
     const double W = a.cell_w[off + q] * Le * Lf;
     double rp, zp, rq, zq;
-    cell_geom(a, e, s, rp, zp);
-    cell_geom(a, f, sp, rq, zq);
+    cell_geom(a.nodes, e, s, rp, zp);
+    cell_geom(a.snodes, f, sp, rq, zq);
     const double Tp[2] = {1.0 - s, s}, Tq[2] = {1.0 - sp, sp};
     const double Dp[2] = {(dre * (1.0 - s) - rp) / Le, (dre * s + rp) / Le};
     const double Dq[2] = {(drf * (1.0 - sp) - rq) / Lf, (drf * sp + rq) / Lf};
@@ -603,7 +609,8 @@ __global__ __launch_bounds__(128) void k_near_blocks(NearBlockArgs b) {
         }
       }
   }
-  // mass term 2 pi int rho T_a T_b dt on the element itself (diagonal pairs)
+  // mass term 2 pi int rho T_a T_b dt on the element itself (diagonal pairs),
+  // a single integral taken over the test geometry
   double Mab[4] = {0, 0, 0, 0};
   if (e == f) {
     for (int gq = 0; gq < 4; ++gq) {

@@ -150,6 +150,23 @@ const char* ghost_bor_kernel_name(int kind);
 /* Per family since set_profile: summed ms, launch count, algorithmic real flops. */
 int ghost_bor_kernel_times(ghost_bor* h, double* ms, int* count, double* flops);
 
+/* Adjoint of a real far-field objective J w.r.t. the radial coordinate of
+ * every generatrix node (no counterpart in the reference, whose solver has no
+ * gradient; it is the "adjoint solve + dZ/dp contraction" entry of the
+ * device-resident interface).  acoef: complex, per problem n_rhs entries in
+ * column order 2 theta_index + (0 VV, 1 HH), with dJ = Re sum_c acoef_c dF_c
+ * and F_c the mode-summed far-field amplitude (-j k eta0 / 4 pi included).
+ * grad: per problem Nn doubles, dJ/d rho_node (0 for nodes within 2 step of
+ * the axis).  Method: one transposed solve lambda = A^-T (a' dF/dX) through
+ * the LU factors of the last solve, Q = [-Lambda X^T | Lambda], and FP64
+ * central differences (half-width step in metres; 0 = 1e-6 of each
+ * problem's shortest element) of the fill, right-hand side
+ * and far-field projection with the quadrature rules of the build frozen,
+ * moving the test and the source geometry separately, one residue class of
+ * nodes mod 3 at a time (12 batched fills).  Needs a GHOST_FP64 handle after
+ * fill + solve; afterwards the systems hold a perturbed fill. */
+int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

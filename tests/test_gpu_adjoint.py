@@ -1,0 +1,74 @@
+"""Adjoint RCS gradient (HIP path) against central differences of the forward
+solve, on the GPU and through the FP64 oracle (run on an MI355X)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import bor_oracle as bo
+from conftest import pkg
+
+pytestmark = pytest.mark.gpu
+
+
+def cone_sphere_spec(n_elems, m_max, n_theta, alpha):
+    wl = pkg("workloads")
+    ad = pkg("bor_adjoint")
+    nodes = wl.cone_sphere(n_elems, 15.0, 0.3)
+    return ad.RcsSpec(nodes, 3e8, np.linspace(2.0, 178.0, n_theta), m_max, cfie_alpha=alpha)
+
+
+def fd_objective(spec, w, q, h):
+    ad = pkg("bor_adjoint")
+    vals = []
+    for sgn in (1.0, -1.0):
+        r = spec.nodes[:, 0].copy()
+        r[q] += sgn * h
+        F, hd = ad.solve_far_field(spec, r)
+        hd.close()
+        vals.append(float(np.sum(w * ad.sigma_db(F))))
+    return (vals[0] - vals[1]) / (2 * h)
+
+
+@pytest.mark.parametrize("alpha", [0.5, 1.0])
+def test_adjoint_gradient_matches_central_differences(alpha):
+    ad = pkg("bor_adjoint")
+    spec = cone_sphere_spec(40, 6, 9, alpha)
+    rng = np.random.default_rng(11)
+    w = rng.uniform(0.5, 1.5, size=(2, spec.thetas_deg.size))
+    x = torch.zeros(spec.nodes.shape[0], dtype=torch.float64, requires_grad=True)
+    s = ad.rcs_db(x, spec)
+    J = (s * torch.from_numpy(w)).sum()
+    J.backward()
+    g = x.grad.numpy()
+    print(f"[case] alpha={alpha} |g|max={np.max(np.abs(g)):.3e}", flush=True)
+    # the axis end nodes do not move
+    assert g[0] == 0.0 and g[-1] == 0.0
+    lam = 299_792_458.0 / spec.freq_hz
+    for q in (1, 2, 7, 13, 20, 27, 33, 38):
+        fd = fd_objective(spec, w, q, 1e-6 * lam)
+        print(f"[case] node {q}: adjoint {g[q]:.9e} fd {fd:.9e}", flush=True)
+        assert abs(g[q] - fd) <= 2e-7 * np.max(np.abs(g)) + 1e-6 * abs(fd), (q, g[q], fd)
+
+
+def test_adjoint_gradient_matches_oracle_differences():
+    """One node against central differences of the FP64 NumPy oracle: an
+    independent implementation of the whole forward path."""
+    ad = pkg("bor_adjoint")
+    spec = cone_sphere_spec(24, 4, 5, 0.5)
+    w = np.ones((2, spec.thetas_deg.size))
+    J, g = ad.objective_and_gradient(spec, w)
+    q = 11
+    h = 1e-5 * 299_792_458.0 / spec.freq_hz
+    vals = []
+    for sgn in (1.0, -1.0):
+        nodes = spec.nodes.copy()
+        nodes[q, 0] += sgn * h
+        out = bo.solve_bor(nodes, spec.freq_hz, spec.thetas_deg, formulation="cfie", n_modes=spec.m_max,
+                           mode_tol=0.0)
+        vals.append(float(np.sum(10 * np.log10(out["sigma_vv"])) + np.sum(10 * np.log10(out["sigma_hh"]))))
+    fd = (vals[0] - vals[1]) / (2 * h)
+    print(f"[case] oracle fd {fd:.9e} adjoint {g[q]:.9e}", flush=True)
+    assert abs(g[q] - fd) <= 1e-6 * abs(fd)
