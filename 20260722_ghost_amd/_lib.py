@@ -63,6 +63,7 @@ SIGNATURES = [
     ("ghost_bor_kernel_name", ctypes.c_char_p, [_ci]),
     ("ghost_bor_kernel_times", _ci, [_vp, _dp, _ip, _dp]),
     ("ghost_bor_adjoint", _ci, [_vp, _dp, _cd, _dp]),
+    ("ghost_copy_calibration", _ci, [_cd, _fp]),
 ]
 
 
@@ -108,6 +109,13 @@ def check(rc: int) -> None:
         if rc == 1:
             raise ValueError(msg)
         raise GhostLibraryError(f"libghost_bor error {rc}: {msg}")
+
+
+def copy_calibration(nbytes: float) -> float:
+    """Milliseconds of one plain device copy of nbytes (see ghost_copy_calibration)."""
+    ms = ctypes.c_float(0.0)
+    check(load().ghost_copy_calibration(float(nbytes), ctypes.byref(ms)))
+    return float(ms.value)
 
 
 def dptr(a: np.ndarray):

@@ -2016,3 +2016,41 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   h->profile = prof;
   return 0;
 }
+
+// ─────────────────────────────────────────────────────────────────────────────
+// measurement utility: a plain device copy of known size (byte-counter calibration)
+// ─────────────────────────────────────────────────────────────────────────────
+
+__global__ void k_copy_bytes(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+extern "C" int ghost_copy_calibration(double bytes, float* ms) {
+  if (!ms || !(bytes >= 16.0)) return fail(GHOST_ERR_ARG, "bytes must be >= 16");
+  const size_t n = (size_t)(bytes / sizeof(float4));
+  float4 *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  if (hipMalloc(&a, n * sizeof(float4)) != hipSuccess || hipMalloc(&b, n * sizeof(float4)) != hipSuccess) {
+    rc = fail(GHOST_ERR_OOM, "copy buffers");
+  } else if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+             hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(GHOST_ERR_HIP, "stream / events");
+  } else {
+    hipMemsetAsync(a, 1, n * sizeof(float4), s);
+    hipLaunchKernelGGL(k_copy_bytes, dim3(8192), dim3(256), 0, s, a, b, n);   // warm
+    hipEventRecord(e0, s);
+    hipLaunchKernelGGL(k_copy_bytes, dim3(8192), dim3(256), 0, s, a, b, n);
+    hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+      rc = fail(GHOST_ERR_HIP, "copy timing");
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (s) hipStreamDestroy(s);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
+}

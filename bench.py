@@ -145,6 +145,68 @@ def cpu_baseline(nodes, freq, thetas, m_max, budget_s=25.0):
     }
 
 
+def run_adjoint(steps):
+    """Config 4: forward RCS + adjoint gradient of J = sum sigma_dBsm over the
+    node radii (FP64), with central differences of the full solve on a few
+    nodes as the check.  Prints its own JSON line."""
+    import torch  # noqa: F401  (the autograd front end needs it importable)
+    wl = pkg("workloads")
+    ad = pkg("bor_adjoint")
+    name, nodes, freq, thetas, m_max = wl.config(4)
+    spec = ad.RcsSpec(nodes, freq, thetas, m_max, cfie_alpha=0.5)
+    log(f"adjoint: {name}, m_max={m_max}, aspects={len(thetas)}")
+    L = pkg("_lib")
+    # warm-up (module load, first build)
+    J, g = ad.objective_and_gradient(spec)
+    L.device_synchronize()
+    t_fwd, t_adj = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        F, h = ad.solve_far_field(spec, spec.nodes[:, 0])
+        L.device_synchronize()
+        t1 = time.perf_counter()
+        s = ad.sigma_db(F)
+        a = 20.0 / math.log(10.0) * np.conj(F) / np.abs(F) ** 2
+        g = h.adjoint(np.ascontiguousarray(a.T).ravel(), 0.0)
+        L.device_synchronize()
+        t2 = time.perf_counter()
+        h.close()
+        t_fwd.append(t1 - t0)
+        t_adj.append(t2 - t1)
+    log(f"adjoint: forward {min(t_fwd) * 1e3:.1f} ms, gradient {min(t_adj) * 1e3:.1f} ms")
+    lam = bench_c0() / freq
+    free = int(np.sum(spec.nodes[:, 0] > 1e-9))
+    errs = []
+    for q in (5, len(nodes) // 3, len(nodes) // 2, 2 * len(nodes) // 3, len(nodes) - 6):
+        vals = []
+        for sgn in (1.0, -1.0):
+            r = spec.nodes[:, 0].copy()
+            r[q] += sgn * 1e-6 * lam
+            Fq, hq = ad.solve_far_field(spec, r)
+            hq.close()
+            vals.append(float(np.sum(ad.sigma_db(Fq))))
+        fd = (vals[0] - vals[1]) / (2e-6 * lam)
+        errs.append(abs(g[q] - fd) / max(abs(fd), 1e-30))
+        log(f"adjoint: node {q} adjoint {g[q]:.9e} fd {fd:.9e}")
+    out = {
+        "metric": "RCS gradient (J = sum sigma_dBsm over aspects x pols) w.r.t. node radii, forward + adjoint",
+        "config": {"workload": name, "n_elems": int(len(nodes) - 1), "m_max": m_max, "modes": 2 * m_max + 1,
+                   "aspects": len(thetas), "polarizations": 2, "free_nodes": free, "precision": "fp64"},
+        "forward_ms": round(min(t_fwd) * 1e3, 3),
+        "adjoint_ms": round(min(t_adj) * 1e3, 3),
+        "central_difference_ms_estimate": round(2 * free * min(t_fwd) * 1e3, 1),
+        "speedup_vs_central_differences": round(2 * free * min(t_fwd) / (min(t_fwd) + min(t_adj)), 1),
+        "fd_check_max_rel_err": float(max(errs)),
+        "J": float(np.sum(s)),
+        "steps": steps,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_c0():
+    return pkg("bor_solver").C0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,11 +217,23 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="run one plain device copy of GIB GiB (byte-counter calibration) and exit")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="time forward + adjoint RCS gradient on the 600-element cone-sphere instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X only: multi-GPU runs are not part of this build "
               "(see DESIGN.md for how the sweep would shard).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        nbytes = args.copy_calibration * 2 ** 30
+        ms = pkg("_lib").copy_calibration(nbytes)
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms, "read_plus_write_GBps": 2 * nbytes / ms / 1e6}))
+        return
+    if args.adjoint:
+        run_adjoint(max(1, args.steps))
+        return
     wl = pkg("workloads")
     name, nodes, freq, thetas, m_max = wl.config(args.config)
     solves = (2 * m_max + 1) * len(thetas)
@@ -187,10 +261,16 @@ def main():
     roof = None
     if gemm_n > 0 and gemm_ms > 0:
         achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12
-        roof = {"kernel": "k_gemm_update<float> (complex FP32 MFMA 16x16x4, LU/solve updates)",
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "ogive1200_m70_mixed_hbm_traffic.json")
+        if args.config == 2 and os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f)["per_launch"]["gemm"]["hbm_bytes"]
+        roof = {"kernel": "k_gemm2<float,2,2> (complex FP32 MFMA 16x16x4, 64x64 tiles; LU updates and solves)",
                 "bound": "mfma", "achieved": round(achieved, 3), "peak": MI355X_PEAK["fp32_mfma_tflops"],
                 "unit": "TFLOP/s", "frac": round(achieved / MI355X_PEAK["fp32_mfma_tflops"], 4),
-                "traffic": None, "launches": gemm_n, "avg_launch_ms": round(gemm_ms / gemm_n, 4),
+                "traffic": traffic, "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
+                "launches": gemm_n, "avg_launch_ms": round(gemm_ms / gemm_n, 4),
                 "flops_per_launch": gemm_fl / gemm_n}
     out = {
         "metric": "BOR monostatic solves/s (mode x freq x angle) at <=0.01 dB vs FP64; MFMA/HBM %",

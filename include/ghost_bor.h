@@ -167,6 +167,11 @@ int ghost_bor_kernel_times(ghost_bor* h, double* ms, int* count, double* flops);
  * fill + solve; afterwards the systems hold a perturbed fill. */
 int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad);
 
+/* Measurement utility (no counterpart in the reference): one plain device
+ * copy of `bytes` bytes (read once, written once) on its own stream, *ms its
+ * duration; the known byte count calibrates HBM byte counters. */
+int ghost_copy_calibration(double bytes, float* ms);
+
 #ifdef __cplusplus
 }
 #endif
