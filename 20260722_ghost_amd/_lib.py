@@ -64,6 +64,13 @@ SIGNATURES = [
     ("ghost_bor_kernel_times", _ci, [_vp, _dp, _ip, _dp]),
     ("ghost_bor_adjoint", _ci, [_vp, _dp, _cd, _dp]),
     ("ghost_copy_calibration", _ci, [_cd, _fp]),
+    ("ghost_bor_add_dense", _ci, [_vp, _ci, _ci, _ip]),
+    ("ghost_bor_set_dense", _ci, [_vp, _ci, _dp, _dp]),
+    ("ghost_bor_get_dense", _ci, [_vp, _ci, _dp]),
+    ("ghost_bor_fill_rhs", _ci, [_vp]),
+    ("ghost_bor_set_solution", _ci, [_vp, _ci, _ci, _dp]),
+    ("ghost_bor_system_count", _ci, [_vp, _ip]),
+    ("ghost_bor_set_lu_super", _ci, [_vp, _ci]),
 ]
 
 
@@ -137,6 +144,7 @@ class Handle:
         self.precision = precision
         self.problems = []
         self.nodes_per_problem = []
+        self.dense = {}
 
     def close(self):
         if self._h:
@@ -162,6 +170,36 @@ class Handle:
         self.nodes_per_problem.append(int(nodes.shape[0]))
         return idx.value
 
+    def add_dense(self, n: int, nrhs: int) -> int:
+        """A caller-filled n x n system with nrhs columns; returns its system index."""
+        idx = ctypes.c_int(-1)
+        check(self.lib.ghost_bor_add_dense(self._h, int(n), int(nrhs), ctypes.byref(idx)))
+        self.dense[idx.value] = (int(n), int(nrhs))
+        return idx.value
+
+    def set_dense(self, idx: int, A, B):
+        n, nrhs = self.dense[idx]
+        A = np.ascontiguousarray(A, dtype=np.complex128)
+        B = np.ascontiguousarray(B, dtype=np.complex128).reshape(n, nrhs)
+        if A.shape != (n, n):
+            raise ValueError(f"A must be {n} x {n}")
+        check(self.lib.ghost_bor_set_dense(self._h, int(idx), A.ctypes.data_as(_dp), B.ctypes.data_as(_dp)))
+
+    def get_dense(self, idx: int) -> np.ndarray:
+        n, nrhs = self.dense[idx]
+        X = np.zeros((n, nrhs), dtype=np.complex128)
+        check(self.lib.ghost_bor_get_dense(self._h, int(idx), X.ctypes.data_as(_dp)))
+        return X
+
+    def fill_rhs(self):
+        check(self.lib.ghost_bor_fill_rhs(self._h))
+
+    def set_solution(self, prob: int, m: int, X):
+        n = self.system_info(prob, m)[0]
+        nrhs = 2 * self.problems[prob][1]
+        X = np.ascontiguousarray(X, dtype=np.complex128).reshape(n, nrhs)
+        check(self.lib.ghost_bor_set_solution(self._h, int(prob), int(m), X.ctypes.data_as(_dp)))
+
     def build(self):
         check(self.lib.ghost_bor_build(self._h))
 
@@ -170,6 +208,9 @@ class Handle:
 
     def solve(self, refine_iters: int = 0):
         check(self.lib.ghost_bor_solve(self._h, int(refine_iters)))
+
+    def set_lu_super(self, panels: int):
+        check(self.lib.ghost_bor_set_lu_super(self._h, int(panels)))
 
     def set_gemm_variant(self, v: int):
         check(self.lib.ghost_bor_set_gemm_variant(self._h, int(v)))
@@ -214,9 +255,15 @@ class Handle:
         check(self.lib.ghost_bor_adjoint(self._h, dptr(ab), float(step), dptr(out)))
         return out
 
+    def num_systems(self) -> int:
+        """Mode systems of every problem plus dense systems (the library's system order)."""
+        n = ctypes.c_int(0)
+        check(self.lib.ghost_bor_system_count(self._h, ctypes.byref(n)))
+        return n.value
+
     def residuals(self) -> np.ndarray:
-        n = sum(2 * m + 1 for m, _ in self.problems)
-        out = np.zeros(n, dtype=np.float64)
+        """||A X - B|| / ||B|| per system, in the order systems were added."""
+        out = np.zeros(self.num_systems(), dtype=np.float64)
         check(self.lib.ghost_bor_residual(self._h, dptr(out)))
         return out
 
@@ -243,8 +290,7 @@ class Handle:
         return X[..., 0] + 1j * X[..., 1]
 
     def zero_pivots(self) -> np.ndarray:
-        n = sum(2 * m + 1 for m, _ in self.problems)
-        out = np.zeros(n, dtype=np.int32)
+        out = np.zeros(self.num_systems(), dtype=np.int32)
         check(self.lib.ghost_bor_info(self._h, iptr(out)))
         return out
 

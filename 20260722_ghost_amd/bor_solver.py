@@ -284,3 +284,267 @@ def solve_bor_pec(points, freq_hz: float, thetas_deg, n_modes: Optional[int] = N
     """PEC EFIE entry point (bor_solver.py:1007-1012)."""
     return solve_bor(points, freq_hz, thetas_deg, formulation="efie", n_modes=n_modes,
                      gauss_order=gauss_order, mode_tol=mode_tol)
+
+
+# ─────────────────────────────────────────────────────────────────────────────
+# generic mode loop (bor_solver.py:711-801) on the batched device solve
+# ─────────────────────────────────────────────────────────────────────────────
+
+def _mode_sweep(n_dofs: int, thetas, pols, m_max: int, mode_tol: float,
+                assemble: Callable, rhs: Callable, farfield: Callable,
+                prepare: Optional[Callable] = None, workers: int = 1,
+                progress: Optional[Callable] = None,
+                check_abort: Optional[Callable] = None,
+                monitor_cond: bool = False):
+    """The reference's closure-driven adaptive mode loop with the same
+    contract (bor_solver.py:711-735): assemble(m) -> (A, mask), rhs(m, theta,
+    pol) -> V (full, or reduced when mask is None), farfield(m, sol, theta,
+    pol) -> complex.  The dense solves of a wave of |m| go to the device as
+    one batched LU (ghost_bor_add_dense / set_dense / solve) in place of
+    np.linalg.solve per mode; the accumulation and the two-quiet-modes stop
+    run in mode order as in the reference, so F, modes_used and the linear
+    residual follow the same rule.  A wave holds max(workers, 8) values of
+    |m|.  monitor_cond evaluates numpy.linalg.cond of each assembled system
+    on the host (a diagnostic, as in the reference)."""
+    thetas = np.atleast_1d(np.asarray(thetas, dtype=float))
+    pols = list(pols)
+    F = np.zeros((len(pols), len(thetas)), dtype=np.complex128)
+    if prepare is not None:
+        prepare(m_max)
+    wave_n = max(8, int(workers))
+    modes_used, quiet, am, max_res = 0, 0, 0, 0.0
+    conds: List[float] = []
+    done = False
+    while am <= m_max and not done:
+        if check_abort is not None:
+            check_abort()
+        wave = list(range(am, min(am + wave_n, m_max + 1)))
+        systems = []
+        for w_am in wave:
+            for m in ([0] if w_am == 0 else [w_am, -w_am]):
+                A, mask = assemble(m)
+                cols = [rhs(m, th, pol) if mask is None else np.asarray(rhs(m, th, pol))[mask]
+                        for th in thetas for pol in pols]
+                systems.append((w_am, m, np.asarray(A, dtype=np.complex128), np.stack(cols, axis=1), mask))
+        h = _lib.Handle(_lib.GHOST_FP64, residual=True)
+        try:
+            idx = [h.add_dense(A.shape[0], B.shape[1]) for (_, _, A, B, _) in systems]
+            h.build()
+            for i, (_, _, A, B, _) in zip(idx, systems):
+                h.set_dense(i, A, B)
+            h.solve(0)
+            h.sync()
+            res = h.residuals()
+            sols = [h.get_dense(i) for i in idx]
+        finally:
+            h.close()
+        dF_am = {w: np.zeros_like(F) for w in wave}
+        res_am = {w: 0.0 for w in wave}
+        cond_am = {w: 0.0 for w in wave}
+        for (w_am, m, A, B, mask), X, r in zip(systems, sols, res):
+            if float(np.linalg.norm(B)) > 0.0:
+                res_am[w_am] = max(res_am[w_am], float(r))
+            if monitor_cond:
+                cond_am[w_am] = max(cond_am[w_am], float(np.linalg.cond(A)))
+            ci = 0
+            for it, th in enumerate(thetas):
+                for ip, pol in enumerate(pols):
+                    if mask is None:
+                        sol = X[:, ci]
+                    else:
+                        sol = np.zeros(n_dofs, dtype=np.complex128)
+                        sol[mask] = X[:, ci]
+                    ci += 1
+                    dF_am[w_am][ip, it] += farfield(m, sol, th, pol)
+        for w_am in wave:
+            dF = dF_am[w_am]
+            F += dF
+            max_res = max(max_res, res_am[w_am])
+            if monitor_cond:
+                conds.append(cond_am[w_am])
+            modes_used = w_am
+            scale = max(float(np.max(np.abs(F))), 1e-30)
+            if float(np.max(np.abs(dF))) < mode_tol * scale:
+                quiet += 1
+                if quiet >= 2:
+                    done = True
+                    break
+            else:
+                quiet = 0
+        am = wave[-1] + 1
+        if progress is not None:
+            progress(modes_used, m_max)
+    stats = {"linear_residual": max_res}
+    if monitor_cond and conds:
+        stats["max_cond"] = max(conds)
+        stats["median_cond"] = float(np.median(conds))
+    return F, modes_used, stats
+
+
+# ─────────────────────────────────────────────────────────────────────────────
+# per-mode operator factory (bor_solver.py:175-708, PEC in air)
+# ─────────────────────────────────────────────────────────────────────────────
+
+class BorPecSolver:
+    """The reference's single-surface operator factory with its per-mode
+    methods, each answered by the device: every assemble_* fills all modes
+    |m| <= m_max of one formulation in one batch (unmasked node-based
+    [2Nn, 2Nn] systems, cached per m_max), rhs_* run the excitation kernels
+    for one aspect, farfield_mode runs the far-field kernels on a caller's
+    solution.  Penetrable media (medium=..., assemble_pmchwt_P) and surface
+    impedance (assemble_ibc_extra, zs_pt) are outside this build and raise
+    NotImplementedError."""
+
+    def __init__(self, points, freq_hz: float, gauss_order: int = 4,
+                 near_depth: int = 4, medium=None, single_tables: bool = False):
+        if medium is not None:
+            raise NotImplementedError("penetrable media (PMCHWT) are not part of the GPU BoR path.")
+        if int(gauss_order) != 4 or int(near_depth) != 4:
+            raise NotImplementedError("the GPU BoR path is built for gauss_order = 4 and near_depth = 4.")
+        self.pts = _validate_nodes(points)
+        self.freq_hz = float(freq_hz)
+        self.k = 2.0 * math.pi * self.freq_hz / C0
+        self.eta = ETA0
+        self.gauss_order = 4
+        self.near_depth = 4
+        self.Nn = int(self.pts.shape[0])
+        self.P = 4 * (self.Nn - 1)
+        self._m_max = None
+        self._ops: Dict = {}
+        self._rhs: Dict = {}
+
+    # set-up (the device fill has one path: both are hints)
+    def enable_streaming(self, m_max: int, efie: bool = True, mfie: bool = False,
+                         ibc_zs_pt=None, single_blocks: bool = False, tile_budget_gb: float = 1.0,
+                         workers: int = 1, mode_block=None) -> None:
+        if ibc_zs_pt is not None:
+            raise NotImplementedError("Surface impedance (IBC-EFIE) is not part of the GPU BoR path.")
+        self._m_max = max(int(m_max), self._m_max or 0)
+
+    def prepare_operators(self, m_max: int, efie: bool = True, mfie: bool = False,
+                          ibc: bool = False, workers: int = 1) -> None:
+        if ibc:
+            raise NotImplementedError("Surface impedance (IBC-EFIE) is not part of the GPU BoR path.")
+        if efie:
+            self._systems("efie", m_max)
+        if mfie:
+            self._systems("mfie", m_max)
+
+    def _n_xi(self, m_max: int):
+        osc = 2.0 * abs(self.k) * gauss_rho_max(self.pts)
+        return n_xi_efie(osc, m_max), n_xi_bracket(osc, m_max)
+
+    def _systems(self, kind: str, m_max: int) -> Dict[int, np.ndarray]:
+        key = (kind, int(m_max))
+        if key not in self._ops:
+            ne, nb = self._n_xi(m_max)
+            cE, cM = (1.0, 0.0) if kind == "efie" else (0.0, 1.0)
+            h = _lib.Handle(_lib.GHOST_FP64, residual=False)
+            try:
+                h.add_problem(self.pts, self.k, m_max, ne, nb, cE, cM, [0.0], unmasked=True)
+                h.build()
+                h.fill()
+                h.sync()
+                self._ops[key] = {m: h.system(0, m)[0] for m in range(-m_max, m_max + 1)}
+            finally:
+                h.close()
+        return self._ops[key]
+
+    def assemble_mode(self, m: int, m_max: int) -> np.ndarray:
+        """EFIE [2Nn, 2Nn] with C = j k eta 2 pi (bor_solver.py:293-345)."""
+        return self._systems("efie", m_max)[int(m)].copy()
+
+    def assemble_mfie_mode(self, m: int, m_max: int) -> np.ndarray:
+        """Z_MFIE = (1/2) M - K [2Nn, 2Nn] (bor_solver.py:399-435)."""
+        return self._systems("mfie", m_max)[int(m)].copy()
+
+    def assemble_ibc_extra(self, *args, **kwargs):
+        raise NotImplementedError("Surface impedance (IBC-EFIE) is not part of the GPU BoR path.")
+
+    def assemble_pmchwt_P(self, *args, **kwargs):
+        raise NotImplementedError("penetrable media (PMCHWT) are not part of the GPU BoR path.")
+
+    def mass_blocks(self, weight=None) -> np.ndarray:
+        """2 pi int w rho T_i T_j dt, node-based [Nn, Nn] (bor_solver.py:389-397):
+        tridiagonal host arithmetic on the order-4 Gauss points."""
+        x, w = np.polynomial.legendre.leggauss(4)
+        s = 0.5 * (x + 1.0)
+        p0, p1 = self.pts[:-1], self.pts[1:]
+        L = np.hypot(*(p1 - p0).T)
+        rho = p0[:, 0, None] + s[None, :] * (p1[:, 0] - p0[:, 0])[:, None]
+        wt = np.ones(self.P) if weight is None else np.asarray(weight).reshape(self.P)
+        K = (0.5 * w[None, :] * L[:, None] * rho).ravel() * wt
+        T = np.stack([np.tile(1.0 - s, self.Nn - 1), np.tile(s, self.Nn - 1)])
+        e = np.repeat(np.arange(self.Nn - 1), 4)
+        M = np.zeros((self.Nn, self.Nn), dtype=np.result_type(K, float))
+        for a in range(2):
+            for b in range(2):
+                np.add.at(M, (e + a, e + b), 2.0 * np.pi * K * T[a] * T[b])
+        return M
+
+    def basis_mask(self, m: int) -> np.ndarray:
+        """Active bases of mode m (bor_solver.py:611-622)."""
+        tol = AXIS_TOL * max(1.0, float(np.max(self.pts[:, 0])))
+        t_act = np.ones(self.Nn, dtype=bool)
+        f_act = np.ones(self.Nn, dtype=bool)
+        for end in (0, self.Nn - 1):
+            if self.pts[end, 0] <= tol:
+                t_act[end] = abs(m) == 1
+                f_act[end] = False
+            else:
+                t_act[end] = False
+                f_act[end] = True
+        return np.concatenate([t_act, f_act])
+
+    def _rhs_all(self, kind: str, m: int, theta_inc_deg: float) -> np.ndarray:
+        key = (kind, float(theta_inc_deg))
+        cache = self._rhs.get(key)
+        if cache is None or abs(m) > cache[0]:
+            mm = max(abs(int(m)), 1, cache[0] if cache else 0)
+            cE, cM = (1.0, 0.0) if kind == "efie" else (0.0, 1.0)
+            h = _lib.Handle(_lib.GHOST_FP64, residual=False)
+            try:
+                h.add_problem(self.pts, self.k, mm, 64, 128, cE, cM, [float(theta_inc_deg)], unmasked=True)
+                h.build()
+                h.fill_rhs()
+                h.sync()
+                cache = (mm, {q: h.system(0, q)[1] for q in range(-mm, mm + 1)})
+            finally:
+                h.close()
+            self._rhs[key] = cache
+        return cache[1][int(m)]
+
+    def rhs_mode(self, m: int, theta_inc_deg: float, pol: str) -> np.ndarray:
+        """<W, E_inc> for the plane wave from theta (bor_solver.py:625-645), [2Nn]."""
+        col = 0 if pol.upper() in ("VV", "THETA", "TM") else 1
+        return self._rhs_all("efie", m, theta_inc_deg)[:, col].copy()
+
+    def rhs_mfie_mode(self, m: int, theta_inc_deg: float, pol: str) -> np.ndarray:
+        """<W, n x H_inc> (bor_solver.py:437-461), [2Nn]."""
+        col = 0 if pol.upper() in ("VV", "THETA", "TM") else 1
+        return self._rhs_all("mfie", m, theta_inc_deg)[:, col].copy()
+
+    def rhs_h_mode(self, m: int, theta_inc_deg: float, pol: str) -> np.ndarray:
+        """Unrotated <W, H_inc> (bor_solver.py:647-657)."""
+        if pol.upper() in ("VV", "THETA", "TM"):
+            return -self.rhs_mode(m, theta_inc_deg, "HH") / ETA0
+        return self.rhs_mode(m, theta_inc_deg, "VV") / ETA0
+
+    def farfield_mode(self, m: int, sol: np.ndarray, theta_s_deg: float, zs_pt=None, msol=None):
+        """(F_theta, F_phi) of one mode's full [2Nn] solution (bor_solver.py:659-708)."""
+        if zs_pt is not None or msol is not None:
+            raise NotImplementedError("magnetic currents (IBC / PMCHWT) are not part of the GPU BoR path.")
+        sol = np.asarray(sol, dtype=np.complex128).reshape(2 * self.Nn)
+        mm = max(abs(int(m)), 1)
+        h = _lib.Handle(_lib.GHOST_FP64, residual=False)
+        try:
+            h.add_problem(self.pts, self.k, mm, 64, 128, 1.0, 0.0, [float(theta_s_deg)], unmasked=True)
+            h.build()
+            h.fill_rhs()   # the far field reads the Bessel table of the aspect
+            h.set_solution(0, int(m), np.stack([sol, sol], axis=1))
+            h.farfield()
+            h.sync()
+            dF = h.dF(0)[int(m) + mm]
+        finally:
+            h.close()
+        return complex(dF[0, 0]), complex(dF[1, 0])

@@ -284,7 +284,8 @@ struct ghost_bor {
   hipGraphExec_t gexec = nullptr;
   int refine_iters = 0;
   double probe_tol = 1e-5;
-  int gemm_variant = 1;              // see gemm_launch (1 measured fastest for n ~ 2400, both precisions)
+  int gemm_variant = -1;             // see gemm_launch; -1: 1 for FP32, 4 for FP64 (measured at n ~ 2400)
+  int lu_super = 4;                  // panels per LU super-block (trailing update K = 64 x lu_super)
   std::vector<DevBuf> allocs;
   // optional per-launch profile
   bool profile = false;
@@ -358,6 +359,7 @@ extern "C" int ghost_bor_destroy(ghost_bor* h) {
   if (h->graph) hipGraphDestroy(h->graph);
   for (auto& a : h->allocs) hipFree(a.p);
   for (int i = 0; i < 8; ++i) hipEventDestroy(h->ev[i]);
+
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -436,6 +438,25 @@ extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, 
   }
   if (idx) *idx = (int)h->probs.size();
   h->probs.push_back(std::move(p));
+  return 0;
+}
+
+// A caller-filled dense system (n x n, n_rhs columns) solved in the same batch
+// as the mode systems; it belongs to no problem (no fill, no far field).
+extern "C" int ghost_bor_add_dense(ghost_bor* h, int n, int nrhs, int* sys_idx) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "add systems before build");
+  if (n <= 0 || nrhs <= 0) return fail(GHOST_ERR_ARG, "n and nrhs must be positive");
+  if (n > 4096) return fail(GHOST_ERR_ARG, "system size %d exceeds the panel limit 4096", n);
+  SysInfo s;
+  memset(&s, 0, sizeof(s));
+  s.prob = -1;
+  s.m = 0;
+  s.n = n;
+  s.nt = n;
+  s.nrhs = nrhs;
+  if (sys_idx) *sys_idx = (int)h->sys.size();
+  h->sys.push_back(s);
   return 0;
 }
 
@@ -698,45 +719,70 @@ static int build_schedule(ghost_bor* h) {
     }
   };
 
-  // ── LU steps on the augmented systems ──
-  for (int b = 0; b < nblk_max; ++b) {
-    const int k = b * nb;
-    Launch lp{L_PANEL, (int)h->h_panel.size(), nsys, nsys, h->rpt};
-    Launch ls{L_SWAP, (int)h->h_swap.size(), nsys, 0, 0};
-    Launch lt{L_TRIL, (int)h->h_trsm.size(), nsys, nsys, 0};
-    Launch g1{L_GEMM_AUG_W, (int)h->h_gemm.size(), nsys, 0, 0};
+  // ── LU steps on the augmented systems, in super-blocks of S panels ──
+  // Inside a super-block starting at column k0, panel b (columns [k, k+w))
+  // first catches up on the super-block's earlier panels (rows [k, n), K =
+  // k - k0), is factored and its interchanges applied, then its row block
+  // catches up on the right (rows [k, k+w), columns [k+w, ncols)) and is
+  // multiplied by L11^-1.  One trailing update with K = k1 - k0 then covers
+  // rows and columns beyond the super-block: the trailing matrix is read and
+  // written once per S panels instead of once per panel.
+  const int S = std::max(1, h->lu_super);
+  auto lu_gemm = [&](int kind, auto make) {
+    Launch g{kind, (int)h->h_gemm.size(), nsys, 0, 0};
     for (int si = 0; si < nsys; ++si) {
-      const SysInfo& s = h->sys[si];
-      const int w = k < s.n ? std::min(nb, s.n - k) : 0;
-      const int ncols = s.n + s.nrhs;
-      h->h_panel.push_back(PanelOp{s.a_off, s.n, s.ld, k, w, 0});
-      SwapOp so{s.a_off, s.ld, k, w, k, k + w, ncols, 0};
-      h->h_swap.push_back(so);
-      ls.gx = std::max(ls.gx, (k + (ncols - k - w) + 255) / 256);
-      h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wl(si, b), s.ld, w});
-      // X = L11^{-1} X on rows [k, k+w), cols [k+w, ncols)
-      GemmOp go{wl(si, b), s.a_off + (long long)k * s.ld + k + w, s.a_off + (long long)k * s.ld + k + w,
-                64, s.ld, s.ld, w, ncols - k - w, w};
-      if (w == 0) go.M = 0;
+      GemmOp go = make(h->sys[si]);
+      if (go.M <= 0 || go.N <= 0 || go.K <= 0) go.M = go.N = go.K = 0;
       h->h_gemm.push_back(go);
-      g1.gx = std::max(g1.gx, tiles(go.M, go.N));
+      g.gx = std::max(g.gx, tiles(go.M, go.N));
     }
-    Launch g2{L_GEMM_AUG, (int)h->h_gemm.size(), nsys, 0, 0};
-    for (int si = 0; si < nsys; ++si) {
-      const SysInfo& s = h->sys[si];
-      const int w = k < s.n ? std::min(nb, s.n - k) : 0;
-      const int ncols = s.n + s.nrhs;
-      GemmOp go{s.a_off + (long long)(k + w) * s.ld + k, s.a_off + (long long)k * s.ld + k + w,
-                s.a_off + (long long)(k + w) * s.ld + k + w, s.ld, s.ld, s.ld, s.n - k - w, ncols - k - w, w};
-      if (w == 0) go.M = 0;
-      h->h_gemm.push_back(go);
-      g2.gx = std::max(g2.gx, tiles(go.M, go.N));
+    h->lu_sched.push_back(g);
+  };
+  for (int b0 = 0; b0 < nblk_max; b0 += S) {
+    const int k0 = b0 * nb, nsb = std::min(S, nblk_max - b0);
+    for (int sb = 0; sb < nsb; ++sb) {
+      const int b = b0 + sb, k = b * nb;
+      if (sb > 0)   // panel catch-up: A[k:n, k:k+w] -= L[k:n, k0:k] U[k0:k, k:k+w]
+        lu_gemm(L_GEMM_AUG, [&](const SysInfo& s) {
+          const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+          return GemmOp{s.a_off + (long long)k * s.ld + k0, s.a_off + (long long)k0 * s.ld + k,
+                        s.a_off + (long long)k * s.ld + k, s.ld, s.ld, s.ld, s.n - k, w, k - k0};
+        });
+      Launch lp{L_PANEL, (int)h->h_panel.size(), nsys, nsys, h->rpt};
+      Launch ls{L_SWAP, (int)h->h_swap.size(), nsys, 0, 0};
+      Launch lt{L_TRIL, (int)h->h_trsm.size(), nsys, nsys, 0};
+      for (int si = 0; si < nsys; ++si) {
+        const SysInfo& s = h->sys[si];
+        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        const int ncols = s.n + s.nrhs;
+        h->h_panel.push_back(PanelOp{s.a_off, s.n, s.ld, k, w, 0});
+        h->h_swap.push_back(SwapOp{s.a_off, s.ld, k, w, k, k + w, ncols, 0});
+        ls.gx = std::max(ls.gx, (k + (ncols - k - w) + 255) / 256);
+        h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wl(si, b), s.ld, w});
+      }
+      h->lu_sched.push_back(lp);
+      h->lu_sched.push_back(ls);
+      h->lu_sched.push_back(lt);
+      if (sb > 0)   // row catch-up: A[k:k+w, k+w:] -= L[k:k+w, k0:k] U[k0:k, k+w:]
+        lu_gemm(L_GEMM_AUG, [&](const SysInfo& s) {
+          const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+          return GemmOp{s.a_off + (long long)k * s.ld + k0, s.a_off + (long long)k0 * s.ld + k + w,
+                        s.a_off + (long long)k * s.ld + k + w, s.ld, s.ld, s.ld, w, s.n + s.nrhs - k - w,
+                        k - k0};
+        });
+      // X = L11^-1 X on rows [k, k+w), columns [k+w, ncols)
+      lu_gemm(L_GEMM_AUG_W, [&](const SysInfo& s) {
+        const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        return GemmOp{wl((int)(&s - h->sys.data()), b), s.a_off + (long long)k * s.ld + k + w,
+                      s.a_off + (long long)k * s.ld + k + w, 64, s.ld, s.ld, w, s.n + s.nrhs - k - w, w};
+      });
     }
-    h->lu_sched.push_back(lp);
-    h->lu_sched.push_back(ls);
-    h->lu_sched.push_back(lt);
-    h->lu_sched.push_back(g1);
-    h->lu_sched.push_back(g2);
+    // trailing update: A[k1:n, k1:] -= L[k1:n, k0:k1] U[k0:k1, k1:]
+    const int k1 = k0 + nsb * nb;
+    lu_gemm(L_GEMM_AUG, [&](const SysInfo& s) {
+      return GemmOp{s.a_off + (long long)k1 * s.ld + k0, s.a_off + (long long)k0 * s.ld + k1,
+                    s.a_off + (long long)k1 * s.ld + k1, s.ld, s.ld, s.ld, s.n - k1, s.n + s.nrhs - k1, k1 - k0};
+    });
   }
   // ── upper diagonal-block inverses (all blocks, one launch) ──
   h->triu_ops = (int)h->h_trsm.size();
@@ -923,7 +969,7 @@ static int build_schedule(ghost_bor* h) {
 extern "C" int ghost_bor_build(ghost_bor* h) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
   if (h->built) return fail(GHOST_ERR_STATE, "already built");
-  if (h->probs.empty()) return fail(GHOST_ERR_STATE, "no problems added");
+  if (h->sys.empty()) return fail(GHOST_ERR_STATE, "no problems or dense systems added");
   HIPCHK(hipSetDevice(h->device));
   const bool mixed = h->precision == GHOST_MIXED;
   // system offsets
@@ -1020,6 +1066,7 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
   FarArgs fa;
   fa.pts = point_set(tg.pts, p.P);
   fa.spts = point_set(sg.pts, p.P);
+  const bool split = tg.pts != sg.pts;   // only the adjoint's perturbed fills
   fa.sin2E = p.d_tabs + p.tab_off[0];
   fa.cosE = p.d_tabs + p.tab_off[1];
   fa.cxB = p.d_tabs + p.tab_off[2];
@@ -1051,7 +1098,11 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
       const size_t tab_bytes = sizeof(R) * (size_t)(p.nE / 2 + 1 + p.nE + 2 * (p.nB / 2 + 1) + 2 * p.nB);
       prof_begin(h);
       switch (NT) {
-#define FARCASE(n) case n: hipLaunchKernelGGL((k_far_fill<R, n>), dim3(grid), dim3(128), tab_bytes, h->stream, fa); break;
+#define FARCASE(n)                                                                                      \
+  case n:                                                                                               \
+    if (split) hipLaunchKernelGGL((k_far_fill<R, n, true>), dim3(grid), dim3(128), tab_bytes, h->stream, fa); \
+    else hipLaunchKernelGGL((k_far_fill<R, n, false>), dim3(grid), dim3(128), tab_bytes, h->stream, fa); \
+    break;
         FARCASE(1) FARCASE(2) FARCASE(3) FARCASE(4) FARCASE(5) FARCASE(6) FARCASE(7) FARCASE(8)
 #undef FARCASE
         default: return fail(GHOST_ERR_ARG, "m_max too large for the far fill");
@@ -1061,6 +1112,8 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
     }
   return 0;
 }
+
+static int rhs_problem(ghost_bor* h, Problem& p, const Geom& tg);
 
 static ExciteArgs excite_args(const Problem& p, const Geom& tg) {
   ExciteArgs ea;
@@ -1080,6 +1133,7 @@ static ExciteArgs excite_args(const Problem& p, const Geom& tg) {
 static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg) {
   const bool mixed = h->precision == GHOST_MIXED;
   const int P = p.P;
+  if ((p.M + 2 + 7) / 8 > 16 || p.M + 1 > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near band");
   const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.nmodes;
   const size_t zero_n = mixed ? (zel + 1) / 2 : zel;  // in double2 units
   prof_begin(h);
@@ -1089,6 +1143,7 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
 
   LAUNCHCHK();
   // near stage 1 + 2 (FP64)
+  hipStream_t ns = h->stream;
   NearArgs na;
   na.nodes = tg.nodes;
   na.L = tg.elem;
@@ -1119,7 +1174,7 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
     const dim3 g((unsigned)p.batches.size());
     prof_begin(h);
     switch (C) {
-#define NEARCASE(c) case c: hipLaunchKernelGGL((k_near_modal<c>), g, dim3(256), 0, h->stream, na); break;
+#define NEARCASE(c) case c: hipLaunchKernelGGL((k_near_modal<c>), g, dim3(256), 0, ns, na); break;
       NEARCASE(1) NEARCASE(2) NEARCASE(3) NEARCASE(4) NEARCASE(5) NEARCASE(6) NEARCASE(7) NEARCASE(8)
       NEARCASE(9) NEARCASE(10) NEARCASE(11) NEARCASE(12) NEARCASE(13) NEARCASE(14) NEARCASE(15) NEARCASE(16)
 #undef NEARCASE
@@ -1141,13 +1196,13 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   const int nthr = ((p.M + 1 + 63) / 64) * 64;
   if (nthr > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near blocks");
   prof_begin(h);
-  hipLaunchKernelGGL(k_near_blocks, dim3(3 * p.Ne), dim3(nthr), 0, h->stream, nb);
+  hipLaunchKernelGGL(k_near_blocks, dim3(3 * p.Ne), dim3(nthr), 0, ns, nb);
   prof_end(h, K_NEAR_BLOCKS, 0.0);
-
   LAUNCHCHK();
   // far
   int rc = mixed ? far_fill<float>(h, p, tg, sg) : far_fill<double>(h, p, tg, sg);
   if (rc) return rc;
+
   // assemble
   AsmArgs aa;
   aa.Zfar = p.d_zfar;
@@ -1164,7 +1219,13 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   else hipLaunchKernelGGL((k_assemble<double, double>), g, dim3(256), 0, h->stream, aa);
   prof_end(h, K_ASSEMBLE, 0.0);
   LAUNCHCHK();
-  // excitation
+  return rhs_problem(h, p, tg);
+}
+
+// Bessel table and right-hand sides of one problem (test geometry tg).
+static int rhs_problem(ghost_bor* h, Problem& p, const Geom& tg) {
+  const bool mixed = h->precision == GHOST_MIXED;
+  const int P = p.P;
   ExciteArgs ea = excite_args(p, tg);
   const size_t nbes = (size_t)P * p.ntheta;
   prof_begin(h);
@@ -1193,20 +1254,57 @@ static int run_fill(ghost_bor* h) {
   return 0;
 }
 
+// Only the right-hand sides of every problem (Bessel table + excitation), for
+// callers that want V_m(theta) without the operator.
+extern "C" int ghost_bor_fill_rhs(ghost_bor* h) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  HIPCHK(hipSetDevice(h->device));
+  h->solved = false;
+  for (auto& p : h->probs) {
+    int rc = rhs_problem(h, p, base_geom(p));
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+// Write a caller's solution X (n x n_rhs, masked rows, complex row-major) of
+// mode m of problem prob where the far field reads it.
+extern "C" int ghost_bor_set_solution(ghost_bor* h, int prob, int m, const double* X) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (!X) return fail(GHOST_ERR_ARG, "null solution");
+  if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index");
+  const Problem& p = h->probs[prob];
+  if (m < -p.M || m > p.M) return fail(GHOST_ERR_ARG, "mode outside range");
+  const SysInfo& s = h->sys[p.sys0 + m + p.M];
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->precision == GHOST_MIXED) {
+    HIPCHK(hipMemcpy(h->d_X64 + s.x_off, X, (size_t)s.n * s.nrhs * sizeof(double2), hipMemcpyHostToDevice));
+  } else {
+    for (int r = 0; r < s.n; ++r)
+      HIPCHK(hipMemcpy((double2*)h->d_A + (size_t)s.a_off + (size_t)r * s.ld + s.n, X + 2 * (size_t)r * s.nrhs,
+                       (size_t)s.nrhs * sizeof(double2), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
 // Batched complex GEMM C -= A B with the handle's tile variant:
 // 0: 64 x 64 single-buffered; 1: 64 x 64, 2: 64 x 128, 3: 128 x 128 (FP32) /
-// 64 x 128 (FP64), double-buffered.  gx = 0 computes the grid from mM, mN.
+// 64 x 128 (FP64), double-buffered; 4: variant 1 with the C tile loaded
+// before the K loop.
 template <typename T, bool TA = false, bool TB = false>
 static void gemm_launch(ghost_bor* h, const GemmOp* ops, int nops, int mM, int mN, const void* A, const void* B,
                         void* C) {
   if (mM <= 0 || mN <= 0) return;
   int v = h->gemm_variant;
+  if (v < 0) v = sizeof(T) == 8 ? 4 : 1;
   if (v == 3 && sizeof(T) == 8) v = 2;
   auto grid = [&](int bm, int bn) { return dim3((unsigned)(((mM + bm - 1) / bm) * ((mN + bn - 1) / bn)), nops); };
   switch (v) {
     case 0: hipLaunchKernelGGL((k_gemm_update<T, TA, TB>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
     case 1: hipLaunchKernelGGL((k_gemm2<T, 2, 2, TA, TB>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
     case 2: hipLaunchKernelGGL((k_gemm2<T, 2, 4, TA, TB>), grid(64, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
+    case 4: hipLaunchKernelGGL((k_gemm2<T, 2, 2, TA, TB, true>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
     default: hipLaunchKernelGGL((k_gemm2<T, 4, 4, TA, TB>), grid(128, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
   }
 }
@@ -1345,20 +1443,20 @@ static int probe_error(ghost_bor* h, std::vector<double>& eta) {
   const int nsys = (int)h->sys.size();
   size_t pe = 0;
   for (auto& s : h->sys) pe += 2 * (size_t)s.n;
-  int gx = 0, maxn = 0;
-  for (auto& s : h->sys) { gx = std::max(gx, (s.n + 63) / 64); maxn = std::max(maxn, s.n); }
+  int maxn = 0;
+  for (auto& s : h->sys) maxn = std::max(maxn, s.n);
   const dim3 gz((unsigned)std::min<size_t>(65535, (pe + 255) / 256));
   hipLaunchKernelGGL(k_zero, gz, dim3(256), 0, h->stream, h->d_BS, pe);
   hipLaunchKernelGGL(k_zero, gz, dim3(256), 0, h->stream, h->d_XS, pe);
   LAUNCHCHK();
   prof_begin(h);
   // S is stored negated, so C -= A (-S) accumulates +A S
-  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
-                     h->d_gemm + h->probe_ops, h->d_B64, h->d_S, h->d_BS);
-  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
-                     h->d_gemm + h->probe_ops + nsys, h->d_X64, h->d_S, h->d_XS);
-  hipLaunchKernelGGL((k_gemm_update<double>), dim3(gx, nsys), dim3(256), 0, h->stream,
-                     h->d_gemm + h->probe_ops + 2 * nsys, h->d_A64, h->d_XS, h->d_BS);
+  const dim3 gp((unsigned)((maxn + 3) / 4), nsys);
+  hipLaunchKernelGGL(k_gemm_n2, gp, dim3(256), 0, h->stream, h->d_gemm + h->probe_ops, h->d_B64, h->d_S, h->d_BS);
+  hipLaunchKernelGGL(k_gemm_n2, gp, dim3(256), 0, h->stream, h->d_gemm + h->probe_ops + nsys, h->d_X64, h->d_S,
+                     h->d_XS);
+  hipLaunchKernelGGL(k_gemm_n2, gp, dim3(256), 0, h->stream, h->d_gemm + h->probe_ops + 2 * nsys, h->d_A64,
+                     h->d_XS, h->d_BS);
   prof_end(h, K_GEMM64, 0.0);
   LAUNCHCHK();
   hipLaunchKernelGGL(k_col_scale, dim3(1, 1, nsys), dim3(64, 4), 0, h->stream, h->d_vec + h->vec_probe,
@@ -1452,9 +1550,17 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
   return 0;
 }
 
+extern "C" int ghost_bor_set_lu_super(ghost_bor* h, int panels) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "set the LU super-block before build");
+  if (panels < 1 || panels > 16) return fail(GHOST_ERR_ARG, "LU super-block must be 1..16 panels");
+  h->lu_super = panels;
+  return 0;
+}
+
 extern "C" int ghost_bor_set_gemm_variant(ghost_bor* h, int variant) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
-  if (variant < 0 || variant > 3) return fail(GHOST_ERR_ARG, "GEMM variant must be 0..3");
+  if (variant < -1 || variant > 4) return fail(GHOST_ERR_ARG, "GEMM variant must be -1..4");
   h->gemm_variant = variant;
   return 0;
 }
@@ -1868,6 +1974,8 @@ static bool adj_free(const Problem& p, int q, int r, double step) {
 extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad) {
   if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
   if (!h->built) return fail(GHOST_ERR_STATE, "build first");
+  for (auto& s : h->sys)
+    if (s.prob < 0) return fail(GHOST_ERR_STATE, "the adjoint covers handles of BoR problems only");
   if (h->precision != GHOST_FP64) return fail(GHOST_ERR_STATE, "the adjoint needs an FP64 handle");
   if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
   if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
@@ -2053,4 +2161,75 @@ extern "C" int ghost_copy_calibration(double bytes, float* ms) {
   if (a) hipFree(a);
   if (b) hipFree(b);
   return rc;
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// caller-filled dense systems
+// ─────────────────────────────────────────────────────────────────────────────
+
+static int dense_sys(ghost_bor* h, int idx, const SysInfo** out) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (idx < 0 || idx >= (int)h->sys.size() || h->sys[idx].prob >= 0)
+    return fail(GHOST_ERR_ARG, "%d is not a dense system of this handle", idx);
+  *out = &h->sys[idx];
+  return 0;
+}
+
+// A: n x n, B: n x nrhs, complex row-major interleaved doubles.
+extern "C" int ghost_bor_set_dense(ghost_bor* h, int idx, const double* A, const double* B) {
+  const SysInfo* sp;
+  int rc = dense_sys(h, idx, &sp);
+  if (rc) return rc;
+  if (!A || !B) return fail(GHOST_ERR_ARG, "null matrix");
+  const SysInfo& s = *sp;
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const bool mixed = h->precision == GHOST_MIXED;
+  const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
+  std::vector<char> row((size_t)s.ld * csz, 0);
+  for (int r = 0; r < s.n; ++r) {
+    for (int c = 0; c < s.n + s.nrhs; ++c) {
+      const double* src = c < s.n ? A + 2 * ((size_t)r * s.n + c) : B + 2 * ((size_t)r * s.nrhs + (c - s.n));
+      if (mixed) ((float2*)row.data())[c] = make_float2((float)src[0], (float)src[1]);
+      else ((double2*)row.data())[c] = make_double2(src[0], src[1]);
+    }
+    HIPCHK(hipMemcpy((char*)h->d_A + ((size_t)s.a_off + (size_t)r * s.ld) * csz, row.data(), (size_t)s.ld * csz,
+                     hipMemcpyHostToDevice));
+  }
+  if (h->d_A64) {
+    HIPCHK(hipMemcpy(h->d_A64 + s.a64_off, A, (size_t)s.n * s.n * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_B64 + s.x_off, B, (size_t)s.n * s.nrhs * sizeof(double2), hipMemcpyHostToDevice));
+  }
+  h->solved = false;
+  return 0;
+}
+
+// X: n x nrhs after ghost_bor_solve.
+extern "C" int ghost_bor_get_dense(ghost_bor* h, int idx, double* X) {
+  const SysInfo* sp;
+  int rc = dense_sys(h, idx, &sp);
+  if (rc) return rc;
+  if (!X) return fail(GHOST_ERR_ARG, "null output");
+  const SysInfo& s = *sp;
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->precision == GHOST_MIXED) {
+    HIPCHK(hipMemcpy(X, h->d_X64 + s.x_off, (size_t)s.n * s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
+    return 0;
+  }
+  std::vector<double2> row(s.ld);
+  for (int r = 0; r < s.n; ++r) {
+    HIPCHK(hipMemcpy(row.data(), (double2*)h->d_A + (size_t)s.a_off + (size_t)r * s.ld, (size_t)s.ld * sizeof(double2),
+                     hipMemcpyDeviceToHost));
+    memcpy(X + 2 * (size_t)r * s.nrhs, row.data() + s.n, (size_t)s.nrhs * sizeof(double2));
+  }
+  return 0;
+}
+
+// Number of systems (mode systems of every problem and dense systems): the
+// length of the per-system arrays of ghost_bor_residual and ghost_bor_info.
+extern "C" int ghost_bor_system_count(ghost_bor* h, int* n) {
+  if (!h || !n) return fail(GHOST_ERR_ARG, "null argument");
+  *n = (int)h->sys.size();
+  return 0;
 }

@@ -89,7 +89,7 @@ __device__ __forceinline__ void row_reduce_scatter32(const R (&v)[32], int lane,
 // Phase 1: EFIE samples -> G_n on MFMA.  Phase 2: per-order EFIE blocks,
 // reduced over the test points and stashed in LDS.  Phase 3: MFIE brackets
 // contracted to nodal rows -> modes on MFMA.  Phase 4: combine, RMW Zfar.
-template <typename R, int NT>
+template <typename R, int NT, bool SPLIT>
 __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
   typedef typename CT<R>::c C2;
   typedef typename CT<R>::v4 V4;
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
   const int c16 = lane & 15, kk = lane >> 4;
   const R k = (R)a.k;
   const PointSet& P = a.pts;
-  const PointSet& Q = a.spts;
+  const PointSet& Q = SPLIT ? a.spts : a.pts;   // source points: the test set unless split
 
   // operand lane: the point pair (g, h) whose result lands in lane group
   // kk = g, accumulator element r = h (row mrow(g, h) = c16)

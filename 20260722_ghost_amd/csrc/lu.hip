@@ -142,10 +142,42 @@ __global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, const vo
       }
 }
 
+// C[M x 2] -= A[M x K] B[K x 2], FP64 complex: the error probe's two-column
+// products.  One wavefront per row of A (coalesced along K), B read through
+// the cache, the two dot products reduced across the wave.  Memory-bound: a
+// 64 x 64 MFMA tile would spend its time multiplying B's zero padding.
+__global__ __launch_bounds__(256) void k_gemm_n2(const GemmOp* ops, const double2* Abase, const double2* Bbase,
+                                                 double2* Cbase) {
+  const GemmOp op = ops[blockIdx.y];
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= op.M || op.N != 2 || op.K <= 0) return;
+  const double2* a = Abase + op.a + (size_t)row * op.lda;
+  const double2* b = Bbase + op.b;
+  double r0 = 0, i0 = 0, r1 = 0, i1 = 0;
+  for (int k = lane; k < op.K; k += 64) {
+    const double2 x = a[k], u = b[(size_t)k * op.ldb], v = b[(size_t)k * op.ldb + 1];
+    r0 += x.x * u.x - x.y * u.y;
+    i0 += x.x * u.y + x.y * u.x;
+    r1 += x.x * v.x - x.y * v.y;
+    i1 += x.x * v.y + x.y * v.x;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    r0 += __shfl_xor(r0, o);
+    i0 += __shfl_xor(i0, o);
+    r1 += __shfl_xor(r1, o);
+    i1 += __shfl_xor(i1, o);
+  }
+  if (lane == 0) {
+    double2* c = Cbase + op.c + (size_t)row * op.ldc;
+    c[0].x -= r0; c[0].y -= i0;
+    c[1].x -= r1; c[1].y -= i1;
+  }
+}
+
 // Block tile (32 WM) x (32 WN) complex, 2 x 2 wavefronts of WM x WN MFMA
 // tiles each; K in chunks of 16 staged through two LDS buffers with the next
 // chunk's global loads in flight while the current one feeds the MFMAs.
-template <typename T, int WM, int WN, bool TA = false, bool TB = false>
+template <typename T, int WM, int WN, bool TA = false, bool TB = false, bool PRE = false>
 __global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, const void* Abase, const void* Bbase,
                                                void* Cbase) {
   typedef typename CT<T>::c C;
@@ -215,9 +247,30 @@ __global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, const void* Ab
   for (int i = 0; i < WM; ++i)
 #pragma unroll
     for (int j = 0; j < WN; ++j) { accR[i][j] = V4{0, 0, 0, 0}; accI[i][j] = V4{0, 0, 0, 0}; }
-
   const int nk = (op.K + KC - 1) / KC;
   gload(0);
+  // the C tile is loaded up front into registers of its own: its latency
+  // overlaps the first chunk, and with the short K of the LU updates (64) the
+  // epilogue is a subtraction and a store rather than a dependent load.  A B
+  // still accumulates from zero, so the rounding of the update stays at its
+  // own scale (accumulating onto C would round every MFMA step at C's scale)
+  // (PRE: the register cost lowers occupancy, so it is a tile variant)
+  C cpre[WM][WN][4];
+  if (PRE) {
+#pragma unroll
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+      for (int j = 0; j < WN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gr = m0 + wy * 16 * WM + i * 16 + mrow<T>(kk, r);
+          const int gc = n0 + wx * 16 * WN + j * 16 + c16;
+          C v;
+          v.x = 0; v.y = 0;
+          if (gr < op.M && gc < op.N) v = Cm[(size_t)gr * op.ldc + gc];
+          cpre[i][j][r] = v;
+        }
+  }
   lstore(0);
   __syncthreads();
   for (int kc = 0; kc < nk; ++kc) {
@@ -260,11 +313,10 @@ __global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, const void* Ab
         const int gr = m0 + wy * 16 * WM + i * 16 + mrow<T>(kk, r);
         const int gc = n0 + wx * 16 * WN + j * 16 + c16;
         if (gr < op.M && gc < op.N) {
-          C* p = Cm + (size_t)gr * op.ldc + gc;
-          C v = *p;
+          C v = PRE ? cpre[i][j][r] : Cm[(size_t)gr * op.ldc + gc];
           v.x -= accR[i][j][r];
           v.y -= accI[i][j][r];
-          *p = v;
+          Cm[(size_t)gr * op.ldc + gc] = v;
         }
       }
 }

@@ -103,8 +103,13 @@ int ghost_bor_fill(ghost_bor* h);
 int ghost_bor_solve(ghost_bor* h, int refine_iters);
 /* Tile variant of the batched complex MFMA GEMM (trailing updates, solves,
  * residuals): 0 = 64x64 single-buffered, 1 = 64x64, 2 = 64x128, 3 = 128x128
- * (FP32; 64x128 for FP64) with double-buffered K chunks.  Default 1. */
+ * (FP32; 64x128 for FP64) with double-buffered K chunks, 4 = variant 1 with
+ * the C tile loaded before the K loop; -1 (default) = 1 for FP32, 4 for FP64. */
 int ghost_bor_set_gemm_variant(ghost_bor* h, int variant);
+/* Panels per LU super-block (before build; default 4): the trailing update
+ * runs once per super-block with K = nb x panels, the panels and row blocks
+ * inside it catch up through narrow GEMMs.  1 = plain right-looking LU. */
+int ghost_bor_set_lu_super(ghost_bor* h, int panels);
 /* Probe tolerance of the adaptive refinement (default 1e-5). */
 int ghost_bor_set_probe_tol(ghost_bor* h, double tol);
 /* Probe errors of the last adaptive solve, one per probe; *n: capacity in, count out. */
@@ -127,7 +132,10 @@ int ghost_bor_graph_ms(ghost_bor* h, float* ms);
 
 /* dF of one problem: complex [2 m_max + 1][2 (VV, HH)][n_theta], mode index m + m_max. */
 int ghost_bor_get_dF(ghost_bor* h, int prob, double* out);
-/* ||A X - B||_F / ||B||_F for every system (bor_solver.py:754-756), out[sum of modes]. */
+/* Number of systems of the handle (every problem's modes and the dense
+ * systems, in the order added): the length of the per-system arrays below. */
+int ghost_bor_system_count(ghost_bor* h, int* n);
+/* ||A X - B||_F / ||B||_F for every system (bor_solver.py:754-756), out[system count]. */
 int ghost_bor_residual(ghost_bor* h, double* out);
 /* Mode-system layout of (prob, m): size n, active J_t count nt and first active t / phi nodes. */
 int ghost_bor_system_info(ghost_bor* h, int prob, int m, int* n, int* nt, int* t0, int* f0);
@@ -135,7 +143,7 @@ int ghost_bor_num_systems(ghost_bor* h, int prob, int* nmodes, int* m_max);
 /* Copy a system out (complex, row-major): which = 0 -> A [n x n] and B [n x nrhs]
  * (after fill, before solve); which = 1 -> X [n x nrhs] (after solve). */
 int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, double* out1, double* out2);
-/* Zero-pivot count per system from the last factorisation. */
+/* Zero-pivot count per system from the last factorisation, out[system count]. */
 int ghost_bor_info(ghost_bor* h, int* out);
 /* Milliseconds of the last fill, solve and far-field phases (HIP events on the
  * handle's stream); -1 for a phase that has not run eagerly since build. */
@@ -166,6 +174,25 @@ int ghost_bor_kernel_times(ghost_bor* h, double* ms, int* count, double* flops);
  * nodes mod 3 at a time (12 batched fills).  Needs a GHOST_FP64 handle after
  * fill + solve; afterwards the systems hold a perturbed fill. */
 int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad);
+
+/* Right-hand sides only (Bessel table and excitation of every problem): the
+ * device side of rhs_mode / rhs_mfie_mode (bor_solver.py:437-461, 625-645). */
+int ghost_bor_fill_rhs(ghost_bor* h);
+/* Put a caller's solution X (n x n_rhs, masked rows, complex row-major) of
+ * mode m where ghost_bor_farfield reads it: the device side of farfield_mode
+ * (bor_solver.py:659-708) for solutions computed elsewhere. */
+int ghost_bor_set_solution(ghost_bor* h, int prob, int m, const double* X);
+
+/* Caller-filled dense complex systems solved in the same batched LU as the
+ * mode systems: the device side of the reference's generic mode loop
+ * (_mode_sweep, bor_solver.py:711-801), whose np.linalg.solve (line 753) this
+ * replaces for closures that assemble their own systems.  add_dense before
+ * build; set_dense (A: n x n, B: n x n_rhs, complex row-major) after build and
+ * before each solve (the LU works in place); get_dense (X: n x n_rhs) after
+ * the solve.  Dense systems have no fill and no far field. */
+int ghost_bor_add_dense(ghost_bor* h, int n, int nrhs, int* sys_idx);
+int ghost_bor_set_dense(ghost_bor* h, int sys_idx, const double* A, const double* B);
+int ghost_bor_get_dense(ghost_bor* h, int sys_idx, double* X);
 
 /* Measurement utility (no counterpart in the reference): one plain device
  * copy of `bytes` bytes (read once, written once) on its own stream, *ms its

@@ -1,0 +1,106 @@
+"""Per-mode operator API (BorPecSolver) and the generic mode loop (_mode_sweep)
+on the device, against the FP64 oracle and the batched solve_bor (run on an
+MI355X)."""
+
+import math
+
+import numpy as np
+import pytest
+
+import bor_oracle as bo
+from conftest import pkg
+
+pytestmark = pytest.mark.gpu
+
+ETA0 = 376.730313668
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def case():
+    wl = pkg("workloads")
+    nodes = wl.cone_sphere(16, 15.0, 0.3)
+    return nodes, 3e8, 4
+
+
+def test_bor_pec_solver_operators_match_oracle(case):
+    nodes, f, mm = case
+    bs = pkg("bor_solver")
+    s = bs.BorPecSolver(nodes, f)
+    ms = bo.ModeSystem(nodes, f, mm, "cfie", 0.5, streaming=False)
+    body, k = ms.body, ms.k
+    assert rel(s.mass_blocks(), bo.mass_matrix(body)) < 1e-13
+    for m in (-3, 0, 1, 2):
+        print(f"[case] m={m}", flush=True)
+        assert rel(s.assemble_mode(m, mm), ms.efie_full(m)) < 1e-11, m
+        assert rel(s.assemble_mfie_mode(m, mm), ms.mfie_full(m)) < 1e-11, m
+        assert np.array_equal(s.basis_mask(m), bo.basis_mask(body, m))
+        for th in (0.0, 37.0, 120.0):
+            for pol in ("VV", "HH"):
+                assert rel(s.rhs_mode(m, th, pol), bo.rhs_efie(body, k, m, th, pol)) < 1e-12
+                assert rel(s.rhs_mfie_mode(m, th, pol), bo.rhs_mfie(body, k, m, th, pol)) < 1e-12
+    rng = np.random.default_rng(5)
+    sol = rng.standard_normal(2 * body.Nn) + 1j * rng.standard_normal(2 * body.Nn)
+    for m, th in ((0, 30.0), (2, 75.0), (-1, 150.0)):
+        got = s.farfield_mode(m, sol, th)
+        want = bo.farfield(body, k, m, sol, th)
+        assert abs(got[0] - want[0]) <= 1e-12 * abs(want[0]) + 1e-300
+        assert abs(got[1] - want[1]) <= 1e-12 * abs(want[1]) + 1e-300
+
+
+def test_mode_sweep_with_device_operators_matches_solve_bor(case):
+    nodes, f, mm = case
+    bs = pkg("bor_solver")
+    s = bs.BorPecSolver(nodes, f)
+    thetas = np.array([10.0, 60.0, 110.0, 170.0])
+    a = 0.5
+
+    def assemble(m):
+        Z = a * s.assemble_mode(m, mm) + (1 - a) * ETA0 * s.assemble_mfie_mode(m, mm)
+        mask = s.basis_mask(m)
+        return Z[np.ix_(mask, mask)], mask
+
+    def rhs(m, th, pol):
+        return a * s.rhs_mode(m, th, pol) + (1 - a) * ETA0 * s.rhs_mfie_mode(m, th, pol)
+
+    def farfield(m, sol, th, pol):
+        fth, fph = s.farfield_mode(m, sol, th)
+        return fth if pol == "VV" else fph
+
+    F, used, stats = bs._mode_sweep(2 * s.Nn, thetas, ["VV", "HH"], mm, 0.0, assemble, rhs, farfield)
+    ref = bs.solve_bor(nodes, f, thetas, formulation="cfie", n_modes=mm, mode_tol=0.0)
+    assert used == ref["modes_used"] == mm
+    assert rel(F[0], ref["amp_vv"]) < 1e-10
+    assert rel(F[1], ref["amp_hh"]) < 1e-10
+    assert stats["linear_residual"] < 1e-12
+
+
+@pytest.mark.parametrize("nb,lu_super", [(64, 1), (64, 2), (64, 4), (32, 5), (16, 4)])
+def test_dense_batch_solves_caller_systems(nb, lu_super):
+    """Uneven caller systems in one batched LU, FP64 and mixed precision, for
+    several block sizes and LU super-blocks (the catch-up GEMMs of a
+    super-block run when a system spans several of its panels)."""
+    L = pkg("_lib")
+    rng = np.random.default_rng(9)
+    sizes = [(37, 5), (130, 17), (64, 64), (201, 3), (517, 40)]
+    mats = []
+    for n, r in sizes:
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 3 * math.sqrt(n) * np.eye(n)
+        B = rng.standard_normal((n, r)) + 1j * rng.standard_normal((n, r))
+        mats.append((A, B, np.linalg.solve(A, B)))
+    for prec, tol, it in ((L.GHOST_FP64, 1e-12, 0), (L.GHOST_MIXED, 1e-12, 3)):
+        print(f"[case] nb={nb} lu_super={lu_super} precision={prec}", flush=True)
+        h = L.Handle(prec, nb=nb)
+        h.set_lu_super(lu_super)
+        idx = [h.add_dense(n, r) for n, r in sizes]
+        h.build()
+        for i, (A, B, _) in zip(idx, mats):
+            h.set_dense(i, A, B)
+        h.solve(it)
+        h.sync()
+        for i, (_, _, X) in zip(idx, mats):
+            assert rel(h.get_dense(i), X) < tol, (prec, i)
+        h.close()

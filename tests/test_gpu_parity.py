@@ -29,7 +29,7 @@ def db(s):
 
 
 def make_handle(nodes, freq_hz, thetas, m_max, alpha, streaming, precision=0, unmasked=False, nb=64,
-                variant=None):
+                variant=None, lu_super=None):
     L = pkg("_lib")
     bs = pkg("bor_solver")
     pts = np.asarray(nodes, dtype=float)
@@ -40,6 +40,8 @@ def make_handle(nodes, freq_hz, thetas, m_max, alpha, streaming, precision=0, un
     h = L.Handle(precision, nb=nb)
     if variant is not None:
         h.set_gemm_variant(variant)
+    if lu_super is not None:
+        h.set_lu_super(lu_super)
     h.add_problem(pts, k, m_max, ne, nbk, alpha, (1 - alpha) * bs.ETA0, thetas, unmasked=unmasked)
     h.build()
     return h
@@ -86,19 +88,20 @@ def test_solve_bor_matches_reference(name, bs):
     assert out["linear_residual"] < 1e-13
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("lu_super", [1, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("nb", [16, 32, 64])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_lu_tile_variants_and_block_sizes(variant, nb, precision):
+def test_lu_tile_variants_and_block_sizes(variant, nb, precision, lu_super):
     """Every GEMM tile variant and LU block size gives the reference's currents:
     FP64 to 1e-9; mixed (FP32 far fill and LU, two FP64 refinement sweeps)
     to 1e-6, the FP32 rounding of the filled matrix times its conditioning."""
     d = load("conesphere16_cfie")
     mm = int(d["m_max"])
     thetas = d["thetas"]
-    print(f"[case] variant={variant} nb={nb} precision={precision}", flush=True)
+    print(f"[case] variant={variant} nb={nb} precision={precision} lu_super={lu_super}", flush=True)
     h = make_handle(d["nodes"], float(d["freq_hz"]), thetas, mm, float(d["alpha"]), False,
-                    precision=precision, nb=nb, variant=variant)
+                    precision=precision, nb=nb, variant=variant, lu_super=lu_super)
     h.fill()
     h.solve(2 if precision == 1 else 0)
     h.sync()
