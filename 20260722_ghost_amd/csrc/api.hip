@@ -190,16 +190,18 @@ struct Launch {  // one batched kernel launch of the solve schedule
   int arg;       // extra (RPT for panels)
   double flops = 0.0;  // algorithmic real flops of the launch
   int mM = 0, mN = 0;  // largest GEMM M and N over the launch's systems
+  int fam = -1;        // profile family of a GEMM launch (-1: K_GEMM)
 };
 
 // kernel families timed by the optional per-launch HIP-event profile
 enum {
   K_ZERO, K_NEAR_MODAL, K_NEAR_BLOCKS, K_FAR_FILL, K_ASSEMBLE, K_BESSEL, K_RHS, K_PANEL, K_SWAP,
-  K_TRI_INV, K_GEMM, K_GEMM64, K_VEC, K_FARFIELD, K_COUNT
+  K_TRI_INV, K_GEMM, K_GEMM64, K_VEC, K_FARFIELD, K_GEMM_BACK, K_GEMM_SOLVE, K_COUNT
 };
 static const char* kKernelNames[K_COUNT] = {"zero", "near_modal", "near_blocks", "far_fill", "assemble",
                                             "bessel", "rhs", "lu_panel", "lu_swap", "tri_inv", "gemm",
-                                            "gemm_fp64_residual", "vec", "farfield"};
+                                            "gemm_fp64_residual", "vec", "farfield", "gemm_back",
+                                            "gemm_solve"};
 
 enum { L_PANEL, L_SWAP, L_TRIL, L_TRIU, L_GEMM_AUG, L_GEMM_AUG_W, L_GEMM_Y, L_GEMM_Y_W, L_GEMM_Y_LU,
        L_GEMM_TW, L_GEMM_TLU };   // transposed-operand solves (adjoint): W^T and factor-block^T
@@ -795,31 +797,55 @@ static int build_schedule(ghost_bor* h) {
     }
   }
   h->triu_n = nsys * nblk_max;
-  // ── back substitution on the RHS columns of the augmented systems ──
-  for (int st = 0; st < nblk_max; ++st) {
-    Launch g1{L_GEMM_AUG_W, (int)h->h_gemm.size(), nsys, 0, 0};
-    std::vector<GemmOp> upd;
+  // ── back substitution on the RHS columns, in super-blocks of S row blocks ──
+  // Walking up from the last block: block b catches up on the super-block's
+  // blocks below it (K = rows solved so far in the super-block), is solved
+  // through W_u, and after the super-block one update with K up to nb x S
+  // carries its rows into every row above.
+  auto back_gemm = [&](int kind, auto make) {
+    Launch g{kind, (int)h->h_gemm.size(), nsys, 0, 0};
     for (int si = 0; si < nsys; ++si) {
-      const SysInfo& s = h->sys[si];
-      const int nbs = (s.n + nb - 1) / nb;
-      const int b = nbs - 1 - st;
-      GemmOp go{0, 0, 0, 64, s.ld, s.ld, 0, 0, 0};
-      GemmOp gu{0, 0, 0, s.ld, s.ld, s.ld, 0, 0, 0};
-      if (b >= 0) {
-        const int kb = b * nb, w = std::min(nb, s.n - kb);
-        go = GemmOp{wu(si, b), s.a_off + (long long)kb * s.ld + s.n, s.a_off + (long long)kb * s.ld + s.n,
-                    64, s.ld, s.ld, w, s.nrhs, w};
-        gu = GemmOp{s.a_off + kb, s.a_off + (long long)kb * s.ld + s.n, s.a_off + s.n, s.ld, s.ld, s.ld,
-                    kb, s.nrhs, w};
-      }
+      GemmOp go = make(si, h->sys[si]);
+      if (go.M <= 0 || go.N <= 0 || go.K <= 0) go.M = go.N = go.K = 0;
       h->h_gemm.push_back(go);
-      g1.gx = std::max(g1.gx, tiles(go.M, go.N));
-      upd.push_back(gu);
+      g.gx = std::max(g.gx, tiles(go.M, go.N));
     }
-    Launch g2{L_GEMM_AUG, (int)h->h_gemm.size(), nsys, 0, 0};
-    for (auto& gu : upd) { h->h_gemm.push_back(gu); g2.gx = std::max(g2.gx, tiles(gu.M, gu.N)); }
-    h->back_sched.push_back(g1);
-    h->back_sched.push_back(g2);
+    h->back_sched.push_back(g);
+  };
+  for (int st0 = 0; st0 < nblk_max; st0 += S) {
+    const int nsb = std::min(S, nblk_max - st0);
+    // first row past the super-block's already-solved part, per system
+    auto k_end = [&](const SysInfo& s) { return std::min(s.n, ((s.n + nb - 1) / nb - st0) * nb); };
+    for (int sb = 0; sb < nsb; ++sb) {
+      const int st = st0 + sb;
+      auto blk = [&](const SysInfo& s, int& kb, int& w) {
+        const int b = (s.n + nb - 1) / nb - 1 - st;
+        kb = b * nb;
+        w = b >= 0 ? std::min(nb, s.n - kb) : 0;
+        return b;
+      };
+      if (sb > 0)   // X_b -= U[kb:kb+w, kb+w:k_end] X[kb+w:k_end]
+        back_gemm(L_GEMM_AUG, [&](int, const SysInfo& s) {
+          int kb, w;
+          if (blk(s, kb, w) < 0) return GemmOp{0, 0, 0, s.ld, s.ld, s.ld, 0, 0, 0};
+          return GemmOp{s.a_off + (long long)kb * s.ld + kb + w, s.a_off + (long long)(kb + w) * s.ld + s.n,
+                        s.a_off + (long long)kb * s.ld + s.n, s.ld, s.ld, s.ld, w, s.nrhs, k_end(s) - kb - w};
+        });
+      back_gemm(L_GEMM_AUG_W, [&](int si, const SysInfo& s) {   // X_b = U_bb^-1 X_b
+        int kb, w;
+        const int b = blk(s, kb, w);
+        if (b < 0) return GemmOp{0, 0, 0, 64, s.ld, s.ld, 0, 0, 0};
+        return GemmOp{wu(si, b), s.a_off + (long long)kb * s.ld + s.n, s.a_off + (long long)kb * s.ld + s.n, 64,
+                      s.ld, s.ld, w, s.nrhs, w};
+      });
+    }
+    // rows above the super-block: X[0:k_lo] -= U[0:k_lo, k_lo:k_end] X[k_lo:k_end]
+    back_gemm(L_GEMM_AUG, [&](int, const SysInfo& s) {
+      const int b_lo = (s.n + nb - 1) / nb - 1 - (st0 + nsb - 1);
+      const int k_lo = std::max(0, b_lo) * nb;
+      return GemmOp{s.a_off + k_lo, s.a_off + (long long)k_lo * s.ld + s.n, s.a_off + s.n, s.ld, s.ld, s.ld, k_lo,
+                    s.nrhs, k_end(s) - k_lo};
+    });
   }
   // ── VecOps over all systems (copies, casts, norms) ──
   std::vector<int> c_off(nsys);
@@ -936,6 +962,9 @@ static int build_schedule(ghost_bor* h) {
       h->h_gemm.push_back(GemmOp{s.x_off, s.x_off, s.a_off, s.nrhs, s.nrhs, s.ld, s.n, s.n, s.nrhs});
     }
   }
+  for (auto& L : h->back_sched) L.fam = K_GEMM_BACK;
+  for (auto* sc : {&h->fwdY_sched, &h->backY_sched, &h->fwdP_sched, &h->backP_sched, &h->adjT_fwd, &h->adjT_back})
+    for (auto& L : *sc) L.fam = K_GEMM_SOLVE;
   set_flops(h, h->lu_sched);
   set_flops(h, h->back_sched);
   set_flops(h, h->adjT_fwd);
@@ -1350,7 +1379,7 @@ template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launc
       default: break;
     }
     {
-      int kind = K_GEMM;
+      int kind = L.fam >= 0 ? L.fam : K_GEMM;
       if (L.kind == L_PANEL) kind = K_PANEL;
       else if (L.kind == L_SWAP) kind = K_SWAP;
       else if (L.kind == L_TRIL || L.kind == L_TRIU) kind = K_TRI_INV;

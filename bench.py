@@ -261,9 +261,14 @@ def main():
         }
         fp64_rate = solves / dt_64
 
-    # dominant kernel of the headline path, roofline from live HIP-event timing
-    fam, (ms, cnt, fl) = max(kt_mx.items(), key=lambda kv: kv[1][0])
-    gemm_ms, gemm_n, gemm_fl = kt_mx.get("gemm", (0.0, 0, 0.0))
+    # dominant kernel of the headline path, roofline from live HIP-event timing:
+    # the GEMM kernel over all its uses in the step (LU updates, back
+    # substitution, probe solves), the FP64 residual kernel aside
+    gl = [v for kk, v in kt_mx.items() if kk.startswith("gemm") and kk != "gemm_fp64_residual"]
+    gemm_ms, gemm_n, gemm_fl = (sum(v[0] for v in gl), sum(v[1] for v in gl), sum(v[2] for v in gl))
+    merged = {kk: v for kk, v in kt_mx.items() if not (kk.startswith("gemm") and kk != "gemm_fp64_residual")}
+    merged["gemm (all uses)"] = (gemm_ms, gemm_n, gemm_fl)
+    fam, (ms, cnt, fl) = max(merged.items(), key=lambda kv: kv[1][0])
     roof = None
     if gemm_n > 0 and gemm_ms > 0:
         achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12
