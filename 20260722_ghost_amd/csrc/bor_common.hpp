@@ -28,8 +28,24 @@ template <typename R> __device__ __forceinline__ constexpr int mrow(int kk, int 
 template <> __device__ __forceinline__ constexpr int mrow<float>(int kk, int r) { return 4 * kk + r; }
 template <> __device__ __forceinline__ constexpr int mrow<double>(int kk, int r) { return kk + 4 * r; }
 
-__device__ __forceinline__ void sincos_r(double x, double* s, double* c) { sincos(x, s, c); }
-__device__ __forceinline__ void sincos_r(float x, float* s, float* c) { sincosf(x, s, c); }
+// Far-fill sample arithmetic.  FP64: the library's sqrt, sincos and
+// division.  FP32 (mixed precision): the hardware v_sqrt_f32, v_rcp_f32 and
+// v_sin_f32 / v_cos_f32 (argument in revolutions, first reduced to
+// [-1/2, 1/2]).  In FP32 the phase k R of a far pair (up to ~70 rad at 10
+// wavelengths) already carries ~4e-6 rad of rounding, which a full-range
+// library reduction cannot recover, so the cheaper forms lose nothing that
+// matters and cut the sample cost several-fold.
+__device__ __forceinline__ double sqrt_s(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_s(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double rcp_s(double x) { return 1.0 / x; }
+__device__ __forceinline__ float rcp_s(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ void sincos_s(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ void sincos_s(float x, float* s, float* c) {
+  const float r = x * 0.159154943091895336f;   // 1 / (2 pi)
+  const float f = r - rintf(r);
+  *s = __builtin_amdgcn_sinf(f);
+  *c = __builtin_amdgcn_cosf(f);
+}
 
 // Row-rotate a 64-bit value within each 16-lane DPP row: lane c receives
 // the value of lane (c - N) mod 16 of the same row.

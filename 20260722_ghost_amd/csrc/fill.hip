@@ -143,11 +143,11 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
         const int j = j0 + kk;
         const int jc = j < hE ? j : hE;
         const R cw = (j == 0 || j == hE) ? (R)1 : (j > hE ? (R)0 : (R)2);
-        R Rd = sqrt(d2 + rr4 * t_sin2E[jc]);
+        R Rd = sqrt_s(d2 + rr4 * t_sin2E[jc]);
         Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
         R s, c;
-        sincos_r(k * Rd, &s, &c);
-        const R amp = cw * sc / Rd;
+        sincos_s(k * Rd, &s, &c);
+        const R amp = cw * sc * rcp_s(Rd);
         const R ar = amp * c, ai = -amp * s;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -261,11 +261,11 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
       const R cw = (j == 0 || j == hB) ? (R)1 : (j > hB ? (R)0 : (R)2);
       const R cx = t_cxB[jc], sx = t_sxB[jc];
       const R Rx = rp - rq * cx, Ry = rq * sx;
-      R Rd = sqrt(Rx * Rx + Ry * Ry + dz * dz);
+      R Rd = sqrt_s(Rx * Rx + Ry * Ry + dz * dz);
       Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
       R s, c;
-      sincos_r(k * Rd, &s, &c);
-      const R inv = (R)1 / Rd;
+      sincos_s(k * Rd, &s, &c);
+      const R inv = rcp_s(Rd);
       const R pre = cw * scB * inv * inv * inv;
       const R kr = k * Rd;
       const R pr = pre * (c + kr * s), pim = pre * (kr * c - s);
