@@ -189,7 +189,7 @@ struct Launch {  // one batched kernel launch of the solve schedule
   int gx;        // grid.x
   int arg;       // extra (RPT for panels)
   double flops = 0.0;  // algorithmic real flops of the launch
-  int mM = 0, mN = 0;  // largest GEMM M and N over the launch's systems
+  int tt[3] = {0, 0, 0};  // flattened block count per tile shape (GEMM launches)
   int fam = -1;        // profile family of a GEMM launch (-1: K_GEMM)
 };
 
@@ -262,6 +262,7 @@ struct ghost_bor {
   int* d_rowoff = nullptr;
   int triu_ops = 0, triu_n = 0;
   int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
+  int res_tt[3] = {0, 0, 0}, res_aug_tt[3] = {0, 0, 0}, adj_q_tt[3] = {0, 0, 0};
   int vec_all = 0, vec_aug = 0;          // VecOps over all systems: x-blocks / augmented RHS -> x-blocks
   int vec_toaug = 0, vec_probe = 0;      // x-blocks -> augmented RHS; the n x 2 probe blocks
   // mixed precision: power-of-two column scales of B (bscale) and of each
@@ -635,14 +636,26 @@ static double gemm_flops(const std::vector<GemmOp>& ops, int first, int n, bool 
   return f;
 }
 
+// Flattened tile starts (GemmOp::t0) of ops [first, first + n) for the three
+// tile shapes; tt receives each shape's block count.
+static void tile_prefix(std::vector<GemmOp>& g, int first, int n, int* tt) {
+  static const int BM[3] = {64, 64, 128}, BN[3] = {64, 128, 128};
+  for (int sh = 0; sh < 3; ++sh) {
+    int acc = 0;
+    for (int i = first; i < first + n; ++i) {
+      GemmOp& o = g[i];
+      o.t0[sh] = acc;
+      if (o.M > 0 && o.N > 0 && o.K > 0) acc += ((o.M + BM[sh] - 1) / BM[sh]) * ((o.N + BN[sh] - 1) / BN[sh]);
+    }
+    tt[sh] = acc;
+  }
+}
+
 static void set_flops(ghost_bor* h, std::vector<Launch>& sched) {
   for (Launch& L : sched) {
     if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W || L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU ||
         L.kind == L_GEMM_TW || L.kind == L_GEMM_TLU)
-      for (int i = L.ops; i < L.ops + L.nops; ++i) {
-        L.mM = std::max(L.mM, h->h_gemm[i].M);
-        L.mN = std::max(L.mN, h->h_gemm[i].N);
-      }
+      tile_prefix(h->h_gemm, L.ops, L.nops, L.tt);
     if (L.kind == L_GEMM_AUG_W || L.kind == L_GEMM_Y_W || L.kind == L_GEMM_TW)
       L.flops = gemm_flops(h->h_gemm, L.ops, L.nops, true);
     else if (L.kind == L_GEMM_AUG || L.kind == L_GEMM_Y_LU || L.kind == L_GEMM_TLU)
@@ -657,6 +670,10 @@ static int build_schedule(ghost_bor* h) {
   for (auto& s : h->sys) maxn = std::max(maxn, s.n);
   h->rpt = (maxn + 1023) / 1024;
   if (h->rpt > 4) return fail(GHOST_ERR_ARG, "system size %d exceeds the panel limit 4096", maxn);
+  // pivot offsets
+  int poff = 0;
+  std::vector<int> pofs(nsys);
+  for (int si = 0; si < nsys; ++si) { pofs[si] = poff; poff += h->sys[si].n; }
   const int nblk_max = (maxn + nb - 1) / nb;
   h->max_blocks = nblk_max;
   const bool mixed = h->precision == GHOST_MIXED;
@@ -750,18 +767,25 @@ static int build_schedule(ghost_bor* h) {
           return GemmOp{s.a_off + (long long)k * s.ld + k0, s.a_off + (long long)k0 * s.ld + k,
                         s.a_off + (long long)k * s.ld + k, s.ld, s.ld, s.ld, s.n - k, w, k - k0};
         });
-      Launch lp{L_PANEL, (int)h->h_panel.size(), nsys, nsys, h->rpt};
-      Launch ls{L_SWAP, (int)h->h_swap.size(), nsys, 0, 0};
-      Launch lt{L_TRIL, (int)h->h_trsm.size(), nsys, nsys, 0};
+      // only the systems that still have a panel here (uneven batches)
+      Launch lp{L_PANEL, (int)h->h_panel.size(), 0, 0, h->rpt};
+      Launch ls{L_SWAP, (int)h->h_swap.size(), 0, 0, 0};
+      Launch lt{L_TRIL, (int)h->h_trsm.size(), 0, 0, 0};
       for (int si = 0; si < nsys; ++si) {
         const SysInfo& s = h->sys[si];
         const int w = k < s.n ? std::min(nb, s.n - k) : 0;
+        if (w <= 0) continue;
         const int ncols = s.n + s.nrhs;
-        h->h_panel.push_back(PanelOp{s.a_off, s.n, s.ld, k, w, 0});
-        h->h_swap.push_back(SwapOp{s.a_off, s.ld, k, w, k, k + w, ncols, 0});
+        h->h_panel.push_back(PanelOp{s.a_off, s.n, s.ld, k, w, pofs[si], si});
+        h->h_swap.push_back(SwapOp{s.a_off, s.ld, k, w, k, k + w, ncols, pofs[si]});
         ls.gx = std::max(ls.gx, (k + (ncols - k - w) + 255) / 256);
         h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wl(si, b), s.ld, w});
+        ++lp.nops;
+        ++ls.nops;
+        ++lt.nops;
       }
+      lp.gx = lp.nops;
+      lt.gx = lt.nops;
       h->lu_sched.push_back(lp);
       h->lu_sched.push_back(ls);
       h->lu_sched.push_back(lt);
@@ -793,10 +817,10 @@ static int build_schedule(ghost_bor* h) {
     for (int b = 0; b < nblk_max; ++b) {
       const int k = b * nb;
       const int w = k < s.n ? std::min(nb, s.n - k) : 0;
-      h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wu(si, b), s.ld, w});
+      if (w > 0) h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wu(si, b), s.ld, w});
     }
   }
-  h->triu_n = nsys * nblk_max;
+  h->triu_n = (int)h->h_trsm.size() - h->triu_ops;
   // ── back substitution on the RHS columns, in super-blocks of S row blocks ──
   // Walking up from the last block: block b catches up on the super-block's
   // blocks below it (K = rows solved so far in the super-block), is solved
@@ -965,6 +989,9 @@ static int build_schedule(ghost_bor* h) {
   for (auto& L : h->back_sched) L.fam = K_GEMM_BACK;
   for (auto* sc : {&h->fwdY_sched, &h->backY_sched, &h->fwdP_sched, &h->backP_sched, &h->adjT_fwd, &h->adjT_back})
     for (auto& L : *sc) L.fam = K_GEMM_SOLVE;
+  tile_prefix(h->h_gemm, h->res_gemm, nsys, h->res_tt);
+  tile_prefix(h->h_gemm, h->res_gemm_aug, nsys, h->res_aug_tt);
+  if (h->adj_q_ops >= 0) tile_prefix(h->h_gemm, h->adj_q_ops, nsys, h->adj_q_tt);
   set_flops(h, h->lu_sched);
   set_flops(h, h->back_sched);
   set_flops(h, h->adjT_fwd);
@@ -973,12 +1000,6 @@ static int build_schedule(ghost_bor* h) {
   set_flops(h, h->backY_sched);
   set_flops(h, h->fwdP_sched);
   set_flops(h, h->backP_sched);
-  // pivot offsets
-  int poff = 0;
-  std::vector<int> pofs(nsys);
-  for (int si = 0; si < nsys; ++si) { pofs[si] = poff; poff += h->sys[si].n; }
-  for (size_t i = 0; i < h->h_panel.size(); ++i) h->h_panel[i].piv = pofs[i % nsys];
-  for (size_t i = 0; i < h->h_swap.size(); ++i) h->h_swap[i].piv = pofs[i % nsys];
   for (size_t i = 0; i < h->h_vec.size(); ++i) h->h_vec[i].piv = pofs[i % nsys];
   h->piv_elems = poff;
   int rc;
@@ -1222,10 +1243,10 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   nb.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
   nb.cM = p.cM;
   nb.NB = p.d_NB;
-  const int nthr = ((p.M + 1 + 63) / 64) * 64;
-  if (nthr > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near blocks");
+  const int mlanes = ((p.M + 1 + 15) / 16) * 16;
+  if (mlanes > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near blocks");
   prof_begin(h);
-  hipLaunchKernelGGL(k_near_blocks, dim3(3 * p.Ne), dim3(nthr), 0, ns, nb);
+  hipLaunchKernelGGL(k_near_blocks, dim3(3 * p.Ne), dim3(256), 0, ns, nb, mlanes);
   prof_end(h, K_NEAR_BLOCKS, 0.0);
   LAUNCHCHK();
   // far
@@ -1322,19 +1343,20 @@ extern "C" int ghost_bor_set_solution(ghost_bor* h, int prob, int m, const doubl
 // 64 x 128 (FP64), double-buffered; 4: variant 1 with the C tile loaded
 // before the K loop.
 template <typename T, bool TA = false, bool TB = false>
-static void gemm_launch(ghost_bor* h, const GemmOp* ops, int nops, int mM, int mN, const void* A, const void* B,
+static void gemm_launch(ghost_bor* h, const GemmOp* ops, int nops, const int* tt, const void* A, const void* B,
                         void* C) {
-  if (mM <= 0 || mN <= 0) return;
   int v = h->gemm_variant;
   if (v < 0) v = sizeof(T) == 8 ? 4 : 1;
   if (v == 3 && sizeof(T) == 8) v = 2;
-  auto grid = [&](int bm, int bn) { return dim3((unsigned)(((mM + bm - 1) / bm) * ((mN + bn - 1) / bn)), nops); };
+  const int sh = (v == 2) ? 1 : (v == 3 ? 2 : 0);
+  if (tt[sh] <= 0) return;
+  const dim3 grid((unsigned)tt[sh]);
   switch (v) {
-    case 0: hipLaunchKernelGGL((k_gemm_update<T, TA, TB>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
-    case 1: hipLaunchKernelGGL((k_gemm2<T, 2, 2, TA, TB>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
-    case 2: hipLaunchKernelGGL((k_gemm2<T, 2, 4, TA, TB>), grid(64, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
-    case 4: hipLaunchKernelGGL((k_gemm2<T, 2, 2, TA, TB, true>), grid(64, 64), dim3(256), 0, h->stream, ops, A, B, C); break;
-    default: hipLaunchKernelGGL((k_gemm2<T, 4, 4, TA, TB>), grid(128, 128), dim3(256), 0, h->stream, ops, A, B, C); break;
+    case 0: hipLaunchKernelGGL((k_gemm_update<T, TA, TB>), grid, dim3(256), 0, h->stream, ops, nops, A, B, C); break;
+    case 1: hipLaunchKernelGGL((k_gemm2<T, 2, 2, TA, TB>), grid, dim3(256), 0, h->stream, ops, nops, A, B, C); break;
+    case 2: hipLaunchKernelGGL((k_gemm2<T, 2, 4, TA, TB>), grid, dim3(256), 0, h->stream, ops, nops, A, B, C); break;
+    case 4: hipLaunchKernelGGL((k_gemm2<T, 2, 2, TA, TB, true>), grid, dim3(256), 0, h->stream, ops, nops, A, B, C); break;
+    default: hipLaunchKernelGGL((k_gemm2<T, 4, 4, TA, TB>), grid, dim3(256), 0, h->stream, ops, nops, A, B, C); break;
   }
 }
 
@@ -1359,22 +1381,22 @@ template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launc
         hipLaunchKernelGGL((k_tri_inv<T, false>), dim3(L.nops), dim3(64), 0, h->stream, h->d_trsm + L.ops, factor, h->d_W);
         break;
       case L_GEMM_AUG_W:
-        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, h->d_W, factor, factor);
+        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.tt, h->d_W, factor, factor);
         break;
       case L_GEMM_AUG:
-        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, factor, factor, factor);
+        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.tt, factor, factor, factor);
         break;
       case L_GEMM_Y_W:
-        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, h->d_W, rhs, rhs);
+        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.tt, h->d_W, rhs, rhs);
         break;
       case L_GEMM_Y_LU:
-        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, factor, rhs, rhs);
+        gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.tt, factor, rhs, rhs);
         break;
       case L_GEMM_TW:
-        gemm_launch<T, true, false>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, h->d_W, rhs, rhs);
+        gemm_launch<T, true, false>(h, h->d_gemm + L.ops, L.nops, L.tt, h->d_W, rhs, rhs);
         break;
       case L_GEMM_TLU:
-        gemm_launch<T, true, false>(h, h->d_gemm + L.ops, L.nops, L.mM, L.mN, factor, rhs, rhs);
+        gemm_launch<T, true, false>(h, h->d_gemm + L.ops, L.nops, L.tt, factor, rhs, rhs);
         break;
       default: break;
     }
@@ -1419,13 +1441,11 @@ static int residual_gemm(ghost_bor* h, int x_in_aug) {
   hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_B64, h->d_R64);
   LAUNCHCHK();
   prof_begin(h);
-  int mM = 0, mN = 0;
-  for (auto& s : h->sys) { mM = std::max(mM, s.n); mN = std::max(mN, s.nrhs); }
   (void)gx;
   if (x_in_aug)
-    gemm_launch<double>(h, h->d_gemm + h->res_gemm_aug, nsys, mM, mN, h->d_A64, h->d_A, h->d_R64);
+    gemm_launch<double>(h, h->d_gemm + h->res_gemm_aug, nsys, h->res_aug_tt, h->d_A64, h->d_A, h->d_R64);
   else
-    gemm_launch<double>(h, h->d_gemm + h->res_gemm, nsys, mM, mN, h->d_A64, h->d_X64, h->d_R64);
+    gemm_launch<double>(h, h->d_gemm + h->res_gemm, nsys, h->res_tt, h->d_A64, h->d_X64, h->d_R64);
   prof_end(h, K_GEMM64, gemm_flops(h->h_gemm, x_in_aug ? h->res_gemm_aug : h->res_gemm, nsys, false));
   LAUNCHCHK();
   return 0;
@@ -2027,13 +2047,10 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   const bool prof = h->profile;
   h->profile = false;
   size_t maxn = 0, maxr = 0, nrow = 0;
-  int mM = 0, mN = 0;
   for (auto& s : h->sys) {
     maxn = std::max(maxn, (size_t)s.n);
     maxr = std::max(maxr, (size_t)s.nrhs);
     nrow += (size_t)s.n;
-    mM = std::max(mM, s.n);
-    mN = std::max(mN, s.n);
   }
   const dim3 gv((unsigned)((maxr + 255) / 256), (unsigned)maxn, nsys);
   // forward solution out of the augmented columns; the factor's row order
@@ -2070,7 +2087,7 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   // Q = [-Lambda X^T | Lambda]
   hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (h->a_elems + 255) / 256)), dim3(256), 0,
                      h->stream, h->d_Q, h->a_elems);
-  gemm_launch<double, false, true>(h, h->d_gemm + h->adj_q_ops, nsys, mM, mN, h->d_lam2, h->d_Xs, h->d_Q);
+  gemm_launch<double, false, true>(h, h->d_gemm + h->adj_q_ops, nsys, h->adj_q_tt, h->d_lam2, h->d_Xs, h->d_Q);
   hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_toaug, h->d_lam2, h->d_Q);
   LAUNCHCHK();
   // perturbed fills: side 0 moves the test geometry (rows, right-hand sides and

@@ -408,8 +408,11 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
   const int nnodes = 48 + pr.n_tail;
   const int n0 = cs * C;
 
-  __shared__ double nv[32][8][12];
-  __shared__ double st[32][8][16];   // e^{j n0 xi} per chunk
+  // per point-pair slot: 8 nodes x 12 values, and 8 nodes x 8 chunk starts
+  // e^{j n0 xi} (re, im); the slot strides (97, 129 doubles) are odd so the
+  // 8 slots of a wavefront fall on different LDS banks
+  __shared__ double nv[32][97];
+  __shared__ double st[32][129];
 
   double2 accG[C], accK[4][C];
 #pragma unroll
@@ -422,7 +425,7 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
   for (int v0 = 0; v0 < nnodes; v0 += 8) {
     // ── node value for node v0 + cs of this point pair ──
     const int vn = v0 + cs;
-    double* out = nv[slot][cs];
+    double* out = &nv[slot][12 * cs];
     if (valid && vn < nnodes) {
       double xi, wq, Rd;
       if (vn < 48) {
@@ -469,8 +472,8 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
       double zr = 1.0, zi = 0.0;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        st[slot][cs][2 * c] = zr;
-        st[slot][cs][2 * c + 1] = zi;
+        st[slot][16 * cs + 2 * c] = zr;
+        st[slot][16 * cs + 2 * c + 1] = zi;
         const double t = zr * cC - zi * sC;
         zi = zr * sC + zi * cC;
         zr = t;
@@ -480,15 +483,15 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
       for (int i = 0; i < 12; ++i) out[i] = 0.0;
       out[10] = 1.0;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) st[slot][cs][c] = 0.0;
+      for (int c = 0; c < 16; ++c) st[slot][16 * cs + c] = 0.0;
     }
     __syncthreads();
     // ── accumulate this thread's orders over the 8 nodes ──
     if (valid) {
       for (int u = 0; u < 8; ++u) {
-        const double* x = nv[slot][u];
+        const double* x = &nv[slot][12 * u];
         const double c1 = x[10], s1 = x[11];
-        double cr = st[slot][u][2 * cs], ci = st[slot][u][2 * cs + 1];
+        double cr = st[slot][16 * u + 2 * cs], ci = st[slot][16 * u + 2 * cs + 1];
 #pragma unroll
         for (int i = 0; i < C; ++i) {
           // EFIE and even brackets project on cos(n xi), odd brackets on sin(n xi)
@@ -548,14 +551,17 @@ struct NearBlockArgs {
   double2* NB;           // [3 Ne][16][2M+1]
 };
 
-// one work-group per near pair slot, one thread per |m|
-__global__ __launch_bounds__(128) void k_near_blocks(NearBlockArgs b) {
+// one work-group per near pair slot; thread = (|m| lane, cell group): with
+// mlanes >= M + 1 lanes per group, blockDim / mlanes groups share the cells
+// (small M would otherwise leave most threads idle) and are summed in LDS
+__global__ __launch_bounds__(256) void k_near_blocks(NearBlockArgs b, int mlanes) {
   const NearArgs& a = b.na;
   const int slot = blockIdx.x;
   const NearPairInfo pr = a.pairs[slot];
   if (pr.e < 0) return;
-  const int am = threadIdx.x;
-  if (am > a.M) return;
+  const int G = (int)blockDim.x / mlanes;
+  const int am = (int)threadIdx.x % mlanes, grp = (int)threadIdx.x / mlanes;
+  const bool act = am <= a.M && grp < G;
   const int e = pr.e, f = pr.f;
   const double trp = a.trho[e], tzp = a.tz[e], trq = a.strho[f], tzq = a.stz[f];
   const double Le = a.L[e], Lf = a.sL[f];
@@ -567,7 +573,7 @@ __global__ __launch_bounds__(128) void k_near_blocks(NearBlockArgs b) {
 #pragma unroll
   for (int i = 0; i < 16; ++i) { E[i] = make_double2(0, 0); K[i] = make_double2(0, 0); }
   const int om1 = am > 0 ? am - 1 : 1;
-  for (int q = 0; q < cnt; ++q) {
+  for (int q = act ? grp : cnt; q < cnt; q += G) {
     const double s = a.cell_s[off + q], sp = a.cell_sp[off + q];
     const double W = a.cell_w[off + q] * Le * Lf;
     double rp, zp, rq, zq;
@@ -609,6 +615,25 @@ __global__ __launch_bounds__(128) void k_near_blocks(NearBlockArgs b) {
         }
       }
   }
+  // sum the cell groups (group 0 keeps the totals)
+  __shared__ double2 red[256][17];
+#define GROUP_SUM(V)                                          \
+  {                                                           \
+    _Pragma("unroll") for (int i = 0; i < 16; ++i) red[threadIdx.x][i] = V[i]; \
+    __syncthreads();                                          \
+    if (grp == 0 && act)                                      \
+      for (int g = 1; g < G; ++g)                             \
+        _Pragma("unroll") for (int i = 0; i < 16; ++i) {      \
+          const double2 w = red[g * mlanes + am][i];          \
+          V[i].x += w.x;                                      \
+          V[i].y += w.y;                                      \
+        }                                                     \
+    __syncthreads();                                          \
+  }
+  GROUP_SUM(E)
+  GROUP_SUM(K)
+#undef GROUP_SUM
+  if (grp != 0 || !act) return;
   // mass term 2 pi int rho T_a T_b dt on the element itself (diagonal pairs),
   // a single integral taken over the test geometry
   double Mab[4] = {0, 0, 0, 0};

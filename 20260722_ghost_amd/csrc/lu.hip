@@ -15,12 +15,29 @@ struct GemmOp {      // C[M x N] -= A[M x K] B[K x N], complex, row-major
   long long a, b, c; // element offsets from the respective base pointers
   int lda, ldb, ldc;
   int M, N, K;
+  int t0[3];         // first block of this op in its launch, per tile shape
+                     // (64x64, 64x128, 128x128): launches are flattened over
+                     // the tiles of all their ops, so systems of uneven size
+                     // cost no idle blocks
 };
+
+// Op of flattened block `bid`: the last op whose first block is <= bid (ops
+// with no tiles share the next op's start and are skipped).
+__device__ __forceinline__ int find_op(const GemmOp* ops, int nops, int shape, int bid) {
+  int lo = 0, hi = nops - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ops[mid].t0[shape] <= bid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 
 struct PanelOp {     // factor rows [k, n) x cols [k, k + w)
   long long a;       // system base (element offset)
   int n, ld, k, w;
   int piv;           // offset of this system's pivot array
+  int sys;           // system index (zero-pivot count)
 };
 
 struct SwapOp {      // apply pivots k..k+w-1 to cols [0, c_lo) and [c_hi, ncols)
@@ -40,16 +57,17 @@ struct TrsmOp {      // diagonal block T (w x w) of a factor and the slot of its
 
 // TA / TB: the operand is stored transposed (A as K x M with lda, B as N x K with ldb).
 template <typename T, bool TA = false, bool TB = false>
-__global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, const void* Abase,
+__global__ __launch_bounds__(256) void k_gemm_update(const GemmOp* ops, int nops, const void* Abase,
                                                      const void* Bbase, void* Cbase) {
   typedef typename CT<T>::c C;
   typedef typename CT<T>::v4 V4;
-  const GemmOp op = ops[blockIdx.y];
+  const GemmOp op = ops[find_op(ops, nops, 0, (int)blockIdx.x)];
   if (op.M <= 0 || op.N <= 0 || op.K <= 0) return;
   const int tn = (op.N + 63) >> 6;
   const int tiles = ((op.M + 63) >> 6) * tn;
-  if ((int)blockIdx.x >= tiles) return;
-  const int m0 = (blockIdx.x / tn) * 64, n0 = (blockIdx.x % tn) * 64;
+  const int tile = (int)blockIdx.x - op.t0[0];
+  if (tile < 0 || tile >= tiles) return;
+  const int m0 = (tile / tn) * 64, n0 = (tile % tn) * 64;
   const C* A = (const C*)Abase + op.a;
   const C* B = (const C*)Bbase + op.b;
   C* Cm = (C*)Cbase + op.c;
@@ -178,18 +196,20 @@ __global__ __launch_bounds__(256) void k_gemm_n2(const GemmOp* ops, const double
 // tiles each; K in chunks of 16 staged through two LDS buffers with the next
 // chunk's global loads in flight while the current one feeds the MFMAs.
 template <typename T, int WM, int WN, bool TA = false, bool TB = false, bool PRE = false>
-__global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, const void* Abase, const void* Bbase,
-                                               void* Cbase) {
+__global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, int nops, const void* Abase,
+                                               const void* Bbase, void* Cbase) {
   typedef typename CT<T>::c C;
   typedef typename CT<T>::v4 V4;
   constexpr int BM = 32 * WM, BN = 32 * WN, KC = 16;
   constexpr int AP = BM * KC / 256, BP = BN * KC / 256;   // complex elements per thread
-  const GemmOp op = ops[blockIdx.y];
+  constexpr int SH = (BM == 64 && BN == 64) ? 0 : (BM == 64 ? 1 : 2);
+  const GemmOp op = ops[find_op(ops, nops, SH, (int)blockIdx.x)];
   if (op.M <= 0 || op.N <= 0 || op.K <= 0) return;
   const int tn = (op.N + BN - 1) / BN;
   const int tiles = ((op.M + BM - 1) / BM) * tn;
-  if ((int)blockIdx.x >= tiles) return;
-  const int m0 = (blockIdx.x / tn) * BM, n0 = (blockIdx.x % tn) * BN;
+  const int tile = (int)blockIdx.x - op.t0[SH];
+  if (tile < 0 || tile >= tiles) return;
+  const int m0 = (tile / tn) * BM, n0 = (tile % tn) * BN;
   const C* A = (const C*)Abase + op.a;
   const C* B = (const C*)Bbase + op.b;
   C* Cm = (C*)Cbase + op.c;
@@ -402,7 +422,7 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
         int bi = red_i[0];
         for (int q = 1; q < 16; ++q)
           if (red_v[q] > b || (red_v[q] == b && red_i[q] < bi)) { b = red_v[q]; bi = red_i[q]; }
-        if (!(b > (T)0)) { bi = col; if (info) atomicAdd(info + blockIdx.x, 1); }
+        if (!(b > (T)0)) { bi = col; if (info) atomicAdd(info + op.sys, 1); }
         piv_row = bi;
         ipiv[col] = bi;
         piv_s[jj] = bi;

@@ -94,6 +94,23 @@ def cone_sphere(n_elems: int = 600, half_angle_deg: float = 15.0, radius: float 
     return _resample(np.vstack([cone, sph]), n_elems)
 
 
+def frequency_sweep(n_freq: int = 64, seed: int = 7, n_theta: int = 181):
+    """Config 3: one seeded smooth body (rho in [0.05, 0.2] m over 1 m) at
+    n_freq frequencies geomspace(0.2, 2 GHz).  The mesh grows with frequency
+    (50 -> ~1000 elements: systems of ~100 -> ~2000 unknowns), m_max =
+    ceil(1.2 k rho_max) + 5 (7 -> 16), n_theta aspects x 2 pols.  Returns
+    solve_bor_batch job dicts."""
+    jobs = []
+    for f in np.geomspace(0.2e9, 2e9, n_freq):
+        ne = int(round(50 * (f / 0.2e9) ** 1.3))
+        nodes = random_smooth(1.0, 0.05, 0.2, ne, seed)
+        k = 2.0 * math.pi * f / C0
+        m_max = int(math.ceil(1.2 * k * 0.2)) + 5     # rho bound 0.2 m
+        jobs.append({"points": nodes, "freq_hz": float(f), "thetas_deg": np.linspace(0.0, 180.0, n_theta),
+                     "formulation": "cfie", "cfie_alpha": 0.5, "n_modes": m_max})
+    return jobs
+
+
 def config(index: int):
     """(name, nodes, freq_hz, thetas_deg, m_max) of a benchmark configuration."""
     if index == 1:

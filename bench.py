@@ -209,6 +209,99 @@ def run_adjoint(steps):
     print(json.dumps(out), flush=True)
 
 
+def run_sweep(steps, warmup):
+    """Config 3: 64 frequencies of one body in ONE batch (uneven systems of
+    ~100-2000 unknowns), eagerly and as a replayed hipGraph, against the same
+    problems run one handle per frequency.  Mixed precision, no refinement
+    sweep (a captured graph has a fixed schedule); dB error vs the FP64 batch.
+    Prints its own JSON line."""
+    wl = pkg("workloads")
+    bs = pkg("bor_solver")
+    L = pkg("_lib")
+    jobs = wl.frequency_sweep()
+    specs = []
+    for j in jobs:
+        k = 2 * math.pi * j["freq_hz"] / bs.C0
+        m_max, ne, nbk, _ = bs.discretisation(j["points"], k, j["thetas_deg"], j["n_modes"], "cfie")
+        specs.append((j["points"], k, m_max, ne, nbk, j["thetas_deg"]))
+    solves = sum((2 * s[2] + 1) * len(s[5]) for s in specs)
+    nmax = max(2 * len(s[0]) for s in specs)
+    log(f"sweep: {len(specs)} frequencies, {sum(2 * s[2] + 1 for s in specs)} systems (n up to ~{nmax})")
+
+    def handle(prec, subset):
+        h = L.Handle(prec, nb=64, residual=False)
+        for (pts, k, m_max, ne, nbk, th) in subset:
+            h.add_problem(pts, k, m_max, ne, nbk, 0.5, 0.5 * bs.ETA0, th)
+        h.build()
+        return h
+
+    h = handle(L.GHOST_MIXED, specs)
+    for _ in range(warmup):
+        h.run(0)
+    h.sync()
+    L.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.run(0)
+    h.sync()
+    L.device_synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    dF_mx = [h.dF(i) for i in range(len(specs))]
+    h.capture(0)
+    h.launch()
+    h.sync()
+    L.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.launch()
+    h.sync()
+    L.device_synchronize()
+    dt_graph = (time.perf_counter() - t0) / steps
+    dF_graph = [h.dF(i) for i in range(len(specs))]
+    h.close()
+    log(f"sweep: batch {dt * 1e3:.1f} ms, graph {dt_graph * 1e3:.1f} ms")
+    # one handle per frequency, run one after the other (the reference's
+    # frequency loop), each after a warm-up run
+    hs = [handle(L.GHOST_MIXED, [s]) for s in specs]
+    for hh in hs:
+        hh.run(0)
+    L.device_synchronize()
+    t0 = time.perf_counter()
+    for hh in hs:
+        hh.run(0)
+        hh.sync()
+    L.device_synchronize()
+    dt_loop = time.perf_counter() - t0
+    for hh in hs:
+        hh.close()
+    log(f"sweep: per-frequency loop {dt_loop * 1e3:.1f} ms")
+    h64 = handle(L.GHOST_FP64, specs)
+    h64.run(0)
+    h64.sync()
+    err = 0.0
+    graph_same = True
+    for i, s in enumerate(specs):
+        F64 = bs.truncate_modes(h64.dF(i), s[2], 0.0)[0]
+        Fmx = bs.truncate_modes(dF_mx[i], s[2], 0.0)[0]
+        err = max(err, float(np.max(np.abs(db(Fmx) - db(F64)))))
+        graph_same = graph_same and bool(np.array_equal(dF_graph[i], dF_mx[i]))
+    h64.close()
+    out = {
+        "metric": "BOR frequency-sweep solves/s (mode x freq x angle), 64 frequencies in one batch",
+        "value": round(solves / dt, 2), "unit": "solves/s", "ms_per_step": round(dt * 1e3, 3),
+        "graph_ms_per_step": round(dt_graph * 1e3, 3), "graph_solves_per_s": round(solves / dt_graph, 2),
+        "per_frequency_loop_ms": round(dt_loop * 1e3, 3),
+        "speedup_batch_vs_loop": round(dt_loop / dt, 2), "graph_replay_equals_eager": graph_same,
+        "err_vs_fp64_max_abs_db": err, "steps": steps, "warmup": warmup,
+        "config": {"workload": "smooth-body-1m x 64 frequencies 0.2-2 GHz", "frequencies": len(specs),
+                   "systems": sum(2 * s[2] + 1 for s in specs), "solves_per_step": solves,
+                   "n_elems": [len(specs[0][0]) - 1, len(specs[-1][0]) - 1],
+                   "m_max": [specs[0][2], specs[-1][2]], "aspects": len(specs[0][5]),
+                   "precision": "mixed", "refine_iters": 0},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_c0():
     return pkg("bor_solver").C0
 
@@ -239,6 +332,9 @@ def main():
         return
     if args.adjoint:
         run_adjoint(max(1, args.steps))
+        return
+    if args.config == 3:
+        run_sweep(max(1, args.steps), args.warmup)
         return
     wl = pkg("workloads")
     name, nodes, freq, thetas, m_max = wl.config(args.config)

@@ -72,3 +72,47 @@ def test_adjoint_gradient_matches_oracle_differences():
     fd = (vals[0] - vals[1]) / (2 * h)
     print(f"[case] oracle fd {fd:.9e} adjoint {g[q]:.9e}", flush=True)
     assert abs(g[q] - fd) <= 1e-6 * abs(fd)
+
+
+def test_adjoint_of_a_batch_equals_single_problem_adjoints():
+    """Two problems (different bodies and frequencies) in one handle: the
+    batched adjoint returns each problem's gradient as its own handle would."""
+    L = pkg("_lib")
+    bs = pkg("bor_solver")
+    wl = pkg("workloads")
+    probs = [(wl.cone_sphere(30, 15.0, 0.3), 3e8, 5, np.linspace(5.0, 175.0, 7)),
+             (wl.random_smooth(1.0, 0.05, 0.2, 36, 3), 4.5e8, 6, np.linspace(0.0, 180.0, 5))]
+
+    def setup(h, nodes, f, mm, th):
+        k = 2 * math.pi * f / bs.C0
+        _, ne, nb, _ = bs.discretisation(nodes, k, th, mm, "cfie")
+        h.add_problem(nodes, k, mm, ne, nb, 0.5, 0.5 * bs.ETA0, th)
+
+    def acoef(dF, w):
+        F = dF.sum(axis=0)
+        a = w * (20.0 / math.log(10.0)) * np.conj(F) / np.abs(F) ** 2
+        return np.ascontiguousarray(a.T).ravel()
+
+    rng = np.random.default_rng(4)
+    ws = [rng.uniform(0.5, 1.5, size=(2, p[3].size)) for p in probs]
+    single = []
+    for p, w in zip(probs, ws):
+        h = L.Handle(L.GHOST_FP64, residual=False)
+        setup(h, *p)
+        h.build()
+        h.run(0)
+        h.sync()
+        single.append(h.adjoint(acoef(h.dF(0), w), 0.0))
+        h.close()
+    h = L.Handle(L.GHOST_FP64, residual=False)
+    for p in probs:
+        setup(h, *p)
+    h.build()
+    h.run(0)
+    h.sync()
+    a = np.concatenate([acoef(h.dF(i), w) for i, w in enumerate(ws)])
+    g = h.adjoint(a, 0.0)
+    h.close()
+    n0 = probs[0][0].shape[0]
+    for got, want in ((g[:n0], single[0]), (g[n0:], single[1])):
+        assert np.max(np.abs(got - want)) <= 1e-9 * np.max(np.abs(want))
