@@ -427,25 +427,34 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
     const int vn = v0 + cs;
     double* out = &nv[slot][12 * cs];
     if (valid && vn < nnodes) {
-      double xi, wq, Rd;
+      // xi enters through s = sin(xi/2) only: cos xi = 1 - 2 s^2 and
+      // sin xi = 2 s cos(xi/2), with cos(xi/2) = sqrt(1 - s^2) on the core
+      // (xi = 2 asin s, s <= 1), so neither asin nor a sincos of xi is needed
+      double wq, Rd, sx, cx;
       if (vn < 48) {
         const double vv = a.core_u[vn] * vmax, wv = a.core_w[vn] * vmax;
-        double sv = (d / aa) * sinh(vv);
+        const double em = expm1(vv), ev = em + 1.0;
+        const double shv = 0.5 * (em + em / ev), chv = 0.5 * (ev + 1.0 / ev);   // sinh, cosh
+        double sv = (d / aa) * shv;
         sv = sv < 1.0 ? sv : 1.0;
-        xi = 2.0 * asin(sv);
-        wq = wv * 2.0 * (d / aa) * cosh(vv) / sqrt(fmax(1.0 - sv * sv, 1e-15));
+        const double cv = sqrt(fmax(1.0 - sv * sv, 0.0));
+        wq = wv * 2.0 * (d / aa) * chv / fmax(cv, sqrt(1e-15));
         Rd = sqrt(d * d + (aa * sv) * (aa * sv));
+        sx = 2.0 * sv * cv;
+        cx = 1.0 - 2.0 * sv * sv;
       } else {
         const int ti = pr.tail_off + (vn - 48);
-        xi = xi0 + a.tail_u[ti] * span;
+        const double xi = xi0 + a.tail_u[ti] * span;
         wq = a.tail_w[ti] * span;
-        const double sh = sin(0.5 * xi);
+        double sh, ch;
+        sincos(0.5 * xi, &sh, &ch);
         Rd = sqrt(d * d + (aa * sh) * (aa * sh));
+        sx = 2.0 * sh * ch;
+        cx = 1.0 - 2.0 * sh * sh;
       }
       Rd = Rd > 1e-300 ? Rd : 1e-300;
-      double sk, ck, sx, cx;
+      double sk, ck;
       sincos(k * Rd, &sk, &ck);
-      sincos(xi, &sx, &cx);
       const double inv = 1.0 / Rd;
       // EFIE: 2 g w  (axis pairs use the closed form instead)
       const double ge = axis ? 0.0 : 2.0 * wq * inv / (4.0 * GH_PI);
@@ -466,9 +475,22 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
       out[6] = bft * pr_; out[7] = bft * pi_;
       out[8] = bff * pr_; out[9] = bff * pi_;
       out[10] = cx; out[11] = sx;
-      // chunk starts e^{j c C xi}, c = 0..7
-      double sC, cC;
-      sincos((double)C * xi, &sC, &cC);
+      // chunk starts e^{j c C xi}, c = 0..7, from e^{j C xi} = (e^{j xi})^C
+      double cC = 1.0, sC = 0.0;
+      {
+        double br = cx, bi = sx;
+#pragma unroll
+        for (int bit = 1; bit <= C; bit <<= 1) {
+          if (C & bit) {
+            const double t = cC * br - sC * bi;
+            sC = cC * bi + sC * br;
+            cC = t;
+          }
+          const double t = br * br - bi * bi;
+          bi = 2.0 * br * bi;
+          br = t;
+        }
+      }
       double zr = 1.0, zi = 0.0;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
@@ -491,18 +513,29 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
       for (int u = 0; u < 8; ++u) {
         const double* x = &nv[slot][12 * u];
         const double c1 = x[10], s1 = x[11];
-        double cr = st[slot][16 * u + 2 * cs], ci = st[slot][16 * u + 2 * cs + 1];
+        const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * c1 * s1;   // e^{2 j xi}
+        // two interleaved rotation chains (even / odd orders of the chunk),
+        // each advanced by e^{2 j xi}: half the serial dependency depth
+        double cr0 = st[slot][16 * u + 2 * cs], ci0 = st[slot][16 * u + 2 * cs + 1];
+        double cr1 = cr0 * c1 - ci0 * s1, ci1 = cr0 * s1 + ci0 * c1;
 #pragma unroll
         for (int i = 0; i < C; ++i) {
+          const double cr = (i & 1) ? cr1 : cr0, ci = (i & 1) ? ci1 : ci0;
           // EFIE and even brackets project on cos(n xi), odd brackets on sin(n xi)
           accG[i].x += x[0] * cr;  accG[i].y += x[1] * cr;
           accK[0][i].x += x[2] * cr; accK[0][i].y += x[3] * cr;
           accK[1][i].x += x[4] * ci; accK[1][i].y += x[5] * ci;
           accK[2][i].x += x[6] * ci; accK[2][i].y += x[7] * ci;
           accK[3][i].x += x[8] * cr; accK[3][i].y += x[9] * cr;
-          const double t = cr * c1 - ci * s1;
-          ci = cr * s1 + ci * c1;
-          cr = t;
+          if (i & 1) {
+            const double t = cr1 * c2 - ci1 * s2;
+            ci1 = cr1 * s2 + ci1 * c2;
+            cr1 = t;
+          } else {
+            const double t = cr0 * c2 - ci0 * s2;
+            ci0 = cr0 * s2 + ci0 * c2;
+            cr0 = t;
+          }
         }
       }
     }
