@@ -359,6 +359,30 @@ template <typename T> __device__ __forceinline__ typename CT<T>::c cdiv1(typenam
   return r;
 }
 
+// SW contiguous elements of one sub-panel row in 16-byte accesses.  The
+// segment is 16-byte aligned: matrix offsets and leading dimensions are
+// multiples of 16 elements and sub-panels start at multiples of SW (even).
+template <int SW> __device__ __forceinline__ void load_seg(const float2* p, float2 (&v)[SW]) {
+#pragma unroll
+  for (int c = 0; c < SW; c += 2) {
+    const float4 q = *(const float4*)(p + c);
+    v[c] = make_float2(q.x, q.y);
+    v[c + 1] = make_float2(q.z, q.w);
+  }
+}
+template <int SW> __device__ __forceinline__ void load_seg(const double2* p, double2 (&v)[SW]) {
+#pragma unroll
+  for (int c = 0; c < SW; ++c) v[c] = p[c];
+}
+template <int SW> __device__ __forceinline__ void store_seg(float2* p, const float2 (&v)[SW]) {
+#pragma unroll
+  for (int c = 0; c < SW; c += 2) *(float4*)(p + c) = make_float4(v[c].x, v[c].y, v[c + 1].x, v[c + 1].y);
+}
+template <int SW> __device__ __forceinline__ void store_seg(double2* p, const double2 (&v)[SW]) {
+#pragma unroll
+  for (int c = 0; c < SW; ++c) p[c] = v[c];
+}
+
 template <typename T, int RPT, int SW>
 __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* base, int* pivots, int* info) {
   typedef typename CT<T>::c C;
@@ -369,9 +393,8 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
   const int n = op.n, ld = op.ld, k = op.k, w = op.w;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
-  __shared__ T red_v[16];
-  __shared__ int red_i[16];
-  __shared__ int piv_row;
+  __shared__ T red_v[2][16];
+  __shared__ int red_i[2][16];
   __shared__ C prow[SW];
   __shared__ C srow[SW];
   __shared__ C L11[SW][SW];
@@ -385,12 +408,16 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int row = c0 + tid + 1024 * i;
+      if (ws == SW && row < n) {
+        load_seg<SW>(A + (size_t)row * ld + c0, v[i]);
+      } else {
 #pragma unroll
-      for (int c = 0; c < SW; ++c) {
-        C z;
-        z.x = 0; z.y = 0;
-        if (row < n && c < ws) z = A[(size_t)row * ld + c0 + c];
-        v[i][c] = z;
+        for (int c = 0; c < SW; ++c) {
+          C z;
+          z.x = 0; z.y = 0;
+          if (row < n && c < ws) z = A[(size_t)row * ld + c0 + c];
+          v[i][c] = z;
+        }
       }
     }
     for (int jj = 0; jj < ws; ++jj) {
@@ -415,20 +442,29 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
         const int oi = __shfl_xor(bidx, off);
         if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
       }
-      if (lane == 0) { red_v[wv] = best; red_i[wv] = bidx; }
+      // the 16 wave maxima, double-buffered by column parity: a wave writes the
+      // next column's entry only after every wave has passed this column's
+      // second barrier, i.e. after every read of this buffer
+      if (lane == 0) { red_v[jj & 1][wv] = best; red_i[jj & 1][wv] = bidx; }
       __syncthreads();
-      if (tid == 0) {
-        T b = red_v[0];
-        int bi = red_i[0];
-        for (int q = 1; q < 16; ++q)
-          if (red_v[q] > b || (red_v[q] == b && red_i[q] < bi)) { b = red_v[q]; bi = red_i[q]; }
-        if (!(b > (T)0)) { bi = col; if (info) atomicAdd(info + op.sys, 1); }
-        piv_row = bi;
-        ipiv[col] = bi;
-        piv_s[jj] = bi;
+      // every wave reduces the 16 entries itself (lanes 0-15, then a broadcast)
+      best = lane < 16 ? red_v[jj & 1][lane] : (T)-1;
+      bidx = lane < 16 ? red_i[jj & 1][lane] : 0x7fffffff;
+#pragma unroll
+      for (int off = 8; off >= 1; off >>= 1) {
+        const T ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bidx, off);
+        if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
       }
-      __syncthreads();
-      const int pr = piv_row;
+      best = __shfl(best, 0);
+      bidx = __shfl(bidx, 0);
+      const bool singular = !(best > (T)0);
+      const int pr = singular ? col : bidx;
+      if (tid == 0) {
+        if (singular && info) atomicAdd(info + op.sys, 1);
+        ipiv[col] = pr;
+        piv_s[jj] = pr;
+      }
       // ── interchange rows col and pr inside the sub-panel ──
       const int oc = col - c0, op_ = pr - c0;
       if ((oc & 1023) == tid) {
@@ -483,16 +519,23 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
           }
         }
       }
-      __syncthreads();
+      // no barrier here: prow / srow are rewritten only after the next column's
+      // first barrier, which every wave reaches after finishing this update
     }
+    __syncthreads();
     // ── write the sub-panel back; keep L11 ──
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int row = c0 + tid + 1024 * i;
-      if (row < n)
+      if (row < n) {
+        if (ws == SW) {
+          store_seg<SW>(A + (size_t)row * ld + c0, v[i]);
+        } else {
 #pragma unroll
-        for (int c = 0; c < SW; ++c)
-          if (c < ws) A[(size_t)row * ld + c0 + c] = v[i][c];
+          for (int c = 0; c < SW; ++c)
+            if (c < ws) A[(size_t)row * ld + c0 + c] = v[i][c];
+        }
+      }
     }
     if (tid < ws) {
 #pragma unroll
@@ -519,33 +562,42 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
       // U12 = L11^{-1} A12 (unit lower), one thread per right column
       if (tid < nright) {
         const int cc = c0 + ws + tid;
-        C x[SW];
 #pragma unroll
         for (int r = 0; r < SW; ++r) {
           if (r < ws) {
             C z = A[(size_t)(c0 + r) * ld + cc];
 #pragma unroll
             for (int q = 0; q < SW; ++q)
-              if (q < r) z = csub(z, cmul(L11[r][q], x[q]));
-            x[r] = z;
+              if (q < r) z = csub(z, cmul(L11[r][q], U12[q][tid]));
             A[(size_t)(c0 + r) * ld + cc] = z;
             U12[r][tid] = z;
           }
         }
       }
       __syncthreads();
-      // rank-ws update of rows [c0 + ws, n) on the right columns
-#pragma unroll
-      for (int i = 0; i < RPT; ++i) {
-        const int row = c0 + tid + 1024 * i;
-        if (row >= c0 + ws && row < n) {
-          C* p = A + (size_t)row * ld + c0 + ws;
-          for (int c = 0; c < nright; ++c) {
-            C acc = p[c];
+      // rank-ws update of rows [c0 + ws, n) on the right columns (nright <= 64 - ws):
+      // 8 lanes per row over the columns with stride 8, so every access of a wave
+      // covers 8 rows x 8 contiguous elements; the multipliers of the row are
+      // read back from the sub-panel just written (same block, after the barrier)
+      {
+        const int rsub = tid >> 3, cq = tid & 7;
+        for (int row = c0 + ws + rsub; row < n; row += 128) {
+          const C* lrow = A + (size_t)row * ld + c0;
+          C l[SW];
+          if (ws == SW) {
+            load_seg<SW>(lrow, l);
+          } else {
 #pragma unroll
             for (int q = 0; q < SW; ++q)
-              if (q < ws) acc = csub(acc, cmul(v[i][q], U12[q][c]));
-            p[c] = acc;
+              if (q < ws) l[q] = lrow[q];
+          }
+          C* p = A + (size_t)row * ld + c0 + ws;
+          for (int c = cq; c < nright; c += 8) {
+            C z = p[c];
+#pragma unroll
+            for (int q = 0; q < SW; ++q)
+              if (q < ws) z = csub(z, cmul(l[q], U12[q][c]));
+            p[c] = z;
           }
         }
       }
