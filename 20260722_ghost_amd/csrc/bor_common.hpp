@@ -51,24 +51,27 @@ __device__ __forceinline__ void sincos_s(float x, float* s, float* c) {
 // the value of lane (c - N) mod 16 of the same row.
 template <int N> __device__ __forceinline__ double row_ror(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_mov_dpp(lo, 0x120 + N, 0xF, 0xF, false);
-  hi = __builtin_amdgcn_mov_dpp(hi, 0x120 + N, 0xF, 0xF, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, 0x120 + N, 0xF, 0xF, true);
+  hi = __builtin_amdgcn_mov_dpp(hi, 0x120 + N, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 template <int N> __device__ __forceinline__ float row_ror(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + N, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + N, 0xF, 0xF, true));
 }
 
 // Value of the DPP partner lane (ctrl: row_mirror 0x140, row_half_mirror
-// 0x141, quad_perm codes 0x00-0xFF), 64-bit and 32-bit.
+// 0x141, row_ror 0x121-0x12F, quad_perm codes 0x00-0xFF), 64-bit and 32-bit.
+// Only permutations, where every lane has a source lane: bound_ctrl then
+// changes nothing, and setting it lets the compiler fold the move into the
+// instruction that uses the value (VOP2 with a DPP operand).
 template <int CTRL> __device__ __forceinline__ double dpp_mov(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
-  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 template <int CTRL> __device__ __forceinline__ float dpp_mov(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }

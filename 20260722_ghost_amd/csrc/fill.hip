@@ -39,50 +39,13 @@ struct FarArgs {
   void* Z;               // Zfar [2Nn][2Nn][2M+1] complex<R>
 };
 
-// Sum 32 per-lane values over the 16 lanes of a DPP row and scatter the
-// result: lane r (= lane & 15) receives the sums of values r and 16 + r.
-// Butterfly reduce-scatter whose stage partners are DPP involutions that flip
-// one lane bit: row_mirror (bit 3), row_half_mirror (bit 2), quad_perm
-// [3,2,1,0] (bit 1) and [1,0,3,2] (bit 0).  At each stage a lane keeps the
-// half of its values whose index bit equals its own lane bit.
+// Reduce-scatter step between two lanes of a DPP row whose lane bit differs:
+// a lane keeps the value of the pair selected by its own bit (b) and adds the
+// partner's copy of the same one.  CTRL is the partner permutation.
 template <int CTRL, typename R>
 __device__ __forceinline__ R rs_step(R lo, R hi, bool b) {
   const R keep = b ? hi : lo, send = b ? lo : hi;
   return keep + dpp_mov<CTRL>(send);
-}
-
-template <typename R>
-__device__ __forceinline__ void row_reduce_scatter32(const R (&v)[32], int lane, R& out_lo, R& out_hi) {
-  R a16[16];
-  {
-    const bool b = (lane >> 3) & 1;
-#pragma unroll
-    for (int i = 0; i < 32; ++i) {
-      if (i & 8) continue;
-      a16[(i & 7) | ((i >> 4) << 3)] = rs_step<0x140>(v[i], v[i | 8], b);
-    }
-  }
-  R a8[8];
-  {
-    const bool b = (lane >> 2) & 1;
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      if (i & 4) continue;
-      a8[(i & 3) | ((i >> 3) << 2)] = rs_step<0x141>(a16[i], a16[i | 4], b);
-    }
-  }
-  R a4[4];
-  {
-    const bool b = (lane >> 1) & 1;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      if (i & 2) continue;
-      a4[(i & 1) | ((i >> 2) << 1)] = rs_step<0x1B>(a8[i], a8[i | 2], b);
-    }
-  }
-  const bool b = lane & 1;
-  out_lo = rs_step<0xB1>(a4[0], a4[1], b);
-  out_hi = rs_step<0xB1>(a4[2], a4[3], b);
 }
 
 // One wavefront per far element pair (e, f); 2 wavefronts per block.
@@ -250,9 +213,28 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
   {
     const R trp = (R)P.trho[ip], tzp = (R)P.tz[ip], trq = (R)Q.trho[iq], tzq = (R)Q.tz[iq];
     const R wp = (R)(P.w[ip] * P.rho[ip]), wq = (R)(Q.w[iq] * Q.rho[iq]);
-    const R Wa0 = (R)P.T0[ip] * wp, Wa1 = (R)P.T1[ip] * wp;
-    const R Wb0 = (R)Q.T0[iq] * wq, Wb1 = (R)Q.T1[iq] * wq;
-    const R Wab[4] = {Wa0 * Wb0, Wa0 * Wb1, Wa1 * Wb0, Wa1 * Wb1};
+    // The nodal weight of the point pair factorises, W_ab = Wa_a(g) Wb_b(h), so
+    // the sum over the 16 point pairs of a DPP row runs as two 2-bit
+    // reduce-scatters: over h with Wb (scattering b and reim), then over g
+    // with Wa (scattering a and uvsub).  The lane bits of (a, uvsub, b, reim)
+    // follow the accumulator row of the transform, mrow(a + 2 uvsub, reim + 2 b):
+    // FP32 uvsub 3, a 2, b 1, reim 0 (g = bits 3:2, h = bits 1:0); FP64 b 3,
+    // reim 2, uvsub 1, a 0 (h = bits 3:2, g = bits 1:0).  Partners: bit 0
+    // quad_perm [1,0,3,2], bit 1 quad_perm [2,3,0,1], bit 2 row_ror 4 (run
+    // while bit 3 is still unscattered: the partner i +- 4 may differ there),
+    // bit 3 row_ror 8.
+    const bool f32 = sizeof(R) == 4;
+    const int bb = f32 ? (lane >> 1) & 1 : (lane >> 3) & 1;    // b
+    const int ab = f32 ? (lane >> 2) & 1 : lane & 1;           // a
+    const bool rb = f32 ? (lane & 1) : ((lane >> 2) & 1);       // reim
+    const bool ub = f32 ? ((lane >> 3) & 1) : ((lane >> 1) & 1);  // uvsub
+    const R Wa[2] = {(R)P.T0[ip] * wp, (R)P.T1[ip] * wp};
+    const R Wb[2] = {(R)Q.T0[iq] * wq, (R)Q.T1[iq] * wq};
+    const R Wak = Wa[ab], Was = Wa[ab ^ 1], Wbk = Wb[bb], Wbs = Wb[bb ^ 1];
+    // the partner's weight for this lane's a (b): its own "other" weight, so
+    // a stage is Wk x + Wp dpp(x) (the move folds into the multiply)
+    const R Wap = f32 ? dpp_mov<0x124>(Was) : dpp_mov<0xB1>(Was);
+    const R Wbp = dpp_mov<0x4E>(Wbs);   // FP32 b stage (bit 1)
     const int hB = a.nB >> 1, mskB = a.nB - 1;
     const R scB = (R)1 / ((R)2 * (R)a.nB);  // (2 pi/N)/(4 pi)
     for (int j0 = 0; j0 <= hB; j0 += 4) {
@@ -272,27 +254,43 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
       const R WtR = trp * Rx + tzp * dz, nR = -tzp * Rx + trp * dz;
       const R n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
       const R Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
-      const R btt = -(WtR * n_t - Wt_t * nR);
-      const R btf = -(WtR * n_f - Wt_f * nR);
-      const R bft = -(Ry * n_t - Wf_t * nR);
-      const R bff = -(Ry * n_f - cx * nR);
-      // value 16*parity + mrow(a + 2 uvsub, reim + 2 b): after the row
-      // reduction the transform leaves (uvsub, a) in lane group kk = a + 2 uvsub
-      // and (b, reim) in accumulator element r = reim + 2 b.
-      // parity 0: uvsub 0 tt, 1 ff;  parity 1: uvsub 0 tf, 1 ft.
-      R v[32];
+      // brackets x4 = tt, ff (even in xi), tf, ft (odd), times (re, im)
+      const R bx[4] = {-(WtR * n_t - Wt_t * nR), -(Ry * n_f - cx * nR), -(WtR * n_f - Wt_f * nR),
+                       -(Ry * n_t - Wf_t * nR)};
+      R X[4][2];
 #pragma unroll
-      for (int ai = 0; ai < 2; ++ai)
+      for (int x = 0; x < 4; ++x) { X[x][0] = bx[x] * pr; X[x][1] = bx[x] * pim; }
+      R Y[4];   // sum over h of Wb_b(h) X[x][reim] for the lane's (b, reim)
+      if constexpr (sizeof(R) == 4) {
+        R U[4][2];
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
-          const R W = Wab[2 * ai + bi];
-          v[mrow<R>(ai, 2 * bi)] = W * btt * pr;          v[mrow<R>(ai, 2 * bi + 1)] = W * btt * pim;
-          v[mrow<R>(ai + 2, 2 * bi)] = W * bff * pr;      v[mrow<R>(ai + 2, 2 * bi + 1)] = W * bff * pim;
-          v[16 + mrow<R>(ai, 2 * bi)] = W * btf * pr;     v[16 + mrow<R>(ai, 2 * bi + 1)] = W * btf * pim;
-          v[16 + mrow<R>(ai + 2, 2 * bi)] = W * bft * pr; v[16 + mrow<R>(ai + 2, 2 * bi + 1)] = W * bft * pim;
-        }
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int ri = 0; ri < 2; ++ri) U[x][ri] = Wbk * X[x][ri] + Wbp * dpp_mov<0x4E>(X[x][ri]);  // b: bit 1
+#pragma unroll
+        for (int x = 0; x < 4; ++x) Y[x] = rs_step<0xB1>(U[x][0], U[x][1], rb);                       // reim: bit 0
+      } else {
+        R U[2][4];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+          for (int x = 0; x < 4; ++x) U[bi][x] = rs_step<0x124>(Wb[bi] * X[x][0], Wb[bi] * X[x][1], rb);  // reim: bit 2
+#pragma unroll
+        for (int x = 0; x < 4; ++x) Y[x] = rs_step<0x128>(U[0][x], U[1][x], bb);                          // b: bit 3
+      }
+      R V[4];   // then over g with Wa_a(g) for the lane's a
       R ve, vo;
-      row_reduce_scatter32<R>(v, lane, ve, vo);
+      if constexpr (sizeof(R) == 4) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) V[x] = Wak * Y[x] + Wap * dpp_mov<0x124>(Y[x]);  // a: bit 2
+        ve = rs_step<0x128>(V[0], V[1], ub);                                          // uvsub: bit 3
+        vo = rs_step<0x128>(V[2], V[3], ub);
+      } else {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) V[x] = Wak * Y[x] + Wap * dpp_mov<0xB1>(Y[x]);   // a: bit 0
+        ve = rs_step<0x4E>(V[0], V[1], ub);                                           // uvsub: bit 1
+        vo = rs_step<0x4E>(V[2], V[3], ub);
+      }
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const int idx = ((16 * t + c16) * j) & mskB;
