@@ -33,6 +33,16 @@ __device__ __forceinline__ int find_op(const GemmOp* ops, int nops, int shape, i
   return lo;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Flattened tile of workgroup b of nb: XCD x = b mod 8 takes the contiguous
+// range [x q + min(x, r), ...) (q = nb / 8, r = nb mod 8; the first r XCDs
+// take q + 1), so neighbouring tiles, which share A row panels and B column
+// panels of one system, run on one L2.  A bijection of [0, nb).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
 struct PanelOp {     // factor rows [k, n) x cols [k, k + w)
   long long a;       // system base (element offset)
   int n, ld, k, w;
@@ -203,13 +213,18 @@ __global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, int nops, cons
   constexpr int BM = 32 * WM, BN = 32 * WN, KC = 16;
   constexpr int AP = BM * KC / 256, BP = BN * KC / 256;   // complex elements per thread
   constexpr int SH = (BM == 64 && BN == 64) ? 0 : (BM == 64 ? 1 : 2);
-  const GemmOp op = ops[find_op(ops, nops, SH, (int)blockIdx.x)];
+  const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const GemmOp op = ops[find_op(ops, nops, SH, bid)];
   if (op.M <= 0 || op.N <= 0 || op.K <= 0) return;
   const int tn = (op.N + BN - 1) / BN;
   const int tiles = ((op.M + BM - 1) / BM) * tn;
-  const int tile = (int)blockIdx.x - op.t0[SH];
+  const int tile = bid - op.t0[SH];
   if (tile < 0 || tile >= tiles) return;
-  const int m0 = (tile / tn) * BM, n0 = (tile % tn) * BN;
+  // tiles in groups of 8 tile rows, column-major inside a group: the blocks
+  // resident on one XCD at a time share a few A row panels and B column panels
+  const int tm = (op.M + BM - 1) / BM, per_group = 8 * tn;
+  const int g0 = (tile / per_group) * 8, gm = min(tm - g0, 8), t_in = tile % per_group;
+  const int m0 = (g0 + t_in % gm) * BM, n0 = (t_in / gm) * BN;
   const C* A = (const C*)Abase + op.a;
   const C* B = (const C*)Bbase + op.b;
   C* Cm = (C*)Cbase + op.c;
