@@ -57,11 +57,9 @@ __global__ __launch_bounds__(256) void k_ffvec(ExciteArgs a, const SysInfo* sys,
       } else {         // HH: Jt tr Isin + Jf Icos
         u = phi ? Icos : make_double2(tr * Isin.x, tr * Isin.y);
       }
-      double sp, cp;
-      sincos(a.k * ct * P.z[p], &sp, &cp);
-      const double wgt = T * P.w[p] * P.rho[p];
-      acc.x += wgt * (u.x * cp - u.y * sp);
-      acc.y += wgt * (u.x * sp + u.y * cp);
+      const double2 E = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
+      acc.x += T * (u.x * E.x - u.y * E.y);
+      acc.y += T * (u.x * E.y + u.y * E.x);
     }
   }
   const double2 ac = acoef[col];

@@ -169,6 +169,7 @@ struct Problem {
   double* d_tabs = nullptr;      // sin2E | cosE | cxB | sxB | cosB | sinB
   double* d_theta = nullptr;
   double* d_J = nullptr;
+  double2* d_E = nullptr;        // [P][ntheta] w rho e^{j k cos(theta) z}
   double* d_cells = nullptr;     // s | sp | w
   double* d_core = nullptr;      // u | w
   double* d_tail = nullptr;      // u | w
@@ -590,6 +591,7 @@ static int prepare_problem(ghost_bor* h, Problem& p) {
   if ((rc = dalloc(h, &p.d_tabs, tabs.size() * 8))) return rc;
   if ((rc = dalloc(h, &p.d_theta, p.theta.size() * 8))) return rc;
   if ((rc = dalloc(h, &p.d_J, (size_t)(p.M + 2) * P * p.ntheta * 8))) return rc;
+  if ((rc = dalloc(h, &p.d_E, (size_t)P * p.ntheta * sizeof(double2)))) return rc;
   std::vector<double> cells;
   for (int kd = 0; kd < 3; ++kd) cells.insert(cells.end(), cs[kd].begin(), cs[kd].end());
   for (int kd = 0; kd < 3; ++kd) cells.insert(cells.end(), csp[kd].begin(), csp[kd].end());
@@ -1177,6 +1179,7 @@ static ExciteArgs excite_args(const Problem& p, const Geom& tg) {
   ea.cM = p.cM;
   ea.theta = p.d_theta;
   ea.J = p.d_J;
+  ea.E = p.d_E;
   return ea;
 }
 
@@ -1286,11 +1289,13 @@ static int rhs_problem(ghost_bor* h, Problem& p, const Geom& tg) {
   LAUNCHCHK();
   int maxn = 0;
   for (int i = 0; i < p.nmodes; ++i) maxn = std::max(maxn, h->sys[p.sys0 + i].n);
-  dim3 gr((p.nrhs + 255) / 256, maxn, p.nmodes);
+  // modes fastest: the systems of one (row, column block) run together and
+  // share the Bessel and phase rows of the row's points in L2
+  dim3 gr(p.nmodes, maxn, (p.ntheta + 127) / 128);
   prof_begin(h);
   // mixed: B in FP64 only; the solve writes the column-scaled FP32 copy into the augmented columns
-  if (mixed) hipLaunchKernelGGL((k_rhs<double>), gr, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0, nullptr, h->d_B64, 0);
-  else hipLaunchKernelGGL((k_rhs<double>), gr, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0, h->d_A, h->d_B64, 0);
+  if (mixed) hipLaunchKernelGGL((k_rhs<double>), gr, dim3(128), 0, h->stream, ea, h->d_sys + p.sys0, nullptr, h->d_B64, 0);
+  else hipLaunchKernelGGL((k_rhs<double>), gr, dim3(128), 0, h->stream, ea, h->d_sys + p.sys0, h->d_A, h->d_B64, 0);
   prof_end(h, K_RHS, 0.0);
   LAUNCHCHK();
   return 0;

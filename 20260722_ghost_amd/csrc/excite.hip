@@ -17,6 +17,9 @@ struct ExciteArgs {
   double cE, cM;          // V = cE V_E + cM V_H
   const double* theta;    // [ntheta] radians
   double* J;              // [M+2][P][ntheta] Bessel table
+  double2* E;             // [P][ntheta] w_p rho_p e^{j k cos(theta) z_p}: the plane-wave phase of a
+                          // point times its quadrature weight, shared by the right-hand sides and the
+                          // far-field projection (both k_bessel outputs follow the same test geometry)
 };
 
 // J_n(x), n = 0..nmax written to out[n * stride].
@@ -86,8 +89,14 @@ __global__ void k_bessel(ExciteArgs a) {
        t += (size_t)gridDim.x * blockDim.x) {
     const int it = (int)(t % a.ntheta);
     const int p = (int)(t / a.ntheta);
-    const double x = a.k * a.pts.rho[p] * sin(a.theta[it]);
+    double st, ct;
+    sincos(a.theta[it], &st, &ct);
+    const double x = a.k * a.pts.rho[p] * st;
     bessel_orders(fabs(x), a.M + 1, a.J + (size_t)p * a.ntheta + it, (size_t)a.P * a.ntheta);
+    double sp, cp;
+    sincos(a.k * ct * a.pts.z[p], &sp, &cp);
+    const double wr = a.pts.w[p] * a.pts.rho[p];
+    a.E[t] = make_double2(wr * cp, wr * sp);
   }
 }
 
@@ -106,15 +115,14 @@ __device__ __forceinline__ double2 jn_val(const double* J, int n, size_t stride)
 // columns [n, n + nrhs) (bor_solver.py:750-752) when Abase is set, and into
 // the FP64 copy B64 when that is set.
 template <typename Ra>
-__global__ __launch_bounds__(256) void k_rhs(ExciteArgs a, const SysInfo* sys, void* Abase,
+__global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, void* Abase,
                                              double2* B64, int sys0) {
   typedef typename CT<Ra>::c Ca;
-  const SysInfo s = sys[blockIdx.z];
+  const SysInfo s = sys[blockIdx.x];   // systems fastest (see the launch)
   const int r = blockIdx.y;
   if (r >= s.n) return;
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= s.nrhs) return;
-  const int it = col >> 1, pol = col & 1;
+  const int it = blockIdx.z * blockDim.x + threadIdx.x;   // aspect: columns 2 it (VV), 2 it + 1 (HH)
+  if (2 * it >= s.nrhs) return;
   const bool phi = r >= s.nt;
   const int node = phi ? s.f0 + (r - s.nt) : s.t0 + r;
   const int am = abs(s.m);
@@ -122,9 +130,10 @@ __global__ __launch_bounds__(256) void k_rhs(ExciteArgs a, const SysInfo* sys, v
   double st, ct;
   sincos(th, &st, &ct);
   const size_t stride = (size_t)a.P * a.ntheta;
-  double2 acc = make_double2(0.0, 0.0);
+  double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
   const PointSet& P = a.pts;
   const int Ne = a.Nn - 1;
+  const double ie = 1.0 / 376.730313668;
   for (int side = 0; side < 2; ++side) {
     const int e = node - 1 + side;  // element e with this node as T1 (side 0) or T0 (side 1)
     if (e < 0 || e >= Ne) continue;
@@ -140,39 +149,38 @@ __global__ __launch_bounds__(256) void k_rhs(ExciteArgs a, const SysInfo* sys, v
       const double2 Is = make_double2(GH_PI * (jm1.y - jp1.y), -GH_PI * (jm1.x - jp1.x));
       const double2 I1 = make_double2(2.0 * GH_PI * jm.x, 2.0 * GH_PI * jm.y);
       const double tr = P.trho[p], tz = P.tz[p];
-      double2 eE, eH;
-      if (pol == 0) {  // VV
-        if (!phi) {
-          eE = make_double2(ct * tr * Ic.x - st * tz * I1.x, ct * tr * Ic.y - st * tz * I1.y);
-          eH = make_double2(Ic.x / 376.730313668, Ic.y / 376.730313668);
-        } else {
-          eE = make_double2(-ct * Is.x, -ct * Is.y);
-          eH = make_double2(-tr * Is.x / 376.730313668, -tr * Is.y / 376.730313668);
-        }
-      } else {         // HH
-        if (!phi) {
-          eE = make_double2(tr * Is.x, tr * Is.y);
-          eH = make_double2(ct * Is.x / 376.730313668, ct * Is.y / 376.730313668);
-        } else {
-          eE = Ic;
-          eH = make_double2((ct * tr * Ic.x - st * tz * I1.x) / 376.730313668,
-                            (ct * tr * Ic.y - st * tz * I1.y) / 376.730313668);
-        }
+      // u = ct tr Ic - st tz I1 (the theta-directed incident field on a t basis)
+      const double2 u = make_double2(ct * tr * Ic.x - st * tz * I1.x, ct * tr * Ic.y - st * tz * I1.y);
+      double2 eE[2], eH[2];   // [VV, HH]
+      if (!phi) {
+        eE[0] = u;                                            eH[0] = make_double2(Ic.x * ie, Ic.y * ie);
+        eE[1] = make_double2(tr * Is.x, tr * Is.y);           eH[1] = make_double2(ct * Is.x * ie, ct * Is.y * ie);
+      } else {
+        eE[0] = make_double2(-ct * Is.x, -ct * Is.y);         eH[0] = make_double2(-tr * Is.x * ie, -tr * Is.y * ie);
+        eE[1] = Ic;                                           eH[1] = make_double2(u.x * ie, u.y * ie);
       }
-      const double2 ev = make_double2(a.cE * eE.x + a.cM * eH.x, a.cE * eE.y + a.cM * eH.y);
-      double sp, cp;
-      sincos(a.k * ct * P.z[p], &sp, &cp);  // e^{j k cos(th) z}
-      const double wgt = T * P.w[p] * P.rho[p];
-      acc.x += wgt * (ev.x * cp - ev.y * sp);
-      acc.y += wgt * (ev.x * sp + ev.y * cp);
+      const double2 E = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
+#pragma unroll
+      for (int pol = 0; pol < 2; ++pol) {
+        const double2 ev = make_double2(a.cE * eE[pol].x + a.cM * eH[pol].x, a.cE * eE[pol].y + a.cM * eH[pol].y);
+        acc[pol].x += T * (ev.x * E.x - ev.y * E.y);
+        acc[pol].y += T * (ev.x * E.y + ev.y * E.x);
+      }
     }
   }
   if (Abase) {
-    Ca* A = (Ca*)Abase + s.a_off;
-    Ca o; o.x = (Ra)acc.x; o.y = (Ra)acc.y;
-    A[(size_t)r * s.ld + s.n + col] = o;
+    Ca* A = (Ca*)Abase + s.a_off + (size_t)r * s.ld + s.n + 2 * it;
+    Ca o0, o1;
+    o0.x = (Ra)acc[0].x; o0.y = (Ra)acc[0].y;
+    o1.x = (Ra)acc[1].x; o1.y = (Ra)acc[1].y;
+    A[0] = o0;
+    A[1] = o1;
   }
-  if (B64) B64[s.x_off + (size_t)r * s.nrhs + col] = acc;
+  if (B64) {
+    double2* b = B64 + s.x_off + (size_t)r * s.nrhs + 2 * it;
+    b[0] = acc[0];
+    b[1] = acc[1];
+  }
 }
 
 // Partial far-field sums over a chunk of Gauss points for every (system, column).
@@ -241,11 +249,9 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
       term = make_double2(tr * (Jt.x * Isin.x - Jt.y * Isin.y) + Jf.x * Icos.x - Jf.y * Icos.y,
                           tr * (Jt.x * Isin.y + Jt.y * Isin.x) + Jf.x * Icos.y + Jf.y * Icos.x);
     }
-    double sp, cp;
-    sincos(a.k * ct * P.z[p], &sp, &cp);
-    const double wgt = P.w[p] * P.rho[p];
-    acc.x += wgt * (term.x * cp - term.y * sp);
-    acc.y += wgt * (term.x * sp + term.y * cp);
+    const double2 E = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
+    acc.x += term.x * E.x - term.y * E.y;
+    acc.y += term.x * E.y + term.y * E.x;
   }
   partial[((size_t)pc * nsys + si) * s.nrhs + col] = acc;
 }
