@@ -1383,7 +1383,7 @@ template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launc
         if (L.gx > 0) hipLaunchKernelGGL((k_lu_swap<T>), dim3(L.gx, L.nops), dim3(256), 0, h->stream, h->d_swap + L.ops, factor, h->d_piv);
         break;
       case L_TRIL:
-        hipLaunchKernelGGL((k_tri_inv<T, false>), dim3(L.nops), dim3(64), 0, h->stream, h->d_trsm + L.ops, factor, h->d_W);
+        hipLaunchKernelGGL((k_tri_inv<T, false>), dim3(L.nops), dim3(256), 0, h->stream, h->d_trsm + L.ops, factor, h->d_W);
         break;
       case L_GEMM_AUG_W:
         gemm_launch<T>(h, h->d_gemm + L.ops, L.nops, L.tt, h->d_W, factor, factor);
@@ -1424,7 +1424,7 @@ template <typename T> static int run_lu(ghost_bor* h) {
   int rc = run_sched<T>(h, h->lu_sched, h->d_A, nullptr);
   if (rc) return rc;
   prof_begin(h);
-  hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(h->triu_n), dim3(64), 0, h->stream, h->d_trsm + h->triu_ops,
+  hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(h->triu_n), dim3(256), 0, h->stream, h->d_trsm + h->triu_ops,
                      h->d_A, h->d_W);
   prof_end(h, K_TRI_INV, 0.0);
 

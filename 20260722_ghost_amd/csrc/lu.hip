@@ -650,53 +650,74 @@ __global__ __launch_bounds__(256) void k_lu_swap(const SwapOp* ops, void* base, 
 // k_gemm_update with A = W and B = C = X then leaves X = T^{-1} X: the whole
 // B column slice is read before any C element of that slice is written.
 template <typename T, bool UPPER>
-__global__ __launch_bounds__(64) void k_tri_inv(const TrsmOp* ops, const void* tbase, void* wbuf) {
+__global__ __launch_bounds__(256) void k_tri_inv(const TrsmOp* ops, const void* tbase, void* wbuf) {
   typedef typename CT<T>::c C;
   const TrsmOp op = ops[blockIdx.x];
   C* W = (C*)wbuf + op.wo;
   if (op.w <= 0) return;
+  // T zero-padded to 64 x 64; with the padding rows and columns >= w come
+  // out as the identity in T^{-1}, i.e. zero in W
   __shared__ C Tm[64][65];
-  __shared__ C Iv[64][65];
+  __shared__ C dinv[64];   // UPPER: 1 / U[r][r] (1 on the padding)
   const C* Tp = (const C*)tbase + op.t;
   const int w = op.w;
-  for (int i = threadIdx.x; i < w * w; i += 64) {
-    const int r = i / w, c = i % w;
-    Tm[r][c] = Tp[(size_t)r * op.ldt + c];
-  }
-  __syncthreads();
-  const int j = threadIdx.x;   // column of the inverse
-  if (j < w) {
-    if (!UPPER) {
-      for (int r = 0; r < w; ++r) {
-        C z;
-        z.x = (r == j) ? (T)1 : (T)0;
-        z.y = 0;
-        if (r > j)
-          for (int q = j; q < r; ++q) z = csub(z, cmul(Tm[r][q], Iv[q][j]));
-        Iv[r][j] = z;
-      }
-    } else {
-      for (int r = w - 1; r >= 0; --r) {
-        C z;
-        z.x = (r == j) ? (T)1 : (T)0;
-        z.y = 0;
-        if (r < j)
-          for (int q = r + 1; q <= j; ++q) z = csub(z, cmul(Tm[r][q], Iv[q][j]));
-        Iv[r][j] = (r <= j) ? cmul(z, cdiv1<T>(Tm[r][r])) : z;
-      }
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < 64 * 64; i += 64) {
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
     const int r = i >> 6, c = i & 63;
     C z;
     z.x = 0; z.y = 0;
-    if (r < w && c < w) {
-      z = Iv[r][c];
-      z.x = -z.x; z.y = -z.y;
-      if (r == c) z.x += (T)1;
+    if (r < w && c < w) z = Tp[(size_t)r * op.ldt + c];
+    Tm[r][c] = z;
+  }
+  __syncthreads();
+  if (UPPER && threadIdx.x < 64) {
+    const int r = threadIdx.x;
+    C one;
+    one.x = (T)1; one.y = (T)0;
+    dinv[r] = r < w ? cdiv1<T>(Tm[r][r]) : one;
+  }
+  if (UPPER) __syncthreads();
+  // column j of the inverse, rows r = 4 i + s held by lane s of a quad in x[i]:
+  // x_r = delta_rj - sum_{q < r} T[r][q] x_q (lower, unit diagonal), or
+  // x_r = (delta_rj - sum_{q > r} U[r][q] x_q) / U[r][r] (upper); each lane
+  // sums its own q = s (mod 4) and the quad reduces by DPP
+  const int j = threadIdx.x >> 2, sl = threadIdx.x & 3;
+  C x[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) { x[i].x = (T)0; x[i].y = (T)0; }
+#pragma clang loop unroll(full)
+  for (int step = 0; step < 64; ++step) {
+    const int r = UPPER ? 63 - step : step;
+    T pr = 0, pi = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int q = 4 * i + sl;
+      const bool use = UPPER ? (4 * i + 3 > r && q > r) : (4 * i < r && q < r);
+      if (UPPER ? (4 * i + 3 > r) : (4 * i < r)) {   // static: some lane of the quad may use it
+        if (use) {
+          const C t = Tm[r][q];
+          pr += t.x * x[i].x - t.y * x[i].y;
+          pi += t.x * x[i].y + t.y * x[i].x;
+        }
+      }
     }
-    W[i] = z;
+    pr += dpp_mov<0xB1>(pr);
+    pi += dpp_mov<0xB1>(pi);
+    pr += dpp_mov<0x4E>(pr);
+    pi += dpp_mov<0x4E>(pi);
+    C z;
+    z.x = (r == j ? (T)1 : (T)0) - pr;
+    z.y = -pi;
+    if (UPPER) z = cmul(z, dinv[r]);
+    if (sl == (r & 3)) x[r >> 2] = z;
+  }
+  // W = I - T^{-1}, row-major 64 x 64
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int r = 4 * i + sl;
+    C z;
+    z.x = (r == j ? (T)1 : (T)0) - x[i].x;
+    z.y = -x[i].y;
+    W[r * 64 + j] = z;
   }
 }
 
