@@ -510,29 +510,25 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
     if (valid) {
       for (int u = 0; u < 8; ++u) {
         const double* x = &nv[slot][12 * u];
-        const double c1 = x[10], s1 = x[11];
-        const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * c1 * s1;   // e^{2 j xi}
-        // two interleaved rotation chains (even / odd orders of the chunk),
-        // each advanced by e^{2 j xi}: half the serial dependency depth
-        double cr0 = st[slot][16 * u + 2 * cs], ci0 = st[slot][16 * u + 2 * cs + 1];
-        double cr1 = cr0 * c1 - ci0 * s1, ci1 = cr0 * s1 + ci0 * c1;
+        const double c1 = x[10], s1 = x[11], tc = 2.0 * c1;
+        // cos / sin of the chunk's orders by the three-term recurrence
+        // f_{n+1} = 2 cos(xi) f_n - f_{n-1} from the two exact starts
+        // e^{j n0 xi} and e^{j (n0 + 1) xi} (9 steps: the error stays a few ulp)
+        double cp = st[slot][16 * u + 2 * cs], sp = st[slot][16 * u + 2 * cs + 1];
+        double cc = cp * c1 - sp * s1, sc = cp * s1 + sp * c1;
 #pragma unroll
         for (int i = 0; i < C; ++i) {
-          const double cr = (i & 1) ? cr1 : cr0, ci = (i & 1) ? ci1 : ci0;
+          const double cr = i == 0 ? cp : cc, ci = i == 0 ? sp : sc;
           // EFIE and even brackets project on cos(n xi), odd brackets on sin(n xi)
           accG[i].x += x[0] * cr;  accG[i].y += x[1] * cr;
           accK[0][i].x += x[2] * cr; accK[0][i].y += x[3] * cr;
           accK[1][i].x += x[4] * ci; accK[1][i].y += x[5] * ci;
           accK[2][i].x += x[6] * ci; accK[2][i].y += x[7] * ci;
           accK[3][i].x += x[8] * cr; accK[3][i].y += x[9] * cr;
-          if (i & 1) {
-            const double t = cr1 * c2 - ci1 * s2;
-            ci1 = cr1 * s2 + ci1 * c2;
-            cr1 = t;
-          } else {
-            const double t = cr0 * c2 - ci0 * s2;
-            ci0 = cr0 * s2 + ci0 * c2;
-            cr0 = t;
+          if (i > 0) {
+            const double cn = tc * cc - cp, sn = tc * sc - sp;
+            cp = cc; sp = sc;
+            cc = cn; sc = sn;
           }
         }
       }
