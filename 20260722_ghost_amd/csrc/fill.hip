@@ -406,11 +406,13 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
   const int nnodes = 48 + pr.n_tail;
   const int n0 = cs * C;
 
-  // per point-pair slot: 8 nodes x 12 values, and 8 nodes x 8 chunk starts
-  // e^{j n0 xi} (re, im); the slot strides (97, 129 doubles) are odd so the
-  // 8 slots of a wavefront fall on different LDS banks
+  // per point-pair slot: 8 nodes x 12 values (slot stride 97 doubles, odd, so
+  // the 8 slots of a wavefront fall on different banks), and 8 nodes x 8 chunk
+  // starts e^{j n0 xi} in separate re / im planes at [slot * 72 + node * 9 +
+  // chunk]: reads (chunk = lane's chunk, one node) and writes (node = lane's
+  // node, one chunk) both cover distinct banks
   __shared__ double nv[32][97];
-  __shared__ double st[32][129];
+  __shared__ double st[2][32 * 72];
 
   double2 accG[C], accK[4][C];
 #pragma unroll
@@ -492,8 +494,8 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
       double zr = 1.0, zi = 0.0;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
-        st[slot][16 * cs + 2 * c] = zr;
-        st[slot][16 * cs + 2 * c + 1] = zi;
+        st[0][slot * 72 + cs * 9 + c] = zr;
+        st[1][slot * 72 + cs * 9 + c] = zi;
         const double t = zr * cC - zi * sC;
         zi = zr * sC + zi * cC;
         zr = t;
@@ -503,7 +505,10 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
       for (int i = 0; i < 12; ++i) out[i] = 0.0;
       out[10] = 1.0;
 #pragma unroll
-      for (int c = 0; c < 16; ++c) st[slot][16 * cs + c] = 0.0;
+      for (int c = 0; c < 8; ++c) {
+        st[0][slot * 72 + cs * 9 + c] = 0.0;
+        st[1][slot * 72 + cs * 9 + c] = 0.0;
+      }
     }
     __syncthreads();
     // ── accumulate this thread's orders over the 8 nodes ──
@@ -514,7 +519,7 @@ __global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
         // cos / sin of the chunk's orders by the three-term recurrence
         // f_{n+1} = 2 cos(xi) f_n - f_{n-1} from the two exact starts
         // e^{j n0 xi} and e^{j (n0 + 1) xi} (9 steps: the error stays a few ulp)
-        double cp = st[slot][16 * u + 2 * cs], sp = st[slot][16 * u + 2 * cs + 1];
+        double cp = st[0][slot * 72 + u * 9 + cs], sp = st[1][slot * 72 + u * 9 + cs];
         double cc = cp * c1 - sp * s1, sc = cp * s1 + sp * c1;
 #pragma unroll
         for (int i = 0; i < C; ++i) {
