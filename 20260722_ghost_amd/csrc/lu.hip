@@ -435,7 +435,11 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
         }
       }
     }
-    for (int jj = 0; jj < ws; ++jj) {
+    // columns unrolled: jj is a constant in each copy, so the column's values
+    // are plain registers and the rank-1 update covers only the columns right of it
+#pragma unroll
+    for (int jj = 0; jj < SW; ++jj) {
+      if (jj >= ws) break;
       const int col = c0 + jj;
       // ── pivot search ──
       T best = (T)-1;
