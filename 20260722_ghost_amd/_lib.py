@@ -70,6 +70,7 @@ SIGNATURES = [
     ("ghost_bor_fill_rhs", _ci, [_vp]),
     ("ghost_bor_set_solution", _ci, [_vp, _ci, _ci, _dp]),
     ("ghost_bor_system_count", _ci, [_vp, _ip]),
+    ("ghost_bor_get_pivots", _ci, [_vp, _ci, _ip]),
     ("ghost_bor_set_lu_super", _ci, [_vp, _ci]),
 ]
 
@@ -260,6 +261,17 @@ class Handle:
         n = ctypes.c_int(0)
         check(self.lib.ghost_bor_system_count(self._h, ctypes.byref(n)))
         return n.value
+
+    def pivots(self, idx: int, n: int | None = None) -> np.ndarray:
+        """Row interchanges of system idx's factor (LAPACK getrf ipiv - 1);
+        n: the system size (known here for dense systems)."""
+        if n is None:
+            if idx not in self.dense:
+                raise ValueError("pass n for a mode system")
+            n = self.dense[idx][0]
+        out = np.zeros(int(n), dtype=np.int32)
+        check(self.lib.ghost_bor_get_pivots(self._h, int(idx), out.ctypes.data_as(_ip)))
+        return out
 
     def residuals(self) -> np.ndarray:
         """||A X - B|| / ||B|| per system, in the order systems were added."""

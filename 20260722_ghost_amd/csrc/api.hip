@@ -2277,6 +2277,20 @@ extern "C" int ghost_bor_get_dense(ghost_bor* h, int idx, double* X) {
   return 0;
 }
 
+// Row interchanges of system idx's factorization: ipiv[r] = the row swapped
+// with row r at step r (0-based within the system), LAPACK getrf's ipiv - 1.
+extern "C" int ghost_bor_get_pivots(ghost_bor* h, int idx, int* ipiv) {
+  if (!h || !ipiv) return fail(GHOST_ERR_ARG, "null argument");
+  if (idx < 0 || idx >= (int)h->sys.size()) return fail(GHOST_ERR_ARG, "system index %d out of range", idx);
+  if (!h->solved) return fail(GHOST_ERR_STATE, "no factorization yet: solve or run first");
+  long long off = 0;
+  for (int si = 0; si < idx; ++si) off += h->sys[si].n;
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(ipiv, h->d_piv + off, (size_t)h->sys[idx].n * sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 // Number of systems (mode systems of every problem and dense systems): the
 // length of the per-system arrays of ghost_bor_residual and ghost_bor_info.
 extern "C" int ghost_bor_system_count(ghost_bor* h, int* n) {

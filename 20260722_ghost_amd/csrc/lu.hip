@@ -374,6 +374,51 @@ template <typename T> __device__ __forceinline__ typename CT<T>::c cdiv1(typenam
   return r;
 }
 
+// Pivot arg-max on DPP (no LDS round trips): (v, i) takes (w, j) when w is
+// larger, or equal with the smaller row index (the first maximum, as izamax).
+template <typename T> __device__ __forceinline__ void amax_take(T& v, int& i, T w, int j) {
+  if (w > v || (w == v && j < i)) { v = w; i = j; }
+}
+// DPP move of x into the rows selected by RM; other rows keep `old`
+template <int CTRL, int RM> __device__ __forceinline__ int dpp_upd(int old, int x) {
+  return __builtin_amdgcn_update_dpp(old, x, CTRL, RM, 0xF, false);
+}
+template <int CTRL, int RM> __device__ __forceinline__ float dpp_upd(float old, float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(x), CTRL, RM, 0xF, false));
+}
+template <int CTRL, int RM> __device__ __forceinline__ double dpp_upd(double old, double x) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(x), CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(x), CTRL, RM, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL, int RM, typename T> __device__ __forceinline__ void amax_dpp(T& v, int& i) {
+  amax_take(v, i, dpp_upd<CTRL, RM>(v, v), dpp_upd<CTRL, RM>(i, i));
+}
+// every lane of a 16-lane DPP row ends with the row's arg-max: quad_perm
+// [1,0,3,2] and [2,3,0,1] combine a quad, row_ror 4 and 8 the four quads
+// (the operation is idempotent, so rotations serve as butterflies)
+template <typename T> __device__ __forceinline__ void row_argmax(T& v, int& i) {
+  amax_dpp<0xB1, 0xF>(v, i);
+  amax_dpp<0x4E, 0xF>(v, i);
+  amax_dpp<0x124, 0xF>(v, i);
+  amax_dpp<0x128, 0xF>(v, i);
+}
+// the wavefront's arg-max, complete in lane 63: row_bcast:15 brings row r's
+// result into row r + 1 (rows 1 and 3 written), row_bcast:31 brings row 1's
+// into rows 2 and 3
+template <typename T> __device__ __forceinline__ void wave_argmax_to63(T& v, int& i) {
+  row_argmax(v, i);
+  amax_dpp<0x142, 0xA>(v, i);
+  amax_dpp<0x143, 0xC>(v, i);
+}
+__device__ __forceinline__ float readlane_t(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double readlane_t(double x, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                          __builtin_amdgcn_readlane(__double2loint(x), l));
+}
+
 // SW contiguous elements of one sub-panel row in 16-byte accesses.  The
 // segment is 16-byte aligned: matrix offsets and leading dimensions are
 // multiples of 16 elements and sub-panels start at multiples of SW (even).
@@ -455,28 +500,19 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
           if (mag > best || (mag == best && row < bidx)) { best = mag; bidx = row; }
         }
       }
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) {
-        const T ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bidx, off);
-        if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
-      }
+      wave_argmax_to63(best, bidx);
       // the 16 wave maxima, double-buffered by column parity: a wave writes the
       // next column's entry only after every wave has passed this column's
       // second barrier, i.e. after every read of this buffer
-      if (lane == 0) { red_v[jj & 1][wv] = best; red_i[jj & 1][wv] = bidx; }
+      if (lane == 63) { red_v[jj & 1][wv] = best; red_i[jj & 1][wv] = bidx; }
       __syncthreads();
-      // every wave reduces the 16 entries itself (lanes 0-15, then a broadcast)
+      // every wave reduces the 16 entries itself (row 0 of the wave, then a
+      // broadcast from lane 0)
       best = lane < 16 ? red_v[jj & 1][lane] : (T)-1;
       bidx = lane < 16 ? red_i[jj & 1][lane] : 0x7fffffff;
-#pragma unroll
-      for (int off = 8; off >= 1; off >>= 1) {
-        const T ob = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bidx, off);
-        if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
-      }
-      best = __shfl(best, 0);
-      bidx = __shfl(bidx, 0);
+      row_argmax(best, bidx);
+      best = readlane_t(best, 0);
+      bidx = __builtin_amdgcn_readlane(bidx, 0);
       const bool singular = !(best > (T)0);
       const int pr = singular ? col : bidx;
       if (tid == 0) {

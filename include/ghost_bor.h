@@ -135,6 +135,12 @@ int ghost_bor_get_dF(ghost_bor* h, int prob, double* out);
 /* Number of systems of the handle (every problem's modes and the dense
  * systems, in the order added): the length of the per-system arrays below. */
 int ghost_bor_system_count(ghost_bor* h, int* n);
+/* Row interchanges of system sys_idx's LU factorization after a solve:
+ * ipiv[r] (n entries) is the row swapped with row r at elimination step r,
+ * 0-based within the system.  No counterpart in the reference, whose
+ * numpy.linalg.solve (bor_solver.py:753) keeps LAPACK zgetrf's ipiv inside:
+ * this is that ipiv minus one, for callers that check or reuse the factor. */
+int ghost_bor_get_pivots(ghost_bor* h, int sys_idx, int* ipiv);
 /* ||A X - B||_F / ||B||_F for every system (bor_solver.py:754-756), out[system count]. */
 int ghost_bor_residual(ghost_bor* h, double* out);
 /* Mode-system layout of (prob, m): size n, active J_t count nt and first active t / phi nodes. */
