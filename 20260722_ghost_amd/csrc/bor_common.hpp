@@ -74,6 +74,31 @@ template <int CTRL> __device__ __forceinline__ float dpp_mov(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
+// Reduce-scatter across wavefront halves (xadd32) or row pairs (xadd16) on
+// gfx950's permlane swaps: lanes 0-31 return lo_l + lo_(l^32) and lanes 32-63
+// hi_l + hi_(l^32); xadd16 the same with rows of 16 (even rows keep lo).
+// v_permlane32_swap(lo, hi) leaves exactly the two addends in its two outputs
+// (measured lane layout: output 0 holds lanes 0-31 of lo in the lower half and
+// of hi in the upper half, output 1 lanes 32-63 of lo and of hi).
+__device__ __forceinline__ float xadd32(float lo, float hi) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(lo), __float_as_int(hi), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ __forceinline__ float xadd16(float lo, float hi) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(lo), __float_as_int(hi), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ __forceinline__ double xadd32(double lo, double hi) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__double2loint(lo), __double2loint(hi), false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(lo), __double2hiint(hi), false, false);
+  return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+}
+__device__ __forceinline__ double xadd16(double lo, double hi) {
+  const auto a = __builtin_amdgcn_permlane16_swap(__double2loint(lo), __double2loint(hi), false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap(__double2hiint(lo), __double2hiint(hi), false, false);
+  return __hiloint2double(b[0], a[0]) + __hiloint2double(b[1], a[1]);
+}
+
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {

@@ -180,29 +180,18 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
             z[3 * 8 + 2 * ab + 1] += TT * Gci - SS * Gi;
           }
       }
-      // reduce over g (lanes c16 + 16 kk): stage 1 keeps {tt, tf} (uvsub 0) or {ff, ft}
+      // reduce over g (lanes c16 + 16 kk): stage 1 (wavefront halves, uvsub =
+      // lane bit 5) keeps {tt, tf} (uvsub 0) or {ff, ft}
       R y16[16];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const R lo = z[0 * 8 + i], hi = z[3 * 8 + i];   // tt <-> ff
-        const R keep = uvsub ? hi : lo, send = uvsub ? lo : hi;
-        y16[i] = keep + __shfl_xor(send, 32);
-      }
+      for (int i = 0; i < 8; ++i) y16[i] = xadd32(z[0 * 8 + i], z[3 * 8 + i]);       // tt <-> ff
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const R lo = z[1 * 8 + i], hi = z[2 * 8 + i];   // tf <-> ft
-        const R keep = uvsub ? hi : lo, send = uvsub ? lo : hi;
-        y16[8 + i] = keep + __shfl_xor(send, 32);
-      }
-      // stage 2 keeps a = asel; stash (even, odd) x (b, reim)
+      for (int i = 0; i < 8; ++i) y16[8 + i] = xadd32(z[1 * 8 + i], z[2 * 8 + i]);   // tf <-> ft
+      // stage 2 (row pairs, asel = lane bit 4) keeps a = asel; stash (even, odd) x (b, reim)
 #pragma unroll
       for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const R lo = y16[8 * q + i], hi = y16[8 * q + 4 + i];
-          const R keep = asel ? hi : lo, send = asel ? lo : hi;
-          stash[wl][t][4 * q + i][lane] = keep + __shfl_xor(send, 16);
-        }
+        for (int i = 0; i < 4; ++i) stash[wl][t][4 * q + i][lane] = xadd16(y16[8 * q + i], y16[8 * q + 4 + i]);
     }
   }
 
