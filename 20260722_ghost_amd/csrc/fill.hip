@@ -57,20 +57,24 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
   typedef typename CT<R>::c C2;
   typedef typename CT<R>::v4 V4;
   __shared__ R stash[2][NT][8][64];
-  extern __shared__ char dyn_smem[];
+  extern __shared__ __align__(16) char dyn_smem[];
   // azimuth tables in LDS, in the working precision:
-  // sin2E [nE/2+1] | cosE [nE] | cxB [nB/2+1] | sxB [nB/2+1] | cosB [nB] | sinB [nB]
-  R* tab = (R*)dyn_smem;
-  R* t_sin2E = tab;
+  // (cosB, sinB) pairs [nB] | sin2E [nE/2+1] | cosE [nE] | cxB [nB/2+1] | sxB [nB/2+1]
+  // (the bracket transform reads its cos and sin twiddle in one 2-element load)
+  C2* t_csB = (C2*)dyn_smem;
+  R* t_sin2E = (R*)(t_csB + a.nB);
   R* t_cosE = t_sin2E + (a.nE / 2 + 1);
   R* t_cxB = t_cosE + a.nE;
   R* t_sxB = t_cxB + (a.nB / 2 + 1);
-  R* t_cosB = t_sxB + (a.nB / 2 + 1);
-  R* t_sinB = t_cosB + a.nB;
   for (int i = threadIdx.x; i <= a.nE / 2; i += blockDim.x) t_sin2E[i] = (R)a.sin2E[i];
   for (int i = threadIdx.x; i < a.nE; i += blockDim.x) t_cosE[i] = (R)a.cosE[i];
   for (int i = threadIdx.x; i <= a.nB / 2; i += blockDim.x) { t_cxB[i] = (R)a.cxB[i]; t_sxB[i] = (R)a.sxB[i]; }
-  for (int i = threadIdx.x; i < a.nB; i += blockDim.x) { t_cosB[i] = (R)a.cosB[i]; t_sinB[i] = (R)a.sinB[i]; }
+  for (int i = threadIdx.x; i < a.nB; i += blockDim.x) {
+    C2 v;
+    v.x = (R)a.cosB[i];
+    v.y = (R)a.sinB[i];
+    t_csB[i] = v;
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const int wid = blockIdx.x * 2 + wl;
@@ -283,8 +287,9 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
         const int idx = ((16 * t + c16) * j) & mskB;
-        accE[t] = mfma4(ve, t_cosB[idx], accE[t]);
-        accO[t] = mfma4(vo, t_sinB[idx], accO[t]);
+        const C2 cs = t_csB[idx];
+        accE[t] = mfma4(ve, cs.x, accE[t]);
+        accO[t] = mfma4(vo, cs.y, accO[t]);
       }
     }
   }
