@@ -48,15 +48,22 @@ __device__ __forceinline__ R rs_step(R lo, R hi, bool b) {
   return keep + dpp_mov<CTRL>(send);
 }
 
-// One wavefront per far element pair (e, f); 2 wavefronts per block.
+// Wavefronts (element pairs) per work-group of k_far_fill: they share one copy
+// of the azimuth tables in LDS.  FP32: 4 (with the register budget below, three
+// wavefronts per SIMD); FP64: 2 (its stash is twice the size).
+template <typename R> constexpr int far_waves() { return sizeof(R) == 4 ? 4 : 2; }
+
+// One wavefront per far element pair (e, f); far_waves<R>() wavefronts per block.
 // Phase 1: EFIE samples -> G_n on MFMA.  Phase 2: per-order EFIE blocks,
 // reduced over the test points and stashed in LDS.  Phase 3: MFIE brackets
 // contracted to nodal rows -> modes on MFMA.  Phase 4: combine, RMW Zfar.
 template <typename R, int NT, bool SPLIT>
-__global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
+__global__ __launch_bounds__(64 * far_waves<R>())
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1))) void k_far_fill(FarArgs a) {
   typedef typename CT<R>::c C2;
   typedef typename CT<R>::v4 V4;
-  __shared__ R stash[2][NT][8][64];
+  constexpr int WPB = far_waves<R>();
+  __shared__ R stash[WPB][NT][8][64];
   extern __shared__ __align__(16) char dyn_smem[];
   // azimuth tables in LDS, in the working precision:
   // (cosB, sinB) pairs [nB] | sin2E [nE/2+1] | cosE [nE] | cxB [nB/2+1] | sxB [nB/2+1]
@@ -77,7 +84,7 @@ __global__ __launch_bounds__(128) void k_far_fill(FarArgs a) {
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
-  const int wid = blockIdx.x * 2 + wl;
+  const int wid = blockIdx.x * WPB + wl;
   if (wid >= a.npairs) return;  // wave-uniform exit (no block barriers below)
   const int e = a.pe + 2 * (wid / a.nfp);
   const int f = a.pf + 2 * (wid % a.nfp);
