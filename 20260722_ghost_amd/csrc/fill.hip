@@ -381,7 +381,7 @@ __device__ __forceinline__ void cell_geom(const double* nodes, int e, double s, 
 // 256 threads = 32 point pairs x 8 order chunks of C orders.  Node values are
 // evaluated once per (point pair, node) and shared through LDS in batches of 8.
 template <int C>
-__global__ __launch_bounds__(256) void k_near_modal(NearArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 : 1))) void k_near_modal(NearArgs a) {
   const int2 bt = a.batches[blockIdx.x];
   const NearPairInfo pr = a.pairs[bt.x];
   const int slot = threadIdx.x >> 3, cs = threadIdx.x & 7;
