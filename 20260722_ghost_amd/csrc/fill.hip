@@ -144,9 +144,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1))) void k_far_fill(Far
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const R am = (R)(16 * t + c16);
-      R z[32];  // [blk(4)][a(2)][b(2)][reim]: blk 0 tt, 1 tf, 2 ft, 3 ff
+      // The weights are separable (test side: a and the lane's g; source side:
+      // b and h), so the source points are contracted first: per source node b
+      // ten sums over h of the order data (Tw G, Tw Gc, Tw Gs, Dw G, Sw G; re
+      // and im), then the 2 x 2 x 4 blocks from those sums and the test weights:
+      //   tt = Tw_a (trp trq Gc + tzp tzq G)_b - Dw_a/k^2 (Dw G)_b
+      //   tf = Tw_a trp (Tw Gs)_b - j am Dw_a/k^2 (Sw G)_b
+      //   ft = -Tw_a trq (Tw Gs)_b + j am Sw_a/k^2 (Dw G)_b
+      //   ff = Tw_a (Tw Gc)_b - am^2 Sw_a/k^2 (Sw G)_b
+      R sTG[2][2], sTC[2][2], sTS[2][2], sDG[2][2], sSG[2][2];   // [b][reim]
 #pragma unroll
-      for (int i = 0; i < 32; ++i) z[i] = (R)0;
+      for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int ri = 0; ri < 2; ++ri) { sTG[bi][ri] = 0; sTC[bi][ri] = 0; sTS[bi][ri] = 0; sDG[bi][ri] = 0; sSG[bi][ri] = 0; }
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
         const int iq2 = 4 * f + h;
@@ -170,26 +180,34 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1))) void k_far_fill(Far
         }
         const R Gcr = (R)0.5 * (Mr + Pr), Gci = (R)0.5 * (Mi + Pi);
         const R Gsr = (R)0.5 * (Mi - Pi), Gsi = (R)-0.5 * (Mr - Pr);  // (M - P)/(2j)
-        const R ktr = trp * trq * Gcr + tzp * tzq * Gr, kti = trp * trq * Gci + tzp * tzq * Gi;
 #pragma unroll
-        for (int ai = 0; ai < 2; ++ai)
+        for (int bi = 0; bi < 2; ++bi) {
+          sTG[bi][0] += Twb[bi] * Gr;  sTG[bi][1] += Twb[bi] * Gi;
+          sTC[bi][0] += Twb[bi] * Gcr; sTC[bi][1] += Twb[bi] * Gci;
+          sTS[bi][0] += Twb[bi] * Gsr; sTS[bi][1] += Twb[bi] * Gsi;
+          sDG[bi][0] += Dwb[bi] * Gr;  sDG[bi][1] += Dwb[bi] * Gi;
+          sSG[bi][0] += Swb[bi] * Gr;  sSG[bi][1] += Swb[bi] * Gi;
+        }
+      }
+      const R tt_c = trp * trq, tt_z = tzp * tzq;
+      R z[32];  // [blk(4)][a(2)][b(2)][reim]: blk 0 tt, 1 tf, 2 ft, 3 ff
 #pragma unroll
-          for (int bi = 0; bi < 2; ++bi) {
-            const int ab = 2 * ai + bi;
-            const R TT = Twa[ai] * Twb[bi];
-            const R DD = Dwa[ai] * Dwb[bi] * ik2;
-            const R DS = Dwa[ai] * Swb[bi] * am * ik2;
-            const R SD = Swa[ai] * Dwb[bi] * am * ik2;
-            const R SS = Swa[ai] * Swb[bi] * am * am * ik2;
-            z[0 * 8 + 2 * ab] += TT * ktr - DD * Gr;            // tt: TT kt - DD G
-            z[0 * 8 + 2 * ab + 1] += TT * kti - DD * Gi;
-            z[1 * 8 + 2 * ab] += TT * trp * Gsr + DS * Gi;      // tf: TT trp Gs - j DS G
-            z[1 * 8 + 2 * ab + 1] += TT * trp * Gsi - DS * Gr;
-            z[2 * 8 + 2 * ab] += -TT * trq * Gsr - SD * Gi;     // ft: -TT trq Gs + j SD G
-            z[2 * 8 + 2 * ab + 1] += -TT * trq * Gsi + SD * Gr;
-            z[3 * 8 + 2 * ab] += TT * Gcr - SS * Gr;            // ff: TT Gc - SS G
-            z[3 * 8 + 2 * ab + 1] += TT * Gci - SS * Gi;
-          }
+      for (int ai = 0; ai < 2; ++ai) {
+        const R kT = Twa[ai], kD = Dwa[ai] * ik2, kS = Swa[ai] * ik2;
+        const R kDm = kD * am, kSm = kS * am, kSmm = kSm * am;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+          const int ab = 2 * ai + bi;
+          const R Kr = tt_c * sTC[bi][0] + tt_z * sTG[bi][0], Ki = tt_c * sTC[bi][1] + tt_z * sTG[bi][1];
+          z[0 * 8 + 2 * ab] = kT * Kr - kD * sDG[bi][0];                        // tt
+          z[0 * 8 + 2 * ab + 1] = kT * Ki - kD * sDG[bi][1];
+          z[1 * 8 + 2 * ab] = kT * trp * sTS[bi][0] + kDm * sSG[bi][1];          // tf
+          z[1 * 8 + 2 * ab + 1] = kT * trp * sTS[bi][1] - kDm * sSG[bi][0];
+          z[2 * 8 + 2 * ab] = -kT * trq * sTS[bi][0] - kSm * sDG[bi][1];         // ft
+          z[2 * 8 + 2 * ab + 1] = -kT * trq * sTS[bi][1] + kSm * sDG[bi][0];
+          z[3 * 8 + 2 * ab] = kT * sTC[bi][0] - kSmm * sSG[bi][0];               // ff
+          z[3 * 8 + 2 * ab + 1] = kT * sTC[bi][1] - kSmm * sSG[bi][1];
+        }
       }
       // reduce over g (lanes c16 + 16 kk): stage 1 (wavefront halves, uvsub =
       // lane bit 5) keeps {tt, tf} (uvsub 0) or {ff, ft}
