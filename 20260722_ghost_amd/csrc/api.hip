@@ -1570,7 +1570,7 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
     hipLaunchKernelGGL(k_widen_scaled, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_aug, (const float2*)h->d_A,
                        h->d_bscale, h->d_X64, 0);
     LAUNCHCHK();
-    hipLaunchKernelGGL(k_perm_from_pivots, dim3((nsys + 63) / 64), dim3(64), 0, h->stream, h->d_vec + h->vec_all,
+    hipLaunchKernelGGL(k_perm_from_pivots, dim3(nsys), dim3(256), 0, h->stream, h->d_vec + h->vec_all,
                        nsys, h->d_piv, h->d_perm);
     LAUNCHCHK();
     // refine_iters >= 0: that many FP64-residual sweeps; < 0: adaptive, up to
@@ -2064,7 +2064,7 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   const dim3 gv((unsigned)((maxr + 255) / 256), (unsigned)maxn, nsys);
   // forward solution out of the augmented columns; the factor's row order
   hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_aug, (const double2*)h->d_A, h->d_Xs);
-  hipLaunchKernelGGL(k_perm_from_pivots, dim3((nsys + 63) / 64), dim3(64), 0, h->stream, h->d_vec + h->vec_all, nsys,
+  hipLaunchKernelGGL(k_perm_from_pivots, dim3(nsys), dim3(256), 0, h->stream, h->d_vec + h->vec_all, nsys,
                      h->d_piv, h->d_perm);
   LAUNCHCHK();
   // a' = a (-j k eta0 / 4 pi) per problem column; G = a' dS/dX

@@ -838,17 +838,27 @@ __global__ void k_copy64(const VecOp* ops, const double2* src, double2* dst) {
 }
 
 // permutation of the factor rows from LAPACK-style interchanges (one thread per system)
-__global__ void k_perm_from_pivots(const VecOp* ops, int nsys, const int* pivots, int* perm) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+// One work-group per system (n <= 4096, the panel limit): the interchanges are
+// replayed in order by one lane on an LDS copy (their chain is sequential;
+// LDS keeps each step's latency short), then written out by the group.
+__global__ __launch_bounds__(256) void k_perm_from_pivots(const VecOp* ops, int nsys, const int* pivots,
+                                                          int* perm) {
+  const int s = blockIdx.x;
   if (s >= nsys) return;
   const VecOp op = ops[s];
-  int* p = perm + op.piv;
-  const int* ip = pivots + op.piv;
-  for (int i = 0; i < op.n; ++i) p[i] = i;
-  for (int i = 0; i < op.n; ++i) {
-    const int j = ip[i];
-    if (j != i) { const int t = p[i]; p[i] = p[j]; p[j] = t; }
+  if (op.n > 4096) return;   // the build rejects such systems
+  __shared__ int p[4096];
+  for (int i = threadIdx.x; i < op.n; i += blockDim.x) p[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int* ip = pivots + op.piv;
+    for (int i = 0; i < op.n; ++i) {
+      const int j = ip[i];
+      if (j != i) { const int t = p[i]; p[i] = p[j]; p[j] = t; }
+    }
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < op.n; i += blockDim.x) perm[op.piv + i] = p[i];
 }
 
 }  // namespace ghost
