@@ -769,8 +769,9 @@ static int build_schedule(ghost_bor* h) {
           return GemmOp{s.a_off + (long long)k * s.ld + k0, s.a_off + (long long)k0 * s.ld + k,
                         s.a_off + (long long)k * s.ld + k, s.ld, s.ld, s.ld, s.n - k, w, k - k0};
         });
-      // only the systems that still have a panel here (uneven batches)
-      Launch lp{L_PANEL, (int)h->h_panel.size(), 0, 0, h->rpt};
+      // only the systems that still have a panel here (uneven batches); the rows
+      // per thread follow the rows left below k, not the largest system
+      Launch lp{L_PANEL, (int)h->h_panel.size(), 0, 0, 1};
       Launch ls{L_SWAP, (int)h->h_swap.size(), 0, 0, 0};
       Launch lt{L_TRIL, (int)h->h_trsm.size(), 0, 0, 0};
       for (int si = 0; si < nsys; ++si) {
@@ -779,6 +780,7 @@ static int build_schedule(ghost_bor* h) {
         if (w <= 0) continue;
         const int ncols = s.n + s.nrhs;
         h->h_panel.push_back(PanelOp{s.a_off, s.n, s.ld, k, w, pofs[si], si});
+        lp.arg = std::max(lp.arg, (s.n - k + 1023) / 1024);
         h->h_swap.push_back(SwapOp{s.a_off, s.ld, k, w, k, k + w, ncols, pofs[si]});
         ls.gx = std::max(ls.gx, (k + (ncols - k - w) + 255) / 256);
         h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wl(si, b), s.ld, w});
