@@ -1148,9 +1148,10 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
       fa.pf = pf;
       fa.nfp = nfp;
       fa.npairs = nep * nfp;
-      const int grid = (fa.npairs + far_waves<R>() - 1) / far_waves<R>();
+      const int wpb = far_waves((int)sizeof(R), NT);
+      const int grid = (fa.npairs + wpb - 1) / wpb;
       const size_t tab_bytes = sizeof(R) * (size_t)(p.nE / 2 + 1 + p.nE + 2 * (p.nB / 2 + 1) + 2 * p.nB);
-      const size_t stash_bytes = sizeof(R) * (size_t)far_waves<R>() * NT * 8 * 64;
+      const size_t stash_bytes = sizeof(R) * (size_t)wpb * NT * 8 * 64;
       if (tab_bytes + stash_bytes > 160 * 1024)
         return fail(GHOST_ERR_ARG, "far fill: azimuth tables (%zu B) and order stash (%zu B) exceed the 160 KiB of LDS",
                     tab_bytes, stash_bytes);
@@ -1158,8 +1159,8 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
       switch (NT) {
 #define FARCASE(n)                                                                                      \
   case n:                                                                                               \
-    if (split) hipLaunchKernelGGL((k_far_fill<R, n, true>), dim3(grid), dim3(64 * far_waves<R>()), tab_bytes, h->stream, fa); \
-    else hipLaunchKernelGGL((k_far_fill<R, n, false>), dim3(grid), dim3(64 * far_waves<R>()), tab_bytes, h->stream, fa); \
+    if (split) hipLaunchKernelGGL((k_far_fill<R, n, true>), dim3(grid), dim3(64 * wpb), tab_bytes, h->stream, fa); \
+    else hipLaunchKernelGGL((k_far_fill<R, n, false>), dim3(grid), dim3(64 * wpb), tab_bytes, h->stream, fa); \
     break;
         FARCASE(1) FARCASE(2) FARCASE(3) FARCASE(4) FARCASE(5) FARCASE(6) FARCASE(7) FARCASE(8)
 #undef FARCASE

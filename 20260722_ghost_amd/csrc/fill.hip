@@ -49,20 +49,24 @@ __device__ __forceinline__ R rs_step(R lo, R hi, bool b) {
 }
 
 // Wavefronts (element pairs) per work-group of k_far_fill: they share one copy
-// of the azimuth tables in LDS.  FP32: 4 (with the register budget below, three
-// wavefronts per SIMD); FP64: 2 (its stash is twice the size).
-template <typename R> constexpr int far_waves() { return sizeof(R) == 4 ? 4 : 2; }
+// of the azimuth tables in LDS.  FP32 up to 6 order tiles (m_max <= 94): 4,
+// with a register budget of three wavefronts per SIMD; otherwise 2 (the FP64
+// stash is twice the size, and 7-8 tiles cannot meet that budget).  Those get
+// a budget of two wavefronts per SIMD where it costs no spills: FP32, and FP64
+// up to 2 tiles.
+__host__ __device__ constexpr bool far_wide(int bytes, int nt) { return bytes == 4 && nt <= 6; }
+__host__ __device__ constexpr int far_waves(int bytes, int nt) { return far_wide(bytes, nt) ? 4 : 2; }
 
-// One wavefront per far element pair (e, f); far_waves<R>() wavefronts per block.
+// One wavefront per far element pair (e, f); far_waves() wavefronts per block.
 // Phase 1: EFIE samples -> G_n on MFMA.  Phase 2: per-order EFIE blocks,
 // reduced over the test points and stashed in LDS.  Phase 3: MFIE brackets
 // contracted to nodal rows -> modes on MFMA.  Phase 4: combine, RMW Zfar.
 template <typename R, int NT, bool SPLIT>
-__global__ __launch_bounds__(64 * far_waves<R>())
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1))) void k_far_fill(FarArgs a) {
+__global__ __launch_bounds__(64 * far_waves(sizeof(R), NT))
+__attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4 || NT <= 2) ? 2 : 1))) void k_far_fill(FarArgs a) {
   typedef typename CT<R>::c C2;
   typedef typename CT<R>::v4 V4;
-  constexpr int WPB = far_waves<R>();
+  constexpr int WPB = far_waves(sizeof(R), NT);
   __shared__ R stash[WPB][NT][8][64];
   extern __shared__ __align__(16) char dyn_smem[];
   // azimuth tables in LDS, in the working precision:

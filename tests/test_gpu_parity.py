@@ -125,6 +125,25 @@ def test_unmasked_blocks_match_oracle():
     h.close()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("mm", [100, 126])
+def test_highest_orders_match_oracle(mm, precision):
+    """Seven and eight 16-order far-fill tiles (and 15-16 near-band chunks):
+    unmasked blocks against the oracle's, 1e-11 in FP64, 1e-5 (relative to
+    the block's largest entry) for the FP32 fill."""
+    d = load("oracle_sphere8_high_orders")
+    print(f"[case] high orders m_max={mm} precision={precision}", flush=True)
+    h = make_handle(d["nodes"], float(d["freq_hz"]), [0.0, 90.0], mm, float(d["alpha"]), False,
+                    precision=precision, unmasked=True)
+    h.fill()
+    h.sync()
+    for m in d[f"m_{mm}"]:
+        A, _ = h.system(0, int(m))
+        err = rel(A, d[f"A_{mm}_{m}"])
+        assert err < (1e-5 if precision else 1e-11), (m, err)
+    h.close()
+
+
 def test_larger_sphere_against_oracle_and_mie(bs):
     nodes = bo.sphere_nodes(0.5, 48)
     thetas = np.array([0.0, 37.0, 90.0, 143.0, 180.0])
