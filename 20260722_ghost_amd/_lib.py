@@ -37,6 +37,7 @@ SIGNATURES = [
     ("ghost_bor_create", _ci, [ctypes.POINTER(_vp), _ci, _ci]),
     ("ghost_bor_destroy", _ci, [_vp]),
     ("ghost_bor_set_residual", _ci, [_vp, _ci]),
+    ("ghost_bor_set_pairing", _ci, [_vp, _ci]),
     ("ghost_bor_add_problem", _ci, [_vp, _dp, _ci, _cd, _ci, _ci, _ci, _cd, _cd, _dp, _ci, _ci, _ip]),
     ("ghost_bor_build", _ci, [_vp]),
     ("ghost_bor_fill", _ci, [_vp]),
@@ -137,11 +138,14 @@ def iptr(a: np.ndarray):
 class Handle:
     """Owner of one ghost_bor handle (a batch of problems on the current device)."""
 
-    def __init__(self, precision: int = GHOST_FP64, nb: int = 64, residual: bool = True):
+    def __init__(self, precision: int = GHOST_FP64, nb: int = 64, residual: bool = True, pairing: bool = True):
+        """pairing: solve mode -m with mode m's LU factors (Z_{-m} = S Z_m S);
+        a handle that will run the adjoint needs it off."""
         self.lib = load()
         self._h = _vp()
         check(self.lib.ghost_bor_create(ctypes.byref(self._h), int(precision), int(nb)))
         check(self.lib.ghost_bor_set_residual(self._h, 1 if residual else 0))
+        check(self.lib.ghost_bor_set_pairing(self._h, 1 if pairing else 0))
         self.precision = precision
         self.problems = []
         self.nodes_per_problem = []

@@ -58,7 +58,7 @@ def solve_far_field(spec: RcsSpec, radii: np.ndarray):
     radii, and the solved FP64 handle (kept for the adjoint)."""
     nodes = spec.nodes.copy()
     nodes[:, 0] = radii
-    h = _lib.Handle(_lib.GHOST_FP64, nb=64, residual=False)
+    h = _lib.Handle(_lib.GHOST_FP64, nb=64, residual=False, pairing=False)
     a = spec.cfie_alpha
     h.add_problem(nodes, spec.k, spec.m_max, spec.n_xi[0], spec.n_xi[1], a, (1.0 - a) * ETA0, spec.thetas_deg)
     h.build()

@@ -216,7 +216,9 @@ struct ghost_bor {
   bool built = false;
   bool solved = false;               // the augmented columns hold the solution of the current fill
   std::vector<Problem> probs;
-  std::vector<SysInfo> sys;
+  std::vector<SysInfo> sys;          // one entry per (problem, mode) and per dense system
+  std::vector<SysInfo> ss;           // the systems factored and solved (nrhs = all their columns)
+  int pairing = 1;                   // solve mode -m with mode m's factors
   SysInfo* d_sys = nullptr;
   size_t a_elems = 0, x_elems = 0, a64_elems = 0;
   int piv_elems = 0;
@@ -376,6 +378,13 @@ extern "C" int ghost_bor_set_residual(ghost_bor* h, int on) {
   return 0;
 }
 
+extern "C" int ghost_bor_set_pairing(ghost_bor* h, int on) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "set mode pairing before build");
+  h->pairing = on ? 1 : 0;
+  return 0;
+}
+
 extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, double k, int m_max,
                                      int n_xi_efie, int n_xi_bracket, double c_efie, double c_mfie,
                                      const double* thetas_deg, int n_theta, int flags, int* idx) {
@@ -438,6 +447,7 @@ extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, 
     const int nf = Nn - (f_first ? 0 : 1) - (f_last ? 0 : 1);
     s.n = s.nt + nf;
     s.nrhs = p.nrhs;
+    s.ncols = p.nrhs;
     h->sys.push_back(s);
   }
   if (idx) *idx = (int)h->probs.size();
@@ -459,6 +469,7 @@ extern "C" int ghost_bor_add_dense(ghost_bor* h, int n, int nrhs, int* sys_idx) 
   s.n = n;
   s.nt = n;
   s.nrhs = nrhs;
+  s.ncols = nrhs;
   if (sys_idx) *sys_idx = (int)h->sys.size();
   h->sys.push_back(s);
   return 0;
@@ -667,15 +678,15 @@ static void set_flops(ghost_bor* h, std::vector<Launch>& sched) {
 
 static int build_schedule(ghost_bor* h) {
   const int nb = h->nb;
-  const int nsys = (int)h->sys.size();
+  const int nsys = (int)h->ss.size();
   int maxn = 0;
-  for (auto& s : h->sys) maxn = std::max(maxn, s.n);
+  for (auto& s : h->ss) maxn = std::max(maxn, s.n);
   h->rpt = (maxn + 1023) / 1024;
   if (h->rpt > 4) return fail(GHOST_ERR_ARG, "system size %d exceeds the panel limit 4096", maxn);
   // pivot offsets
   int poff = 0;
   std::vector<int> pofs(nsys);
-  for (int si = 0; si < nsys; ++si) { pofs[si] = poff; poff += h->sys[si].n; }
+  for (int si = 0; si < nsys; ++si) { pofs[si] = poff; poff += h->ss[si].n; }
   const int nblk_max = (maxn + nb - 1) / nb;
   h->max_blocks = nblk_max;
   const bool mixed = h->precision == GHOST_MIXED;
@@ -690,7 +701,7 @@ static int build_schedule(ghost_bor* h) {
       const int k = b * nb;
       Launch g1{L_GEMM_Y_W, (int)h->h_gemm.size(), nsys, 0, 0};
       for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
+        const SysInfo& s = h->ss[si];
         const int w = k < s.n ? std::min(nb, s.n - k) : 0;
         const long long o = off(si);
         const int c = nc(si);
@@ -700,7 +711,7 @@ static int build_schedule(ghost_bor* h) {
       }
       Launch g2{L_GEMM_Y_LU, (int)h->h_gemm.size(), nsys, 0, 0};
       for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
+        const SysInfo& s = h->ss[si];
         const int w = k < s.n ? std::min(nb, s.n - k) : 0;
         const long long o = off(si);
         const int c = nc(si);
@@ -717,7 +728,7 @@ static int build_schedule(ghost_bor* h) {
       Launch g1{L_GEMM_Y_W, (int)h->h_gemm.size(), nsys, 0, 0};
       std::vector<GemmOp> upd;
       for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
+        const SysInfo& s = h->ss[si];
         const int nbs = (s.n + nb - 1) / nb;
         const int b = nbs - 1 - st;
         const long long o = off(si);
@@ -752,7 +763,7 @@ static int build_schedule(ghost_bor* h) {
   auto lu_gemm = [&](int kind, auto make) {
     Launch g{kind, (int)h->h_gemm.size(), nsys, 0, 0};
     for (int si = 0; si < nsys; ++si) {
-      GemmOp go = make(h->sys[si]);
+      GemmOp go = make(h->ss[si]);
       if (go.M <= 0 || go.N <= 0 || go.K <= 0) go.M = go.N = go.K = 0;
       h->h_gemm.push_back(go);
       g.gx = std::max(g.gx, tiles(go.M, go.N));
@@ -775,7 +786,7 @@ static int build_schedule(ghost_bor* h) {
       Launch ls{L_SWAP, (int)h->h_swap.size(), 0, 0, 0};
       Launch lt{L_TRIL, (int)h->h_trsm.size(), 0, 0, 0};
       for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
+        const SysInfo& s = h->ss[si];
         const int w = k < s.n ? std::min(nb, s.n - k) : 0;
         if (w <= 0) continue;
         const int ncols = s.n + s.nrhs;
@@ -803,7 +814,7 @@ static int build_schedule(ghost_bor* h) {
       // X = L11^-1 X on rows [k, k+w), columns [k+w, ncols)
       lu_gemm(L_GEMM_AUG_W, [&](const SysInfo& s) {
         const int w = k < s.n ? std::min(nb, s.n - k) : 0;
-        return GemmOp{wl((int)(&s - h->sys.data()), b), s.a_off + (long long)k * s.ld + k + w,
+        return GemmOp{wl((int)(&s - h->ss.data()), b), s.a_off + (long long)k * s.ld + k + w,
                       s.a_off + (long long)k * s.ld + k + w, 64, s.ld, s.ld, w, s.n + s.nrhs - k - w, w};
       });
     }
@@ -817,7 +828,7 @@ static int build_schedule(ghost_bor* h) {
   // ── upper diagonal-block inverses (all blocks, one launch) ──
   h->triu_ops = (int)h->h_trsm.size();
   for (int si = 0; si < nsys; ++si) {
-    const SysInfo& s = h->sys[si];
+    const SysInfo& s = h->ss[si];
     for (int b = 0; b < nblk_max; ++b) {
       const int k = b * nb;
       const int w = k < s.n ? std::min(nb, s.n - k) : 0;
@@ -833,7 +844,7 @@ static int build_schedule(ghost_bor* h) {
   auto back_gemm = [&](int kind, auto make) {
     Launch g{kind, (int)h->h_gemm.size(), nsys, 0, 0};
     for (int si = 0; si < nsys; ++si) {
-      GemmOp go = make(si, h->sys[si]);
+      GemmOp go = make(si, h->ss[si]);
       if (go.M <= 0 || go.N <= 0 || go.K <= 0) go.M = go.N = go.K = 0;
       h->h_gemm.push_back(go);
       g.gx = std::max(g.gx, tiles(go.M, go.N));
@@ -877,31 +888,31 @@ static int build_schedule(ghost_bor* h) {
   }
   // ── VecOps over all systems (copies, casts, norms) ──
   std::vector<int> c_off(nsys);
-  for (int si = 0, co = 0; si < nsys; ++si) { c_off[si] = co; co += h->sys[si].nrhs; }
+  for (int si = 0, co = 0; si < nsys; ++si) { c_off[si] = co; co += h->ss[si].nrhs; }
   h->vec_all = (int)h->h_vec.size();
   for (int si = 0; si < nsys; ++si) {
-    const SysInfo& s = h->sys[si];
+    const SysInfo& s = h->ss[si];
     h->h_vec.push_back(VecOp{s.x_off, s.x_off, s.nrhs, s.nrhs, s.n, s.nrhs, 0, c_off[si]});
   }
   h->vec_aug = (int)h->h_vec.size();
   for (int si = 0; si < nsys; ++si) {
-    const SysInfo& s = h->sys[si];
+    const SysInfo& s = h->ss[si];
     h->h_vec.push_back(VecOp{s.a_off + s.n, s.x_off, s.ld, s.nrhs, s.n, s.nrhs, 0, c_off[si]});
   }
   h->vec_toaug = (int)h->h_vec.size();
   for (int si = 0; si < nsys; ++si) {
-    const SysInfo& s = h->sys[si];
+    const SysInfo& s = h->ss[si];
     h->h_vec.push_back(VecOp{s.x_off, s.a_off + s.n, s.nrhs, s.ld, s.n, s.nrhs, 0, c_off[si]});
   }
   // ── residual / refinement GEMM: R = B - A64 X (X in X64 or in the augmented columns) ──
   h->res_gemm = (int)h->h_gemm.size();
   for (int si = 0; si < nsys; ++si) {
-    const SysInfo& s = h->sys[si];
+    const SysInfo& s = h->ss[si];
     h->h_gemm.push_back(GemmOp{s.a64_off, s.x_off, s.x_off, s.n, s.nrhs, s.nrhs, s.n, s.nrhs, s.n});
   }
   h->res_gemm_aug = (int)h->h_gemm.size();
   for (int si = 0; si < nsys; ++si) {
-    const SysInfo& s = h->sys[si];
+    const SysInfo& s = h->ss[si];
     h->h_gemm.push_back(GemmOp{s.a64_off, s.a_off + s.n, s.x_off, s.n, s.ld, s.nrhs, s.n, s.nrhs, s.n});
   }
   if (mixed) {
@@ -909,26 +920,26 @@ static int build_schedule(ghost_bor* h) {
     h->probe_ops = (int)h->h_gemm.size();
     long long po = 0;
     h->p_off.assign(nsys, 0);
-    for (int si = 0; si < nsys; ++si) { h->p_off[si] = po; po += 2LL * h->sys[si].n; }
+    for (int si = 0; si < nsys; ++si) { h->p_off[si] = po; po += 2LL * h->ss[si].n; }
     h->vec_probe = (int)h->h_vec.size();
     for (int si = 0; si < nsys; ++si)
-      h->h_vec.push_back(VecOp{h->p_off[si], h->p_off[si], 2, 2, h->sys[si].n, 2, 0, 2 * si});
+      h->h_vec.push_back(VecOp{h->p_off[si], h->p_off[si], 2, 2, h->ss[si].n, 2, 0, 2 * si});
     for (int si = 0; si < nsys; ++si) {   // C = 0 - B (-S): written into BS as +B S (see ghost_bor_probe)
-      const SysInfo& s = h->sys[si];
+      const SysInfo& s = h->ss[si];
       h->h_gemm.push_back(GemmOp{s.x_off, 0, h->p_off[si], s.nrhs, 2, 2, s.n, 2, s.nrhs});
     }
     for (int si = 0; si < nsys; ++si) {
-      const SysInfo& s = h->sys[si];
+      const SysInfo& s = h->ss[si];
       h->h_gemm.push_back(GemmOp{s.x_off, 0, h->p_off[si], s.nrhs, 2, 2, s.n, 2, s.nrhs});
     }
     for (int si = 0; si < nsys; ++si) {
-      const SysInfo& s = h->sys[si];
+      const SysInfo& s = h->ss[si];
       h->h_gemm.push_back(GemmOp{s.a64_off, h->p_off[si], h->p_off[si], s.n, 2, 2, s.n, 2, s.n});
     }
     // forward / backward solves with the FP32 factors on separate RHS blocks:
     // Y (n x nrhs at x_off, the refinement corrections) and P (n x 2 at p_off, the probe)
-    add_tri_solves(h->fwdY_sched, h->backY_sched, [&](int si) { return h->sys[si].x_off; },
-                   [&](int si) { return h->sys[si].nrhs; });
+    add_tri_solves(h->fwdY_sched, h->backY_sched, [&](int si) { return h->ss[si].x_off; },
+                   [&](int si) { return h->ss[si].nrhs; });
     add_tri_solves(h->fwdP_sched, h->backP_sched, [&](int si) { return h->p_off[si]; },
                    [&](int si) { (void)si; return 2; });
   }
@@ -940,7 +951,7 @@ static int build_schedule(ghost_bor* h) {
       const int k = b * nb;
       Launch g1{L_GEMM_TW, (int)h->h_gemm.size(), nsys, 0, 0};
       for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
+        const SysInfo& s = h->ss[si];
         const int w = k < s.n ? std::min(nb, s.n - k) : 0;
         const long long o = s.x_off;
         const int c = s.nrhs;
@@ -948,7 +959,7 @@ static int build_schedule(ghost_bor* h) {
       }
       Launch g2{L_GEMM_TLU, (int)h->h_gemm.size(), nsys, 0, 0};
       for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
+        const SysInfo& s = h->ss[si];
         const int w = k < s.n ? std::min(nb, s.n - k) : 0;
         const long long o = s.x_off;
         const int c = s.nrhs;
@@ -964,7 +975,7 @@ static int build_schedule(ghost_bor* h) {
       Launch g1{L_GEMM_TW, (int)h->h_gemm.size(), nsys, 0, 0};
       std::vector<GemmOp> upd;
       for (int si = 0; si < nsys; ++si) {
-        const SysInfo& s = h->sys[si];
+        const SysInfo& s = h->ss[si];
         const int b = (s.n + nb - 1) / nb - 1 - st;
         const long long o = s.x_off;
         const int c = s.nrhs;
@@ -986,7 +997,7 @@ static int build_schedule(ghost_bor* h) {
     // Q[:, :n] = -Lambda X^T (X stored n x nrhs, used transposed)
     h->adj_q_ops = (int)h->h_gemm.size();
     for (int si = 0; si < nsys; ++si) {
-      const SysInfo& s = h->sys[si];
+      const SysInfo& s = h->ss[si];
       h->h_gemm.push_back(GemmOp{s.x_off, s.x_off, s.a_off, s.nrhs, s.nrhs, s.ld, s.n, s.n, s.nrhs});
     }
   }
@@ -1026,9 +1037,33 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
   if (h->sys.empty()) return fail(GHOST_ERR_STATE, "no problems or dense systems added");
   HIPCHK(hipSetDevice(h->device));
   const bool mixed = h->precision == GHOST_MIXED;
-  // system offsets
-  size_t a = 0, x = 0, a64 = 0;
+  // systems to solve: every mode, or with pairing modes m >= 0 only, mode m
+  // carrying mode -m's right-hand sides as columns [nrhs, 2 nrhs)
+  h->ss.clear();
   for (auto& s : h->sys) {
+    const bool paired = h->pairing && s.prob >= 0;
+    s.col = 0;
+    s.flip = 0;
+    if (paired && s.m < 0) continue;
+    s.ncols = s.nrhs + (paired && s.m > 0 ? s.nrhs : 0);
+    s.owner = (int)h->ss.size();
+    SysInfo o = s;
+    o.nrhs = o.ncols;
+    h->ss.push_back(o);
+  }
+  for (auto& s : h->sys)
+    if (h->pairing && s.prob >= 0 && s.m < 0) {
+      const SysInfo& tw = h->sys[h->probs[s.prob].sys0 + h->probs[s.prob].M - s.m];
+      if (tw.n != s.n || tw.nt != s.nt || tw.t0 != s.t0 || tw.f0 != s.f0)
+        return fail(GHOST_ERR_STATE, "modes %d and %d have different bases", s.m, -s.m);
+      s.owner = tw.owner;
+      s.ncols = tw.ncols;
+      s.col = s.nrhs;
+      s.flip = 1;
+    }
+  // offsets of the solved systems, shared by their mode entries
+  size_t a = 0, x = 0, a64 = 0;
+  for (auto& s : h->ss) {
     s.ld = ((s.n + s.nrhs + 15) / 16) * 16;
     s.a_off = (long long)a;
     s.x_off = (long long)x;
@@ -1036,6 +1071,13 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
     a += (size_t)s.n * s.ld;
     x += (size_t)s.n * s.nrhs;
     a64 += (size_t)s.n * s.n;
+  }
+  for (auto& s : h->sys) {
+    const SysInfo& o = h->ss[s.owner];
+    s.ld = o.ld;
+    s.a_off = o.a_off;
+    s.x_off = o.x_off;
+    s.a64_off = o.a64_off;
   }
   h->a_elems = a;
   h->x_elems = x;
@@ -1057,16 +1099,16 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
     if ((rc = dalloc(h, &h->d_X64, h->x_elems * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_Y32, h->x_elems * sizeof(float2)))) return rc;
     size_t pe = 0, nrmax = 0;
-    for (auto& s : h->sys) { pe += 2 * (size_t)s.n; nrmax = std::max(nrmax, (size_t)s.nrhs); }
+    for (auto& s : h->ss) { pe += 2 * (size_t)s.n; nrmax = std::max(nrmax, (size_t)s.nrhs); }
     if ((rc = dalloc(h, &h->d_S, nrmax * 2 * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_BS, pe * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_XS, pe * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_P32, pe * sizeof(float2)))) return rc;
     size_t ncol = 0;
-    for (auto& s : h->sys) ncol += (size_t)s.nrhs;
+    for (auto& s : h->ss) ncol += (size_t)s.nrhs;
     if ((rc = dalloc(h, &h->d_bscale, ncol * sizeof(double)))) return rc;
     if ((rc = dalloc(h, &h->d_rscale, ncol * sizeof(double)))) return rc;
-    if ((rc = dalloc(h, &h->d_pscale, 2 * h->sys.size() * sizeof(double)))) return rc;
+    if ((rc = dalloc(h, &h->d_pscale, 2 * h->ss.size() * sizeof(double)))) return rc;
     // S: fixed pseudo-random unit-modulus entries, -1 * S so that k_gemm_update's C -= A B adds B S
     std::vector<double2> S(nrmax * 2);
     unsigned long long st = 0x9E3779B97F4A7C15ULL;
@@ -1079,8 +1121,8 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
   }
   if ((rc = dalloc(h, &h->d_piv, (size_t)h->piv_elems * sizeof(int)))) return rc;
   if ((rc = dalloc(h, &h->d_perm, (size_t)h->piv_elems * sizeof(int)))) return rc;
-  if ((rc = dalloc(h, &h->d_info, h->sys.size() * sizeof(int)))) return rc;
-  if ((rc = dalloc(h, &h->d_W, h->sys.size() * (size_t)h->max_blocks * 2 * 64 * 64 * csz))) return rc;
+  if ((rc = dalloc(h, &h->d_info, h->ss.size() * sizeof(int)))) return rc;
+  if ((rc = dalloc(h, &h->d_W, h->ss.size() * (size_t)h->max_blocks * 2 * 64 * 64 * csz))) return rc;
   h->dF_elems = 0;
   h->partial_elems = 0;
   for (auto& p : h->probs) {
@@ -1138,6 +1180,7 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
   fa.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
   fa.cm = -2.0 * M_PI * p.cM;
   fa.Z = p.d_zfar;
+  fa.neg = h->pairing ? 0 : 1;
   const int NT = (p.M + 2 + 15) / 16;
   if (p.nE > 4096 || p.nB > 4096) return fail(GHOST_ERR_ARG, "azimuth grid above 4096 samples");
   for (int pe = 0; pe < 2; ++pe)
@@ -1273,7 +1316,8 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   aa.M = p.M;
   aa.A = h->d_A;
   aa.A64 = h->d_A64;
-  dim3 g(2 * p.Nn, (2 * p.Nn + 31) / 32, (p.nmodes + 31) / 32);
+  aa.mlo = h->pairing ? p.M : 0;
+  dim3 g(2 * p.Nn, (2 * p.Nn + 31) / 32, (p.nmodes - aa.mlo + 31) / 32);
   prof_begin(h);
   if (mixed) hipLaunchKernelGGL((k_assemble<float, float>), g, dim3(256), 0, h->stream, aa);
   else hipLaunchKernelGGL((k_assemble<double, double>), g, dim3(256), 0, h->stream, aa);
@@ -1340,12 +1384,14 @@ extern "C" int ghost_bor_set_solution(ghost_bor* h, int prob, int m, const doubl
   const SysInfo& s = h->sys[p.sys0 + m + p.M];
   HIPCHK(hipSetDevice(h->device));
   HIPCHK(hipStreamSynchronize(h->stream));
-  if (h->precision == GHOST_MIXED) {
-    HIPCHK(hipMemcpy(h->d_X64 + s.x_off, X, (size_t)s.n * s.nrhs * sizeof(double2), hipMemcpyHostToDevice));
-  } else {
-    for (int r = 0; r < s.n; ++r)
-      HIPCHK(hipMemcpy((double2*)h->d_A + (size_t)s.a_off + (size_t)r * s.ld + s.n, X + 2 * (size_t)r * s.nrhs,
-                       (size_t)s.nrhs * sizeof(double2), hipMemcpyHostToDevice));
+  std::vector<double2> row(s.nrhs);
+  for (int r = 0; r < s.n; ++r) {
+    const double f = (s.flip && r >= s.nt) ? -1.0 : 1.0;   // mode -m of a pair is stored as S x
+    for (int c = 0; c < s.nrhs; ++c)
+      row[c] = make_double2(f * X[2 * ((size_t)r * s.nrhs + c)], f * X[2 * ((size_t)r * s.nrhs + c) + 1]);
+    double2* dst = h->precision == GHOST_MIXED ? h->d_X64 + s.x_off + (size_t)r * s.ncols + s.col
+                                               : (double2*)h->d_A + (size_t)s.a_off + (size_t)r * s.ld + s.n + s.col;
+    HIPCHK(hipMemcpy(dst, row.data(), (size_t)s.nrhs * sizeof(double2), hipMemcpyHostToDevice));
   }
   return 0;
 }
@@ -1425,7 +1471,7 @@ template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launc
 }
 
 template <typename T> static int run_lu(ghost_bor* h) {
-  const int nsys = (int)h->sys.size();
+  const int nsys = (int)h->ss.size();
   hipLaunchKernelGGL(k_zero_int, dim3((nsys + 255) / 256), dim3(256), 0, h->stream, h->d_info, nsys);
   LAUNCHCHK();
   int rc = run_sched<T>(h, h->lu_sched, h->d_A, nullptr);
@@ -1441,10 +1487,10 @@ template <typename T> static int run_lu(ghost_bor* h) {
 
 // R64 = B64 - A64 X with X in X64 (x_in_aug = 0) or in the FP64 augmented columns
 static int residual_gemm(ghost_bor* h, int x_in_aug) {
-  const int nsys = (int)h->sys.size();
+  const int nsys = (int)h->ss.size();
   size_t maxn = 0, maxr = 0;
   int gx = 0;
-  for (auto& s : h->sys) {
+  for (auto& s : h->ss) {
     maxn = std::max(maxn, (size_t)s.n);
     maxr = std::max(maxr, (size_t)s.nrhs);
     gx = std::max(gx, ((s.n + 63) / 64) * ((s.nrhs + 63) / 64));
@@ -1478,7 +1524,7 @@ extern "C" int ghost_bor_fill(ghost_bor* h) {
 }
 
 static int frob2(ghost_bor* h, const double2* buf, std::vector<double>& out) {
-  // sum |z|^2 per system over an n x nrhs block laid out by x_off
+  // sum |z|^2 per mode entry over its n x nrhs columns of the block laid out by x_off
   const int nsys = (int)h->sys.size();
   out.assign(nsys, 0.0);
   std::vector<double2> host(h->x_elems);
@@ -1487,10 +1533,11 @@ static int frob2(ghost_bor* h, const double2* buf, std::vector<double>& out) {
   for (int si = 0; si < nsys; ++si) {
     const SysInfo& s = h->sys[si];
     double acc = 0;
-    for (size_t i = 0; i < (size_t)s.n * s.nrhs; ++i) {
-      const double2 z = host[s.x_off + i];
-      acc += z.x * z.x + z.y * z.y;
-    }
+    for (int r = 0; r < s.n; ++r)
+      for (int c = 0; c < s.nrhs; ++c) {
+        const double2 z = host[s.x_off + (size_t)r * s.ncols + s.col + c];
+        acc += z.x * z.x + z.y * z.y;
+      }
     out[si] = acc;
   }
   return 0;
@@ -1501,11 +1548,11 @@ static int frob2(ghost_bor* h, const double2* buf, std::vector<double>& out) {
 // the FP32 factors, eta[s] = ||E|| / ||X S|| estimates the relative error of
 // system s's currents (the correction the next refinement sweep would make).
 static int probe_error(ghost_bor* h, std::vector<double>& eta) {
-  const int nsys = (int)h->sys.size();
+  const int nsys = (int)h->ss.size();
   size_t pe = 0;
-  for (auto& s : h->sys) pe += 2 * (size_t)s.n;
+  for (auto& s : h->ss) pe += 2 * (size_t)s.n;
   int maxn = 0;
-  for (auto& s : h->sys) maxn = std::max(maxn, s.n);
+  for (auto& s : h->ss) maxn = std::max(maxn, s.n);
   const dim3 gz((unsigned)std::min<size_t>(65535, (pe + 255) / 256));
   hipLaunchKernelGGL(k_zero, gz, dim3(256), 0, h->stream, h->d_BS, pe);
   hipLaunchKernelGGL(k_zero, gz, dim3(256), 0, h->stream, h->d_XS, pe);
@@ -1538,7 +1585,7 @@ static int probe_error(ghost_bor* h, std::vector<double>& eta) {
   eta.assign(nsys, 0.0);
   for (int si = 0; si < nsys; ++si) {
     double e2 = 0, x2 = 0;
-    for (long long i = h->p_off[si], j = 0; i < h->p_off[si] + 2LL * h->sys[si].n; ++i, ++j) {
+    for (long long i = h->p_off[si], j = 0; i < h->p_off[si] + 2LL * h->ss[si].n; ++i, ++j) {
       const double f = 1.0 / sc[2 * si + (j & 1)];
       e2 += ((double)e[i].x * e[i].x + (double)e[i].y * e[i].y) * f * f;
       x2 += xs[i].x * xs[i].x + xs[i].y * xs[i].y;
@@ -1551,7 +1598,7 @@ static int probe_error(ghost_bor* h, std::vector<double>& eta) {
 extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
   if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
   HIPCHK(hipSetDevice(h->device));
-  const int nsys = (int)h->sys.size();
+  const int nsys = (int)h->ss.size();
   const bool mixed = h->precision == GHOST_MIXED;
   if (!h->capturing) HIPCHK(hipEventRecord(h->ev[2], h->stream));
   int rc;
@@ -1559,7 +1606,7 @@ extern "C" int ghost_bor_solve(ghost_bor* h, int refine_iters) {
     if ((rc = run_lu<double>(h))) return rc;
   } else {
     size_t maxn = 0, maxr = 0;
-    for (auto& s : h->sys) { maxn = std::max(maxn, (size_t)s.n); maxr = std::max(maxr, (size_t)s.nrhs); }
+    for (auto& s : h->ss) { maxn = std::max(maxn, (size_t)s.n); maxr = std::max(maxr, (size_t)s.nrhs); }
     dim3 gv((unsigned)((maxr + 255) / 256), (unsigned)maxn, nsys);
     dim3 gs((unsigned)((maxr + 63) / 64), 1, nsys);
     // FP32 right-hand sides = B64 with power-of-two column scales (no underflow
@@ -1834,6 +1881,7 @@ extern "C" int ghost_bor_get_dF(ghost_bor* h, int prob, double* out) {
 
 // Copy one mode system out: A (n x n), B (n x nrhs), X (n x nrhs), complex interleaved, row-major.
 // which: 0 = A and B as assembled (call after fill, before solve), 1 = X (after solve).
+// Mode -m of a pair is read from mode m's storage: A = S A_m S, B and X = S (its column block).
 extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, double* out1, double* out2) {
   if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
   if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index");
@@ -1844,40 +1892,52 @@ extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, do
   HIPCHK(hipStreamSynchronize(h->stream));
   const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
   std::vector<char> rowbuf((size_t)s.ld * csz);
-  auto get = [&](int r, int c) -> double2 {
+  std::vector<double2> row64((size_t)std::max(s.n, s.ncols));
+  auto get = [&](int c) -> double2 {
     if (mixed) { float2 v = ((float2*)rowbuf.data())[c]; return make_double2(v.x, v.y); }
     return ((double2*)rowbuf.data())[c];
   };
-  if (which == 0) {
-    for (int r = 0; r < s.n; ++r) {
-      HIPCHK(hipMemcpy(rowbuf.data(), (char*)h->d_A + ((size_t)s.a_off + (size_t)r * s.ld) * csz, (size_t)s.ld * csz,
-                       hipMemcpyDeviceToHost));
-      for (int c = 0; c < s.n; ++c) { double2 v = get(r, c); out1[2 * ((size_t)r * s.n + c)] = v.x; out1[2 * ((size_t)r * s.n + c) + 1] = v.y; }
-      if (out2)
-        for (int c = 0; c < s.nrhs; ++c) { double2 v = get(r, s.n + c); out2[2 * ((size_t)r * s.nrhs + c)] = v.x; out2[2 * ((size_t)r * s.nrhs + c) + 1] = v.y; }
-    }
-    if (mixed && h->d_A64) {   // the FP64 copies are exact; prefer them
-      HIPCHK(hipMemcpy(out1, h->d_A64 + s.a64_off, (size_t)s.n * s.n * sizeof(double2), hipMemcpyDeviceToHost));
-      if (out2) HIPCHK(hipMemcpy(out2, h->d_B64 + s.x_off, (size_t)s.n * s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
-    }
-  } else {
-    if (mixed) {
-      HIPCHK(hipMemcpy(out1, h->d_X64 + s.x_off, (size_t)s.n * s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
-    } else {
-      for (int r = 0; r < s.n; ++r) {
-        HIPCHK(hipMemcpy(rowbuf.data(), (char*)h->d_A + ((size_t)s.a_off + (size_t)r * s.ld) * csz, (size_t)s.ld * csz,
+  auto sgn = [&](int r, int c) { return (s.flip && ((r >= s.nt) != (c >= s.nt))) ? -1.0 : 1.0; };
+  auto rsgn = [&](int r) { return (s.flip && r >= s.nt) ? -1.0 : 1.0; };
+  auto put = [](double* o, size_t i, double2 v, double f) { o[2 * i] = f * v.x; o[2 * i + 1] = f * v.y; };
+  for (int r = 0; r < s.n; ++r) {
+    const bool aug_row = which == 0 ? !(mixed && h->d_A64) : !mixed;
+    if (aug_row)
+      HIPCHK(hipMemcpy(rowbuf.data(), (char*)h->d_A + ((size_t)s.a_off + (size_t)r * s.ld) * csz,
+                       (size_t)s.ld * csz, hipMemcpyDeviceToHost));
+    if (which == 0) {
+      if (aug_row) {
+        for (int c = 0; c < s.n; ++c) put(out1, (size_t)r * s.n + c, get(c), sgn(r, c));
+        if (out2)
+          for (int c = 0; c < s.nrhs; ++c) put(out2, (size_t)r * s.nrhs + c, get(s.n + s.col + c), rsgn(r));
+      } else {   // mixed: the FP64 copies are exact; prefer them
+        HIPCHK(hipMemcpy(row64.data(), h->d_A64 + s.a64_off + (size_t)r * s.n, (size_t)s.n * sizeof(double2),
                          hipMemcpyDeviceToHost));
-        for (int c = 0; c < s.nrhs; ++c) { double2 v = get(r, s.n + c); out1[2 * ((size_t)r * s.nrhs + c)] = v.x; out1[2 * ((size_t)r * s.nrhs + c) + 1] = v.y; }
+        for (int c = 0; c < s.n; ++c) put(out1, (size_t)r * s.n + c, row64[c], sgn(r, c));
+        if (out2) {
+          HIPCHK(hipMemcpy(row64.data(), h->d_B64 + s.x_off + (size_t)r * s.ncols + s.col,
+                           (size_t)s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
+          for (int c = 0; c < s.nrhs; ++c) put(out2, (size_t)r * s.nrhs + c, row64[c], rsgn(r));
+        }
       }
+    } else if (mixed) {
+      HIPCHK(hipMemcpy(row64.data(), h->d_X64 + s.x_off + (size_t)r * s.ncols + s.col,
+                       (size_t)s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
+      for (int c = 0; c < s.nrhs; ++c) put(out1, (size_t)r * s.nrhs + c, row64[c], rsgn(r));
+    } else {
+      for (int c = 0; c < s.nrhs; ++c) put(out1, (size_t)r * s.nrhs + c, get(s.n + s.col + c), rsgn(r));
     }
   }
   return 0;
 }
 
 extern "C" int ghost_bor_info(ghost_bor* h, int* out) {
-  // per-system count of zero pivots met by the panel factorisation
+  // per-system count of zero pivots met by the panel factorisation (a pair's
+  // two modes report their shared factorisation)
   if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
-  HIPCHK(hipMemcpy(out, h->d_info, h->sys.size() * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> info(h->ss.size());
+  HIPCHK(hipMemcpy(info.data(), h->d_info, info.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < h->sys.size(); ++i) out[i] = info[h->sys[i].owner];
   return 0;
 }
 
@@ -2035,8 +2095,10 @@ static bool adj_free(const Problem& p, int q, int r, double step) {
 extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad) {
   if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
   if (!h->built) return fail(GHOST_ERR_STATE, "build first");
-  for (auto& s : h->sys)
+  for (auto& s : h->sys) {
     if (s.prob < 0) return fail(GHOST_ERR_STATE, "the adjoint covers handles of BoR problems only");
+    if (s.ncols != s.nrhs) return fail(GHOST_ERR_STATE, "the adjoint needs a handle built with mode pairing off");
+  }
   if (h->precision != GHOST_FP64) return fail(GHOST_ERR_STATE, "the adjoint needs an FP64 handle");
   if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
   if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
@@ -2286,12 +2348,13 @@ extern "C" int ghost_bor_get_dense(ghost_bor* h, int idx, double* X) {
 
 // Row interchanges of system idx's factorization: ipiv[r] = the row swapped
 // with row r at step r (0-based within the system), LAPACK getrf's ipiv - 1.
+// Mode -m of a pair reports mode m's (S A_m S pivots on the same rows).
 extern "C" int ghost_bor_get_pivots(ghost_bor* h, int idx, int* ipiv) {
   if (!h || !ipiv) return fail(GHOST_ERR_ARG, "null argument");
   if (idx < 0 || idx >= (int)h->sys.size()) return fail(GHOST_ERR_ARG, "system index %d out of range", idx);
   if (!h->solved) return fail(GHOST_ERR_STATE, "no factorization yet: solve or run first");
-  long long off = 0;
-  for (int si = 0; si < idx; ++si) off += h->sys[si].n;
+  long long off = 0;   // pivots are stored per solved system (a pair shares mode m's)
+  for (int si = 0; si < h->sys[idx].owner; ++si) off += h->ss[si].n;
   HIPCHK(hipSetDevice(h->device));
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(ipiv, h->d_piv + off, (size_t)h->sys[idx].n * sizeof(int), hipMemcpyDeviceToHost));

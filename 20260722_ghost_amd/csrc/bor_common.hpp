@@ -125,7 +125,10 @@ struct PointSet {
   const double* D1;   // d(rho T1)/dt
 };
 
-// One assembled mode system inside a batch.
+// One mode system inside a batch.  Paired modes: Z_{-m} = S Z_m S with S =
+// diag(I on the J_t rows, -I on the J_phi rows), so mode -m is solved with
+// mode m's factors as extra right-hand-side columns, Z_m (S x) = S b: its
+// entry (flip = 1) points into mode m's storage at column block col.
 struct SysInfo {
   int prob;      // owning problem (body x frequency)
   int m;         // azimuthal mode
@@ -134,8 +137,12 @@ struct SysInfo {
   int t0;        // first active J_t node
   int f0;        // first active J_phi node (rows [nt, n))
   int ld;        // leading dimension of the augmented row-major system
-  int nrhs;      // right-hand sides (2 x n_theta)
+  int nrhs;      // this mode's right-hand sides (2 x n_theta)
+  int ncols;     // right-hand-side columns of the solved system (nrhs, or 2 nrhs for a pair)
+  int col;       // first of this mode's columns among them
+  int flip;      // 1: S applies (mode -m of a pair)
+  int owner;     // index of the solved system (in solve order)
   long long a_off;   // element offset of the augmented system [n x ld]
-  long long x_off;   // element offset of the n x nrhs solution / RHS copy
+  long long x_off;   // element offset of the n x ncols solution / RHS copy
   long long a64_off; // element offset of the FP64 n x n copy (mixed precision)
 };

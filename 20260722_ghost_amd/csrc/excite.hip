@@ -112,8 +112,8 @@ __device__ __forceinline__ double2 jn_val(const double* J, int n, size_t stride)
 }
 
 // RHS columns theta-major then (VV, HH), written into the augmented system
-// columns [n, n + nrhs) (bor_solver.py:750-752) when Abase is set, and into
-// the FP64 copy B64 when that is set.
+// columns [n + col, n + col + nrhs) (bor_solver.py:750-752) when Abase is
+// set, and into the FP64 copy B64 when that is set.
 template <typename Ra>
 __global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, void* Abase,
                                              double2* B64, int sys0) {
@@ -168,8 +168,10 @@ __global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, v
       }
     }
   }
+  if (s.flip && phi)   // mode -m of a pair: S b, in mode m's columns [col, col + nrhs)
+    for (int pol = 0; pol < 2; ++pol) { acc[pol].x = -acc[pol].x; acc[pol].y = -acc[pol].y; }
   if (Abase) {
-    Ca* A = (Ca*)Abase + s.a_off + (size_t)r * s.ld + s.n + 2 * it;
+    Ca* A = (Ca*)Abase + s.a_off + (size_t)r * s.ld + s.n + s.col + 2 * it;
     Ca o0, o1;
     o0.x = (Ra)acc[0].x; o0.y = (Ra)acc[0].y;
     o1.x = (Ra)acc[1].x; o1.y = (Ra)acc[1].y;
@@ -177,15 +179,15 @@ __global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, v
     A[1] = o1;
   }
   if (B64) {
-    double2* b = B64 + s.x_off + (size_t)r * s.nrhs + 2 * it;
+    double2* b = B64 + s.x_off + (size_t)r * s.ncols + s.col + 2 * it;
     b[0] = acc[0];
     b[1] = acc[1];
   }
 }
 
 // Partial far-field sums over a chunk of Gauss points for every (system, column).
-// X is read from rows of an n x nrhs block at element offset x_off with leading
-// dimension ldx (the augmented system or a separate solution buffer).
+// X is read from the mode's columns of the augmented system or of the n x
+// ncols solution block at element offset x_off.
 template <typename Rx>
 __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const SysInfo* sys,
                                                           const void* Xbase, int x_in_aug, int chunk,
@@ -204,8 +206,9 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
   const size_t stride = (size_t)a.P * a.ntheta;
   const Cx* X;
   size_t ldx;
-  if (x_in_aug) { X = (const Cx*)Xbase + s.a_off + s.n; ldx = s.ld; }
-  else { X = (const Cx*)Xbase + s.x_off; ldx = s.nrhs; }
+  if (x_in_aug) { X = (const Cx*)Xbase + s.a_off + s.n + s.col; ldx = s.ld; }
+  else { X = (const Cx*)Xbase + s.x_off + s.col; ldx = s.ncols; }
+  const double fs = s.flip ? -1.0 : 1.0;   // mode -m of a pair holds S x
   const int nf = s.n - s.nt;
   const PointSet& P = a.pts;
   auto xt = [&](int node) -> double2 {
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
     const int r = node - s.f0;
     if (r < 0 || r >= nf) return make_double2(0.0, 0.0);
     const Cx v = X[(size_t)(s.nt + r) * ldx + col];
-    return make_double2((double)v.x, (double)v.y);
+    return make_double2(fs * (double)v.x, fs * (double)v.y);
   };
   double2 acc = make_double2(0.0, 0.0);
   int e_prev = -1;

@@ -35,6 +35,7 @@ struct FarArgs {
   const double* cosB;    // [nB]
   const double* sinB;    // [nB]
   int Ne, Nn, M, nE, nB, pe, pf, nfp, npairs;
+  int neg;               // also write modes -n (off with pairing: they are never read)
   double k, ce_re, ce_im, cm;
   void* Z;               // Zfar [2Nn][2Nn][2M+1] complex<R>
 };
@@ -350,7 +351,7 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       C2 v;
       v = pe_[n]; v.x += zer; v.y += zei; pe_[n] = v;
       v = po_[n]; v.x += zor; v.y += zoi; po_[n] = v;
-      if (n > 0) {
+      if (n > 0 && a.neg) {
         v = pe_[-n]; v.x += zer; v.y += zei; pe_[-n] = v;
         v = po_[-n]; v.x -= zor; v.y -= zoi; po_[-n] = v;
       }
@@ -731,6 +732,7 @@ struct AsmArgs {
   const double2* NB;         // [3 Ne][16][nmodes]
   const SysInfo* sys;        // systems of this problem, index = m + M
   int Nn, Ne, M;
+  int mlo;                   // first mode index assembled (M with pairing: modes -m are not stored)
   void* A;                   // augmented systems (complex<Ra>), base pointer
   double2* A64;              // optional FP64 copy of the n x n block (nullptr if unused)
 };
@@ -742,7 +744,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
   __shared__ double2 tile[32][33];
   const int i_row = blockIdx.x;            // node-level row in [0, 2Nn)
   const int j0 = blockIdx.y * 32;          // node-level column chunk
-  const int m0 = blockIdx.z * 32;          // mode chunk (index into [0, nmodes))
+  const int m0 = a.mlo + blockIdx.z * 32;  // mode chunk (index into [0, nmodes))
   const int nmodes = 2 * a.M + 1;
   const size_t ncol = 2 * (size_t)a.Nn;
   const Cf* Z = (const Cf*)a.Zfar;

@@ -75,6 +75,15 @@ int ghost_bor_destroy(ghost_bor* h);
 /* Keep FP64 copies of A and B for ghost_bor_residual (default on; before build). */
 int ghost_bor_set_residual(ghost_bor* h, int on);
 
+/* Mode pairing (default on; before build).  The reference solves every mode
+ * system on its own (bor_solver.py:744-753); here Z_{-m} = S Z_m S, with S
+ * the identity on the J_t rows and minus it on the J_phi rows, so mode -m is
+ * solved with mode m's LU factors as extra right-hand-side columns,
+ * Z_m (S x) = S b, and only m >= 0 is factored.  Accessors by (problem,
+ * mode) are unchanged.  No counterpart in the reference; ghost_bor_adjoint
+ * needs a handle built with pairing off. */
+int ghost_bor_set_pairing(ghost_bor* h, int on);
+
 /* Add one problem.  nodes: [Nn][2] (rho, z), traversed +z end first
  * (bor_kernels.py:38-81).  k: real wavenumber (2 pi f / c0).  m_max: modes
  * -m_max..m_max.  n_xi_efie / n_xi_bracket: azimuth FFT sizes (powers of two;

@@ -97,14 +97,14 @@ def test_adjoint_of_a_batch_equals_single_problem_adjoints():
     ws = [rng.uniform(0.5, 1.5, size=(2, p[3].size)) for p in probs]
     single = []
     for p, w in zip(probs, ws):
-        h = L.Handle(L.GHOST_FP64, residual=False)
+        h = L.Handle(L.GHOST_FP64, residual=False, pairing=False)
         setup(h, *p)
         h.build()
         h.run(0)
         h.sync()
         single.append(h.adjoint(acoef(h.dF(0), w), 0.0))
         h.close()
-    h = L.Handle(L.GHOST_FP64, residual=False)
+    h = L.Handle(L.GHOST_FP64, residual=False, pairing=False)
     for p in probs:
         setup(h, *p)
     h.build()
