@@ -150,6 +150,7 @@ struct DevBuf {
 
 struct Problem {
   int Nn = 0, Ne = 0, P = 0, M = 0, nE = 0, nB = 0, ntheta = 0, nrhs = 0, nmodes = 0;
+  int zmodes = 0, zoff = 0;      // far-block mode stride and index of mode 0
   double k = 0, cE = 0, cM = 0;
   std::vector<double> nodes, theta;
   bool axis0 = false, axis1 = false;
@@ -620,7 +621,10 @@ static int prepare_problem(ghost_bor* h, Problem& p) {
   if ((rc = dalloc(h, &p.d_batches, p.batches.size() * sizeof(int2)))) return rc;
   if ((rc = dalloc(h, &p.d_NP, (size_t)p.npp * (p.M + 2) * 5 * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &p.d_NB, (size_t)3 * Ne * 16 * p.nmodes * sizeof(double2)))) return rc;
-  const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.nmodes;
+  // far blocks [2Nn][2Nn][zmodes]: modes -M..M, or 0..M with pairing
+  p.zmodes = h->pairing ? p.M + 1 : p.nmodes;
+  p.zoff = h->pairing ? 0 : p.M;
+  const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.zmodes;
   const size_t zsz = h->precision == GHOST_FP64 ? sizeof(double2) : sizeof(float2);
   if ((rc = dalloc(h, &p.d_zfar, zel * zsz))) return rc;
 
@@ -1181,6 +1185,8 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
   fa.cm = -2.0 * M_PI * p.cM;
   fa.Z = p.d_zfar;
   fa.neg = h->pairing ? 0 : 1;
+  fa.zmodes = p.zmodes;
+  fa.zoff = p.zoff;
   const int NT = (p.M + 2 + 15) / 16;
   if (p.nE > 4096 || p.nB > 4096) return fail(GHOST_ERR_ARG, "azimuth grid above 4096 samples");
   for (int pe = 0; pe < 2; ++pe)
@@ -1237,7 +1243,7 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   const bool mixed = h->precision == GHOST_MIXED;
   const int P = p.P;
   if ((p.M + 2 + 7) / 8 > 16 || p.M + 1 > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near band");
-  const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.nmodes;
+  const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.zmodes;
   const size_t zero_n = mixed ? (zel + 1) / 2 : zel;  // in double2 units
   prof_begin(h);
   hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (zero_n + 255) / 256)), dim3(256), 0,
@@ -1317,6 +1323,8 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   aa.A = h->d_A;
   aa.A64 = h->d_A64;
   aa.mlo = h->pairing ? p.M : 0;
+  aa.zmodes = p.zmodes;
+  aa.zoff = p.zoff;
   dim3 g(2 * p.Nn, (2 * p.Nn + 31) / 32, (p.nmodes - aa.mlo + 31) / 32);
   prof_begin(h);
   if (mixed) hipLaunchKernelGGL((k_assemble<float, float>), g, dim3(256), 0, h->stream, aa);

@@ -35,7 +35,8 @@ struct FarArgs {
   const double* cosB;    // [nB]
   const double* sinB;    // [nB]
   int Ne, Nn, M, nE, nB, pe, pf, nfp, npairs;
-  int neg;               // also write modes -n (off with pairing: they are never read)
+  int neg;               // also write modes -n (off with pairing: they are never stored)
+  int zmodes, zoff;      // Z mode stride and the index of mode 0 (2M+1, M; or M+1, 0)
   double k, ce_re, ce_im, cm;
   void* Z;               // Zfar [2Nn][2Nn][2M+1] complex<R>
 };
@@ -326,7 +327,6 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
 
   // ── phase 4: Z = ce * EFIE + cm * K for modes +n and -n ──
   const R ce_re = (R)a.ce_re, ce_im = (R)a.ce_im, cm = (R)a.cm;
-  const int nmodes = 2 * a.M + 1;
   const size_t ncol = 2 * (size_t)a.Nn;
   C2* Z = (C2*)a.Z;
   const int rowE = (uvsub ? a.Nn : 0) + e + asel;   // tt / ff, and tf / ft rows
@@ -346,8 +346,8 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       const R zei = ce_re * ei + ce_im * er + cm * Kei;
       const R zor = ce_re * orr - ce_im * oi + cm * Kor;
       const R zoi = ce_re * oi + ce_im * orr + cm * Koi;
-      C2* pe_ = Z + ((size_t)rowE * ncol + colE + bi) * nmodes + a.M;
-      C2* po_ = Z + ((size_t)rowE * ncol + colO + bi) * nmodes + a.M;
+      C2* pe_ = Z + ((size_t)rowE * ncol + colE + bi) * a.zmodes + a.zoff;
+      C2* po_ = Z + ((size_t)rowE * ncol + colO + bi) * a.zmodes + a.zoff;
       C2 v;
       v = pe_[n]; v.x += zer; v.y += zei; pe_[n] = v;
       v = po_[n]; v.x += zor; v.y += zoi; po_[n] = v;
@@ -728,11 +728,12 @@ __global__ __launch_bounds__(256) void k_near_blocks(NearBlockArgs b, int mlanes
 // ─────────────────────────────────────────────────────────────────────────────
 
 struct AsmArgs {
-  const void* Zfar;          // [2Nn][2Nn][nmodes] complex<Rf>
+  const void* Zfar;          // [2Nn][2Nn][zmodes] complex<Rf>, mode m at m + zoff
   const double2* NB;         // [3 Ne][16][nmodes]
   const SysInfo* sys;        // systems of this problem, index = m + M
   int Nn, Ne, M;
   int mlo;                   // first mode index assembled (M with pairing: modes -m are not stored)
+  int zmodes, zoff;
   void* A;                   // augmented systems (complex<Ra>), base pointer
   double2* A64;              // optional FP64 copy of the n x n block (nullptr if unused)
 };
@@ -754,7 +755,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
     const int j = j0 + jj, mi = m0 + mm;
     double2 v = make_double2(0, 0);
     if (j < (int)ncol && mi < nmodes) {
-      const Cf z = Z[((size_t)i_row * ncol + j) * nmodes + mi];
+      const Cf z = Z[((size_t)i_row * ncol + j) * a.zmodes + (mi - a.M + a.zoff)];
       v = make_double2((double)z.x, (double)z.y);
     }
     tile[jj][mm] = v;
