@@ -1137,7 +1137,7 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
   }
   if ((rc = dalloc(h, &h->d_dF, h->dF_elems * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &h->d_partial, h->partial_elems * sizeof(double2)))) return rc;
-  if ((rc = dalloc(h, &h->d_red, 2 * h->sys.size() * sizeof(double)))) return rc;
+  if ((rc = dalloc(h, &h->d_red, 2 * h->sys.size() * sizeof(double)))) return rc;   // per-system probe ratios
   h->built = true;
   return 0;
 }
@@ -1583,23 +1583,13 @@ static int probe_error(ghost_bor* h, std::vector<double>& eta) {
   int rc;
   if ((rc = run_sched<float>(h, h->fwdP_sched, h->d_A, h->d_P32))) return rc;
   if ((rc = run_sched<float>(h, h->backP_sched, h->d_A, h->d_P32))) return rc;
-  std::vector<float2> e(pe);
-  std::vector<double2> xs(pe);
-  std::vector<double> sc(2 * (size_t)nsys);
-  HIPCHK(hipMemcpyAsync(e.data(), h->d_P32, pe * sizeof(float2), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(xs.data(), h->d_XS, pe * sizeof(double2), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipMemcpyAsync(sc.data(), h->d_pscale, sc.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  // ratios on the device: only nsys doubles come back to decide the sweep
+  hipLaunchKernelGGL(k_probe_eta, dim3(nsys), dim3(256), 0, h->stream, h->d_vec + h->vec_probe, h->d_P32, h->d_XS,
+                     h->d_pscale, h->d_red);
+  LAUNCHCHK();
   eta.assign(nsys, 0.0);
-  for (int si = 0; si < nsys; ++si) {
-    double e2 = 0, x2 = 0;
-    for (long long i = h->p_off[si], j = 0; i < h->p_off[si] + 2LL * h->ss[si].n; ++i, ++j) {
-      const double f = 1.0 / sc[2 * si + (j & 1)];
-      e2 += ((double)e[i].x * e[i].x + (double)e[i].y * e[i].y) * f * f;
-      x2 += xs[i].x * xs[i].x + xs[i].y * xs[i].y;
-    }
-    eta[si] = x2 > 0 ? std::sqrt(e2 / x2) : (e2 > 0 ? INFINITY : 0.0);
-  }
+  HIPCHK(hipMemcpyAsync(eta.data(), h->d_red, nsys * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
 

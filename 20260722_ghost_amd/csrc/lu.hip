@@ -800,6 +800,36 @@ __global__ __launch_bounds__(256) void k_col_scale(const VecOp* ops, const doubl
   }
 }
 
+// Error probe ratios per system: eta = ||E / scale|| / ||XS|| over the n x 2
+// probe block (E: FP32 corrections at op.d, XS: FP64 at op.s, both ld 2;
+// column scales at op.sc), 0 for a zero block and +inf for E != 0 with XS = 0.
+// One work-group per system; the sums run in a fixed order.
+__global__ __launch_bounds__(256) void k_probe_eta(const VecOp* ops, const float2* E, const double2* XS,
+                                                   const double* scale, double* eta) {
+  const VecOp op = ops[blockIdx.x];
+  __shared__ double re[256], rx[256];
+  double e2 = 0.0, x2 = 0.0;
+  const double f0 = 1.0 / scale[op.sc], f1 = 1.0 / scale[op.sc + 1];
+  for (int i = threadIdx.x; i < 2 * op.n; i += 256) {
+    const float2 e = E[op.d + i];
+    const double2 x = XS[op.s + i];
+    const double f = (i & 1) ? f1 : f0;
+    e2 += ((double)e.x * e.x + (double)e.y * e.y) * f * f;
+    x2 += x.x * x.x + x.y * x.y;
+  }
+  re[threadIdx.x] = e2;
+  rx[threadIdx.x] = x2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) { re[threadIdx.x] += re[threadIdx.x + o]; rx[threadIdx.x] += rx[threadIdx.x + o]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double a = re[0], b = rx[0];
+    eta[blockIdx.x] = b > 0 ? sqrt(a / b) : (a > 0 ? INFINITY : 0.0);
+  }
+}
+
 // dst32[r][c] = (float) (src64[perm ? perm[r] : r][c] * scale[c])
 __global__ void k_scale_cast(const VecOp* ops, const int* perm, const double2* src, const double* scale,
                              float2* dst) {
