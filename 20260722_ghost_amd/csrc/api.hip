@@ -250,7 +250,14 @@ struct ghost_bor {
   TrsmOp* d_trsm = nullptr;
   GemmOp* d_gemm = nullptr;
   VecOp* d_vec = nullptr;
-  std::vector<Launch> lu_sched, back_sched, fwdY_sched, backY_sched, fwdP_sched, backP_sched;
+  // LU + back substitution in up to two interleaved parts of the systems (even
+  // / odd solve index), run on two streams so that one part's latency-bound
+  // panel phases overlap the other part's trailing updates
+  int nparts = 1;
+  std::vector<Launch> lu_sched[2], back_sched[2];
+  std::vector<Launch> fwdY_sched, backY_sched, fwdP_sched, backP_sched;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // adjoint (FP64 handles): A^T lambda = G through the LU factors, Q = -Lambda X^T
   std::vector<Launch> adjT_fwd, adjT_back;
   int adj_q_ops = -1;
@@ -264,7 +271,7 @@ struct ghost_bor {
   double2* d_ffpart = nullptr;       // per-element far-field partial sums of one problem
   double2* d_ffs = nullptr;          // per-element contraction with a'
   int* d_rowoff = nullptr;
-  int triu_ops = 0, triu_n = 0;
+  int triu_ops[2] = {0, 0}, triu_n[2] = {0, 0};
   int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
   int res_tt[3] = {0, 0, 0}, res_aug_tt[3] = {0, 0, 0}, adj_q_tt[3] = {0, 0, 0};
   int vec_all = 0, vec_aug = 0;          // VecOps over all systems: x-blocks / augmented RHS -> x-blocks
@@ -348,6 +355,9 @@ extern "C" int ghost_bor_create(ghost_bor** out, int precision, int nb) {
   h->precision = precision;
   h->nb = nb;
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) { delete h; return fail(GHOST_ERR_HIP, "stream create: %s", hipGetErrorString(e)); }
   for (int i = 0; i < 8; ++i) {
     e = hipEventCreate(&h->ev[i]);
@@ -361,12 +371,16 @@ extern "C" int ghost_bor_destroy(ghost_bor* h) {
   if (!h) return 0;
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->stream2) hipStreamSynchronize(h->stream2);
   for (auto e : h->pev) hipEventDestroy(e);
   if (h->gexec) hipGraphExecDestroy(h->gexec);
   if (h->graph) hipGraphDestroy(h->graph);
   for (auto& a : h->allocs) hipFree(a.p);
   for (int i = 0; i < 8; ++i) hipEventDestroy(h->ev[i]);
 
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -764,15 +778,20 @@ static int build_schedule(ghost_bor* h) {
   // rows and columns beyond the super-block: the trailing matrix is read and
   // written once per S panels instead of once per panel.
   const int S = std::max(1, h->lu_super);
+  h->nparts = nsys >= 2 ? 2 : 1;
+  for (int part = 0; part < h->nparts; ++part) {
+  std::vector<int> psys;
+  for (int si = part; si < nsys; si += h->nparts) psys.push_back(si);
+  const int np = (int)psys.size();
   auto lu_gemm = [&](int kind, auto make) {
-    Launch g{kind, (int)h->h_gemm.size(), nsys, 0, 0};
-    for (int si = 0; si < nsys; ++si) {
+    Launch g{kind, (int)h->h_gemm.size(), np, 0, 0};
+    for (int si : psys) {
       GemmOp go = make(h->ss[si]);
       if (go.M <= 0 || go.N <= 0 || go.K <= 0) go.M = go.N = go.K = 0;
       h->h_gemm.push_back(go);
       g.gx = std::max(g.gx, tiles(go.M, go.N));
     }
-    h->lu_sched.push_back(g);
+    h->lu_sched[part].push_back(g);
   };
   for (int b0 = 0; b0 < nblk_max; b0 += S) {
     const int k0 = b0 * nb, nsb = std::min(S, nblk_max - b0);
@@ -789,7 +808,7 @@ static int build_schedule(ghost_bor* h) {
       Launch lp{L_PANEL, (int)h->h_panel.size(), 0, 0, 1};
       Launch ls{L_SWAP, (int)h->h_swap.size(), 0, 0, 0};
       Launch lt{L_TRIL, (int)h->h_trsm.size(), 0, 0, 0};
-      for (int si = 0; si < nsys; ++si) {
+      for (int si : psys) {
         const SysInfo& s = h->ss[si];
         const int w = k < s.n ? std::min(nb, s.n - k) : 0;
         if (w <= 0) continue;
@@ -805,9 +824,11 @@ static int build_schedule(ghost_bor* h) {
       }
       lp.gx = lp.nops;
       lt.gx = lt.nops;
-      h->lu_sched.push_back(lp);
-      h->lu_sched.push_back(ls);
-      h->lu_sched.push_back(lt);
+      if (lp.nops > 0) {   // a part may have run out of panels before the largest system
+        h->lu_sched[part].push_back(lp);
+        h->lu_sched[part].push_back(ls);
+        h->lu_sched[part].push_back(lt);
+      }
       if (sb > 0)   // row catch-up: A[k:k+w, k+w:] -= L[k:k+w, k0:k] U[k0:k, k+w:]
         lu_gemm(L_GEMM_AUG, [&](const SysInfo& s) {
           const int w = k < s.n ? std::min(nb, s.n - k) : 0;
@@ -830,8 +851,8 @@ static int build_schedule(ghost_bor* h) {
     });
   }
   // ── upper diagonal-block inverses (all blocks, one launch) ──
-  h->triu_ops = (int)h->h_trsm.size();
-  for (int si = 0; si < nsys; ++si) {
+  h->triu_ops[part] = (int)h->h_trsm.size();
+  for (int si : psys) {
     const SysInfo& s = h->ss[si];
     for (int b = 0; b < nblk_max; ++b) {
       const int k = b * nb;
@@ -839,21 +860,21 @@ static int build_schedule(ghost_bor* h) {
       if (w > 0) h->h_trsm.push_back(TrsmOp{s.a_off + (long long)k * s.ld + k, wu(si, b), s.ld, w});
     }
   }
-  h->triu_n = (int)h->h_trsm.size() - h->triu_ops;
+  h->triu_n[part] = (int)h->h_trsm.size() - h->triu_ops[part];
   // ── back substitution on the RHS columns, in super-blocks of S row blocks ──
   // Walking up from the last block: block b catches up on the super-block's
   // blocks below it (K = rows solved so far in the super-block), is solved
   // through W_u, and after the super-block one update with K up to nb x S
   // carries its rows into every row above.
   auto back_gemm = [&](int kind, auto make) {
-    Launch g{kind, (int)h->h_gemm.size(), nsys, 0, 0};
-    for (int si = 0; si < nsys; ++si) {
+    Launch g{kind, (int)h->h_gemm.size(), np, 0, 0};
+    for (int si : psys) {
       GemmOp go = make(si, h->ss[si]);
       if (go.M <= 0 || go.N <= 0 || go.K <= 0) go.M = go.N = go.K = 0;
       h->h_gemm.push_back(go);
       g.gx = std::max(g.gx, tiles(go.M, go.N));
     }
-    h->back_sched.push_back(g);
+    h->back_sched[part].push_back(g);
   };
   for (int st0 = 0; st0 < nblk_max; st0 += S) {
     const int nsb = std::min(S, nblk_max - st0);
@@ -890,6 +911,7 @@ static int build_schedule(ghost_bor* h) {
                     s.nrhs, k_end(s) - k_lo};
     });
   }
+  }  // parts
   // ── VecOps over all systems (copies, casts, norms) ──
   std::vector<int> c_off(nsys);
   for (int si = 0, co = 0; si < nsys; ++si) { c_off[si] = co; co += h->ss[si].nrhs; }
@@ -1005,14 +1027,17 @@ static int build_schedule(ghost_bor* h) {
       h->h_gemm.push_back(GemmOp{s.x_off, s.x_off, s.a_off, s.nrhs, s.nrhs, s.ld, s.n, s.n, s.nrhs});
     }
   }
-  for (auto& L : h->back_sched) L.fam = K_GEMM_BACK;
+  for (auto& sc : h->back_sched)
+    for (auto& L : sc) L.fam = K_GEMM_BACK;
   for (auto* sc : {&h->fwdY_sched, &h->backY_sched, &h->fwdP_sched, &h->backP_sched, &h->adjT_fwd, &h->adjT_back})
     for (auto& L : *sc) L.fam = K_GEMM_SOLVE;
   tile_prefix(h->h_gemm, h->res_gemm, nsys, h->res_tt);
   tile_prefix(h->h_gemm, h->res_gemm_aug, nsys, h->res_aug_tt);
   if (h->adj_q_ops >= 0) tile_prefix(h->h_gemm, h->adj_q_ops, nsys, h->adj_q_tt);
-  set_flops(h, h->lu_sched);
-  set_flops(h, h->back_sched);
+  for (int part = 0; part < 2; ++part) {
+    set_flops(h, h->lu_sched[part]);
+    set_flops(h, h->back_sched[part]);
+  }
   set_flops(h, h->adjT_fwd);
   set_flops(h, h->adjT_back);
   set_flops(h, h->fwdY_sched);
@@ -1482,15 +1507,34 @@ template <typename T> static int run_lu(ghost_bor* h) {
   const int nsys = (int)h->ss.size();
   hipLaunchKernelGGL(k_zero_int, dim3((nsys + 255) / 256), dim3(256), 0, h->stream, h->d_info, nsys);
   LAUNCHCHK();
-  int rc = run_sched<T>(h, h->lu_sched, h->d_A, nullptr);
+  // the second part on stream2 between a fork and a join event (kept on one
+  // stream while the per-launch profile is on, whose events time one stream)
+  const bool split = h->nparts > 1 && !h->profile;
+  hipStream_t s0 = h->stream;
+  if (split) {
+    HIPCHK(hipEventRecord(h->ev_fork, s0));
+    HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
+  }
+  int rc = 0;
+  for (int part = 0; part < h->nparts && !rc; ++part) {
+    h->stream = (split && part == 1) ? h->stream2 : s0;
+    if ((rc = run_sched<T>(h, h->lu_sched[part], h->d_A, nullptr))) break;
+    if (h->triu_n[part] > 0) {
+      prof_begin(h);
+      hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(h->triu_n[part]), dim3(256), 0, h->stream,
+                         h->d_trsm + h->triu_ops[part], h->d_A, h->d_W);
+      prof_end(h, K_TRI_INV, 0.0);
+      if (hipGetLastError() != hipSuccess) { rc = fail(GHOST_ERR_HIP, "k_tri_inv launch failed"); break; }
+    }
+    rc = run_sched<T>(h, h->back_sched[part], h->d_A, nullptr);
+  }
+  h->stream = s0;
   if (rc) return rc;
-  prof_begin(h);
-  hipLaunchKernelGGL((k_tri_inv<T, true>), dim3(h->triu_n), dim3(256), 0, h->stream, h->d_trsm + h->triu_ops,
-                     h->d_A, h->d_W);
-  prof_end(h, K_TRI_INV, 0.0);
-
-  LAUNCHCHK();
-  return run_sched<T>(h, h->back_sched, h->d_A, nullptr);
+  if (split) {
+    HIPCHK(hipEventRecord(h->ev_join, h->stream2));
+    HIPCHK(hipStreamWaitEvent(s0, h->ev_join, 0));
+  }
+  return 0;
 }
 
 // R64 = B64 - A64 X with X in X64 (x_in_aug = 0) or in the FP64 augmented columns
