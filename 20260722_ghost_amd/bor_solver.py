@@ -536,7 +536,7 @@ class BorPecSolver:
             raise NotImplementedError("magnetic currents (IBC / PMCHWT) are not part of the GPU BoR path.")
         sol = np.asarray(sol, dtype=np.complex128).reshape(2 * self.Nn)
         mm = max(abs(int(m)), 1)
-        h = _lib.Handle(_lib.GHOST_FP64, residual=False)
+        h = _lib.Handle(_lib.GHOST_FP64, residual=False, pairing=False)   # one mode's own currents
         try:
             h.add_problem(self.pts, self.k, mm, 64, 128, 1.0, 0.0, [float(theta_s_deg)], unmasked=True)
             h.build()

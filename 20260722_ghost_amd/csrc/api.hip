@@ -1066,15 +1066,15 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
   if (h->sys.empty()) return fail(GHOST_ERR_STATE, "no problems or dense systems added");
   HIPCHK(hipSetDevice(h->device));
   const bool mixed = h->precision == GHOST_MIXED;
-  // systems to solve: every mode, or with pairing modes m >= 0 only, mode m
-  // carrying mode -m's right-hand sides as columns [nrhs, 2 nrhs)
+  // systems to solve: every mode, or with pairing modes m >= 0 only (mode -m
+  // reads mode m's right-hand sides and currents with the signs of the pair)
   h->ss.clear();
   for (auto& s : h->sys) {
     const bool paired = h->pairing && s.prob >= 0;
     s.col = 0;
     s.flip = 0;
     if (paired && s.m < 0) continue;
-    s.ncols = s.nrhs + (paired && s.m > 0 ? s.nrhs : 0);
+    s.ncols = s.nrhs;
     s.owner = (int)h->ss.size();
     SysInfo o = s;
     o.nrhs = o.ncols;
@@ -1087,7 +1087,7 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
         return fail(GHOST_ERR_STATE, "modes %d and %d have different bases", s.m, -s.m);
       s.owner = tw.owner;
       s.ncols = tw.ncols;
-      s.col = s.nrhs;
+      s.col = 0;
       s.flip = 1;
     }
   // offsets of the solved systems, shared by their mode entries
@@ -1375,11 +1375,12 @@ static int rhs_problem(ghost_bor* h, Problem& p, const Geom& tg) {
   for (int i = 0; i < p.nmodes; ++i) maxn = std::max(maxn, h->sys[p.sys0 + i].n);
   // modes fastest: the systems of one (row, column block) run together and
   // share the Bessel and phase rows of the row's points in L2
-  dim3 gr(p.nmodes, maxn, (p.ntheta + 127) / 128);
+  const int mlo = h->pairing ? p.M : 0;   // paired: modes -m share mode m's right-hand sides
+  dim3 gr(p.nmodes - mlo, maxn, (p.ntheta + 127) / 128);
   prof_begin(h);
   // mixed: B in FP64 only; the solve writes the column-scaled FP32 copy into the augmented columns
-  if (mixed) hipLaunchKernelGGL((k_rhs<double>), gr, dim3(128), 0, h->stream, ea, h->d_sys + p.sys0, nullptr, h->d_B64, 0);
-  else hipLaunchKernelGGL((k_rhs<double>), gr, dim3(128), 0, h->stream, ea, h->d_sys + p.sys0, h->d_A, h->d_B64, 0);
+  if (mixed) hipLaunchKernelGGL((k_rhs<double>), gr, dim3(128), 0, h->stream, ea, h->d_sys + p.sys0 + mlo, nullptr, h->d_B64, 0);
+  else hipLaunchKernelGGL((k_rhs<double>), gr, dim3(128), 0, h->stream, ea, h->d_sys + p.sys0 + mlo, h->d_A, h->d_B64, 0);
   prof_end(h, K_RHS, 0.0);
   LAUNCHCHK();
   return 0;
@@ -1419,9 +1420,11 @@ extern "C" int ghost_bor_set_solution(ghost_bor* h, int prob, int m, const doubl
   HIPCHK(hipStreamSynchronize(h->stream));
   std::vector<double2> row(s.nrhs);
   for (int r = 0; r < s.n; ++r) {
-    const double f = (s.flip && r >= s.nt) ? -1.0 : 1.0;   // mode -m of a pair is stored as S x
-    for (int c = 0; c < s.nrhs; ++c)
+    for (int c = 0; c < s.nrhs; ++c) {
+      // mode -m of a pair shares mode m's storage: x_m = S [x_{-m}(VV), -x_{-m}(HH)]
+      const double f = (s.flip && ((r >= s.nt) != ((c & 1) == 1))) ? -1.0 : 1.0;
       row[c] = make_double2(f * X[2 * ((size_t)r * s.nrhs + c)], f * X[2 * ((size_t)r * s.nrhs + c) + 1]);
+    }
     double2* dst = h->precision == GHOST_MIXED ? h->d_X64 + s.x_off + (size_t)r * s.ncols + s.col
                                                : (double2*)h->d_A + (size_t)s.a_off + (size_t)r * s.ld + s.n + s.col;
     HIPCHK(hipMemcpy(dst, row.data(), (size_t)s.nrhs * sizeof(double2), hipMemcpyHostToDevice));
@@ -1923,7 +1926,7 @@ extern "C" int ghost_bor_get_dF(ghost_bor* h, int prob, double* out) {
 
 // Copy one mode system out: A (n x n), B (n x nrhs), X (n x nrhs), complex interleaved, row-major.
 // which: 0 = A and B as assembled (call after fill, before solve), 1 = X (after solve).
-// Mode -m of a pair is read from mode m's storage: A = S A_m S, B and X = S (its column block).
+// Mode -m of a pair is read from mode m's storage: A = S A_m S, B and X = S [VV, -HH] of mode m's.
 extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, double* out1, double* out2) {
   if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
   if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index");
@@ -1940,7 +1943,8 @@ extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, do
     return ((double2*)rowbuf.data())[c];
   };
   auto sgn = [&](int r, int c) { return (s.flip && ((r >= s.nt) != (c >= s.nt))) ? -1.0 : 1.0; };
-  auto rsgn = [&](int r) { return (s.flip && r >= s.nt) ? -1.0 : 1.0; };
+  // right-hand sides and currents of mode -m: S [VV, -HH] of mode m's
+  auto rsgn = [&](int r, int c) { return (s.flip && ((r >= s.nt) != ((c & 1) == 1))) ? -1.0 : 1.0; };
   auto put = [](double* o, size_t i, double2 v, double f) { o[2 * i] = f * v.x; o[2 * i + 1] = f * v.y; };
   for (int r = 0; r < s.n; ++r) {
     const bool aug_row = which == 0 ? !(mixed && h->d_A64) : !mixed;
@@ -1951,7 +1955,7 @@ extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, do
       if (aug_row) {
         for (int c = 0; c < s.n; ++c) put(out1, (size_t)r * s.n + c, get(c), sgn(r, c));
         if (out2)
-          for (int c = 0; c < s.nrhs; ++c) put(out2, (size_t)r * s.nrhs + c, get(s.n + s.col + c), rsgn(r));
+          for (int c = 0; c < s.nrhs; ++c) put(out2, (size_t)r * s.nrhs + c, get(s.n + s.col + c), rsgn(r, c));
       } else {   // mixed: the FP64 copies are exact; prefer them
         HIPCHK(hipMemcpy(row64.data(), h->d_A64 + s.a64_off + (size_t)r * s.n, (size_t)s.n * sizeof(double2),
                          hipMemcpyDeviceToHost));
@@ -1959,15 +1963,15 @@ extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, do
         if (out2) {
           HIPCHK(hipMemcpy(row64.data(), h->d_B64 + s.x_off + (size_t)r * s.ncols + s.col,
                            (size_t)s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
-          for (int c = 0; c < s.nrhs; ++c) put(out2, (size_t)r * s.nrhs + c, row64[c], rsgn(r));
+          for (int c = 0; c < s.nrhs; ++c) put(out2, (size_t)r * s.nrhs + c, row64[c], rsgn(r, c));
         }
       }
     } else if (mixed) {
       HIPCHK(hipMemcpy(row64.data(), h->d_X64 + s.x_off + (size_t)r * s.ncols + s.col,
                        (size_t)s.nrhs * sizeof(double2), hipMemcpyDeviceToHost));
-      for (int c = 0; c < s.nrhs; ++c) put(out1, (size_t)r * s.nrhs + c, row64[c], rsgn(r));
+      for (int c = 0; c < s.nrhs; ++c) put(out1, (size_t)r * s.nrhs + c, row64[c], rsgn(r, c));
     } else {
-      for (int c = 0; c < s.nrhs; ++c) put(out1, (size_t)r * s.nrhs + c, get(s.n + s.col + c), rsgn(r));
+      for (int c = 0; c < s.nrhs; ++c) put(out1, (size_t)r * s.nrhs + c, get(s.n + s.col + c), rsgn(r, c));
     }
   }
   return 0;

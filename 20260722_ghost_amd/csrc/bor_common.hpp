@@ -126,9 +126,10 @@ struct PointSet {
 };
 
 // One mode system inside a batch.  Paired modes: Z_{-m} = S Z_m S with S =
-// diag(I on the J_t rows, -I on the J_phi rows), so mode -m is solved with
-// mode m's factors as extra right-hand-side columns, Z_m (S x) = S b: its
-// entry (flip = 1) points into mode m's storage at column block col.
+// diag(I on the J_t rows, -I on the J_phi rows), and the plane-wave right-hand
+// sides obey S b_{-m} = [b_m(VV), -b_m(HH)] column by column, so the currents
+// of mode -m are x_{-m} = S [x_m(VV), -x_m(HH)]: only m >= 0 is solved, and
+// the entry of mode -m (flip = 1) reads mode m's storage with those signs.
 struct SysInfo {
   int prob;      // owning problem (body x frequency)
   int m;         // azimuthal mode
@@ -138,9 +139,9 @@ struct SysInfo {
   int f0;        // first active J_phi node (rows [nt, n))
   int ld;        // leading dimension of the augmented row-major system
   int nrhs;      // this mode's right-hand sides (2 x n_theta)
-  int ncols;     // right-hand-side columns of the solved system (nrhs, or 2 nrhs for a pair)
+  int ncols;     // right-hand-side columns of the solved system
   int col;       // first of this mode's columns among them
-  int flip;      // 1: S applies (mode -m of a pair)
+  int flip;      // 1: mode -m of a pair (J_phi rows and HH columns change sign)
   int owner;     // index of the solved system (in solve order)
   long long a_off;   // element offset of the augmented system [n x ld]
   long long x_off;   // element offset of the n x ncols solution / RHS copy

@@ -168,8 +168,6 @@ __global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, v
       }
     }
   }
-  if (s.flip && phi)   // mode -m of a pair: S b, in mode m's columns [col, col + nrhs)
-    for (int pol = 0; pol < 2; ++pol) { acc[pol].x = -acc[pol].x; acc[pol].y = -acc[pol].y; }
   if (Abase) {
     Ca* A = (Ca*)Abase + s.a_off + (size_t)r * s.ld + s.n + s.col + 2 * it;
     Ca o0, o1;
@@ -208,20 +206,21 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
   size_t ldx;
   if (x_in_aug) { X = (const Cx*)Xbase + s.a_off + s.n + s.col; ldx = s.ld; }
   else { X = (const Cx*)Xbase + s.x_off + s.col; ldx = s.ncols; }
-  const double fs = s.flip ? -1.0 : 1.0;   // mode -m of a pair holds S x
+  // mode -m of a pair reads mode m's currents: x_{-m} = S [x_m(VV), -x_m(HH)]
+  const double ft = (s.flip && pol == 1) ? -1.0 : 1.0, ff = (s.flip && pol == 0) ? -1.0 : 1.0;
   const int nf = s.n - s.nt;
   const PointSet& P = a.pts;
   auto xt = [&](int node) -> double2 {
     const int r = node - s.t0;
     if (r < 0 || r >= s.nt) return make_double2(0.0, 0.0);
     const Cx v = X[(size_t)r * ldx + col];
-    return make_double2((double)v.x, (double)v.y);
+    return make_double2(ft * (double)v.x, ft * (double)v.y);
   };
   auto xf = [&](int node) -> double2 {
     const int r = node - s.f0;
     if (r < 0 || r >= nf) return make_double2(0.0, 0.0);
     const Cx v = X[(size_t)(s.nt + r) * ldx + col];
-    return make_double2(fs * (double)v.x, fs * (double)v.y);
+    return make_double2(ff * (double)v.x, ff * (double)v.y);
   };
   double2 acc = make_double2(0.0, 0.0);
   int e_prev = -1;

@@ -76,11 +76,13 @@ int ghost_bor_destroy(ghost_bor* h);
 int ghost_bor_set_residual(ghost_bor* h, int on);
 
 /* Mode pairing (default on; before build).  The reference solves every mode
- * system on its own (bor_solver.py:744-753); here Z_{-m} = S Z_m S, with S
- * the identity on the J_t rows and minus it on the J_phi rows, so mode -m is
- * solved with mode m's LU factors as extra right-hand-side columns,
- * Z_m (S x) = S b, and only m >= 0 is factored.  Accessors by (problem,
- * mode) are unchanged.  No counterpart in the reference; ghost_bor_adjoint
+ * system on its own (bor_solver.py:744-753).  Here Z_{-m} = S Z_m S, with S
+ * the identity on the J_t rows and minus it on the J_phi rows, and the
+ * plane-wave right-hand sides obey S b_{-m} = [b_m(VV), -b_m(HH)], so
+ * x_{-m} = S [x_m(VV), -x_m(HH)]: only modes m >= 0 are filled, factored and
+ * solved, and mode -m reads mode m's storage with those signs.  Accessors by
+ * (problem, mode) are unchanged; ghost_bor_set_solution of mode -m sets mode
+ * m to the mirror image.  No counterpart in the reference; ghost_bor_adjoint
  * needs a handle built with pairing off. */
 int ghost_bor_set_pairing(ghost_bor* h, int on);
 
