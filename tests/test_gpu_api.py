@@ -144,3 +144,39 @@ def test_pivot_order_matches_lapack_getrf(nb, lu_super):
             want = sla.lu_factor(A, check_finite=False)[1]
             assert np.array_equal(got, want), (make.__name__, i, int(np.argmax(got != want)))
         h.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_mode_pairing_matches_separate_mode_solves(precision):
+    """Pairing (only m >= 0 factored; mode -m from mode m's currents by the
+    signs of S [VV, -HH]) against every mode solved on its own: systems,
+    right-hand sides, currents and far field of each mode agree to rounding."""
+    L = pkg("_lib")
+    bs = pkg("bor_solver")
+    nodes = bo.sphere_nodes(0.5, 40)
+    k = 2 * math.pi * 3e8 / bs.C0
+    thetas = np.array([0.0, 25.0, 90.0, 140.0, 180.0])
+    mm = 9
+    _, ne, nbk, _ = bs.discretisation(nodes, k, thetas, mm, "cfie", "tables")
+    print(f"[case] pairing precision={precision}", flush=True)
+    out = []
+    for pairing in (True, False):
+        h = L.Handle(precision, pairing=pairing)
+        h.add_problem(nodes, k, mm, ne, nbk, 0.5, 0.5 * bs.ETA0, thetas)
+        h.build()
+        h.fill()
+        h.sync()
+        AB = [h.system(0, m) for m in range(-mm, mm + 1)]
+        h.solve(2 if precision else 0)
+        h.farfield()
+        h.sync()
+        X = [h.solution(0, m) for m in range(-mm, mm + 1)]
+        out.append((AB, X, h.dF(0)))
+        h.close()
+    (ab1, x1, f1), (ab0, x0, f0) = out
+    tol = 1e-12 if precision == 0 else 1e-9
+    for m, (a, b, xa, xb) in enumerate(zip(ab1, ab0, x1, x0)):
+        assert np.max(np.abs(a[0] - b[0])) <= 1e-13 * np.max(np.abs(b[0])), m - mm
+        assert np.max(np.abs(a[1] - b[1])) <= 1e-13 * np.max(np.abs(b[1])), m - mm
+        assert np.max(np.abs(xa - xb)) <= tol * np.max(np.abs(xb)), m - mm
+    assert np.max(np.abs(f1 - f0)) <= tol * np.max(np.abs(f0))
