@@ -1743,21 +1743,28 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
     ExciteArgs ea = excite_args(p, base_geom(p));
     const int P = p.P;
     const int nch = (P + h->ff_chunk - 1) / h->ff_chunk;
-    dim3 g((p.nrhs + 255) / 256, p.nmodes, nch);
+    // paired modes: only m >= 0 is summed, mode -m's far field is mode m's
+    const int mlo = h->pairing ? p.M : 0, nv = p.nmodes - mlo;
+    dim3 g((p.nrhs + 255) / 256, nv, nch);
     // partial layout [chunk][system-in-problem][nrhs], reduced into this problem's dF block
     prof_begin(h);
     if (mixed)
-      hipLaunchKernelGGL((k_farfield_partial<double>), g, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0,
-                         (const void*)h->d_X64, 0, h->ff_chunk, h->d_partial, p.nmodes);
+      hipLaunchKernelGGL((k_farfield_partial<double>), g, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0 + mlo,
+                         (const void*)h->d_X64, 0, h->ff_chunk, h->d_partial, nv);
     else
-      hipLaunchKernelGGL((k_farfield_partial<double>), g, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0,
-                         (const void*)h->d_A, 1, h->ff_chunk, h->d_partial, p.nmodes);
+      hipLaunchKernelGGL((k_farfield_partial<double>), g, dim3(256), 0, h->stream, ea, h->d_sys + p.sys0 + mlo,
+                         (const void*)h->d_A, 1, h->ff_chunk, h->d_partial, nv);
     prof_end(h, K_FARFIELD, 0.0);
     LAUNCHCHK();
-    const int tot = p.nmodes * p.nrhs;
+    const int tot = nv * p.nrhs;
     hipLaunchKernelGGL(k_farfield_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream, h->d_partial, nch,
-                       p.nmodes, p.nrhs, p.k, h->d_dF + p.dF_off);
+                       nv, p.nrhs, p.k, h->d_dF + p.dF_off + (size_t)mlo * p.nrhs);
     LAUNCHCHK();
+    if (mlo > 0) {
+      hipLaunchKernelGGL(k_mirror_dF, dim3((p.M * p.nrhs + 255) / 256), dim3(256), 0, h->stream,
+                         h->d_dF + p.dF_off, p.M, p.nrhs);
+      LAUNCHCHK();
+    }
   }
   (void)nsys;
   if (!h->capturing) {

@@ -258,6 +258,15 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
   partial[((size_t)pc * nsys + si) * s.nrhs + col] = acc;
 }
 
+// Mode pairs: the far field of mode -m equals mode m's (both polarizations),
+// dF [2M+1][nrhs] with mode m at row M + m.
+__global__ void k_mirror_dF(double2* dF, int M, int nrhs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= M * nrhs) return;
+  const int m = 1 + t / nrhs, c = t % nrhs;
+  dF[(size_t)(M - m) * nrhs + c] = dF[(size_t)(M + m) * nrhs + c];
+}
+
 // Sum the chunk partials in chunk order and apply -j k eta0 / (4 pi).
 __global__ void k_farfield_reduce(const double2* partial, int nchunk, int nsys, int nrhs, double k,
                                   double2* dF) {
