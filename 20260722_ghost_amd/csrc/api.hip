@@ -252,9 +252,11 @@ struct ghost_bor {
   VecOp* d_vec = nullptr;
   // LU + back substitution in up to two interleaved parts of the systems (even
   // / odd solve index), run on two streams so that one part's latency-bound
-  // panel phases overlap the other part's trailing updates
+  // panel phases overlap the other part's trailing updates; schedule 2 covers
+  // every system in one launch per step (used while the per-launch profile is
+  // on, so that each launch is timed over the whole batch)
   int nparts = 1;
-  std::vector<Launch> lu_sched[2], back_sched[2];
+  std::vector<Launch> lu_sched[3], back_sched[3];
   std::vector<Launch> fwdY_sched, backY_sched, fwdP_sched, backP_sched;
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -271,7 +273,7 @@ struct ghost_bor {
   double2* d_ffpart = nullptr;       // per-element far-field partial sums of one problem
   double2* d_ffs = nullptr;          // per-element contraction with a'
   int* d_rowoff = nullptr;
-  int triu_ops[2] = {0, 0}, triu_n[2] = {0, 0};
+  int triu_ops[3] = {0, 0, 0}, triu_n[3] = {0, 0, 0};
   int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
   int res_tt[3] = {0, 0, 0}, res_aug_tt[3] = {0, 0, 0}, adj_q_tt[3] = {0, 0, 0};
   int vec_all = 0, vec_aug = 0;          // VecOps over all systems: x-blocks / augmented RHS -> x-blocks
@@ -779,9 +781,10 @@ static int build_schedule(ghost_bor* h) {
   // written once per S panels instead of once per panel.
   const int S = std::max(1, h->lu_super);
   h->nparts = nsys >= 2 ? 2 : 1;
-  for (int part = 0; part < h->nparts; ++part) {
+  for (int part = 0; part < 3; ++part) {
+  if (part == 1 && h->nparts < 2) continue;
   std::vector<int> psys;
-  for (int si = part; si < nsys; si += h->nparts) psys.push_back(si);
+  for (int si = part == 2 ? 0 : part; si < nsys; si += part == 2 ? 1 : h->nparts) psys.push_back(si);
   const int np = (int)psys.size();
   auto lu_gemm = [&](int kind, auto make) {
     Launch g{kind, (int)h->h_gemm.size(), np, 0, 0};
@@ -1034,7 +1037,7 @@ static int build_schedule(ghost_bor* h) {
   tile_prefix(h->h_gemm, h->res_gemm, nsys, h->res_tt);
   tile_prefix(h->h_gemm, h->res_gemm_aug, nsys, h->res_aug_tt);
   if (h->adj_q_ops >= 0) tile_prefix(h->h_gemm, h->adj_q_ops, nsys, h->adj_q_tt);
-  for (int part = 0; part < 2; ++part) {
+  for (int part = 0; part < 3; ++part) {
     set_flops(h, h->lu_sched[part]);
     set_flops(h, h->back_sched[part]);
   }
@@ -1520,8 +1523,9 @@ template <typename T> static int run_lu(ghost_bor* h) {
   const int nsys = (int)h->ss.size();
   hipLaunchKernelGGL(k_zero_int, dim3((nsys + 255) / 256), dim3(256), 0, h->stream, h->d_info, nsys);
   LAUNCHCHK();
-  // the second part on stream2 between a fork and a join event (kept on one
-  // stream while the per-launch profile is on, whose events time one stream)
+  // the second part on stream2 between a fork and a join event; while the
+  // per-launch profile is on (its events time one stream) the whole batch
+  // runs as schedule 2 on one stream
   const bool split = h->nparts > 1 && !h->profile;
   hipStream_t s0 = h->stream;
   if (split) {
@@ -1529,7 +1533,8 @@ template <typename T> static int run_lu(ghost_bor* h) {
     HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_fork, 0));
   }
   int rc = 0;
-  for (int part = 0; part < h->nparts && !rc; ++part) {
+  const int p_lo = split ? 0 : 2, p_hi = split ? 2 : 3;
+  for (int part = p_lo; part < p_hi && !rc; ++part) {
     h->stream = (split && part == 1) ? h->stream2 : s0;
     if ((rc = run_sched<T>(h, h->lu_sched[part], h->d_A, nullptr))) break;
     if (h->triu_n[part] > 0) {
