@@ -38,6 +38,7 @@ SIGNATURES = [
     ("ghost_bor_destroy", _ci, [_vp]),
     ("ghost_bor_set_residual", _ci, [_vp, _ci]),
     ("ghost_bor_set_pairing", _ci, [_vp, _ci]),
+    ("ghost_bor_set_lu_streams", _ci, [_vp, _ci]),
     ("ghost_bor_add_problem", _ci, [_vp, _dp, _ci, _cd, _ci, _ci, _ci, _cd, _cd, _dp, _ci, _ci, _ip]),
     ("ghost_bor_build", _ci, [_vp]),
     ("ghost_bor_fill", _ci, [_vp]),
@@ -216,6 +217,9 @@ class Handle:
 
     def set_lu_super(self, panels: int):
         check(self.lib.ghost_bor_set_lu_super(self._h, int(panels)))
+
+    def set_lu_streams(self, n: int):
+        check(self.lib.ghost_bor_set_lu_streams(self._h, int(n)))
 
     def set_gemm_variant(self, v: int):
         check(self.lib.ghost_bor_set_gemm_variant(self._h, int(v)))

@@ -256,6 +256,7 @@ struct ghost_bor {
   // every system in one launch per step (used while the per-launch profile is
   // on, so that each launch is timed over the whole batch)
   int nparts = 1;
+  int lu_streams = 2;                // 1: the whole-batch schedule on one stream
   std::vector<Launch> lu_sched[3], back_sched[3];
   std::vector<Launch> fwdY_sched, backY_sched, fwdP_sched, backP_sched;
   hipStream_t stream2 = nullptr;
@@ -1517,7 +1518,7 @@ template <typename T> static int run_lu(ghost_bor* h) {
   // the second part on stream2 between a fork and a join event; while the
   // per-launch profile is on (its events time one stream) the whole batch
   // runs as schedule 2 on one stream
-  const bool split = h->nparts > 1 && !h->profile;
+  const bool split = h->nparts > 1 && h->lu_streams > 1 && !h->profile;
   hipStream_t s0 = h->stream;
   if (split) {
     HIPCHK(hipEventRecord(h->ev_fork, s0));
@@ -1721,6 +1722,13 @@ extern "C" int ghost_bor_set_gemm_variant(ghost_bor* h, int variant) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
   if (variant < -1 || variant > 4) return fail(GHOST_ERR_ARG, "GEMM variant must be -1..4");
   h->gemm_variant = variant;
+  return 0;
+}
+
+extern "C" int ghost_bor_set_lu_streams(ghost_bor* h, int n) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (n != 1 && n != 2) return fail(GHOST_ERR_ARG, "LU streams must be 1 or 2");
+  h->lu_streams = n;
   return 0;
 }
 

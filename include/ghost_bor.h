@@ -121,6 +121,10 @@ int ghost_bor_set_gemm_variant(ghost_bor* h, int variant);
  * runs once per super-block with K = nb x panels, the panels and row blocks
  * inside it catch up through narrow GEMMs.  1 = plain right-looking LU. */
 int ghost_bor_set_lu_super(ghost_bor* h, int panels);
+/* LU and back substitution on 2 streams (default: two interleaved halves of
+ * the systems, one half's panel phases beside the other's trailing updates)
+ * or 1 (the whole batch in one schedule).  No counterpart in the reference. */
+int ghost_bor_set_lu_streams(ghost_bor* h, int n);
 /* Probe tolerance of the adaptive refinement (default 1e-5). */
 int ghost_bor_set_probe_tol(ghost_bor* h, double tol);
 /* Probe errors of the last adaptive solve, one per probe; *n: capacity in, count out. */

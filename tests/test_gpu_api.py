@@ -180,3 +180,33 @@ def test_mode_pairing_matches_separate_mode_solves(precision):
         assert np.max(np.abs(a[1] - b[1])) <= 1e-13 * np.max(np.abs(b[1])), m - mm
         assert np.max(np.abs(xa - xb)) <= tol * np.max(np.abs(xb)), m - mm
     assert np.max(np.abs(f1 - f0)) <= tol * np.max(np.abs(f0))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_two_stream_lu_equals_one_stream(precision):
+    """The two interleaved halves of the systems on two streams run the same
+    per-system operations in the same order as the whole-batch schedule on
+    one stream: the currents are bitwise equal, also on uneven batches
+    (two frequencies, so the halves hold systems of different sizes)."""
+    L = pkg("_lib")
+    bs = pkg("bor_solver")
+    nodes = bo.sphere_nodes(0.5, 30)
+    thetas = np.array([0.0, 60.0, 120.0])
+    print(f"[case] lu streams precision={precision}", flush=True)
+    h = L.Handle(precision)
+    for f in (3e8, 4.5e8):
+        k = 2 * math.pi * f / bs.C0
+        _, ne, nbk, _ = bs.discretisation(nodes, k, thetas, 7, "cfie", "tables")
+        h.add_problem(nodes, k, 7, ne, nbk, 0.5, 0.5 * bs.ETA0, thetas)
+    h.build()
+    out = []
+    for n in (2, 1):
+        h.set_lu_streams(n)
+        h.fill()
+        h.solve(1 if precision else 0)
+        h.farfield()
+        h.sync()
+        out.append([h.solution(p, m) for p in range(2) for m in range(-7, 8)])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    h.close()
