@@ -1280,9 +1280,16 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   prof_end(h, K_ZERO, 0.0);
 
   LAUNCHCHK();
-  // near stage 1 + 2 (FP64).  (Measured: on a second stream beside the far
-  // fill it gains nothing, both kernels keep every SIMD busy.)
-  hipStream_t ns = h->stream;
+  // near stage 1 + 2 (FP64) on stream2, beside the far fill (disjoint outputs;
+  // joined before the assembly): both keep every SIMD busy on a large problem,
+  // but a batch of small problems fills each other's launch tails; one stream
+  // while the per-launch profile is on
+  const bool split = !h->profile;
+  hipStream_t ns = split ? h->stream2 : h->stream;
+  if (split) {
+    HIPCHK(hipEventRecord(h->ev_fork, h->stream));
+    HIPCHK(hipStreamWaitEvent(ns, h->ev_fork, 0));
+  }
   NearArgs na;
   na.nodes = tg.nodes;
   na.L = tg.elem;
@@ -1340,6 +1347,10 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   LAUNCHCHK();
   // far
   int rc = mixed ? far_fill<float>(h, p, tg, sg) : far_fill<double>(h, p, tg, sg);
+  if (split) {
+    HIPCHK(hipEventRecord(h->ev_join, ns));
+    HIPCHK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
+  }
   if (rc) return rc;
 
   // assemble
