@@ -210,3 +210,37 @@ def test_two_stream_lu_equals_one_stream(precision):
     for a, b in zip(*out):
         assert np.array_equal(a, b)
     h.close()
+
+
+def test_pairing_with_dense_systems_in_the_batch():
+    """Paired mode systems and caller-filled dense systems in one batch: the
+    dense solve is np.linalg.solve's, the mode currents equal an unpaired
+    handle's, and the dense system keeps its own pivots and zero-pivot count."""
+    L = pkg("_lib")
+    bs = pkg("bor_solver")
+    rng = np.random.default_rng(3)
+    nodes = bo.sphere_nodes(0.5, 24)
+    k = 2 * math.pi * 3e8 / bs.C0
+    thetas = np.array([10.0, 80.0])
+    _, ne, nbk, _ = bs.discretisation(nodes, k, thetas, 5, "cfie", "tables")
+    n, r = 150, 7
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 4 * math.sqrt(n) * np.eye(n)
+    B = rng.standard_normal((n, r)) + 1j * rng.standard_normal((n, r))
+    print("[case] pairing with a dense system", flush=True)
+    X = []
+    for pairing in (True, False):
+        h = L.Handle(L.GHOST_FP64, pairing=pairing)
+        h.add_problem(nodes, k, 5, ne, nbk, 0.5, 0.5 * bs.ETA0, thetas)
+        d = h.add_dense(n, r)
+        h.build()
+        h.fill()
+        h.set_dense(d, A, B)
+        h.solve(0)
+        h.sync()
+        Xd = h.get_dense(d)
+        assert np.max(np.abs(Xd - np.linalg.solve(A, B))) <= 1e-12 * np.max(np.abs(Xd))
+        assert not np.any(h.zero_pivots())
+        X.append([h.solution(0, m) for m in range(-5, 6)])
+        h.close()
+    for a, b in zip(*X):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
