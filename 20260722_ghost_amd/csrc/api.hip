@@ -258,7 +258,7 @@ struct ghost_bor {
   int nparts = 1;
   int lu_streams = 2;                // 1: the whole-batch schedule on one stream
   std::vector<Launch> lu_sched[3], back_sched[3];
-  std::vector<Launch> fwdY_sched, backY_sched, fwdP_sched, backP_sched;
+  std::vector<Launch> fwdY_sched, backY_sched;
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // adjoint (FP64 handles): A^T lambda = G through the LU factors, Q = -Lambda X^T
@@ -285,6 +285,7 @@ struct ghost_bor {
   double* d_rscale = nullptr;
   double* d_pscale = nullptr;
   float2* d_P32 = nullptr;               // FP32 probe corrections, n x 2 per system at p_off
+  TrsvOp* d_trsv = nullptr;              // the probe's two-column solves, one op per system
   int rpt = 1;
   // error probe (adaptive refinement): two fixed random combinations of the columns
   int probe_ops = -1;                // 3 GemmOps per system: BS = B S, XS = X S, RP = BS - A XS
@@ -966,12 +967,11 @@ static int build_schedule(ghost_bor* h) {
       const SysInfo& s = h->ss[si];
       h->h_gemm.push_back(GemmOp{s.a64_off, h->p_off[si], h->p_off[si], s.n, 2, 2, s.n, 2, s.n});
     }
-    // forward / backward solves with the FP32 factors on separate RHS blocks:
-    // Y (n x nrhs at x_off, the refinement corrections) and P (n x 2 at p_off, the probe)
+    // forward / backward solves with the FP32 factors on the refinement
+    // corrections Y (n x nrhs at x_off); the probe's n x 2 blocks at p_off
+    // take k_trsv2 instead
     add_tri_solves(h->fwdY_sched, h->backY_sched, [&](int si) { return h->ss[si].x_off; },
                    [&](int si) { return h->ss[si].nrhs; });
-    add_tri_solves(h->fwdP_sched, h->backP_sched, [&](int si) { return h->p_off[si]; },
-                   [&](int si) { (void)si; return 2; });
   }
   if (!mixed) {
     // transposed solves on the lambda block (n x nrhs at x_off): with P A = L U,
@@ -1033,7 +1033,7 @@ static int build_schedule(ghost_bor* h) {
   }
   for (auto& sc : h->back_sched)
     for (auto& L : sc) L.fam = K_GEMM_BACK;
-  for (auto* sc : {&h->fwdY_sched, &h->backY_sched, &h->fwdP_sched, &h->backP_sched, &h->adjT_fwd, &h->adjT_back})
+  for (auto* sc : {&h->fwdY_sched, &h->backY_sched, &h->adjT_fwd, &h->adjT_back})
     for (auto& L : *sc) L.fam = K_GEMM_SOLVE;
   tile_prefix(h->h_gemm, h->res_gemm, nsys, h->res_tt);
   tile_prefix(h->h_gemm, h->res_gemm_aug, nsys, h->res_aug_tt);
@@ -1046,8 +1046,6 @@ static int build_schedule(ghost_bor* h) {
   set_flops(h, h->adjT_back);
   set_flops(h, h->fwdY_sched);
   set_flops(h, h->backY_sched);
-  set_flops(h, h->fwdP_sched);
-  set_flops(h, h->backP_sched);
   for (size_t i = 0; i < h->h_vec.size(); ++i) h->h_vec[i].piv = pofs[i % nsys];
   h->piv_elems = poff;
   int rc;
@@ -1137,6 +1135,13 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
     if ((rc = dalloc(h, &h->d_BS, pe * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_XS, pe * sizeof(double2)))) return rc;
     if ((rc = dalloc(h, &h->d_P32, pe * sizeof(float2)))) return rc;
+    std::vector<TrsvOp> tv;
+    for (size_t si = 0; si < h->ss.size(); ++si) {
+      const SysInfo& s = h->ss[si];
+      tv.push_back(TrsvOp{s.a_off, (long long)si * h->max_blocks * 2 * 64 * 64, h->p_off[si], s.ld, s.n});
+    }
+    if ((rc = dalloc(h, &h->d_trsv, tv.size() * sizeof(TrsvOp)))) return rc;
+    HIPCHK(hipMemcpy(h->d_trsv, tv.data(), tv.size() * sizeof(TrsvOp), hipMemcpyHostToDevice));
     size_t ncol = 0;
     for (auto& s : h->ss) ncol += (size_t)s.nrhs;
     if ((rc = dalloc(h, &h->d_bscale, ncol * sizeof(double)))) return rc;
@@ -1646,8 +1651,13 @@ static int probe_error(ghost_bor* h, std::vector<double>& eta) {
                      h->d_perm, h->d_BS, h->d_pscale, h->d_P32);
   LAUNCHCHK();
   int rc;
-  if ((rc = run_sched<float>(h, h->fwdP_sched, h->d_A, h->d_P32))) return rc;
-  if ((rc = run_sched<float>(h, h->backP_sched, h->d_A, h->d_P32))) return rc;
+  prof_begin(h);
+  hipLaunchKernelGGL(k_trsv2<false>, dim3(nsys), dim3(1024), 0, h->stream, h->d_trsv, (const float2*)h->d_A,
+                     (const float2*)h->d_W, h->d_P32);
+  hipLaunchKernelGGL(k_trsv2<true>, dim3(nsys), dim3(1024), 0, h->stream, h->d_trsv, (const float2*)h->d_A,
+                     (const float2*)h->d_W, h->d_P32);
+  prof_end(h, K_GEMM_SOLVE, 0.0);
+  LAUNCHCHK();
   // ratios on the device: only nsys doubles come back to decide the sweep
   hipLaunchKernelGGL(k_probe_eta, dim3(nsys), dim3(256), 0, h->stream, h->d_vec + h->vec_probe, h->d_P32, h->d_XS,
                      h->d_pscale, h->d_red);

@@ -356,6 +356,102 @@ __global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, int nops, cons
       }
 }
 
+// Two-column triangular solves of the error probe with the FP32 factors, one
+// 1024-thread work-group per system (in place on the n x 2 block at x):
+// forward (UPPER = false) through the unit lower factor, y_b = T^-1 y_b with
+// W_l = I - T^-1, then the rows below take y_b's contribution; backward
+// (UPPER = true) the same with U and the rows above.  The same algebra as the
+// blocked GEMM schedule of the wide solves, without its ~150 launches of a
+// few microseconds each: every row of the factor is streamed once per sweep.
+struct TrsvOp {
+  long long a;       // element offset of the system's factor
+  long long w;       // element offset of the system's first W slot (block b at w + 2 * 4096 b)
+  long long x;       // element offset of the n x 2 block
+  int ld, n;
+};
+
+template <bool UPPER>
+__global__ __launch_bounds__(1024) void k_trsv2(const TrsvOp* ops, const float2* F, const float2* Wbuf,
+                                                float2* X) {
+  const TrsvOp op = ops[blockIdx.x];
+  const int n = op.n, tid = threadIdx.x;
+  if (n <= 0) return;
+  const int nblk = (n + 63) / 64;
+  __shared__ float2 Wt[64][65];
+  __shared__ float2 yb[64][2];
+  float2* x = X + op.x;
+  for (int s = 0; s < nblk; ++s) {
+    const int b = UPPER ? nblk - 1 - s : s;
+    const int k = 64 * b, w = min(64, n - k);
+    const float2* W = Wbuf + op.w + (size_t)b * 2 * 4096 + (UPPER ? 4096 : 0);
+    for (int i = tid; i < 4096; i += 1024) Wt[i >> 6][i & 63] = W[i];
+    if (tid < 128) {
+      const int r = tid >> 1, c = tid & 1;
+      yb[r][c] = r < w ? x[(size_t)(k + r) * 2 + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    // y_b = y_b - W y_b (W row-major 64 x 64; its padding beyond w is not read)
+    float2 v = make_float2(0.f, 0.f);
+    const int r = tid >> 1, c = tid & 1;
+    if (tid < 128 && r < w) {
+      v = yb[r][c];
+      for (int q = 0; q < w; ++q) {
+        const float2 a = Wt[r][q], y = yb[q][c];
+        v.x -= a.x * y.x - a.y * y.y;
+        v.y -= a.x * y.y + a.y * y.x;
+      }
+    }
+    __syncthreads();
+    if (tid < 128 && r < w) {
+      yb[r][c] = v;
+      x[(size_t)(k + r) * 2 + c] = v;
+    }
+    __syncthreads();
+    // the other rows: below the block (forward) or above it (backward).  A
+    // 16-lane group per row, 4 columns per lane (two 16-byte loads), so a
+    // wavefront reads 4 rows x 512 contiguous bytes; the group's partial dot
+    // products are summed by row rotations.
+    const int lo = UPPER ? 0 : k + w, hi = UPPER ? k : n;
+    const int lane = tid & 63, wv = tid >> 6, sub = lane >> 4, ql = (lane & 15) * 4;
+    for (int row0 = lo + 4 * wv; row0 < hi; row0 += 64) {
+      const int row = row0 + sub;
+      float p0r = 0.f, p0i = 0.f, p1r = 0.f, p1i = 0.f;
+      if (row < hi) {
+        const float2* fr = F + op.a + (size_t)row * op.ld + k + ql;
+        float2 l[4];
+        if (ql + 4 <= w) {
+          const float4 u0 = *(const float4*)fr, u1 = *(const float4*)(fr + 2);
+          l[0] = make_float2(u0.x, u0.y); l[1] = make_float2(u0.z, u0.w);
+          l[2] = make_float2(u1.x, u1.y); l[3] = make_float2(u1.z, u1.w);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) l[j] = ql + j < w ? fr[j] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int q = min(ql + j, 63);   // l[j] = 0 past w
+          const float2 y0 = yb[q][0], y1 = yb[q][1];
+          p0r += l[j].x * y0.x - l[j].y * y0.y;
+          p0i += l[j].x * y0.y + l[j].y * y0.x;
+          p1r += l[j].x * y1.x - l[j].y * y1.y;
+          p1i += l[j].x * y1.y + l[j].y * y1.x;
+        }
+      }
+      p0r += row_ror<8>(p0r); p0i += row_ror<8>(p0i); p1r += row_ror<8>(p1r); p1i += row_ror<8>(p1i);
+      p0r += row_ror<4>(p0r); p0i += row_ror<4>(p0i); p1r += row_ror<4>(p1r); p1i += row_ror<4>(p1i);
+      p0r += row_ror<2>(p0r); p0i += row_ror<2>(p0i); p1r += row_ror<2>(p1r); p1i += row_ror<2>(p1i);
+      p0r += row_ror<1>(p0r); p0i += row_ror<1>(p0i); p1r += row_ror<1>(p1r); p1i += row_ror<1>(p1i);
+      if ((lane & 15) == 0 && row < hi) {
+        float2* xr = x + (size_t)row * 2;
+        const float2 a0 = xr[0], a1 = xr[1];
+        xr[0] = make_float2(a0.x - p0r, a0.y - p0i);
+        xr[1] = make_float2(a1.x - p1r, a1.y - p1i);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ─────────────────────────────────────────────────────────────────────────────
 // panel factorisation: one 1024-thread block per system, SW-column sub-panels
 // held in registers (rows r0 + tid + 1024 i), pivots by max |Re| + |Im|
