@@ -198,12 +198,12 @@ struct Launch {  // one batched kernel launch of the solve schedule
 // kernel families timed by the optional per-launch HIP-event profile
 enum {
   K_ZERO, K_NEAR_MODAL, K_NEAR_BLOCKS, K_FAR_FILL, K_ASSEMBLE, K_BESSEL, K_RHS, K_PANEL, K_SWAP,
-  K_TRI_INV, K_GEMM, K_GEMM64, K_VEC, K_FARFIELD, K_GEMM_BACK, K_GEMM_SOLVE, K_COUNT
+  K_TRI_INV, K_GEMM, K_GEMM64, K_VEC, K_FARFIELD, K_GEMM_BACK, K_GEMM_SOLVE, K_TRSV, K_COUNT
 };
 static const char* kKernelNames[K_COUNT] = {"zero", "near_modal", "near_blocks", "far_fill", "assemble",
                                             "bessel", "rhs", "lu_panel", "lu_swap", "tri_inv", "gemm",
                                             "gemm_fp64_residual", "vec", "farfield", "gemm_back",
-                                            "gemm_solve"};
+                                            "gemm_solve", "probe_solve"};
 
 enum { L_PANEL, L_SWAP, L_TRIL, L_TRIU, L_GEMM_AUG, L_GEMM_AUG_W, L_GEMM_Y, L_GEMM_Y_W, L_GEMM_Y_LU,
        L_GEMM_TW, L_GEMM_TLU };   // transposed-operand solves (adjoint): W^T and factor-block^T
@@ -1656,7 +1656,7 @@ static int probe_error(ghost_bor* h, std::vector<double>& eta) {
                      (const float2*)h->d_W, h->d_P32);
   hipLaunchKernelGGL(k_trsv2<true>, dim3(nsys), dim3(1024), 0, h->stream, h->d_trsv, (const float2*)h->d_A,
                      (const float2*)h->d_W, h->d_P32);
-  prof_end(h, K_GEMM_SOLVE, 0.0);
+  prof_end(h, K_TRSV, 0.0);
   LAUNCHCHK();
   // ratios on the device: only nsys doubles come back to decide the sweep
   hipLaunchKernelGGL(k_probe_eta, dim3(nsys), dim3(256), 0, h->stream, h->d_vec + h->vec_probe, h->d_P32, h->d_XS,

@@ -359,7 +359,7 @@ def main():
 
     # dominant kernel of the headline path, roofline from live HIP-event timing:
     # the GEMM kernel over all its uses in the step (LU updates, back
-    # substitution, probe solves), the FP64 residual kernel aside
+    # substitution, refinement solves), the FP64 residual kernel aside
     gl = [v for kk, v in kt_mx.items() if kk.startswith("gemm") and kk != "gemm_fp64_residual"]
     gemm_ms, gemm_n, gemm_fl = (sum(v[0] for v in gl), sum(v[1] for v in gl), sum(v[2] for v in gl))
     merged = {kk: v for kk, v in kt_mx.items() if not (kk.startswith("gemm") and kk != "gemm_fp64_residual")}
