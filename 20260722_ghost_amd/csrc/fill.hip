@@ -6,8 +6,10 @@
 // folded samples into every azimuthal order at once with a cosine/sine
 // transform run as a 16x16x4 MFMA contraction against a twiddle table.  The
 // 16 point pairs of the element pair are contracted to the 2x2 nodal blocks
-// in registers (DPP/shuffle reductions) and written as complex Z entries for
-// modes +m and -m into a mode-innermost buffer Zfar[2Nn][2Nn][2M+1].
+// in registers (DPP/shuffle reductions) and written as complex Z entries into
+// a mode-innermost buffer: Zfar[2Nn][2Nn][M+1] for modes 0..M when the modes
+// are paired (mode -m is S Z_m S, S negating the J_phi rows and columns), or
+// Zfar[2Nn][2Nn][2M+1] for modes -M..M.
 //
 // Reference algorithm: bor_kernels.py:132-167 (G_m by FFT), 273-331 (MFIE
 // brackets), bor_streaming.py:320-414 (nodal contraction, 9 EFIE order
@@ -38,7 +40,7 @@ struct FarArgs {
   int neg;               // also write modes -n (off with pairing: they are never stored)
   int zmodes, zoff;      // Z mode stride and the index of mode 0 (2M+1, M; or M+1, 0)
   double k, ce_re, ce_im, cm;
-  void* Z;               // Zfar [2Nn][2Nn][2M+1] complex<R>
+  void* Z;               // Zfar [2Nn][2Nn][zmodes] complex<R>
 };
 
 // Reduce-scatter step between two lanes of a DPP row whose lane bit differs:
