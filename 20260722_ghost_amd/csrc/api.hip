@@ -1230,7 +1230,19 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
       fa.pe = pe;
       fa.pf = pf;
       fa.nfp = nfp;
-      fa.npairs = nep * nfp;
+      if (split) {   // ordered pairs (e, f)
+        fa.npairs = nep * nfp;
+        fa.delta = 0;
+        fa.imax = 0;
+      } else {
+        // unordered pairs e < f with f - e >= 2: each wavefront also fills
+        // (f, e), from the same order samples for the EFIE part
+        fa.delta = pf >= pe ? 1 : 2;
+        fa.imax = std::min(nep, std::max(0, nfp - fa.delta));
+        const long long base = nfp - fa.delta, im = fa.imax;
+        fa.npairs = (int)(im * base - im * (im - 1) / 2);
+        if (fa.npairs <= 0) continue;
+      }
       const int wpb = far_waves((int)sizeof(R), NT);
       const int grid = (fa.npairs + wpb - 1) / wpb;
       const size_t tab_bytes = sizeof(R) * (size_t)(p.nE / 2 + 1 + p.nE + 2 * (p.nB / 2 + 1) + 2 * p.nB);

@@ -38,6 +38,7 @@ struct FarArgs {
   const double* sinB;    // [nB]
   int Ne, Nn, M, nE, nB, pe, pf, nfp, npairs;
   int neg;               // also write modes -n (off with pairing: they are never stored)
+  int delta, imax;       // unordered pairs (not SPLIT): f = pf + 2 j with j >= i + delta, rows i < imax
   int zmodes, zoff;      // Z mode stride and the index of mode 0 (2M+1, M; or M+1, 0)
   double k, ce_re, ce_im, cm;
   void* Z;               // Zfar [2Nn][2Nn][zmodes] complex<R>
@@ -50,6 +51,42 @@ template <int CTRL, typename R>
 __device__ __forceinline__ R rs_step(R lo, R hi, bool b) {
   const R keep = b ? hi : lo, send = b ? lo : hi;
   return keep + dpp_mov<CTRL>(send);
+}
+
+// 4 x 4 transpose of (lane group kk = lane / 16) x (accumulator element r) of
+// a 16x16x4 MFMA result: the entry of lane group kk, element r moves to lane
+// group r, element kk.  Two butterflies: the off-diagonal 2 x 2 blocks across
+// wavefront halves (v_permlane32_swap), then inside each block across row
+// pairs (v_permlane16_swap); it is its own inverse.
+__device__ __forceinline__ void perm_swap32(float& x, float& y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(y), false, false);
+  x = __int_as_float(r[0]);
+  y = __int_as_float(r[1]);
+}
+__device__ __forceinline__ void perm_swap16(float& x, float& y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(x), __float_as_int(y), false, false);
+  x = __int_as_float(r[0]);
+  y = __int_as_float(r[1]);
+}
+__device__ __forceinline__ void perm_swap32(double& x, double& y) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
+  x = __hiloint2double(b[0], a[0]);
+  y = __hiloint2double(b[1], a[1]);
+}
+__device__ __forceinline__ void perm_swap16(double& x, double& y) {
+  const auto a = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+  x = __hiloint2double(b[0], a[0]);
+  y = __hiloint2double(b[1], a[1]);
+}
+template <typename R, typename V4> __device__ __forceinline__ void transpose_lane_groups(V4& v) {
+  R x0 = v[0], x1 = v[1], x2 = v[2], x3 = v[3];
+  perm_swap32(x0, x2);
+  perm_swap32(x1, x3);
+  perm_swap16(x0, x1);
+  perm_swap16(x2, x3);
+  v[0] = x0; v[1] = x1; v[2] = x2; v[3] = x3;
 }
 
 // Wavefronts (element pairs) per work-group of k_far_fill: they share one copy
@@ -94,9 +131,24 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const int wid = blockIdx.x * WPB + wl;
   if (wid >= a.npairs) return;  // wave-uniform exit (no block barriers below)
-  const int e = a.pe + 2 * (wid / a.nfp);
-  const int f = a.pf + 2 * (wid % a.nfp);
-  if (e >= a.Ne || f >= a.Ne || abs(e - f) <= 1) return;
+  int e, f;
+  if (SPLIT) {   // ordered pairs (e, f) of the parity class
+    e = a.pe + 2 * (wid / a.nfp);
+    f = a.pf + 2 * (wid % a.nfp);
+    if (e >= a.Ne || f >= a.Ne || abs(e - f) <= 1) return;
+  } else {
+    // unordered pairs e < f of the class, row i = (e - pe)/2 holding
+    // j = (f - pf)/2 from i + delta to nfp - 1: S(i) pairs before row i
+    const long long base = a.nfp - a.delta;
+    auto S = [&](long long i) { return i * base - i * (i - 1) / 2; };
+    int lo = 0, hi = a.imax;   // S(lo) <= wid < S(hi)
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (S(mid) <= wid) lo = mid; else hi = mid;
+    }
+    e = a.pe + 2 * lo;
+    f = a.pf + 2 * (lo + a.delta + (int)(wid - S(lo)));
+  }
 
   const int c16 = lane & 15, kk = lane >> 4;
   const R k = (R)a.k;
@@ -112,6 +164,50 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
   const R dz = zp - zq, dr = rp - rq;
   const R d2 = dr * dr + dz * dz, rr4 = (R)4 * rp * rq;
   const int uvsub = kk >> 1, asel = kk & 1;
+
+  // ── phase 4: Z += ce * EFIE (the stash) + cm * K (accE, accO) of the pair
+  //    (test te, source se) for modes +n (and -n unpaired) ──
+  auto write_z = [&](int te, int se, const V4* kE, const V4* kO, bool withE, bool withK) {
+  const R ce_re = (R)a.ce_re, ce_im = (R)a.ce_im, cm = (R)a.cm;
+  const size_t ncol = 2 * (size_t)a.Nn;
+  C2* Z = (C2*)a.Z;
+  const int rowE = (uvsub ? a.Nn : 0) + te + asel;   // tt / ff, and tf / ft rows
+  const int colE = (uvsub ? a.Nn : 0) + se;          // tt / ff column (+ b)
+  const int colO = (uvsub ? 0 : a.Nn) + se;          // tf / ft column (+ b)
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int n = 16 * t + c16;
+    if (n > a.M) continue;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+      const R er = withE ? stash[wl][t][2 * bi][lane] : (R)0, ei = withE ? stash[wl][t][2 * bi + 1][lane] : (R)0;
+      const R orr = withE ? stash[wl][t][4 + 2 * bi][lane] : (R)0, oi = withE ? stash[wl][t][4 + 2 * bi + 1][lane] : (R)0;
+      const R Ker = withK ? kE[t][2 * bi] : (R)0, Kei = withK ? kE[t][2 * bi + 1] : (R)0;
+      const R Kor = withK ? kO[t][2 * bi + 1] : (R)0, Koi = withK ? -kO[t][2 * bi] : (R)0;  // -j (re + j im)
+      const R zer = ce_re * er - ce_im * ei + cm * Ker;
+      const R zei = ce_re * ei + ce_im * er + cm * Kei;
+      const R zor = ce_re * orr - ce_im * oi + cm * Kor;
+      const R zoi = ce_re * oi + ce_im * orr + cm * Koi;
+      C2* pe_ = Z + ((size_t)rowE * ncol + colE + bi) * a.zmodes + a.zoff;
+      C2* po_ = Z + ((size_t)rowE * ncol + colO + bi) * a.zmodes + a.zoff;
+      C2 v;
+      v = pe_[n]; v.x += zer; v.y += zei; pe_[n] = v;
+      v = po_[n]; v.x += zor; v.y += zoi; po_[n] = v;
+      if (n > 0 && a.neg) {
+        v = pe_[-n]; v.x += zer; v.y += zei; pe_[-n] = v;
+        v = po_[-n]; v.x -= zor; v.y -= zoi; po_[-n] = v;
+      }
+    }
+  }
+  };
+
+  auto transpose_gh = [&](V4* gr, V4* gi) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      transpose_lane_groups<R>(gr[t]);
+      transpose_lane_groups<R>(gi[t]);
+    }
+  };
 
   // ── phase 1: G_n = sum_j c_j (2pi/N) g(2 pi j/N) cos(2 pi n j / N) ──
   {
@@ -140,14 +236,16 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       }
     }
 
-    // ── phase 2: EFIE blocks per order; lane holds G[g = kk][h = r][n = 16 t + c16] ──
-    const int ig = 4 * e + kk;
+    // ── phase 2: EFIE blocks per order of the pair (test te, source se); lane
+    //    holds G[g = kk][h = r][n = 16 t + c16] ──
+    auto efie_blocks = [&](int te, int se) {
+    const int ig = 4 * te + kk;
     const R trp = (R)P.trho[ig], tzp = (R)P.tz[ig];
     const R wrp = (R)(P.w[ig] * P.rho[ig]);
     const R Twa[2] = {(R)P.T0[ig] * wrp, (R)P.T1[ig] * wrp};
     const R Swa[2] = {(R)(P.T0[ig] * P.w[ig]), (R)(P.T1[ig] * P.w[ig])};
     const R Dwa[2] = {(R)(P.D0[ig] * P.w[ig]), (R)(P.D1[ig] * P.w[ig])};
-    const R trq = (R)Q.trho[4 * f], tzq = (R)Q.tz[4 * f];
+    const R trq = (R)Q.trho[4 * se], tzq = (R)Q.tz[4 * se];
     const R ik2 = (R)1 / (k * k);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
@@ -167,7 +265,7 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
         for (int ri = 0; ri < 2; ++ri) { sTG[bi][ri] = 0; sTC[bi][ri] = 0; sTS[bi][ri] = 0; sDG[bi][ri] = 0; sSG[bi][ri] = 0; }
 #pragma unroll
       for (int h = 0; h < 4; ++h) {
-        const int iq2 = 4 * f + h;
+        const int iq2 = 4 * se + h;
         const R wr = (R)(Q.w[iq2] * Q.rho[iq2]);
         const R Twb[2] = {(R)Q.T0[iq2] * wr, (R)Q.T1[iq2] * wr};
         const R Swb[2] = {(R)(Q.T0[iq2] * Q.w[iq2]), (R)(Q.T1[iq2] * Q.w[iq2])};
@@ -230,13 +328,28 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
 #pragma unroll
         for (int i = 0; i < 4; ++i) stash[wl][t][4 * q + i][lane] = xadd16(y16[8 * q + i], y16[8 * q + 4 + i]);
     }
+    };
+    if (!SPLIT) {
+      // the pair (f, e) has the transposed order samples, G(p, q) = G(q, p):
+      // its EFIE blocks go to Z now, and G returns to (e, f)
+      transpose_gh(accR, accI);
+      efie_blocks(f, e);
+      write_z(f, e, nullptr, nullptr, true, false);
+      transpose_gh(accR, accI);
+    }
+    efie_blocks(e, f);
   }
 
-  // ── phase 3: MFIE brackets, contracted to nodal (a, b) rows before the transform ──
+  // ── phase 3: MFIE brackets of the pair (test te, source se), contracted to
+  //    nodal (a, b) rows before the transform ──
   V4 accE[NT], accO[NT];
+  auto mfie_brackets = [&](int te, int se) {
 #pragma unroll
   for (int t = 0; t < NT; ++t) { accE[t] = V4{0, 0, 0, 0}; accO[t] = V4{0, 0, 0, 0}; }
   {
+    const int ip = 4 * te + g_in, iq = 4 * se + h_in;
+    const R rp = (R)P.rho[ip], zp = (R)P.z[ip], rq = (R)Q.rho[iq], zq = (R)Q.z[iq];
+    const R dz = zp - zq;
     const R trp = (R)P.trho[ip], tzp = (R)P.tz[ip], trq = (R)Q.trho[iq], tzq = (R)Q.tz[iq];
     const R wp = (R)(P.w[ip] * P.rho[ip]), wq = (R)(Q.w[iq] * Q.rho[iq]);
     // The nodal weight of the point pair factorises, W_ab = Wa_a(g) Wb_b(h), so
@@ -326,38 +439,13 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       }
     }
   }
+  };
 
-  // ── phase 4: Z = ce * EFIE + cm * K for modes +n and -n ──
-  const R ce_re = (R)a.ce_re, ce_im = (R)a.ce_im, cm = (R)a.cm;
-  const size_t ncol = 2 * (size_t)a.Nn;
-  C2* Z = (C2*)a.Z;
-  const int rowE = (uvsub ? a.Nn : 0) + e + asel;   // tt / ff, and tf / ft rows
-  const int colE = (uvsub ? a.Nn : 0) + f;          // tt / ff column (+ b)
-  const int colO = (uvsub ? 0 : a.Nn) + f;          // tf / ft column (+ b)
-#pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int n = 16 * t + c16;
-    if (n > a.M) continue;
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
-      const R er = stash[wl][t][2 * bi][lane], ei = stash[wl][t][2 * bi + 1][lane];
-      const R orr = stash[wl][t][4 + 2 * bi][lane], oi = stash[wl][t][4 + 2 * bi + 1][lane];
-      const R Ker = accE[t][2 * bi], Kei = accE[t][2 * bi + 1];
-      const R Kor = accO[t][2 * bi + 1], Koi = -accO[t][2 * bi];  // -j (re + j im)
-      const R zer = ce_re * er - ce_im * ei + cm * Ker;
-      const R zei = ce_re * ei + ce_im * er + cm * Kei;
-      const R zor = ce_re * orr - ce_im * oi + cm * Kor;
-      const R zoi = ce_re * oi + ce_im * orr + cm * Koi;
-      C2* pe_ = Z + ((size_t)rowE * ncol + colE + bi) * a.zmodes + a.zoff;
-      C2* po_ = Z + ((size_t)rowE * ncol + colO + bi) * a.zmodes + a.zoff;
-      C2 v;
-      v = pe_[n]; v.x += zer; v.y += zei; pe_[n] = v;
-      v = po_[n]; v.x += zor; v.y += zoi; po_[n] = v;
-      if (n > 0 && a.neg) {
-        v = pe_[-n]; v.x += zer; v.y += zei; pe_[-n] = v;
-        v = po_[-n]; v.x -= zor; v.y -= zoi; po_[-n] = v;
-      }
-    }
+  mfie_brackets(e, f);
+  write_z(e, f, accE, accO, true, true);
+  if (!SPLIT) {   // the MFIE operator has no such symmetry: (f, e) on its own
+    mfie_brackets(f, e);
+    write_z(f, e, accE, accO, false, true);
   }
 }
 
