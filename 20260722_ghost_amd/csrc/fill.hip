@@ -53,26 +53,14 @@ __device__ __forceinline__ R rs_step(R lo, R hi, bool b) {
   return keep + dpp_mov<CTRL>(send);
 }
 
-// 4 x 4 transpose of (lane group kk = lane / 16) x (accumulator element r) of
-// a 16x16x4 MFMA result: the entry of lane group kk, element r moves to lane
-// group r, element kk.  Two butterflies: the off-diagonal 2 x 2 blocks across
-// wavefront halves (v_permlane32_swap), then inside each block across row
-// pairs (v_permlane16_swap); it is its own inverse.
-__device__ __forceinline__ void perm_swap32(float& x, float& y) {
-  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(y), false, false);
-  x = __int_as_float(r[0]);
-  y = __int_as_float(r[1]);
-}
+// Lane exchanges for the (f, e) EFIE blocks of an unordered element pair (see
+// k_far_fill): v_permlane16_swap exchanges the odd rows of x with the even
+// rows of y (a 2 x 2 transpose of lane bit 4 against the x / y index), and
+// half_swap reads the other wavefront half.
 __device__ __forceinline__ void perm_swap16(float& x, float& y) {
   const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(x), __float_as_int(y), false, false);
   x = __int_as_float(r[0]);
   y = __int_as_float(r[1]);
-}
-__device__ __forceinline__ void perm_swap32(double& x, double& y) {
-  const auto a = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
-  const auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
-  x = __hiloint2double(b[0], a[0]);
-  y = __hiloint2double(b[1], a[1]);
 }
 __device__ __forceinline__ void perm_swap16(double& x, double& y) {
   const auto a = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
@@ -80,13 +68,15 @@ __device__ __forceinline__ void perm_swap16(double& x, double& y) {
   x = __hiloint2double(b[0], a[0]);
   y = __hiloint2double(b[1], a[1]);
 }
-template <typename R, typename V4> __device__ __forceinline__ void transpose_lane_groups(V4& v) {
-  R x0 = v[0], x1 = v[1], x2 = v[2], x3 = v[3];
-  perm_swap32(x0, x2);
-  perm_swap32(x1, x3);
-  perm_swap16(x0, x1);
-  perm_swap16(x2, x3);
-  v[0] = x0; v[1] = x1; v[2] = x2; v[3] = x3;
+// value of lane l ^ 32 (the other wavefront half)
+__device__ __forceinline__ float half_swap(float x, int lane) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(x), __float_as_int(x), false, false);
+  return __int_as_float(lane < 32 ? r[1] : r[0]);
+}
+__device__ __forceinline__ double half_swap(double x, int lane) {
+  const auto a = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(x), false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(x), false, false);
+  return lane < 32 ? __hiloint2double(b[1], a[1]) : __hiloint2double(b[0], a[0]);
 }
 
 // Wavefronts (element pairs) per work-group of k_far_fill: they share one copy
@@ -166,8 +156,14 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
   const int uvsub = kk >> 1, asel = kk & 1;
 
   // ── phase 4: Z += ce * EFIE (the stash) + cm * K (accE, accO) of the pair
-  //    (test te, source se) for modes +n (and -n unpaired) ──
-  auto write_z = [&](int te, int se, const V4* kE, const V4* kO, bool withE, bool withK) {
+  //    (test te, source se) for modes +n (and -n unpaired).  With a
+  //    transposed stash the EFIE blocks come from the opposite pair: the
+  //    EFIE operator is symmetric, Z(f, e) = Z(e, f)^T with tf(f, e) =
+  //    -ft(e, f)^T and ft(f, e) = -tf(e, f)^T (the test and source weights of
+  //    every block term factorise and G, Gc, Gs are symmetric in the point
+  //    pair), so lane (uvsub, a, n) takes register b of lane (uvsub, b, n)
+  //    register a for tt / ff, and of the other half (ft <-> tf), negated ──
+  auto write_z = [&](int te, int se, const V4* kE, const V4* kO, bool transposed) {
   const R ce_re = (R)a.ce_re, ce_im = (R)a.ce_im, cm = (R)a.cm;
   const size_t ncol = 2 * (size_t)a.Nn;
   C2* Z = (C2*)a.Z;
@@ -178,12 +174,24 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
   for (int t = 0; t < NT; ++t) {
     const int n = 16 * t + c16;
     if (n > a.M) continue;
+    // the tile's EFIE values [b][reim], even (tt / ff) and odd (tf / ft) blocks
+    R ev[4], ov[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { ev[i] = stash[wl][t][i][lane]; ov[i] = stash[wl][t][4 + i][lane]; }
+    if (transposed) {
+      perm_swap16(ev[0], ev[2]);
+      perm_swap16(ev[1], ev[3]);
+      perm_swap16(ov[0], ov[2]);
+      perm_swap16(ov[1], ov[3]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ov[i] = -half_swap(ov[i], lane);
+    }
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
-      const R er = withE ? stash[wl][t][2 * bi][lane] : (R)0, ei = withE ? stash[wl][t][2 * bi + 1][lane] : (R)0;
-      const R orr = withE ? stash[wl][t][4 + 2 * bi][lane] : (R)0, oi = withE ? stash[wl][t][4 + 2 * bi + 1][lane] : (R)0;
-      const R Ker = withK ? kE[t][2 * bi] : (R)0, Kei = withK ? kE[t][2 * bi + 1] : (R)0;
-      const R Kor = withK ? kO[t][2 * bi + 1] : (R)0, Koi = withK ? -kO[t][2 * bi] : (R)0;  // -j (re + j im)
+      const R er = ev[2 * bi], ei = ev[2 * bi + 1];
+      const R orr = ov[2 * bi], oi = ov[2 * bi + 1];
+      const R Ker = kE[t][2 * bi], Kei = kE[t][2 * bi + 1];
+      const R Kor = kO[t][2 * bi + 1], Koi = -kO[t][2 * bi];  // -j (re + j im)
       const R zer = ce_re * er - ce_im * ei + cm * Ker;
       const R zei = ce_re * ei + ce_im * er + cm * Kei;
       const R zor = ce_re * orr - ce_im * oi + cm * Kor;
@@ -199,14 +207,6 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       }
     }
   }
-  };
-
-  auto transpose_gh = [&](V4* gr, V4* gi) {
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      transpose_lane_groups<R>(gr[t]);
-      transpose_lane_groups<R>(gi[t]);
-    }
   };
 
   // ── phase 1: G_n = sum_j c_j (2pi/N) g(2 pi j/N) cos(2 pi n j / N) ──
@@ -329,14 +329,6 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
         for (int i = 0; i < 4; ++i) stash[wl][t][4 * q + i][lane] = xadd16(y16[8 * q + i], y16[8 * q + 4 + i]);
     }
     };
-    if (!SPLIT) {
-      // the pair (f, e) has the transposed order samples, G(p, q) = G(q, p):
-      // its EFIE blocks go to Z now, and G returns to (e, f)
-      transpose_gh(accR, accI);
-      efie_blocks(f, e);
-      write_z(f, e, nullptr, nullptr, true, false);
-      transpose_gh(accR, accI);
-    }
     efie_blocks(e, f);
   }
 
@@ -442,10 +434,10 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
   };
 
   mfie_brackets(e, f);
-  write_z(e, f, accE, accO, true, true);
-  if (!SPLIT) {   // the MFIE operator has no such symmetry: (f, e) on its own
+  write_z(e, f, accE, accO, false);
+  if (!SPLIT) {   // (f, e): EFIE from the transposed stash, MFIE (no such symmetry) on its own
     mfie_brackets(f, e);
-    write_z(f, e, accE, accO, false, true);
+    write_z(f, e, accE, accO, true);
   }
 }
 
