@@ -639,9 +639,11 @@ static int prepare_problem(ghost_bor* h, Problem& p) {
   if ((rc = dalloc(h, &p.d_batches, p.batches.size() * sizeof(int2)))) return rc;
   if ((rc = dalloc(h, &p.d_NP, (size_t)p.npp * (p.M + 2) * 5 * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &p.d_NB, (size_t)3 * Ne * 16 * p.nmodes * sizeof(double2)))) return rc;
-  // far blocks [2Nn][2Nn][zmodes]: modes -M..M, or 0..M with pairing
-  p.zmodes = h->pairing ? p.M + 1 : p.nmodes;
-  p.zoff = h->pairing ? 0 : p.M;
+  // far blocks [2Nn][2Nn][zmodes]: modes -M..M, or 0..M with pairing, mode 0
+  // and the stride at multiples of 16 so that every 16-order tile the far fill
+  // reads and writes is whole cache lines (128 B in FP32, 256 B in FP64)
+  p.zoff = h->pairing ? 0 : (p.M + 15) / 16 * 16;
+  p.zmodes = (p.zoff + p.M + 1 + 15) / 16 * 16;
   const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.zmodes;
   const size_t zsz = h->precision == GHOST_FP64 ? sizeof(double2) : sizeof(float2);
   if ((rc = dalloc(h, &p.d_zfar, zel * zsz))) return rc;
@@ -1404,16 +1406,16 @@ static int rhs_problem(ghost_bor* h, Problem& p, const Geom& tg) {
   prof_end(h, K_BESSEL, 0.0);
 
   LAUNCHCHK();
-  int maxn = 0;
-  for (int i = 0; i < p.nmodes; ++i) maxn = std::max(maxn, h->sys[p.sys0 + i].n);
-  // modes fastest: the systems of one (row, column block) run together and
-  // share the Bessel and phase rows of the row's points in L2
+  // one thread per (node, aspect) and chunk of 8 consecutive modes
   const int mlo = h->pairing ? p.M : 0;   // paired: modes -m share mode m's right-hand sides
-  dim3 gr(p.nmodes - mlo, maxn, (p.ntheta + 127) / 128);
+  const int nsys = p.nmodes - mlo;
+  constexpr int MC = 8;
+  dim3 gr((p.ntheta + 127) / 128, p.Nn, (nsys + MC - 1) / MC);
+  const SysInfo* dsys = h->d_sys + p.sys0 + mlo;
   prof_begin(h);
   // mixed: B in FP64 only; the solve writes the column-scaled FP32 copy into the augmented columns
-  if (mixed) hipLaunchKernelGGL((k_rhs<double>), gr, dim3(128), 0, h->stream, ea, h->d_sys + p.sys0 + mlo, nullptr, h->d_B64, 0);
-  else hipLaunchKernelGGL((k_rhs<double>), gr, dim3(128), 0, h->stream, ea, h->d_sys + p.sys0 + mlo, h->d_A, h->d_B64, 0);
+  if (mixed) hipLaunchKernelGGL((k_rhs<double, MC>), gr, dim3(128), 0, h->stream, ea, dsys, nsys, nullptr, h->d_B64);
+  else hipLaunchKernelGGL((k_rhs<double, MC>), gr, dim3(128), 0, h->stream, ea, dsys, nsys, h->d_A, h->d_B64);
   prof_end(h, K_RHS, 0.0);
   LAUNCHCHK();
   return 0;

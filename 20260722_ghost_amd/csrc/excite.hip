@@ -100,9 +100,8 @@ __global__ void k_bessel(ExciteArgs a) {
   }
 }
 
-// j_n = i^n J_n(u) with n >= 0 (j_{-n} = j_n for integer n)
-__device__ __forceinline__ double2 jn_val(const double* J, int n, size_t stride) {
-  const double v = J[(size_t)n * stride];
+// j_n = i^n J_n(u) with n >= 0 (j_{-n} = j_n for integer n), from v = J_n
+__device__ __forceinline__ double2 jn_phase(int n, double v) {
   switch (n & 3) {
     case 0: return make_double2(v, 0.0);
     case 1: return make_double2(0.0, v);
@@ -110,76 +109,135 @@ __device__ __forceinline__ double2 jn_val(const double* J, int n, size_t stride)
     default: return make_double2(0.0, -v);
   }
 }
+__device__ __forceinline__ double2 jn_val(const double* J, int n, size_t stride) {
+  return jn_phase(n, J[(size_t)n * stride]);
+}
 
 // RHS columns theta-major then (VV, HH), written into the augmented system
 // columns [n + col, n + col + nrhs) (bor_solver.py:750-752) when Abase is
-// set, and into the FP64 copy B64 when that is set.
-template <typename Ra>
-__global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, void* Abase,
-                                             double2* B64, int sys0) {
+// set, and into the FP64 copy B64 when that is set.  One thread = (node,
+// aspect) over a chunk of MC consecutive systems (modes m, m+1, ...): it
+// writes both of the node's rows (its J_t basis and its J_phi basis, where
+// the mode's mask keeps them), and the Bessel values j_{m-1}, j_m, j_{m+1}
+// of the node's 8 points slide through registers from mode to mode, so every
+// Bessel table entry is read once per node and chunk.
+template <typename Ra, int MC>
+__global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, int nsys, void* Abase,
+                                             double2* B64) {
   typedef typename CT<Ra>::c Ca;
-  const SysInfo s = sys[blockIdx.x];   // systems fastest (see the launch)
-  const int r = blockIdx.y;
-  if (r >= s.n) return;
-  const int it = blockIdx.z * blockDim.x + threadIdx.x;   // aspect: columns 2 it (VV), 2 it + 1 (HH)
-  if (2 * it >= s.nrhs) return;
-  const bool phi = r >= s.nt;
-  const int node = phi ? s.f0 + (r - s.nt) : s.t0 + r;
-  const int am = abs(s.m);
+  const int node = blockIdx.y;
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;   // aspect: columns 2 it (VV), 2 it + 1 (HH)
+  const int i0 = blockIdx.z * MC, i1 = min(nsys, i0 + MC);
+  if (it >= a.ntheta || i0 >= i1) return;
   const double th = a.theta[it];
   double st, ct;
   sincos(th, &st, &ct);
   const size_t stride = (size_t)a.P * a.ntheta;
-  double2 acc[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
   const PointSet& P = a.pts;
   const int Ne = a.Nn - 1;
   const double ie = 1.0 / 376.730313668;
-  for (int side = 0; side < 2; ++side) {
-    const int e = node - 1 + side;  // element e with this node as T1 (side 0) or T0 (side 1)
-    if (e < 0 || e >= Ne) continue;
-    for (int gq = 0; gq < 4; ++gq) {
-      const int p = 4 * e + gq;
-      const double T = side == 0 ? P.T1[p] : P.T0[p];
-      const double* Jp = a.J + (size_t)p * a.ntheta + it;
-      const double2 jm = jn_val(Jp, am, stride);
-      const double2 jm1 = jn_val(Jp, abs(s.m - 1), stride);
-      const double2 jp1 = jn_val(Jp, abs(s.m + 1), stride);
+  // the node's points: element node - 1 (the node is its T1 end), then
+  // element node (T0 end), 4 Gauss points each
+  double T[8], tr[8], tz[8];
+  double2 E[8];
+  const double* Jp[8];
+  bool ok[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int e = node - 1 + (q >> 2);
+    ok[q] = e >= 0 && e < Ne;
+    const int p = ok[q] ? 4 * e + (q & 3) : 0;
+    T[q] = q < 4 ? P.T1[p] : P.T0[p];
+    tr[q] = P.trho[p];
+    tz[q] = P.tz[p];
+    E[q] = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
+    Jp[q] = a.J + (size_t)p * a.ntheta + it;
+  }
+  // J_|m-1|, J_|m|, J_|m+1| of each point for the chunk's first mode
+  const int mfirst = sys[i0].m;
+  double w0[8], w1[8], w2[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    w0[q] = Jp[q][(size_t)abs(mfirst - 1) * stride];
+    w1[q] = Jp[q][(size_t)abs(mfirst) * stride];
+    w2[q] = Jp[q][(size_t)abs(mfirst + 1) * stride];
+  }
+  for (int i = i0; i < i1; ++i) {
+    const SysInfo s = sys[i];
+    const int m = s.m;
+    const int am = abs(m), am1 = abs(m - 1), ap1 = abs(m + 1);
+    double2 acT[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+    double2 acF[2] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (!ok[q]) continue;
+      const double2 jm = jn_phase(am, w1[q]), jm1 = jn_phase(am1, w0[q]), jp1 = jn_phase(ap1, w2[q]);
       // Ic = pi (j_{m-1} + j_{m+1}),  Is = -j pi (j_{m-1} - j_{m+1}),  I1 = 2 pi j_m
       const double2 Ic = make_double2(GH_PI * (jm1.x + jp1.x), GH_PI * (jm1.y + jp1.y));
       const double2 Is = make_double2(GH_PI * (jm1.y - jp1.y), -GH_PI * (jm1.x - jp1.x));
       const double2 I1 = make_double2(2.0 * GH_PI * jm.x, 2.0 * GH_PI * jm.y);
-      const double tr = P.trho[p], tz = P.tz[p];
       // u = ct tr Ic - st tz I1 (the theta-directed incident field on a t basis)
-      const double2 u = make_double2(ct * tr * Ic.x - st * tz * I1.x, ct * tr * Ic.y - st * tz * I1.y);
-      double2 eE[2], eH[2];   // [VV, HH]
-      if (!phi) {
-        eE[0] = u;                                            eH[0] = make_double2(Ic.x * ie, Ic.y * ie);
-        eE[1] = make_double2(tr * Is.x, tr * Is.y);           eH[1] = make_double2(ct * Is.x * ie, ct * Is.y * ie);
-      } else {
-        eE[0] = make_double2(-ct * Is.x, -ct * Is.y);         eH[0] = make_double2(-tr * Is.x * ie, -tr * Is.y * ie);
-        eE[1] = Ic;                                           eH[1] = make_double2(u.x * ie, u.y * ie);
-      }
-      const double2 E = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
+      const double2 u = make_double2(ct * tr[q] * Ic.x - st * tz[q] * I1.x, ct * tr[q] * Ic.y - st * tz[q] * I1.y);
+      double2 tE[2], tH[2], fE[2], fH[2];   // [VV, HH] of the J_t and the J_phi basis
+      tE[0] = u;                                          tH[0] = make_double2(Ic.x * ie, Ic.y * ie);
+      tE[1] = make_double2(tr[q] * Is.x, tr[q] * Is.y);   tH[1] = make_double2(ct * Is.x * ie, ct * Is.y * ie);
+      fE[0] = make_double2(-ct * Is.x, -ct * Is.y);       fH[0] = make_double2(-tr[q] * Is.x * ie, -tr[q] * Is.y * ie);
+      fE[1] = Ic;                                         fH[1] = make_double2(u.x * ie, u.y * ie);
 #pragma unroll
       for (int pol = 0; pol < 2; ++pol) {
-        const double2 ev = make_double2(a.cE * eE[pol].x + a.cM * eH[pol].x, a.cE * eE[pol].y + a.cM * eH[pol].y);
-        acc[pol].x += T * (ev.x * E.x - ev.y * E.y);
-        acc[pol].y += T * (ev.x * E.y + ev.y * E.x);
+        const double2 vt = make_double2(a.cE * tE[pol].x + a.cM * tH[pol].x, a.cE * tE[pol].y + a.cM * tH[pol].y);
+        const double2 vf = make_double2(a.cE * fE[pol].x + a.cM * fH[pol].x, a.cE * fE[pol].y + a.cM * fH[pol].y);
+        acT[pol].x += T[q] * (vt.x * E[q].x - vt.y * E[q].y);
+        acT[pol].y += T[q] * (vt.x * E[q].y + vt.y * E[q].x);
+        acF[pol].x += T[q] * (vf.x * E[q].x - vf.y * E[q].y);
+        acF[pol].y += T[q] * (vf.x * E[q].y + vf.y * E[q].x);
       }
     }
-  }
-  if (Abase) {
-    Ca* A = (Ca*)Abase + s.a_off + (size_t)r * s.ld + s.n + s.col + 2 * it;
-    Ca o0, o1;
-    o0.x = (Ra)acc[0].x; o0.y = (Ra)acc[0].y;
-    o1.x = (Ra)acc[1].x; o1.y = (Ra)acc[1].y;
-    A[0] = o0;
-    A[1] = o1;
-  }
-  if (B64) {
-    double2* b = B64 + s.x_off + (size_t)r * s.ncols + s.col + 2 * it;
-    b[0] = acc[0];
-    b[1] = acc[1];
+    if (2 * it < s.nrhs) {
+      const int nf = s.n - s.nt;
+      const int rt = node - s.t0, rf = node - s.f0;
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        int r;
+        if (b == 0) { if (rt < 0 || rt >= s.nt) continue; r = rt; }
+        else { if (rf < 0 || rf >= nf) continue; r = s.nt + rf; }
+        const double2* acc = b == 0 ? acT : acF;
+        if (Abase) {
+          Ca* A = (Ca*)Abase + s.a_off + (size_t)r * s.ld + s.n + s.col + 2 * it;
+          Ca o0, o1;
+          o0.x = (Ra)acc[0].x; o0.y = (Ra)acc[0].y;
+          o1.x = (Ra)acc[1].x; o1.y = (Ra)acc[1].y;
+          A[0] = o0;
+          A[1] = o1;
+        }
+        if (B64) {
+          double2* bb = B64 + s.x_off + (size_t)r * s.ncols + s.col + 2 * it;
+          bb[0] = acc[0];
+          bb[1] = acc[1];
+        }
+      }
+    }
+    // slide the Bessel window one order up (a problem's systems are its
+    // consecutive modes; any other next mode reloads the window)
+    if (i + 1 < i1) {
+      const int mn = sys[i + 1].m;
+      if (mn == m + 1) {
+        const int an = abs(m + 2);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          w0[q] = w1[q];
+          w1[q] = w2[q];
+          w2[q] = Jp[q][(size_t)an * stride];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          w0[q] = Jp[q][(size_t)abs(mn - 1) * stride];
+          w1[q] = Jp[q][(size_t)abs(mn) * stride];
+          w2[q] = Jp[q][(size_t)abs(mn + 1) * stride];
+        }
+      }
+    }
   }
 }
 

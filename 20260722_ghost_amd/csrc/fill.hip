@@ -39,7 +39,7 @@ struct FarArgs {
   int Ne, Nn, M, nE, nB, pe, pf, nfp, npairs;
   int neg;               // also write modes -n (off with pairing: they are never stored)
   int delta, imax;       // unordered pairs (not SPLIT): f = pf + 2 j with j >= i + delta, rows i < imax
-  int zmodes, zoff;      // Z mode stride and the index of mode 0 (2M+1, M; or M+1, 0)
+  int zmodes, zoff;      // Z mode stride and the index of mode 0 (multiples of 16; zoff = 0 with pairing)
   double k, ce_re, ce_im, cm;
   void* Z;               // Zfar [2Nn][2Nn][zmodes] complex<R>
 };
