@@ -502,15 +502,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
   const double k = a.k;
   const double d2 = (rp - rq) * (rp - rq) + (zp - zq) * (zp - zq);
   const double rr4 = 4.0 * rp * rq;
-  const bool axis = rr4 <= 1e-30;
+  const bool axis0 = rr4 <= 1e-30;
   const double d = sqrt(d2 > 1e-300 ? d2 : 1e-300);
   const double aa = sqrt(rr4 > 1e-300 ? rr4 : 1e-300);
-  const double s0 = axis ? 1.0 : fmin(0.25, 20.0 * d / aa);
-  const double vmax = asinh((aa / d) * s0);
-  const double xi0 = 2.0 * asin(fmin(s0, 1.0));
-  const double span = GH_PI - xi0;
+  const double s0 = axis0 ? 1.0 : fmin(0.25, 20.0 * d / aa);
   const int nnodes = 48 + pr.n_tail;
   const int n0 = cs * C;
+  // The point pair's geometry goes to LDS (one copy per slot) and the node
+  // values below read it from there: held in registers across the node loop
+  // it would sit beside the C x 10 accumulators and push them into scratch.
+  __shared__ double pg[32][15];
+  if (cs == 0) {
+    double* g = pg[slot];
+    g[0] = asinh((aa / d) * s0);          // vmax
+    g[1] = d;
+    g[2] = aa;
+    g[3] = d / aa;
+    g[4] = 2.0 * asin(fmin(s0, 1.0));      // xi0
+    g[5] = GH_PI - g[4];                   // span
+    g[6] = rp;
+    g[7] = rq;
+    g[8] = zp - zq;                        // Rz
+    g[9] = trp;
+    g[10] = tzp;
+    g[11] = trq;
+    g[12] = tzq;
+    g[13] = axis0 ? 1.0 : 0.0;
+    g[14] = d2;
+  }
+  __syncthreads();
 
   // per point-pair slot: 8 nodes x 12 values (slot stride 97 doubles, odd, so
   // the 8 slots of a wavefront fall on different banks), and 8 nodes x 8 chunk
@@ -533,6 +553,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
     const int vn = v0 + cs;
     double* out = &nv[slot][12 * cs];
     if (valid && vn < nnodes) {
+      const double* g = pg[slot];
+      const double vmax = g[0], d = g[1], aa = g[2], doa = g[3], xi0 = g[4], span = g[5];
+      const double rp = g[6], rq = g[7], Rz = g[8], trp = g[9], tzp = g[10], trq = g[11], tzq = g[12];
+      const bool axis = g[13] != 0.0;
       // xi enters through s = sin(xi/2) only: cos xi = 1 - 2 s^2 and
       // sin xi = 2 s cos(xi/2), with cos(xi/2) = sqrt(1 - s^2) on the core
       // (xi = 2 asin s, s <= 1), so neither asin nor a sincos of xi is needed
@@ -541,10 +565,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
         const double vv = a.core_u[vn] * vmax, wv = a.core_w[vn] * vmax;
         const double em = expm1(vv), ev = em + 1.0;
         const double shv = 0.5 * (em + em / ev), chv = 0.5 * (ev + 1.0 / ev);   // sinh, cosh
-        double sv = (d / aa) * shv;
+        double sv = doa * shv;
         sv = sv < 1.0 ? sv : 1.0;
         const double cv = sqrt(fmax(1.0 - sv * sv, 0.0));
-        wq = wv * 2.0 * (d / aa) * chv / fmax(cv, sqrt(1e-15));
+        wq = wv * 2.0 * doa * chv / fmax(cv, sqrt(1e-15));
         Rd = sqrt(d * d + (aa * sv) * (aa * sv));
         sx = 2.0 * sv * cv;
         cx = 1.0 - 2.0 * sv * sv;
@@ -570,7 +594,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
       const double pre = 2.0 * wq * inv * inv * inv / (4.0 * GH_PI);
       const double kr = k * Rd;
       const double pr_ = pre * (ck + kr * sk), pi_ = pre * (kr * ck - sk);
-      const double Rx = rp - rq * cx, Ry = rq * sx, Rz = zp - zq;
+      const double Rx = rp - rq * cx, Ry = rq * sx;
       const double WtR = trp * Rx + tzp * Rz, nR = -tzp * Rx + trp * Rz;
       const double n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
       const double Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
@@ -647,6 +671,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
     __syncthreads();
   }
   if (!valid) return;
+  const bool axis = pg[slot][13] != 0.0;
+  const double d2g = pg[slot][14];
   double2* out = a.NP + ((size_t)(pr.pp_off + q) * a.nord) * 5;
 #pragma unroll
   for (int i = 0; i < C; ++i) {
@@ -656,7 +682,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
     if (axis) {
       // closed form: only order 0, 2 pi g(d)
       if (n == 0) {
-        const double R0 = sqrt(d2);
+        const double R0 = sqrt(d2g);
         const double Rc = R0 > 1e-300 ? R0 : 1e-300;
         double sk, ck;
         sincos(k * R0, &sk, &ck);
