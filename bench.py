@@ -322,8 +322,8 @@ def main():
                     help="time forward + adjoint RCS gradient on the 600-element cone-sphere instead")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X only: multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the sweep would shard).", file=sys.stderr)
+        print("bench.py measures one MI355X only: runs on several GPUs are not supported "
+              "(the sweep would shard by frequency and mode, one process per GPU).", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration > 0:
         nbytes = args.copy_calibration * 2 ** 30
