@@ -52,12 +52,15 @@ __device__ void bessel_orders(double x, int nmax, double* out, size_t stride) {
   int N = (int)(top + 20.0 + sqrt(40.0 * top));
   N += N & 1;
   const double BIG = 1e250, SMALL = 1e-250;
+  // 2n/x as n (2/x): one division per (point, angle) instead of one per step
+  // (within 1.5 ulp of the quotient, the size of each step's own rounding)
+  const double tx = 2.0 / x;
   int nres = 0;
   double norm = 0.0;
   {
     double fp = 0.0, f = 1e-280;
     for (int n = N; n >= 1; --n) {
-      const double fm = (2.0 * n / x) * f - fp;
+      const double fm = ((double)n * tx) * f - fp;
       fp = f;
       f = fm;
       if ((n - 1) > 0 && ((n - 1) & 1) == 0) norm += 2.0 * f;
@@ -69,7 +72,7 @@ __device__ void bessel_orders(double x, int nmax, double* out, size_t stride) {
     double fp = 0.0, f = 1e-280;
     int r = 0;
     for (int n = N; n >= 1; --n) {
-      const double fm = (2.0 * n / x) * f - fp;
+      const double fm = ((double)n * tx) * f - fp;
       fp = f;
       f = fm;
       if (fabs(f) > BIG) { f *= SMALL; fp *= SMALL; ++r; }
