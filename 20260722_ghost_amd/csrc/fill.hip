@@ -88,10 +88,14 @@ __device__ __forceinline__ double half_swap(double x, int lane) {
 __host__ __device__ constexpr bool far_wide(int bytes, int nt) { return bytes == 4 && nt <= 6; }
 __host__ __device__ constexpr int far_waves(int bytes, int nt) { return far_wide(bytes, nt) ? 4 : 2; }
 
-// One wavefront per far element pair (e, f); far_waves() wavefronts per block.
+// One wavefront per far element pair: an unordered pair e < f of one parity
+// class (SPLIT = false), or an ordered pair (e, f) when the test and source
+// point sets differ (SPLIT = true); far_waves() wavefronts per block.
 // Phase 1: EFIE samples -> G_n on MFMA.  Phase 2: per-order EFIE blocks,
 // reduced over the test points and stashed in LDS.  Phase 3: MFIE brackets
 // contracted to nodal rows -> modes on MFMA.  Phase 4: combine, RMW Zfar.
+// Unordered pairs run phases 3 and 4 twice, for (e, f) and then (f, e); the
+// second phase 4 takes its EFIE blocks from the same stash by lane exchanges.
 template <typename R, int NT, bool SPLIT>
 __global__ __launch_bounds__(64 * far_waves(sizeof(R), NT))
 __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4 || NT <= 2) ? 2 : 1))) void k_far_fill(FarArgs a) {
