@@ -1247,7 +1247,7 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
       }
       const int wpb = far_waves((int)sizeof(R), NT);
       const int grid = (fa.npairs + wpb - 1) / wpb;
-      const size_t tab_bytes = sizeof(R) * (size_t)(p.nE / 2 + 1 + p.nE + 2 * (p.nB / 2 + 1) + 2 * p.nB);
+      const size_t tab_bytes = sizeof(R) * (size_t)(2 * p.nB + 4 * (p.nB / 2 + 4) + 2 * (p.nE / 2 + 4) + p.nE);
       const size_t stash_bytes = sizeof(R) * (size_t)wpb * NT * 8 * 64;
       if (tab_bytes + stash_bytes > 160 * 1024)
         return fail(GHOST_ERR_ARG, "far fill: azimuth tables (%zu B) and order stash (%zu B) exceed the 160 KiB of LDS",

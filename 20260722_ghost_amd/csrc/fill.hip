@@ -105,16 +105,34 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
   __shared__ R stash[WPB][NT][8][64];
   extern __shared__ __align__(16) char dyn_smem[];
   // azimuth tables in LDS, in the working precision:
-  // (cosB, sinB) pairs [nB] | sin2E [nE/2+1] | cosE [nE] | cxB [nB/2+1] | sxB [nB/2+1]
-  // (the bracket transform reads its cos and sin twiddle in one 2-element load)
+  // (cosB, sinB) pairs [nB] | bracket samples (cx, sx, w, 0) [nB/2+4] |
+  // EFIE samples (sin^2(xi/2), w) [nE/2+4] | cosE [nE]
+  // The sample tables run 3 entries past N/2 (the last group of 4 samples),
+  // with the even-fold weight w = c_j (2 pi/N)/(4 pi), c_j = 1 at j = 0 and
+  // N/2, 2 between and 0 past N/2 (where the geometry repeats entry N/2), so
+  // a sample takes one table load and no selects.  The bracket transform
+  // reads its cos and sin twiddle in one 2-element load.
   C2* t_csB = (C2*)dyn_smem;
-  R* t_sin2E = (R*)(t_csB + a.nB);
-  R* t_cosE = t_sin2E + (a.nE / 2 + 1);
-  R* t_cxB = t_cosE + a.nE;
-  R* t_sxB = t_cxB + (a.nB / 2 + 1);
-  for (int i = threadIdx.x; i <= a.nE / 2; i += blockDim.x) t_sin2E[i] = (R)a.sin2E[i];
+  V4* t_smpB = (V4*)(t_csB + a.nB);
+  C2* t_smpE = (C2*)(t_smpB + (a.nB / 2 + 4));
+  R* t_cosE = (R*)(t_smpE + (a.nE / 2 + 4));
+  {
+    const int hE = a.nE / 2, hB = a.nB / 2;
+    const R sc = (R)1 / ((R)2 * (R)a.nE), scB = (R)1 / ((R)2 * (R)a.nB);   // (2 pi / N) / (4 pi)
+    for (int i = threadIdx.x; i < hE + 4; i += blockDim.x) {
+      const R cw = (i == 0 || i == hE) ? (R)1 : (i > hE ? (R)0 : (R)2);
+      C2 v;
+      v.x = (R)a.sin2E[i < hE ? i : hE];
+      v.y = cw * sc;
+      t_smpE[i] = v;
+    }
+    for (int i = threadIdx.x; i < hB + 4; i += blockDim.x) {
+      const int ic = i < hB ? i : hB;
+      const R cw = (i == 0 || i == hB) ? (R)1 : (i > hB ? (R)0 : (R)2);
+      t_smpB[i] = V4{(R)a.cxB[ic], (R)a.sxB[ic], cw * scB, (R)0};
+    }
+  }
   for (int i = threadIdx.x; i < a.nE; i += blockDim.x) t_cosE[i] = (R)a.cosE[i];
-  for (int i = threadIdx.x; i <= a.nB / 2; i += blockDim.x) { t_cxB[i] = (R)a.cxB[i]; t_sxB[i] = (R)a.sxB[i]; }
   for (int i = threadIdx.x; i < a.nB; i += blockDim.x) {
     C2 v;
     v.x = (R)a.cosB[i];
@@ -220,16 +238,14 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
     for (int t = 0; t < NT; ++t) { accR[t] = V4{0, 0, 0, 0}; accI[t] = V4{0, 0, 0, 0}; }
     {
       const int hE = a.nE >> 1, mskE = a.nE - 1;
-      const R sc = (R)1 / ((R)2 * (R)a.nE);  // (2 pi / N) / (4 pi)
       for (int j0 = 0; j0 <= hE; j0 += 4) {
         const int j = j0 + kk;
-        const int jc = j < hE ? j : hE;
-        const R cw = (j == 0 || j == hE) ? (R)1 : (j > hE ? (R)0 : (R)2);
-        R Rd = sqrt_s(d2 + rr4 * t_sin2E[jc]);
+        const C2 smp = t_smpE[j];   // (sin^2(xi_j/2), weight)
+        R Rd = sqrt_s(d2 + rr4 * smp.x);
         Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
         R s, c;
         sincos_s(k * Rd, &s, &c);
-        const R amp = cw * sc * rcp_s(Rd);
+        const R amp = smp.y * rcp_s(Rd);
         const R ar = amp * c, ai = -amp * s;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -371,19 +387,17 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
     const R Wap = f32 ? dpp_mov<0x124>(Was) : dpp_mov<0xB1>(Was);
     const R Wbp = dpp_mov<0x4E>(Wbs);   // FP32 b stage (bit 1)
     const int hB = a.nB >> 1, mskB = a.nB - 1;
-    const R scB = (R)1 / ((R)2 * (R)a.nB);  // (2 pi/N)/(4 pi)
     for (int j0 = 0; j0 <= hB; j0 += 4) {
       const int j = j0 + kk;
-      const int jc = j < hB ? j : hB;
-      const R cw = (j == 0 || j == hB) ? (R)1 : (j > hB ? (R)0 : (R)2);
-      const R cx = t_cxB[jc], sx = t_sxB[jc];
+      const V4 smp = t_smpB[j];   // (cos xi_j, sin xi_j, weight, 0)
+      const R cx = smp[0], sx = smp[1];
       const R Rx = rp - rq * cx, Ry = rq * sx;
       R Rd = sqrt_s(Rx * Rx + Ry * Ry + dz * dz);
       Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
       R s, c;
       sincos_s(k * Rd, &s, &c);
       const R inv = rcp_s(Rd);
-      const R pre = cw * scB * inv * inv * inv;
+      const R pre = smp[2] * inv * inv * inv;
       const R kr = k * Rd;
       const R pr = pre * (c + kr * s), pim = pre * (kr * c - s);
       const R WtR = trp * Rx + tzp * dz, nR = -tzp * Rx + trp * dz;
