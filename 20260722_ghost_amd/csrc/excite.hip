@@ -284,11 +284,18 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
     return make_double2(ff * (double)v.x, ff * (double)v.y);
   };
   double2 acc = make_double2(0.0, 0.0);
-  int e_prev = -1;
+  int e_prev = -2;   // no element yet (element 0 must not look like a successor)
   double2 t0v = make_double2(0, 0), t1v = t0v, f0v = t0v, f1v = t0v;
   for (int p = p0; p < p1; ++p) {
     const int e = p >> 2;
-    if (e != e_prev) { t0v = xt(e); t1v = xt(e + 1); f0v = xf(e); f1v = xf(e + 1); e_prev = e; }
+    if (e != e_prev) {
+      // the next element's first node is this element's second: carry it over
+      if (e == e_prev + 1) { t0v = t1v; f0v = f1v; }
+      else { t0v = xt(e); f0v = xf(e); }
+      t1v = xt(e + 1);
+      f1v = xf(e + 1);
+      e_prev = e;
+    }
     const double T0 = P.T0[p], T1 = P.T1[p];
     const double2 Jt = make_double2(T0 * t0v.x + T1 * t1v.x, T0 * t0v.y + T1 * t1v.y);
     const double2 Jf = make_double2(T0 * f0v.x + T1 * f1v.x, T0 * f0v.y + T1 * f1v.y);
