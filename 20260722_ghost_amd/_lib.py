@@ -74,6 +74,7 @@ SIGNATURES = [
     ("ghost_bor_system_count", _ci, [_vp, _ip]),
     ("ghost_bor_get_pivots", _ci, [_vp, _ci, _ip]),
     ("ghost_bor_set_lu_super", _ci, [_vp, _ci]),
+    ("ghost_far_fill_waves", _ci, [_ci, _ci, _ci, _ci]),
 ]
 
 

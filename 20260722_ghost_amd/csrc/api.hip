@@ -404,6 +404,37 @@ extern "C" int ghost_bor_set_pairing(ghost_bor* h, int on) {
   return 0;
 }
 
+// LDS of one k_far_fill work-group: the azimuth tables (dynamic) and the
+// per-wavefront order stash (static); far_fill_waves() picks the default
+// wavefronts per block, one when the tables leave no room for two stashes,
+// and 0 when not even one fits.
+static size_t far_table_bytes(int rbytes, int nE, int nB) {
+  return (size_t)rbytes * (size_t)(2 * nB + 4 * (nB / 2 + 4) + 2 * (nE / 2 + 4) + nE);
+}
+static size_t far_stash_bytes(int rbytes, int wpb, int NT) { return (size_t)rbytes * wpb * NT * 8 * 64; }
+static int far_fill_waves(int rbytes, int NT, int nE, int nB) {
+  const size_t lds = 160 * 1024, tab = far_table_bytes(rbytes, nE, nB);
+  const int w = far_waves(rbytes, NT);
+  if (tab + far_stash_bytes(rbytes, w, NT) <= lds) return w;
+  if (rbytes == 8 && tab + far_stash_bytes(rbytes, 1, NT) <= lds) return 1;
+  return 0;
+}
+extern "C" int ghost_far_fill_waves(int precision, int m_max, int n_xi_efie, int n_xi_bracket) {
+  return far_fill_waves(precision == GHOST_MIXED ? 4 : 8, (m_max + 2 + 15) / 16, n_xi_efie, n_xi_bracket);
+}
+template <typename R, int NT>
+static void launch_far_fill(bool split, int wpb, int grid, size_t tab_bytes, hipStream_t s, const FarArgs& fa) {
+  if constexpr (sizeof(R) == 8) {
+    if (wpb == 1) {
+      if (split) hipLaunchKernelGGL((k_far_fill<R, NT, true, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
+      else hipLaunchKernelGGL((k_far_fill<R, NT, false, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
+      return;
+    }
+  }
+  if (split) hipLaunchKernelGGL((k_far_fill<R, NT, true>), dim3(grid), dim3(64 * wpb), tab_bytes, s, fa);
+  else hipLaunchKernelGGL((k_far_fill<R, NT, false>), dim3(grid), dim3(64 * wpb), tab_bytes, s, fa);
+}
+
 extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, double k, int m_max,
                                      int n_xi_efie, int n_xi_bracket, double c_efie, double c_mfie,
                                      const double* thetas_deg, int n_theta, int flags, int* idx) {
@@ -415,6 +446,12 @@ extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, 
   auto pow2 = [](int v) { return v >= 4 && (v & (v - 1)) == 0; };
   if (!pow2(n_xi_efie) || !pow2(n_xi_bracket))
     return fail(GHOST_ERR_ARG, "xi grid sizes must be powers of two >= 4");
+  if (n_xi_efie > 4096 || n_xi_bracket > 4096)
+    return fail(GHOST_ERR_ARG, "xi grid sizes above 4096 samples are not supported");
+  if (far_fill_waves(h->precision == GHOST_MIXED ? 4 : 8, (m_max + 2 + 15) / 16, n_xi_efie, n_xi_bracket) == 0)
+    return fail(GHOST_ERR_ARG,
+                "far fill: the %s azimuth tables of n_xi %d / %d with %d order tiles exceed the 160 KiB of LDS",
+                h->precision == GHOST_MIXED ? "FP32" : "FP64", n_xi_efie, n_xi_bracket, (m_max + 2 + 15) / 16);
   if (!thetas_deg || n_theta < 1) return fail(GHOST_ERR_ARG, "need at least one aspect angle");
   Problem p;
   p.Nn = Nn;
@@ -1245,19 +1282,19 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
         fa.npairs = (int)(im * base - im * (im - 1) / 2);
         if (fa.npairs <= 0) continue;
       }
-      const int wpb = far_waves((int)sizeof(R), NT);
+      const size_t tab_bytes = far_table_bytes((int)sizeof(R), p.nE, p.nB);
+      const int wpb = far_fill_waves((int)sizeof(R), NT, p.nE, p.nB);
+      if (wpb == 0)
+        return fail(GHOST_ERR_ARG,
+                    "far fill: azimuth tables (%zu B) and a one-wavefront order stash (%zu B) exceed the 160 KiB of LDS"
+                    " (n_xi %d / %d, %d order tiles)",
+                    tab_bytes, far_stash_bytes((int)sizeof(R), 1, NT), p.nE, p.nB, NT);
       const int grid = (fa.npairs + wpb - 1) / wpb;
-      const size_t tab_bytes = sizeof(R) * (size_t)(2 * p.nB + 4 * (p.nB / 2 + 4) + 2 * (p.nE / 2 + 4) + p.nE);
-      const size_t stash_bytes = sizeof(R) * (size_t)wpb * NT * 8 * 64;
-      if (tab_bytes + stash_bytes > 160 * 1024)
-        return fail(GHOST_ERR_ARG, "far fill: azimuth tables (%zu B) and order stash (%zu B) exceed the 160 KiB of LDS",
-                    tab_bytes, stash_bytes);
       prof_begin(h);
       switch (NT) {
 #define FARCASE(n)                                                                                      \
   case n:                                                                                               \
-    if (split) hipLaunchKernelGGL((k_far_fill<R, n, true>), dim3(grid), dim3(64 * wpb), tab_bytes, h->stream, fa); \
-    else hipLaunchKernelGGL((k_far_fill<R, n, false>), dim3(grid), dim3(64 * wpb), tab_bytes, h->stream, fa); \
+    launch_far_fill<R, n>(split, wpb, grid, tab_bytes, h->stream, fa);                                   \
     break;
         FARCASE(1) FARCASE(2) FARCASE(3) FARCASE(4) FARCASE(5) FARCASE(6) FARCASE(7) FARCASE(8)
 #undef FARCASE
@@ -2069,6 +2106,7 @@ extern "C" int ghost_sample_g(int nr, int np_, int nxi, const double* rho_p, con
                               const double* rho_q, const double* z_q, double k, const double* sin2_tab,
                               double* out) {
   if (nr <= 0 || np_ <= 0 || nxi <= 0) return 0;
+  if (!rho_p || !z_p || !rho_q || !z_q || !sin2_tab || !out) return fail(GHOST_ERR_ARG, "sample_g: null array");
   ScopedBufs b;
   HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
   const double *a = b.up(rho_p, nr), *bz = b.up(z_p, nr), *c = b.up(rho_q, np_), *d = b.up(z_q, np_);
@@ -2089,6 +2127,9 @@ static int sample_brackets(bool ibc, int nr, int np_, int nxi, const double* rho
                            const double* tr_q, const double* tz_q, double k, const double* cx_tab,
                            const double* sx_tab, double* o_tt, double* o_tf, double* o_ft, double* o_ff) {
   if (nr <= 0 || np_ <= 0 || nxi <= 0) return 0;
+  if (!rho_p || !z_p || !tr_p || !tz_p || !rho_q || !z_q || !tr_q || !tz_q || !cx_tab || !sx_tab || !o_tt || !o_tf ||
+      !o_ft || !o_ff)
+    return fail(GHOST_ERR_ARG, "sample brackets: null array");
   ScopedBufs b;
   HIPCHK(hipStreamCreateWithFlags(&b.s, hipStreamNonBlocking));
   const double *rp = b.up(rho_p, nr), *zp = b.up(z_p, nr), *trp = b.up(tr_p, nr), *tzp = b.up(tz_p, nr);
@@ -2135,25 +2176,46 @@ extern "C" int ghost_sample_ibc(int nr, int np_, int nxi, const double* rho_p, c
 // void-returning names with the exact bor_stream_kernel.c signatures
 // (bor_stream_kernel.c:26-29, 65-72, 108-115): errors surface through
 // ghost_last_error() because the reference ABI has no error channel.
+// The reference's samplers return nothing, and its loader allocates the
+// outputs with np.empty (bor_streaming.py:287, 327) without ever asking for an
+// error: on any failure the void forms below fill every output with NaN, so
+// the failure surfaces in the caller's fill instead of as uninitialised memory,
+// and report the first failure of the process on stderr.
+static void compat_failed(const char* name, size_t tot, double* const* outs, int nout) {
+  static std::once_flag once;
+  std::call_once(once, [&] { fprintf(stderr, "libghost_bor: %s failed: %s (outputs set to NaN)\n", name, g_err.c_str()); });
+  const double nan = std::nan("");
+  for (int i = 0; i < nout; ++i)
+    if (outs[i])
+      for (size_t j = 0; j < 2 * tot; ++j) outs[i][j] = nan;
+}
+static size_t compat_total(int nr, int np_, int nxi) {
+  return (nr > 0 && np_ > 0 && nxi > 0) ? (size_t)nr * np_ * nxi : 0;
+}
 extern "C" void sample_g(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* rho_q,
                          const double* z_q, double k, const double* sin2_tab, double* out) {
-  ghost_sample_g(nr, np_, nxi, rho_p, z_p, rho_q, z_q, k, sin2_tab, out);
+  if (ghost_sample_g(nr, np_, nxi, rho_p, z_p, rho_q, z_q, k, sin2_tab, out) != 0)
+    compat_failed("sample_g", compat_total(nr, np_, nxi), &out, 1);
 }
-
 extern "C" void sample_mfie(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* tr_p,
                             const double* tz_p, const double* rho_q, const double* z_q, const double* tr_q,
                             const double* tz_q, double k, const double* cx_tab, const double* sx_tab,
                             double* o_tt, double* o_tf, double* o_ft, double* o_ff) {
-  ghost_sample_mfie(nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab, o_tt, o_tf,
-                    o_ft, o_ff);
+  if (ghost_sample_mfie(nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab, o_tt,
+                        o_tf, o_ft, o_ff) != 0) {
+    double* const outs[4] = {o_tt, o_tf, o_ft, o_ff};
+    compat_failed("sample_mfie", compat_total(nr, np_, nxi), outs, 4);
+  }
 }
-
 extern "C" void sample_ibc(int nr, int np_, int nxi, const double* rho_p, const double* z_p, const double* tr_p,
                            const double* tz_p, const double* rho_q, const double* z_q, const double* tr_q,
                            const double* tz_q, double k, const double* cx_tab, const double* sx_tab,
                            double* o_tt, double* o_tf, double* o_ft, double* o_ff) {
-  ghost_sample_ibc(nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab, o_tt, o_tf,
-                   o_ft, o_ff);
+  if (ghost_sample_ibc(nr, np_, nxi, rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, k, cx_tab, sx_tab, o_tt,
+                       o_tf, o_ft, o_ff) != 0) {
+    double* const outs[4] = {o_tt, o_tf, o_ft, o_ff};
+    compat_failed("sample_ibc", compat_total(nr, np_, nxi), outs, 4);
+  }
 }
 
 // ─────────────────────────────────────────────────────────────────────────────

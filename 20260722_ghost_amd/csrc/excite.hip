@@ -95,7 +95,12 @@ __global__ void k_bessel(ExciteArgs a) {
     double st, ct;
     sincos(a.theta[it], &st, &ct);
     const double x = a.k * a.pts.rho[p] * st;
-    bessel_orders(fabs(x), a.M + 1, a.J + (size_t)p * a.ntheta + it, (size_t)a.P * a.ntheta);
+    double* Jt = a.J + (size_t)p * a.ntheta + it;
+    const size_t js = (size_t)a.P * a.ntheta;
+    bessel_orders(fabs(x), a.M + 1, Jt, js);
+    // sin(theta) < 0 (aspects beyond 180 degrees or below 0): J_n(-x) = (-1)^n J_n(x)
+    if (x < 0.0)
+      for (int n = 1; n <= a.M + 1; n += 2) Jt[(size_t)n * js] = -Jt[(size_t)n * js];
     double sp, cp;
     sincos(a.k * ct * a.pts.z[p], &sp, &cp);
     const double wr = a.pts.w[p] * a.pts.rho[p];

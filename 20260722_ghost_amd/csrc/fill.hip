@@ -96,12 +96,13 @@ __host__ __device__ constexpr int far_waves(int bytes, int nt) { return far_wide
 // contracted to nodal rows -> modes on MFMA.  Phase 4: combine, RMW Zfar.
 // Unordered pairs run phases 3 and 4 twice, for (e, f) and then (f, e); the
 // second phase 4 takes its EFIE blocks from the same stash by lane exchanges.
-template <typename R, int NT, bool SPLIT>
-__global__ __launch_bounds__(64 * far_waves(sizeof(R), NT))
+// WPB = 1 is the fallback for FP64 azimuth tables too large to sit beside a
+// two-wavefront stash in LDS (n_xi = 2048 with 7-8 order tiles).
+template <typename R, int NT, bool SPLIT, int WPB = far_waves(sizeof(R), NT)>
+__global__ __launch_bounds__(64 * WPB)
 __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4 || NT <= 2) ? 2 : 1))) void k_far_fill(FarArgs a) {
   typedef typename CT<R>::c C2;
   typedef typename CT<R>::v4 V4;
-  constexpr int WPB = far_waves(sizeof(R), NT);
   __shared__ R stash[WPB][NT][8][64];
   extern __shared__ __align__(16) char dyn_smem[];
   // azimuth tables in LDS, in the working precision:

@@ -220,6 +220,13 @@ int ghost_bor_get_dense(ghost_bor* h, int sys_idx, double* X);
  * duration; the known byte count calibrates HBM byte counters. */
 int ghost_copy_calibration(double bytes, float* ms);
 
+/* Host-only query (no device call): wavefronts per far-fill work-group for a
+ * problem of this precision, m_max and azimuth grid sizes, as chosen by the
+ * LDS budget (tables + per-wavefront order stash within 160 KiB); 0 means the
+ * combination does not fit and ghost_bor_add_problem rejects it.  No
+ * counterpart in the reference (a tile-size query of the device interface). */
+int ghost_far_fill_waves(int precision, int m_max, int n_xi_efie, int n_xi_bracket);
+
 #ifdef __cplusplus
 }
 #endif

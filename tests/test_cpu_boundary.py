@@ -115,3 +115,52 @@ def test_reference_c_sampler_agrees_with_oracle():
                     trq[None, :], tzq[None, :], k, xi, r_min=1e-30)
         for o, r in zip(outs, refs):
             assert np.max(np.abs(o - r)) < 1e-12 * np.max(np.abs(r))
+
+
+def test_far_fill_lds_budget_covers_accepted_problems():
+    """Every (precision, m_max, n_xi_efie, n_xi_bracket) that add_problem
+    accepts gets a far-fill launch whose azimuth tables and order stash fit
+    the 160 KiB of LDS; FP64 at m_max = 126 with 2048-sample grids (a default
+    discretisation for k rho_max ~ 110) must fit with one wavefront per group."""
+    lib = pkg("_lib").load()
+    lds = 160 * 1024
+    grids = [64, 128, 256, 512, 1024, 2048, 4096]
+    for prec, rb in ((0, 8), (1, 4)):
+        for m_max in range(0, 127, 5):
+            nt = (m_max + 2 + 15) // 16
+            for ne in grids:
+                for nb in grids:
+                    w = lib.ghost_far_fill_waves(prec, m_max, ne, nb)
+                    assert w in (0, 1, 2, 4)
+                    tab = rb * (2 * nb + 4 * (nb // 2 + 4) + 2 * (ne // 2 + 4) + ne)
+                    stash = rb * max(w, 1) * nt * 8 * 64
+                    if w:
+                        assert tab + stash <= lds, (prec, m_max, ne, nb, w)
+                    else:
+                        assert tab + rb * nt * 8 * 64 > lds, (prec, m_max, ne, nb)
+    assert lib.ghost_far_fill_waves(0, 126, 2048, 2048) == 1
+    assert lib.ghost_far_fill_waves(0, 110, 2048, 2048) == 2
+    for ne in grids:
+        for nb in grids:
+            assert lib.ghost_far_fill_waves(1, 126, ne, nb) >= 1   # FP32 always fits
+
+
+def test_void_compat_samplers_fill_nan_on_failure():
+    """The reference ABI has no error channel: a failing sample_g /
+    sample_mfie / sample_ibc leaves every output NaN (here a null input array,
+    rejected on the host before any device call)."""
+    lib = pkg("_lib").load()
+    dp = ctypes.POINTER(ctypes.c_double)
+    nr, np_, nxi = 2, 3, 8
+    a = np.ones(max(nr, np_, nxi))
+    P = lambda x: x.ctypes.data_as(dp)
+    null = ctypes.cast(None, dp)
+    out = np.zeros((nr, np_, nxi), dtype=np.complex128)
+    lib.sample_g(nr, np_, nxi, null, P(a), P(a), P(a), ctypes.c_double(1.0), P(a), out.ctypes.data_as(dp))
+    assert np.all(np.isnan(out.real)) and np.all(np.isnan(out.imag))
+    for fname in ("sample_mfie", "sample_ibc"):
+        outs = [np.zeros((nr, np_, nxi), dtype=np.complex128) for _ in range(4)]
+        getattr(lib, fname)(nr, np_, nxi, P(a), P(a), P(a), P(a), P(a), null, P(a), P(a),
+                            ctypes.c_double(1.0), P(a), P(a), *[o.ctypes.data_as(dp) for o in outs])
+        for o in outs:
+            assert np.all(np.isnan(o.real)) and np.all(np.isnan(o.imag))
