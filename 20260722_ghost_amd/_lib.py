@@ -75,6 +75,7 @@ SIGNATURES = [
     ("ghost_bor_get_pivots", _ci, [_vp, _ci, _ip]),
     ("ghost_bor_set_lu_super", _ci, [_vp, _ci]),
     ("ghost_far_fill_waves", _ci, [_ci, _ci, _ci, _ci]),
+    ("ghost_sample_mfie_noalloc", _ci, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
 ]
 
 

@@ -2176,6 +2176,83 @@ extern "C" int ghost_sample_ibc(int nr, int np_, int nxi, const double* rho_p, c
 // void-returning names with the exact bor_stream_kernel.c signatures
 // (bor_stream_kernel.c:26-29, 65-72, 108-115): errors surface through
 // ghost_last_error() because the reference ABI has no error channel.
+// ── no-allocation bracket sampler ──────────────────────────────────────────
+// ghost_sample_mfie_noalloc stages its inputs and outputs through `scratch`, a
+// file-scope device array, instead of device buffers allocated per call: for
+// callers that cannot call hipMalloc (tight loops, harnesses that forbid it).
+// Each device holds its own copy of the array at its own address, looked up
+// once per device with hipGetSymbolAddress; host code uses only that address.
+// The sampler overwrites every staged word before reading it, so nothing in
+// the array needs clearing between calls.  Calls that overlap on one device
+// share the array and race.
+static constexpr size_t kScratchBytes = (size_t)48 << 20;   // GHOST_NOALLOC_SCRATCH_BYTES
+static __device__ __align__(256) char scratch[kScratchBytes];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[kMaxDevices];
+
+static int scratch_ptr(char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(GHOST_ERR_ARG, "device index %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(GHOST_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (char*)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" int ghost_sample_mfie_noalloc(int nr, int np_, int nxi, const double* rho_p, const double* z_p,
+                                         const double* tr_p, const double* tz_p, const double* rho_q,
+                                         const double* z_q, const double* tr_q, const double* tz_q, double k,
+                                         const double* cx_tab, const double* sx_tab, double* o_tt, double* o_tf,
+                                         double* o_ft, double* o_ff) {
+  if (nr <= 0 || np_ <= 0 || nxi <= 0) return 0;
+  if (!rho_p || !z_p || !tr_p || !tz_p || !rho_q || !z_q || !tr_q || !tz_q || !cx_tab || !sx_tab || !o_tt || !o_tf ||
+      !o_ft || !o_ff)
+    return fail(GHOST_ERR_ARG, "sample_mfie_noalloc: null array");
+  // staging layout, every piece on a 256-byte boundary: 4 test arrays [nr],
+  // 4 source arrays [np_], 2 azimuth tables [nxi], 4 outputs [nr np_ nxi] complex
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t tot = (size_t)nr * np_ * nxi;
+  const size_t bP = up(8 * (size_t)nr), bQ = up(8 * (size_t)np_), bX = up(8 * (size_t)nxi), bO = up(16 * tot);
+  const size_t need = 4 * bP + 4 * bQ + 2 * bX + 4 * bO;
+  if (need > kScratchBytes)
+    return fail(GHOST_ERR_ARG,
+                "sample_mfie_noalloc: %d x %d x %d samples need %zu bytes of staging, above the %zu-byte scratch array"
+                " (split the rows)",
+                nr, np_, nxi, need, kScratchBytes);
+  char* base = nullptr;
+  if (int rc = scratch_ptr(&base)) return rc;
+  const double* hin[10] = {rho_p, z_p, tr_p, tz_p, rho_q, z_q, tr_q, tz_q, cx_tab, sx_tab};
+  const size_t nin[10] = {(size_t)nr, (size_t)nr, (size_t)nr, (size_t)nr, (size_t)np_, (size_t)np_,
+                          (size_t)np_, (size_t)np_, (size_t)nxi, (size_t)nxi};
+  const double* din[10];
+  size_t off = 0;
+  for (int i = 0; i < 10; ++i) {
+    din[i] = (const double*)(base + off);
+    HIPCHK(hipMemcpy((void*)din[i], hin[i], 8 * nin[i], hipMemcpyHostToDevice));
+    off += i < 4 ? bP : (i < 8 ? bQ : bX);
+  }
+  double2* o[4];
+  for (int i = 0; i < 4; ++i) {
+    o[i] = (double2*)(base + off);
+    off += bO;
+  }
+  const dim3 g((unsigned)std::min<size_t>(65535, (tot + 255) / 256));
+  hipLaunchKernelGGL(k_sample_brackets<false>, g, dim3(256), 0, 0, nr, np_, nxi, din[0], din[1], din[2], din[3],
+                     din[4], din[5], din[6], din[7], k, din[8], din[9], o[0], o[1], o[2], o[3]);
+  LAUNCHCHK();
+  double* outs[4] = {o_tt, o_tf, o_ft, o_ff};
+  for (int i = 0; i < 4; ++i) HIPCHK(hipMemcpy(outs[i], o[i], tot * sizeof(double2), hipMemcpyDeviceToHost));
+  return 0;
+}
+
 // The reference's samplers return nothing, and its loader allocates the
 // outputs with np.empty (bor_streaming.py:287, 327) without ever asking for an
 // error: on any failure the void forms below fill every output with NaN, so

@@ -60,6 +60,22 @@ int ghost_sample_ibc(int nr, int np_, int nxi, const double* rho_p, const double
                      const double* tz_q, double k, const double* cx_tab, const double* sx_tab,
                      double* o_tt, double* o_tf, double* o_ft, double* o_ff);
 
+/* ghost_sample_mfie with no device allocation: the same arguments, meaning
+ * and results (bitwise), but inputs and outputs are staged through a
+ * file-scope device array of GHOST_NOALLOC_SCRATCH_BYTES per device instead of
+ * buffers allocated per call, and the work runs on the device's null stream.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight loops, harnesses that forbid hipMalloc).  Limit: the staged
+ * bytes, 8 (4 nr + 4 np_ + 2 nxi) + 64 nr np_ nxi with each of the 14 pieces
+ * rounded up to 256, must not exceed GHOST_NOALLOC_SCRATCH_BYTES (else
+ * GHOST_ERR_ARG; split the rows).  Calls that overlap on one device share
+ * the array and race. */
+#define GHOST_NOALLOC_SCRATCH_BYTES (48u << 20)
+int ghost_sample_mfie_noalloc(int nr, int np_, int nxi, const double* rho_p, const double* z_p,
+                              const double* tr_p, const double* tz_p, const double* rho_q, const double* z_q,
+                              const double* tr_q, const double* tz_q, double k, const double* cx_tab,
+                              const double* sx_tab, double* o_tt, double* o_tf, double* o_ft, double* o_ff);
+
 /* ── device-resident mode sweep ──────────────────────────────────────────
  * One handle holds a batch of problems (generatrix x frequency), each with
  * its 2 m_max + 1 mode systems, on the device current at creation and on a
