@@ -76,6 +76,7 @@ SIGNATURES = [
     ("ghost_bor_set_lu_super", _ci, [_vp, _ci]),
     ("ghost_far_fill_waves", _ci, [_ci, _ci, _ci, _ci]),
     ("ghost_sample_mfie_noalloc", _ci, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
+    ("ghost_bor_set_impedance", _ci, [_vp, _ci, _dp, _cd]),
 ]
 
 
@@ -177,6 +178,14 @@ class Handle:
         self.problems.append((int(m_max), int(th.size)))
         self.nodes_per_problem.append(int(nodes.shape[0]))
         return idx.value
+
+    def set_impedance(self, prob: int, zs, c_ibc: float = 1.0):
+        """Per-element complex surface impedance of problem prob (before build)."""
+        ne = self.nodes_per_problem[prob] - 1
+        z = np.asarray(zs, dtype=np.complex128)
+        z = np.full(ne, complex(z)) if z.ndim == 0 else z.reshape(ne)
+        zz = np.ascontiguousarray(np.column_stack([z.real, z.imag]).ravel())
+        check(self.lib.ghost_bor_set_impedance(self._h, int(prob), dptr(zz), float(c_ibc)))
 
     def add_dense(self, n: int, nrhs: int) -> int:
         """A caller-filled n x n system with nrhs columns; returns its system index."""

@@ -98,10 +98,11 @@ def gauss_rho_max(pts: np.ndarray, order: int = 4) -> float:
 
 
 def discretisation(pts: np.ndarray, k: float, thetas: np.ndarray, n_modes: Optional[int], form: str,
-                   assembly: str = "auto"):
+                   assembly: str = "auto", has_ibc: bool = False):
     """(m_max, n_xi_efie, n_xi_bracket, streaming) as the reference picks them
     (bor_solver.py:842-847, 870-885; bor_kernels.py:149-152, 315-319;
-    bor_streaming.py:179-183)."""
+    bor_streaming.py:179-183).  has_ibc: the impedance tables count toward
+    the table-size estimate that selects the streamed rule."""
     rho_max = float(np.max(pts[:, 0]))
     if n_modes is None:
         st_max = float(np.max(np.abs(np.sin(np.radians(thetas)))))
@@ -111,7 +112,7 @@ def discretisation(pts: np.ndarray, k: float, thetas: np.ndarray, n_modes: Optio
     if asm not in ("auto", "tables", "streaming"):
         raise ValueError("assembly must be 'auto', 'tables', or 'streaming'.")
     streaming = asm == "streaming" or (asm == "auto" and
-                                       estimate_bor_table_gb(len(pts) - 1, m_max, form) > 2.0)
+                                       estimate_bor_table_gb(len(pts) - 1, m_max, form, has_ibc) > 2.0)
     osc = 2.0 * abs(k) * (rho_max if streaming else gauss_rho_max(pts))
     return m_max, n_xi_efie(osc, m_max), n_xi_bracket(osc, m_max), streaming
 
@@ -140,6 +141,35 @@ def truncate_modes(dF: np.ndarray, m_max: int, mode_tol: float):
 _PRECISION = {"fp64": _lib.GHOST_FP64, "double": _lib.GHOST_FP64, "mixed": _lib.GHOST_MIXED}
 
 
+def impedance_per_element(zs, n_elems: int) -> Optional[np.ndarray]:
+    """Z_s as one complex value per element, or None when the body is PEC
+    (bor_solver.py:850-866: a scalar applies to every element)."""
+    if zs is None:
+        return None
+    z = np.asarray(zs, dtype=np.complex128)
+    if z.ndim == 0:
+        z = np.full(n_elems, complex(z))
+    elif z.shape != (n_elems,):
+        raise ValueError(f"zs needs one complex value per generatrix element ({n_elems}), got shape {z.shape}.")
+    if not np.all(np.isfinite(z)):
+        raise ValueError("zs holds non-finite values.")
+    return None if np.all(z == 0.0) else z
+
+
+def _ibc_warnings(pts: np.ndarray, zs: Optional[np.ndarray], form: str) -> List[str]:
+    """A closed body on the EFIE with a purely reactive Z_s has undamped
+    cavity resonances (the reference's guard, bor_solver.py:944-968)."""
+    if zs is None or form != "efie":
+        return []
+    tol = AXIS_TOL * max(1.0, float(np.max(pts[:, 0])))
+    closed = pts[0, 0] <= tol and pts[-1, 0] <= tol
+    if closed and float(np.max(np.abs(zs.real))) < 1e-3 * float(np.max(np.abs(zs))):
+        return ["Closed body with a lossless (purely reactive) surface impedance on the EFIE: its "
+                "interior resonances are undamped, so results near a resonant frequency may be "
+                "wrong without any sign; add loss to Z_s or check the sweep for spikes."]
+    return []
+
+
 def _coefficients(form: str, cfie_alpha: float):
     alpha = float(cfie_alpha) if form == "cfie" else (1.0 if form == "efie" else 0.0)
     return alpha, (1.0 - alpha) * ETA0
@@ -150,7 +180,7 @@ def _coefficients(form: str, cfie_alpha: float):
 # ─────────────────────────────────────────────────────────────────────────────
 
 def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, prec, refine_iters, times,
-            probe=None):
+            probe=None, warnings=()):
     F, used = truncate_modes(dF, m_max, mode_tol)
     out = {
         "theta_deg": thetas.tolist(),
@@ -165,7 +195,7 @@ def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, pre
         "table_precision": "single" if prec == "mixed" else "double",
         "stream_mode_block": None,
         "stream_sweeps": 0,
-        "warnings": [],
+        "warnings": list(warnings),
         "precision": prec,
         "refine_iters": refine_iters if prec == "mixed" else 0,
         "probe_errors": probe,
@@ -189,8 +219,9 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
     system of every problem goes through the same batched fill, LU/solve and
     far-field launches (systems of uneven size share each launch).  Each job
     is a dict with keys points, freq_hz, thetas_deg and optionally
-    formulation ("cfie"), cfie_alpha (0.5), n_modes, assembly.  Returns one
-    solve_bor-style dict per job.  graph=True captures the batch in a
+    formulation ("cfie"), cfie_alpha (0.5), n_modes, assembly, zs (surface
+    impedance: scalar or one complex value per element, EFIE only).  Returns
+    one solve_bor-style dict per job.  graph=True captures the batch in a
     hipGraph and replays it."""
     prec = str(precision).strip().lower()
     if prec not in _PRECISION:
@@ -203,16 +234,23 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
         pts = _validate_nodes(j["points"])
         thetas = np.atleast_1d(np.asarray(j["thetas_deg"], dtype=float))
         k = 2.0 * math.pi * float(j["freq_hz"]) / C0
+        zs = impedance_per_element(j.get("zs"), len(pts) - 1)
+        if zs is not None and form == "cfie":
+            raise ValueError("A surface impedance needs formulation='efie' (CFIE with Z_s is not "
+                             "defined here; a lossy Z_s already damps interior resonances).")
         m_max, ne, nbk, streaming = discretisation(pts, k, thetas, j.get("n_modes"), form,
-                                                   j.get("assembly", "auto"))
+                                                   j.get("assembly", "auto"), has_ibc=zs is not None)
         cE, cM = _coefficients(form, j.get("cfie_alpha", 0.5))
-        specs.append((pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM))
+        specs.append((pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs))
     if check_abort is not None:
         check_abort()
     h = _lib.Handle(_PRECISION[prec], nb=nb, residual=residual)
     try:
-        for (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM) in specs:
-            h.add_problem(pts, k, m_max, ne, nbk, cE, cM, thetas)
+        for (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs) in specs:
+            idx = h.add_problem(pts, k, m_max, ne, nbk, cE, cM, thetas)
+            if zs is not None:
+                # the impedance operator carries the EFIE's weight (bor_solver.py:929-934)
+                h.set_impedance(idx, zs, cE)
         h.build()
         it = refine_iters if prec == "mixed" else 0
         if graph:
@@ -229,12 +267,12 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
     if check_abort is not None:
         check_abort()
     outs, off = [], 0
-    for i, (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM) in enumerate(specs):
+    for i, (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs) in enumerate(specs):
         nm = 2 * m_max + 1
         r = res[off:off + nm] if res is not None else None
         off += nm
         outs.append(_result(pts, thetas, form, m_max, ne, nbk, streaming, dFs[i], r, mode_tol, prec,
-                            refine_iters, times, probe))
+                            refine_iters, times, probe, _ibc_warnings(pts, zs, form)))
     return outs
 
 
@@ -252,9 +290,10 @@ def solve_bor(points, freq_hz: float, thetas_deg, formulation: str = "efie",
               refine_iters: int = -3, residual: bool = True, nb: int = 64) -> Dict:
     """Monostatic RCS of a PEC body of revolution at aspects thetas_deg (from +z).
 
-    Mirrors bor_solver.solve_bor (bor_solver.py:820-1004) for zs = None.
-    Surface impedance (IBC-EFIE) is not part of this build and raises
-    NotImplementedError.  ``workers``, ``table_precision`` and
+    Mirrors bor_solver.solve_bor (bor_solver.py:820-1004): PEC bodies
+    (zs = None) on EFIE / MFIE / CFIE, and impedance bodies (zs a complex
+    scalar or one value per element, IBC-EFIE) on the EFIE.  ``workers``,
+    ``table_precision`` and
     ``stream_budget_gb`` are accepted for signature compatibility: the GPU
     path holds every table in FP64 (or FP32 far blocks under
     precision="mixed") and needs no host threads or re-sweeps.
@@ -262,8 +301,6 @@ def solve_bor(points, freq_hz: float, thetas_deg, formulation: str = "efie",
     form = formulation.strip().lower()
     if form not in ("efie", "mfie", "cfie"):
         raise ValueError("formulation must be 'efie', 'mfie' or 'cfie'.")
-    if zs is not None and np.any(np.abs(np.asarray(zs, dtype=complex)) != 0.0):
-        raise NotImplementedError("Surface impedance (IBC-EFIE) is not part of the GPU BoR path.")
     if int(gauss_order) != 4:
         raise ValueError("the GPU BoR path is built for gauss_order = 4 (the reference default).")
     tp = str(table_precision).strip().lower()
@@ -271,7 +308,7 @@ def solve_bor(points, freq_hz: float, thetas_deg, formulation: str = "efie",
         raise ValueError("table_precision must be 'auto', 'single', or 'double'.")
     out = solve_bor_batch([{"points": points, "freq_hz": freq_hz, "thetas_deg": thetas_deg,
                             "formulation": form, "cfie_alpha": cfie_alpha, "n_modes": n_modes,
-                            "assembly": assembly}],
+                            "assembly": assembly, "zs": zs}],
                           precision=precision, refine_iters=refine_iters, residual=residual, nb=nb,
                           mode_tol=mode_tol, check_abort=check_abort)[0]
     if progress is not None:
@@ -391,9 +428,9 @@ class BorPecSolver:
     |m| <= m_max of one formulation in one batch (unmasked node-based
     [2Nn, 2Nn] systems, cached per m_max), rhs_* run the excitation kernels
     for one aspect, farfield_mode runs the far-field kernels on a caller's
-    solution.  Penetrable media (medium=..., assemble_pmchwt_P) and surface
-    impedance (assemble_ibc_extra, zs_pt) are outside this build and raise
-    NotImplementedError."""
+    solution (with the impedance body's radiating magnetic current when
+    zs_pt is given).  Penetrable media (medium=..., assemble_pmchwt_P,
+    msol) are outside this build and raise NotImplementedError."""
 
     def __init__(self, points, freq_hz: float, gauss_order: int = 4,
                  near_depth: int = 4, medium=None, single_tables: bool = False):
@@ -417,14 +454,12 @@ class BorPecSolver:
     def enable_streaming(self, m_max: int, efie: bool = True, mfie: bool = False,
                          ibc_zs_pt=None, single_blocks: bool = False, tile_budget_gb: float = 1.0,
                          workers: int = 1, mode_block=None) -> None:
-        if ibc_zs_pt is not None:
-            raise NotImplementedError("Surface impedance (IBC-EFIE) is not part of the GPU BoR path.")
         self._m_max = max(int(m_max), self._m_max or 0)
 
     def prepare_operators(self, m_max: int, efie: bool = True, mfie: bool = False,
                           ibc: bool = False, workers: int = 1) -> None:
-        if ibc:
-            raise NotImplementedError("Surface impedance (IBC-EFIE) is not part of the GPU BoR path.")
+        """The impedance operator depends on Z_s, so it is filled at
+        assemble_ibc_extra time."""
         if efie:
             self._systems("efie", m_max)
         if mfie:
@@ -434,14 +469,21 @@ class BorPecSolver:
         osc = 2.0 * abs(self.k) * gauss_rho_max(self.pts)
         return n_xi_efie(osc, m_max), n_xi_bracket(osc, m_max)
 
-    def _systems(self, kind: str, m_max: int) -> Dict[int, np.ndarray]:
-        key = (kind, int(m_max))
+    def _elem_zs(self, zs_pt) -> np.ndarray:
+        """Per-element Z_s from the per-Gauss-point array (constant on each element)."""
+        z = np.asarray(zs_pt, dtype=np.complex128).reshape(self.Nn - 1, self.gauss_order)
+        return z[:, 0].copy()
+
+    def _systems(self, kind: str, m_max: int, zs_elem=None) -> Dict[int, np.ndarray]:
+        key = (kind, int(m_max)) if zs_elem is None else (kind, int(m_max), np.asarray(zs_elem).tobytes())
         if key not in self._ops:
             ne, nb = self._n_xi(m_max)
-            cE, cM = (1.0, 0.0) if kind == "efie" else (0.0, 1.0)
+            cE, cM = {"efie": (1.0, 0.0), "mfie": (0.0, 1.0), "ibc": (0.0, 0.0)}[kind]
             h = _lib.Handle(_lib.GHOST_FP64, residual=False)
             try:
                 h.add_problem(self.pts, self.k, m_max, ne, nb, cE, cM, [0.0], unmasked=True)
+                if kind == "ibc":
+                    h.set_impedance(0, zs_elem, 1.0)
                 h.build()
                 h.fill()
                 h.sync()
@@ -458,8 +500,15 @@ class BorPecSolver:
         """Z_MFIE = (1/2) M - K [2Nn, 2Nn] (bor_solver.py:399-435)."""
         return self._systems("mfie", m_max)[int(m)].copy()
 
-    def assemble_ibc_extra(self, *args, **kwargs):
-        raise NotImplementedError("Surface impedance (IBC-EFIE) is not part of the GPU BoR path.")
+    def assemble_ibc_extra(self, m: int, m_max: int, zs_pt, zs_elem) -> np.ndarray:
+        """IBC-EFIE additions (1/2) <W, Z_s J> + <W, PV curl int G M>,
+        M = -Z_s n x J, Z_s at the source [2Nn, 2Nn] (bor_solver.py:534-552).
+        zs_pt (per Gauss point) weights the mass and the far brackets, zs_elem
+        the near brackets; both come from one per-element impedance here."""
+        ze = np.asarray(zs_elem, dtype=np.complex128).reshape(self.Nn - 1)
+        if not np.array_equal(self._elem_zs(zs_pt), ze):
+            raise ValueError("zs_pt must repeat zs_elem over each element's Gauss points.")
+        return self._systems("ibc", m_max, ze)[int(m)].copy()
 
     def assemble_pmchwt_P(self, *args, **kwargs):
         raise NotImplementedError("penetrable media (PMCHWT) are not part of the GPU BoR path.")
@@ -531,14 +580,17 @@ class BorPecSolver:
         return self.rhs_mode(m, theta_inc_deg, "VV") / ETA0
 
     def farfield_mode(self, m: int, sol: np.ndarray, theta_s_deg: float, zs_pt=None, msol=None):
-        """(F_theta, F_phi) of one mode's full [2Nn] solution (bor_solver.py:659-708)."""
-        if zs_pt is not None or msol is not None:
-            raise NotImplementedError("magnetic currents (IBC / PMCHWT) are not part of the GPU BoR path.")
+        """(F_theta, F_phi) of one mode's full [2Nn] solution (bor_solver.py:659-708);
+        zs_pt adds the radiation of M = -Z_s n x J."""
+        if msol is not None:
+            raise NotImplementedError("independent magnetic currents (PMCHWT) are not part of the GPU BoR path.")
         sol = np.asarray(sol, dtype=np.complex128).reshape(2 * self.Nn)
         mm = max(abs(int(m)), 1)
         h = _lib.Handle(_lib.GHOST_FP64, residual=False, pairing=False)   # one mode's own currents
         try:
             h.add_problem(self.pts, self.k, mm, 64, 128, 1.0, 0.0, [float(theta_s_deg)], unmasked=True)
+            if zs_pt is not None:
+                h.set_impedance(0, self._elem_zs(zs_pt), 0.0)   # far field only
             h.build()
             h.fill_rhs()   # the far field reads the Bessel table of the aspect
             h.set_solution(0, int(m), np.stack([sol, sol], axis=1))

@@ -152,6 +152,12 @@ struct Problem {
   int Nn = 0, Ne = 0, P = 0, M = 0, nE = 0, nB = 0, ntheta = 0, nrhs = 0, nmodes = 0;
   int zmodes = 0, zoff = 0;      // far-block mode stride and index of mode 0
   double k = 0, cE = 0, cM = 0;
+  // surface impedance (IBC-EFIE): per-element Z_s, the coefficient of the
+  // impedance operator, and whether any Z_s is nonzero
+  std::vector<double2> zs;
+  double cI = 0;
+  bool has_zs = false;
+  double2* d_zs = nullptr;
   std::vector<double> nodes, theta;
   bool axis0 = false, axis1 = false;
   int sys0 = 0;
@@ -423,15 +429,18 @@ extern "C" int ghost_far_fill_waves(int precision, int m_max, int n_xi_efie, int
   return far_fill_waves(precision == GHOST_MIXED ? 4 : 8, (m_max + 2 + 15) / 16, n_xi_efie, n_xi_bracket);
 }
 template <typename R, int NT>
-static void launch_far_fill(bool split, int wpb, int grid, size_t tab_bytes, hipStream_t s, const FarArgs& fa) {
+static void launch_far_fill(bool split, bool ibc, int wpb, int grid, size_t tab_bytes, hipStream_t s,
+                            const FarArgs& fa) {
   if constexpr (sizeof(R) == 8) {
     if (wpb == 1) {
-      if (split) hipLaunchKernelGGL((k_far_fill<R, NT, true, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
-      else hipLaunchKernelGGL((k_far_fill<R, NT, false, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
+      if (split) hipLaunchKernelGGL((k_far_fill<R, NT, true, false, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
+      else if (ibc) hipLaunchKernelGGL((k_far_fill<R, NT, false, true, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
+      else hipLaunchKernelGGL((k_far_fill<R, NT, false, false, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
       return;
     }
   }
   if (split) hipLaunchKernelGGL((k_far_fill<R, NT, true>), dim3(grid), dim3(64 * wpb), tab_bytes, s, fa);
+  else if (ibc) hipLaunchKernelGGL((k_far_fill<R, NT, false, true>), dim3(grid), dim3(64 * wpb), tab_bytes, s, fa);
   else hipLaunchKernelGGL((k_far_fill<R, NT, false>), dim3(grid), dim3(64 * wpb), tab_bytes, s, fa);
 }
 
@@ -508,6 +517,30 @@ extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, 
   }
   if (idx) *idx = (int)h->probs.size();
   h->probs.push_back(std::move(p));
+  return 0;
+}
+
+// Surface impedance of one problem (IBC-EFIE, bor_solver.py:850-866,
+// 929-934): zs complex per element; Z += c_ibc ((1/2) M_Zs + K'_Zs).
+extern "C" int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs, double c_ibc) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "set the surface impedance before build");
+  if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index %d", prob);
+  if (!zs) return fail(GHOST_ERR_ARG, "null impedance array");
+  Problem& p = h->probs[prob];
+  std::vector<double2> z(p.Ne);
+  bool any = false;
+  for (int e = 0; e < p.Ne; ++e) {
+    z[e] = make_double2(zs[2 * e], zs[2 * e + 1]);
+    if (!std::isfinite(z[e].x) || !std::isfinite(z[e].y))
+      return fail(GHOST_ERR_ARG, "surface impedance of element %d is not finite", e);
+    any = any || z[e].x != 0.0 || z[e].y != 0.0;
+  }
+  if (any && c_ibc != 0.0 && p.cM != 0.0)
+    return fail(GHOST_ERR_ARG, "a surface impedance needs c_mfie = 0 (the impedance operator takes the MFIE's place)");
+  p.zs = z;
+  p.has_zs = any;
+  p.cI = any ? c_ibc : 0.0;
   return 0;
 }
 
@@ -685,6 +718,10 @@ static int prepare_problem(ghost_bor* h, Problem& p) {
   const size_t zsz = h->precision == GHOST_FP64 ? sizeof(double2) : sizeof(float2);
   if ((rc = dalloc(h, &p.d_zfar, zel * zsz))) return rc;
 
+  if (p.has_zs) {
+    if ((rc = dalloc(h, &p.d_zs, (size_t)Ne * sizeof(double2)))) return rc;
+    HIPCHK(hipMemcpy(p.d_zs, p.zs.data(), (size_t)Ne * sizeof(double2), hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMemcpy(p.d_pts, p.pts.data(), p.pts.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.d_nodes, p.nodes.data(), p.nodes.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p.d_elem, elem.data(), elem.size() * 8, hipMemcpyHostToDevice));
@@ -1256,6 +1293,10 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
   fa.ce_re = 0.0;
   fa.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
   fa.cm = -2.0 * M_PI * p.cM;
+  const bool ibc = p.has_zs && p.cI != 0.0;
+  fa.zs = p.d_zs;
+  fa.cz = 2.0 * M_PI * p.cI;
+  if (ibc && split) return fail(GHOST_ERR_ARG, "perturbed (adjoint) fills of an impedance body are not supported");
   fa.Z = p.d_zfar;
   fa.neg = h->pairing ? 0 : 1;
   fa.zmodes = p.zmodes;
@@ -1294,7 +1335,7 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
       switch (NT) {
 #define FARCASE(n)                                                                                      \
   case n:                                                                                               \
-    launch_far_fill<R, n>(split, wpb, grid, tab_bytes, h->stream, fa);                                   \
+    launch_far_fill<R, n>(split, ibc, wpb, grid, tab_bytes, h->stream, fa);                              \
     break;
         FARCASE(1) FARCASE(2) FARCASE(3) FARCASE(4) FARCASE(5) FARCASE(6) FARCASE(7) FARCASE(8)
 #undef FARCASE
@@ -1321,6 +1362,7 @@ static ExciteArgs excite_args(const Problem& p, const Geom& tg) {
   ea.theta = p.d_theta;
   ea.J = p.d_J;
   ea.E = p.d_E;
+  ea.zs = p.has_zs ? p.d_zs : nullptr;
   return ea;
 }
 
@@ -1369,6 +1411,7 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   na.Ne = p.Ne;
   na.M = p.M;
   na.nord = p.M + 2;
+  na.ibc = (p.has_zs && p.cI != 0.0) ? 1 : 0;
   na.k = p.k;
   na.NP = p.d_NP;
   if (!p.batches.empty()) {
@@ -1394,6 +1437,8 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   nb.ce_re = 0.0;
   nb.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
   nb.cM = p.cM;
+  nb.zs = p.d_zs;
+  nb.cI = p.cI;
   nb.NB = p.d_NB;
   const int mlanes = ((p.M + 1 + 15) / 16) * 16;
   if (mlanes > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near blocks");
@@ -2341,6 +2386,8 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
     if (s.ncols != s.nrhs) return fail(GHOST_ERR_STATE, "the adjoint needs a handle built with mode pairing off");
   }
   if (h->precision != GHOST_FP64) return fail(GHOST_ERR_STATE, "the adjoint needs an FP64 handle");
+  for (auto& p : h->probs)
+    if (p.has_zs) return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies only (no surface impedance)");
   if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
   if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
   HIPCHK(hipSetDevice(h->device));

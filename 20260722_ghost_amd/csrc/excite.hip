@@ -20,6 +20,8 @@ struct ExciteArgs {
   double2* E;             // [P][ntheta] w_p rho_p e^{j k cos(theta) z_p}: the plane-wave phase of a
                           // point times its quadrature weight, shared by the right-hand sides and the
                           // far-field projection (both k_bessel outputs follow the same test geometry)
+  const double2* zs;      // per-element surface impedance, or null: the magnetic current
+                          // M = -Z_s n x J radiates too (bor_solver.py:697-707)
 };
 
 // J_n(x), n = 0..nmax written to out[n * stride].
@@ -323,6 +325,24 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
       // Jt tr Isin + Jf Icos
       term = make_double2(tr * (Jt.x * Isin.x - Jt.y * Isin.y) + Jf.x * Icos.x - Jf.y * Icos.y,
                           tr * (Jt.x * Isin.y + Jt.y * Isin.x) + Jf.x * Icos.y + Jf.y * Icos.x);
+    }
+    if (a.zs) {
+      // M_t = Z_s J_phi, M_phi = -Z_s J_t; with the common -j k eta0 / (4 pi):
+      // F_theta += (1/eta0) proj_phi(M) = (Z_s/eta0) (Jf tr Isin - Jt Icos),
+      // F_phi   -= (1/eta0) proj_theta(M) = -(Z_s/eta0) (Jf u + Jt ct Isin)
+      const double2 z = a.zs[e];
+      const double ie = 1.0 / 376.730313668;
+      double2 mt;
+      if (pol == 0) {
+        mt = make_double2(tr * (Jf.x * Isin.x - Jf.y * Isin.y) - (Jt.x * Icos.x - Jt.y * Icos.y),
+                          tr * (Jf.x * Isin.y + Jf.y * Isin.x) - (Jt.x * Icos.y + Jt.y * Icos.x));
+      } else {
+        const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
+        mt = make_double2(-(Jf.x * u.x - Jf.y * u.y) - ct * (Jt.x * Isin.x - Jt.y * Isin.y),
+                          -(Jf.x * u.y + Jf.y * u.x) - ct * (Jt.x * Isin.y + Jt.y * Isin.x));
+      }
+      term.x += ie * (z.x * mt.x - z.y * mt.y);
+      term.y += ie * (z.x * mt.y + z.y * mt.x);
     }
     const double2 E = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
     acc.x += term.x * E.x - term.y * E.y;

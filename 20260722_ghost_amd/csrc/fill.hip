@@ -41,6 +41,8 @@ struct FarArgs {
   int delta, imax;       // unordered pairs (not SPLIT): f = pf + 2 j with j >= i + delta, rows i < imax
   int zmodes, zoff;      // Z mode stride and the index of mode 0 (multiples of 16; zoff = 0 with pairing)
   double k, ce_re, ce_im, cm;
+  const double2* zs;     // IBC: per-element surface impedance (source element's applies)
+  double cz;             // IBC: bracket coefficient 2 pi c_ibc (times zs of the source element)
   void* Z;               // Zfar [2Nn][2Nn][zmodes] complex<R>
 };
 
@@ -97,8 +99,11 @@ __host__ __device__ constexpr int far_waves(int bytes, int nt) { return far_wide
 // Unordered pairs run phases 3 and 4 twice, for (e, f) and then (f, e); the
 // second phase 4 takes its EFIE blocks from the same stash by lane exchanges.
 // WPB = 1 is the fallback for FP64 azimuth tables too large to sit beside a
-// two-wavefront stash in LDS (n_xi = 2048 with 7-8 order tiles).
-template <typename R, int NT, bool SPLIT, int WPB = far_waves(sizeof(R), NT)>
+// two-wavefront stash in LDS (n_xi = 2048 with 7-8 order tiles).  IBC: the
+// bracket phase samples the rotated-PV brackets of the impedance operator
+// (bor_kernels.py:449-478) in place of the MFIE ones, with the coefficient
+// 2 pi c_ibc Z_s(source element) (bor_solver.py:497-518).
+template <typename R, int NT, bool SPLIT, bool IBC = false, int WPB = far_waves(sizeof(R), NT)>
 __global__ __launch_bounds__(64 * WPB)
 __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4 || NT <= 2) ? 2 : 1))) void k_far_fill(FarArgs a) {
   typedef typename CT<R>::c C2;
@@ -188,6 +193,12 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
   //    register a for tt / ff, and of the other half (ft <-> tf), negated ──
   auto write_z = [&](int te, int se, const V4* kE, const V4* kO, bool transposed) {
   const R ce_re = (R)a.ce_re, ce_im = (R)a.ce_im, cm = (R)a.cm;
+  R cmi = (R)0, cmr = cm;
+  if constexpr (IBC) {
+    const double2 z = a.zs[se];
+    cmr = (R)(a.cz * z.x);
+    cmi = (R)(a.cz * z.y);
+  }
   const size_t ncol = 2 * (size_t)a.Nn;
   C2* Z = (C2*)a.Z;
   const int rowE = (uvsub ? a.Nn : 0) + te + asel;   // tt / ff, and tf / ft rows
@@ -215,10 +226,18 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       const R orr = ov[2 * bi], oi = ov[2 * bi + 1];
       const R Ker = kE[t][2 * bi], Kei = kE[t][2 * bi + 1];
       const R Kor = kO[t][2 * bi + 1], Koi = -kO[t][2 * bi];  // -j (re + j im)
-      const R zer = ce_re * er - ce_im * ei + cm * Ker;
-      const R zei = ce_re * ei + ce_im * er + cm * Kei;
-      const R zor = ce_re * orr - ce_im * oi + cm * Kor;
-      const R zoi = ce_re * oi + ce_im * orr + cm * Koi;
+      R zer, zei, zor, zoi;
+      if constexpr (IBC) {
+        zer = ce_re * er - ce_im * ei + (cmr * Ker - cmi * Kei);
+        zei = ce_re * ei + ce_im * er + (cmr * Kei + cmi * Ker);
+        zor = ce_re * orr - ce_im * oi + (cmr * Kor - cmi * Koi);
+        zoi = ce_re * oi + ce_im * orr + (cmr * Koi + cmi * Kor);
+      } else {
+        zer = ce_re * er - ce_im * ei + cm * Ker;
+        zei = ce_re * ei + ce_im * er + cm * Kei;
+        zor = ce_re * orr - ce_im * oi + cm * Kor;
+        zoi = ce_re * oi + ce_im * orr + cm * Koi;
+      }
       C2* pe_ = Z + ((size_t)rowE * ncol + colE + bi) * a.zmodes + a.zoff;
       C2* po_ = Z + ((size_t)rowE * ncol + colO + bi) * a.zmodes + a.zoff;
       C2 v;
@@ -401,12 +420,27 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       const R pre = smp[2] * inv * inv * inv;
       const R kr = k * Rd;
       const R pr = pre * (c + kr * s), pim = pre * (kr * c - s);
-      const R WtR = trp * Rx + tzp * dz, nR = -tzp * Rx + trp * dz;
-      const R n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
       const R Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
       // brackets x4 = tt, ff (even in xi), tf, ft (odd), times (re, im)
-      const R bx[4] = {-(WtR * n_t - Wt_t * nR), -(Ry * n_f - cx * nR), -(WtR * n_f - Wt_f * nR),
-                       -(Ry * n_t - Wf_t * nR)};
+      R bx[4];
+      if constexpr (IBC) {
+        // p(R) [(W_u . n_q)(R . f_v) - (W_u . f_v)(R . n_q)], source normal n_q
+        const R D = rp * cx - rq;
+        const R Wt_n = -trp * tzq * cx + tzp * trq, Wf_n = tzq * sx;
+        const R R_t = trq * D + tzq * dz, R_f = rp * sx, R_n = -tzq * D + trq * dz;
+        bx[0] = Wt_n * R_t - Wt_t * R_n;
+        bx[1] = Wf_n * R_f - cx * R_n;
+        bx[2] = Wt_n * R_f - Wt_f * R_n;
+        bx[3] = Wf_n * R_t - Wf_t * R_n;
+      } else {
+        // -p(R) [(W_u . R)(n_p . f_v) - (W_u . f_v)(n_p . R)], test normal n_p
+        const R WtR = trp * Rx + tzp * dz, nR = -tzp * Rx + trp * dz;
+        const R n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
+        bx[0] = -(WtR * n_t - Wt_t * nR);
+        bx[1] = -(Ry * n_f - cx * nR);
+        bx[2] = -(WtR * n_f - Wt_f * nR);
+        bx[3] = -(Ry * n_t - Wf_t * nR);
+      }
       R X[4][2];
 #pragma unroll
       for (int x = 0; x < 4; ++x) { X[x][0] = bx[x] * pr; X[x][1] = bx[x] * pim; }
@@ -491,6 +525,7 @@ struct NearArgs {
   const NearPairInfo* pairs;
   const int2* batches;    // (pair slot, first cell point) per work-group
   int Ne, M, nord;        // nord = M + 2 orders stored per point pair
+  int ibc;                // brackets: 0 MFIE, 1 rotated-PV (IBC) (bor_kernels.py:515-560)
   double k;
   double2* NP;            // [pp][nord][5] : G, Ktt, Ktf, Kft, Kff
 };
@@ -614,11 +649,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
       const double kr = k * Rd;
       const double pr_ = pre * (ck + kr * sk), pi_ = pre * (kr * ck - sk);
       const double Rx = rp - rq * cx, Ry = rq * sx;
-      const double WtR = trp * Rx + tzp * Rz, nR = -tzp * Rx + trp * Rz;
-      const double n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
       const double Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
-      const double btt = -(WtR * n_t - Wt_t * nR), btf = -(WtR * n_f - Wt_f * nR);
-      const double bft = -(Ry * n_t - Wf_t * nR), bff = -(Ry * n_f - cx * nR);
+      double btt, btf, bft, bff;
+      if (a.ibc) {
+        const double D = rp * cx - rq;
+        const double Wt_n = -trp * tzq * cx + tzp * trq, Wf_n = tzq * sx;
+        const double R_t = trq * D + tzq * Rz, R_f = rp * sx, R_n = -tzq * D + trq * Rz;
+        btt = Wt_n * R_t - Wt_t * R_n;
+        btf = Wt_n * R_f - Wt_f * R_n;
+        bft = Wf_n * R_t - Wf_t * R_n;
+        bff = Wf_n * R_f - cx * R_n;
+      } else {
+        const double WtR = trp * Rx + tzp * Rz, nR = -tzp * Rx + trp * Rz;
+        const double n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
+        btt = -(WtR * n_t - Wt_t * nR);
+        btf = -(WtR * n_f - Wt_f * nR);
+        bft = -(Ry * n_t - Wf_t * nR);
+        bff = -(Ry * n_f - cx * nR);
+      }
       out[2] = btt * pr_; out[3] = btt * pi_;
       out[4] = btf * pr_; out[5] = btf * pi_;
       out[6] = bft * pr_; out[7] = bft * pi_;
@@ -730,7 +778,9 @@ struct NearBlockArgs {
   const double* gT0;
   const double* gT1;
   double ce_re, ce_im;   // EFIE coefficient
-  double cM;             // MFIE coefficient
+  double cM;             // MFIE coefficient (brackets of NearArgs::ibc == 0)
+  const double2* zs;     // IBC (NearArgs::ibc == 1): per-element surface impedance
+  double cI;             // IBC coefficient: Z += cI ((1/2) Z_s mass + brackets x Z_s(source element))
   double2* NB;           // [3 Ne][16][2M+1]
 };
 
@@ -831,20 +881,29 @@ __global__ __launch_bounds__(256) void k_near_blocks(NearBlockArgs b, int mlanes
   }
   const int nmodes = 2 * a.M + 1;
   double2* out = b.NB + (size_t)slot * 16 * nmodes;
+  // bracket coefficient kc and mass coefficient mc: MFIE Z_M = (1/2) M - K
+  // (bor_solver.py:428-435) with c_mfie; IBC (1/2) M_Zs + K'_Zs with c_ibc,
+  // Z_s of the source element on the brackets and of the element itself on
+  // the mass (bor_solver.py:519-552)
+  double2 kc = make_double2(-b.cM, 0.0), mc = make_double2(0.5 * b.cM, 0.0);
+  if (a.ibc) {
+    kc = make_double2(b.cI * b.zs[f].x, b.cI * b.zs[f].y);
+    mc = make_double2(0.5 * b.cI * b.zs[e].x, 0.5 * b.cI * b.zs[e].y);
+  }
   for (int blk = 0; blk < 4; ++blk)
     for (int ab = 0; ab < 4; ++ab) {
       const double2 Ev = E[4 * blk + ab], Kv = K[4 * blk + ab];
-      double zr = b.ce_re * Ev.x - b.ce_im * Ev.y - b.cM * Kv.x;
-      double zi = b.ce_re * Ev.y + b.ce_im * Ev.x - b.cM * Kv.y;
-      if (blk == 0 || blk == 3) zr += 0.5 * b.cM * Mab[ab];
+      const double er = b.ce_re * Ev.x - b.ce_im * Ev.y + (kc.x * Kv.x - kc.y * Kv.y);
+      const double ei = b.ce_re * Ev.y + b.ce_im * Ev.x + (kc.x * Kv.y + kc.y * Kv.x);
+      const bool odd = (blk == 1 || blk == 2);
+      double zr = er, zi = ei;
+      if (!odd) { zr += mc.x * Mab[ab]; zi += mc.y * Mab[ab]; }
       double2* o = out + (size_t)(4 * blk + ab) * nmodes + a.M;
       o[am] = make_double2(zr, zi);
       if (am > 0) {
-        const bool odd = (blk == 1 || blk == 2);
         const double sg = odd ? -1.0 : 1.0;
-        double zr2 = sg * (b.ce_re * Ev.x - b.ce_im * Ev.y - b.cM * Kv.x);
-        double zi2 = sg * (b.ce_re * Ev.y + b.ce_im * Ev.x - b.cM * Kv.y);
-        if (!odd) zr2 += 0.5 * b.cM * Mab[ab];
+        double zr2 = sg * er, zi2 = sg * ei;
+        if (!odd) { zr2 += mc.x * Mab[ab]; zi2 += mc.y * Mab[ab]; }
         o[-am] = make_double2(zr2, zi2);
       }
     }

@@ -113,6 +113,17 @@ int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, double k, i
                           int n_xi_bracket, double c_efie, double c_mfie, const double* thetas_deg,
                           int n_theta, int flags, int* idx);
 
+/* Surface impedance of problem prob (before build): zs complex [Ne]
+ * (interleaved), one Leontovich Z_s (ohm) per element.  Adds the IBC-EFIE
+ * operator c_ibc ((1/2) <W, Z_s J> + <W, PV curl int G M>) with the
+ * eliminated magnetic current M = -Z_s n x J (bor_solver.py:463-552;
+ * solve_bor's assemble closure, bor_solver.py:929-934, uses c_ibc = alpha),
+ * and the far field radiates M as well (bor_solver.py:697-707).  The
+ * impedance operator takes the MFIE brackets' place in the fill: c_mfie must
+ * be 0 when c_ibc != 0 (the reference refuses CFIE with Z_s, bor_solver.py:
+ * 867-869).  All-zero zs leaves the problem a PEC one. */
+int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs, double c_ibc);
+
 /* Geometry tables, device buffers and the launch schedule. */
 int ghost_bor_build(ghost_bor* h);
 

@@ -478,7 +478,11 @@ class FarBlocks:
     per mode [4, 2m_max+1, Nn, Nn] from far point pairs (|e-f| >= 2)."""
 
     def __init__(self, body: Body, k: float, m_max: int, n_e: int, n_b: int,
-                 efie: bool = True, mfie: bool = True, rows_per_tile: int = 8):
+                 efie: bool = True, mfie: bool = True, rows_per_tile: int = 8,
+                 zs_pt: Optional[np.ndarray] = None):
+        """zs_pt (per Gauss point, complex): also the IBC rotated-PV bracket
+        blocks with Z_s applied at the source point (bor_solver.py:463-478,
+        508-518; bor_streaming.py:153-155, 255-258)."""
         g = body.g
         Ne, Nn, go = body.Ne, body.Nn, body.order
         lv = nodal_weights(g)
@@ -486,6 +490,10 @@ class FarBlocks:
         self.k = k
         self.E = np.zeros((9, m_max + 2, Nn, Nn), dtype=np.complex128) if efie else None
         self.K = np.zeros((4, 2 * m_max + 1, Nn, Nn), dtype=np.complex128) if mfie else None
+        self.KI = None
+        if zs_pt is not None:
+            self.KI = np.zeros((4, 2 * m_max + 1, Nn, Nn), dtype=np.complex128)
+            rw_ibc = lv["1"] * np.asarray(zs_pt, dtype=np.complex128)[None, :]
         xe, xb = xi_grid(n_e), xi_grid(n_b)
         for e0 in range(0, Ne, rows_per_tile):
             ne_rows = min(rows_per_tile, Ne - e0)
@@ -505,6 +513,13 @@ class FarBlocks:
                     Km = bracket_modes_fft(Fs[uv], m_max) * (2.0 * np.pi)
                     _zero_near_rows(Km, e0, ne_rows, Ne, go)
                     _contract(Km, lv["1"][:, rows], lv["1"], e0, ne_rows, Ne, go, self.K[uv])
+            if self.KI is not None:
+                Fs = ibc_bracket_samples(P_(g.rho), P_(g.z), P_(g.trho), P_(g.tz),
+                                         Q_(g.rho), Q_(g.z), Q_(g.trho), Q_(g.tz), k, xb)
+                for uv in range(4):
+                    Km = bracket_modes_fft(Fs[uv], m_max) * (2.0 * np.pi)
+                    _zero_near_rows(Km, e0, ne_rows, Ne, go)
+                    _contract(Km, lv["1"][:, rows], rw_ibc, e0, ne_rows, Ne, go, self.KI[uv])
 
     def efie(self, m: int):
         """EFIE blocks (no C factor) for mode m (bor_streaming.py:388-407)."""
@@ -522,6 +537,11 @@ class FarBlocks:
     def mfie(self, m: int):
         i = m + self.m_max
         return tuple(self.K[uv, i].copy() for uv in range(4))
+
+    def ibc(self, m: int):
+        """IBC bracket blocks (tt, tf, ft, ff) of mode m, Z_s at the source, 2 pi included."""
+        i = m + self.m_max
+        return tuple(self.KI[uv, i].copy() for uv in range(4))
 
 
 def table_efie_blocks(body: Body, k: float, m: int, G_table: np.ndarray):
@@ -573,10 +593,13 @@ class NearBlocks:
     mfie[(e,f)] -> [4, 2m_max+1, 2, 2] (2 pi factor included)."""
 
     def __init__(self, body: Body, k: float, m_max: int, efie: bool = True,
-                 mfie: bool = True):
+                 mfie: bool = True, zs_elem: Optional[np.ndarray] = None):
+        """zs_elem (per element, complex): also ibc[(e,f)], the IBC brackets
+        with the source element's Z_s (bor_solver.py:480-495, 519-530)."""
         self.m_max = m_max
         self.efie: Dict[Tuple[int, int], np.ndarray] = {}
         self.mfie: Dict[Tuple[int, int], np.ndarray] = {}
+        self.ibc: Dict[Tuple[int, int], np.ndarray] = {}
         ms = np.arange(-m_max, m_max + 1)
         for (e, f) in near_pairs(body.Ne):
             s, sp, w = cell_points(near_kind(e, f))
@@ -613,6 +636,14 @@ class NearBlocks:
                 for uv in range(4):
                     blk[uv] = 2.0 * np.pi * np.einsum("ip,pm,jp->mij", Tp, rr[:, None] * Kn[uv], Tq)
                 self.mfie[(e, f)] = blk
+            if zs_elem is not None and abs(zs_elem[f]) != 0.0:
+                Kn = bracket_modes_near(p["rho"], p["z"], p["trho"], p["tz"],
+                                        q["rho"], q["z"], q["trho"], q["tz"], k, m_max, ibc=True)
+                blk = np.zeros((4, len(ms), 2, 2), dtype=np.complex128)
+                for uv in range(4):
+                    blk[uv] = 2.0 * np.pi * np.einsum("ip,pm,jp->mij", Tp,
+                                                      (rr * zs_elem[f])[:, None] * Kn[uv], Tq)
+                self.ibc[(e, f)] = blk
 
 
 # ─────────────────────────────────────────────────────────────────────────────
@@ -674,10 +705,13 @@ def rhs_mfie(body: Body, k: float, m: int, theta_deg: float, pol: str) -> np.nda
     return np.concatenate([BT @ (base * et), BT @ (base * ef)])
 
 
-def farfield(body: Body, k: float, m: int, sol: np.ndarray, theta_deg: float):
+def farfield(body: Body, k: float, m: int, sol: np.ndarray, theta_deg: float,
+             zs_pt: Optional[np.ndarray] = None):
     """Modal (F_theta, F_phi) of the full-length solution vector
     (bor_solver.py:659-696; the far-field pairs (j_{m+1}, j_{m-1}) are
-    reversed relative to the excitation)."""
+    reversed relative to the excitation).  zs_pt: the eliminated magnetic
+    current M = -Z_s n x J also radiates, M_t = Z_s J_phi, M_phi = -Z_s J_t
+    (bor_solver.py:697-707)."""
     g = body.g
     st, ct, ph, J0, Jm1, Jp1 = _bessel_terms(k, g, m, theta_deg)
     Icos = math.pi * (Jp1 + Jm1)
@@ -689,16 +723,24 @@ def farfield(body: Body, k: float, m: int, sol: np.ndarray, theta_deg: float):
     Jf = BT.T @ sol[Nn:]
     base = g.w * g.rho * ph
     pref = -1j * k * ETA0 / (4.0 * math.pi)
-    fth = pref * np.sum(base * (Jt * (ct * g.trho * Icos - st * g.tz * I1) + Jf * (-ct * Isin)))
-    fph = pref * np.sum(base * (Jt * g.trho * Isin + Jf * Icos))
+    proj_th = lambda Xt, Xf: np.sum(base * (Xt * (ct * g.trho * Icos - st * g.tz * I1) + Xf * (-ct * Isin)))
+    proj_ph = lambda Xt, Xf: np.sum(base * (Xt * g.trho * Isin + Xf * Icos))
+    fth = pref * proj_th(Jt, Jf)
+    fph = pref * proj_ph(Jt, Jf)
+    if zs_pt is not None:
+        Mt, Mf = zs_pt * Jf, -zs_pt * Jt
+        pm = 1j * k / (4.0 * math.pi)
+        fth = fth - pm * proj_ph(Mt, Mf)
+        fph = fph + pm * proj_th(Mt, Mf)
     return fth, fph
 
 
-def mass_matrix(body: Body) -> np.ndarray:
-    """2 pi int rho T_i T_j dt (bor_solver.py:389-397)."""
+def mass_matrix(body: Body, weight: Optional[np.ndarray] = None) -> np.ndarray:
+    """2 pi int rho T_i T_j (weight) dt (bor_solver.py:389-397); weight per Gauss point."""
     g = body.g
     BT, _ = body.nodal_matrices()
-    return 2.0 * np.pi * (BT * (g.w * g.rho)[None, :]) @ BT.T
+    wt = g.w * g.rho if weight is None else g.w * g.rho * weight
+    return 2.0 * np.pi * (BT * wt[None, :]) @ BT.T
 
 
 def default_m_max(body: Body, k: float, thetas_deg) -> int:
@@ -707,13 +749,28 @@ def default_m_max(body: Body, k: float, thetas_deg) -> int:
     return int(math.ceil(k * float(np.max(body.nodes[:, 0])) * max(st, 0.05))) + 12
 
 
-def table_gb(n_elems: int, m_max: int, form: str, gauss_order: int = 4) -> float:
-    """bor_solver.py:804-817 (PEC, double)."""
+def table_gb(n_elems: int, m_max: int, form: str, gauss_order: int = 4, has_ibc: bool = False) -> float:
+    """bor_solver.py:804-817 (double)."""
     P = float(gauss_order * n_elems)
     t = P * P * (m_max + 2) * 16.0
     if form in ("cfie", "mfie"):
         t += 4.0 * P * P * (2 * m_max + 1) * 16.0
+    if has_ibc:
+        t += 4.0 * P * P * (2 * m_max + 1) * 16.0
     return t / 1e9
+
+
+def impedance_per_element(body: Body, zs) -> Optional[np.ndarray]:
+    """Z_s as a per-element complex array, or None for a PEC body
+    (bor_solver.py:850-866: a scalar applies to every element)."""
+    if zs is None:
+        return None
+    z = np.asarray(zs, dtype=np.complex128)
+    if z.ndim == 0:
+        z = np.full(body.Ne, complex(z))
+    elif len(z) != body.Ne:
+        raise ValueError("zs array must have one entry per generatrix element.")
+    return None if np.all(np.abs(z) == 0.0) else z
 
 
 def osc_for(body: Body, k: float, streaming: bool) -> float:
@@ -729,22 +786,31 @@ class ModeSystem:
     m = -m_max..m_max, excitation and far-field projection."""
 
     def __init__(self, nodes, freq_hz: float, m_max: int, formulation: str = "cfie",
-                 cfie_alpha: float = 0.5, streaming: Optional[bool] = None):
+                 cfie_alpha: float = 0.5, streaming: Optional[bool] = None, zs=None):
         self.body = Body(nodes)
         self.k = 2.0 * math.pi * freq_hz / C0
         self.m_max = m_max
         form = formulation.lower()
         self.alpha = float(cfie_alpha) if form == "cfie" else (1.0 if form == "efie" else 0.0)
+        # surface impedance (IBC-EFIE, bor_solver.py:850-866, 929-934)
+        self.zs_elem = impedance_per_element(self.body, zs)
+        self.zs_pt = None if self.zs_elem is None else self.zs_elem[self.body.g.elem]
+        if self.zs_elem is not None and form == "cfie":
+            raise ValueError("CFIE with a surface impedance is not supported; use formulation='efie'.")
+        ibc = self.zs_elem is not None and self.alpha > 0.0
         if streaming is None:
-            streaming = table_gb(self.body.Ne, m_max, form) > 2.0
+            streaming = table_gb(self.body.Ne, m_max, form, has_ibc=self.zs_elem is not None) > 2.0
         self.streaming = streaming
         osc = osc_for(self.body, self.k, streaming)
         self.n_e = n_xi_efie(osc, m_max)
         self.n_b = n_xi_bracket(osc, m_max)
         efie, mfie = self.alpha > 0.0, self.alpha < 1.0
-        self.far = FarBlocks(self.body, self.k, m_max, self.n_e, self.n_b, efie, mfie)
-        self.near = NearBlocks(self.body, self.k, m_max, efie, mfie)
+        self.far = FarBlocks(self.body, self.k, m_max, self.n_e, self.n_b, efie, mfie,
+                             zs_pt=self.zs_pt if ibc else None)
+        self.near = NearBlocks(self.body, self.k, m_max, efie, mfie,
+                               zs_elem=self.zs_elem if ibc else None)
         self.M = mass_matrix(self.body)
+        self.Mz = mass_matrix(self.body, self.zs_pt) if ibc else None
 
     def efie_full(self, m: int) -> np.ndarray:
         """Unmasked Z_EFIE [2Nn, 2Nn] (bor_solver.py:293-345)."""
@@ -767,12 +833,25 @@ class ModeSystem:
         M = self.M
         return np.block([[0.5 * M - blocks[0], -blocks[1]], [-blocks[2], 0.5 * M - blocks[3]]])
 
+    def ibc_full(self, m: int) -> np.ndarray:
+        """Unmasked IBC-EFIE additions (1/2) <W, Z_s J> + K'_Zs [2Nn, 2Nn]
+        (bor_solver.py:534-552)."""
+        blocks = list(self.far.ibc(m))
+        mi = m + self.m_max
+        for (e, f), blk in self.near.ibc.items():
+            for b in range(4):
+                blocks[b][e:e + 2, f:f + 2] += blk[b, mi]
+        Mz = self.Mz
+        return np.block([[0.5 * Mz + blocks[0], blocks[1]], [blocks[2], 0.5 * Mz + blocks[3]]])
+
     def system(self, m: int) -> Tuple[np.ndarray, np.ndarray]:
         """Masked A_m and its mask (bor_solver.py:929-938)."""
         n = 2 * self.body.Nn
         Z = np.zeros((n, n), dtype=np.complex128)
         if self.alpha > 0.0:
             Z += self.alpha * self.efie_full(m)
+            if self.Mz is not None:
+                Z += self.alpha * self.ibc_full(m)
         if self.alpha < 1.0:
             Z += (1.0 - self.alpha) * ETA0 * self.mfie_full(m)
         mask = basis_mask(self.body, m)
@@ -809,7 +888,7 @@ class ModeSystem:
             for ip in range(2):
                 sol = np.zeros(n, dtype=np.complex128)
                 sol[mask] = X[:, 2 * it + ip]
-                fth, fph = farfield(self.body, self.k, m, sol, th)
+                fth, fph = farfield(self.body, self.k, m, sol, th, self.zs_pt)
                 dF[ip, it] = fth if ip == 0 else fph
         return dF
 
@@ -837,13 +916,13 @@ def truncate_modes(dF_by_am: Sequence[np.ndarray], mode_tol: float):
 def solve_bor(nodes, freq_hz: float, thetas_deg, formulation: str = "efie",
               cfie_alpha: float = 0.5, n_modes: Optional[int] = None,
               mode_tol: float = 1e-6, streaming: Optional[bool] = None,
-              keep: bool = False) -> Dict:
-    """FP64 monostatic sweep of a PEC body (bor_solver.py:820-1004, PEC branch)."""
+              keep: bool = False, zs=None) -> Dict:
+    """FP64 monostatic sweep of a PEC or impedance body (bor_solver.py:820-1004)."""
     thetas = np.atleast_1d(np.asarray(thetas_deg, dtype=float))
     body = Body(nodes)
     k = 2.0 * math.pi * freq_hz / C0
     m_max = default_m_max(body, k, thetas) if n_modes is None else int(n_modes)
-    sysm = ModeSystem(nodes, freq_hz, m_max, formulation, cfie_alpha, streaming)
+    sysm = ModeSystem(nodes, freq_hz, m_max, formulation, cfie_alpha, streaming, zs=zs)
     dFs, res, sols = [], 0.0, {}
     for am in range(m_max + 1):
         dF = np.zeros((2, len(thetas)), dtype=np.complex128)

@@ -3,10 +3,14 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, never by the product path.
 
-Restates mie_sphere.py:42-101, 178-183, 231-235 in the e^{+jwt} convention
+Restates mie_sphere.py:42-125, 178-183, 231-242 in the e^{+jwt} convention
 (outgoing Riccati-Hankel xi_n = psi_n - j zeta_n):
     A_n = -psi_n'(x)/xi_n'(x)  (TM: U' = 0),  B_n = -psi_n(x)/xi_n(x)  (TE: U = 0)
     sigma = lambda^2/(4 pi) |sum_{n>=1} (-1)^n (2n+1)(A_n - B_n)|^2
+Leontovich (impedance) sphere, g = j Z_s / eta0 (mie_sphere.py:105-121):
+    A_n = -(psi' - g psi)/(xi' - g xi),  B_n = -(psi + g psi')/(xi + g xi')
+which reduces to the PEC pair at Z_s = 0 and gives A_n = B_n (zero
+backscatter, Weston's theorem) at Z_s = eta0.
 Riccati functions are built from scipy's spherical Bessel routines.
 """
 
@@ -45,6 +49,26 @@ def sigma_pec_sphere(radius_m: float, freq_hz: float) -> float:
     k = 2.0 * math.pi * freq_hz / C0
     x = k * radius_m
     a, b = pec_sphere_coeffs(x, _nmax(x))
+    n = np.arange(1, len(a))
+    s = np.sum(((-1.0) ** n) * (2 * n + 1) * (a[1:] - b[1:]))
+    lam = 2.0 * math.pi / k
+    return float(lam ** 2 / (4.0 * math.pi) * abs(s) ** 2)
+
+
+ETA0 = 376.730313668
+
+
+def impedance_sphere_coeffs(x: float, zs: complex, nmax: int):
+    psi, dpsi, xi, dxi = _riccati(nmax, x)
+    g = 1j * complex(zs) / ETA0
+    return -(dpsi - g * psi) / (dxi - g * xi), -(psi + g * dpsi) / (xi + g * dxi)
+
+
+def sigma_impedance_sphere(radius_m: float, freq_hz: float, zs_ohm: complex) -> float:
+    """Monostatic RCS (m^2) of a sphere with surface impedance zs_ohm."""
+    k = 2.0 * math.pi * freq_hz / C0
+    x = k * radius_m
+    a, b = impedance_sphere_coeffs(x, zs_ohm, _nmax(x))
     n = np.arange(1, len(a))
     s = np.sum(((-1.0) ** n) * (2 * n + 1) * (a[1:] - b[1:]))
     lam = 2.0 * math.pi / k
