@@ -88,29 +88,6 @@ def test_solve_bor_matches_reference(name, bs):
     assert out["linear_residual"] < 1e-13
 
 
-@pytest.mark.parametrize("lu_super", [1, 3, 4])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
-@pytest.mark.parametrize("nb", [16, 32, 64])
-@pytest.mark.parametrize("precision", [0, 1])
-def test_lu_tile_variants_and_block_sizes(variant, nb, precision, lu_super):
-    """Every GEMM tile variant and LU block size gives the reference's currents:
-    FP64 to 1e-9; mixed (FP32 far fill and LU, two FP64 refinement sweeps)
-    to 1e-6, the FP32 rounding of the filled matrix times its conditioning."""
-    d = load("conesphere16_cfie")
-    mm = int(d["m_max"])
-    thetas = d["thetas"]
-    print(f"[case] variant={variant} nb={nb} precision={precision} lu_super={lu_super}", flush=True)
-    h = make_handle(d["nodes"], float(d["freq_hz"]), thetas, mm, float(d["alpha"]), False,
-                    precision=precision, nb=nb, variant=variant, lu_super=lu_super)
-    h.fill()
-    h.solve(2 if precision == 1 else 0)
-    h.sync()
-    for m in range(-mm, mm + 1):
-        X = h.solution(0, m)
-        assert rel(X, d[f"X_{m}"]) < (1e-6 if precision == 1 else 1e-9), (m, rel(X, d[f"X_{m}"]))
-    h.close()
-
-
 def test_unmasked_blocks_match_oracle():
     nodes = bo.sphere_nodes(0.5, 14)
     f, mm = 3.5e8, 4

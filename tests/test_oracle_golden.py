@@ -203,3 +203,34 @@ def test_cfg1_sphere120_end_to_end_against_mie():
         for pol in ("sigma_vv", "sigma_hh"):
             err = np.max(np.abs(10 * np.log10(d[f"{form}_{pol}"]) - s_mie))
             assert err < 0.05, (form, pol, err)
+
+
+def test_row_tile_oracle_equals_full_oracle():
+    """The row-tile restriction that writes the 1200-element ogive rows
+    (tests/golden/make_oracle_cfg2_rows.py) gives the full oracle's rows on
+    a small body."""
+    import sys
+    sys.path.insert(0, GOLD)
+    import make_oracle_cfg2_rows as mr
+    nodes = bo.sphere_nodes(0.5, 16)
+    f, mm, E = 4e8, 5, 7
+    body = bo.Body(nodes)
+    k = 2 * math.pi * f / bo.C0
+    sysm = bo.ModeSystem(nodes, f, mm, "cfie", 0.5, streaming=False)
+    far = mr.RowFar(body, k, mm, sysm.n_e, sysm.n_b, E - 1)
+    near = mr.RowNear(body, k, mm, E - 1)
+    C = 1j * k * bo.ETA0 * 2.0 * np.pi
+    Ml = sysm.M[E - 1:E + 3]
+    for m in (-2, 0, 3):
+        eb, kb = list(far.efie(m)), list(far.mfie(m))
+        for (e, ff), blk in near.efie.items():
+            for b in range(4):
+                eb[b][e - E + 1:e - E + 3, ff:ff + 2] += blk[b, m + mm]
+        for (e, ff), blk in near.mfie.items():
+            for b in range(4):
+                kb[b][e - E + 1:e - E + 3, ff:ff + 2] += blk[b, m + mm]
+        Z = 0.5 * C * np.block([[eb[0], eb[1]], [eb[2], eb[3]]]) + 0.5 * bo.ETA0 * np.block(
+            [[0.5 * Ml - kb[0], -kb[1]], [-kb[2], 0.5 * Ml - kb[3]]])
+        full = 0.5 * sysm.efie_full(m) + 0.5 * bo.ETA0 * sysm.mfie_full(m)
+        Nn = body.Nn
+        assert rel(Z[[1, 2, 5, 6]], full[[E, E + 1, Nn + E, Nn + E + 1]]) < 1e-13
