@@ -18,12 +18,13 @@ def test_mesh_matches_reference():
     bd = pkg("bor_dispatch")
     d = np.load(os.path.join(ROOT, "tests", "golden", "dispatch_sphere.npz"))
     snap = dispatch_snapshot()
-    chains = bd.chains_from_snapshot(snap, 1.0)
-    assert bd.classify(chains) == "conductor"
-    diag = max(float(np.ptp(np.vstack([c.pts for c in chains]), axis=0).max()), 1e-9)
-    run = bd.stitch(chains, "TYPE 2", max(1e-12, 1e-9 * diag))
-    bd.preflight(run, "body", 1e-6 * diag)
-    pts = bd.mesh(run, bd.C0 / 0.3e9, 50000, 1e-6 * diag)
+    pieces = bd.read_pieces(snap, 1.0)
+    assert bd.layout(pieces) == "conductor"
+    diag = max(float(np.ptp(np.vstack([c.pts for c in pieces]), axis=0).max()), 1e-9)
+    run = bd.single_run(pieces, "TYPE 2", max(1e-12, 1e-9 * diag))
+    bd.check_generatrix(run, "body", 1e-6 * diag)
+    pts, owner, arc = bd.mesh_run(run, bd.C0 / 0.3e9, 50000, 1e-6 * diag)
+    assert len(owner) == len(pts) - 1 and np.all((arc > 0) & (arc < 1))
     assert pts.shape == d["nodes"].shape
     assert np.max(np.abs(pts - d["nodes"])) < 1e-15
 
@@ -40,13 +41,12 @@ def test_validation_errors():
     rev = {"segments": [dict(s, point_pairs=[{"x1": p["x2"], "y1": p["y2"], "x2": p["x1"], "y2": p["y1"]}
                                               for p in reversed(s["point_pairs"])])
                         for s in reversed(snap["segments"])]}
-    chains = bd.chains_from_snapshot(rev, 1.0)
-    run = bd.stitch(chains, "TYPE 2", 1e-9)
+    run = bd.single_run(bd.read_pieces(rev, 1.0), "TYPE 2", 1e-9)
     with pytest.raises(ValueError, match="traversed"):
-        bd.preflight(run, "body", 1e-6)
+        bd.check_generatrix(run, "body", 1e-6)
     diel = {"segments": [dict(s, properties=[3, 2, 0, 1, 0]) for s in snap["segments"]]}
     with pytest.raises(NotImplementedError):
-        bd.classify(bd.chains_from_snapshot(diel, 1.0))
+        bd.layout(bd.read_pieces(diel, 1.0))
     abort = __import__("threading").Event()
     abort.set()
     with pytest.raises(InterruptedError):
