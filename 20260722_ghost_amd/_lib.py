@@ -77,7 +77,15 @@ SIGNATURES = [
     ("ghost_far_fill_waves", _ci, [_ci, _ci, _ci, _ci]),
     ("ghost_sample_mfie_noalloc", _ci, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
     ("ghost_bor_set_impedance", _ci, [_vp, _ci, _dp, _cd]),
+    ("ghost_device_name", _ci, [ctypes.c_char_p, _ci]),
 ]
+
+
+def device_name() -> str:
+    """'name|gfx arch|cuN' of the current HIP device (from the library's own runtime)."""
+    buf = ctypes.create_string_buffer(256)
+    check(load().ghost_device_name(buf, 256))
+    return buf.value.decode(errors="replace")
 
 
 def device_synchronize() -> None:

@@ -213,8 +213,9 @@ def _result(pts, thetas, form, m_max, ne, nbk, streaming, dF, res, mode_tol, pre
 
 
 def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_iters: int = -3,
-                    residual: bool = True, nb: int = 64, mode_tol: float = 1e-6,
-                    graph: bool = False, check_abort: Optional[Callable] = None) -> List[Dict]:
+                    residual: bool = True, nb: Optional[int] = None, mode_tol: float = 1e-6,
+                    graph: bool = False, check_abort: Optional[Callable] = None,
+                    tune: bool = True) -> List[Dict]:
     """Solve several (generatrix, frequency) problems as ONE batch: every mode
     system of every problem goes through the same batched fill, LU/solve and
     far-field launches (systems of uneven size share each launch).  Each job
@@ -222,7 +223,11 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
     formulation ("cfie"), cfie_alpha (0.5), n_modes, assembly, zs (surface
     impedance: scalar or one complex value per element, EFIE only).  Returns
     one solve_bor-style dict per job.  graph=True captures the batch in a
-    hipGraph and replays it."""
+    hipGraph and replays it.  nb = None takes the LU block size and GEMM tile
+    from the autotuner (``autotune.best`` for the batch's largest system,
+    system count and column count; measured once per shape bucket and
+    device when tune is true, else its defaults); an explicit nb keeps the
+    default tile.  The choice is reported as ``lu_tiles``."""
     prec = str(precision).strip().lower()
     if prec not in _PRECISION:
         raise ValueError("precision must be 'fp64' or 'mixed'.")
@@ -244,8 +249,16 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
         specs.append((pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs))
     if check_abort is not None:
         check_abort()
+    variant = -1
+    if nb is None:
+        from . import autotune
+        n_max = max(2 * len(sp[0]) for sp in specs)
+        nsys = sum(sp[4] + 1 for sp in specs)
+        nrhs = max(2 * len(sp[1]) for sp in specs)
+        nb, variant = autotune.best(_PRECISION[prec], n_max, nsys, nrhs, tune=tune)
     h = _lib.Handle(_PRECISION[prec], nb=nb, residual=residual)
     try:
+        h.set_gemm_variant(variant)
         for (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs) in specs:
             idx = h.add_problem(pts, k, m_max, ne, nbk, cE, cM, thetas)
             if zs is not None:
@@ -273,6 +286,7 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
         off += nm
         outs.append(_result(pts, thetas, form, m_max, ne, nbk, streaming, dFs[i], r, mode_tol, prec,
                             refine_iters, times, probe, _ibc_warnings(pts, zs, form)))
+        outs[-1]["lu_tiles"] = (int(nb), int(variant))
     return outs
 
 
@@ -287,7 +301,7 @@ def solve_bor(points, freq_hz: float, thetas_deg, formulation: str = "efie",
               check_abort: Optional[Callable] = None,
               table_precision: str = "auto", assembly: str = "auto",
               stream_budget_gb: float = 8.0, *, precision: str = "fp64",
-              refine_iters: int = -3, residual: bool = True, nb: int = 64) -> Dict:
+              refine_iters: int = -3, residual: bool = True, nb: Optional[int] = None) -> Dict:
     """Monostatic RCS of a PEC body of revolution at aspects thetas_deg (from +z).
 
     Mirrors bor_solver.solve_bor (bor_solver.py:820-1004): PEC bodies

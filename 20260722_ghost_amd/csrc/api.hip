@@ -2003,6 +2003,16 @@ extern "C" int ghost_bor_graph_ms(ghost_bor* h, float* ms) {
 }
 
 // Device-wide barrier on the current device (every stream, every handle).
+extern "C" int ghost_device_name(char* buf, int len) {
+  if (!buf || len <= 0) return fail(GHOST_ERR_ARG, "null or empty buffer");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t pr;
+  HIPCHK(hipGetDeviceProperties(&pr, dev));
+  snprintf(buf, (size_t)len, "%s|%s|cu%d", pr.name, pr.gcnArchName, pr.multiProcessorCount);
+  return 0;
+}
+
 extern "C" int ghost_device_synchronize(void) {
   HIPCHK(hipDeviceSynchronize());
   return 0;

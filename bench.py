@@ -40,13 +40,26 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profile=True):
+def tuned_tiles(precision, nodes, thetas, m_max):
+    """The autotuner's (LU block size, GEMM tile) for this sweep's shape,
+    measured (untimed, before the benchmark) on first use per device."""
+    L = pkg("_lib")
+    at = pkg("autotune")
+    prec = L.GHOST_MIXED if precision == "mixed" else L.GHOST_FP64
+    t0 = time.perf_counter()
+    tiles = at.best(prec, 2 * len(nodes), m_max + 1, 2 * len(thetas))
+    log(f"{precision}: autotuned (nb, variant) = {tiles} in {time.perf_counter() - t0:.1f} s")
+    return tiles
+
+
+def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profile=True, tiles=(64, -1)):
     log(f"{precision}: build (m_max={m_max}, n_elems={len(nodes) - 1}, aspects={len(thetas)})")
     L = pkg("_lib")
     bs = pkg("bor_solver")
     k = 2 * math.pi * freq / bs.C0
     _, ne, nbk, streaming = bs.discretisation(nodes, k, thetas, m_max, "cfie")
-    h = L.Handle(L.GHOST_MIXED if precision == "mixed" else L.GHOST_FP64, nb=64, residual=False)
+    h = L.Handle(L.GHOST_MIXED if precision == "mixed" else L.GHOST_FP64, nb=tiles[0], residual=False)
+    h.set_gemm_variant(tiles[1])
     h.add_problem(nodes, k, m_max, ne, nbk, 0.5, 0.5 * bs.ETA0, thetas)
     h.build()
     for i in range(warmup):
@@ -340,14 +353,17 @@ def main():
     name, nodes, freq, thetas, m_max = wl.config(args.config)
     solves = (2 * m_max + 1) * len(thetas)
 
+    tiles_mx = tuned_tiles("mixed", nodes, thetas, m_max)
     dt_mx, dF_mx, kt_mx, ph_mx, nxi = run_gpu(nodes, freq, thetas, m_max, "mixed", args.steps, args.warmup,
-                                               args.refine)
+                                               args.refine, tiles=tiles_mx)
     F_mx = pkg("bor_solver").truncate_modes(dF_mx, m_max, 0.0)[0]
     err_db = None
     fp64_rate = None
     kt_64 = {}
+    tiles_64 = None
     if not args.no_fp64:
-        dt_64, dF_64, kt_64, ph_64, _ = run_gpu(nodes, freq, thetas, m_max, "fp64", 1, 1, 0)
+        tiles_64 = tuned_tiles("fp64", nodes, thetas, m_max)
+        dt_64, dF_64, kt_64, ph_64, _ = run_gpu(nodes, freq, thetas, m_max, "fp64", 1, 1, 0, tiles=tiles_64)
         F_64 = pkg("bor_solver").truncate_modes(dF_64, m_max, 0.0)[0]
         s64, smx = db(F_64), db(F_mx)
         peak = np.max(s64)
@@ -403,6 +419,8 @@ def main():
         "kernel_ms_per_step": {k: round(v[0], 3) for k, v in kt_mx.items() if v[1]},
         "fp64_kernel_ms": {k: round(v[0], 3) for k, v in kt_64.items() if v[1]},
         "phase_ms_last_step": ph_mx,
+        "autotuned_lu_tiles": {"mixed": list(tiles_mx), "fp64": list(tiles_64) if tiles_64 else None,
+                               "cache": pkg("autotune").cache_path()},
         "roofline": roof,
         "cpu_baseline": None if args.no_cpu else cpu_baseline(nodes, freq, thetas, m_max),
     }

@@ -161,6 +161,10 @@ int ghost_bor_farfield(ghost_bor* h);
 /* fill + solve + farfield. */
 int ghost_bor_run(ghost_bor* h, int refine_iters);
 int ghost_bor_sync(ghost_bor* h);
+/* "name|gfx arch|cuN" of the current device (NUL-terminated, truncated to
+ * len): the key under which tuned tile choices are cached.  No counterpart
+ * in the reference (part of the tuner query). */
+int ghost_device_name(char* buf, int len);
 /* Device-wide barrier (hipDeviceSynchronize) on the current device. */
 int ghost_device_synchronize(void);
 
