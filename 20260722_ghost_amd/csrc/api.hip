@@ -1419,7 +1419,11 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
     const dim3 g((unsigned)p.batches.size());
     prof_begin(h);
     switch (C) {
-#define NEARCASE(c) case c: hipLaunchKernelGGL((k_near_modal<c>), g, dim3(256), 0, ns, na); break;
+#define NEARCASE(c)                                                                  \
+  case c:                                                                            \
+    if (mixed) hipLaunchKernelGGL((k_near_modal<c, float>), g, dim3(256), 0, ns, na);  \
+    else hipLaunchKernelGGL((k_near_modal<c, double>), g, dim3(256), 0, ns, na);      \
+    break;
       NEARCASE(1) NEARCASE(2) NEARCASE(3) NEARCASE(4) NEARCASE(5) NEARCASE(6) NEARCASE(7) NEARCASE(8)
       NEARCASE(9) NEARCASE(10) NEARCASE(11) NEARCASE(12) NEARCASE(13) NEARCASE(14) NEARCASE(15) NEARCASE(16)
 #undef NEARCASE

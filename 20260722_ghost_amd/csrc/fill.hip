@@ -537,9 +537,47 @@ __device__ __forceinline__ void cell_geom(const double* nodes, int e, double s, 
   z = z0 + s * (z1 - z0);
 }
 
+// Accumulate one thread's C orders over one batch of 8 nodes: nvs = the
+// point pair's node values (12 per node), st0 / st1 = its chunk starts
+// e^{j c C xi} (re / im, [node * 9 + chunk]).  EFIE and even brackets
+// project on cos(n xi), odd brackets on sin(n xi); cos / sin of the chunk's
+// orders follow the three-term recurrence f_{n+1} = 2 cos(xi) f_n - f_{n-1}
+// from the two exact starts e^{j n0 xi} and e^{j (n0 + 1) xi} (9 steps: the
+// error stays a few ulp of R).
+template <int C, typename R>
+__device__ __forceinline__ void near_acc8(const double* nvs, const double* st0, const double* st1, int cs,
+                                          R (&gr)[C], R (&gi)[C], R (&kr)[4][C], R (&ki)[4][C]) {
+  for (int u = 0; u < 8; ++u) {
+    const double* xd = nvs + 12 * u;
+    R x[10];
+#pragma unroll
+    for (int t = 0; t < 10; ++t) x[t] = (R)xd[t];
+    const R c1 = (R)xd[10], s1 = (R)xd[11], tc = (R)2 * c1;
+    R cp = (R)st0[u * 9 + cs], sp = (R)st1[u * 9 + cs];
+    R cc = cp * c1 - sp * s1, sc = cp * s1 + sp * c1;
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      const R cr = i == 0 ? cp : cc, ci = i == 0 ? sp : sc;
+      gr[i] += x[0] * cr;    gi[i] += x[1] * cr;
+      kr[0][i] += x[2] * cr; ki[0][i] += x[3] * cr;
+      kr[1][i] += x[4] * ci; ki[1][i] += x[5] * ci;
+      kr[2][i] += x[6] * ci; ki[2][i] += x[7] * ci;
+      kr[3][i] += x[8] * cr; ki[3][i] += x[9] * cr;
+      if (i > 0) {
+        const R cn = tc * cc - cp, sn = tc * sc - sp;
+        cp = cc; sp = sc;
+        cc = cn; sc = sn;
+      }
+    }
+  }
+}
+
 // 256 threads = 32 point pairs x 8 order chunks of C orders.  Node values are
 // evaluated once per (point pair, node) and shared through LDS in batches of 8.
-template <int C>
+// The sums over the 48 nodes of the singular sinh core run in FP64; those over
+// the smooth [xi0, pi] tail in RT (FP64, or FP32 in mixed precision, where the
+// far blocks are FP32 already).
+template <int C, typename RT = double>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 : 1))) void k_near_modal(NearArgs a) {
   const int2 bt = a.batches[blockIdx.x];
   const NearPairInfo pr = a.pairs[bt.x];
@@ -594,15 +632,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
   __shared__ double nv[32][97];
   __shared__ double st[2][32 * 72];
 
-  double2 accG[C], accK[4][C];
+  double gR[C], gI[C], kR[4][C], kI[4][C];
 #pragma unroll
   for (int i = 0; i < C; ++i) {
-    accG[i] = make_double2(0, 0);
+    gR[i] = 0.0; gI[i] = 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) accK[u][i] = make_double2(0, 0);
+    for (int u = 0; u < 4; ++u) { kR[u][i] = 0.0; kI[u][i] = 0.0; }
   }
-
-  for (int v0 = 0; v0 < nnodes; v0 += 8) {
+  auto node_values = [&](int v0) {
     // ── node value for node v0 + cs of this point pair ──
     const int vn = v0 + cs;
     double* out = &nv[slot][12 * cs];
@@ -707,35 +744,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
         st[1][slot * 72 + cs * 9 + c] = 0.0;
       }
     }
+  };
+  // singular core (nodes 0..47) in FP64
+  for (int v0 = 0; v0 < 48; v0 += 8) {
+    node_values(v0);
     __syncthreads();
-    // ── accumulate this thread's orders over the 8 nodes ──
-    if (valid) {
-      for (int u = 0; u < 8; ++u) {
-        const double* x = &nv[slot][12 * u];
-        const double c1 = x[10], s1 = x[11], tc = 2.0 * c1;
-        // cos / sin of the chunk's orders by the three-term recurrence
-        // f_{n+1} = 2 cos(xi) f_n - f_{n-1} from the two exact starts
-        // e^{j n0 xi} and e^{j (n0 + 1) xi} (9 steps: the error stays a few ulp)
-        double cp = st[0][slot * 72 + u * 9 + cs], sp = st[1][slot * 72 + u * 9 + cs];
-        double cc = cp * c1 - sp * s1, sc = cp * s1 + sp * c1;
+    if (valid) near_acc8<C, double>(nv[slot], &st[0][slot * 72], &st[1][slot * 72], cs, gR, gI, kR, kI);
+    __syncthreads();
+  }
+  // smooth tail in RT
+  {
+    RT tR[C], tI[C], uR[4][C], uI[4][C];
 #pragma unroll
-        for (int i = 0; i < C; ++i) {
-          const double cr = i == 0 ? cp : cc, ci = i == 0 ? sp : sc;
-          // EFIE and even brackets project on cos(n xi), odd brackets on sin(n xi)
-          accG[i].x += x[0] * cr;  accG[i].y += x[1] * cr;
-          accK[0][i].x += x[2] * cr; accK[0][i].y += x[3] * cr;
-          accK[1][i].x += x[4] * ci; accK[1][i].y += x[5] * ci;
-          accK[2][i].x += x[6] * ci; accK[2][i].y += x[7] * ci;
-          accK[3][i].x += x[8] * cr; accK[3][i].y += x[9] * cr;
-          if (i > 0) {
-            const double cn = tc * cc - cp, sn = tc * sc - sp;
-            cp = cc; sp = sc;
-            cc = cn; sc = sn;
-          }
-        }
-      }
+    for (int i = 0; i < C; ++i) {
+      tR[i] = (RT)gR[i]; tI[i] = (RT)gI[i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) { uR[u][i] = (RT)kR[u][i]; uI[u][i] = (RT)kI[u][i]; }
     }
-    __syncthreads();
+    for (int v0 = 48; v0 < nnodes; v0 += 8) {
+      node_values(v0);
+      __syncthreads();
+      if (valid) near_acc8<C, RT>(nv[slot], &st[0][slot * 72], &st[1][slot * 72], cs, tR, tI, uR, uI);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < C; ++i) {
+      gR[i] = (double)tR[i]; gI[i] = (double)tI[i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) { kR[u][i] = (double)uR[u][i]; kI[u][i] = (double)uI[u][i]; }
+    }
   }
   if (!valid) return;
   const bool axis = pg[slot][13] != 0.0;
@@ -745,7 +782,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
   for (int i = 0; i < C; ++i) {
     const int n = n0 + i;
     if (n >= a.nord) break;
-    double2 G = accG[i];
+    double2 G = make_double2(gR[i], gI[i]);
     if (axis) {
       // closed form: only order 0, 2 pi g(d)
       if (n == 0) {
@@ -760,10 +797,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
     }
     double2* o = out + (size_t)n * 5;
     o[0] = G;
-    o[1] = accK[0][i];
-    o[2] = make_double2(accK[1][i].y, -accK[1][i].x);  // -j * sum
-    o[3] = make_double2(accK[2][i].y, -accK[2][i].x);
-    o[4] = accK[3][i];
+    o[1] = make_double2(kR[0][i], kI[0][i]);
+    o[2] = make_double2(kI[1][i], -kR[1][i]);  // -j * sum
+    o[3] = make_double2(kI[2][i], -kR[2][i]);
+    o[4] = make_double2(kR[3][i], kI[3][i]);
   }
 }
 
