@@ -1341,7 +1341,13 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
 #undef FARCASE
         default: return fail(GHOST_ERR_ARG, "m_max too large for the far fill");
       }
-      prof_end(h, K_FAR_FILL, 0.0);
+      {
+        // MFMA flops issued (16x16x4: 2048 per wavefront instruction): per
+        // element pair, (n_xi/8 + 1) sample groups x NT order tiles x 2 (re, im)
+        // for the EFIE orders and the same per bracket orientation
+        const double ge = p.nE / 8 + 1, gb = p.nB / 8 + 1, orient = split ? 1.0 : 2.0;
+        prof_end(h, K_FAR_FILL, (double)fa.npairs * (ge + orient * gb) * NT * 2.0 * 2048.0);
+      }
       LAUNCHCHK();
     }
   return 0;
@@ -1429,7 +1435,14 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
 #undef NEARCASE
       default: return fail(GHOST_ERR_ARG, "m_max too large for the near quadrature");
     }
-    prof_end(h, K_NEAR_MODAL, 0.0);
+    {
+      // projection flops: 10 real FMAs (G and four brackets, re and im) per
+      // (point pair, azimuth node, order)
+      double fl = 0.0;
+      for (const auto& pi : p.pairs)
+        if (pi.e >= 0) fl += (double)p.cell_cnt[pi.kind] * (48 + pi.n_tail) * na.nord * 20.0;
+      prof_end(h, K_NEAR_MODAL, fl);
+    }
     LAUNCHCHK();
   }
   NearBlockArgs nb;

@@ -25,7 +25,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MI355X_PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
+MI355X_PEAK = {"fp64_mfma_tflops": 78.6, "fp64_vector_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
 
 
 def pkg(m):
@@ -93,75 +93,112 @@ def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profil
     return dt, dF, ktimes, phase, (ne, nbk)
 
 
-def cpu_baseline(nodes, freq, thetas, m_max, budget_s=25.0):
-    """Oracle (NumPy/SciPy + OpenBLAS) on a bounded sample of the same sweep:
-    far fill of a few test-element rows, near blocks of a few element pairs,
-    LU + solve of one mode system and its far-field projections, each scaled
-    to the full sweep."""
+def cpu_baseline(nodes, freq, thetas, m_max):
+    """The FP64 oracle (NumPy/SciPy + OpenBLAS) on a bounded sample of the
+    same sweep, with the reference's thread model: a pool of nproc - 1
+    workers over independent units (bor_dispatch.py:386-387), one BLAS thread
+    each (run_hpc_bor_monostatic.py:107, 155-160).  Each leg runs one unit per
+    worker concurrently and is scaled by (units in the sweep) / (units run):
+    far fill of 2-element test-row tiles, near element pairs, mode LU +
+    solve, and the per-aspect excitation and far-field projection of a mode.
+    A whole mode cannot be timed at this size: the far sampling is shared by
+    all modes and alone takes the better part of an hour of CPU time."""
+    from concurrent.futures import ThreadPoolExecutor
+    from threadpoolctl import threadpool_limits
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import bor_oracle as bo
     log("cpu baseline: oracle sample")
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 2
+    workers = max(1, min(cores, 16) - 1)
     body = bo.Body(nodes)
     k = 2 * math.pi * freq / bo.C0
     osc = bo.osc_for(body, k, streaming=True)
     ne, nbk = bo.n_xi_efie(osc, m_max), bo.n_xi_bracket(osc, m_max)
     Ne = body.Ne
-    rows = 2
-    # far fill of a 2-element test-row tile: samples, FFT over xi, near zeroing
-    t0 = time.perf_counter()
     g = body.g
-    e0 = Ne // 2
-    sl = slice(e0 * 4, (e0 + rows) * 4)
     xe, xb = bo.xi_grid(ne), bo.xi_grid(nbk)
-    P_ = lambda v: v[sl][:, None]
-    Q_ = lambda v: v[None, :]
-    G = bo.green_orders_fft(bo.green_samples(P_(g.rho), P_(g.z), Q_(g.rho), Q_(g.z), k, xe), m_max + 2)
-    bo._zero_near_rows(G, e0, rows, Ne, 4)
-    Fs = bo.bracket_samples(P_(g.rho), P_(g.z), P_(g.trho), P_(g.tz), Q_(g.rho), Q_(g.z), Q_(g.trho), Q_(g.tz), k, xb)
-    Ks = [bo.bracket_modes_fft(F, m_max) for F in Fs]
-    t_far = (time.perf_counter() - t0) * Ne / rows
-    # near: a few element pairs
-    t0 = time.perf_counter()
-    npairs = 6
-    for (e, f) in [(e0, e0), (e0, e0 + 1), (e0 + 1, e0)] * (npairs // 3):
+    n = 2 * body.Nn - 4
+    nrhs = 2 * len(thetas)
+
+    def far_tile(e0):
+        sl = slice(e0 * 4, (e0 + 2) * 4)
+        P_ = lambda v: v[sl][:, None]
+        Q_ = lambda v: v[None, :]
+        G = bo.green_orders_fft(bo.green_samples(P_(g.rho), P_(g.z), Q_(g.rho), Q_(g.z), k, xe), m_max + 2)
+        bo._zero_near_rows(G, e0, 2, Ne, 4)
+        Fs = bo.bracket_samples(P_(g.rho), P_(g.z), P_(g.trho), P_(g.tz), Q_(g.rho), Q_(g.z), Q_(g.trho),
+                                Q_(g.tz), k, xb)
+        return [bo.bracket_modes_fft(F, m_max) for F in Fs]
+
+    def near_pair(i):
+        e = Ne // 4 + i
+        f = e + (i % 3) - 1
         s, sp, w = bo.cell_points(bo.near_kind(e, f))
         p, q = body.element_points(e, s), body.element_points(f, sp)
         bo.green_orders_near(p["rho"], p["z"], q["rho"], q["z"], k, m_max)
         bo.bracket_modes_near(p["rho"], p["z"], p["trho"], p["tz"], q["rho"], q["z"], q["trho"], q["tz"], k, m_max)
-    t_near = (time.perf_counter() - t0) * (3 * Ne - 2) / npairs
-    # one mode: LU + solve of a random complex system of the same size, RHS and far field
-    n = 2 * body.Nn - 4
-    rng = np.random.default_rng(0)
-    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + n * np.eye(n)
-    B = rng.standard_normal((n, 2 * len(thetas))) + 1j * rng.standard_normal((n, 2 * len(thetas)))
-    t0 = time.perf_counter()
-    X = np.linalg.solve(A, B)
-    t_solve = (time.perf_counter() - t0) * (2 * m_max + 1)
-    t0 = time.perf_counter()
-    nth = 12
-    for th in thetas[:nth]:
-        bo.rhs_efie(body, k, 3, th, "VV")
-        bo.rhs_mfie(body, k, 3, th, "VV")
-        sol = np.zeros(2 * body.Nn, dtype=complex)
-        bo.farfield(body, k, 3, sol, th)
-    t_exc = (time.perf_counter() - t0) * (2 * m_max + 1) * (2 * len(thetas)) / nth
-    total = t_far + t_near + t_solve + t_exc
+
+    def mode_solve(i):
+        rng = np.random.default_rng(i)
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + n * np.eye(n)
+        B = rng.standard_normal((n, nrhs)) + 1j * rng.standard_normal((n, nrhs))
+        t0 = time.perf_counter()
+        np.linalg.solve(A, B)
+        return time.perf_counter() - t0
+
+    nth = 4
+
+    def excite(i):
+        m = i % (m_max + 1)
+        for th in thetas[i * nth:(i + 1) * nth]:
+            for pol in ("VV", "HH"):
+                bo.rhs_efie(body, k, m, th, pol)
+                bo.rhs_mfie(body, k, m, th, pol)
+            sol = np.zeros(2 * body.Nn, dtype=complex)
+            bo.farfield(body, k, m, sol, th)
+
+    def leg(fn, units):
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(workers) as ex:
+            out = list(ex.map(fn, units))
+        return time.perf_counter() - t0, out
+
+    with threadpool_limits(limits=1):
+        t_far, _ = leg(far_tile, [2 * Ne // (2 * workers + 2) * (i + 1) for i in range(workers)])
+        t_near, _ = leg(near_pair, range(workers))
+        t_sol, per = leg(mode_solve, range(workers))
+        t_exc, _ = leg(excite, range(workers))
+    est = {"far": t_far * Ne / (2 * workers), "near": t_near * (3 * Ne - 2) / workers,
+           "solve": t_sol * (2 * m_max + 1) / workers,
+           "rhs_farfield": t_exc * (2 * m_max + 1) * len(thetas) / (workers * nth)}
+    total = sum(est.values())
     solves = (2 * m_max + 1) * len(thetas)
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count()
     return {
         "value": solves / total,
         "unit": "solves/s",
-        "cores": cores,
+        "cores": workers,
         "kind": "port",
-        "sample": (f"FP64 oracle timed on {rows} of {Ne} far test rows, {npairs} of {3 * Ne - 2} near pairs, "
-                   f"1 of {2 * m_max + 1} mode LU+solves (n={n}, nrhs={2 * len(thetas)}) and {nth} of "
-                   f"{len(thetas)} aspects of RHS/far field, each scaled to the full sweep; "
-                   f"est. total {total:.0f} s (far {t_far:.0f}, near {t_near:.0f}, solve {t_solve:.0f}, "
-                   f"rhs/far-field {t_exc:.0f}); OpenBLAS threads = {cores}"),
+        "sample": (f"FP64 oracle, {workers} worker threads x 1 BLAS thread (the reference's mode pool); each "
+                   f"leg runs {workers} units concurrently and is scaled to the sweep: far fill "
+                   f"{2 * workers} of {Ne} test rows, near {workers} of {3 * Ne - 2} element pairs, "
+                   f"{workers} of {2 * m_max + 1} mode LU+solves (n={n}, nrhs={nrhs}, "
+                   f"{np.mean(per):.1f} s each), RHS/far field {workers * nth} of "
+                   f"{(2 * m_max + 1) * len(thetas)} (mode, aspect) pairs; est. total {total:.0f} s "
+                   f"(far {est['far']:.0f}, near {est['near']:.0f}, solve {est['solve']:.0f}, "
+                   f"rhs/far-field {est['rhs_farfield']:.0f}); {time.perf_counter() - T_CPU0:.0f} s of wall time"),
     }
+
+
+T_CPU0 = 0.0
+
+
+def cpu_baseline_timed(*args):
+    global T_CPU0
+    T_CPU0 = time.perf_counter()
+    return cpu_baseline(*args)
 
 
 def run_adjoint(steps):
@@ -327,6 +364,8 @@ def main():
     ap.add_argument("--refine", type=int, default=-2,
                     help="FP64 refinement sweeps; negative = adaptive (error probe), up to |n|")
     ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
+                    help="path of the timed steps (fp64: for profiling the plain FP64 path)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -353,9 +392,10 @@ def main():
     name, nodes, freq, thetas, m_max = wl.config(args.config)
     solves = (2 * m_max + 1) * len(thetas)
 
-    tiles_mx = tuned_tiles("mixed", nodes, thetas, m_max)
-    dt_mx, dF_mx, kt_mx, ph_mx, nxi = run_gpu(nodes, freq, thetas, m_max, "mixed", args.steps, args.warmup,
-                                               args.refine, tiles=tiles_mx)
+    tiles_mx = tuned_tiles(args.precision, nodes, thetas, m_max)
+    dt_mx, dF_mx, kt_mx, ph_mx, nxi = run_gpu(nodes, freq, thetas, m_max, args.precision, args.steps,
+                                               args.warmup, args.refine if args.precision == "mixed" else 0,
+                                               tiles=tiles_mx)
     F_mx = pkg("bor_solver").truncate_modes(dF_mx, m_max, 0.0)[0]
     err_db = None
     fp64_rate = None
@@ -385,16 +425,34 @@ def main():
     if gemm_n > 0 and gemm_ms > 0:
         achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12
         traffic = None
+        fp32 = args.precision == "mixed"
         tpath = os.path.join(ROOT, "profiles", "ogive1200_m70_mixed_hbm_traffic.json")
-        if args.config == 2 and os.path.exists(tpath):
+        if args.config == 2 and fp32 and os.path.exists(tpath):
             with open(tpath) as f:
                 traffic = json.load(f)["per_launch"]["gemm"]["hbm_bytes"]
-        roof = {"kernel": "k_gemm2<float,2,2> (complex FP32 MFMA 16x16x4, 64x64 tiles; LU updates and solves)",
-                "bound": "mfma", "achieved": round(achieved, 3), "peak": MI355X_PEAK["fp32_mfma_tflops"],
-                "unit": "TFLOP/s", "frac": round(achieved / MI355X_PEAK["fp32_mfma_tflops"], 4),
+        fp32 = args.precision == "mixed"
+        pk = MI355X_PEAK["fp32_mfma_tflops" if fp32 else "fp64_mfma_tflops"]
+        roof = {"kernel": ("k_gemm2<float,2,2> (complex FP32 MFMA 16x16x4, 64x64 tiles; LU updates and solves)"
+                           if fp32 else "k_gemm2<double,...> (complex FP64 MFMA 16x16x4; LU updates and solves)"),
+                "bound": "mfma", "achieved": round(achieved, 3), "peak": pk,
+                "unit": "TFLOP/s", "frac": round(achieved / pk, 4),
                 "traffic": traffic, "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
                 "launches": gemm_n, "avg_launch_ms": round(gemm_ms / gemm_n, 4),
                 "flops_per_launch": gemm_fl / gemm_n}
+    # the fill's two big kernels against their own bounds (live HIP-event time of the profiled step):
+    # the far fill's MFMA flops issued against FP32 MFMA peak (it also issues the sample arithmetic on
+    # the VALU, which does not co-issue with MFMA on gfx950), the near band's projection flops against
+    # FP64 vector peak (its FP32 tail runs at the same issue rate with packed FP32 off)
+    more = {}
+    for fam, peak_key, what in (("far_fill", "fp32_mfma_tflops", "k_far_fill (FP32 MFMA transform + VALU samples)"),
+                                ("near_modal", "fp64_vector_tflops", "k_near_modal (FP64 core / FP32 tail VALU projection)")):
+        ms_f, n_f, fl_f = kt_mx.get(fam, (0.0, 0, 0.0))
+        if ms_f > 0 and fl_f > 0:
+            ach = fl_f / (ms_f * 1e-3) / 1e12
+            more[fam] = {"kernel": what, "bound": "mfma" if fam == "far_fill" else "valu",
+                         "achieved": round(ach, 3), "peak": MI355X_PEAK[peak_key], "unit": "TFLOP/s",
+                         "frac": round(ach / MI355X_PEAK[peak_key], 4), "ms_per_step": round(ms_f, 3),
+                         "flops_per_step": fl_f}
     out = {
         "metric": "BOR monostatic solves/s (mode x freq x angle) at <=0.01 dB vs FP64; MFMA/HBM %",
         "value": round(solves / dt_mx, 2),
@@ -406,11 +464,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "fp32+fp64" if args.precision == "mixed" else "fp64",
         "data": "synthetic (seeded generatrix)",
         "config": {"workload": name, "modes": 2 * m_max + 1, "m_max": m_max, "aspects": len(thetas),
                    "polarizations": 2, "frequencies": 1, "n_elems": int(len(nodes) - 1),
-                   "n_xi": list(nxi), "formulation": "cfie(alpha=0.5)", "precision": "mixed",
+                   "n_xi": list(nxi), "formulation": "cfie(alpha=0.5)", "precision": args.precision,
                    "refine_iters": args.refine, "solves_per_step": solves},
         "fp64_path_solves_per_s": round(fp64_rate, 2) if fp64_rate else None,
         "speedup_vs_fp64_path": round((solves / dt_mx) / fp64_rate, 3) if fp64_rate else None,
@@ -419,10 +477,11 @@ def main():
         "kernel_ms_per_step": {k: round(v[0], 3) for k, v in kt_mx.items() if v[1]},
         "fp64_kernel_ms": {k: round(v[0], 3) for k, v in kt_64.items() if v[1]},
         "phase_ms_last_step": ph_mx,
-        "autotuned_lu_tiles": {"mixed": list(tiles_mx), "fp64": list(tiles_64) if tiles_64 else None,
+        "autotuned_lu_tiles": {args.precision: list(tiles_mx), "fp64": list(tiles_64) if tiles_64 else None,
                                "cache": pkg("autotune").cache_path()},
         "roofline": roof,
-        "cpu_baseline": None if args.no_cpu else cpu_baseline(nodes, freq, thetas, m_max),
+        "roofline_fill_kernels": more,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline_timed(nodes, freq, thetas, m_max),
     }
     print(json.dumps(out), flush=True)
 
