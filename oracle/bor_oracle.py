@@ -22,6 +22,9 @@ It restates, in NumPy/SciPy, the discretization of the reference solver:
 * a12 dense solve + residual ......... bor_solver.py:744-758
 * a13 far field + mode truncation .... bor_solver.py:659-708, 771-796, 987-992
 * a14 orchestration defaults ......... bor_solver.py:804-916
+* f1  IBC-EFIE (Z_s brackets, Z_s mass, magnetic-current far field)
+                                       bor_kernels.py:440-560, bor_solver.py:463-552,
+                                       697-707, 850-869
 
 Pinning (what checks each function, besides golden vectors written from the
 reference itself under tests/golden/):
@@ -33,6 +36,8 @@ reference itself under tests/golden/):
 * a6  convergence to adaptive quadrature; golden vectors.
 * a11/a13 Bessel identities; sphere invariants; Mie series end to end.
 * a12 residual ~1e-15; agreement with LAPACK.
+* f1  golden vectors (scalar and tapered Z_s); the impedance Mie series and
+      Weston's null (Z_s = eta0) end to end (oracle/mie.py).
 """
 
 from __future__ import annotations
@@ -706,7 +711,7 @@ def rhs_mfie(body: Body, k: float, m: int, theta_deg: float, pol: str) -> np.nda
 
 
 def farfield(body: Body, k: float, m: int, sol: np.ndarray, theta_deg: float,
-             zs_pt: Optional[np.ndarray] = None):
+             zs_pt: Optional[np.ndarray] = None, msol: Optional[np.ndarray] = None):
     """Modal (F_theta, F_phi) of the full-length solution vector
     (bor_solver.py:659-696; the far-field pairs (j_{m+1}, j_{m-1}) are
     reversed relative to the excitation).  zs_pt: the eliminated magnetic
@@ -727,8 +732,13 @@ def farfield(body: Body, k: float, m: int, sol: np.ndarray, theta_deg: float,
     proj_ph = lambda Xt, Xf: np.sum(base * (Xt * g.trho * Isin + Xf * Icos))
     fth = pref * proj_th(Jt, Jf)
     fph = pref * proj_ph(Jt, Jf)
+    Mt = Mf = None
     if zs_pt is not None:
         Mt, Mf = zs_pt * Jf, -zs_pt * Jt
+    elif msol is not None:
+        # an independent magnetic current on the same bases (PMCHWT, bor_solver.py:700-702)
+        Mt, Mf = BT.T @ msol[:Nn], BT.T @ msol[Nn:]
+    if Mt is not None:
         pm = 1j * k / (4.0 * math.pi)
         fth = fth - pm * proj_ph(Mt, Mf)
         fph = fph + pm * proj_th(Mt, Mf)
@@ -946,4 +956,118 @@ def solve_bor(nodes, freq_hz: float, thetas_deg, formulation: str = "efie",
     if keep:
         out["_modes"] = sols
         out["_dF"] = dFs
+    return out
+
+
+# ─────────────────────────────────────────────────────────────────────────────
+# f3  PMCHWT: homogeneous penetrable body (bor_solver.py:161-172, 175-230,
+#     554-575, 1019-1094)
+# ─────────────────────────────────────────────────────────────────────────────
+
+def causal_medium(eps_r, mu_r):
+    """(m, eta_r): refractive index with Im(m) <= 0 and eta_r = mu_r / m
+    (bor_solver.py:161-172)."""
+    m = np.sqrt(complex(eps_r) * complex(mu_r))
+    if m.imag > 0.0:
+        m = -m
+    if m.real < 0.0:
+        m = -m
+    return m, complex(mu_r) / m
+
+
+class MediumOperators:
+    """The EFIE operator T (with C = j k eta 2 pi) and the rotated-PV operator
+    P of one homogeneous medium, node-based [2Nn, 2Nn], on the table rules
+    (bor_solver.py:293-345, 497-575).  medium None is free space."""
+
+    def __init__(self, nodes, freq_hz: float, m_max: int, medium=None):
+        self.body = Body(nodes)
+        k0 = 2.0 * math.pi * freq_hz / C0
+        if medium is None:
+            self.k, self.eta = k0, ETA0
+        else:
+            mi, er = causal_medium(*medium)
+            self.k, self.eta = k0 * mi, ETA0 * er
+        self.m_max = m_max
+        osc = osc_for(self.body, self.k, streaming=False)
+        self.n_e, self.n_b = n_xi_efie(osc, m_max), n_xi_bracket(osc, m_max)
+        b = self.body
+        self.far = FarBlocks(b, self.k, m_max, self.n_e, self.n_b, efie=True, mfie=False,
+                             zs_pt=np.ones(b.P, dtype=np.complex128))
+        self.near = NearBlocks(b, self.k, m_max, efie=True, mfie=False,
+                               zs_elem=np.ones(b.Ne, dtype=np.complex128))
+
+    def T(self, m: int) -> np.ndarray:
+        blocks = list(self.far.efie(m))
+        mi = m + self.m_max
+        for (e, f), blk in self.near.efie.items():
+            for q in range(4):
+                blocks[q][e:e + 2, f:f + 2] += blk[q, mi]
+        C = 1j * self.k * self.eta * 2.0 * np.pi
+        return C * np.block([[blocks[0], blocks[1]], [blocks[2], blocks[3]]])
+
+    def P(self, m: int) -> np.ndarray:
+        """Columns remapped from the rotated-PV brackets B (unit weight):
+        P = [[-Btf, Btt], [-Bff, Bft]] (bor_solver.py:554-575)."""
+        B = list(self.far.ibc(m))
+        mi = m + self.m_max
+        for (e, f), blk in self.near.ibc.items():
+            for q in range(4):
+                B[q][e:e + 2, f:f + 2] += blk[q, mi]
+        return np.block([[-B[1], B[0]], [-B[3], B[2]]])
+
+
+def pmchwt_system(ext: MediumOperators, inner: MediumOperators, m: int) -> np.ndarray:
+    """[[Te + Ti, -eta0 (Pe + Pi)], [eta0 (Pe + Pi), Te + (eta0/eta_i)^2 Ti]]
+    for unknowns (J, M / eta0) (bor_solver.py:1047-1056)."""
+    Te, Ti = ext.T(m), inner.T(m)
+    Ps = ETA0 * (ext.P(m) + inner.P(m))
+    r2 = (ETA0 / inner.eta) ** 2
+    return np.block([[Te + Ti, -Ps], [Ps, Te + r2 * Ti]])
+
+
+def solve_bor_dielectric(nodes, freq_hz: float, thetas_deg, eps_r, mu_r=1.0,
+                         n_modes: Optional[int] = None, mode_tol: float = 1e-6, keep: bool = False) -> Dict:
+    """Monostatic RCS of a closed homogeneous penetrable body by per-mode
+    PMCHWT (bor_solver.py:1019-1094): RHS [<W, E_inc>; eta0 <W, H_inc>], the
+    far field radiating J and M = eta0 M'."""
+    thetas = np.atleast_1d(np.asarray(thetas_deg, dtype=float))
+    body = Body(nodes)
+    k0 = 2.0 * math.pi * freq_hz / C0
+    m_max = default_m_max(body, k0, thetas) if n_modes is None else int(n_modes)
+    ext = MediumOperators(nodes, freq_hz, m_max)
+    inner = MediumOperators(nodes, freq_hz, m_max, (eps_r, mu_r))
+    Nn = body.Nn
+    dFs, res, sols = [], 0.0, {}
+    for am in range(m_max + 1):
+        dF = np.zeros((2, len(thetas)), dtype=np.complex128)
+        for m in ([0] if am == 0 else [am, -am]):
+            mask = np.tile(basis_mask(body, m), 2)
+            A = pmchwt_system(ext, inner, m)[np.ix_(mask, mask)]
+            cols = []
+            for th in thetas:
+                for pol in ("VV", "HH"):
+                    hpol = rhs_efie(body, k0, m, th, "HH" if pol == "VV" else "VV")
+                    h = -hpol if pol == "VV" else hpol        # eta0 <W, H_inc> (bor_solver.py:647-657)
+                    cols.append(np.concatenate([rhs_efie(body, k0, m, th, pol), h])[mask])
+            B = np.stack(cols, axis=1)
+            X = np.linalg.solve(A, B)
+            bn = float(np.linalg.norm(B))
+            res = max(res, float(np.linalg.norm(A @ X - B)) / bn if bn > 0 else 0.0)
+            for it, th in enumerate(thetas):
+                for ip in range(2):
+                    full = np.zeros(4 * Nn, dtype=np.complex128)
+                    full[mask] = X[:, 2 * it + ip]
+                    fth, fph = farfield(body, k0, m, full[:2 * Nn], th, msol=ETA0 * full[2 * Nn:])
+                    dF[ip, it] += fth if ip == 0 else fph
+            if keep:
+                sols[m] = (X, B, A, mask)
+        dFs.append(dF)
+    F, used = truncate_modes(dFs, mode_tol)
+    out = {"theta_deg": thetas.tolist(), "sigma_vv": (4.0 * math.pi * np.abs(F[0]) ** 2).tolist(),
+           "sigma_hh": (4.0 * math.pi * np.abs(F[1]) ** 2).tolist(), "amp_vv": F[0].tolist(),
+           "amp_hh": F[1].tolist(), "modes_used": used, "n_unknowns": 4 * Nn, "linear_residual": res,
+           "formulation": "pmchwt", "m_max": m_max}
+    if keep:
+        out["_modes"] = sols
     return out

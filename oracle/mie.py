@@ -11,6 +11,9 @@ Leontovich (impedance) sphere, g = j Z_s / eta0 (mie_sphere.py:105-121):
     A_n = -(psi' - g psi)/(xi' - g xi),  B_n = -(psi + g psi')/(xi + g xi')
 which reduces to the PEC pair at Z_s = 0 and gives A_n = B_n (zero
 backscatter, Weston's theorem) at Z_s = eta0.
+Homogeneous dielectric sphere (mie_sphere.py:124-141), index m = sqrt(eps mu)
+with Im(m) <= 0: A_n, B_n from continuity with the interior psi_n(m x),
+which vanish for eps = mu = 1 and tend to the PEC pair as eps -> infinity.
 Riccati functions are built from scipy's spherical Bessel routines.
 """
 
@@ -69,6 +72,47 @@ def sigma_impedance_sphere(radius_m: float, freq_hz: float, zs_ohm: complex) -> 
     k = 2.0 * math.pi * freq_hz / C0
     x = k * radius_m
     a, b = impedance_sphere_coeffs(x, zs_ohm, _nmax(x))
+    n = np.arange(1, len(a))
+    s = np.sum(((-1.0) ** n) * (2 * n + 1) * (a[1:] - b[1:]))
+    lam = 2.0 * math.pi / k
+    return float(lam ** 2 / (4.0 * math.pi) * abs(s) ** 2)
+
+
+def _riccati_complex(nmax: int, z: complex):
+    """psi_n(z) = z j_n(z) and psi_n'(z) for complex z."""
+    n = np.arange(nmax + 1)
+    j = special.spherical_jn(n, z)
+    dj = special.spherical_jn(n, z, derivative=True)
+    return z * j, j + z * dj
+
+
+def causal_index(eps_r: complex, mu_r: complex) -> complex:
+    m = np.sqrt(complex(eps_r) * complex(mu_r))
+    if m.imag > 0.0:
+        m = -m
+    if m.real < 0.0:
+        m = -m
+    return m
+
+
+def dielectric_sphere_coeffs(x: float, eps_r: complex, mu_r: complex, nmax: int):
+    """Homogeneous sphere (mie_sphere.py:124-141): interior psi_n(m x),
+    continuity of U and U'/(eps or mu) scaled by the index m."""
+    m = causal_index(eps_r, mu_r)
+    psi, dpsi, xi, dxi = _riccati(nmax, x)
+    pm, dpm = _riccati_complex(nmax, m * x)
+    wa, wb = m / complex(eps_r), m / complex(mu_r)
+    a = -(dpsi * pm - wa * dpm * psi) / (dxi * pm - wa * dpm * xi)
+    b = -(dpsi * pm - wb * dpm * psi) / (dxi * pm - wb * dpm * xi)
+    return a, b
+
+
+def sigma_dielectric_sphere(radius_m: float, eps_r: complex, mu_r: complex, freq_hz: float) -> float:
+    """Monostatic RCS (m^2) of a homogeneous dielectric / magnetic sphere."""
+    k = 2.0 * math.pi * freq_hz / C0
+    x = k * radius_m
+    m = causal_index(eps_r, mu_r)
+    a, b = dielectric_sphere_coeffs(x, eps_r, mu_r, _nmax(max(x, abs(m) * x)))
     n = np.arange(1, len(a))
     s = np.sum(((-1.0) ** n) * (2 * n + 1) * (a[1:] - b[1:]))
     lam = 2.0 * math.pi / k
