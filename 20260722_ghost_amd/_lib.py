@@ -8,6 +8,7 @@ loaded, every call raises GhostLibraryError.
 import ctypes
 import os
 import threading
+from typing import Optional
 
 import numpy as np
 
@@ -76,7 +77,8 @@ SIGNATURES = [
     ("ghost_bor_set_lu_super", _ci, [_vp, _ci]),
     ("ghost_far_fill_waves", _ci, [_ci, _ci, _ci, _ci]),
     ("ghost_sample_mfie_noalloc", _ci, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
-    ("ghost_bor_set_impedance", _ci, [_vp, _ci, _dp, _cd]),
+    ("ghost_bor_set_impedance", _ci, [_vp, _ci, _dp, _cd, _cd]),
+    ("ghost_bor_set_medium", _ci, [_vp, _ci, _cd, _cd, _cd]),
     ("ghost_device_name", _ci, [ctypes.c_char_p, _ci]),
 ]
 
@@ -187,13 +189,21 @@ class Handle:
         self.nodes_per_problem.append(int(nodes.shape[0]))
         return idx.value
 
-    def set_impedance(self, prob: int, zs, c_ibc: float = 1.0):
-        """Per-element complex surface impedance of problem prob (before build)."""
+    def set_impedance(self, prob: int, zs, c_ibc: float = 1.0, c_mass: Optional[float] = None):
+        """Per-element complex surface impedance of problem prob (before
+        build); c_mass (default c_ibc) weighs the (1/2) Z_s mass term."""
         ne = self.nodes_per_problem[prob] - 1
         z = np.asarray(zs, dtype=np.complex128)
         z = np.full(ne, complex(z)) if z.ndim == 0 else z.reshape(ne)
         zz = np.ascontiguousarray(np.column_stack([z.real, z.imag]).ravel())
-        check(self.lib.ghost_bor_set_impedance(self._h, int(prob), dptr(zz), float(c_ibc)))
+        cm = float(c_ibc) if c_mass is None else float(c_mass)
+        check(self.lib.ghost_bor_set_impedance(self._h, int(prob), dptr(zz), float(c_ibc), cm))
+
+    def set_medium(self, prob: int, k_im: float, eta: complex):
+        """Homogeneous medium of problem prob's fill (before build): Im k and
+        the complex wave impedance."""
+        eta = complex(eta)
+        check(self.lib.ghost_bor_set_medium(self._h, int(prob), float(k_im), eta.real, eta.imag))
 
     def add_dense(self, n: int, nrhs: int) -> int:
         """A caller-filled n x n system with nrhs columns; returns its system index."""

@@ -156,6 +156,19 @@ def impedance_per_element(zs, n_elems: int) -> Optional[np.ndarray]:
     return None if np.all(z == 0.0) else z
 
 
+def causal_medium(eps_r, mu_r):
+    """(m, eta_r) of a homogeneous medium: the refractive index sqrt(eps mu)
+    on the branch with Im(m) <= 0 (a passive medium under e^{j w t}) and
+    Re(m) >= 0, and the relative wave impedance mu_r / m (bor_solver.py:
+    161-172)."""
+    m = complex(np.sqrt(complex(eps_r) * complex(mu_r)))
+    if m.imag > 0.0:
+        m = -m
+    if m.real < 0.0:
+        m = -m
+    return m, complex(mu_r) / m
+
+
 def _ibc_warnings(pts: np.ndarray, zs: Optional[np.ndarray], form: str) -> List[str]:
     """A closed body on the EFIE with a purely reactive Z_s has undamped
     cavity resonances (the reference's guard, bor_solver.py:944-968)."""
@@ -443,19 +456,23 @@ class BorPecSolver:
     [2Nn, 2Nn] systems, cached per m_max), rhs_* run the excitation kernels
     for one aspect, farfield_mode runs the far-field kernels on a caller's
     solution (with the impedance body's radiating magnetic current when
-    zs_pt is given).  Penetrable media (medium=..., assemble_pmchwt_P,
-    msol) are outside this build and raise NotImplementedError."""
+    zs_pt is given, or an independent one, msol).  medium=(eps_r, mu_r)
+    fills the operators of that homogeneous medium: k = k0 m with the
+    causal index m and eta = eta0 mu_r / m (bor_solver.py:175-199)."""
 
     def __init__(self, points, freq_hz: float, gauss_order: int = 4,
                  near_depth: int = 4, medium=None, single_tables: bool = False):
-        if medium is not None:
-            raise NotImplementedError("penetrable media (PMCHWT) are not part of the GPU BoR path.")
         if int(gauss_order) != 4 or int(near_depth) != 4:
             raise NotImplementedError("the GPU BoR path is built for gauss_order = 4 and near_depth = 4.")
         self.pts = _validate_nodes(points)
         self.freq_hz = float(freq_hz)
-        self.k = 2.0 * math.pi * self.freq_hz / C0
-        self.eta = ETA0
+        self.k0 = 2.0 * math.pi * self.freq_hz / C0
+        self.medium = None if medium is None else (complex(medium[0]), complex(medium[1]))
+        if self.medium is None:
+            self.k, self.eta = self.k0, ETA0
+        else:
+            mi, er = causal_medium(*self.medium)
+            self.k, self.eta = self.k0 * mi, ETA0 * er
         self.gauss_order = 4
         self.near_depth = 4
         self.Nn = int(self.pts.shape[0])
@@ -492,12 +509,16 @@ class BorPecSolver:
         key = (kind, int(m_max)) if zs_elem is None else (kind, int(m_max), np.asarray(zs_elem).tobytes())
         if key not in self._ops:
             ne, nb = self._n_xi(m_max)
-            cE, cM = {"efie": (1.0, 0.0), "mfie": (0.0, 1.0), "ibc": (0.0, 0.0)}[kind]
+            cE, cM = {"efie": (1.0, 0.0), "mfie": (0.0, 1.0), "ibc": (0.0, 0.0), "pv": (0.0, 0.0)}[kind]
             h = _lib.Handle(_lib.GHOST_FP64, residual=False)
             try:
-                h.add_problem(self.pts, self.k, m_max, ne, nb, cE, cM, [0.0], unmasked=True)
+                h.add_problem(self.pts, complex(self.k).real, m_max, ne, nb, cE, cM, [0.0], unmasked=True)
                 if kind == "ibc":
                     h.set_impedance(0, zs_elem, 1.0)
+                elif kind == "pv":   # bare rotated-PV brackets: unit Z_s, no mass term
+                    h.set_impedance(0, np.ones(self.Nn - 1), 1.0, 0.0)
+                if self.medium is not None:
+                    h.set_medium(0, complex(self.k).imag, self.eta)
                 h.build()
                 h.fill()
                 h.sync()
@@ -524,8 +545,16 @@ class BorPecSolver:
             raise ValueError("zs_pt must repeat zs_elem over each element's Gauss points.")
         return self._systems("ibc", m_max, ze)[int(m)].copy()
 
-    def assemble_pmchwt_P(self, *args, **kwargs):
-        raise NotImplementedError("penetrable media (PMCHWT) are not part of the GPU BoR path.")
+    def assemble_pmchwt_P(self, m: int, m_max: int) -> np.ndarray:
+        """The PMCHWT coupling operator of this medium [2Nn, 2Nn]: the
+        rotated-PV brackets B (unit weight, no mass term) with their columns
+        remapped, P = [[-Btf, Btt], [-Bff, Bft]] (bor_solver.py:554-575)."""
+        B = self._systems("pv", m_max)[int(m)]
+        n = self.Nn
+        P = np.empty_like(B)
+        P[:, :n] = -B[:, n:]
+        P[:, n:] = B[:, :n]
+        return P
 
     def mass_blocks(self, weight=None) -> np.ndarray:
         """2 pi int w rho T_i T_j dt, node-based [Nn, Nn] (bor_solver.py:389-397):
@@ -559,7 +588,12 @@ class BorPecSolver:
                 f_act[end] = True
         return np.concatenate([t_act, f_act])
 
+    def _free_space(self, what: str) -> None:
+        if self.medium is not None:
+            raise ValueError(f"{what} is defined for the exterior (free-space) solver only.")
+
     def _rhs_all(self, kind: str, m: int, theta_inc_deg: float) -> np.ndarray:
+        self._free_space("the plane-wave excitation")
         key = (kind, float(theta_inc_deg))
         cache = self._rhs.get(key)
         if cache is None or abs(m) > cache[0]:
@@ -596,6 +630,7 @@ class BorPecSolver:
     def farfield_mode(self, m: int, sol: np.ndarray, theta_s_deg: float, zs_pt=None, msol=None):
         """(F_theta, F_phi) of one mode's full [2Nn] solution (bor_solver.py:659-708);
         zs_pt adds the radiation of M = -Z_s n x J."""
+        self._free_space("the far field")
         if msol is not None:
             raise NotImplementedError("independent magnetic currents (PMCHWT) are not part of the GPU BoR path.")
         sol = np.asarray(sol, dtype=np.complex128).reshape(2 * self.Nn)

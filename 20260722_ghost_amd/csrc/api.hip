@@ -155,8 +155,12 @@ struct Problem {
   // surface impedance (IBC-EFIE): per-element Z_s, the coefficient of the
   // impedance operator, and whether any Z_s is nonzero
   std::vector<double2> zs;
-  double cI = 0;
+  double cI = 0, cMass = 0;
   bool has_zs = false;
+  // homogeneous medium of the fill (ghost_bor_set_medium): k = k + j ki and
+  // the wave impedance eta; free space by default
+  double ki = 0.0;
+  double eta_re = 376.730313668, eta_im = 0.0;
   double2* d_zs = nullptr;
   std::vector<double> nodes, theta;
   bool axis0 = false, axis1 = false;
@@ -429,9 +433,21 @@ extern "C" int ghost_far_fill_waves(int precision, int m_max, int n_xi_efie, int
   return far_fill_waves(precision == GHOST_MIXED ? 4 : 8, (m_max + 2 + 15) / 16, n_xi_efie, n_xi_bracket);
 }
 template <typename R, int NT>
-static void launch_far_fill(bool split, bool ibc, int wpb, int grid, size_t tab_bytes, hipStream_t s,
+static void launch_far_fill(bool split, bool ibc, bool ck, int wpb, int grid, size_t tab_bytes, hipStream_t s,
                             const FarArgs& fa) {
   if constexpr (sizeof(R) == 8) {
+    if (ck) {   // lossy medium (never split: the adjoint refuses media)
+      const dim3 b(64 * wpb);
+      if (wpb == 1) {
+        if (ibc) hipLaunchKernelGGL((k_far_fill<R, NT, false, true, 1, true>), dim3(grid), b, tab_bytes, s, fa);
+        else hipLaunchKernelGGL((k_far_fill<R, NT, false, false, 1, true>), dim3(grid), b, tab_bytes, s, fa);
+      } else {
+        constexpr int W = far_waves(sizeof(R), NT);
+        if (ibc) hipLaunchKernelGGL((k_far_fill<R, NT, false, true, W, true>), dim3(grid), b, tab_bytes, s, fa);
+        else hipLaunchKernelGGL((k_far_fill<R, NT, false, false, W, true>), dim3(grid), b, tab_bytes, s, fa);
+      }
+      return;
+    }
     if (wpb == 1) {
       if (split) hipLaunchKernelGGL((k_far_fill<R, NT, true, false, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
       else if (ibc) hipLaunchKernelGGL((k_far_fill<R, NT, false, true, 1>), dim3(grid), dim3(64), tab_bytes, s, fa);
@@ -521,12 +537,13 @@ extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, 
 }
 
 // Surface impedance of one problem (IBC-EFIE, bor_solver.py:850-866,
-// 929-934): zs complex per element; Z += c_ibc ((1/2) M_Zs + K'_Zs).
-extern "C" int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs, double c_ibc) {
+// 929-934): zs complex per element; Z += c_mass (1/2) M_Zs + c_ibc K'_Zs.
+extern "C" int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs, double c_ibc, double c_mass) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
   if (h->built) return fail(GHOST_ERR_STATE, "set the surface impedance before build");
   if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index %d", prob);
   if (!zs) return fail(GHOST_ERR_ARG, "null impedance array");
+  if (!std::isfinite(c_ibc) || !std::isfinite(c_mass)) return fail(GHOST_ERR_ARG, "impedance coefficients must be finite");
   Problem& p = h->probs[prob];
   std::vector<double2> z(p.Ne);
   bool any = false;
@@ -541,6 +558,26 @@ extern "C" int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs,
   p.zs = z;
   p.has_zs = any;
   p.cI = any ? c_ibc : 0.0;
+  p.cMass = any ? c_mass : 0.0;
+  return 0;
+}
+
+// Homogeneous medium of one problem's fill (bor_solver.py:175-199 with
+// medium=(eps, mu)): k = k_real + j k_im (k_im <= 0 for a lossy medium) and
+// the wave impedance eta.  FP64 handles only.
+extern "C" int ghost_bor_set_medium(ghost_bor* h, int prob, double k_im, double eta_re, double eta_im) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "set the medium before build");
+  if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index %d", prob);
+  if (!std::isfinite(k_im) || !std::isfinite(eta_re) || !std::isfinite(eta_im))
+    return fail(GHOST_ERR_ARG, "medium parameters must be finite");
+  if (k_im > 0.0) return fail(GHOST_ERR_ARG, "Im k must be <= 0 (a causal, passive medium)");
+  if (k_im != 0.0 && h->precision != GHOST_FP64)
+    return fail(GHOST_ERR_ARG, "a lossy medium (complex k) needs an FP64 handle");
+  Problem& p = h->probs[prob];
+  p.ki = k_im;
+  p.eta_re = eta_re;
+  p.eta_im = eta_im;
   return 0;
 }
 
@@ -632,7 +669,7 @@ static int prepare_problem(ghost_bor* h, Problem& p) {
         const double rp = r0e + cs[kind][q] * dre, rq = r0f + csp[kind][q] * drf;
         const double rr4 = 4.0 * rp * rq;
         const double a = std::sqrt(std::max(rr4, 1e-300));
-        osc = std::max(osc, std::fabs(p.k) * a);
+        osc = std::max(osc, std::hypot(p.k, p.ki) * a);
       }
       const double o = osc / M_PI + (p.M + 2);
       const int nt = (int)std::min(1024.0, std::max(64.0, std::ceil(4.0 * o)));
@@ -1289,9 +1326,13 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
   fa.nE = p.nE;
   fa.nB = p.nB;
   fa.k = p.k;
-  // EFIE coefficient cE * j k eta0 2 pi ; far MFIE coefficient -2 pi cM
-  fa.ce_re = 0.0;
-  fa.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
+  // EFIE coefficient cE * j k eta 2 pi (complex for a lossy medium); far MFIE coefficient -2 pi cM
+  const double ke_re = p.k * p.eta_re - p.ki * p.eta_im, ke_im = p.k * p.eta_im + p.ki * p.eta_re;
+  fa.ce_re = -p.cE * ke_im * 2.0 * M_PI;
+  fa.ce_im = p.cE * ke_re * 2.0 * M_PI;
+  fa.ki = p.ki;
+  const bool ck = p.ki != 0.0;
+  if (ck && sizeof(R) != 8) return fail(GHOST_ERR_ARG, "complex k needs an FP64 fill");
   fa.cm = -2.0 * M_PI * p.cM;
   const bool ibc = p.has_zs && p.cI != 0.0;
   fa.zs = p.d_zs;
@@ -1335,7 +1376,7 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
       switch (NT) {
 #define FARCASE(n)                                                                                      \
   case n:                                                                                               \
-    launch_far_fill<R, n>(split, ibc, wpb, grid, tab_bytes, h->stream, fa);                              \
+    launch_far_fill<R, n>(split, ibc, ck, wpb, grid, tab_bytes, h->stream, fa);                          \
     break;
         FARCASE(1) FARCASE(2) FARCASE(3) FARCASE(4) FARCASE(5) FARCASE(6) FARCASE(7) FARCASE(8)
 #undef FARCASE
@@ -1419,6 +1460,7 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   na.nord = p.M + 2;
   na.ibc = (p.has_zs && p.cI != 0.0) ? 1 : 0;
   na.k = p.k;
+  na.ki = p.ki;
   na.NP = p.d_NP;
   if (!p.batches.empty()) {
     const int C = (na.nord + 7) / 8;
@@ -1451,11 +1493,15 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   nb.grho = tg.pts;
   nb.gT0 = tg.pts + 5 * P;
   nb.gT1 = tg.pts + 6 * P;
-  nb.ce_re = 0.0;
-  nb.ce_im = p.cE * p.k * 376.730313668 * 2.0 * M_PI;
+  {
+    const double ke_re = p.k * p.eta_re - p.ki * p.eta_im, ke_im = p.k * p.eta_im + p.ki * p.eta_re;
+    nb.ce_re = -p.cE * ke_im * 2.0 * M_PI;
+    nb.ce_im = p.cE * ke_re * 2.0 * M_PI;
+  }
   nb.cM = p.cM;
   nb.zs = p.d_zs;
   nb.cI = p.cI;
+  nb.cMass = p.cMass;
   nb.NB = p.d_NB;
   const int mlanes = ((p.M + 1 + 15) / 16) * 16;
   if (mlanes > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near blocks");
@@ -2414,7 +2460,8 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   }
   if (h->precision != GHOST_FP64) return fail(GHOST_ERR_STATE, "the adjoint needs an FP64 handle");
   for (auto& p : h->probs)
-    if (p.has_zs) return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies only (no surface impedance)");
+    if (p.has_zs || p.ki != 0.0 || p.eta_re != 376.730313668 || p.eta_im != 0.0)
+      return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies in free space only");
   if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
   if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
   HIPCHK(hipSetDevice(h->device));

@@ -41,6 +41,7 @@ struct FarArgs {
   int delta, imax;       // unordered pairs (not SPLIT): f = pf + 2 j with j >= i + delta, rows i < imax
   int zmodes, zoff;      // Z mode stride and the index of mode 0 (multiples of 16; zoff = 0 with pairing)
   double k, ce_re, ce_im, cm;
+  double ki;             // CK: imaginary part of k (a lossy medium; k is the real part)
   const double2* zs;     // IBC: per-element surface impedance (source element's applies)
   double cz;             // IBC: bracket coefficient 2 pi c_ibc (times zs of the source element)
   void* Z;               // Zfar [2Nn][2Nn][zmodes] complex<R>
@@ -103,7 +104,10 @@ __host__ __device__ constexpr int far_waves(int bytes, int nt) { return far_wide
 // bracket phase samples the rotated-PV brackets of the impedance operator
 // (bor_kernels.py:449-478) in place of the MFIE ones, with the coefficient
 // 2 pi c_ibc Z_s(source element) (bor_solver.py:497-518).
-template <typename R, int NT, bool SPLIT, bool IBC = false, int WPB = far_waves(sizeof(R), NT)>
+// CK: complex k = k + j ki (penetrable media, FP64 only): every sample's
+// e^{-jkR} carries the decay e^{ki R}, the brackets' (1 + jkR) its imaginary
+// part, and the EFIE blocks' 1/k^2 is complex.
+template <typename R, int NT, bool SPLIT, bool IBC = false, int WPB = far_waves(sizeof(R), NT), bool CK = false>
 __global__ __launch_bounds__(64 * WPB)
 __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4 || NT <= 2) ? 2 : 1))) void k_far_fill(FarArgs a) {
   typedef typename CT<R>::c C2;
@@ -265,7 +269,8 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
         Rd = Rd > (R)1e-30 ? Rd : (R)1e-30;
         R s, c;
         sincos_s(k * Rd, &s, &c);
-        const R amp = smp.y * rcp_s(Rd);
+        R amp = smp.y * rcp_s(Rd);
+        if constexpr (CK) amp *= exp((R)a.ki * Rd);
         const R ar = amp * c, ai = -amp * s;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
@@ -286,7 +291,15 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
     const R Swa[2] = {(R)(P.T0[ig] * P.w[ig]), (R)(P.T1[ig] * P.w[ig])};
     const R Dwa[2] = {(R)(P.D0[ig] * P.w[ig]), (R)(P.D1[ig] * P.w[ig])};
     const R trq = (R)Q.trho[4 * se], tzq = (R)Q.tz[4 * se];
-    const R ik2 = (R)1 / (k * k);
+    // 1/k^2: a real factor of the divergence terms, or (CK) complex and
+    // applied to the source sums sDG, sSG below
+    R ik2 = (R)1 / (k * k), i2r = ik2, i2i = (R)0;
+    if constexpr (CK) {
+      const R kr_ = k, ki_ = (R)a.ki, den = (kr_ * kr_ + ki_ * ki_) * (kr_ * kr_ + ki_ * ki_);
+      i2r = (kr_ * kr_ - ki_ * ki_) / den;
+      i2i = -2 * kr_ * ki_ / den;
+      ik2 = (R)1;
+    }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       const R am = (R)(16 * t + c16);
@@ -333,6 +346,17 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
           sTS[bi][0] += Twb[bi] * Gsr; sTS[bi][1] += Twb[bi] * Gsi;
           sDG[bi][0] += Dwb[bi] * Gr;  sDG[bi][1] += Dwb[bi] * Gi;
           sSG[bi][0] += Swb[bi] * Gr;  sSG[bi][1] += Swb[bi] * Gi;
+        }
+      }
+      if constexpr (CK) {
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+          R x = sDG[bi][0], y = sDG[bi][1];
+          sDG[bi][0] = i2r * x - i2i * y;
+          sDG[bi][1] = i2r * y + i2i * x;
+          x = sSG[bi][0]; y = sSG[bi][1];
+          sSG[bi][0] = i2r * x - i2i * y;
+          sSG[bi][1] = i2r * y + i2i * x;
         }
       }
       const R tt_c = trp * trq, tt_z = tzp * tzq;
@@ -417,9 +441,19 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       R s, c;
       sincos_s(k * Rd, &s, &c);
       const R inv = rcp_s(Rd);
-      const R pre = smp[2] * inv * inv * inv;
+      R pre = smp[2] * inv * inv * inv;
       const R kr = k * Rd;
-      const R pr = pre * (c + kr * s), pim = pre * (kr * c - s);
+      R pr, pim;
+      if constexpr (CK) {
+        // (1 + jkR) e^{-jkR} with k = k + j ki: e^{ki R} ((1 - ki R) + j kR) (c - j s)
+        pre *= exp((R)a.ki * Rd);
+        const R u = (R)1 - (R)a.ki * Rd;
+        pr = pre * (u * c + kr * s);
+        pim = pre * (kr * c - u * s);
+      } else {
+        pr = pre * (c + kr * s);
+        pim = pre * (kr * c - s);
+      }
       const R Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
       // brackets x4 = tt, ff (even in xi), tf, ft (odd), times (re, im)
       R bx[4];
@@ -527,6 +561,7 @@ struct NearArgs {
   int Ne, M, nord;        // nord = M + 2 orders stored per point pair
   int ibc;                // brackets: 0 MFIE, 1 rotated-PV (IBC) (bor_kernels.py:515-560)
   double k;
+  double ki;              // imaginary part of k (lossy medium), 0 in free space
   double2* NP;            // [pp][nord][5] : G, Ktt, Ktf, Kft, Kff
 };
 
@@ -677,14 +712,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
       double sk, ck;
       sincos(k * Rd, &sk, &ck);
       const double inv = 1.0 / Rd;
+      // complex k = k + j ki: e^{-jkR} carries the decay e^{ki R}, and
+      // (1 + jkR) = (1 - ki R) + j k R
+      const double dec = a.ki != 0.0 ? exp(a.ki * Rd) : 1.0, u = 1.0 - a.ki * Rd;
       // EFIE: 2 g w  (axis pairs use the closed form instead)
-      const double ge = axis ? 0.0 : 2.0 * wq * inv / (4.0 * GH_PI);
+      const double ge = axis ? 0.0 : 2.0 * wq * inv / (4.0 * GH_PI) * dec;
       out[0] = ge * ck;
       out[1] = -ge * sk;
       // brackets: 2 w p(R) b_uv(xi)
-      const double pre = 2.0 * wq * inv * inv * inv / (4.0 * GH_PI);
+      const double pre = 2.0 * wq * inv * inv * inv / (4.0 * GH_PI) * dec;
       const double kr = k * Rd;
-      const double pr_ = pre * (ck + kr * sk), pi_ = pre * (kr * ck - sk);
+      const double pr_ = pre * (u * ck + kr * sk), pi_ = pre * (kr * ck - u * sk);
       const double Rx = rp - rq * cx, Ry = rq * sx;
       const double Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
       double btt, btf, bft, bff;
@@ -790,7 +828,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
         const double Rc = R0 > 1e-300 ? R0 : 1e-300;
         double sk, ck;
         sincos(k * R0, &sk, &ck);
-        G = make_double2(2.0 * GH_PI * ck / (4.0 * GH_PI * Rc), -2.0 * GH_PI * sk / (4.0 * GH_PI * Rc));
+        const double dec = a.ki != 0.0 ? exp(a.ki * R0) : 1.0;
+        G = make_double2(2.0 * GH_PI * ck / (4.0 * GH_PI * Rc) * dec, -2.0 * GH_PI * sk / (4.0 * GH_PI * Rc) * dec);
       } else {
         G = make_double2(0.0, 0.0);
       }
@@ -817,7 +856,8 @@ struct NearBlockArgs {
   double ce_re, ce_im;   // EFIE coefficient
   double cM;             // MFIE coefficient (brackets of NearArgs::ibc == 0)
   const double2* zs;     // IBC (NearArgs::ibc == 1): per-element surface impedance
-  double cI;             // IBC coefficient: Z += cI ((1/2) Z_s mass + brackets x Z_s(source element))
+  double cI;             // IBC coefficient of the brackets x Z_s(source element)
+  double cMass;          // IBC coefficient of (1/2) Z_s mass (cI for the IBC-EFIE, 0 for a bare rotated-PV operator)
   double2* NB;           // [3 Ne][16][2M+1]
 };
 
@@ -837,7 +877,15 @@ __global__ __launch_bounds__(256) void k_near_blocks(NearBlockArgs b, int mlanes
   const double Le = a.L[e], Lf = a.sL[f];
   const double dre = a.nodes[2 * e + 2] - a.nodes[2 * e];
   const double drf = a.snodes[2 * f + 2] - a.snodes[2 * f];
-  const double k = a.k, ik2 = 1.0 / (k * k);
+  const double k = a.k;
+  // 1/k^2: real, or (lossy medium) complex and applied to ksc below
+  double ik2 = 1.0 / (k * k), i2r = ik2, i2i = 0.0;
+  if (a.ki != 0.0) {
+    const double n2 = k * k + a.ki * a.ki;
+    i2r = (k * k - a.ki * a.ki) / (n2 * n2);
+    i2i = -2.0 * k * a.ki / (n2 * n2);
+    ik2 = 1.0;
+  }
   const int cnt = a.cell_cnt[pr.kind], off = a.cell_off[pr.kind];
   double2 E[16], K[16];
 #pragma unroll
@@ -862,7 +910,8 @@ __global__ __launch_bounds__(256) void k_near_blocks(NearBlockArgs b, int mlanes
     const double2 ktf = make_double2(rr * trp * Gs.x, rr * trp * Gs.y);
     const double2 kft = make_double2(-rr * trq * Gs.x, -rr * trq * Gs.y);
     const double2 kff = make_double2(rr * Gc.x, rr * Gc.y);
-    const double2 ksc = make_double2(W * G.x, W * G.y);
+    double2 ksc = make_double2(W * G.x, W * G.y);
+    if (a.ki != 0.0) ksc = make_double2(i2r * ksc.x - i2i * ksc.y, i2r * ksc.y + i2i * ksc.x);
     const double m = (double)am;
 #pragma unroll
     for (int ai = 0; ai < 2; ++ai)
@@ -925,7 +974,7 @@ __global__ __launch_bounds__(256) void k_near_blocks(NearBlockArgs b, int mlanes
   double2 kc = make_double2(-b.cM, 0.0), mc = make_double2(0.5 * b.cM, 0.0);
   if (a.ibc) {
     kc = make_double2(b.cI * b.zs[f].x, b.cI * b.zs[f].y);
-    mc = make_double2(0.5 * b.cI * b.zs[e].x, 0.5 * b.cI * b.zs[e].y);
+    mc = make_double2(0.5 * b.cMass * b.zs[e].x, 0.5 * b.cMass * b.zs[e].y);
   }
   for (int blk = 0; blk < 4; ++blk)
     for (int ab = 0; ab < 4; ++ab) {

@@ -115,14 +115,24 @@ int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, double k, i
 
 /* Surface impedance of problem prob (before build): zs complex [Ne]
  * (interleaved), one Leontovich Z_s (ohm) per element.  Adds the IBC-EFIE
- * operator c_ibc ((1/2) <W, Z_s J> + <W, PV curl int G M>) with the
+ * operator c_mass (1/2) <W, Z_s J> + c_ibc <W, PV curl int G M> with the
  * eliminated magnetic current M = -Z_s n x J (bor_solver.py:463-552;
- * solve_bor's assemble closure, bor_solver.py:929-934, uses c_ibc = alpha),
- * and the far field radiates M as well (bor_solver.py:697-707).  The
- * impedance operator takes the MFIE brackets' place in the fill: c_mfie must
- * be 0 when c_ibc != 0 (the reference refuses CFIE with Z_s, bor_solver.py:
- * 867-869).  All-zero zs leaves the problem a PEC one. */
-int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs, double c_ibc);
+ * solve_bor's assemble closure, bor_solver.py:929-934, uses c_ibc = c_mass =
+ * alpha), and the far field radiates M as well (bor_solver.py:697-707).
+ * c_mass = 0 with zs = 1 leaves the bare rotated principal-value operator
+ * (the PMCHWT blocks, bor_solver.py:554-583).  The impedance operator takes
+ * the MFIE brackets' place in the fill: c_mfie must be 0 when c_ibc != 0 (the
+ * reference refuses CFIE with Z_s, bor_solver.py:867-869).  All-zero zs
+ * leaves the problem a PEC one. */
+int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs, double c_ibc, double c_mass);
+
+/* Homogeneous medium of problem prob's fill (before build): wavenumber
+ * k + j k_im (k from ghost_bor_add_problem, k_im <= 0 for a lossy medium)
+ * and wave impedance eta_re + j eta_im (ohm); the EFIE operator is then
+ * j k eta (A - grad Phi / k^2) with the medium's Green function
+ * (bor_solver.py:175-199, medium=(eps, mu); bor_kernels.py:96-99 complex k).
+ * k_im != 0 needs a GHOST_FP64 handle.  Free space by default. */
+int ghost_bor_set_medium(ghost_bor* h, int prob, double k_im, double eta_re, double eta_im);
 
 /* Geometry tables, device buffers and the launch schedule. */
 int ghost_bor_build(ghost_bor* h);
