@@ -79,6 +79,7 @@ SIGNATURES = [
     ("ghost_sample_mfie_noalloc", _ci, [_ci, _ci, _ci] + [_dp] * 8 + [_cd, _dp, _dp] + [_dp] * 4),
     ("ghost_bor_set_impedance", _ci, [_vp, _ci, _dp, _cd, _cd]),
     ("ghost_bor_set_medium", _ci, [_vp, _ci, _cd, _cd, _cd]),
+    ("ghost_bor_set_interior", _ci, [_vp, _ci, _cd, _cd, _cd, _cd, _ci, _ci]),
     ("ghost_device_name", _ci, [ctypes.c_char_p, _ci]),
 ]
 
@@ -204,6 +205,13 @@ class Handle:
         the complex wave impedance."""
         eta = complex(eta)
         check(self.lib.ghost_bor_set_medium(self._h, int(prob), float(k_im), eta.real, eta.imag))
+
+    def set_interior(self, prob: int, k_in: complex, eta_in: complex, n_xi_efie: int, n_xi_bracket: int):
+        """Make problem prob a homogeneous penetrable body with this interior
+        medium (complex wavenumber and wave impedance; before build)."""
+        k_in, eta_in = complex(k_in), complex(eta_in)
+        check(self.lib.ghost_bor_set_interior(self._h, int(prob), k_in.real, k_in.imag, eta_in.real, eta_in.imag,
+                                              int(n_xi_efie), int(n_xi_bracket)))
 
     def add_dense(self, n: int, nrhs: int) -> int:
         """A caller-filled n x n system with nrhs columns; returns its system index."""

@@ -234,8 +234,10 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
     far-field launches (systems of uneven size share each launch).  Each job
     is a dict with keys points, freq_hz, thetas_deg and optionally
     formulation ("cfie"), cfie_alpha (0.5), n_modes, assembly, zs (surface
-    impedance: scalar or one complex value per element, EFIE only).  Returns
-    one solve_bor-style dict per job.  graph=True captures the batch in a
+    impedance: scalar or one complex value per element, EFIE only), medium
+    ((eps_r, mu_r): a homogeneous penetrable body on PMCHWT, FP64 only; the
+    formulation key is then ignored).  Returns one solve_bor-style dict per
+    job (solve_bor_dielectric's for a penetrable body).  graph=True captures the batch in a
     hipGraph and replays it.  nb = None takes the LU block size and GEMM tile
     from the autotuner (``autotune.best`` for the batch's largest system,
     system count and column count; measured once per shape bucket and
@@ -246,37 +248,55 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
         raise ValueError("precision must be 'fp64' or 'mixed'.")
     specs = []
     for j in jobs:
-        form = str(j.get("formulation", "cfie")).strip().lower()
-        if form not in ("efie", "mfie", "cfie"):
-            raise ValueError("formulation must be 'efie', 'mfie' or 'cfie'.")
         pts = _validate_nodes(j["points"])
         thetas = np.atleast_1d(np.asarray(j["thetas_deg"], dtype=float))
         k = 2.0 * math.pi * float(j["freq_hz"]) / C0
-        zs = impedance_per_element(j.get("zs"), len(pts) - 1)
-        if zs is not None and form == "cfie":
-            raise ValueError("A surface impedance needs formulation='efie' (CFIE with Z_s is not "
-                             "defined here; a lossy Z_s already damps interior resonances).")
-        m_max, ne, nbk, streaming = discretisation(pts, k, thetas, j.get("n_modes"), form,
-                                                   j.get("assembly", "auto"), has_ibc=zs is not None)
-        cE, cM = _coefficients(form, j.get("cfie_alpha", 0.5))
-        specs.append((pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs))
+        med = None
+        if j.get("medium") is not None:
+            # PMCHWT (bor_solver.py:1019-1094): both media on the table rules
+            if j.get("zs") is not None:
+                raise ValueError("A penetrable body takes no surface impedance.")
+            if prec != "fp64":
+                raise ValueError("Penetrable bodies (complex k) run in FP64: use precision='fp64'.")
+            eps, mu = complex(j["medium"][0]), complex(j["medium"][1])
+            mi, er = causal_medium(eps, mu)
+            form, zs = "pmchwt", None
+            m_max, ne, nbk, streaming = discretisation(pts, k, thetas, j.get("n_modes"), "efie", "tables")
+            osc_i = 2.0 * abs(k * mi) * gauss_rho_max(pts)
+            med = {"k_in": k * mi, "eta_in": ETA0 * er, "eps_r": eps, "mu_r": mu,
+                   "n_xi_in": (n_xi_efie(osc_i, m_max), n_xi_bracket(osc_i, m_max))}
+            cE, cM = 1.0, 0.0
+        else:
+            form = str(j.get("formulation", "cfie")).strip().lower()
+            if form not in ("efie", "mfie", "cfie"):
+                raise ValueError("formulation must be 'efie', 'mfie' or 'cfie'.")
+            zs = impedance_per_element(j.get("zs"), len(pts) - 1)
+            if zs is not None and form == "cfie":
+                raise ValueError("A surface impedance needs formulation='efie' (CFIE with Z_s is not "
+                                 "defined here; a lossy Z_s already damps interior resonances).")
+            m_max, ne, nbk, streaming = discretisation(pts, k, thetas, j.get("n_modes"), form,
+                                                       j.get("assembly", "auto"), has_ibc=zs is not None)
+            cE, cM = _coefficients(form, j.get("cfie_alpha", 0.5))
+        specs.append((pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs, med))
     if check_abort is not None:
         check_abort()
     variant = -1
     if nb is None:
         from . import autotune
-        n_max = max(2 * len(sp[0]) for sp in specs)
+        n_max = max((4 if sp[11] else 2) * len(sp[0]) for sp in specs)
         nsys = sum(sp[4] + 1 for sp in specs)
         nrhs = max(2 * len(sp[1]) for sp in specs)
         nb, variant = autotune.best(_PRECISION[prec], n_max, nsys, nrhs, tune=tune)
     h = _lib.Handle(_PRECISION[prec], nb=nb, residual=residual)
     try:
         h.set_gemm_variant(variant)
-        for (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs) in specs:
+        for (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs, med) in specs:
             idx = h.add_problem(pts, k, m_max, ne, nbk, cE, cM, thetas)
             if zs is not None:
                 # the impedance operator carries the EFIE's weight (bor_solver.py:929-934)
                 h.set_impedance(idx, zs, cE)
+            if med is not None:
+                h.set_interior(idx, med["k_in"], med["eta_in"], *med["n_xi_in"])
         h.build()
         it = refine_iters if prec == "mixed" else 0
         if graph:
@@ -293,13 +313,16 @@ def solve_bor_batch(jobs: Sequence[Dict], *, precision: str = "fp64", refine_ite
     if check_abort is not None:
         check_abort()
     outs, off = [], 0
-    for i, (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs) in enumerate(specs):
+    for i, (pts, thetas, form, k, m_max, ne, nbk, streaming, cE, cM, zs, med) in enumerate(specs):
         nm = 2 * m_max + 1
         r = res[off:off + nm] if res is not None else None
         off += nm
         outs.append(_result(pts, thetas, form, m_max, ne, nbk, streaming, dFs[i], r, mode_tol, prec,
                             refine_iters, times, probe, _ibc_warnings(pts, zs, form)))
         outs[-1]["lu_tiles"] = (int(nb), int(variant))
+        if med is not None:
+            outs[-1].update({"n_unknowns": int(4 * len(pts)), "eps_r": med["eps_r"], "mu_r": med["mu_r"],
+                             "n_xi_interior": med["n_xi_in"]})
     return outs
 
 
@@ -348,6 +371,28 @@ def solve_bor_pec(points, freq_hz: float, thetas_deg, n_modes: Optional[int] = N
     """PEC EFIE entry point (bor_solver.py:1007-1012)."""
     return solve_bor(points, freq_hz, thetas_deg, formulation="efie", n_modes=n_modes,
                      gauss_order=gauss_order, mode_tol=mode_tol)
+
+
+def solve_bor_dielectric(points, freq_hz: float, thetas_deg, eps_r: complex, mu_r: complex = 1.0,
+                         n_modes: Optional[int] = None, gauss_order: int = 4, mode_tol: float = 1e-6,
+                         workers: int = 1, progress: Optional[Callable] = None,
+                         check_abort: Optional[Callable] = None, *, residual: bool = True,
+                         nb: Optional[int] = None) -> Dict:
+    """Monostatic RCS of a closed homogeneous penetrable body of revolution
+    by per-mode PMCHWT (bor_solver.py:1019-1094): unknowns J and M / eta0,
+    the exterior (free space) and interior (eps_r, mu_r; lossy when Im < 0)
+    operators filled on the device, the 2 x 2 block systems assembled,
+    factored and solved there, and the far field radiating both currents.
+    ``workers`` is accepted for signature compatibility."""
+    if int(gauss_order) != 4:
+        raise ValueError("the GPU BoR path is built for gauss_order = 4 (the reference default).")
+    out = solve_bor_batch([{"points": points, "freq_hz": freq_hz, "thetas_deg": thetas_deg,
+                            "n_modes": n_modes, "medium": (eps_r, mu_r)}],
+                          precision="fp64", residual=residual, nb=nb, mode_tol=mode_tol,
+                          check_abort=check_abort)[0]
+    if progress is not None:
+        progress(out["modes_used"], out["m_max"])
+    return out
 
 
 # ─────────────────────────────────────────────────────────────────────────────
@@ -631,15 +676,20 @@ class BorPecSolver:
         """(F_theta, F_phi) of one mode's full [2Nn] solution (bor_solver.py:659-708);
         zs_pt adds the radiation of M = -Z_s n x J."""
         self._free_space("the far field")
-        if msol is not None:
-            raise NotImplementedError("independent magnetic currents (PMCHWT) are not part of the GPU BoR path.")
         sol = np.asarray(sol, dtype=np.complex128).reshape(2 * self.Nn)
+        if msol is not None:
+            if zs_pt is not None:
+                raise ValueError("give zs_pt or msol, not both.")
+            # a penetrable pair's rows: J, then M / eta0
+            sol = np.concatenate([sol, np.asarray(msol, dtype=np.complex128).reshape(2 * self.Nn) / ETA0])
         mm = max(abs(int(m)), 1)
         h = _lib.Handle(_lib.GHOST_FP64, residual=False, pairing=False)   # one mode's own currents
         try:
             h.add_problem(self.pts, self.k, mm, 64, 128, 1.0, 0.0, [float(theta_s_deg)], unmasked=True)
             if zs_pt is not None:
                 h.set_impedance(0, self._elem_zs(zs_pt), 0.0)   # far field only
+            if msol is not None:
+                h.set_interior(0, self.k, ETA0, 64, 128)   # only the pair's row layout is used
             h.build()
             h.fill_rhs()   # the far field reads the Bessel table of the aspect
             h.set_solution(0, int(m), np.stack([sol, sol], axis=1))

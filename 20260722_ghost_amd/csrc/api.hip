@@ -161,6 +161,13 @@ struct Problem {
   // the wave impedance eta; free space by default
   double ki = 0.0;
   double eta_re = 376.730313668, eta_im = 0.0;
+  // penetrable body (PMCHWT): this problem's fill keeps its EFIE and bracket
+  // operators apart (d_zfar / d_NB and d_zfar2 / d_NB2); the exterior problem
+  // names its interior-medium twin in ghost_bor::inner
+  bool pen = false;
+  int inner = -1;
+  void* d_zfar2 = nullptr;
+  double2* d_NB2 = nullptr;
   double2* d_zs = nullptr;
   std::vector<double> nodes, theta;
   bool axis0 = false, axis1 = false;
@@ -227,6 +234,7 @@ struct ghost_bor {
   bool built = false;
   bool solved = false;               // the augmented columns hold the solution of the current fill
   std::vector<Problem> probs;
+  std::vector<Problem> inner;        // interior media of penetrable problems (fill only)
   std::vector<SysInfo> sys;          // one entry per (problem, mode) and per dense system
   std::vector<SysInfo> ss;           // the systems factored and solved (nrhs = all their columns)
   int pairing = 1;                   // solve mode -m with mode m's factors
@@ -545,6 +553,7 @@ extern "C" int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs,
   if (!zs) return fail(GHOST_ERR_ARG, "null impedance array");
   if (!std::isfinite(c_ibc) || !std::isfinite(c_mass)) return fail(GHOST_ERR_ARG, "impedance coefficients must be finite");
   Problem& p = h->probs[prob];
+  if (p.pen) return fail(GHOST_ERR_ARG, "a penetrable body takes no surface impedance");
   std::vector<double2> z(p.Ne);
   bool any = false;
   for (int e = 0; e < p.Ne; ++e) {
@@ -575,6 +584,7 @@ extern "C" int ghost_bor_set_medium(ghost_bor* h, int prob, double k_im, double 
   if (k_im != 0.0 && h->precision != GHOST_FP64)
     return fail(GHOST_ERR_ARG, "a lossy medium (complex k) needs an FP64 handle");
   Problem& p = h->probs[prob];
+  if (p.pen) return fail(GHOST_ERR_ARG, "the exterior of a penetrable body is free space");
   p.ki = k_im;
   p.eta_re = eta_re;
   p.eta_im = eta_im;
@@ -583,6 +593,62 @@ extern "C" int ghost_bor_set_medium(ghost_bor* h, int prob, double k_im, double 
 
 // A caller-filled dense system (n x n, n_rhs columns) solved in the same batch
 // as the mode systems; it belongs to no problem (no fill, no far field).
+// Turn problem prob into a homogeneous penetrable body (PMCHWT,
+// bor_solver.py:1019-1094): the problem's own medium is the exterior (free
+// space), the interior medium has k = k_re + j k_im and wave impedance eta;
+// each mode system becomes the 2 x 2 block system in (J, M / eta0) of twice
+// the single-surface size.  FP64 handles only.
+extern "C" int ghost_bor_set_interior(ghost_bor* h, int prob, double k_re, double k_im, double eta_re,
+                                      double eta_im, int n_xi_efie, int n_xi_bracket) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "set the interior medium before build");
+  if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index %d", prob);
+  if (h->precision != GHOST_FP64) return fail(GHOST_ERR_ARG, "a penetrable body needs an FP64 handle");
+  if (!std::isfinite(k_re) || !std::isfinite(k_im) || !std::isfinite(eta_re) || !std::isfinite(eta_im))
+    return fail(GHOST_ERR_ARG, "interior medium parameters must be finite");
+  if (!(k_re > 0.0) || k_im > 0.0) return fail(GHOST_ERR_ARG, "interior k needs Re k > 0 and Im k <= 0");
+  if (eta_re == 0.0 && eta_im == 0.0) return fail(GHOST_ERR_ARG, "interior wave impedance must be nonzero");
+  auto pow2 = [](int v) { return v >= 4 && (v & (v - 1)) == 0; };
+  if (!pow2(n_xi_efie) || !pow2(n_xi_bracket) || n_xi_efie > 4096 || n_xi_bracket > 4096)
+    return fail(GHOST_ERR_ARG, "interior xi grid sizes must be powers of two in [4, 4096]");
+  Problem& p = h->probs[prob];
+  if (p.pen) return fail(GHOST_ERR_STATE, "problem %d already has an interior medium", prob);
+  if (p.has_zs) return fail(GHOST_ERR_ARG, "a penetrable body takes no surface impedance");
+  if (p.cE != 1.0 || p.cM != 0.0) return fail(GHOST_ERR_ARG, "add a penetrable body with c_efie = 1, c_mfie = 0");
+  if (p.ki != 0.0 || p.eta_re != 376.730313668 || p.eta_im != 0.0)
+    return fail(GHOST_ERR_ARG, "the exterior of a penetrable body is free space");
+  if (far_fill_waves(8, (p.M + 2 + 15) / 16, n_xi_efie, n_xi_bracket) == 0)
+    return fail(GHOST_ERR_ARG, "far fill: interior azimuth tables of n_xi %d / %d exceed the 160 KiB of LDS",
+                n_xi_efie, n_xi_bracket);
+  for (int m = -p.M; m <= p.M; ++m)
+    if (2 * h->sys[p.sys0 + m + p.M].n > 4096)
+      return fail(GHOST_ERR_ARG, "PMCHWT system size %d exceeds the panel limit 4096", 2 * h->sys[p.sys0 + m + p.M].n);
+  // the bracket operators of both media at unit weight, no mass term
+  p.zs.assign(p.Ne, make_double2(1.0, 0.0));
+  p.has_zs = true;
+  p.cI = 1.0;
+  p.cMass = 0.0;
+  p.pen = true;
+  Problem q;
+  q.Nn = p.Nn; q.Ne = p.Ne; q.P = p.P; q.M = p.M; q.nmodes = p.nmodes;
+  q.nE = n_xi_efie; q.nB = n_xi_bracket;
+  q.k = k_re; q.ki = k_im; q.eta_re = eta_re; q.eta_im = eta_im;
+  q.cE = 1.0; q.cM = 0.0;
+  q.zs = p.zs; q.has_zs = true; q.cI = 1.0; q.cMass = 0.0; q.pen = true;
+  q.ntheta = 1; q.nrhs = 0; q.theta.assign(1, 0.0);
+  q.nodes = p.nodes; q.L = p.L; q.trho = p.trho; q.tz = p.tz;
+  q.axis0 = p.axis0; q.axis1 = p.axis1;
+  q.sys0 = -1;
+  p.inner = (int)h->inner.size();
+  h->inner.push_back(std::move(q));
+  for (int m = -p.M; m <= p.M; ++m) {
+    SysInfo& sy = h->sys[p.sys0 + m + p.M];
+    sy.pen = sy.n;
+    sy.n *= 2;
+  }
+  return 0;
+}
+
 extern "C" int ghost_bor_add_dense(ghost_bor* h, int n, int nrhs, int* sys_idx) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
   if (h->built) return fail(GHOST_ERR_STATE, "add systems before build");
@@ -754,6 +820,10 @@ static int prepare_problem(ghost_bor* h, Problem& p) {
   const size_t zel = (size_t)(2 * p.Nn) * (2 * p.Nn) * p.zmodes;
   const size_t zsz = h->precision == GHOST_FP64 ? sizeof(double2) : sizeof(float2);
   if ((rc = dalloc(h, &p.d_zfar, zel * zsz))) return rc;
+  if (p.pen) {
+    if ((rc = dalloc(h, &p.d_zfar2, zel * zsz))) return rc;
+    if ((rc = dalloc(h, &p.d_NB2, (size_t)3 * Ne * 16 * p.nmodes * sizeof(double2)))) return rc;
+  }
 
   if (p.has_zs) {
     if ((rc = dalloc(h, &p.d_zs, (size_t)Ne * sizeof(double2)))) return rc;
@@ -1229,6 +1299,8 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
   int rc;
   for (auto& p : h->probs)
     if ((rc = prepare_problem(h, p))) return rc;
+  for (auto& p : h->inner)
+    if ((rc = prepare_problem(h, p))) return rc;
   if ((rc = build_schedule(h))) return rc;
   const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
   if ((rc = dalloc(h, &h->d_sys, h->sys.size() * sizeof(SysInfo)))) return rc;
@@ -1339,6 +1411,7 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
   fa.cz = 2.0 * M_PI * p.cI;
   if (ibc && split) return fail(GHOST_ERR_ARG, "perturbed (adjoint) fills of an impedance body are not supported");
   fa.Z = p.d_zfar;
+  fa.Z2 = p.pen ? p.d_zfar2 : nullptr;
   fa.neg = h->pairing ? 0 : 1;
   fa.zmodes = p.zmodes;
   fa.zoff = p.zoff;
@@ -1409,11 +1482,13 @@ static ExciteArgs excite_args(const Problem& p, const Geom& tg) {
   ea.theta = p.d_theta;
   ea.J = p.d_J;
   ea.E = p.d_E;
-  ea.zs = p.has_zs ? p.d_zs : nullptr;
+  ea.zs = (p.has_zs && !p.pen) ? p.d_zs : nullptr;   // a penetrable body's unit weights radiate nothing
   return ea;
 }
 
-static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg) {
+// Operators of one problem's medium: far blocks (d_zfar, and d_zfar2 for a
+// penetrable body's brackets) and the near band (d_NB, d_NB2).
+static int fill_operators(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg) {
   const bool mixed = h->precision == GHOST_MIXED;
   const int P = p.P;
   if ((p.M + 2 + 7) / 8 > 16 || p.M + 1 > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near band");
@@ -1422,6 +1497,9 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   prof_begin(h);
   hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (zero_n + 255) / 256)), dim3(256), 0,
                      h->stream, (double2*)p.d_zfar, zero_n);
+  if (p.pen)
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (zero_n + 255) / 256)), dim3(256), 0,
+                       h->stream, (double2*)p.d_zfar2, zero_n);
   prof_end(h, K_ZERO, 0.0);
 
   LAUNCHCHK();
@@ -1503,6 +1581,7 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   nb.cI = p.cI;
   nb.cMass = p.cMass;
   nb.NB = p.d_NB;
+  nb.NB2 = p.pen ? p.d_NB2 : nullptr;
   const int mlanes = ((p.M + 1 + 15) / 16) * 16;
   if (mlanes > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near blocks");
   prof_begin(h);
@@ -1515,7 +1594,60 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
     HIPCHK(hipEventRecord(h->ev_join, ns));
     HIPCHK(hipStreamWaitEvent(h->stream, h->ev_join, 0));
   }
-  if (rc) return rc;
+  return rc;
+}
+
+// PMCHWT systems of a penetrable problem from its exterior and interior
+// operators (both filled).
+static int assemble_pen(ghost_bor* h, Problem& p) {
+  const Problem& q = h->inner[p.inner];
+  AsmPenArgs aa;
+  aa.Te = (const double2*)p.d_zfar;
+  aa.Be = (const double2*)p.d_zfar2;
+  aa.Ti = (const double2*)q.d_zfar;
+  aa.Bi = (const double2*)q.d_zfar2;
+  aa.NTe = p.d_NB;
+  aa.NBe = p.d_NB2;
+  aa.NTi = q.d_NB;
+  aa.NBi = q.d_NB2;
+  aa.sys = h->d_sys + p.sys0;
+  aa.Nn = p.Nn;
+  aa.Ne = p.Ne;
+  aa.M = p.M;
+  aa.mlo = h->pairing ? p.M : 0;
+  if (q.zmodes != p.zmodes || q.zoff != p.zoff) return fail(GHOST_ERR_STATE, "interior far blocks differ in layout");
+  aa.zmodes = p.zmodes;
+  aa.zoff = p.zoff;
+  aa.eta0 = 376.730313668;
+  {
+    // (eta0 / eta_i)^2
+    const double n2 = q.eta_re * q.eta_re + q.eta_im * q.eta_im;
+    const double ir = aa.eta0 * q.eta_re / n2, ii = -aa.eta0 * q.eta_im / n2;
+    aa.r2_re = ir * ir - ii * ii;
+    aa.r2_im = 2.0 * ir * ii;
+  }
+  aa.A = (double2*)h->d_A;
+  aa.A64 = h->d_A64;
+  dim3 g(2 * p.Nn, (2 * p.Nn + 31) / 32, (p.nmodes - aa.mlo + 31) / 32);
+  prof_begin(h);
+  hipLaunchKernelGGL(k_assemble_pen, g, dim3(256), 0, h->stream, aa);
+  prof_end(h, K_ASSEMBLE, 0.0);
+  LAUNCHCHK();
+  return 0;
+}
+
+static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg) {
+  const bool mixed = h->precision == GHOST_MIXED;
+  int rc;
+  if (p.pen) {
+    if (p.inner < 0 || mixed) return fail(GHOST_ERR_STATE, "penetrable problem without its FP64 interior");
+    Problem& q = h->inner[p.inner];
+    if ((rc = fill_operators(h, q, base_geom(q), base_geom(q)))) return rc;
+    if ((rc = fill_operators(h, p, tg, sg))) return rc;
+    if ((rc = assemble_pen(h, p))) return rc;
+    return rhs_problem(h, p, tg);
+  }
+  if ((rc = fill_operators(h, p, tg, sg))) return rc;
 
   // assemble
   AsmArgs aa;
@@ -1602,7 +1734,7 @@ extern "C" int ghost_bor_set_solution(ghost_bor* h, int prob, int m, const doubl
   for (int r = 0; r < s.n; ++r) {
     for (int c = 0; c < s.nrhs; ++c) {
       // mode -m of a pair shares mode m's storage: x_m = S [x_{-m}(VV), -x_{-m}(HH)]
-      const double f = (s.flip && ((r >= s.nt) != ((c & 1) == 1))) ? -1.0 : 1.0;
+      const double f = s.flip ? pair_row_sign(s, r) * ((c & 1) ? -1.0 : 1.0) : 1.0;
       row[c] = make_double2(f * X[2 * ((size_t)r * s.nrhs + c)], f * X[2 * ((size_t)r * s.nrhs + c) + 1]);
     }
     double2* dst = h->precision == GHOST_MIXED ? h->d_X64 + s.x_off + (size_t)r * s.ncols + s.col
@@ -2153,9 +2285,9 @@ extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, do
     if (mixed) { float2 v = ((float2*)rowbuf.data())[c]; return make_double2(v.x, v.y); }
     return ((double2*)rowbuf.data())[c];
   };
-  auto sgn = [&](int r, int c) { return (s.flip && ((r >= s.nt) != (c >= s.nt))) ? -1.0 : 1.0; };
+  auto sgn = [&](int r, int c) { return s.flip ? pair_row_sign(s, r) * pair_row_sign(s, c) : 1.0; };
   // right-hand sides and currents of mode -m: S [VV, -HH] of mode m's
-  auto rsgn = [&](int r, int c) { return (s.flip && ((r >= s.nt) != ((c & 1) == 1))) ? -1.0 : 1.0; };
+  auto rsgn = [&](int r, int c) { return s.flip ? pair_row_sign(s, r) * ((c & 1) ? -1.0 : 1.0) : 1.0; };
   auto put = [](double* o, size_t i, double2 v, double f) { o[2 * i] = f * v.x; o[2 * i + 1] = f * v.y; };
   for (int r = 0; r < s.n; ++r) {
     const bool aug_row = which == 0 ? !(mixed && h->d_A64) : !mixed;

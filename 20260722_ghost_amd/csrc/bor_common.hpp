@@ -143,7 +143,19 @@ struct SysInfo {
   int col;       // first of this mode's columns among them
   int flip;      // 1: mode -m of a pair (J_phi rows and HH columns change sign)
   int owner;     // index of the solved system (in solve order)
+  int pen;       // PMCHWT pair (penetrable body): the single-surface size n1,
+                 // rows [0, n1) J and [n1, 2 n1) M / eta0, each laid out as
+                 // above (J_t then J_phi); 0 for a single-surface system
   long long a_off;   // element offset of the augmented system [n x ld]
   long long x_off;   // element offset of the n x ncols solution / RHS copy
   long long a64_off; // element offset of the FP64 n x n copy (mixed precision)
 };
+
+// Sign of row r under the pairing's diagonal: S = diag(I, -I) on (J_t, J_phi)
+// of a single-surface system, diag(S, -S) on (J, M / eta0) of a PMCHWT pair.
+__host__ __device__ inline double pair_row_sign(const SysInfo& s, int r) {
+  const bool second = s.pen && r >= s.pen;
+  const int rr = second ? r - s.pen : r;
+  const double b = rr >= s.nt ? -1.0 : 1.0;
+  return second ? -b : b;
+}

@@ -204,7 +204,11 @@ __global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, i
       }
     }
     if (2 * it < s.nrhs) {
-      const int nf = s.n - s.nt;
+      // a PMCHWT pair also takes eta0 <W, H_inc> in rows n1 + r: by duality
+      // -<W, E_inc>(HH) in the VV column and <W, E_inc>(VV) in the HH column
+      // (bor_solver.py:647-657)
+      const int n1 = s.pen ? s.pen : s.n;
+      const int nf = n1 - s.nt;
       const int rt = node - s.t0, rf = node - s.f0;
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
@@ -212,6 +216,7 @@ __global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, i
         if (b == 0) { if (rt < 0 || rt >= s.nt) continue; r = rt; }
         else { if (rf < 0 || rf >= nf) continue; r = s.nt + rf; }
         const double2* acc = b == 0 ? acT : acF;
+        const double2 h0 = make_double2(-acc[1].x, -acc[1].y), h1 = acc[0];
         if (Abase) {
           Ca* A = (Ca*)Abase + s.a_off + (size_t)r * s.ld + s.n + s.col + 2 * it;
           Ca o0, o1;
@@ -219,11 +224,22 @@ __global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, i
           o1.x = (Ra)acc[1].x; o1.y = (Ra)acc[1].y;
           A[0] = o0;
           A[1] = o1;
+          if (s.pen) {
+            Ca* AH = A + (size_t)n1 * s.ld;
+            o0.x = (Ra)h0.x; o0.y = (Ra)h0.y;
+            o1.x = (Ra)h1.x; o1.y = (Ra)h1.y;
+            AH[0] = o0;
+            AH[1] = o1;
+          }
         }
         if (B64) {
           double2* bb = B64 + s.x_off + (size_t)r * s.ncols + s.col + 2 * it;
           bb[0] = acc[0];
           bb[1] = acc[1];
+          if (s.pen) {
+            bb[(size_t)n1 * s.ncols] = h0;
+            bb[(size_t)n1 * s.ncols + 1] = h1;
+          }
         }
       }
     }
@@ -276,31 +292,39 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
   else { X = (const Cx*)Xbase + s.x_off + s.col; ldx = s.ncols; }
   // mode -m of a pair reads mode m's currents: x_{-m} = S [x_m(VV), -x_m(HH)]
   const double ft = (s.flip && pol == 1) ? -1.0 : 1.0, ff = (s.flip && pol == 0) ? -1.0 : 1.0;
-  const int nf = s.n - s.nt;
+  const int n1 = s.pen ? s.pen : s.n;
+  const int nf = n1 - s.nt;
   const PointSet& P = a.pts;
-  auto xt = [&](int node) -> double2 {
+  // rows: off 0 the electric current J; off n1 (a PMCHWT pair) M / eta0,
+  // whose mode -m entries take the opposite signs (x_{-m} = diag(S, -S) ...)
+  auto xt = [&](int node, int off, double sg) -> double2 {
     const int r = node - s.t0;
     if (r < 0 || r >= s.nt) return make_double2(0.0, 0.0);
-    const Cx v = X[(size_t)r * ldx + col];
-    return make_double2(ft * (double)v.x, ft * (double)v.y);
+    const Cx v = X[(size_t)(off + r) * ldx + col];
+    return make_double2(sg * (double)v.x, sg * (double)v.y);
   };
-  auto xf = [&](int node) -> double2 {
+  auto xf = [&](int node, int off, double sg) -> double2 {
     const int r = node - s.f0;
     if (r < 0 || r >= nf) return make_double2(0.0, 0.0);
-    const Cx v = X[(size_t)(s.nt + r) * ldx + col];
-    return make_double2(ff * (double)v.x, ff * (double)v.y);
+    const Cx v = X[(size_t)(off + s.nt + r) * ldx + col];
+    return make_double2(sg * (double)v.x, sg * (double)v.y);
   };
   double2 acc = make_double2(0.0, 0.0);
   int e_prev = -2;   // no element yet (element 0 must not look like a successor)
   double2 t0v = make_double2(0, 0), t1v = t0v, f0v = t0v, f1v = t0v;
+  double2 mt0 = t0v, mt1 = t0v, mf0 = t0v, mf1 = t0v;   // M / eta0 of a PMCHWT pair
   for (int p = p0; p < p1; ++p) {
     const int e = p >> 2;
     if (e != e_prev) {
       // the next element's first node is this element's second: carry it over
-      if (e == e_prev + 1) { t0v = t1v; f0v = f1v; }
-      else { t0v = xt(e); f0v = xf(e); }
-      t1v = xt(e + 1);
-      f1v = xf(e + 1);
+      if (e == e_prev + 1) { t0v = t1v; f0v = f1v; mt0 = mt1; mf0 = mf1; }
+      else {
+        t0v = xt(e, 0, ft); f0v = xf(e, 0, ff);
+        if (s.pen) { mt0 = xt(e, n1, ff); mf0 = xf(e, n1, ft); }
+      }
+      t1v = xt(e + 1, 0, ft);
+      f1v = xf(e + 1, 0, ff);
+      if (s.pen) { mt1 = xt(e + 1, n1, ff); mf1 = xf(e + 1, n1, ft); }
       e_prev = e;
     }
     const double T0 = P.T0[p], T1 = P.T1[p];
@@ -343,6 +367,21 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
       }
       term.x += ie * (z.x * mt.x - z.y * mt.y);
       term.y += ie * (z.x * mt.y + z.y * mt.x);
+    }
+    if (s.pen) {
+      // an independent magnetic current M = eta0 M' (bor_solver.py:697-707):
+      // F_theta += proj_phi(M'), F_phi -= proj_theta(M') under the common
+      // -j k eta0 / (4 pi)
+      const double2 Mt = make_double2(T0 * mt0.x + T1 * mt1.x, T0 * mt0.y + T1 * mt1.y);
+      const double2 Mf = make_double2(T0 * mf0.x + T1 * mf1.x, T0 * mf0.y + T1 * mf1.y);
+      if (pol == 0) {
+        term.x += tr * (Mt.x * Isin.x - Mt.y * Isin.y) + Mf.x * Icos.x - Mf.y * Icos.y;
+        term.y += tr * (Mt.x * Isin.y + Mt.y * Isin.x) + Mf.x * Icos.y + Mf.y * Icos.x;
+      } else {
+        const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
+        term.x += -(Mt.x * u.x - Mt.y * u.y) + ct * (Mf.x * Isin.x - Mf.y * Isin.y);
+        term.y += -(Mt.x * u.y + Mt.y * u.x) + ct * (Mf.x * Isin.y + Mf.y * Isin.x);
+      }
     }
     const double2 E = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
     acc.x += term.x * E.x - term.y * E.y;

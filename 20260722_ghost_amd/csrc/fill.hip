@@ -45,6 +45,7 @@ struct FarArgs {
   const double2* zs;     // IBC: per-element surface impedance (source element's applies)
   double cz;             // IBC: bracket coefficient 2 pi c_ibc (times zs of the source element)
   void* Z;               // Zfar [2Nn][2Nn][zmodes] complex<R>
+  void* Z2;              // IBC: bracket part into its own far blocks (same layout), nullptr: summed into Z
 };
 
 // Reduce-scatter step between two lanes of a DPP row whose lane bit differs:
@@ -232,10 +233,30 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
       const R Kor = kO[t][2 * bi + 1], Koi = -kO[t][2 * bi];  // -j (re + j im)
       R zer, zei, zor, zoi;
       if constexpr (IBC) {
-        zer = ce_re * er - ce_im * ei + (cmr * Ker - cmi * Kei);
-        zei = ce_re * ei + ce_im * er + (cmr * Kei + cmi * Ker);
-        zor = ce_re * orr - ce_im * oi + (cmr * Kor - cmi * Koi);
-        zoi = ce_re * oi + ce_im * orr + (cmr * Koi + cmi * Kor);
+        const R ber = cmr * Ker - cmi * Kei, bei = cmr * Kei + cmi * Ker;
+        const R bor = cmr * Kor - cmi * Koi, boi = cmr * Koi + cmi * Kor;
+        if (a.Z2) {
+          // the bracket operator on its own (a penetrable body's coupling blocks)
+          C2* Zb = (C2*)a.Z2;
+          C2* qe = Zb + ((size_t)rowE * ncol + colE + bi) * a.zmodes + a.zoff;
+          C2* qo = Zb + ((size_t)rowE * ncol + colO + bi) * a.zmodes + a.zoff;
+          C2 v;
+          v = qe[n]; v.x += ber; v.y += bei; qe[n] = v;
+          v = qo[n]; v.x += bor; v.y += boi; qo[n] = v;
+          if (n > 0 && a.neg) {
+            v = qe[-n]; v.x += ber; v.y += bei; qe[-n] = v;
+            v = qo[-n]; v.x -= bor; v.y -= boi; qo[-n] = v;
+          }
+          zer = ce_re * er - ce_im * ei;
+          zei = ce_re * ei + ce_im * er;
+          zor = ce_re * orr - ce_im * oi;
+          zoi = ce_re * oi + ce_im * orr;
+        } else {
+          zer = ce_re * er - ce_im * ei + ber;
+          zei = ce_re * ei + ce_im * er + bei;
+          zor = ce_re * orr - ce_im * oi + bor;
+          zoi = ce_re * oi + ce_im * orr + boi;
+        }
       } else {
         zer = ce_re * er - ce_im * ei + cm * Ker;
         zei = ce_re * ei + ce_im * er + cm * Kei;
@@ -859,6 +880,7 @@ struct NearBlockArgs {
   double cI;             // IBC coefficient of the brackets x Z_s(source element)
   double cMass;          // IBC coefficient of (1/2) Z_s mass (cI for the IBC-EFIE, 0 for a bare rotated-PV operator)
   double2* NB;           // [3 Ne][16][2M+1]
+  double2* NB2;          // bracket and mass part into its own blocks (same layout), nullptr: summed into NB
 };
 
 // one work-group per near pair slot; thread = (|m| lane, cell group): with
@@ -976,21 +998,28 @@ __global__ __launch_bounds__(256) void k_near_blocks(NearBlockArgs b, int mlanes
     kc = make_double2(b.cI * b.zs[f].x, b.cI * b.zs[f].y);
     mc = make_double2(0.5 * b.cMass * b.zs[e].x, 0.5 * b.cMass * b.zs[e].y);
   }
+  // NB2 set: the EFIE part to NB, the bracket and mass part to NB2
+  double2* out2 = b.NB2 ? b.NB2 + (size_t)slot * 16 * nmodes : nullptr;
   for (int blk = 0; blk < 4; ++blk)
     for (int ab = 0; ab < 4; ++ab) {
       const double2 Ev = E[4 * blk + ab], Kv = K[4 * blk + ab];
-      const double er = b.ce_re * Ev.x - b.ce_im * Ev.y + (kc.x * Kv.x - kc.y * Kv.y);
-      const double ei = b.ce_re * Ev.y + b.ce_im * Ev.x + (kc.x * Kv.y + kc.y * Kv.x);
+      const double xr = b.ce_re * Ev.x - b.ce_im * Ev.y, xi = b.ce_re * Ev.y + b.ce_im * Ev.x;
+      const double kr = kc.x * Kv.x - kc.y * Kv.y, ki = kc.x * Kv.y + kc.y * Kv.x;
       const bool odd = (blk == 1 || blk == 2);
-      double zr = er, zi = ei;
-      if (!odd) { zr += mc.x * Mab[ab]; zi += mc.y * Mab[ab]; }
+      const double mr = odd ? 0.0 : mc.x * Mab[ab], mi = odd ? 0.0 : mc.y * Mab[ab];
+      const double sg = odd ? -1.0 : 1.0;   // mode -m: odd blocks change sign, the mass does not
       double2* o = out + (size_t)(4 * blk + ab) * nmodes + a.M;
-      o[am] = make_double2(zr, zi);
-      if (am > 0) {
-        const double sg = odd ? -1.0 : 1.0;
-        double zr2 = sg * er, zi2 = sg * ei;
-        if (!odd) { zr2 += mc.x * Mab[ab]; zi2 += mc.y * Mab[ab]; }
-        o[-am] = make_double2(zr2, zi2);
+      if (out2) {
+        double2* o2 = out2 + (size_t)(4 * blk + ab) * nmodes + a.M;
+        o[am] = make_double2(xr, xi);
+        o2[am] = make_double2(kr + mr, ki + mi);
+        if (am > 0) {
+          o[-am] = make_double2(sg * xr, sg * xi);
+          o2[-am] = make_double2(sg * kr + mr, sg * ki + mi);
+        }
+      } else {
+        o[am] = make_double2(xr + kr + mr, xi + ki + mi);
+        if (am > 0) o[-am] = make_double2(sg * (xr + kr) + mr, sg * (xi + ki) + mi);
       }
     }
 }
@@ -1067,6 +1096,114 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
     Ca o; o.x = (Ra)v.x; o.y = (Ra)v.y;
     A[(size_t)r * s.ld + c] = o;
     if (a.A64) a.A64[s.a64_off + (size_t)r * s.n + c] = v;
+  }
+}
+
+// Near-band part of one node-level entry (row node ii, column node jj, block
+// blk = 2 (row half) + column half) of mode index mi: the element pairs
+// (e, f) with e in {ii - 1, ii}, f in {jj - 1, jj} and |e - f| <= 1.
+__device__ inline double2 near_entry(const double2* NB, int ii, int jj, int blk, int mi, int Ne, int nmodes) {
+  double2 v = make_double2(0.0, 0.0);
+  if (abs(ii - jj) > 2) return v;
+#pragma unroll
+  for (int ai = 0; ai < 2; ++ai)
+#pragma unroll
+    for (int bb = 0; bb < 2; ++bb) {
+      const int e = ii - ai, f = jj - bb;
+      if (e < 0 || f < 0 || e >= Ne || f >= Ne || abs(e - f) > 1) continue;
+      const int slot = 3 * e + (f - e + 1);
+      const double2 w = NB[((size_t)slot * 16 + 4 * blk + 2 * ai + bb) * nmodes + mi];
+      v.x += w.x;
+      v.y += w.y;
+    }
+  return v;
+}
+
+// PMCHWT assembly of a homogeneous penetrable body (bor_solver.py:1047-1056):
+// from the exterior and interior EFIE operators T and rotated-PV bracket
+// operators B (far blocks + near band, node based), the masked system
+//   [[Te + Ti, -eta0 (Pe + Pi)], [eta0 (Pe + Pi), Te + (eta0/eta_i)^2 Ti]]
+// for the unknowns (J, M / eta0), where P takes B's columns remapped,
+// P = [[-Btf, Btt], [-Bff, Bft]] (bor_solver.py:554-575): column j < Nn of P
+// is -(column j + Nn of B), column j >= Nn is column j - Nn.  FP64 only.
+struct AsmPenArgs {
+  const double2* Te;         // far blocks [2Nn][2Nn][zmodes] of the four operators
+  const double2* Be;
+  const double2* Ti;
+  const double2* Bi;
+  const double2* NTe;        // near blocks [3 Ne][16][nmodes]
+  const double2* NBe;
+  const double2* NTi;
+  const double2* NBi;
+  const SysInfo* sys;        // systems of this problem, index = m + M (pen = single-surface size)
+  int Nn, Ne, M, mlo, zmodes, zoff;
+  double eta0;
+  double r2_re, r2_im;       // (eta0 / eta_i)^2
+  double2* A;                // augmented systems
+  double2* A64;              // optional FP64 copy of the n x n block
+};
+
+__global__ __launch_bounds__(256) void k_assemble_pen(AsmPenArgs a) {
+  __shared__ double2 tile[4][32][33];   // Te, Ti at column j; Be, Bi at the partner column
+  const int i_row = blockIdx.x;
+  const int j0 = blockIdx.y * 32;
+  const int m0 = a.mlo + blockIdx.z * 32;
+  const int nmodes = 2 * a.M + 1;
+  const int Nn = a.Nn;
+  const size_t ncol = 2 * (size_t)Nn;
+  for (int t = threadIdx.x; t < 4 * 32 * 32; t += 256) {
+    const int op = t >> 10, jj = (t >> 5) & 31, mm = t & 31;
+    const int j = j0 + jj, mi = m0 + mm;
+    double2 v = make_double2(0, 0);
+    if (j < (int)ncol && mi < nmodes) {
+      const int jc = op < 2 ? j : (j < Nn ? j + Nn : j - Nn);
+      const double2* Z = op == 0 ? a.Te : op == 1 ? a.Ti : op == 2 ? a.Be : a.Bi;
+      v = Z[((size_t)i_row * ncol + jc) * a.zmodes + (mi - a.M + a.zoff)];
+    }
+    tile[op][jj][mm] = v;
+  }
+  __syncthreads();
+  const int bi = i_row >= Nn ? 1 : 0;
+  const int ii = i_row - bi * Nn;
+  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
+    const int mm = t >> 5, jj = t & 31;
+    const int j = j0 + jj, mi = m0 + mm;
+    if (j >= (int)ncol || mi >= nmodes) continue;
+    const SysInfo s = a.sys[mi];
+    const int n1 = s.pen, nf = n1 - s.nt;
+    int r;
+    if (bi == 0) { r = ii - s.t0; if (r < 0 || r >= s.nt) continue; }
+    else { r = ii - s.f0; if (r < 0 || r >= nf) continue; r += s.nt; }
+    const int bj = j >= Nn ? 1 : 0;
+    const int jn = j - bj * Nn;
+    int c;
+    if (bj == 0) { c = jn - s.t0; if (c < 0 || c >= s.nt) continue; }
+    else { c = jn - s.f0; if (c < 0 || c >= nf) continue; c += s.nt; }
+    const int blk = 2 * bi + bj, blkp = 2 * bi + (1 - bj);
+    double2 te = tile[0][jj][mm], ti = tile[1][jj][mm], be = tile[2][jj][mm], bq = tile[3][jj][mm];
+    const double2 nte = near_entry(a.NTe, ii, jn, blk, mi, a.Ne, nmodes);
+    const double2 nti = near_entry(a.NTi, ii, jn, blk, mi, a.Ne, nmodes);
+    const double2 nbe = near_entry(a.NBe, ii, jn, blkp, mi, a.Ne, nmodes);
+    const double2 nbi = near_entry(a.NBi, ii, jn, blkp, mi, a.Ne, nmodes);
+    te.x += nte.x; te.y += nte.y;
+    ti.x += nti.x; ti.y += nti.y;
+    // eta0 (Pe + Pi) at column j
+    const double ps = (bj == 0 ? -a.eta0 : a.eta0);
+    const double2 pv = make_double2(ps * (be.x + nbe.x + bq.x + nbi.x), ps * (be.y + nbe.y + bq.y + nbi.y));
+    const double2 v00 = make_double2(te.x + ti.x, te.y + ti.y);
+    const double2 v11 = make_double2(te.x + a.r2_re * ti.x - a.r2_im * ti.y, te.y + a.r2_re * ti.y + a.r2_im * ti.x);
+    double2* A = a.A + s.a_off;
+    A[(size_t)r * s.ld + c] = v00;
+    A[(size_t)r * s.ld + n1 + c] = make_double2(-pv.x, -pv.y);
+    A[(size_t)(n1 + r) * s.ld + c] = pv;
+    A[(size_t)(n1 + r) * s.ld + n1 + c] = v11;
+    if (a.A64) {
+      double2* B = a.A64 + s.a64_off;
+      B[(size_t)r * s.n + c] = v00;
+      B[(size_t)r * s.n + n1 + c] = make_double2(-pv.x, -pv.y);
+      B[(size_t)(n1 + r) * s.n + c] = pv;
+      B[(size_t)(n1 + r) * s.n + n1 + c] = v11;
+    }
   }
 }
 

@@ -134,6 +134,19 @@ int ghost_bor_set_impedance(ghost_bor* h, int prob, const double* zs, double c_i
  * k_im != 0 needs a GHOST_FP64 handle.  Free space by default. */
 int ghost_bor_set_medium(ghost_bor* h, int prob, double k_im, double eta_re, double eta_im);
 
+/* Make problem prob a closed homogeneous penetrable body (PMCHWT,
+ * bor_solver.py:1019-1094, solve_bor_dielectric): the problem, added with
+ * c_efie = 1 and c_mfie = 0 in free space, is the exterior; the interior
+ * medium has wavenumber k_re + j k_im (k_im <= 0) and wave impedance
+ * eta_re + j eta_im, filled on its own azimuth grids n_xi_efie /
+ * n_xi_bracket.  Each mode system becomes
+ *   [[Te + Ti, -eta0 (Pe + Pi)], [eta0 (Pe + Pi), Te + (eta0/eta_i)^2 Ti]]
+ * in (J, M / eta0), masked per half (twice the single-surface size), its
+ * right-hand sides [<W, E_inc>; eta0 <W, H_inc>], and the far field
+ * radiates J and M.  GHOST_FP64 handles only; before build. */
+int ghost_bor_set_interior(ghost_bor* h, int prob, double k_re, double k_im, double eta_re, double eta_im,
+                           int n_xi_efie, int n_xi_bracket);
+
 /* Geometry tables, device buffers and the launch schedule. */
 int ghost_bor_build(ghost_bor* h);
 
