@@ -2,11 +2,14 @@
 
 ``solve_monostatic_rcs_bor`` keeps the reference's signature, validation
 contract (ValueError for bad input or geometry, InterruptedError on abort)
-and result format (sample rows and metadata), for conductor bodies: every
-segment TYPE 2, PEC or with impedance flags (IBC-EFIE, tapered and tabulated
-Z_s through ``materials.MaterialLibrary``).  All frequencies of a request go
-to the GPU as one batch (``bor_solver.solve_bor_batch``).  Penetrable
-layouts (TYPE 3/4/5) are outside this build and raise NotImplementedError.
+and result format (sample rows and metadata), for conductor bodies (every
+segment TYPE 2, PEC or with impedance flags: IBC-EFIE, tapered and tabulated
+Z_s through ``materials.MaterialLibrary``) and homogeneous penetrable bodies
+(every segment TYPE 3 with one pos_mat dielectric flag: PMCHWT, meshed at the
+interior wavelength).  All frequencies of a request go to the GPU as one
+batch (``bor_solver.solve_bor_batch``).  The multi-region layouts (coated
+TYPE 3 + 4, partially coated TYPE 2 + 3 + 4, layered TYPE 3 + 4 + 5) need
+cross-surface operators and raise NotImplementedError.
 
 Geometry path: the snapshot's segments become polylines (``Piece``) in
 metres; the conductor's pieces must join end-to-start as drawn into one run
@@ -240,16 +243,22 @@ def mesh_run(run: Sequence[Piece], wavelength: float, max_elements: int, axis_to
 
 
 def layout(pieces: Sequence[Piece]) -> str:
-    """'conductor' (all TYPE 2) or an error: penetrable layouts are not part
-    of this build, other TYPEs are not BoR surfaces."""
+    """'conductor' (all TYPE 2), 'dielectric' (all TYPE 3 on one pos_mat
+    flag) or an error (bor_dispatch.py:254-298): other TYPEs are not BoR
+    surfaces, and the multi-region layouts are not part of this build."""
     kinds = {pc.kind for pc in pieces}
     if kinds == {2}:
         return "conductor"
+    if kinds == {3}:
+        if len({pc.pos for pc in pieces}) != 1:
+            raise ValueError("a homogeneous penetrable body needs the same pos_mat flag on every TYPE 3 segment.")
+        return "dielectric"
     odd = sorted(kinds - {2, 3, 4, 5})
     if odd:
         raise ValueError(f"segment TYPE {odd} cannot be solved as a body of revolution (TYPE 2 conductor "
                          "or impedance, 3 dielectric interface, 4 coated conductor, 5 layer interface).")
-    raise NotImplementedError("penetrable BoR layouts (TYPE 3 / 4 / 5: PMCHWT) are not part of the GPU BoR path.")
+    raise NotImplementedError("multi-region BoR layouts (coated, partially coated, layered: TYPE 3 + 4 / "
+                              "2 + 3 + 4 / 3 + 4 + 5) are not part of the GPU BoR path.")
 
 
 def _sample_rows(fg: float, aspects: Sequence[float], out: Dict[str, Any], pol: str) -> List[Dict[str, Any]]:
@@ -314,7 +323,7 @@ def solve_monostatic_rcs_bor(
     kind = layout(pieces)
     span = float(np.ptp(np.vstack([pc.pts for pc in pieces]), axis=0).max())
     diag = max(span, 1e-9)
-    run = single_run(pieces, "TYPE 2", max(1e-12, 1e-9 * diag))
+    run = single_run(pieces, "TYPE 2" if kind == "conductor" else "TYPE 3", max(1e-12, 1e-9 * diag))
     axis_tol = 1e-6 * diag
     check_generatrix(run, "body", axis_tol)
 
@@ -324,8 +333,19 @@ def solve_monostatic_rcs_bor(
 
     check_abort()
     jobs, forms = [], []
+    # a penetrable body runs in FP64 (complex interior wavenumber)
+    prec = "fp64" if kind == "dielectric" else precision
     for fg in freqs:
         wavelength = C0 / ((float(mesh_reference_ghz) if mesh_reference_ghz else fg) * 1e9)
+        if kind == "dielectric":
+            # meshed at the interior wavelength (bor_dispatch.py:543-547)
+            eps, mu = materials.get_medium(run[0].pos, fg)
+            m_idx, _ = bor_solver.causal_medium(eps, mu)
+            nodes, _, _ = mesh_run(run, wavelength / max(1.0, abs(m_idx)), max_elements, axis_tol)
+            forms.append(("pmchwt", False))
+            jobs.append({"points": nodes, "freq_hz": fg * 1e9, "thetas_deg": aspects, "n_modes": n_modes,
+                         "medium": (eps, mu)})
+            continue
         nodes, owner, arc = mesh_run(run, wavelength, max_elements, axis_tol)
         zs = np.array([materials.get_impedance(run[o].ibc, fg, arc_s=float(s)) if run[o].ibc > 0 else 0.0
                        for o, s in zip(owner, arc)], dtype=np.complex128)
@@ -335,7 +355,7 @@ def solve_monostatic_rcs_bor(
         jobs.append({"points": nodes, "freq_hz": fg * 1e9, "thetas_deg": aspects, "formulation": form,
                      "cfie_alpha": cfie_alpha, "n_modes": n_modes, "assembly": assembly,
                      "zs": zs if has_zs else None})
-    outs = bor_solver.solve_bor_batch(jobs, precision=precision, refine_iters=refine_iters,
+    outs = bor_solver.solve_bor_batch(jobs, precision=prec, refine_iters=refine_iters,
                                       mode_tol=mode_tol, check_abort=check_abort)
     samples: List[Dict[str, Any]] = []
     per_freq: List[Dict[str, Any]] = []
@@ -356,8 +376,11 @@ def solve_monostatic_rcs_bor(
                   for s in samples if 0.0 < s["theta_inc_deg"] < 180.0]
         samples = sorted(samples + mirror, key=lambda s: (s["frequency_ghz"], s["theta_inc_deg"]))
     any_zs = any(h for _, h in forms)
-    label = ("BoR-MoM IBC-EFIE (Leontovich)" if any_zs
-             else f"BoR-MoM PEC {forms[0][0].upper()}")
+    if kind == "dielectric":
+        label = "BoR-MoM PMCHWT (homogeneous dielectric)"
+    else:
+        label = ("BoR-MoM IBC-EFIE (Leontovich)" if any_zs
+                 else f"BoR-MoM PEC {forms[0][0].upper()}")
     return {
         "solver": "bor_mom_rcs",
         "scattering_mode": "monostatic",
@@ -374,7 +397,7 @@ def solve_monostatic_rcs_bor(
             "cfie_alpha": float(cfie_alpha),
             "workers": int(workers),
             "warnings": list(materials.warnings),
-            "precision": precision,
+            "precision": prec,
         },
     }
 

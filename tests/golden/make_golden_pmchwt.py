@@ -11,7 +11,12 @@ the interior medium's EFIE operator T_i and rotated-PV operator P_i
 (unmasked, bor_solver.py:293-345, 554-575 with complex k), the masked PMCHWT
 system, its right-hand sides, currents and far field
 (bor_solver.py:1019-1094), and the end-to-end result of solve_bor_dielectric.
+dispatch_dielectric.npz holds solve_monostatic_rcs_bor on a TYPE 3 sphere
+(one inline dielectric flag, elements per interior wavelength) at
+two frequencies.  A second argument "dispatch" writes only that file.
 """
+
+import json
 
 import os
 import sys
@@ -62,10 +67,42 @@ def case(bs, bk, nodes, f_hz, nm, eps, mu):
     return data
 
 
-def main(ref_dir: str):
+def dielectric_snapshot():
+    """The 0.4 m sphere of make_golden.dispatch_snapshot as TYPE 3 segments
+    on dielectric flag 1, 10 elements per (interior) wavelength."""
+    from make_golden import dispatch_snapshot
+    snap = dispatch_snapshot()
+    for seg in snap["segments"]:
+        seg["seg_type"] = 3
+        seg["properties"] = [3, -10, 0, 1, 0]
+    snap["dielectrics"] = [["1", "2.5", "-0.1", "1.0", "0.0"]]
+    return snap
+
+
+def dispatch_case(bd):
+    snap = dielectric_snapshot()
+    data = {"snapshot": json.dumps(snap)}
+    for pol in ("VV", "HH"):
+        out = bd.solve_monostatic_rcs_bor(snap, [0.3, 0.45], [0.0, 45.0, 90.0, 135.0, 180.0], pol,
+                                          geometry_units="meters", n_modes=4, workers=1)
+        data[f"{pol}_amp"] = np.array([s["rcs_amp_real"] + 1j * s["rcs_amp_imag"] for s in out["samples"]])
+        data[f"{pol}_rcs_db"] = np.array([s["rcs_db"] for s in out["samples"]])
+        data["modes_used"] = np.array([m["modes_used"] for m in out["metadata"]["per_frequency"]])
+        data["n_unknowns"] = np.array([m["n_unknowns"] for m in out["metadata"]["per_frequency"]])
+        data["formulation"] = out["metadata"]["formulation"]
+    np.savez_compressed(os.path.join(HERE, "dispatch_dielectric.npz"), **data)
+
+
+def main(ref_dir: str, parts: str = "all"):
     sys.path.insert(0, ref_dir)
+    import bor_dispatch as bd
     import bor_kernels as bk
     import bor_solver as bs
+    if parts == "dispatch":
+        dispatch_case(bd)
+        print("dispatch dielectric fixture written to", HERE)
+        return
+    dispatch_case(bd)
     np.savez_compressed(os.path.join(HERE, "sphere12_pmchwt.npz"),
                         **case(bs, bk, bs.sphere_generatrix(0.5, 12), 300e6, 4, 2.5 - 0.1j, 1.0 + 0j))
     np.savez_compressed(os.path.join(HERE, "conesphere16_pmchwt.npz"),
@@ -74,4 +111,4 @@ def main(ref_dir: str):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference", sys.argv[2] if len(sys.argv) > 2 else "all")

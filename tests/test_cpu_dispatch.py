@@ -45,8 +45,16 @@ def test_validation_errors():
     with pytest.raises(ValueError, match="traversed"):
         bd.check_generatrix(run, "body", 1e-6)
     diel = {"segments": [dict(s, properties=[3, 2, 0, 1, 0]) for s in snap["segments"]]}
+    assert bd.layout(bd.read_pieces(diel, 1.0)) == "dielectric"
+    two = {"segments": [dict(s, properties=[3, 2, 0, 1 + i, 0]) for i, s in enumerate(snap["segments"])]}
+    with pytest.raises(ValueError, match="pos_mat"):
+        bd.layout(bd.read_pieces(two, 1.0))
+    coated = {"segments": [dict(s, properties=[3 + i, 2, 0, 1, 0]) for i, s in enumerate(snap["segments"])]}
     with pytest.raises(NotImplementedError):
-        bd.layout(bd.read_pieces(diel, 1.0))
+        bd.layout(bd.read_pieces(coated, 1.0))
+    with pytest.raises(ValueError, match="TYPE"):
+        bd.layout(bd.read_pieces({"segments": [dict(s, properties=[7, 2, 0, 0, 0]) for s in snap["segments"]]},
+                                 1.0))
     abort = __import__("threading").Event()
     abort.set()
     with pytest.raises(InterruptedError):
@@ -59,3 +67,38 @@ def test_element_count_rule():
     assert bd.element_count(-10, 1.0, 0.1) == 100
     assert bd.element_count(0, 1.0, 0.1) == 200
     assert bd.element_count(0, 1.0, 1e-9) == 2000
+
+
+def test_dielectric_body_meshed_at_the_interior_wavelength():
+    """TYPE 3 sphere on one dielectric flag: the element counts the
+    reference meshed at lambda0 / |m| (tests/golden/dispatch_dielectric.npz
+    holds its 4 Nn unknowns per frequency)."""
+    import json
+    bd = pkg("bor_dispatch")
+    bs = pkg("bor_solver")
+    mt = pkg("materials")
+    d = np.load(os.path.join(ROOT, "tests", "golden", "dispatch_dielectric.npz"))
+    snap = json.loads(str(d["snapshot"]))
+    pieces = bd.read_pieces(snap, 1.0)
+    assert bd.layout(pieces) == "dielectric"
+    diag = max(float(np.ptp(np.vstack([c.pts for c in pieces]), axis=0).max()), 1e-9)
+    run = bd.single_run(pieces, "TYPE 3", max(1e-12, 1e-9 * diag))
+    lib = mt.MaterialLibrary.from_entries([], snap["dielectrics"], ".")
+    for fg, nu in zip((0.3, 0.45), d["n_unknowns"]):
+        eps, mu = lib.get_medium(run[0].pos, fg)
+        m_idx, _ = bs.causal_medium(eps, mu)
+        pts, _, _ = bd.mesh_run(run, bd.C0 / (fg * 1e9) / max(1.0, abs(m_idx)), 50000, 1e-6 * diag)
+        assert 4 * len(pts) == int(nu)
+
+
+def test_penetrable_job_validation():
+    bs = pkg("bor_solver")
+    nodes = bs.sphere_generatrix(0.3, 8)
+    job = {"points": nodes, "freq_hz": 3e8, "thetas_deg": [0.0], "medium": (2.0, 1.0)}
+    with pytest.raises(ValueError, match="FP64"):
+        bs.solve_bor_batch([job], precision="mixed")
+    with pytest.raises(ValueError, match="impedance"):
+        bs.solve_bor_batch([dict(job, zs=50.0)])
+    m, er = bs.causal_medium(3.0 - 0.4j, 1.3 - 0.05j)
+    assert m.imag <= 0.0 and m.real > 0.0 and abs(m * m - (3.0 - 0.4j) * (1.3 - 0.05j)) < 1e-14
+    assert abs(er - (1.3 - 0.05j) / m) < 1e-15
