@@ -352,6 +352,188 @@ def run_sweep(steps, warmup):
     print(json.dumps(out), flush=True)
 
 
+def dielectric_cpu_baseline(nodes, freq, thetas, eps, mu, m_max, stride=10, near_sample=9):
+    """The FP64 oracle's PMCHWT sweep (bor_solver.py:1019-1094 restated), one
+    thread, timed per leg on a bounded sample and scaled to the full sweep:
+    both media's far tables in full (shared by every mode and aspect), the
+    near-band projections of ``near_sample`` element pairs per medium
+    (scaled by the pair count), and per mode the block assembly (far part)
+    and LU + solve with one aspect in ``stride`` plus the excitation and far
+    field of those aspects (scaled by the stride)."""
+    from threadpoolctl import threadpool_limits
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import bor_oracle as bo
+    log("dielectric cpu baseline: oracle sample")
+    th = np.asarray(thetas)[::stride]
+    k0 = 2 * math.pi * freq / bo.C0
+    body = bo.Body(nodes)
+    Nn, Ne = body.Nn, body.Ne
+    mi, er = bo.causal_medium(eps, mu)
+    media = [(k0, bo.ETA0), (k0 * mi, bo.ETA0 * er)]
+    pairs = bo.near_pairs(Ne)
+    # stratified by cell kind (a self pair has 46 cells, a neighbour pair 13)
+    by_kind = {}
+    for pr in pairs:
+        by_kind.setdefault(bo.near_kind(*pr), []).append(pr)
+    per = max(1, near_sample // len(by_kind))
+    pick = {kd: [v[i] for i in np.linspace(len(v) // 4, 3 * len(v) // 4, per).astype(int)]
+            for kd, v in by_kind.items()}
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        fars = []
+        for k, _ in media:
+            osc = bo.osc_for(body, k, streaming=False)
+            fars.append(bo.FarBlocks(body, k, m_max, bo.n_xi_efie(osc, m_max), bo.n_xi_bracket(osc, m_max),
+                                     efie=True, mfie=False, zs_pt=np.ones(body.P, dtype=np.complex128)))
+        t_far = time.perf_counter() - t0
+        t_near = 0.0
+        for kd, sel in pick.items():
+            t0 = time.perf_counter()
+            for k, _ in media:
+                for (e, f) in sel:
+                    s_, sp, _w = bo.cell_points(kd)
+                    p, q = body.element_points(e, s_), body.element_points(f, sp)
+                    bo.green_orders_near(p["rho"], p["z"], q["rho"], q["z"], k, m_max)
+                    bo.bracket_modes_near(p["rho"], p["z"], p["trho"], p["tz"], q["rho"], q["z"], q["trho"],
+                                          q["tz"], k, m_max, ibc=True)
+            t_near += (time.perf_counter() - t0) * len(by_kind[kd]) / len(sel)
+        r2 = (bo.ETA0 / media[1][1]) ** 2
+        t_sys = t_exc = 0.0
+        for m in range(-m_max, m_max + 1):
+            t0 = time.perf_counter()
+            T, P = [], []
+            for (k, eta), fb in zip(media, fars):
+                b = fb.efie(m)
+                T.append(1j * k * eta * 2.0 * np.pi * np.block([[b[0], b[1]], [b[2], b[3]]]))
+                c = fb.ibc(m)
+                P.append(np.block([[-c[1], c[0]], [-c[3], c[2]]]))
+            Ps = bo.ETA0 * (P[0] + P[1])
+            mask = np.tile(bo.basis_mask(body, m), 2)
+            A = np.block([[T[0] + T[1], -Ps], [Ps, T[0] + r2 * T[1]]])[np.ix_(mask, mask)]
+            t1 = time.perf_counter()
+            cols = []
+            for a in th:
+                for pol in ("VV", "HH"):
+                    hp = bo.rhs_efie(body, k0, m, a, "HH" if pol == "VV" else "VV")
+                    cols.append(np.concatenate([bo.rhs_efie(body, k0, m, a, pol), -hp if pol == "VV" else hp])[mask])
+            t2 = time.perf_counter()
+            X = np.linalg.solve(A, np.stack(cols, axis=1))
+            t3 = time.perf_counter()
+            for it, a in enumerate(th):
+                for ip in range(2):
+                    full = np.zeros(4 * Nn, dtype=complex)
+                    full[mask] = X[:, 2 * it + ip]
+                    bo.farfield(body, k0, m, full[:2 * Nn], a, msol=bo.ETA0 * full[2 * Nn:])
+            t4 = time.perf_counter()
+            t_sys += (t1 - t0) + (t3 - t2)
+            t_exc += (t2 - t1) + (t4 - t3)
+    scale = len(thetas) / len(th)
+    total = t_far + t_near + t_sys + t_exc * scale
+    solves = (2 * m_max + 1) * len(thetas)
+    return {"value": solves / total, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": (f"FP64 oracle PMCHWT, 1 thread: far tables of both media in full ({t_far:.1f} s); near "
+                       f"projections of {sum(len(v) for v in pick.values())} of {len(pairs)} element pairs per medium "
+                       f"(stratified by cell kind), scaled "
+                       f"({t_near:.1f} s); {2 * m_max + 1} mode assemblies + LU/solves with {len(th)} of "
+                       f"{len(thetas)} aspects ({t_sys:.1f} s, not rescaled); excitation + far field of "
+                       f"{len(th)} aspects x {scale:.1f} ({t_exc * scale:.1f} s); est. total {total:.0f} s")}
+
+
+def run_dielectric(steps, warmup, no_cpu):
+    """Homogeneous penetrable body on PMCHWT: a lossy dielectric sphere
+    (a = 0.5 m, eps_r = 2.5 - 0.1j, 300 MHz, 120 segments, 181 aspects, the
+    reference's default mode count).  One step = both media's fills, the
+    block assembly, batched LU/solve of the (J, M) systems and the far field,
+    in FP64.  Prints its own JSON line."""
+    wl = pkg("workloads")
+    bs = pkg("bor_solver")
+    L = pkg("_lib")
+    at = pkg("autotune")
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mie
+    a, freq, eps, mu = 0.5, 3e8, 2.5 - 0.1j, 1.0
+    nodes = wl.sphere(a, 120)
+    thetas = np.arange(0.0, 181.0, 1.0)
+    k0 = 2 * math.pi * freq / bs.C0
+    m_max, ne, nbk, _ = bs.discretisation(nodes, k0, thetas, None, "efie", "tables")
+    mi, er = bs.causal_medium(eps, mu)
+    osc_i = 2.0 * abs(k0 * mi) * bs.gauss_rho_max(nodes)
+    nxi_in = (bs.n_xi_efie(osc_i, m_max), bs.n_xi_bracket(osc_i, m_max))
+    tiles = at.best(L.GHOST_FP64, 4 * len(nodes), m_max + 1, 2 * len(thetas))
+    log(f"dielectric: m_max={m_max}, n_xi {ne}/{nbk} exterior, {nxi_in} interior, tiles {tiles}")
+    h = L.Handle(L.GHOST_FP64, nb=tiles[0], residual=False)
+    h.set_gemm_variant(tiles[1])
+    h.add_problem(nodes, k0, m_max, ne, nbk, 1.0, 0.0, thetas)
+    h.set_interior(0, k0 * mi, bs.ETA0 * er, *nxi_in)
+    h.build()
+    n_sys = h.system_info(0, 0)[0]
+    for i in range(warmup):
+        h.run(0)
+        h.sync()
+        log(f"dielectric: warmup {i + 1}/{warmup} done")
+    L.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        h.run(0)
+    h.sync()
+    L.device_synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    log(f"dielectric: {steps} timed steps, {dt * 1e3:.1f} ms/step")
+    phase = h.timings()
+    h.set_profile(True)
+    h.run(0)
+    h.sync()
+    kt = h.kernel_times()
+    dF = h.dF(0)
+    h.close()
+    F = bs.truncate_modes(dF, m_max, 0.0)[0]
+    ref = 10.0 * math.log10(mie.sigma_dielectric_sphere(a, eps, mu, freq))
+    err = float(np.max(np.abs(db(F) - ref)))
+    solves = (2 * m_max + 1) * len(thetas)
+    # the assembly kernel against HBM: per assembled mode it reads the four
+    # node-based operators' far blocks (2 Nn)^2 and writes the n x n system,
+    # complex FP64 (the near band's few entries aside)
+    Nn = len(nodes)
+    n_asm = m_max + 1
+    asm_bytes = n_asm * (4 * (2 * Nn) ** 2 + n_sys ** 2) * 16.0
+    rows = {}
+    for fam, peak_key, bound in (("far_fill", "fp64_mfma_tflops", "mfma"), ("near_modal", "fp64_vector_tflops", "valu"),
+                                 ("assemble", "hbm_gbs", "hbm")):
+        ms_f, n_f, fl_f = kt.get(fam, (0.0, 0, 0.0))
+        if ms_f <= 0:
+            continue
+        if bound == "hbm":
+            ach = asm_bytes / (ms_f * 1e-3) / 1e9
+            rows[fam] = {"bound": "hbm", "achieved": round(ach, 1), "peak": MI355X_PEAK[peak_key], "unit": "GB/s",
+                         "frac": round(ach / MI355X_PEAK[peak_key], 4), "ms_per_step": round(ms_f, 3),
+                         "bytes_per_step": asm_bytes}
+        elif fl_f > 0:
+            ach = fl_f / (ms_f * 1e-3) / 1e12
+            rows[fam] = {"bound": bound, "achieved": round(ach, 3), "peak": MI355X_PEAK[peak_key], "unit": "TFLOP/s",
+                         "frac": round(ach / MI355X_PEAK[peak_key], 4), "ms_per_step": round(ms_f, 3)}
+    gl = [v for kk, v in kt.items() if kk.startswith("gemm")]
+    gemm_ms, gemm_fl = sum(v[0] for v in gl), sum(v[2] for v in gl)
+    if gemm_ms > 0:
+        ach = gemm_fl / (gemm_ms * 1e-3) / 1e12
+        rows["gemm (all uses)"] = {"bound": "mfma", "achieved": round(ach, 3), "peak": MI355X_PEAK["fp64_mfma_tflops"],
+                                   "unit": "TFLOP/s", "frac": round(ach / MI355X_PEAK["fp64_mfma_tflops"], 4),
+                                   "ms_per_step": round(gemm_ms, 3)}
+    out = {
+        "metric": "BOR PMCHWT (homogeneous dielectric) monostatic solves/s (mode x angle)",
+        "value": round(solves / dt, 2), "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "dtype": "fp64", "data": "synthetic",
+        "config": {"workload": "dielectric-sphere-a0.5m-eps2.5-0.1j-300MHz-120seg", "m_max": m_max,
+                   "modes": 2 * m_max + 1, "aspects": len(thetas), "polarizations": 2, "system_n": n_sys,
+                   "n_xi_exterior": [ne, nbk], "n_xi_interior": list(nxi_in), "lu_tiles": list(tiles)},
+        "err_vs_mie_max_abs_db": err,
+        "kernel_ms_per_step": {k: round(v[0], 3) for k, v in kt.items() if v[1]},
+        "phase_ms_last_step": phase,
+        "rooflines": rows,
+        "cpu_baseline": None if no_cpu else dielectric_cpu_baseline(nodes, freq, thetas, eps, mu, m_max),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_c0():
     return pkg("bor_solver").C0
 
@@ -370,6 +552,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="run one plain device copy of GIB GiB (byte-counter calibration) and exit")
+    ap.add_argument("--dielectric", action="store_true",
+                    help="time the PMCHWT sweep of a lossy dielectric sphere (120 segments) instead")
     ap.add_argument("--adjoint", action="store_true",
                     help="time forward + adjoint RCS gradient on the 600-element cone-sphere instead")
     args = ap.parse_args()
@@ -384,6 +568,9 @@ def main():
         return
     if args.adjoint:
         run_adjoint(max(1, args.steps))
+        return
+    if args.dielectric:
+        run_dielectric(max(1, args.steps), args.warmup, args.no_cpu)
         return
     if args.config == 3:
         run_sweep(max(1, args.steps), args.warmup)
