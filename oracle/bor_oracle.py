@@ -25,6 +25,10 @@ It restates, in NumPy/SciPy, the discretization of the reference solver:
 * f1  IBC-EFIE (Z_s brackets, Z_s mass, magnetic-current far field)
                                        bor_kernels.py:440-560, bor_solver.py:463-552,
                                        697-707, 850-869
+* f3  PMCHWT, homogeneous body ....... bor_solver.py:161-230, 554-575, 647-657,
+                                       1019-1094
+* f3  PMCHWT, multi-region (cross-surface operators, region system, junction
+      reduction) ..................... bor_solver.py:1100-1297, 1805-2131
 
 Pinning (what checks each function, besides golden vectors written from the
 reference itself under tests/golden/):
@@ -38,6 +42,11 @@ reference itself under tests/golden/):
 * a12 residual ~1e-15; agreement with LAPACK.
 * f1  golden vectors (scalar and tapered Z_s); the impedance Mie series and
       Weston's null (Z_s = eta0) end to end (oracle/mie.py).
+* f3  golden vectors (every mode's operators, systems, currents and far
+      field for a lossy dielectric and a lossy magnetic body, a coated PEC
+      sphere with tensor-grid near pairs, two layers); the dielectric and
+      layered Mie series end to end (oracle/mie.py, whose layered series is
+      itself checked by the free-space-layer and split-layer identities).
 """
 
 from __future__ import annotations
@@ -1071,3 +1080,377 @@ def solve_bor_dielectric(nodes, freq_hz: float, thetas_deg, eps_r, mu_r=1.0,
     if keep:
         out["_modes"] = sols
     return out
+
+
+# ─────────────────────────────────────────────────────────────────────────────
+# f3  multi-region PMCHWT: cross-surface operators and the region system
+#     (bor_solver.py:1100-1297, 1805-2172)
+# ─────────────────────────────────────────────────────────────────────────────
+
+def segment_distance(p0, p1, q0, q1) -> float:
+    """Least distance between two segments that do not cross: one of the
+    four endpoint-to-segment distances (bor_solver.py:1100-1112)."""
+    def to_seg(c, a, b):
+        ab = b - a
+        t = float(np.clip(np.dot(c - a, ab) / max(float(np.dot(ab, ab)), 1e-300), 0.0, 1.0))
+        return float(np.hypot(*(c - (a + t * ab))))
+    return min(to_seg(p0, q0, q1), to_seg(p1, q0, q1), to_seg(q0, p0, p1), to_seg(q1, p0, p1))
+
+
+def cross_pair_kinds(bp: Body, bq: Body, near_factor: float = 2.0) -> Dict[Tuple[int, int], Optional[str]]:
+    """Near element pairs between two generatrices (bor_solver.py:1131-1164):
+    a pair sharing one endpoint (a junction) is 'corner<a><b>' (a, b the
+    shared ends), a pair closer than near_factor x the longer element is
+    None (re-integrated on a tensor grid); touching elsewhere is an error."""
+    diag = max(float(np.max(bp.nodes)) - float(np.min(bp.nodes)),
+               float(np.max(bq.nodes)) - float(np.min(bq.nodes)), 1e-9)
+    tol = 1e-9 * diag
+    kinds: Dict[Tuple[int, int], Optional[str]] = {}
+    for e in range(bp.Ne):
+        pe = (bp.nodes[e], bp.nodes[e + 1])
+        for f in range(bq.Ne):
+            qf = (bq.nodes[f], bq.nodes[f + 1])
+            shared = [(a, b) for a in (0, 1) for b in (0, 1) if float(np.hypot(*(pe[a] - qf[b]))) <= tol]
+            if len(shared) > 1:
+                raise ValueError("two surfaces share a whole element")
+            d = segment_distance(pe[0], pe[1], qf[0], qf[1])
+            if shared:
+                kinds[(e, f)] = "corner%d%d" % shared[0]
+            elif d <= tol:
+                raise ValueError("two surfaces cross or touch away from a junction endpoint")
+            elif d < near_factor * max(bp.L[e], bq.L[f]):
+                kinds[(e, f)] = None
+    return kinds
+
+
+def tensor_points(order: int = 12):
+    """order x order Gauss tensor grid on the unit square, test axis slowest
+    (bor_solver.py:1203-1209)."""
+    x, w = np.polynomial.legendre.leggauss(order)
+    u, w1 = 0.5 * (x + 1.0), 0.5 * w
+    S, SP = np.meshgrid(u, u, indexing="ij")
+    return S.ravel(), SP.ravel(), np.outer(w1, w1).ravel()
+
+
+class CrossOperators:
+    """EFIE (T) and rotated-PV (P) Galerkin operators between the test bases
+    of generatrix bp and the source bases of bq in one homogeneous medium
+    (k, eta), node-based [2 Nn_p, 2 Nn_q] (bor_solver.py:1115-1297): far
+    point pairs on the uniform-xi tables (n_xi from 2|k| max sqrt(rho rho')
+    over the two point sets), near pairs re-integrated with the near
+    kernels on corner cells or a tensor grid."""
+
+    def __init__(self, bp: Body, bq: Body, k, eta, m_max: int, near_factor: float = 2.0,
+                 near_order: int = 12):
+        self.bp, self.bq, self.k, self.eta, self.m_max = bp, bq, k, eta, m_max
+        self.kinds = cross_pair_kinds(bp, bq, near_factor)
+        gp, gq = bp.g, bq.g
+        osc = float(np.max(2.0 * abs(k) * np.sqrt(np.outer(gp.rho, gq.rho))))
+        self.n_e, self.n_b = n_xi_efie(osc, m_max), n_xi_bracket(osc, m_max)
+        G = green_orders_fft(green_samples(gp.rho[:, None], gp.z[:, None], gq.rho[None, :], gq.z[None, :],
+                                           k, xi_grid(self.n_e)), m_max + 2)
+        Fs = ibc_bracket_samples(gp.rho[:, None], gp.z[:, None], gp.trho[:, None], gp.tz[:, None],
+                                 gq.rho[None, :], gq.z[None, :], gq.trho[None, :], gq.tz[None, :],
+                                 k, xi_grid(self.n_b))
+        B = [bracket_modes_fft(F, m_max) * (2.0 * np.pi) for F in Fs]
+        near = np.zeros((bp.P, bq.P), dtype=bool)
+        for (e, f) in self.kinds:
+            near[np.ix_(gp.elem == e, gq.elem == f)] = True
+        G[near] = 0.0
+        for Bu in B:
+            Bu[near] = 0.0
+        self.G, self.B = G, B
+        self.near = {}
+        for (e, f), kind in self.kinds.items():
+            s, sp, W = cell_points(kind) if kind is not None else tensor_points(near_order)
+            p, q = bp.element_points(e, s), bq.element_points(f, sp)
+            Gn = green_orders_near(p["rho"], p["z"], q["rho"], q["z"], k, m_max)
+            Bn = bracket_modes_near(p["rho"], p["z"], p["trho"], p["tz"], q["rho"], q["z"], q["trho"],
+                                    q["tz"], k, m_max, ibc=True)
+            self.near[(e, f)] = (W * p["L"] * q["L"], p, q, Gn, Bn)
+
+    def T(self, m: int) -> np.ndarray:
+        bp, bq, k = self.bp, self.bq, self.k
+        gp, gq = bp.g, bq.g
+        BTp, BDp = bp.nodal_matrices()
+        BTq, BDq = bq.nodal_matrices()
+        am = abs(m)
+        sg = -1.0 if m < 0 else 1.0
+
+        def orders(Gt):
+            G = Gt[..., am]
+            Gc = 0.5 * (Gt[..., abs(am - 1)] + Gt[..., am + 1])
+            Gs = sg * (Gt[..., abs(am - 1)] - Gt[..., am + 1]) / 2j
+            return G, Gc, Gs
+
+        G, Gc, Gs = orders(self.G)
+        rrw = (gp.rho * gp.w)[:, None] * (gq.rho * gq.w)[None, :]
+        Ksc = np.outer(gp.w, gq.w) * G
+        ztt = BTp @ (rrw * (np.outer(gp.trho, gq.trho) * Gc + np.outer(gp.tz, gq.tz) * G)) @ BTq.T \
+            - (BDp @ Ksc @ BDq.T) / k ** 2
+        ztf = BTp @ (rrw * gp.trho[:, None] * Gs) @ BTq.T - (1j * m / k ** 2) * (BDp @ Ksc @ BTq.T)
+        zft = BTp @ (-rrw * gq.trho[None, :] * Gs) @ BTq.T + (1j * m / k ** 2) * (BTp @ Ksc @ BDq.T)
+        zff = BTp @ (rrw * Gc) @ BTq.T - (m ** 2 / k ** 2) * (BTp @ Ksc @ BTq.T)
+        bil = lambda A, v, Bm: np.einsum("ip,p,jp->ij", A, v, Bm)
+        for (e, f), (w, p, q, Gt, _) in self.near.items():
+            G, Gc, Gs = orders(Gt)
+            rr = p["rho"] * q["rho"] * w
+            Tp, Tq, Dp, Dq = p["T"], q["T"], p["D"], q["D"]
+            ksc = w * G
+            r, c = slice(e, e + 2), slice(f, f + 2)
+            ztt[r, c] += bil(Tp, rr * (p["trho"] * q["trho"] * Gc + p["tz"] * q["tz"] * G), Tq) \
+                - bil(Dp, ksc, Dq) / k ** 2
+            ztf[r, c] += bil(Tp, rr * p["trho"] * Gs, Tq) - (1j * m / k ** 2) * bil(Dp, ksc, Tq)
+            zft[r, c] += bil(Tp, -rr * q["trho"] * Gs, Tq) + (1j * m / k ** 2) * bil(Tp, ksc, Dq)
+            zff[r, c] += bil(Tp, rr * Gc, Tq) - (m ** 2 / k ** 2) * bil(Tp, ksc, Tq)
+        return 1j * k * self.eta * 2.0 * np.pi * np.block([[ztt, ztf], [zft, zff]])
+
+    def P(self, m: int) -> np.ndarray:
+        bp, bq = self.bp, self.bq
+        gp, gq = bp.g, bq.g
+        BTp, _ = bp.nodal_matrices()
+        BTq, _ = bq.nodal_matrices()
+        mi = m + self.m_max
+        Lw, Rw = BTp * (gp.w * gp.rho)[None, :], BTq * (gq.w * gq.rho)[None, :]
+        blk = [Lw @ Bu[..., mi] @ Rw.T for Bu in self.B]
+        for (e, f), (w, p, q, _, Bn) in self.near.items():
+            rr = p["rho"] * q["rho"] * w
+            for uv in range(4):
+                blk[uv][e:e + 2, f:f + 2] += 2.0 * np.pi * np.einsum("ip,p,jp->ij", p["T"], rr * Bn[uv][:, mi],
+                                                                     q["T"])
+        Btt, Btf, Bft, Bff = blk
+        return np.block([[-Btf, Btt], [-Bff, Bft]])
+
+
+class MultiRegion:
+    """Surfaces (generatrix, is_conductor) bounding homogeneous regions
+    ({"medium": None | (eps, mu), "bounds": [(surface, sigma)], "exterior":
+    bool}); unknowns per surface J and, for an interface, M / eta0; the
+    system sum over regions of sigma_i sigma_j x [[T, -eta0 P], [eta0 P,
+    (eta0/eta_r)^2 T]] blocks, reduced by the junction map Q
+    (bor_solver.py:1805-2041)."""
+
+    def __init__(self, surfaces, regions, freq_hz: float, m_max: int, near_factor: float = 2.0,
+                 near_order: int = 12):
+        self.regions = regions
+        self.bodies = [Body(p) for (p, _) in surfaces]
+        self.cond = [bool(c) for (_, c) in surfaces]
+        ns = len(surfaces)
+        self.adj = [[] for _ in range(ns)]
+        self.sigma = {}
+        for ri, reg in enumerate(regions):
+            for (si, sg) in reg["bounds"]:
+                self.adj[si].append(ri)
+                self.sigma[(ri, si)] = int(sg)
+        self.ext = next(ri for ri, r in enumerate(regions) if r.get("exterior"))
+        if any(sg != 1 for (_, sg) in regions[self.ext]["bounds"]):
+            raise ValueError("surfaces bounding the exterior need sigma = +1 there")
+        self.k0 = 2.0 * math.pi * freq_hz / C0
+        self.m_max = m_max
+        self.media = []
+        for reg in regions:
+            if reg["medium"] is None:
+                self.media.append((self.k0, ETA0))
+            else:
+                mi, er = causal_medium(*reg["medium"])
+                self.media.append((self.k0 * mi, ETA0 * er))
+        self.ops = {(si, ri): MediumOperators(surfaces[si][0], freq_hz, m_max, regions[ri]["medium"])
+                    for si in range(ns) for ri in self.adj[si]}
+        self.X = {}
+        for ri, reg in enumerate(regions):
+            ids = [si for (si, _) in reg["bounds"]]
+            for si in ids:
+                for sj in ids:
+                    if si != sj:
+                        self.X[(ri, si, sj)] = CrossOperators(self.bodies[si], self.bodies[sj], *self.media[ri],
+                                                              m_max, near_factor, near_order)
+        self.Nn = [b.Nn for b in self.bodies]
+        self.off_J, self.off_M, acc = [], [], 0
+        for si in range(ns):
+            self.off_J.append(acc)
+            acc += 2 * self.Nn[si]
+            self.off_M.append(None if self.cond[si] else acc)
+            acc += 0 if self.cond[si] else 2 * self.Nn[si]
+        self.n_full = acc
+        self.junctions = self._junctions()
+
+    def _junctions(self):
+        """Off-axis generatrix ends that coincide, grouped; the first
+        interface of a group leads it (bor_solver.py:1856-1895)."""
+        allp = np.vstack([b.nodes for b in self.bodies])
+        tol = 1e-8 * max(float(np.ptp(allp[:, 0])) + float(np.ptp(allp[:, 1])), 1e-9)
+        groups = []
+        for si, b in enumerate(self.bodies):
+            for node in (0, b.Nn - 1):
+                if b.on_axis(node):
+                    continue
+                for g in groups:
+                    s0, n0 = g[0]
+                    if float(np.hypot(*(self.bodies[s0].nodes[n0] - b.nodes[node]))) <= tol:
+                        g.append((si, node))
+                        break
+                else:
+                    groups.append([(si, node)])
+        for g in groups:
+            if len(g) < 2:
+                raise ValueError(f"surface {g[0][0]} has a free off-axis end that is no junction")
+            lead = next((i for i, (si, _) in enumerate(g) if not self.cond[si]), 0)
+            g[0], g[lead] = g[lead], g[0]
+            for (si, _) in g[1:]:
+                if not set(self.adj[si]) & set(self.adj[g[0][0]]):
+                    raise ValueError("a junction member shares no region with its lead")
+        return groups
+
+    def reduction(self, m: int):
+        """(red_of_full [n_full] (-1: dropped), coef [n_full], n_red): each
+        kept full index maps to one reduced index with a real coefficient;
+        junction followers tie to their lead's index (bor_solver.py:1908-1986)."""
+        members = {x for g in self.junctions for x in g}
+        follower = {x for g in self.junctions for x in g[1:]}
+        mt_off = {x for g in self.junctions if any(self.cond[s] for (s, _) in g)
+                  for x in g if not self.cond[x[0]]}
+        red = np.full(self.n_full, -1, dtype=int)
+        coef = np.zeros(self.n_full)
+        n = 0
+        for si, b in enumerate(self.bodies):
+            for off, is_m in ((self.off_J[si], False), (self.off_M[si], True)):
+                if off is None:
+                    continue
+                Nn = self.Nn[si]
+                t_on, f_on = np.ones(Nn, dtype=bool), np.ones(Nn, dtype=bool)
+                for end in (0, Nn - 1):
+                    if b.on_axis(end):
+                        t_on[end], f_on[end] = abs(m) == 1, False
+                    elif (si, end) in follower:
+                        t_on[end] = f_on[end] = False
+                    elif (si, end) in members:
+                        if is_m and (si, end) in mt_off:
+                            t_on[end] = False
+                    else:
+                        t_on[end] = False
+                for blk, on in ((0, t_on), (Nn, f_on)):
+                    for i in np.flatnonzero(on):
+                        red[off + blk + i] = n
+                        coef[off + blk + i] = 1.0
+                        n += 1
+        direction = lambda node: 1 if node != 0 else -1
+        for g in self.junctions:
+            sl, nl = g[0]
+            for (sf, nf) in g[1:]:
+                r = next(iter(set(self.adj[sf]) & set(self.adj[sl])))
+                sgl, sgf = self.sigma[(r, sl)], self.sigma[(r, sf)]
+                ct = -(sgl * direction(nl)) / (sgf * direction(nf))
+                cf = sgl / sgf
+                pairs = [(self.off_J[sl], self.off_J[sf])]
+                if self.off_M[sl] is not None and self.off_M[sf] is not None:
+                    pairs.append((self.off_M[sl], self.off_M[sf]))
+                for ol, of in pairs:
+                    for blk, c in ((0, ct), (1, cf)):
+                        lead = red[ol + blk * self.Nn[sl] + nl]
+                        if lead >= 0:
+                            red[of + blk * self.Nn[sf] + nf] = lead
+                            coef[of + blk * self.Nn[sf] + nf] = c
+        return red, coef, n
+
+    def Q(self, m: int) -> np.ndarray:
+        red, coef, n = self.reduction(m)
+        Q = np.zeros((self.n_full, n))
+        on = red >= 0
+        Q[np.flatnonzero(on), red[on]] = coef[on]
+        return Q
+
+    def full_system(self, m: int) -> np.ndarray:
+        A = np.zeros((self.n_full, self.n_full), dtype=np.complex128)
+        for ri, reg in enumerate(self.regions):
+            r2 = (ETA0 / self.media[ri][1]) ** 2
+            for (si, sgi) in reg["bounds"]:
+                for (sj, sgj) in reg["bounds"]:
+                    ss = sgi * sgj
+                    if si == sj:
+                        T = self.ops[(si, ri)].T(m)
+                        P = None if self.cond[si] else self.ops[(si, ri)].P(m)
+                    else:
+                        T, P = self.X[(ri, si, sj)].T(m), self.X[(ri, si, sj)].P(m)
+                    Ji = slice(self.off_J[si], self.off_J[si] + 2 * self.Nn[si])
+                    Jj = slice(self.off_J[sj], self.off_J[sj] + 2 * self.Nn[sj])
+                    A[Ji, Jj] += ss * T
+                    if self.off_M[sj] is not None:
+                        Mj = slice(self.off_M[sj], self.off_M[sj] + 2 * self.Nn[sj])
+                        A[Ji, Mj] += -ETA0 * ss * P
+                    if self.off_M[si] is not None:
+                        Mi = slice(self.off_M[si], self.off_M[si] + 2 * self.Nn[si])
+                        A[Mi, Jj] += ETA0 * ss * P
+                        if self.off_M[sj] is not None:
+                            A[Mi, Mj] += r2 * ss * T
+        return A
+
+    def full_rhs(self, m: int, th: float, pol: str) -> np.ndarray:
+        V = np.zeros(self.n_full, dtype=np.complex128)
+        for (si, _) in self.regions[self.ext]["bounds"]:
+            b, oj, om = self.bodies[si], self.off_J[si], self.off_M[si]
+            V[oj:oj + 2 * b.Nn] = rhs_efie(b, self.k0, m, th, pol)
+            if om is not None:
+                h = rhs_efie(b, self.k0, m, th, "HH" if pol == "VV" else "VV")
+                V[om:om + 2 * b.Nn] = -h if pol == "VV" else h
+        return V
+
+    def farfield(self, m: int, x_full: np.ndarray, th: float):
+        ft = fp = 0.0
+        for (si, _) in self.regions[self.ext]["bounds"]:
+            b, oj, om = self.bodies[si], self.off_J[si], self.off_M[si]
+            ms = None if om is None else ETA0 * x_full[om:om + 2 * b.Nn]
+            a, c = farfield(b, self.k0, m, x_full[oj:oj + 2 * b.Nn], th, msol=ms)
+            ft, fp = ft + a, fp + c
+        return ft, fp
+
+    def rho_max(self) -> float:
+        return max(float(np.max(b.nodes[:, 0])) for b in self.bodies)
+
+
+def default_modes_multiregion(rho_max: float, k0: float, thetas) -> int:
+    """bor_solver.py:2050-2053."""
+    st = float(np.max(np.abs(np.sin(np.radians(np.atleast_1d(thetas))))))
+    return int(math.ceil(k0 * rho_max * max(st, 0.05))) + 12
+
+
+def solve_multiregion(sys_: "MultiRegion", thetas_deg, mode_tol: float = 1e-6, keep: bool = False) -> Dict:
+    """Per-mode solve of Q^T A Q x = Q^T V, far field of Q x
+    (bor_solver.py:2044-2071)."""
+    thetas = np.atleast_1d(np.asarray(thetas_deg, dtype=float))
+    dFs, res, sols = [], 0.0, {}
+    for am in range(sys_.m_max + 1):
+        dF = np.zeros((2, len(thetas)), dtype=np.complex128)
+        for m in ([0] if am == 0 else [am, -am]):
+            Q = sys_.Q(m)
+            A = Q.T @ sys_.full_system(m) @ Q
+            B = np.stack([Q.T @ sys_.full_rhs(m, th, pol) for th in thetas for pol in ("VV", "HH")], axis=1)
+            X = np.linalg.solve(A, B)
+            bn = float(np.linalg.norm(B))
+            res = max(res, float(np.linalg.norm(A @ X - B)) / bn if bn > 0 else 0.0)
+            for it, th in enumerate(thetas):
+                for ip in range(2):
+                    ft, fp = sys_.farfield(m, Q @ X[:, 2 * it + ip], th)
+                    dF[ip, it] += ft if ip == 0 else fp
+            if keep:
+                sols[m] = (X, B, A)
+        dFs.append(dF)
+    F, used = truncate_modes(dFs, mode_tol)
+    out = {"theta_deg": thetas.tolist(), "sigma_vv": (4.0 * math.pi * np.abs(F[0]) ** 2).tolist(),
+           "sigma_hh": (4.0 * math.pi * np.abs(F[1]) ** 2).tolist(), "amp_vv": F[0].tolist(),
+           "amp_hh": F[1].tolist(), "modes_used": used, "n_unknowns": sys_.n_full,
+           "n_junctions": len(sys_.junctions), "linear_residual": res, "m_max": sys_.m_max}
+    if keep:
+        out["_modes"] = sols
+    return out
+
+
+def coated_layers(interface_points, core_points, eps_list, mu_list):
+    """Surfaces and regions of a PEC core under N full layers, interfaces
+    outermost first, media innermost first (bor_solver.py:2102-2131)."""
+    N = len(interface_points)
+    surfaces = [(p, False) for p in interface_points] + [(core_points, True)]
+    regions = [{"medium": None, "bounds": [(0, 1)], "exterior": True}]
+    for i in range(N):
+        lay = N - i
+        regions.append({"medium": (eps_list[lay - 1], mu_list[lay - 1]), "bounds": [(i, -1), (i + 1, 1)]})
+    return surfaces, regions

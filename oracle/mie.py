@@ -117,3 +117,62 @@ def sigma_dielectric_sphere(radius_m: float, eps_r: complex, mu_r: complex, freq
     s = np.sum(((-1.0) ** n) * (2 * n + 1) * (a[1:] - b[1:]))
     lam = 2.0 * math.pi / k
     return float(lam ** 2 / (4.0 * math.pi) * abs(s) ** 2)
+
+
+def _riccati_pair(nmax: int, z: complex):
+    """psi_n(z) = z j_n(z), chi_n(z) = z y_n(z) and their derivatives for
+    complex z (two independent radial solutions inside a layer)."""
+    n = np.arange(nmax + 1)
+    j = special.spherical_jn(n, z)
+    dj = special.spherical_jn(n, z, derivative=True)
+    y = special.spherical_yn(n, z)
+    dy = special.spherical_yn(n, z, derivative=True)
+    return z * j, j + z * dj, z * y, y + z * dy
+
+
+def multilayer_pec_coeffs(radii, eps_list, mu_list, k0: float, nmax: int):
+    """PEC core of radius radii[0] under layers j = 1..L (outer radii
+    radii[1:], media inner to outer).  In layer j the radial function is
+    U = psi(k_j r) + R_j chi(k_j r); the admittance Y = w_j U'/U (w_j =
+    k_j / (k0 eps_j) for TM, k_j / (k0 mu_j) for TE) is continuous across
+    every interface, so R_{j+1} follows from Y at r_j and the exterior
+    U = psi(k0 r) + A xi(k0 r) from Y at the outer radius:
+    A = (Y psi - psi') / (xi' - Y xi).  The core sets U' = 0 (TM) or U = 0
+    (TE) at radii[0].  The same convention as dielectric_sphere_coeffs (the
+    e^{+jwt} outgoing xi)."""
+    L = len(eps_list)
+    ks = [k0 * causal_index(e, u) for e, u in zip(eps_list, mu_list)]
+    out = []
+    for tm in (True, False):
+        w = [ks[j] / (k0 * complex(eps_list[j] if tm else mu_list[j])) for j in range(L)]
+        ps, dps, ch, dch = _riccati_pair(nmax, ks[0] * radii[0])
+        R = -(dps / dch) if tm else -(ps / ch)
+        for j in range(L):
+            r = radii[j + 1]
+            ps, dps, ch, dch = _riccati_pair(nmax, ks[j] * r)
+            Y = w[j] * (dps + R * dch) / (ps + R * ch)
+            if j + 1 < L:
+                ps, dps, ch, dch = _riccati_pair(nmax, ks[j + 1] * r)
+                R = (Y * ps - w[j + 1] * dps) / (w[j + 1] * dch - Y * ch)
+        psi, dpsi, xi, dxi = _riccati(nmax, k0 * radii[-1])
+        out.append((Y * psi - dpsi) / (dxi - Y * xi))
+    return out[0], out[1]
+
+
+def sigma_multilayer_pec_sphere(radii, eps_list, mu_list, freq_hz: float) -> float:
+    """Monostatic RCS (m^2) of a PEC sphere under concentric layers
+    (radii = [core, r1, ..., rL] ascending, media inner to outer)."""
+    k = 2.0 * math.pi * freq_hz / C0
+    scale = max([abs(causal_index(e, u)) for e, u in zip(eps_list, mu_list)] + [1.0])
+    a, b = multilayer_pec_coeffs([float(r) for r in radii], [complex(e) for e in eps_list],
+                                 [complex(u) for u in mu_list], k, _nmax(scale * k * radii[-1]))
+    n = np.arange(1, len(a))
+    s = np.sum(((-1.0) ** n) * (2 * n + 1) * (a[1:] - b[1:]))
+    lam = 2.0 * math.pi / k
+    return float(lam ** 2 / (4.0 * math.pi) * abs(s) ** 2)
+
+
+def sigma_coated_pec_sphere(core_radius: float, outer_radius: float, eps_r: complex, mu_r: complex,
+                            freq_hz: float) -> float:
+    """One coating layer (mie_sphere.py:254-264)."""
+    return sigma_multilayer_pec_sphere([core_radius, outer_radius], [eps_r], [mu_r], freq_hz)
