@@ -80,6 +80,14 @@ SIGNATURES = [
     ("ghost_bor_set_impedance", _ci, [_vp, _ci, _dp, _cd, _cd]),
     ("ghost_bor_set_medium", _ci, [_vp, _ci, _cd, _cd, _cd]),
     ("ghost_bor_set_interior", _ci, [_vp, _ci, _cd, _cd, _cd, _cd, _ci, _ci]),
+    ("ghost_bor_add_operator", _ci, [_vp, _dp, _ci, _cd, _cd, _cd, _cd, _ci, _ci, _ci, _ci, _dp, _ci, _ip]),
+    ("ghost_bor_add_cross", _ci, [_vp, _ci, _ci, _cd, _ci, _ip]),
+    ("ghost_bor_add_region", _ci, [_vp, _ci, _ci, _dp, _ci, _ip]),
+    ("ghost_bor_region_block", _ci, [_vp, _ci, _ci, _ci, _ci, _cd, _cd, _ci, _ci]),
+    ("ghost_bor_region_map", _ci, [_vp, _ci, _ci, _ci, _ip, _dp]),
+    ("ghost_bor_region_exterior", _ci, [_vp, _ci, _ci, _ci, _ci]),
+    ("ghost_bor_region_get_dF", _ci, [_vp, _ci, _dp]),
+    ("ghost_bor_region_system", _ci, [_vp, _ci, _ci, _ci, _ip, _dp, _dp]),
     ("ghost_device_name", _ci, [ctypes.c_char_p, _ci]),
 ]
 
@@ -164,6 +172,8 @@ class Handle:
         self.precision = precision
         self.problems = []
         self.nodes_per_problem = []
+        self.ops = []
+        self.region_sets = []
         self.dense = {}
 
     def close(self):
@@ -212,6 +222,73 @@ class Handle:
         k_in, eta_in = complex(k_in), complex(eta_in)
         check(self.lib.ghost_bor_set_interior(self._h, int(prob), k_in.real, k_in.imag, eta_in.real, eta_in.imag,
                                               int(n_xi_efie), int(n_xi_bracket)))
+
+    # ── multi-region bodies ──
+    def add_operator(self, nodes, k, eta, m_max: int, n_xi_efie: int, n_xi_bracket: int, brackets: bool,
+                     thetas_deg=()) -> int:
+        """One generatrix in one medium (complex k, eta), operators only."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        th = np.ascontiguousarray(np.atleast_1d(np.asarray(thetas_deg, dtype=np.float64)))
+        k, eta = complex(k), complex(eta)
+        idx = ctypes.c_int(-1)
+        check(self.lib.ghost_bor_add_operator(self._h, dptr(nodes), int(nodes.shape[0]), k.real, k.imag, eta.real,
+                                              eta.imag, int(m_max), int(n_xi_efie), int(n_xi_bracket),
+                                              1 if brackets else 0, dptr(th), int(th.size), ctypes.byref(idx)))
+        self.ops.append((int(nodes.shape[0]), int(m_max)))
+        return idx.value
+
+    def add_cross(self, op_test: int, op_src: int, near_factor: float = 2.0, near_order: int = 12) -> int:
+        idx = ctypes.c_int(-1)
+        check(self.lib.ghost_bor_add_cross(self._h, int(op_test), int(op_src), float(near_factor), int(near_order),
+                                           ctypes.byref(idx)))
+        return idx.value
+
+    def add_region(self, m_max: int, n_full: int, thetas_deg) -> int:
+        th = np.ascontiguousarray(np.atleast_1d(np.asarray(thetas_deg, dtype=np.float64)))
+        idx = ctypes.c_int(-1)
+        check(self.lib.ghost_bor_add_region(self._h, int(m_max), int(n_full), dptr(th), int(th.size),
+                                            ctypes.byref(idx)))
+        self.region_sets.append((int(m_max), int(th.size)))
+        return idx.value
+
+    def region_block(self, rset: int, src: int, idx: int, which: int, coef: complex, row_off: int, col_off: int):
+        coef = complex(coef)
+        check(self.lib.ghost_bor_region_block(self._h, int(rset), int(src), int(idx), int(which), coef.real,
+                                              coef.imag, int(row_off), int(col_off)))
+
+    def region_map(self, rset: int, m: int, red_of_full, coef):
+        red = np.ascontiguousarray(red_of_full, dtype=np.int32)
+        cf = np.ascontiguousarray(coef, dtype=np.float64)
+        n_red = int(red.max()) + 1
+        check(self.lib.ghost_bor_region_map(self._h, int(rset), int(m), n_red, iptr(red), dptr(cf)))
+
+    def region_exterior(self, rset: int, op: int, off_J: int, off_M: int = -1):
+        check(self.lib.ghost_bor_region_exterior(self._h, int(rset), int(op), int(off_J), int(off_M)))
+
+    def region_dF(self, rset: int) -> np.ndarray:
+        """complex [2 m_max + 1, 2 (VV, HH), n_theta]"""
+        m_max, nth = self.region_sets[rset]
+        out = np.zeros((2 * m_max + 1, 2, nth, 2), dtype=np.float64)
+        check(self.lib.ghost_bor_region_get_dF(self._h, int(rset), dptr(out)))
+        return out[..., 0] + 1j * out[..., 1]
+
+    def region_system(self, rset: int, m: int):
+        """(A [n, n], B [n, nrhs]) of one mode as assembled (after fill, before solve)."""
+        nth = self.region_sets[rset][1]
+        n = ctypes.c_int(0)
+        check(self.lib.ghost_bor_region_system(self._h, int(rset), int(m), 0, ctypes.byref(n), None, None))
+        A = np.zeros((n.value, n.value, 2))
+        B = np.zeros((n.value, 2 * nth, 2))
+        check(self.lib.ghost_bor_region_system(self._h, int(rset), int(m), 0, ctypes.byref(n), dptr(A), dptr(B)))
+        return A[..., 0] + 1j * A[..., 1], B[..., 0] + 1j * B[..., 1]
+
+    def region_solution(self, rset: int, m: int) -> np.ndarray:
+        nth = self.region_sets[rset][1]
+        n = ctypes.c_int(0)
+        check(self.lib.ghost_bor_region_system(self._h, int(rset), int(m), 1, ctypes.byref(n), None, None))
+        X = np.zeros((n.value, 2 * nth, 2))
+        check(self.lib.ghost_bor_region_system(self._h, int(rset), int(m), 1, ctypes.byref(n), dptr(X), None))
+        return X[..., 0] + 1j * X[..., 1]
 
     def add_dense(self, n: int, nrhs: int) -> int:
         """A caller-filled n x n system with nrhs columns; returns its system index."""

@@ -19,7 +19,7 @@ default True).
 from __future__ import annotations
 
 import math
-from typing import Callable, Dict, List, Optional, Sequence
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -393,6 +393,384 @@ def solve_bor_dielectric(points, freq_hz: float, thetas_deg, eps_r: complex, mu_
     if progress is not None:
         progress(out["modes_used"], out["m_max"])
     return out
+
+
+# ─────────────────────────────────────────────────────────────────────────────
+# multi-region bodies (bor_solver.py:1100-1297, 1805-2172)
+# ─────────────────────────────────────────────────────────────────────────────
+
+def _medium(freq_hz: float, medium):
+    """(k, eta) of free space (None) or of (eps_r, mu_r)."""
+    k0 = 2.0 * math.pi * freq_hz / C0
+    if medium is None:
+        return complex(k0), complex(ETA0)
+    mi, er = causal_medium(*medium)
+    return k0 * mi, ETA0 * er
+
+
+class MultiRegionBor:
+    """Surfaces (generatrix, is_conductor) bounding homogeneous regions
+    ({"medium": None | (eps_r, mu_r), "bounds": [(surface, sigma), ...],
+    "exterior": bool}, sigma = +1 where the surface's normal points into the
+    region), solved per mode on PMCHWT (bor_solver.py:1805-2041).
+
+    The host keeps the layout: per surface its J unknowns (2 Nn) and, for an
+    interface, M / eta0 (2 Nn); the sign sigma_i sigma_j of each block; the
+    junctions (coinciding off-axis generatrix ends) and the reduction that
+    ties a follower's ends to the junction's lead and drops the bases the
+    axis or a conductor line removes.  The device fills every surface's
+    operators in each region it bounds, the cross-surface operators between
+    the surfaces of a region, sums the blocks, applies the reduction,
+    factors, solves and radiates the exterior surfaces' currents."""
+
+    def __init__(self, surfaces, regions, freq_hz: float, gauss_order: int = 4, near_factor: float = 2.0,
+                 near_order: int = 12):
+        if int(gauss_order) != 4:
+            raise ValueError("the GPU BoR path is built for gauss_order = 4 (the reference default).")
+        self.freq_hz = float(freq_hz)
+        self.pts = [_validate_nodes(p) for (p, _) in surfaces]
+        self.cond = [bool(c) for (_, c) in surfaces]
+        self.regions = regions
+        self.near_factor, self.near_order = float(near_factor), int(near_order)
+        ns = len(surfaces)
+        self.adj: List[List[int]] = [[] for _ in range(ns)]
+        self.sigma: Dict = {}
+        for ri, reg in enumerate(regions):
+            for (si, sg) in reg["bounds"]:
+                self.adj[si].append(ri)
+                self.sigma[(ri, si)] = int(sg)
+        ext = [ri for ri, r in enumerate(regions) if r.get("exterior")]
+        if len(ext) != 1:
+            raise ValueError("exactly one region must be the exterior.")
+        self.ext = ext[0]
+        if regions[self.ext].get("medium") is not None:
+            raise ValueError("the exterior region is free space.")
+        if any(sg != 1 for (_, sg) in regions[self.ext]["bounds"]):
+            raise ValueError("surfaces bounding the exterior need sigma = +1 there.")
+        for si in range(ns):
+            if not self.adj[si]:
+                raise ValueError(f"surface {si} bounds no region.")
+        self.media = [_medium(self.freq_hz, r.get("medium")) for r in regions]
+        self.Nn = [len(p) for p in self.pts]
+        self.off_J, self.off_M, acc = [], [], 0
+        for si in range(ns):
+            self.off_J.append(acc)
+            acc += 2 * self.Nn[si]
+            self.off_M.append(None if self.cond[si] else acc)
+            acc += 0 if self.cond[si] else 2 * self.Nn[si]
+        self.n_full = acc
+        self.junctions = self._junctions()
+
+    def _on_axis(self, si: int, node: int) -> bool:
+        p = self.pts[si]
+        return p[node, 0] <= AXIS_TOL * max(1.0, float(np.max(p[:, 0])))
+
+    def _junctions(self):
+        """Coinciding off-axis ends, grouped; the first interface of a group
+        leads it (bor_solver.py:1856-1895)."""
+        allp = np.vstack(self.pts)
+        tol = 1e-8 * max(float(np.ptp(allp[:, 0])) + float(np.ptp(allp[:, 1])), 1e-9)
+        groups: List[List[Tuple[int, int]]] = []
+        for si, p in enumerate(self.pts):
+            for node in (0, len(p) - 1):
+                if self._on_axis(si, node):
+                    continue
+                for g in groups:
+                    s0, n0 = g[0]
+                    if float(np.hypot(*(self.pts[s0][n0] - p[node]))) <= tol:
+                        g.append((si, node))
+                        break
+                else:
+                    groups.append([(si, node)])
+        for g in groups:
+            if len(g) < 2:
+                raise ValueError(f"surface {g[0][0]} ends off the axis at a point no other surface reaches.")
+            lead = next((i for i, (si, _) in enumerate(g) if not self.cond[si]), 0)
+            g[0], g[lead] = g[lead], g[0]
+            for (si, _) in g[1:]:
+                if not set(self.adj[si]) & set(self.adj[g[0][0]]):
+                    raise ValueError("a junction surface shares no region with the junction's lead.")
+        return groups
+
+    def reduction(self, m: int):
+        """(red_of_full, coef, n_red): the reduced unknown and coefficient of
+        each full unknown (-1: dropped).  Axis ends keep J_t only for |m| = 1
+        and never J_phi; a follower's end takes its lead's unknowns with the
+        signs of the shared region's sigma and of the traversal direction
+        (t: -sigma_l dir_l / (sigma_f dir_f), phi: sigma_l / sigma_f); where a
+        conductor meets the junction, M_t of the interfaces there is 0
+        (bor_solver.py:1908-1986)."""
+        members = {x for g in self.junctions for x in g}
+        follower = {x for g in self.junctions for x in g[1:]}
+        mt_off = {x for g in self.junctions if any(self.cond[s] for (s, _) in g)
+                  for x in g if not self.cond[x[0]]}
+        red = np.full(self.n_full, -1, dtype=np.int64)
+        coef = np.zeros(self.n_full)
+        n = 0
+        for si in range(len(self.pts)):
+            Nn = self.Nn[si]
+            for off, is_m in ((self.off_J[si], False), (self.off_M[si], True)):
+                if off is None:
+                    continue
+                t_on, f_on = np.ones(Nn, dtype=bool), np.ones(Nn, dtype=bool)
+                for end in (0, Nn - 1):
+                    if self._on_axis(si, end):
+                        t_on[end], f_on[end] = abs(m) == 1, False
+                    elif (si, end) in follower:
+                        t_on[end] = f_on[end] = False
+                    elif (si, end) in members:
+                        if is_m and (si, end) in mt_off:
+                            t_on[end] = False
+                    else:
+                        t_on[end] = False
+                for base, on in ((off, t_on), (off + Nn, f_on)):
+                    idx = base + np.flatnonzero(on)
+                    red[idx] = np.arange(n, n + idx.size)
+                    coef[idx] = 1.0
+                    n += idx.size
+        for g in self.junctions:
+            sl, nl = g[0]
+            for (sf, nf) in g[1:]:
+                r = sorted(set(self.adj[sf]) & set(self.adj[sl]))[0]
+                dl, df = (1 if nl != 0 else -1), (1 if nf != 0 else -1)
+                sgl, sgf = self.sigma[(r, sl)], self.sigma[(r, sf)]
+                ties = [(self.off_J[sl], self.off_J[sf])]
+                if self.off_M[sl] is not None and self.off_M[sf] is not None:
+                    ties.append((self.off_M[sl], self.off_M[sf]))
+                for ol, of in ties:
+                    for half, c in ((0, -(sgl * dl) / (sgf * df)), (1, sgl / sgf)):
+                        lead = red[ol + half * self.Nn[sl] + nl]
+                        if lead >= 0:
+                            red[of + half * self.Nn[sf] + nf] = lead
+                            coef[of + half * self.Nn[sf] + nf] = c
+        return red, coef, n
+
+    def rho_max(self) -> float:
+        return max(float(np.max(p[:, 0])) for p in self.pts)
+
+    def build(self, h, thetas, m_max: int):
+        """Add this body's operators, cross operators and region set to
+        handle h; returns the set index."""
+        ops = {}
+        for si, p in enumerate(self.pts):
+            for ri in self.adj[si]:
+                k, eta = self.media[ri]
+                osc = 2.0 * abs(k) * gauss_rho_max(p)
+                ops[(si, ri)] = h.add_operator(p, k, eta, m_max, n_xi_efie(osc, m_max), n_xi_bracket(osc, m_max),
+                                               not self.cond[si], thetas if ri == self.ext else ())
+        cross = {}
+        for ri, reg in enumerate(self.regions):
+            ids = [si for (si, _) in reg["bounds"]]
+            for si in ids:
+                for sj in ids:
+                    if si != sj:
+                        cross[(ri, si, sj)] = h.add_cross(ops[(si, ri)], ops[(sj, ri)], self.near_factor,
+                                                          self.near_order)
+        rs = h.add_region(m_max, self.n_full, thetas)
+        for ri, reg in enumerate(self.regions):
+            r2 = (ETA0 / self.media[ri][1]) ** 2
+            for (si, sgi) in reg["bounds"]:
+                for (sj, sgj) in reg["bounds"]:
+                    ss = float(sgi * sgj)
+                    src, idx = (0, ops[(si, ri)]) if si == sj else (1, cross[(ri, si, sj)])
+                    h.region_block(rs, src, idx, 0, ss, self.off_J[si], self.off_J[sj])
+                    if self.off_M[sj] is not None:
+                        h.region_block(rs, src, idx, 1, -ETA0 * ss, self.off_J[si], self.off_M[sj])
+                    if self.off_M[si] is not None:
+                        h.region_block(rs, src, idx, 1, ETA0 * ss, self.off_M[si], self.off_J[sj])
+                        if self.off_M[sj] is not None:
+                            h.region_block(rs, src, idx, 0, r2 * ss, self.off_M[si], self.off_M[sj])
+        for m in range(-m_max, m_max + 1):
+            red, coef, _ = self.reduction(m)
+            h.region_map(rs, m, red, coef)
+        for (si, _) in self.regions[self.ext]["bounds"]:
+            h.region_exterior(rs, ops[(si, self.ext)], self.off_J[si],
+                              -1 if self.off_M[si] is None else self.off_M[si])
+        return rs
+
+    def solve(self, thetas_deg, n_modes: Optional[int] = None, mode_tol: float = 1e-6, *, residual: bool = True,
+              nb: Optional[int] = None, tune: bool = True, pairing: bool = True,
+              check_abort: Optional[Callable] = None) -> Dict:
+        """Monostatic far field over thetas_deg (bor_solver.py:2044-2071)."""
+        thetas = np.atleast_1d(np.asarray(thetas_deg, dtype=float))
+        k0 = 2.0 * math.pi * self.freq_hz / C0
+        if n_modes is None:
+            st = float(np.max(np.abs(np.sin(np.radians(thetas)))))
+            n_modes = int(math.ceil(k0 * self.rho_max() * max(st, 0.05))) + 12
+        m_max = int(n_modes)
+        variant = -1
+        if nb is None:
+            from . import autotune
+            n_max = max(self.reduction(m)[2] for m in (0, 1))
+            nb, variant = autotune.best(_lib.GHOST_FP64, n_max, 2 * m_max + 1, 2 * len(thetas), tune=tune)
+        if check_abort is not None:
+            check_abort()
+        h = _lib.Handle(_lib.GHOST_FP64, nb=nb, residual=residual, pairing=pairing)
+        try:
+            h.set_gemm_variant(variant)
+            rs = self.build(h, thetas, m_max)
+            h.build()
+            h.run(0)
+            dF = h.region_dF(rs)
+            res = h.residuals() if residual else None
+            times = h.timings()
+        finally:
+            h.close()
+        if check_abort is not None:
+            check_abort()
+        F, used = truncate_modes(dF, m_max, mode_tol)
+        out = {"theta_deg": thetas.tolist(), "sigma_vv": (4.0 * math.pi * np.abs(F[0]) ** 2).tolist(),
+               "sigma_hh": (4.0 * math.pi * np.abs(F[1]) ** 2).tolist(), "amp_vv": F[0].tolist(),
+               "amp_hh": F[1].tolist(), "modes_used": used, "n_unknowns": int(self.n_full),
+               "n_junctions": len(self.junctions), "m_max": m_max, "timings_ms": times,
+               "lu_tiles": (int(nb), int(variant))}
+        if res is not None:
+            ms = np.arange(-m_max, m_max + 1)
+            out["linear_residual"] = float(np.max(res[np.abs(ms) <= used]))
+        else:
+            out["linear_residual"] = float("nan")
+        return out
+
+
+def _multiregion(surfaces, regions, freq_hz, thetas_deg, n_modes, mode_tol, gauss_order, near_factor, near_order,
+                 progress, check_abort, formulation: str, extra: Dict) -> Dict:
+    sys_ = MultiRegionBor(surfaces, regions, freq_hz, gauss_order, near_factor, near_order)
+    out = sys_.solve(thetas_deg, n_modes, mode_tol, check_abort=check_abort)
+    out["formulation"] = formulation
+    out.update(extra)
+    if progress is not None:
+        progress(out["modes_used"], out["m_max"])
+    return out
+
+
+def coated_layers(interface_points, points_core, eps_list, mu_list):
+    """Surfaces and regions of a PEC core under N full layers: interfaces
+    outermost first, media innermost first (bor_solver.py:2102-2131)."""
+    N = len(interface_points)
+    if len(eps_list) != N or len(mu_list) != N:
+        raise ValueError("One (eps, mu) per layer, innermost first.")
+    surfaces = [(p, False) for p in interface_points] + [(points_core, True)]
+    regions = [{"medium": None, "bounds": [(0, +1)], "exterior": True}]
+    for i in range(N):
+        lay = N - i
+        regions.append({"medium": (eps_list[lay - 1], mu_list[lay - 1]), "bounds": [(i, -1), (i + 1, +1)]})
+    return surfaces, regions
+
+
+def solve_bor_coated_pec(points_outer, points_core, freq_hz: float, thetas_deg, eps_r: complex,
+                         mu_r: complex = 1.0, n_modes: Optional[int] = None, gauss_order: int = 4,
+                         mode_tol: float = 1e-6, near_factor: float = 2.0, near_order: int = 12,
+                         workers: int = 1, progress: Optional[Callable] = None,
+                         check_abort: Optional[Callable] = None) -> Dict:
+    """PEC core fully covered by one homogeneous layer (bor_solver.py:1299-1393):
+    unknowns J, M / eta0 on the outer interface and J on the core; the region
+    system of that layout (the reference's explicit 3 x 3 block form differs
+    from it by the sign of the core rows, which leaves the currents alone)."""
+    surfaces, regions = coated_layers([points_outer], points_core, [eps_r], [mu_r])
+    return _multiregion(surfaces, regions, freq_hz, thetas_deg, n_modes, mode_tol, gauss_order, near_factor,
+                        near_order, progress, check_abort, "pmchwt-coated",
+                        {"eps_r": complex(eps_r), "mu_r": complex(mu_r)})
+
+
+def solve_bor_coated2_pec(points_outer, points_mid, points_core, freq_hz: float, thetas_deg,
+                          eps_inner: complex, mu_inner: complex, eps_outer: complex, mu_outer: complex,
+                          n_modes: Optional[int] = None, gauss_order: int = 4, mode_tol: float = 1e-6,
+                          near_factor: float = 2.0, near_order: int = 12, workers: int = 1,
+                          progress: Optional[Callable] = None, check_abort: Optional[Callable] = None) -> Dict:
+    """PEC core under two full layers (bor_solver.py:2074-2099)."""
+    surfaces, regions = coated_layers([points_outer, points_mid], points_core, [eps_inner, eps_outer],
+                                      [mu_inner, mu_outer])
+    return _multiregion(surfaces, regions, freq_hz, thetas_deg, n_modes, mode_tol, gauss_order, near_factor,
+                        near_order, progress, check_abort, "pmchwt-coated-2layer",
+                        {"eps_inner": complex(eps_inner), "eps_outer": complex(eps_outer)})
+
+
+def solve_bor_coated_n_pec(interface_points, points_core, freq_hz: float, thetas_deg, eps_list, mu_list,
+                           n_modes: Optional[int] = None, gauss_order: int = 4, mode_tol: float = 1e-6,
+                           near_factor: float = 2.0, near_order: int = 12, workers: int = 1,
+                           progress: Optional[Callable] = None, check_abort: Optional[Callable] = None) -> Dict:
+    """PEC core under N full layers; interfaces outermost first, media
+    innermost first (bor_solver.py:2102-2131)."""
+    surfaces, regions = coated_layers(list(interface_points), points_core, list(eps_list), list(mu_list))
+    N = len(interface_points)
+    return _multiregion(surfaces, regions, freq_hz, thetas_deg, n_modes, mode_tol, gauss_order, near_factor,
+                        near_order, progress, check_abort, f"pmchwt-coated-{N}layer",
+                        {"eps_layers": [complex(e) for e in eps_list]})
+
+
+def solve_bor_coating_patch(points_patch, points_mid_covered, points_mid_bare, points_core, freq_hz: float,
+                            thetas_deg, eps_inner: complex, mu_inner: complex, eps_patch: complex,
+                            mu_patch: complex, n_modes: Optional[int] = None, gauss_order: int = 4,
+                            mode_tol: float = 1e-6, near_factor: float = 2.0, near_order: int = 12,
+                            workers: int = 1, progress: Optional[Callable] = None,
+                            check_abort: Optional[Callable] = None) -> Dict:
+    """A second coating layer over part of a fully coated PEC body, meeting
+    the inner coating's interface at dielectric triple junctions
+    (bor_solver.py:2134-2176)."""
+    bare = list(points_mid_bare) if isinstance(points_mid_bare, (list, tuple)) else [points_mid_bare]
+    surfaces = [(points_patch, False), (points_mid_covered, False)] + [(p, False) for p in bare]
+    surfaces.append((points_core, True))
+    core = len(surfaces) - 1
+    bare_ids = list(range(2, 2 + len(bare)))
+    regions = [{"medium": None, "exterior": True, "bounds": [(0, +1)] + [(b, +1) for b in bare_ids]},
+               {"medium": (eps_patch, mu_patch), "bounds": [(0, -1), (1, +1)]},
+               {"medium": (eps_inner, mu_inner), "bounds": [(1, -1)] + [(b, -1) for b in bare_ids] + [(core, +1)]}]
+    return _multiregion(surfaces, regions, freq_hz, thetas_deg, n_modes, mode_tol, gauss_order, near_factor,
+                        near_order, progress, check_abort, "pmchwt-coating-patch",
+                        {"eps_inner": complex(eps_inner), "eps_patch": complex(eps_patch)})
+
+
+class BorCrossOperators:
+    """T and P between the test bases of one BorPecSolver's surface and the
+    source bases of another's in one medium (bor_solver.py:1115-1297),
+    filled on the device for every mode |m| <= m_max at once and cached."""
+
+    def __init__(self, sp: "BorPecSolver", sq: "BorPecSolver", near_factor: float = 2.0, near_order: int = 12):
+        if not np.isclose(complex(sp.k), complex(sq.k)):
+            raise ValueError("Cross operators need both solvers in the same medium.")
+        self.sp, self.sq = sp, sq
+        self.near_factor, self.near_order = float(near_factor), int(near_order)
+        self._cache: Dict = {}
+
+    def prepare(self, m_max: int) -> None:
+        self._blocks(m_max)
+
+    def _blocks(self, m_max: int):
+        if m_max not in self._cache:
+            sp, sq = self.sp, self.sq
+            Np, Nq = sp.Nn, sq.Nn
+            n = 2 * (Np + Nq)
+            h = _lib.Handle(_lib.GHOST_FP64, residual=False)
+            try:
+                ops = []
+                for s_ in (sp, sq):
+                    osc = 2.0 * abs(s_.k) * gauss_rho_max(s_.pts)
+                    ops.append(h.add_operator(s_.pts, s_.k, s_.eta, m_max, n_xi_efie(osc, m_max),
+                                              n_xi_bracket(osc, m_max), True))
+                x = h.add_cross(ops[0], ops[1], self.near_factor, self.near_order)
+                sets = []
+                for which in (0, 1):
+                    rs = h.add_region(m_max, n, [0.0])
+                    h.region_block(rs, 1, x, which, 1.0, 0, 2 * Np)
+                    for m in range(-m_max, m_max + 1):
+                        h.region_map(rs, m, np.arange(n), np.ones(n))
+                    sets.append(rs)
+                h.build()
+                h.fill()
+                h.sync()
+                self._cache[m_max] = {m: tuple(h.region_system(rs, m)[0][:2 * Np, 2 * Np:] for rs in sets)
+                                      for m in range(-m_max, m_max + 1)}
+            finally:
+                h.close()
+        return self._cache[m_max]
+
+    def assemble_T(self, m: int, m_max: int) -> np.ndarray:
+        """Cross EFIE operator [2 Np, 2 Nq] with C = j k eta 2 pi of the medium."""
+        return self._blocks(m_max)[int(m)][0].copy()
+
+    def assemble_P(self, m: int, m_max: int) -> np.ndarray:
+        """Cross rotated-PV operator [2 Np, 2 Nq] (columns remapped as assemble_pmchwt_P)."""
+        return self._blocks(m_max)[int(m)][1].copy()
 
 
 # ─────────────────────────────────────────────────────────────────────────────

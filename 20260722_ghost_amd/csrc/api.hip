@@ -97,33 +97,28 @@ static void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w
   if (n & 1) x[n / 2] = 0.0;
 }
 
-// Graded cells of [0,1]^2 toward s = s' (kind 0) or a shared corner (kind 1:
-// s = 1, s' = 0 for f = e + 1; kind 2: s = 0, s' = 1 for f = e - 1)
-// (bor_solver.py:54-79), depth 4.
-static void graded_cells(int kind, double a0, double a1, double b0, double b1, int lev,
+// Graded cells of [0,1]^2 toward s = s' (diag) or toward the corner
+// (s, s') = (ca, cb) (bor_solver.py:54-79), depth 4.
+static void graded_cells(bool diag, double ca, double cb, double a0, double a1, double b0, double b1, int lev,
                          std::vector<double>& cells) {
-  bool hit;
-  if (kind == 0) hit = !(a1 <= b0 || b1 <= a0);
-  else {
-    const double ca = kind == 1 ? 1.0 : 0.0, cb = kind == 1 ? 0.0 : 1.0;
-    hit = a0 <= ca && ca <= a1 && b0 <= cb && cb <= b1;
-  }
+  const bool hit = diag ? !(a1 <= b0 || b1 <= a0) : (a0 <= ca && ca <= a1 && b0 <= cb && cb <= b1);
   if (lev >= 4 || !hit) {
     cells.push_back(a0); cells.push_back(a1); cells.push_back(b0); cells.push_back(b1);
     return;
   }
   const double am = 0.5 * (a0 + a1), bm = 0.5 * (b0 + b1);
-  graded_cells(kind, a0, am, b0, bm, lev + 1, cells);
-  graded_cells(kind, a0, am, bm, b1, lev + 1, cells);
-  graded_cells(kind, am, a1, b0, bm, lev + 1, cells);
-  graded_cells(kind, am, a1, bm, b1, lev + 1, cells);
+  graded_cells(diag, ca, cb, a0, am, b0, bm, lev + 1, cells);
+  graded_cells(diag, ca, cb, a0, am, bm, b1, lev + 1, cells);
+  graded_cells(diag, ca, cb, am, a1, b0, bm, lev + 1, cells);
+  graded_cells(diag, ca, cb, am, a1, bm, b1, lev + 1, cells);
 }
 
-// Tensor Gauss points on each cell: order 4 on the test axis, 5 on the source
-// axis (bor_solver.py:85-110).
-static void cell_points(int kind, std::vector<double>& s, std::vector<double>& sp, std::vector<double>& w) {
+// Tensor Gauss points on each graded cell: order 4 on the test axis, 5 on the
+// source axis (bor_solver.py:85-110).
+static void graded_points(bool diag, double ca, double cb, std::vector<double>& s, std::vector<double>& sp,
+                          std::vector<double>& w) {
   std::vector<double> cells;
-  graded_cells(kind, 0.0, 1.0, 0.0, 1.0, 0, cells);
+  graded_cells(diag, ca, cb, 0.0, 1.0, 0.0, 1.0, 0, cells);
   std::vector<double> x4, w4, x5, w5;
   gauss_legendre(4, x4, w4);
   gauss_legendre(5, x5, w5);
@@ -137,6 +132,24 @@ static void cell_points(int kind, std::vector<double>& s, std::vector<double>& s
         w.push_back((0.5 * w4[i] * ha) * (0.5 * w5[j] * hb));
       }
   }
+}
+
+// Same-surface near kinds: 0 the element with itself, 1 f = e + 1 (shared
+// corner s = 1, s' = 0), 2 f = e - 1 (s = 0, s' = 1).
+static void cell_points(int kind, std::vector<double>& s, std::vector<double>& sp, std::vector<double>& w) {
+  graded_points(kind == 0, kind == 1 ? 1.0 : 0.0, kind == 1 ? 0.0 : 1.0, s, sp, w);
+}
+
+// n x n Gauss tensor grid on [0,1]^2, test axis slowest (bor_solver.py:1203-1209).
+static void tensor_points(int n, std::vector<double>& s, std::vector<double>& sp, std::vector<double>& w) {
+  std::vector<double> x, wx;
+  gauss_legendre(n, x, wx);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      s.push_back(0.5 * (x[i] + 1.0));
+      sp.push_back(0.5 * (x[j] + 1.0));
+      w.push_back(0.25 * wx[i] * wx[j]);
+    }
 }
 
 // ─────────────────────────────────────────────────────────────────────────────
@@ -166,6 +179,7 @@ struct Problem {
   // names its interior-medium twin in ghost_bor::inner
   bool pen = false;
   int inner = -1;
+  bool standalone = false;       // an operator of a multi-region body (ghost_bor_add_operator)
   void* d_zfar2 = nullptr;
   double2* d_NB2 = nullptr;
   double2* d_zs = nullptr;
@@ -197,8 +211,65 @@ struct Problem {
   double2* d_NB = nullptr;
   void* d_zfar = nullptr;
   double* d_alt = nullptr;       // adjoint: perturbed pts | nodes | elem
-  int cell_off[3] = {0, 0, 0}, cell_cnt[3] = {0, 0, 0};
+  int cell_off[8] = {0}, cell_cnt[8] = {0};
   size_t tab_off[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// Cross-surface operator: EFIE T and rotated-PV brackets B between the test
+// bases of one operator surface and the source bases of another in one
+// medium (bor_solver.py:1115-1297), node-based [2 Np][2 Nq] far blocks with
+// the near pairs added in.
+struct Cross {
+  int op_p = -1, op_q = -1;
+  int Np = 0, Nq = 0, Nep = 0, Neq = 0, M = 0, nmodes = 0, nE = 0, nB = 0, zmodes = 0, zoff = 0;
+  double k = 0, ki = 0, eta_re = 0, eta_im = 0;
+  double near_factor = 2.0;
+  int near_order = 12;
+  std::vector<NearPairInfo> pairs;   // near pairs: kind 0 tensor grid, 1 + 2a + b corner<a><b>
+  std::vector<int2> batches;
+  int npp = 0;
+  std::vector<unsigned char> skip;   // [Nep][Neq]
+  std::vector<int> plist;            // pair indices grouped by parity class
+  int par_off[5] = {0, 0, 0, 0, 0};
+  int cell_off[8] = {0}, cell_cnt[8] = {0};
+  std::vector<double> cells, tail_u, tail_w;
+  size_t tab_off[6] = {0, 0, 0, 0, 0, 0};
+  double* d_tabs = nullptr;
+  unsigned char* d_skip = nullptr;
+  double* d_cells = nullptr;
+  double* d_core = nullptr;
+  double* d_tail = nullptr;
+  NearPairInfo* d_pairs = nullptr;
+  int2* d_batches = nullptr;
+  int* d_plist = nullptr;
+  double2* d_NP = nullptr;
+  double2* d_NB = nullptr;
+  double2* d_NB2 = nullptr;
+  double2* d_zfar = nullptr;
+  double2* d_zfar2 = nullptr;
+  double2* d_zs = nullptr;           // unit source weights of the brackets
+};
+
+// Multi-region system set (bor_solver.py:1805-2041): per mode a system
+// Q^T (sum of coefficient x operator blocks) Q, the right-hand sides of the
+// surfaces that bound the exterior, and their far field.
+struct Region {
+  int M = 0, nmodes = 0, n_full = 0, ntheta = 0, nrhs = 0, sys0 = 0;
+  std::vector<double> theta;
+  struct Block { int src, idx, which; double cre, cim; int row_off, col_off; };
+  std::vector<Block> blocks;
+  std::vector<std::vector<int>> red;     // per mode: reduced index of each full index (-1: dropped)
+  std::vector<std::vector<double>> cf;   // per mode: its coefficient
+  struct Ext { int op, off_J, off_M; };
+  std::vector<Ext> ext;
+  size_t dF_off = 0;
+  double2* d_full = nullptr;             // [nmodes][n_full][n_full]
+  double2* d_bfull = nullptr;            // [nmodes][n_full][nrhs]
+  double2* d_xfull = nullptr;            // [nmodes][n_full][nrhs]
+  int* d_ptr = nullptr;
+  int* d_mem = nullptr;
+  double* d_cf = nullptr;
+  std::vector<SysInfo*> d_vsys;          // per exterior surface: one entry per mode
 };
 
 struct Launch {  // one batched kernel launch of the solve schedule
@@ -234,7 +305,10 @@ struct ghost_bor {
   bool built = false;
   bool solved = false;               // the augmented columns hold the solution of the current fill
   std::vector<Problem> probs;
-  std::vector<Problem> inner;        // interior media of penetrable problems (fill only)
+  std::vector<Problem> inner;        // fill-only problems: interior media of penetrable problems and
+                                     // the operators of multi-region bodies
+  std::vector<Cross> cross;          // cross-surface operators
+  std::vector<Region> regions;       // multi-region system sets
   std::vector<SysInfo> sys;          // one entry per (problem, mode) and per dense system
   std::vector<SysInfo> ss;           // the systems factored and solved (nrhs = all their columns)
   int pairing = 1;                   // solve mode -m with mode m's factors
@@ -444,6 +518,14 @@ template <typename R, int NT>
 static void launch_far_fill(bool split, bool ibc, bool ck, int wpb, int grid, size_t tab_bytes, hipStream_t s,
                             const FarArgs& fa) {
   if constexpr (sizeof(R) == 8) {
+    if (split && ibc) {   // cross-surface operators of multi-region bodies (any medium)
+      if (wpb == 1) hipLaunchKernelGGL((k_far_fill<R, NT, true, true, 1, true>), dim3(grid), dim3(64), tab_bytes, s, fa);
+      else {
+        constexpr int W = far_waves(sizeof(R), NT);
+        hipLaunchKernelGGL((k_far_fill<R, NT, true, true, W, true>), dim3(grid), dim3(64 * W), tab_bytes, s, fa);
+      }
+      return;
+    }
     if (ck) {   // lossy medium (never split: the adjoint refuses media)
       const dim3 b(64 * wpb);
       if (wpb == 1) {
@@ -646,6 +728,286 @@ extern "C" int ghost_bor_set_interior(ghost_bor* h, int prob, double k_re, doubl
     sy.pen = sy.n;
     sy.n *= 2;
   }
+  return 0;
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// C ABI: multi-region bodies (bor_solver.py:1100-1297, 1805-2172)
+// ─────────────────────────────────────────────────────────────────────────────
+
+static int pow2_xi(int v) { return v >= 4 && v <= 4096 && (v & (v - 1)) == 0; }
+
+// Generatrix of a fill-only problem: nodes (rho >= 0, snapped), element
+// lengths and tangents, axis ends.
+static int load_generatrix(Problem& p, const double* nodes, int Nn) {
+  if (!nodes || Nn < 3) return fail(GHOST_ERR_ARG, "need at least 3 generatrix nodes");
+  p.Nn = Nn;
+  p.Ne = Nn - 1;
+  p.P = 4 * p.Ne;
+  p.nodes.assign(nodes, nodes + 2 * Nn);
+  double rmax = 0;
+  for (int i = 0; i < Nn; ++i) {
+    if (!std::isfinite(p.nodes[2 * i]) || !std::isfinite(p.nodes[2 * i + 1]))
+      return fail(GHOST_ERR_ARG, "generatrix node %d is not finite", i);
+    if (p.nodes[2 * i] < -1e-12) return fail(GHOST_ERR_ARG, "generatrix rho must be >= 0");
+    p.nodes[2 * i] = std::max(p.nodes[2 * i], 0.0);
+    rmax = std::max(rmax, p.nodes[2 * i]);
+  }
+  p.L.clear(); p.trho.clear(); p.tz.clear();
+  for (int e = 0; e < p.Ne; ++e) {
+    const double dr = p.nodes[2 * e + 2] - p.nodes[2 * e], dz = p.nodes[2 * e + 3] - p.nodes[2 * e + 1];
+    const double L = std::hypot(dr, dz);
+    if (!(L > 0)) return fail(GHOST_ERR_ARG, "generatrix has a zero-length element (%d)", e);
+    p.L.push_back(L);
+    p.trho.push_back(dr / L);
+    p.tz.push_back(dz / L);
+  }
+  const double tol = 1e-12 * std::max(1.0, rmax);
+  p.axis0 = p.nodes[0] <= tol;
+  p.axis1 = p.nodes[2 * (Nn - 1)] <= tol;
+  return 0;
+}
+
+// One surface in one homogeneous medium, filled as operators only (no
+// systems of its own): the EFIE operator T, and with brackets != 0 also the
+// unit-weight rotated-PV brackets B that the PMCHWT coupling blocks take
+// (bor_solver.py:554-575).  thetas: aspects of the plane-wave excitation and
+// far field when the surface bounds the exterior (n_theta may be 0).
+extern "C" int ghost_bor_add_operator(ghost_bor* h, const double* nodes, int Nn, double k_re, double k_im,
+                                      double eta_re, double eta_im, int m_max, int n_xi_efie, int n_xi_bracket,
+                                      int brackets, const double* thetas_deg, int n_theta, int* op) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "add operators before build");
+  if (h->precision != GHOST_FP64) return fail(GHOST_ERR_ARG, "multi-region bodies need an FP64 handle");
+  if (!std::isfinite(k_re) || !std::isfinite(k_im) || !std::isfinite(eta_re) || !std::isfinite(eta_im))
+    return fail(GHOST_ERR_ARG, "medium parameters must be finite");
+  if (!(k_re > 0.0) || k_im > 0.0) return fail(GHOST_ERR_ARG, "k needs Re k > 0 and Im k <= 0");
+  if (eta_re == 0.0 && eta_im == 0.0) return fail(GHOST_ERR_ARG, "wave impedance must be nonzero");
+  if (m_max < 0 || m_max > 126) return fail(GHOST_ERR_ARG, "m_max must lie in [0, 126]");
+  if (!pow2_xi(n_xi_efie) || !pow2_xi(n_xi_bracket))
+    return fail(GHOST_ERR_ARG, "xi grid sizes must be powers of two in [4, 4096]");
+  if (far_fill_waves(8, (m_max + 2 + 15) / 16, n_xi_efie, n_xi_bracket) == 0)
+    return fail(GHOST_ERR_ARG, "far fill: azimuth tables of n_xi %d / %d exceed the 160 KiB of LDS", n_xi_efie,
+                n_xi_bracket);
+  if (n_theta < 0 || (n_theta > 0 && !thetas_deg)) return fail(GHOST_ERR_ARG, "bad aspect list");
+  Problem q;
+  int rc = load_generatrix(q, nodes, Nn);
+  if (rc) return rc;
+  q.k = k_re; q.ki = k_im; q.eta_re = eta_re; q.eta_im = eta_im;
+  q.M = m_max; q.nmodes = 2 * m_max + 1;
+  q.nE = n_xi_efie; q.nB = n_xi_bracket;
+  q.cE = 1.0; q.cM = 0.0;
+  if (brackets) {
+    q.zs.assign(q.Ne, make_double2(1.0, 0.0));
+    q.has_zs = true; q.cI = 1.0; q.cMass = 0.0; q.pen = true;
+  }
+  q.ntheta = n_theta; q.nrhs = 2 * n_theta;
+  for (int i = 0; i < n_theta; ++i) q.theta.push_back(thetas_deg[i] * M_PI / 180.0);
+  q.standalone = true;
+  q.sys0 = -1;
+  if (op) *op = (int)h->inner.size();
+  h->inner.push_back(std::move(q));
+  return 0;
+}
+
+static double gauss_rho_max(const Problem& p) {
+  std::vector<double> x4, w4;
+  gauss_legendre(4, x4, w4);
+  double r = 0.0;
+  for (int e = 0; e < p.Ne; ++e)
+    for (int g = 0; g < 4; ++g) {
+      const double s = 0.5 * (x4[g] + 1.0);
+      r = std::max(r, p.nodes[2 * e] + s * (p.nodes[2 * e + 2] - p.nodes[2 * e]));
+    }
+  return r;
+}
+
+static double seg_point(const double* c, const double* a, const double* b) {
+  const double abx = b[0] - a[0], aby = b[1] - a[1];
+  double t = ((c[0] - a[0]) * abx + (c[1] - a[1]) * aby) / std::max(abx * abx + aby * aby, 1e-300);
+  t = std::min(1.0, std::max(0.0, t));
+  return std::hypot(c[0] - (a[0] + t * abx), c[1] - (a[1] + t * aby));
+}
+
+// The cross-surface operator between the test bases of operator op_test and
+// the source bases of op_src (both in one medium).  Element pairs sharing an
+// endpoint (a junction) are integrated on corner-graded cells, pairs closer
+// than near_factor x the longer element on a near_order^2 tensor grid, both
+// with the near kernels; the rest on the far tables, whose n_xi follow the
+// reference's rule for the two point sets (bor_kernels.py:149-152, 489-492).
+extern "C" int ghost_bor_add_cross(ghost_bor* h, int op_test, int op_src, double near_factor, int near_order,
+                                   int* idx) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "add cross operators before build");
+  const int ni = (int)h->inner.size();
+  if (op_test < 0 || op_test >= ni || op_src < 0 || op_src >= ni || !h->inner[op_test].standalone ||
+      !h->inner[op_src].standalone)
+    return fail(GHOST_ERR_ARG, "bad operator index");
+  if (op_test == op_src) return fail(GHOST_ERR_ARG, "a cross operator joins two different surfaces");
+  if (!(near_factor > 0.0) || near_order < 1 || near_order > 64)
+    return fail(GHOST_ERR_ARG, "near_factor must be > 0 and near_order in [1, 64]");
+  const Problem& a = h->inner[op_test];
+  const Problem& b = h->inner[op_src];
+  if (a.k != b.k || a.ki != b.ki || a.eta_re != b.eta_re || a.eta_im != b.eta_im)
+    return fail(GHOST_ERR_ARG, "a cross operator needs both surfaces in the same medium");
+  if (a.M != b.M) return fail(GHOST_ERR_ARG, "both operators need the same m_max");
+  Cross c;
+  c.op_p = op_test; c.op_q = op_src;
+  c.Np = a.Nn; c.Nq = b.Nn; c.Nep = a.Ne; c.Neq = b.Ne; c.M = a.M; c.nmodes = a.nmodes;
+  c.k = a.k; c.ki = a.ki; c.eta_re = a.eta_re; c.eta_im = a.eta_im;
+  c.near_factor = near_factor; c.near_order = near_order;
+  const double osc = 2.0 * std::hypot(a.k, a.ki) * std::sqrt(gauss_rho_max(a) * gauss_rho_max(b));
+  c.nE = (int)std::pow(2.0, std::ceil(std::log2(std::max({64.0, 4.0 * (c.M + 2), 6.0 * (osc + 4.0)}))));
+  c.nB = (int)std::pow(2.0, std::ceil(std::log2(std::max({128.0, 6.0 * (c.M + 2), 8.0 * (osc + 4.0)}))));
+  if (c.nE > 4096 || c.nB > 4096 || far_fill_waves(8, (c.M + 2 + 15) / 16, c.nE, c.nB) == 0)
+    return fail(GHOST_ERR_ARG, "cross operator: azimuth grids %d / %d beyond the far fill's limits", c.nE, c.nB);
+  // near classification (bor_solver.py:1131-1164)
+  auto span = [](const Problem& p) {
+    double lo = p.nodes[0], hi = p.nodes[0];
+    for (double v : p.nodes) { lo = std::min(lo, v); hi = std::max(hi, v); }
+    return hi - lo;
+  };
+  const double tol = 1e-9 * std::max({span(a), span(b), 1e-9});
+  c.skip.assign((size_t)c.Nep * c.Neq, 0);
+  for (int e = 0; e < c.Nep; ++e)
+    for (int f = 0; f < c.Neq; ++f) {
+      const double* pe[2] = {&a.nodes[2 * e], &a.nodes[2 * e + 2]};
+      const double* qf[2] = {&b.nodes[2 * f], &b.nodes[2 * f + 2]};
+      int nshared = 0, sa = 0, sb = 0;
+      for (int u = 0; u < 2; ++u)
+        for (int v = 0; v < 2; ++v)
+          if (std::hypot(pe[u][0] - qf[v][0], pe[u][1] - qf[v][1]) <= tol) { ++nshared; sa = u; sb = v; }
+      if (nshared > 1) return fail(GHOST_ERR_ARG, "surfaces share a whole element (%d, %d)", e, f);
+      const double d = std::min({seg_point(pe[0], qf[0], qf[1]), seg_point(pe[1], qf[0], qf[1]),
+                                 seg_point(qf[0], pe[0], pe[1]), seg_point(qf[1], pe[0], pe[1])});
+      int kind = -1;
+      if (nshared == 1) kind = 1 + 2 * sa + sb;
+      else if (d <= tol) return fail(GHOST_ERR_ARG, "surfaces cross or touch away from a junction (%d, %d)", e, f);
+      else if (d < near_factor * std::max(a.L[e], b.L[f])) kind = 0;
+      if (kind < 0) continue;
+      NearPairInfo pi;
+      pi.e = e; pi.f = f; pi.kind = kind; pi.pp_off = 0; pi.n_tail = 0; pi.tail_off = 0;
+      c.pairs.push_back(pi);
+      c.skip[(size_t)e * c.Neq + f] = 1;
+    }
+  if (idx) *idx = (int)h->cross.size();
+  h->cross.push_back(std::move(c));
+  return 0;
+}
+
+// A multi-region system set: one system per mode -m_max..m_max over n_full
+// unknowns before the junction map, thetas the aspects.
+extern "C" int ghost_bor_add_region(ghost_bor* h, int m_max, int n_full, const double* thetas_deg, int n_theta,
+                                    int* set) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "add region systems before build");
+  if (h->precision != GHOST_FP64) return fail(GHOST_ERR_ARG, "multi-region bodies need an FP64 handle");
+  if (m_max < 0 || m_max > 126) return fail(GHOST_ERR_ARG, "m_max must lie in [0, 126]");
+  if (n_full < 1) return fail(GHOST_ERR_ARG, "n_full must be positive");
+  if (!thetas_deg || n_theta < 1) return fail(GHOST_ERR_ARG, "need at least one aspect angle");
+  Region r;
+  r.M = m_max; r.nmodes = 2 * m_max + 1; r.n_full = n_full; r.ntheta = n_theta; r.nrhs = 2 * n_theta;
+  for (int i = 0; i < n_theta; ++i) r.theta.push_back(thetas_deg[i] * M_PI / 180.0);
+  r.red.assign(r.nmodes, std::vector<int>());
+  r.cf.assign(r.nmodes, std::vector<double>());
+  r.sys0 = (int)h->sys.size();
+  for (int m = -m_max; m <= m_max; ++m) {
+    SysInfo sy;
+    memset(&sy, 0, sizeof(sy));
+    sy.prob = -1;
+    sy.m = m;
+    sy.nrhs = r.nrhs;
+    sy.ncols = r.nrhs;
+    h->sys.push_back(sy);
+  }
+  if (set) *set = (int)h->regions.size();
+  h->regions.push_back(std::move(r));
+  return 0;
+}
+
+// Add coef x block to every mode's full system at (row_off, col_off): src 0
+// operator idx (its own surface), 1 cross operator idx; which 0 the EFIE
+// operator T, 1 the coupling operator P (brackets, columns remapped).
+extern "C" int ghost_bor_region_block(ghost_bor* h, int set, int src, int idx, int which, double c_re, double c_im,
+                                      int row_off, int col_off) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "add blocks before build");
+  if (set < 0 || set >= (int)h->regions.size()) return fail(GHOST_ERR_ARG, "bad region set");
+  Region& r = h->regions[set];
+  if (!std::isfinite(c_re) || !std::isfinite(c_im)) return fail(GHOST_ERR_ARG, "block coefficient must be finite");
+  if (which != 0 && which != 1) return fail(GHOST_ERR_ARG, "which is 0 (T) or 1 (P)");
+  int Np, Nq, M;
+  if (src == 0) {
+    if (idx < 0 || idx >= (int)h->inner.size() || !h->inner[idx].standalone)
+      return fail(GHOST_ERR_ARG, "bad operator index");
+    const Problem& o = h->inner[idx];
+    if (which == 1 && !o.pen) return fail(GHOST_ERR_ARG, "operator %d was added without brackets", idx);
+    Np = Nq = o.Nn;
+    M = o.M;
+  } else if (src == 1) {
+    if (idx < 0 || idx >= (int)h->cross.size()) return fail(GHOST_ERR_ARG, "bad cross operator index");
+    Np = h->cross[idx].Np;
+    Nq = h->cross[idx].Nq;
+    M = h->cross[idx].M;
+  } else {
+    return fail(GHOST_ERR_ARG, "src is 0 (operator) or 1 (cross operator)");
+  }
+  if (M != r.M) return fail(GHOST_ERR_ARG, "block m_max %d differs from the set's %d", M, r.M);
+  if (row_off < 0 || col_off < 0 || row_off + 2 * Np > r.n_full || col_off + 2 * Nq > r.n_full)
+    return fail(GHOST_ERR_ARG, "block outside the full system");
+  r.blocks.push_back(Region::Block{src, idx, which, c_re, c_im, row_off, col_off});
+  return 0;
+}
+
+// Junction map of mode m: full index i goes to reduced index red_of_full[i]
+// (-1: dropped) with coefficient coef[i] (x_full = Q x_red; A_red = Q^T A Q).
+extern "C" int ghost_bor_region_map(ghost_bor* h, int set, int m, int n_red, const int* red_of_full,
+                                    const double* coef) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "set maps before build");
+  if (set < 0 || set >= (int)h->regions.size()) return fail(GHOST_ERR_ARG, "bad region set");
+  Region& r = h->regions[set];
+  if (m < -r.M || m > r.M) return fail(GHOST_ERR_ARG, "mode outside range");
+  if (n_red < 1 || n_red > 4096) return fail(GHOST_ERR_ARG, "reduced size %d outside [1, 4096]", n_red);
+  if (!red_of_full || !coef) return fail(GHOST_ERR_ARG, "null map");
+  std::vector<int> used(n_red, 0);
+  for (int i = 0; i < r.n_full; ++i) {
+    if (red_of_full[i] < -1 || red_of_full[i] >= n_red) return fail(GHOST_ERR_ARG, "map entry %d out of range", i);
+    if (red_of_full[i] >= 0) {
+      if (!std::isfinite(coef[i])) return fail(GHOST_ERR_ARG, "map coefficient %d is not finite", i);
+      used[red_of_full[i]] = 1;
+    }
+  }
+  for (int u = 0; u < n_red; ++u)
+    if (!used[u]) return fail(GHOST_ERR_ARG, "reduced unknown %d of mode %d has no full index", u, m);
+  r.red[m + r.M].assign(red_of_full, red_of_full + r.n_full);
+  r.cf[m + r.M].assign(coef, coef + r.n_full);
+  SysInfo& sy = h->sys[r.sys0 + m + r.M];
+  sy.n = n_red;
+  sy.nt = n_red;
+  return 0;
+}
+
+// A surface bounding the exterior: operator op (free space, the set's
+// aspects) carries J in full rows [off_J, off_J + 2 Nn) and, off_M >= 0, M /
+// eta0 right after them (off_M = off_J + 2 Nn).  Its rows take the plane
+// wave's <W, E_inc> (and eta0 <W, H_inc>), and its currents radiate.
+extern "C" int ghost_bor_region_exterior(ghost_bor* h, int set, int op, int off_J, int off_M) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
+  if (h->built) return fail(GHOST_ERR_STATE, "set exterior surfaces before build");
+  if (set < 0 || set >= (int)h->regions.size()) return fail(GHOST_ERR_ARG, "bad region set");
+  Region& r = h->regions[set];
+  if (op < 0 || op >= (int)h->inner.size() || !h->inner[op].standalone) return fail(GHOST_ERR_ARG, "bad operator");
+  const Problem& o = h->inner[op];
+  if (o.M != r.M) return fail(GHOST_ERR_ARG, "operator m_max differs from the set's");
+  if (o.ntheta != r.ntheta) return fail(GHOST_ERR_ARG, "an exterior operator needs the set's aspects");
+  for (int i = 0; i < r.ntheta; ++i)
+    if (std::fabs(o.theta[i] - r.theta[i]) > 1e-15) return fail(GHOST_ERR_ARG, "an exterior operator needs the set's aspects");
+  if (o.ki != 0.0) return fail(GHOST_ERR_ARG, "an exterior operator is in free space");
+  if (off_J < 0 || off_J + 2 * o.Nn > r.n_full) return fail(GHOST_ERR_ARG, "off_J outside the full system");
+  if (off_M >= 0 && (off_M != off_J + 2 * o.Nn || off_M + 2 * o.Nn > r.n_full))
+    return fail(GHOST_ERR_ARG, "M rows must follow the surface's J rows");
+  r.ext.push_back(Region::Ext{op, off_J, off_M});
   return 0;
 }
 
@@ -1245,10 +1607,192 @@ static int build_schedule(ghost_bor* h) {
   return 0;
 }
 
+// Far azimuth tables of n_xi nE / nB with exact values at 0, pi/2, pi, 3pi/2:
+// sin^2(pi j/nE) | cos(2 pi i/nE) | cos, sin(2 pi j/nB) half tables | full cos, sin.
+static void azimuth_tables(int nE, int nB, std::vector<double>& tabs, size_t* off) {
+  auto cos_exact = [](long long i, int N) {
+    i %= N;
+    if (i == 0) return 1.0;
+    if (2 * i == N) return -1.0;
+    if (4 * i == N || 4 * i == 3 * (long long)N) return 0.0;
+    return std::cos(2.0 * M_PI * (double)i / N);
+  };
+  auto sin_exact = [](long long i, int N) {
+    i %= N;
+    if (i == 0 || 2 * i == N) return 0.0;
+    if (4 * i == N) return 1.0;
+    if (4 * i == 3 * (long long)N) return -1.0;
+    return std::sin(2.0 * M_PI * (double)i / N);
+  };
+  tabs.clear();
+  off[0] = tabs.size();
+  for (int j = 0; j <= nE / 2; ++j) { const double v = std::sin(M_PI * j / nE); tabs.push_back(j == nE / 2 ? 1.0 : v * v); }
+  off[1] = tabs.size();
+  for (int i = 0; i < nE; ++i) tabs.push_back(cos_exact(i, nE));
+  off[2] = tabs.size();
+  for (int j = 0; j <= nB / 2; ++j) tabs.push_back(cos_exact(j, nB));
+  off[3] = tabs.size();
+  for (int j = 0; j <= nB / 2; ++j) tabs.push_back(sin_exact(j, nB));
+  off[4] = tabs.size();
+  for (int i = 0; i < nB; ++i) tabs.push_back(cos_exact(i, nB));
+  off[5] = tabs.size();
+  for (int i = 0; i < nB; ++i) tabs.push_back(sin_exact(i, nB));
+}
+
+template <typename T> static int upload(ghost_bor* h, T** d, const std::vector<T>& v) {
+  int rc = dalloc(h, d, std::max<size_t>(1, v.size()) * sizeof(T));
+  if (rc) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int prepare_cross(ghost_bor* h, Cross& c) {
+  const Problem& a = h->inner[c.op_p];
+  const Problem& b = h->inner[c.op_q];
+  // cell tables: 0 tensor grid, 1 + 2a + b corner<a><b>
+  std::vector<double> cs[5], csp[5], cw[5];
+  tensor_points(c.near_order, cs[0], csp[0], cw[0]);
+  for (int kd = 1; kd < 5; ++kd) graded_points(false, (double)((kd - 1) >> 1), (double)((kd - 1) & 1), cs[kd], csp[kd], cw[kd]);
+  int off = 0;
+  for (int kd = 0; kd < 5; ++kd) { c.cell_off[kd] = off; c.cell_cnt[kd] = (int)cs[kd].size(); off += c.cell_cnt[kd]; }
+  c.cells.clear();
+  for (int kd = 0; kd < 5; ++kd) c.cells.insert(c.cells.end(), cs[kd].begin(), cs[kd].end());
+  for (int kd = 0; kd < 5; ++kd) c.cells.insert(c.cells.end(), csp[kd].begin(), csp[kd].end());
+  for (int kd = 0; kd < 5; ++kd) c.cells.insert(c.cells.end(), cw[kd].begin(), cw[kd].end());
+  // tail orders per pair (bor_kernels.py:238-240), point-pair offsets, batches
+  std::map<int, int> tail_of;
+  int pp = 0;
+  c.batches.clear();
+  for (size_t i = 0; i < c.pairs.size(); ++i) {
+    NearPairInfo& pi = c.pairs[i];
+    const int kd = pi.kind;
+    double osc = 0.0;
+    const double r0e = a.nodes[2 * pi.e], dre = a.nodes[2 * pi.e + 2] - r0e;
+    const double r0f = b.nodes[2 * pi.f], drf = b.nodes[2 * pi.f + 2] - r0f;
+    for (size_t q = 0; q < cs[kd].size(); ++q) {
+      const double rp = r0e + cs[kd][q] * dre, rq = r0f + csp[kd][q] * drf;
+      osc = std::max(osc, std::hypot(c.k, c.ki) * std::sqrt(std::max(4.0 * rp * rq, 1e-300)));
+    }
+    const int nt = (int)std::min(1024.0, std::max(64.0, std::ceil(4.0 * (osc / M_PI + (c.M + 2)))));
+    if (!tail_of.count(nt)) {
+      tail_of[nt] = (int)c.tail_u.size();
+      std::vector<double> xt, wt;
+      gauss_legendre(nt, xt, wt);
+      for (int j = 0; j < nt; ++j) { c.tail_u.push_back(0.5 * (xt[j] + 1.0)); c.tail_w.push_back(0.5 * wt[j]); }
+    }
+    pi.n_tail = nt;
+    pi.tail_off = tail_of[nt];
+    pi.pp_off = pp;
+    for (int q = 0; q < c.cell_cnt[kd]; q += 32) c.batches.push_back(make_int2((int)i, q));
+    pp += c.cell_cnt[kd];
+  }
+  c.npp = pp;
+  // pairs by parity class of (e, f): disjoint footprints within a class
+  c.plist.clear();
+  for (int cls = 0; cls < 4; ++cls) {
+    c.par_off[cls] = (int)c.plist.size();
+    for (size_t i = 0; i < c.pairs.size(); ++i)
+      if (2 * (c.pairs[i].e & 1) + (c.pairs[i].f & 1) == cls) c.plist.push_back((int)i);
+  }
+  c.par_off[4] = (int)c.plist.size();
+  c.zoff = h->pairing ? 0 : (c.M + 15) / 16 * 16;
+  c.zmodes = (c.zoff + c.M + 1 + 15) / 16 * 16;
+  std::vector<double> tabs;
+  azimuth_tables(c.nE, c.nB, tabs, c.tab_off);
+  std::vector<double> core;
+  {
+    std::vector<double> x48, w48;
+    gauss_legendre(48, x48, w48);
+    for (int i = 0; i < 48; ++i) core.push_back(0.5 * (x48[i] + 1.0));
+    for (int i = 0; i < 48; ++i) core.push_back(0.5 * w48[i]);
+  }
+  std::vector<double> tail(c.tail_u);
+  tail.insert(tail.end(), c.tail_w.begin(), c.tail_w.end());
+  // unit weights: the brackets take the source element's, the (zero) mass
+  // term reads the test element's, so both surfaces' element counts fit
+  std::vector<double2> ones(std::max(c.Nep, c.Neq), make_double2(1.0, 0.0));
+  int rc;
+  if ((rc = upload(h, &c.d_tabs, tabs)) || (rc = upload(h, &c.d_skip, c.skip)) || (rc = upload(h, &c.d_cells, c.cells)) ||
+      (rc = upload(h, &c.d_core, core)) || (rc = upload(h, &c.d_tail, tail)) || (rc = upload(h, &c.d_pairs, c.pairs)) ||
+      (rc = upload(h, &c.d_batches, c.batches)) || (rc = upload(h, &c.d_plist, c.plist)))
+    return rc;
+  double2* d_ones = nullptr;
+  if ((rc = upload(h, &d_ones, ones))) return rc;
+  c.d_zs = d_ones;
+  const size_t np = std::max<size_t>(1, c.pairs.size());
+  if ((rc = dalloc(h, &c.d_NP, std::max<size_t>(1, (size_t)c.npp) * (c.M + 2) * 5 * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &c.d_NB, np * 16 * c.nmodes * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &c.d_NB2, np * 16 * c.nmodes * sizeof(double2)))) return rc;
+  const size_t zel = (size_t)(2 * c.Np) * (2 * c.Nq) * c.zmodes;
+  if ((rc = dalloc(h, &c.d_zfar, zel * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &c.d_zfar2, zel * sizeof(double2)))) return rc;
+  return 0;
+}
+
+// Device side of a region set: full-system, right-hand-side and current
+// buffers, the junction map as lists, one virtual system entry per mode for
+// each exterior surface (its rows of the full vectors, unmasked).
+static int prepare_region(ghost_bor* h, Region& r) {
+  int rc;
+  const size_t nf = (size_t)r.n_full;
+  if ((rc = dalloc(h, &r.d_full, (size_t)r.nmodes * nf * nf * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &r.d_bfull, (size_t)r.nmodes * nf * r.nrhs * sizeof(double2)))) return rc;
+  if ((rc = dalloc(h, &r.d_xfull, (size_t)r.nmodes * nf * r.nrhs * sizeof(double2)))) return rc;
+  std::vector<int> ptr((size_t)r.nmodes * (nf + 1), 0), mem((size_t)r.nmodes * nf, 0);
+  std::vector<double> cf((size_t)r.nmodes * nf, 0.0);
+  for (int mi = 0; mi < r.nmodes; ++mi) {
+    const int nr = h->sys[r.sys0 + mi].n;
+    std::vector<std::vector<int>> lists(nr);
+    for (int i = 0; i < r.n_full; ++i)
+      if (r.red[mi][i] >= 0) lists[r.red[mi][i]].push_back(i);
+    int* P = &ptr[(size_t)mi * (nf + 1)];
+    int u = 0;
+    for (int q = 0; q < nr; ++q) {
+      P[q] = u;
+      for (int i : lists[q]) {
+        mem[(size_t)mi * nf + u] = i;
+        cf[(size_t)mi * nf + u] = r.cf[mi][i];
+        ++u;
+      }
+    }
+    P[nr] = u;
+  }
+  if ((rc = upload(h, &r.d_ptr, ptr)) || (rc = upload(h, &r.d_mem, mem)) || (rc = upload(h, &r.d_cf, cf))) return rc;
+  r.d_vsys.clear();
+  for (const auto& e : r.ext) {
+    const Problem& o = h->inner[e.op];
+    std::vector<SysInfo> vs(r.nmodes);
+    for (int mi = 0; mi < r.nmodes; ++mi) {
+      SysInfo& v = vs[mi];
+      memset(&v, 0, sizeof(v));
+      v.prob = -1;
+      v.m = mi - r.M;
+      v.nt = o.Nn;
+      v.t0 = 0;
+      v.f0 = 0;
+      v.pen = e.off_M >= 0 ? 2 * o.Nn : 0;
+      v.n = e.off_M >= 0 ? 4 * o.Nn : 2 * o.Nn;
+      v.nrhs = r.nrhs;
+      v.ncols = r.nrhs;
+      v.x_off = ((long long)mi * r.n_full + e.off_J) * r.nrhs;
+    }
+    SysInfo* d = nullptr;
+    if ((rc = upload(h, &d, vs))) return rc;
+    r.d_vsys.push_back(d);
+  }
+  return 0;
+}
+
 extern "C" int ghost_bor_build(ghost_bor* h) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
   if (h->built) return fail(GHOST_ERR_STATE, "already built");
   if (h->sys.empty()) return fail(GHOST_ERR_STATE, "no problems or dense systems added");
+  for (size_t ri = 0; ri < h->regions.size(); ++ri) {
+    const Region& r = h->regions[ri];
+    for (int mi = 0; mi < r.nmodes; ++mi)
+      if (h->sys[r.sys0 + mi].n < 1) return fail(GHOST_ERR_STATE, "region set %zu: no map for mode %d", ri, mi - r.M);
+    if (r.blocks.empty()) return fail(GHOST_ERR_STATE, "region set %zu has no blocks", ri);
+  }
   HIPCHK(hipSetDevice(h->device));
   const bool mixed = h->precision == GHOST_MIXED;
   // systems to solve: every mode, or with pairing modes m >= 0 only (mode -m
@@ -1301,6 +1845,10 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
     if ((rc = prepare_problem(h, p))) return rc;
   for (auto& p : h->inner)
     if ((rc = prepare_problem(h, p))) return rc;
+  for (auto& c : h->cross)
+    if ((rc = prepare_cross(h, c))) return rc;
+  for (auto& r : h->regions)
+    if ((rc = prepare_region(h, r))) return rc;
   if ((rc = build_schedule(h))) return rc;
   const size_t csz = mixed ? sizeof(float2) : sizeof(double2);
   if ((rc = dalloc(h, &h->d_sys, h->sys.size() * sizeof(SysInfo)))) return rc;
@@ -1354,6 +1902,14 @@ extern "C" int ghost_bor_build(ghost_bor* h) {
     const size_t nch = (p.P + h->ff_chunk - 1) / h->ff_chunk;
     h->partial_elems = std::max(h->partial_elems, nch * p.nmodes * p.nrhs);
   }
+  for (auto& r : h->regions) {
+    r.dF_off = h->dF_elems;
+    h->dF_elems += (size_t)r.nmodes * r.nrhs;
+    for (const auto& e : r.ext) {
+      const size_t nch = (h->inner[e.op].P + h->ff_chunk - 1) / h->ff_chunk;
+      h->partial_elems = std::max(h->partial_elems, nch * r.nmodes * r.nrhs);
+    }
+  }
   if ((rc = dalloc(h, &h->d_dF, h->dF_elems * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &h->d_partial, h->partial_elems * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &h->d_red, 2 * h->sys.size() * sizeof(double)))) return rc;   // per-system probe ratios
@@ -1394,6 +1950,9 @@ template <typename R> static int far_fill(ghost_bor* h, Problem& p, const Geom& 
   fa.sinB = p.d_tabs + p.tab_off[5];
   fa.Ne = p.Ne;
   fa.Nn = p.Nn;
+  fa.Neq = p.Ne;
+  fa.Nnq = p.Nn;
+  fa.skip = nullptr;
   fa.M = p.M;
   fa.nE = p.nE;
   fa.nB = p.nB;
@@ -1698,11 +2257,190 @@ static int rhs_problem(ghost_bor* h, Problem& p, const Geom& tg) {
   return 0;
 }
 
-static int run_fill(ghost_bor* h) {
-  for (auto& p : h->probs) {
-    int rc = fill_problem(h, p, base_geom(p), base_geom(p));
-    if (rc) return rc;
+// Cross-surface operator: near pairs (modal values, then blocks), far pairs
+// on the split far fill with a skip mask, then the near blocks added into
+// the far blocks by parity class.
+static int cross_fill(ghost_bor* h, Cross& c) {
+  const Problem& a = h->inner[c.op_p];
+  const Problem& b = h->inner[c.op_q];
+  const size_t zel = (size_t)(2 * c.Np) * (2 * c.Nq) * c.zmodes;
+  prof_begin(h);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (zel + 255) / 256)), dim3(256), 0, h->stream,
+                     c.d_zfar, zel);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (zel + 255) / 256)), dim3(256), 0, h->stream,
+                     c.d_zfar2, zel);
+  prof_end(h, K_ZERO, 0.0);
+  LAUNCHCHK();
+  NearArgs na;
+  memset(&na, 0, sizeof(na));
+  na.nodes = a.d_nodes; na.L = a.d_elem; na.trho = a.d_elem + a.Ne; na.tz = a.d_elem + 2 * a.Ne;
+  na.snodes = b.d_nodes; na.sL = b.d_elem; na.strho = b.d_elem + b.Ne; na.stz = b.d_elem + 2 * b.Ne;
+  const size_t ncell = c.cells.size() / 3;
+  na.cell_s = c.d_cells; na.cell_sp = c.d_cells + ncell; na.cell_w = c.d_cells + 2 * ncell;
+  for (int i = 0; i < 8; ++i) { na.cell_off[i] = c.cell_off[i]; na.cell_cnt[i] = c.cell_cnt[i]; }
+  na.core_u = c.d_core; na.core_w = c.d_core + 48;
+  na.tail_u = c.d_tail; na.tail_w = c.d_tail + c.tail_w.size();
+  na.pairs = c.d_pairs; na.batches = c.d_batches;
+  na.Ne = c.Nep; na.M = c.M; na.nord = c.M + 2; na.ibc = 1; na.k = c.k; na.ki = c.ki; na.NP = c.d_NP;
+  if (!c.pairs.empty()) {
+    const int C = (na.nord + 7) / 8;
+    const dim3 g((unsigned)c.batches.size());
+    prof_begin(h);
+    switch (C) {
+#define NEARCASE(cc) \
+  case cc: hipLaunchKernelGGL((k_near_modal<cc, double>), g, dim3(256), 0, h->stream, na); break;
+      NEARCASE(1) NEARCASE(2) NEARCASE(3) NEARCASE(4) NEARCASE(5) NEARCASE(6) NEARCASE(7) NEARCASE(8)
+      NEARCASE(9) NEARCASE(10) NEARCASE(11) NEARCASE(12) NEARCASE(13) NEARCASE(14) NEARCASE(15) NEARCASE(16)
+#undef NEARCASE
+      default: return fail(GHOST_ERR_ARG, "m_max too large for the near quadrature");
+    }
+    prof_end(h, K_NEAR_MODAL, 0.0);
+    LAUNCHCHK();
+    NearBlockArgs nb;
+    memset(&nb, 0, sizeof(nb));
+    nb.na = na;
+    nb.gw = a.d_pts + 4 * a.P; nb.grho = a.d_pts; nb.gT0 = a.d_pts + 5 * a.P; nb.gT1 = a.d_pts + 6 * a.P;
+    const double ke_re = c.k * c.eta_re - c.ki * c.eta_im, ke_im = c.k * c.eta_im + c.ki * c.eta_re;
+    nb.ce_re = -ke_im * 2.0 * M_PI;
+    nb.ce_im = ke_re * 2.0 * M_PI;
+    nb.cM = 0.0; nb.zs = c.d_zs; nb.cI = 1.0; nb.cMass = 0.0;
+    nb.NB = c.d_NB; nb.NB2 = c.d_NB2;
+    const int mlanes = ((c.M + 1 + 15) / 16) * 16;
+    if (mlanes > 128) return fail(GHOST_ERR_ARG, "m_max too large for the near blocks");
+    prof_begin(h);
+    hipLaunchKernelGGL(k_near_blocks, dim3((unsigned)c.pairs.size()), dim3(256), 0, h->stream, nb, mlanes);
+    prof_end(h, K_NEAR_BLOCKS, 0.0);
+    LAUNCHCHK();
   }
+  FarArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  fa.pts = point_set(a.d_pts, a.P);
+  fa.spts = point_set(b.d_pts, b.P);
+  fa.sin2E = c.d_tabs + c.tab_off[0]; fa.cosE = c.d_tabs + c.tab_off[1];
+  fa.cxB = c.d_tabs + c.tab_off[2]; fa.sxB = c.d_tabs + c.tab_off[3];
+  fa.cosB = c.d_tabs + c.tab_off[4]; fa.sinB = c.d_tabs + c.tab_off[5];
+  fa.Ne = c.Nep; fa.Nn = c.Np; fa.Neq = c.Neq; fa.Nnq = c.Nq; fa.skip = c.d_skip;
+  fa.M = c.M; fa.nE = c.nE; fa.nB = c.nB; fa.k = c.k; fa.ki = c.ki;
+  {
+    const double ke_re = c.k * c.eta_re - c.ki * c.eta_im, ke_im = c.k * c.eta_im + c.ki * c.eta_re;
+    fa.ce_re = -ke_im * 2.0 * M_PI;
+    fa.ce_im = ke_re * 2.0 * M_PI;
+  }
+  fa.cm = 0.0; fa.zs = c.d_zs; fa.cz = 2.0 * M_PI;
+  fa.Z = c.d_zfar; fa.Z2 = c.d_zfar2;
+  fa.neg = h->pairing ? 0 : 1; fa.zmodes = c.zmodes; fa.zoff = c.zoff;
+  const int NT = (c.M + 2 + 15) / 16;
+  const size_t tab_bytes = far_table_bytes(8, c.nE, c.nB);
+  const int wpb = far_fill_waves(8, NT, c.nE, c.nB);
+  if (wpb == 0) return fail(GHOST_ERR_ARG, "cross far fill exceeds the LDS");
+  for (int pe = 0; pe < 2; ++pe)
+    for (int pf = 0; pf < 2; ++pf) {
+      const int nep = (c.Nep - pe + 1) / 2, nfp = (c.Neq - pf + 1) / 2;
+      if (nep <= 0 || nfp <= 0) continue;
+      fa.pe = pe; fa.pf = pf; fa.nfp = nfp; fa.npairs = nep * nfp; fa.delta = 0; fa.imax = 0;
+      const int grid = (fa.npairs + wpb - 1) / wpb;
+      prof_begin(h);
+      switch (NT) {
+#define FARCASE(n) \
+  case n: launch_far_fill<double, n>(true, true, true, wpb, grid, tab_bytes, h->stream, fa); break;
+        FARCASE(1) FARCASE(2) FARCASE(3) FARCASE(4) FARCASE(5) FARCASE(6) FARCASE(7) FARCASE(8)
+#undef FARCASE
+        default: return fail(GHOST_ERR_ARG, "m_max too large for the far fill");
+      }
+      prof_end(h, K_FAR_FILL, 0.0);
+      LAUNCHCHK();
+    }
+  const int mlo = h->pairing ? c.M : 0;
+  for (int cls = 0; cls < 4; ++cls) {
+    const int n = c.par_off[cls + 1] - c.par_off[cls];
+    if (n == 0) continue;
+    hipLaunchKernelGGL(k_cross_near_add, dim3(n), dim3(256), 0, h->stream, c.d_NB, c.d_NB2, c.d_pairs,
+                       c.d_plist + c.par_off[cls], c.M, c.Np, c.Nq, c.zmodes, c.zoff, mlo, c.d_zfar, c.d_zfar2);
+    LAUNCHCHK();
+  }
+  return 0;
+}
+
+// Region set: the full systems from their blocks, reduced by the junction
+// map into the augmented systems, and the right-hand sides of the exterior
+// surfaces, reduced the same way.
+static int region_fill(ghost_bor* h, Region& r) {
+  const size_t nf = (size_t)r.n_full;
+  const size_t nA = (size_t)r.nmodes * nf * nf, nB = (size_t)r.nmodes * nf * r.nrhs;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (nA + 255) / 256)), dim3(256), 0, h->stream,
+                     r.d_full, nA);
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (nB + 255) / 256)), dim3(256), 0, h->stream,
+                     r.d_bfull, nB);
+  LAUNCHCHK();
+  for (const auto& bk : r.blocks) {
+    RegionBlockArgs ra;
+    memset(&ra, 0, sizeof(ra));
+    if (bk.src == 0) {
+      const Problem& o = h->inner[bk.idx];
+      ra.Z = (const double2*)(bk.which ? o.d_zfar2 : o.d_zfar);
+      ra.NB = bk.which ? o.d_NB2 : o.d_NB;
+      ra.Np = ra.Nq = o.Nn; ra.Ne = o.Ne; ra.zmodes = o.zmodes; ra.zoff = o.zoff;
+    } else {
+      const Cross& c = h->cross[bk.idx];
+      ra.Z = bk.which ? c.d_zfar2 : c.d_zfar;
+      ra.NB = nullptr;
+      ra.Np = c.Np; ra.Nq = c.Nq; ra.Ne = c.Nep; ra.zmodes = c.zmodes; ra.zoff = c.zoff;
+    }
+    ra.M = r.M; ra.paired = h->pairing; ra.remap = bk.which;
+    ra.cre = bk.cre; ra.cim = bk.cim; ra.row_off = bk.row_off; ra.col_off = bk.col_off; ra.n_full = r.n_full;
+    ra.A = r.d_full;
+    dim3 g(2 * ra.Np, (2 * ra.Nq + 31) / 32, (r.nmodes + 31) / 32);
+    prof_begin(h);
+    hipLaunchKernelGGL(k_region_block, g, dim3(256), 0, h->stream, ra);
+    prof_end(h, K_ASSEMBLE, 0.0);
+    LAUNCHCHK();
+  }
+  int nmax = 0;
+  for (int mi = 0; mi < r.nmodes; ++mi) nmax = std::max(nmax, h->sys[r.sys0 + mi].n);
+  RegionMap q{r.d_ptr, r.d_mem, r.d_cf, r.n_full};
+  prof_begin(h);
+  hipLaunchKernelGGL(k_region_reduce, dim3(nmax, (nmax + 255) / 256, r.nmodes), dim3(256), 0, h->stream, q,
+                     r.d_full, h->d_sys + r.sys0, (double2*)h->d_A, h->d_A64);
+  prof_end(h, K_ASSEMBLE, 0.0);
+  LAUNCHCHK();
+  constexpr int MC = 8;
+  for (size_t e = 0; e < r.ext.size(); ++e) {
+    const Problem& o = h->inner[r.ext[e].op];
+    ExciteArgs ea = excite_args(o, base_geom(o));
+    dim3 gr((o.ntheta + 127) / 128, o.Nn, (r.nmodes + MC - 1) / MC);
+    prof_begin(h);
+    hipLaunchKernelGGL((k_rhs<double, MC>), gr, dim3(128), 0, h->stream, ea, r.d_vsys[e], r.nmodes, nullptr,
+                       r.d_bfull);
+    prof_end(h, K_RHS, 0.0);
+    LAUNCHCHK();
+  }
+  hipLaunchKernelGGL(k_region_rhs, dim3(nmax, (r.nrhs + 255) / 256, r.nmodes), dim3(256), 0, h->stream, q,
+                     r.d_bfull, h->d_sys + r.sys0, (double2*)h->d_A, h->d_B64);
+  LAUNCHCHK();
+  return 0;
+}
+
+static int run_fill(ghost_bor* h) {
+  int rc;
+  for (auto& o : h->inner) {
+    if (!o.standalone) continue;   // interior twins fill with their exterior problem
+    if ((rc = fill_operators(h, o, base_geom(o), base_geom(o)))) return rc;
+    if (o.ntheta > 0) {   // an exterior surface: the Bessel and phase tables of its aspects
+      ExciteArgs ea = excite_args(o, base_geom(o));
+      const size_t nbes = (size_t)o.P * o.ntheta;
+      prof_begin(h);
+      hipLaunchKernelGGL(k_bessel, dim3((unsigned)std::min<size_t>(65535, (nbes + 127) / 128)), dim3(128), 0,
+                         h->stream, ea);
+      prof_end(h, K_BESSEL, 0.0);
+      LAUNCHCHK();
+    }
+  }
+  for (auto& c : h->cross)
+    if ((rc = cross_fill(h, c))) return rc;
+  for (auto& p : h->probs)
+    if ((rc = fill_problem(h, p, base_geom(p), base_geom(p)))) return rc;
+  for (auto& r : h->regions)
+    if ((rc = region_fill(h, r))) return rc;
   return 0;
 }
 
@@ -2084,7 +2822,7 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
     LAUNCHCHK();
     const int tot = nv * p.nrhs;
     hipLaunchKernelGGL(k_farfield_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream, h->d_partial, nch,
-                       nv, p.nrhs, p.k, h->d_dF + p.dF_off + (size_t)mlo * p.nrhs);
+                       nv, p.nrhs, p.k, h->d_dF + p.dF_off + (size_t)mlo * p.nrhs, 0);
     LAUNCHCHK();
     if (mlo > 0) {
       hipLaunchKernelGGL(k_mirror_dF, dim3((p.M * p.nrhs + 255) / 256), dim3(256), 0, h->stream,
@@ -2093,6 +2831,33 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
     }
   }
   (void)nsys;
+  // region sets: x_full = Q x_red, then every exterior surface's J (and M)
+  for (auto& r : h->regions) {
+    const size_t nX = (size_t)r.nmodes * r.n_full * r.nrhs;
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (nX + 255) / 256)), dim3(256), 0, h->stream,
+                       r.d_xfull, nX);
+    int nmax = 0;
+    for (int mi = 0; mi < r.nmodes; ++mi) nmax = std::max(nmax, h->sys[r.sys0 + mi].n);
+    RegionMap q{r.d_ptr, r.d_mem, r.d_cf, r.n_full};
+    hipLaunchKernelGGL(k_region_expand, dim3(nmax, (r.nrhs + 255) / 256, r.nmodes), dim3(256), 0, h->stream, q,
+                       (const double2*)h->d_A, h->d_sys + r.sys0, r.d_xfull);
+    LAUNCHCHK();
+    for (size_t e = 0; e < r.ext.size(); ++e) {
+      const Problem& o = h->inner[r.ext[e].op];
+      ExciteArgs ea = excite_args(o, base_geom(o));
+      const int nch = (o.P + h->ff_chunk - 1) / h->ff_chunk;
+      dim3 g((r.nrhs + 255) / 256, r.nmodes, nch);
+      prof_begin(h);
+      hipLaunchKernelGGL(k_farfield_partial<double>, g, dim3(256), 0, h->stream, ea, r.d_vsys[e],
+                         (const void*)r.d_xfull, 0, h->ff_chunk, h->d_partial, r.nmodes);
+      prof_end(h, K_FARFIELD, 0.0);
+      LAUNCHCHK();
+      const int tot = r.nmodes * r.nrhs;
+      hipLaunchKernelGGL(k_farfield_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream, h->d_partial, nch,
+                         r.nmodes, r.nrhs, o.k, h->d_dF + r.dF_off, e > 0 ? 1 : 0);
+      LAUNCHCHK();
+    }
+  }
   if (!h->capturing) {
     HIPCHK(hipEventRecord(h->ev[5], h->stream));
     h->ev_recorded |= 4u;
@@ -2270,6 +3035,59 @@ extern "C" int ghost_bor_get_dF(ghost_bor* h, int prob, double* out) {
 // Copy one mode system out: A (n x n), B (n x nrhs), X (n x nrhs), complex interleaved, row-major.
 // which: 0 = A and B as assembled (call after fill, before solve), 1 = X (after solve).
 // Mode -m of a pair is read from mode m's storage: A = S A_m S, B and X = S [VV, -HH] of mode m's.
+// Far field of a region set: complex [2M+1][2][n_theta] as ghost_bor_get_dF.
+extern "C" int ghost_bor_region_get_dF(ghost_bor* h, int set, double* out) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (set < 0 || set >= (int)h->regions.size()) return fail(GHOST_ERR_ARG, "bad region set");
+  if (!out) return fail(GHOST_ERR_ARG, "null output");
+  const Region& r = h->regions[set];
+  std::vector<double2> buf((size_t)r.nmodes * r.nrhs);
+  HIPCHK(hipMemcpyAsync(buf.data(), h->d_dF + r.dF_off, buf.size() * sizeof(double2), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int mi = 0; mi < r.nmodes; ++mi)
+    for (int it = 0; it < r.ntheta; ++it)
+      for (int pol = 0; pol < 2; ++pol) {
+        const double2 v = buf[(size_t)mi * r.nrhs + 2 * it + pol];
+        const size_t o = (((size_t)mi * 2 + pol) * r.ntheta + it) * 2;
+        out[o] = v.x;
+        out[o + 1] = v.y;
+      }
+  return 0;
+}
+
+// One mode of a region set: size (n) and, which = 0, the reduced system A [n][n]
+// and right-hand sides B [n][nrhs] after fill; which = 1, the currents X
+// [n][nrhs] after solve (complex, interleaved).  Any output may be null.
+extern "C" int ghost_bor_region_system(ghost_bor* h, int set, int m, int which, int* n, double* out1,
+                                       double* out2) {
+  if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (set < 0 || set >= (int)h->regions.size()) return fail(GHOST_ERR_ARG, "bad region set");
+  const Region& r = h->regions[set];
+  if (m < -r.M || m > r.M) return fail(GHOST_ERR_ARG, "mode outside range");
+  const SysInfo& sy = h->sys[r.sys0 + m + r.M];
+  if (n) *n = sy.n;
+  if (!out1 && !out2) return 0;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double2> row((size_t)sy.ld);
+  for (int i = 0; i < sy.n; ++i) {
+    HIPCHK(hipMemcpy(row.data(), (double2*)h->d_A + sy.a_off + (size_t)i * sy.ld, (size_t)sy.ld * sizeof(double2),
+                     hipMemcpyDeviceToHost));
+    if (which == 0 && out1)
+      for (int c = 0; c < sy.n; ++c) {
+        out1[2 * ((size_t)i * sy.n + c)] = row[c].x;
+        out1[2 * ((size_t)i * sy.n + c) + 1] = row[c].y;
+      }
+    double* o = which == 0 ? out2 : out1;
+    if (o)
+      for (int c = 0; c < sy.nrhs; ++c) {
+        o[2 * ((size_t)i * sy.nrhs + c)] = row[sy.n + c].x;
+        o[2 * ((size_t)i * sy.nrhs + c) + 1] = row[sy.n + c].y;
+      }
+  }
+  return 0;
+}
+
 extern "C" int ghost_bor_get_system(ghost_bor* h, int prob, int m, int which, double* out1, double* out2) {
   if (!h || !h->built) return fail(GHOST_ERR_STATE, "build first");
   if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index");
@@ -2594,6 +3412,8 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   for (auto& p : h->probs)
     if (p.has_zs || p.ki != 0.0 || p.eta_re != 376.730313668 || p.eta_im != 0.0)
       return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies in free space only");
+  if (!h->regions.empty() || !h->cross.empty())
+    return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies in free space only (no multi-region sets)");
   if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
   if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
   HIPCHK(hipSetDevice(h->device));

@@ -399,18 +399,21 @@ __global__ void k_mirror_dF(double2* dF, int M, int nrhs) {
   dF[(size_t)(M - m) * nrhs + c] = dF[(size_t)(M + m) * nrhs + c];
 }
 
-// Sum the chunk partials in chunk order and apply -j k eta0 / (4 pi).
+// Sum the chunk partials in chunk order and apply -j k eta0 / (4 pi); acc:
+// add to dF (the next radiating surface of a multi-region body).
 __global__ void k_farfield_reduce(const double2* partial, int nchunk, int nsys, int nrhs, double k,
-                                  double2* dF) {
+                                  double2* dF, int acc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nsys * nrhs) return;
-  double2 acc = make_double2(0.0, 0.0);
+  double2 sum = make_double2(0.0, 0.0);
   for (int c = 0; c < nchunk; ++c) {
     const double2 v = partial[(size_t)c * nsys * nrhs + t];
-    acc.x += v.x; acc.y += v.y;
+    sum.x += v.x; sum.y += v.y;
   }
   const double pref = k * 376.730313668 / (4.0 * GH_PI);  // -j * pref
-  dF[t] = make_double2(pref * acc.y, -pref * acc.x);
+  double2 out = make_double2(pref * sum.y, -pref * sum.x);
+  if (acc) { out.x += dF[t].x; out.y += dF[t].y; }
+  dF[t] = out;
 }
 
 }  // namespace ghost

@@ -37,6 +37,8 @@ struct FarArgs {
   const double* cosB;    // [nB]
   const double* sinB;    // [nB]
   int Ne, Nn, M, nE, nB, pe, pf, nfp, npairs;
+  int Neq, Nnq;          // SPLIT: source elements and nodes (another generatrix for cross-surface operators)
+  const unsigned char* skip;  // SPLIT: [Ne][Neq] 1 = near pair (integrated elsewhere); nullptr: |e - f| <= 1
   int neg;               // also write modes -n (off with pairing: they are never stored)
   int delta, imax;       // unordered pairs (not SPLIT): f = pf + 2 j with j >= i + delta, rows i < imax
   int zmodes, zoff;      // Z mode stride and the index of mode 0 (multiples of 16; zoff = 0 with pairing)
@@ -44,7 +46,7 @@ struct FarArgs {
   double ki;             // CK: imaginary part of k (a lossy medium; k is the real part)
   const double2* zs;     // IBC: per-element surface impedance (source element's applies)
   double cz;             // IBC: bracket coefficient 2 pi c_ibc (times zs of the source element)
-  void* Z;               // Zfar [2Nn][2Nn][zmodes] complex<R>
+  void* Z;               // Zfar [2Nn][2Nnq][zmodes] complex<R>
   void* Z2;              // IBC: bracket part into its own far blocks (same layout), nullptr: summed into Z
 };
 
@@ -158,7 +160,8 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
   if (SPLIT) {   // ordered pairs (e, f) of the parity class
     e = a.pe + 2 * (wid / a.nfp);
     f = a.pf + 2 * (wid % a.nfp);
-    if (e >= a.Ne || f >= a.Ne || abs(e - f) <= 1) return;
+    if (e >= a.Ne || f >= a.Neq) return;
+    if (a.skip ? a.skip[(size_t)e * a.Neq + f] != 0 : abs(e - f) <= 1) return;
   } else {
     // unordered pairs e < f of the class, row i = (e - pe)/2 holding
     // j = (f - pf)/2 from i + delta to nfp - 1: S(i) pairs before row i
@@ -204,11 +207,11 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
     cmr = (R)(a.cz * z.x);
     cmi = (R)(a.cz * z.y);
   }
-  const size_t ncol = 2 * (size_t)a.Nn;
+  const size_t ncol = 2 * (size_t)a.Nnq;
   C2* Z = (C2*)a.Z;
   const int rowE = (uvsub ? a.Nn : 0) + te + asel;   // tt / ff, and tf / ft rows
-  const int colE = (uvsub ? a.Nn : 0) + se;          // tt / ff column (+ b)
-  const int colO = (uvsub ? 0 : a.Nn) + se;          // tf / ft column (+ b)
+  const int colE = (uvsub ? a.Nnq : 0) + se;         // tt / ff column (+ b)
+  const int colO = (uvsub ? 0 : a.Nnq) + se;         // tf / ft column (+ b)
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int n = 16 * t + c16;
@@ -554,7 +557,8 @@ __attribute__((amdgpu_waves_per_eu(far_wide(sizeof(R), NT) ? 3 : (sizeof(R) == 4
 // ─────────────────────────────────────────────────────────────────────────────
 
 struct NearPairInfo {
-  int e, f, kind;   // kind 0 diag, 1 corner10 (f = e+1), 2 corner01 (f = e-1)
+  int e, f, kind;   // cell table: same surface 0 diag, 1 corner10 (f = e+1), 2 corner01 (f = e-1);
+                    // two surfaces 0 tensor grid, 1 + 2a + b corner<a><b> (shared ends a, b)
   int pp_off;       // first point pair in the NP buffer
   int n_tail;       // Gauss order of the [xi0, pi] tail
   int tail_off;     // offset of that rule in the tail tables
@@ -569,10 +573,10 @@ struct NearArgs {
   const double* sL;       // adjoint's perturbed fills)
   const double* strho;
   const double* stz;
-  const double* cell_s;   // concatenated cell points of the 3 kinds
+  const double* cell_s;   // concatenated cell points of the kinds
   const double* cell_sp;
   const double* cell_w;
-  int cell_off[3], cell_cnt[3];
+  int cell_off[8], cell_cnt[8];
   const double* core_u;   // [48] Gauss nodes on [0,1]
   const double* core_w;
   const double* tail_u;   // concatenated tail rules on [0,1]
@@ -1205,6 +1209,175 @@ __global__ __launch_bounds__(256) void k_assemble_pen(AsmPenArgs a) {
       B[(size_t)(n1 + r) * s.n + n1 + c] = v11;
     }
   }
+}
+
+// ─────────────────────────────────────────────────────────────────────────────
+// multi-region systems (cross-surface operators, block sums, junction map)
+// ─────────────────────────────────────────────────────────────────────────────
+
+// Near pairs of a cross-surface operator added into its far blocks: one
+// work-group per pair of one parity class (e mod 2, f mod 2), whose 2 x 2
+// node footprints are disjoint, so the adds need no atomics and run in a
+// fixed order.  NB / NB2 [pair][16][nmodes] (EFIE / brackets); Z / Z2
+// [2Np][2Nq][zmodes], modes from mlo.
+__global__ __launch_bounds__(256) void k_cross_near_add(const double2* NB, const double2* NB2,
+                                                        const NearPairInfo* pairs, const int* plist, int M,
+                                                        int Np, int Nq, int zmodes, int zoff, int mlo,
+                                                        double2* Z, double2* Z2) {
+  const int pi = plist[blockIdx.x];
+  const NearPairInfo pr = pairs[pi];
+  const int nmodes = 2 * M + 1, nm = nmodes - mlo;
+  const size_t ncol = 2 * (size_t)Nq;
+  for (int t = threadIdx.x; t < 16 * nm; t += blockDim.x) {
+    const int q = t / nm, mi = mlo + t % nm;
+    const int blk = q >> 2, ai = (q >> 1) & 1, bi = q & 1;
+    const int row = ((blk >> 1) ? Np : 0) + pr.e + ai, col = ((blk & 1) ? Nq : 0) + pr.f + bi;
+    const size_t zi = ((size_t)row * ncol + col) * zmodes + zoff + (mi - M);
+    const size_t ni = ((size_t)pi * 16 + q) * nmodes + mi;
+    double2 v = Z[zi], w = NB[ni];
+    v.x += w.x; v.y += w.y;
+    Z[zi] = v;
+    if (Z2) {
+      v = Z2[zi]; w = NB2[ni];
+      v.x += w.x; v.y += w.y;
+      Z2[zi] = v;
+    }
+  }
+}
+
+// One operator block added into the full (node-based, unreduced) system of
+// every mode: Afull[m][row_off + i][col_off + j] += coef x op(i, j, m), op
+// = T, or P = B with columns remapped (column j < Nq is -(column j + Nq),
+// column j >= Nq is column j - Nq; bor_solver.py:554-575).  The operator's
+// far blocks hold modes 0..M only when the handle pairs modes: mode -m is
+// then S op_m S (S = +1 on J_t rows / columns, -1 on J_phi), as its near
+// blocks (stored for every mode) already are.  Same-surface operators add
+// their near band from NB (|e - f| <= 1 slots); cross-surface ones have it
+// in their far blocks.  Modes sit on the LDS tile's fast axis for the far
+// block reads, the columns for the writes.
+struct RegionBlockArgs {
+  const double2* Z;          // far blocks [2Np][2Nq][zmodes]
+  const double2* NB;         // near band [3 Ne][16][nmodes] of a same-surface operator, or nullptr
+  int Np, Nq, Ne, M, zmodes, zoff, paired, remap;
+  double cre, cim;
+  int row_off, col_off, n_full;
+  double2* A;                // [nmodes][n_full][n_full]
+};
+
+__global__ __launch_bounds__(256) void k_region_block(RegionBlockArgs a) {
+  __shared__ double2 tile[32][33];
+  const int i = blockIdx.x;
+  const int j0 = blockIdx.y * 32, m0 = blockIdx.z * 32;
+  const int nmodes = 2 * a.M + 1, ncol = 2 * a.Nq;
+  const double si = i < a.Np ? 1.0 : -1.0;
+  const int bi = i < a.Np ? 0 : 1, ii = i - bi * a.Np;
+  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
+    const int jj = t >> 5, mm = t & 31;
+    const int j = j0 + jj, mi = m0 + mm;
+    double2 v = make_double2(0.0, 0.0);
+    if (j < ncol && mi < nmodes) {
+      const int m = mi - a.M;
+      const int jc = a.remap ? (j < a.Nq ? j + a.Nq : j - a.Nq) : j;
+      const int zi = a.paired ? abs(m) : a.zoff + m;
+      v = a.Z[((size_t)i * ncol + jc) * a.zmodes + zi];
+      if (a.paired && m < 0) {
+        const double sg = si * (jc < a.Nq ? 1.0 : -1.0);
+        v.x *= sg; v.y *= sg;
+      }
+      if (a.NB) {
+        const int bj = jc < a.Nq ? 0 : 1;
+        const double2 w = near_entry(a.NB, ii, jc - bj * a.Nq, 2 * bi + bj, mi, a.Ne, nmodes);
+        v.x += w.x; v.y += w.y;
+      }
+      if (a.remap && j < a.Nq) { v.x = -v.x; v.y = -v.y; }
+    }
+    tile[jj][mm] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
+    const int mm = t >> 5, jj = t & 31;
+    const int j = j0 + jj, mi = m0 + mm;
+    if (j >= ncol || mi >= nmodes) continue;
+    const double2 v = tile[jj][mm];
+    double2* o = a.A + ((size_t)mi * a.n_full + a.row_off + i) * a.n_full + a.col_off + j;
+    double2 r = *o;
+    r.x += a.cre * v.x - a.cim * v.y;
+    r.y += a.cre * v.y + a.cim * v.x;
+    *o = r;
+  }
+}
+
+// Junction map of one mode, as lists: reduced index r gathers the full
+// indices mem[ptr[r] .. ptr[r+1]) with coefficients cf (x_full = Q x_red,
+// each full index in at most one list).
+struct RegionMap {
+  const int* ptr;            // [nmodes][n_full + 1] (only n_red + 1 used per mode)
+  const int* mem;            // [nmodes][n_full]
+  const double* cf;          // [nmodes][n_full]
+  int n_full;
+};
+
+// A_red = Q^T A_full Q into the mode's augmented system (and its FP64 copy).
+__global__ __launch_bounds__(256) void k_region_reduce(RegionMap q, const double2* Afull, const SysInfo* sys,
+                                                       double2* A, double2* A64) {
+  const int mi = blockIdx.z;
+  const SysInfo s = sys[mi];
+  const int r = blockIdx.x;
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  if (r >= s.n || c >= s.n) return;
+  const int* ptr = q.ptr + (size_t)mi * (q.n_full + 1);
+  const int* mem = q.mem + (size_t)mi * q.n_full;
+  const double* cf = q.cf + (size_t)mi * q.n_full;
+  const double2* F = Afull + (size_t)mi * q.n_full * q.n_full;
+  double2 v = make_double2(0.0, 0.0);
+  for (int u = ptr[r]; u < ptr[r + 1]; ++u)
+    for (int w = ptr[c]; w < ptr[c + 1]; ++w) {
+      const double2 x = F[(size_t)mem[u] * q.n_full + mem[w]];
+      const double g = cf[u] * cf[w];
+      v.x += g * x.x;
+      v.y += g * x.y;
+    }
+  A[s.a_off + (size_t)r * s.ld + c] = v;
+  if (A64) A64[s.a64_off + (size_t)r * s.n + c] = v;
+}
+
+// B_red = Q^T B_full into the augmented columns (and the FP64 copy).
+__global__ __launch_bounds__(256) void k_region_rhs(RegionMap q, const double2* Bfull, const SysInfo* sys,
+                                                    double2* A, double2* B64) {
+  const int mi = blockIdx.z;
+  const SysInfo s = sys[mi];
+  const int r = blockIdx.x;
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  if (r >= s.n || c >= s.nrhs) return;
+  const int* ptr = q.ptr + (size_t)mi * (q.n_full + 1);
+  const int* mem = q.mem + (size_t)mi * q.n_full;
+  const double* cf = q.cf + (size_t)mi * q.n_full;
+  const double2* F = Bfull + (size_t)mi * q.n_full * s.nrhs;
+  double2 v = make_double2(0.0, 0.0);
+  for (int u = ptr[r]; u < ptr[r + 1]; ++u) {
+    const double2 x = F[(size_t)mem[u] * s.nrhs + c];
+    v.x += cf[u] * x.x;
+    v.y += cf[u] * x.y;
+  }
+  A[s.a_off + (size_t)r * s.ld + s.n + s.col + c] = v;
+  if (B64) B64[s.x_off + (size_t)r * s.ncols + s.col + c] = v;
+}
+
+// x_full = Q x_red from the augmented columns, every full index (0 where
+// the map drops it), for the far field.
+__global__ __launch_bounds__(256) void k_region_expand(RegionMap q, const double2* A, const SysInfo* sys,
+                                                       double2* Xfull) {
+  const int mi = blockIdx.z;
+  const SysInfo s = sys[mi];
+  const int r = blockIdx.x;
+  const int c = blockIdx.y * blockDim.x + threadIdx.x;
+  if (r >= s.n || c >= s.nrhs) return;
+  const int* ptr = q.ptr + (size_t)mi * (q.n_full + 1);
+  const int* mem = q.mem + (size_t)mi * q.n_full;
+  const double* cf = q.cf + (size_t)mi * q.n_full;
+  const double2 x = A[s.a_off + (size_t)r * s.ld + s.n + s.col + c];
+  for (int u = ptr[r]; u < ptr[r + 1]; ++u)
+    Xfull[((size_t)mi * q.n_full + mem[u]) * s.nrhs + c] = make_double2(cf[u] * x.x, cf[u] * x.y);
 }
 
 // ─────────────────────────────────────────────────────────────────────────────

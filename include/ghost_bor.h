@@ -147,6 +147,49 @@ int ghost_bor_set_medium(ghost_bor* h, int prob, double k_im, double eta_re, dou
 int ghost_bor_set_interior(ghost_bor* h, int prob, double k_re, double k_im, double eta_re, double eta_im,
                            int n_xi_efie, int n_xi_bracket);
 
+/* ── Multi-region bodies (PEC cores under coating layers, coating patches;
+ * bor_solver.py:1100-1297, 1805-2172).  FP64 handles only.  A body is
+ * described by operators, cross operators and one region set:
+ *
+ * ghost_bor_add_operator: one generatrix (nodes [Nn][2], +z end first) in
+ * one homogeneous medium (k = k_re + j k_im, k_im <= 0; eta), filled as
+ * operators only: its EFIE operator T and, brackets != 0, the unit-weight
+ * rotated-PV brackets of its coupling operator P (bor_solver.py:554-575,
+ * BorPecSolver.assemble_mode / assemble_pmchwt_P of that medium).  thetas:
+ * the aspects, for a free-space surface that bounds the exterior (n_theta may
+ * be 0 otherwise). */
+int ghost_bor_add_operator(ghost_bor* h, const double* nodes, int Nn, double k_re, double k_im, double eta_re,
+                           double eta_im, int m_max, int n_xi_efie, int n_xi_bracket, int brackets,
+                           const double* thetas_deg, int n_theta, int* op);
+/* The cross-surface T and P between the test bases of op_test and the
+ * source bases of op_src (same medium, same m_max; BorCrossOperators,
+ * bor_solver.py:1115-1297): element pairs sharing an endpoint on corner
+ * cells, pairs closer than near_factor x the longer element on a
+ * near_order^2 tensor grid (the reference's defaults 2.0 and 12). */
+int ghost_bor_add_cross(ghost_bor* h, int op_test, int op_src, double near_factor, int near_order, int* cross);
+/* A region set: per mode -m_max..m_max a system over n_full unknowns before
+ * the junction map, at aspects thetas_deg. */
+int ghost_bor_add_region(ghost_bor* h, int m_max, int n_full, const double* thetas_deg, int n_theta, int* set);
+/* Add (c_re + j c_im) x block at (row_off, col_off) of every mode's full
+ * system: src 0 operator idx, 1 cross operator idx; which 0 T, 1 P
+ * (_MultiRegionBor.assemble, bor_solver.py:1988-2015). */
+int ghost_bor_region_block(ghost_bor* h, int set, int src, int idx, int which, double c_re, double c_im,
+                           int row_off, int col_off);
+/* Junction map of mode m (_MultiRegionBor.build_Q, bor_solver.py:1908-1986):
+ * full index i -> reduced index red_of_full[i] (-1: dropped) with real
+ * coefficient coef[i]; the mode's system is Q^T A Q (n_red <= 4096). */
+int ghost_bor_region_map(ghost_bor* h, int set, int m, int n_red, const int* red_of_full, const double* coef);
+/* A free-space operator surface bounding the exterior: its J rows start at
+ * off_J of the full system and, off_M >= 0, its M / eta0 rows follow at
+ * off_M = off_J + 2 Nn.  Its rows take <W, E_inc> (and eta0 <W, H_inc>) and
+ * its currents radiate (bor_solver.py:2017-2037). */
+int ghost_bor_region_exterior(ghost_bor* h, int set, int op, int off_J, int off_M);
+/* Far field of a region set, complex [2m_max+1][2][n_theta] (as get_dF). */
+int ghost_bor_region_get_dF(ghost_bor* h, int set, double* out);
+/* One mode of a region set: *n its size; which 0: A [n][n] and B [n][nrhs]
+ * after fill; which 1: X [n][nrhs] after solve (complex interleaved). */
+int ghost_bor_region_system(ghost_bor* h, int set, int m, int which, int* n, double* out1, double* out2);
+
 /* Geometry tables, device buffers and the launch schedule. */
 int ghost_bor_build(ghost_bor* h);
 
