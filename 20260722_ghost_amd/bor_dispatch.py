@@ -6,10 +6,13 @@ and result format (sample rows and metadata), for conductor bodies (every
 segment TYPE 2, PEC or with impedance flags: IBC-EFIE, tapered and tabulated
 Z_s through ``materials.MaterialLibrary``) and homogeneous penetrable bodies
 (every segment TYPE 3 with one pos_mat dielectric flag: PMCHWT, meshed at the
-interior wavelength).  All frequencies of a request go to the GPU as one
-batch (``bor_solver.solve_bor_batch``).  The multi-region layouts (coated
-TYPE 3 + 4, partially coated TYPE 2 + 3 + 4, layered TYPE 3 + 4 + 5) need
-cross-surface operators and raise NotImplementedError.
+interior wavelength) and multi-region bodies (a PEC core fully or partly
+under a coating, TYPE 3 + 4 and 2 + 3 + 4; two layers or a patch over a
+coating, TYPE 3 + 4 + 5; a stack of layers, several TYPE 5 pairs: the region
+PMCHWT of ``bor_solver.MultiRegionBor``).  All frequencies of a conductor or
+homogeneous request go to the GPU as one batch
+(``bor_solver.solve_bor_batch``); multi-region bodies are solved one
+frequency at a time.
 
 Geometry path: the snapshot's segments become polylines (``Piece``) in
 metres; the conductor's pieces must join end-to-start as drawn into one run
@@ -242,23 +245,165 @@ def mesh_run(run: Sequence[Piece], wavelength: float, max_elements: int, axis_to
     return pts, np.array(owner, dtype=int), np.array(arc, dtype=float)
 
 
+MULTI_REGION = ("coated", "partial", "layered", "layered_n")
+
+
 def layout(pieces: Sequence[Piece]) -> str:
-    """'conductor' (all TYPE 2), 'dielectric' (all TYPE 3 on one pos_mat
-    flag) or an error (bor_dispatch.py:254-298): other TYPEs are not BoR
-    surfaces, and the multi-region layouts are not part of this build."""
+    """Which body the segment TYPEs describe (bor_dispatch.py:254-298):
+    'conductor' (all TYPE 2), 'dielectric' (all TYPE 3 on one pos_mat flag),
+    'coated' (a TYPE 3 interface over a TYPE 4 core, one coating flag),
+    'partial' (the same with bare TYPE 2 conductor pieces), 'layered' (two
+    layers: TYPE 3 outer interface, one TYPE 5 interface pair, TYPE 4 core;
+    TYPE 3 pieces on the inner flag are an exposed inner interface under a
+    patch) or 'layered_n' (several TYPE 5 pairs: a full stack)."""
     kinds = {pc.kind for pc in pieces}
-    if kinds == {2}:
-        return "conductor"
-    if kinds == {3}:
-        if len({pc.pos for pc in pieces}) != 1:
-            raise ValueError("a homogeneous penetrable body needs the same pos_mat flag on every TYPE 3 segment.")
-        return "dielectric"
     odd = sorted(kinds - {2, 3, 4, 5})
     if odd:
         raise ValueError(f"segment TYPE {odd} cannot be solved as a body of revolution (TYPE 2 conductor "
                          "or impedance, 3 dielectric interface, 4 coated conductor, 5 layer interface).")
-    raise NotImplementedError("multi-region BoR layouts (coated, partially coated, layered: TYPE 3 + 4 / "
-                              "2 + 3 + 4 / 3 + 4 + 5) are not part of the GPU BoR path.")
+    flags = lambda t: {pc.pos for pc in pieces if pc.kind == t}
+    if kinds == {2}:
+        return "conductor"
+    if kinds == {3}:
+        if len(flags(3)) != 1:
+            raise ValueError("a homogeneous penetrable body needs the same pos_mat flag on every TYPE 3 segment.")
+        return "dielectric"
+    if kinds in ({3, 4}, {2, 3, 4}):
+        if len(flags(3)) != 1 or flags(3) != flags(4):
+            raise ValueError("a coated conductor has one coating: every TYPE 3 interface segment and every "
+                             "TYPE 4 covered-conductor segment must carry the same pos_mat flag.")
+        return "partial" if 2 in kinds else "coated"
+    if kinds == {3, 4, 5}:
+        pairs = {(pc.pos, pc.neg) for pc in pieces if pc.kind == 5}
+        if len(pairs) > 1:
+            return "layered_n"
+        outer, inner = next(iter(pairs))
+        if outer <= 0 or inner <= 0 or outer == inner:
+            raise ValueError("a TYPE 5 layer interface needs two different positive flags: pos_mat the outer "
+                             "layer, neg_mat the inner one.")
+        if flags(4) != {inner}:
+            raise ValueError("the TYPE 4 core of a two-layer coating must carry the inner layer's flag "
+                             "(the TYPE 5 interface's neg_mat).")
+        if not flags(3) <= {outer, inner}:
+            raise ValueError("every TYPE 3 segment of a two-layer coating must carry the outer or the inner "
+                             "layer's flag.")
+        return "layered"
+    raise ValueError("TYPE 2 conductors combine with penetrable layers only as a partial coating "
+                     "(TYPE 2 + 3 + 4).")
+
+
+@dataclass
+class RegionPlan:
+    """The runs of a multi-region body and how each is meshed: ``runs``
+    maps a role to its runs, ``flags`` the layer flags outermost first."""
+
+    kind: str
+    runs: Dict[str, List[List[Piece]]]
+    flags: List[int]
+
+
+def region_plan(kind: str, pieces: Sequence[Piece], tol: float, axis_tol: float) -> RegionPlan:
+    """Group a multi-region body's pieces into runs by role and check that
+    the closed surfaces close (bor_dispatch.py:400-487)."""
+    of = lambda pred: [pc for pc in pieces if pred(pc)]
+    closed = lambda run, what: (check_generatrix(run, what, axis_tol), run)[1]
+    if kind in ("coated", "partial"):
+        flag = pieces[[pc.kind for pc in pieces].index(3)].pos
+        iface = single_run(of(lambda p: p.kind == 3), "coating interface (TYPE 3)", tol)
+        covered = single_run(of(lambda p: p.kind == 4), "covered conductor (TYPE 4)", tol)
+        if kind == "coated":
+            return RegionPlan(kind, {"interface": [closed(iface, "coating interface")],
+                                     "core": [closed(covered, "core")]}, [flag])
+        bare_pcs = of(lambda p: p.kind == 2)
+        bare = order_runs(bare_pcs, tol, "bare conductor (TYPE 2)")
+        # the conductor as a whole (covered and bare pieces) is one closed body
+        closed(single_run(of(lambda p: p.kind == 4) + bare_pcs, "conductor (TYPE 2 + 4)", tol), "conductor")
+        return RegionPlan(kind, {"interface": [iface], "core": [covered], "bare": bare}, [flag])
+    if kind == "layered":
+        outer, inner = next(iter({(pc.pos, pc.neg) for pc in pieces if pc.kind == 5}))
+        mid = single_run(of(lambda p: p.kind == 5), "layer interface (TYPE 5)", tol)
+        core = closed(single_run(of(lambda p: p.kind == 4), "core (TYPE 4)", tol), "core")
+        patch = single_run(of(lambda p: p.kind == 3 and p.pos == outer), "outer interface (TYPE 3)", tol)
+        exposed_pcs = of(lambda p: p.kind == 3 and p.pos == inner)
+        exposed = order_runs(exposed_pcs, tol, "exposed inner interface (TYPE 3)") if exposed_pcs else []
+        closed(single_run(of(lambda p: p.kind == 5) + exposed_pcs, "inner layer interface (TYPE 5 + 3)", tol),
+               "inner layer interface")
+        if not exposed:
+            closed(patch, "outer interface")
+        return RegionPlan(kind, {"patch": [patch], "mid": [mid], "exposed": exposed, "core": [core]},
+                          [outer, inner])
+    # layered_n: walk the TYPE 5 pairs from the outer interface's flag to the core's
+    top = {pc.pos for pc in pieces if pc.kind == 3}
+    if len(top) != 1:
+        raise ValueError("a stack of more than two layers covers the whole core: every TYPE 3 segment must "
+                         "carry the outermost layer's flag.")
+    bottom = {pc.pos for pc in pieces if pc.kind == 4}
+    if len(bottom) != 1:
+        raise ValueError("the TYPE 4 core segments must all carry one flag (the innermost layer's).")
+    below: Dict[int, int] = {}
+    for po, ne in {(pc.pos, pc.neg) for pc in pieces if pc.kind == 5}:
+        if po in below:
+            raise ValueError(f"two TYPE 5 interfaces have the same outer-layer flag {po}.")
+        below[po] = ne
+    order = [next(iter(top))]
+    target = next(iter(bottom))
+    while order[-1] != target:
+        if order[-1] not in below or len(order) > len(pieces):
+            raise ValueError(f"no TYPE 5 interface leads from layer flag {order[-1]} toward the core's flag "
+                             f"{target}.")
+        order.append(below.pop(order[-1]))
+    if below:
+        raise ValueError(f"TYPE 5 interfaces on layer flags {sorted(below)} are not between the outer "
+                         "interface and the core.")
+    ifaces = [closed(single_run(of(lambda p: p.kind == 3), "outer interface (TYPE 3)", tol), "outer interface")]
+    for po, ne in zip(order[:-1], order[1:]):
+        ifaces.append(closed(single_run(of(lambda p, a=po, b=ne: p.kind == 5 and (p.pos, p.neg) == (a, b)),
+                                        f"layer interface {po}|{ne} (TYPE 5)", tol), f"layer interface {po}|{ne}"))
+    core = closed(single_run(of(lambda p: p.kind == 4), "core (TYPE 4)", tol), "core")
+    return RegionPlan(kind, {"interfaces": ifaces, "core": [core]}, order)
+
+
+def solve_region_plan(plan: RegionPlan, materials: MaterialLibrary, fg: float, wavelength: float,
+                      aspects: Sequence[float], n_modes, mode_tol: float, max_elements: int, axis_tol: float,
+                      check_abort: Callable) -> Tuple[Dict[str, Any], str]:
+    """Mesh a multi-region body at one frequency (the layers at the shortest
+    interior wavelength, bare conductor pieces at the free-space one) and
+    solve it on the GPU (bor_dispatch.py:565-640)."""
+    media = [materials.get_medium(fl, fg) for fl in plan.flags]
+    index = max([1.0] + [abs(bor_solver.causal_medium(e, u)[0]) for (e, u) in media])
+    mesh = lambda run, lam=wavelength / index: mesh_run(run, lam, max_elements, axis_tol)[0]
+    kw = {"n_modes": n_modes, "mode_tol": mode_tol, "check_abort": check_abort}
+    f_hz = fg * 1e9
+    R = plan.runs
+    if plan.kind == "coated":
+        (eps, mu), = media
+        out = bor_solver.solve_bor_coated_pec(mesh(R["interface"][0]), mesh(R["core"][0]), f_hz, aspects, eps,
+                                              mu, **kw)
+        return out, "BoR-MoM PMCHWT coated PEC (multi-region)"
+    if plan.kind == "partial":
+        (eps, mu), = media
+        bare = [mesh(run, wavelength) for run in R["bare"]]
+        for run in R["bare"]:
+            if any(pc.ibc > 0 for pc in run):
+                raise NotImplementedError("impedance flags on the bare conductor of a partial coating are not "
+                                          "part of the GPU BoR path.")
+        out = bor_solver.solve_bor_partial_coating(mesh(R["interface"][0]), mesh(R["core"][0]), bare, f_hz,
+                                                   aspects, eps, mu, **kw)
+        return out, f"BoR-MoM PMCHWT partial coating ({out['n_junctions']} junction(s))"
+    if plan.kind == "layered":
+        (eps_o, mu_o), (eps_i, mu_i) = media
+        patch, mid, core = mesh(R["patch"][0]), mesh(R["mid"][0]), mesh(R["core"][0])
+        if R["exposed"]:
+            exposed = [mesh(run) for run in R["exposed"]]
+            out = bor_solver.solve_bor_coating_patch(patch, mid, exposed, core, f_hz, aspects, eps_i, mu_i,
+                                                     eps_o, mu_o, **kw)
+            return out, f"BoR-MoM PMCHWT coating patch ({out['n_junctions']} junction(s))"
+        out = bor_solver.solve_bor_coated2_pec(patch, mid, core, f_hz, aspects, eps_i, mu_i, eps_o, mu_o, **kw)
+        return out, "BoR-MoM PMCHWT two-layer coated PEC"
+    ifaces = [mesh(run) for run in R["interfaces"]]
+    out = bor_solver.solve_bor_coated_n_pec(ifaces, mesh(R["core"][0]), f_hz, aspects,
+                                            [e for (e, _) in media[::-1]], [u for (_, u) in media[::-1]], **kw)
+    return out, f"BoR-MoM PMCHWT {len(ifaces)}-layer coated PEC"
 
 
 def _sample_rows(fg: float, aspects: Sequence[float], out: Dict[str, Any], pol: str) -> List[Dict[str, Any]]:
@@ -323,14 +468,21 @@ def solve_monostatic_rcs_bor(
     kind = layout(pieces)
     span = float(np.ptp(np.vstack([pc.pts for pc in pieces]), axis=0).max())
     diag = max(span, 1e-9)
-    run = single_run(pieces, "TYPE 2" if kind == "conductor" else "TYPE 3", max(1e-12, 1e-9 * diag))
+    tol = max(1e-12, 1e-9 * diag)
     axis_tol = 1e-6 * diag
-    check_generatrix(run, "body", axis_tol)
 
     def check_abort():
         if abort_event is not None and abort_event.is_set():
             raise InterruptedError("Solve cancelled by user.")
 
+    if kind in MULTI_REGION:
+        plan = region_plan(kind, pieces, tol, axis_tol)
+        check_abort()
+        return _solve_multi_region(plan, materials, freqs, aspects, pol, mesh_reference_ghz, n_modes, mode_tol,
+                                   max_elements, axis_tol, check_abort, progress_callback, cfie_alpha, workers,
+                                   expand_to_360)
+    run = single_run(pieces, "TYPE 2" if kind == "conductor" else "TYPE 3", tol)
+    check_generatrix(run, "body", axis_tol)
     check_abort()
     jobs, forms = [], []
     # a penetrable body runs in FP64 (complex interior wavenumber)
@@ -371,16 +523,50 @@ def solve_monostatic_rcs_bor(
                 progress_callback(fi + 1, len(freqs), f"Solved {fg:g} GHz")
             except Exception:
                 pass
-    if expand_to_360:
-        mirror = [dict(s, theta_inc_deg=360.0 - s["theta_inc_deg"], theta_scat_deg=360.0 - s["theta_inc_deg"])
-                  for s in samples if 0.0 < s["theta_inc_deg"] < 180.0]
-        samples = sorted(samples + mirror, key=lambda s: (s["frequency_ghz"], s["theta_inc_deg"]))
     any_zs = any(h for _, h in forms)
     if kind == "dielectric":
         label = "BoR-MoM PMCHWT (homogeneous dielectric)"
     else:
         label = ("BoR-MoM IBC-EFIE (Leontovich)" if any_zs
                  else f"BoR-MoM PEC {forms[0][0].upper()}")
+    return _result(samples, per_freq, pol, label, kind, expand_to_360, cfie_alpha, workers, materials, prec)
+
+
+def _mirror_360(samples: List[Dict[str, Any]]) -> List[Dict[str, Any]]:
+    """Add the aspects 360 - theta (exact for a body of revolution)."""
+    mirror = [dict(s, theta_inc_deg=360.0 - s["theta_inc_deg"], theta_scat_deg=360.0 - s["theta_inc_deg"])
+              for s in samples if 0.0 < s["theta_inc_deg"] < 180.0]
+    return sorted(samples + mirror, key=lambda s: (s["frequency_ghz"], s["theta_inc_deg"]))
+
+
+def _solve_multi_region(plan, materials, freqs, aspects, pol, mesh_reference_ghz, n_modes, mode_tol, max_elements,
+                        axis_tol, check_abort, progress_callback, cfie_alpha, workers, expand_to_360):
+    """One GPU region solve per frequency (FP64: the layers' wavenumbers are complex)."""
+    samples: List[Dict[str, Any]] = []
+    per_freq: List[Dict[str, Any]] = []
+    label = ""
+    for fi, fg in enumerate(freqs):
+        check_abort()
+        wavelength = C0 / ((float(mesh_reference_ghz) if mesh_reference_ghz else fg) * 1e9)
+        out, label = solve_region_plan(plan, materials, fg, wavelength, aspects, n_modes, mode_tol, max_elements,
+                                       axis_tol, check_abort)
+        for w in out.get("warnings") or []:
+            materials.warn_once(str(w))
+        samples += _sample_rows(fg, aspects, out, pol)
+        per_freq.append({"frequency_ghz": fg, "modes_used": int(out["modes_used"]),
+                         "n_unknowns": int(out["n_unknowns"]),
+                         "linear_residual": float(out.get("linear_residual", 0.0)), "max_cond": None})
+        if progress_callback is not None:
+            try:
+                progress_callback(fi + 1, len(freqs), f"Solved {fg:g} GHz")
+            except Exception:
+                pass
+    return _result(samples, per_freq, pol, label, plan.kind, expand_to_360, cfie_alpha, workers, materials, "fp64")
+
+
+def _result(samples, per_freq, pol, label, kind, expand_to_360, cfie_alpha, workers, materials, prec):
+    if expand_to_360:
+        samples = _mirror_360(samples)
     return {
         "solver": "bor_mom_rcs",
         "scattering_mode": "monostatic",

@@ -698,6 +698,46 @@ def solve_bor_coated_n_pec(interface_points, points_core, freq_hz: float, thetas
                         {"eps_layers": [complex(e) for e in eps_list]})
 
 
+def partial_coating_layout(points_interface, points_covered, bare_pieces, eps_r, mu_r):
+    """Surfaces and regions of a PEC body partly under one coating: the
+    interface and the bare conductor pieces bound the exterior, the interface
+    (reversed) and the covered core bound the coating; every off-axis end is
+    a junction of the interface, the covered core and one bare piece
+    (bor_solver.py:1402-1450)."""
+    bare = list(bare_pieces)
+    surfaces = [(points_interface, False), (points_covered, True)] + [(p, True) for p in bare]
+    regions = [{"medium": None, "exterior": True, "bounds": [(0, +1)] + [(2 + i, +1) for i in range(len(bare))]},
+               {"medium": (eps_r, mu_r), "bounds": [(0, -1), (1, +1)]}]
+    return surfaces, regions
+
+
+def solve_bor_partial_coating(points_interface, points_covered, bare_pieces, freq_hz: float, thetas_deg,
+                              eps_r: complex, mu_r: complex = 1.0, bare_zs=None, n_modes: Optional[int] = None,
+                              gauss_order: int = 4, mode_tol: float = 1e-6, near_factor: float = 2.0,
+                              near_order: int = 12, workers: int = 1, progress: Optional[Callable] = None,
+                              check_abort: Optional[Callable] = None) -> Dict:
+    """PEC body partly covered by a homogeneous coating that ends on the
+    conductor at junction circles (cap or band coatings;
+    bor_solver.py:1402-1803), solved as the region system of
+    ``partial_coating_layout``: J and M / eta0 on the interface, J on the
+    covered core and on each bare piece, the three chain ends of a junction
+    tied to the interface's, M_t = 0 there.  Bare pieces are perfect
+    conductors here: a nonzero ``bare_zs`` raises NotImplementedError."""
+    bare = list(bare_pieces)
+    if bare_zs is not None:
+        if len(bare_zs) != len(bare):
+            raise ValueError("bare_zs needs one entry (None, a scalar or per-element values) per bare piece.")
+        if any(z is not None and np.any(np.abs(np.asarray(z, dtype=complex)) > 0.0) for z in bare_zs):
+            raise NotImplementedError("surface impedance on the bare pieces of a partial coating is not part of "
+                                      "the GPU BoR path; the bare pieces are solved as perfect conductors.")
+    surfaces, regions = partial_coating_layout(points_interface, points_covered, bare, eps_r, mu_r)
+    out = _multiregion(surfaces, regions, freq_hz, thetas_deg, n_modes, mode_tol, gauss_order, near_factor,
+                       near_order, progress, check_abort, "pmchwt-partial-coating",
+                       {"eps_r": complex(eps_r), "mu_r": complex(mu_r)})
+    out.setdefault("warnings", [])
+    return out
+
+
 def solve_bor_coating_patch(points_patch, points_mid_covered, points_mid_bare, points_core, freq_hz: float,
                             thetas_deg, eps_inner: complex, mu_inner: complex, eps_patch: complex,
                             mu_patch: complex, n_modes: Optional[int] = None, gauss_order: int = 4,

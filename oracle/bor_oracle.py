@@ -28,7 +28,9 @@ It restates, in NumPy/SciPy, the discretization of the reference solver:
 * f3  PMCHWT, homogeneous body ....... bor_solver.py:161-230, 554-575, 647-657,
                                        1019-1094
 * f3  PMCHWT, multi-region (cross-surface operators, region system, junction
-      reduction) ..................... bor_solver.py:1100-1297, 1805-2131
+      reduction; coated, layered, partial-coating and coating-patch
+      layouts) ....................... bor_solver.py:1100-1297, 1402-1450,
+                                       1805-2176
 
 Pinning (what checks each function, besides golden vectors written from the
 reference itself under tests/golden/):
@@ -44,7 +46,8 @@ reference itself under tests/golden/):
       Weston's null (Z_s = eta0) end to end (oracle/mie.py).
 * f3  golden vectors (every mode's operators, systems, currents and far
       field for a lossy dielectric and a lossy magnetic body, a coated PEC
-      sphere with tensor-grid near pairs, two layers); the dielectric and
+      sphere with tensor-grid near pairs, two layers; the reference's
+      partial-coating (cap and band) and coating-patch far fields); the dielectric and
       layered Mie series end to end (oracle/mie.py, whose layered series is
       itself checked by the free-space-layer and split-layer identities).
 """
@@ -1453,4 +1456,29 @@ def coated_layers(interface_points, core_points, eps_list, mu_list):
     for i in range(N):
         lay = N - i
         regions.append({"medium": (eps_list[lay - 1], mu_list[lay - 1]), "bounds": [(i, -1), (i + 1, 1)]})
+    return surfaces, regions
+
+
+def partial_coating_layout(interface_points, covered_points, bare_pieces, eps_r, mu_r):
+    """Surfaces and regions of a PEC body partly under one coating: air
+    bounded by the interface and the bare conductor pieces, the coating by the
+    interface (reversed) and the covered core; each off-axis end is a
+    junction of the three chains (bor_solver.py:1402-1450).  The region
+    system with this layout gives the reference's partial-coating far field."""
+    surfaces = [(interface_points, False), (covered_points, True)] + [(p, True) for p in bare_pieces]
+    regions = [{"medium": None, "exterior": True, "bounds": [(0, 1)] + [(2 + i, 1) for i in range(len(bare_pieces))]},
+               {"medium": (eps_r, mu_r), "bounds": [(0, -1), (1, 1)]}]
+    return surfaces, regions
+
+
+def coating_patch_layout(patch_points, mid_covered_points, mid_bare_points, core_points, eps_inner, mu_inner,
+                         eps_patch, mu_patch):
+    """A second layer (the patch) over part of a fully coated PEC core; the
+    inner layer's interface is split into the covered part and the exposed
+    part, which meet the patch at dielectric triple junctions
+    (bor_solver.py:2134-2176)."""
+    surfaces = [(patch_points, False), (mid_covered_points, False), (mid_bare_points, False), (core_points, True)]
+    regions = [{"medium": None, "exterior": True, "bounds": [(0, 1), (2, 1)]},
+               {"medium": (eps_patch, mu_patch), "bounds": [(0, -1), (1, 1)]},
+               {"medium": (eps_inner, mu_inner), "bounds": [(1, -1), (2, -1), (3, 1)]}]
     return surfaces, regions

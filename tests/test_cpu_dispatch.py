@@ -50,8 +50,10 @@ def test_validation_errors():
     with pytest.raises(ValueError, match="pos_mat"):
         bd.layout(bd.read_pieces(two, 1.0))
     coated = {"segments": [dict(s, properties=[3 + i, 2, 0, 1, 0]) for i, s in enumerate(snap["segments"])]}
-    with pytest.raises(NotImplementedError):
-        bd.layout(bd.read_pieces(coated, 1.0))
+    assert bd.layout(bd.read_pieces(coated, 1.0)) == "coated"
+    bare = {"segments": [dict(s, properties=[2 + i, 2, 0, 1, 0]) for i, s in enumerate(snap["segments"])]}
+    with pytest.raises(ValueError, match="partial coating"):   # TYPE 2 + 3 with no covered core
+        bd.layout(bd.read_pieces(bare, 1.0))
     with pytest.raises(ValueError, match="TYPE"):
         bd.layout(bd.read_pieces({"segments": [dict(s, properties=[7, 2, 0, 0, 0]) for s in snap["segments"]]},
                                  1.0))
@@ -102,3 +104,62 @@ def test_penetrable_job_validation():
     m, er = bs.causal_medium(3.0 - 0.4j, 1.3 - 0.05j)
     assert m.imag <= 0.0 and m.real > 0.0 and abs(m * m - (3.0 - 0.4j) * (1.3 - 0.05j)) < 1e-14
     assert abs(er - (1.3 - 0.05j) / m) < 1e-15
+
+
+def _unknowns(plan, bd, mat, fg, axis_tol):
+    """Full unknown count of a planned multi-region body meshed as the
+    dispatch meshes it: 2 Nn per conductor surface, 4 Nn per interface."""
+    media = [mat.get_medium(fl, fg) for fl in plan.flags]
+    lam = bd.C0 / (fg * 1e9)
+    idx = max([1.0] + [abs(pkg("bor_solver").causal_medium(e, u)[0]) for (e, u) in media])
+    nn = lambda run, l=lam / idx: len(bd.mesh_run(run, l, 50000, axis_tol)[0])
+    R = plan.runs
+    if plan.kind == "coated":
+        return 4 * nn(R["interface"][0]) + 2 * nn(R["core"][0])
+    if plan.kind == "partial":
+        return 4 * nn(R["interface"][0]) + 2 * nn(R["core"][0]) + sum(2 * nn(r, lam) for r in R["bare"])
+    if plan.kind == "layered":
+        return (4 * nn(R["patch"][0]) + 4 * nn(R["mid"][0]) + sum(4 * nn(r) for r in R["exposed"])
+                + 2 * nn(R["core"][0]))
+    return sum(4 * nn(r) for r in R["interfaces"]) + 2 * nn(R["core"][0])
+
+
+def test_multi_region_layouts_and_meshes_match_reference():
+    """The coated, partially coated, two-layer, patch and three-layer
+    snapshots classify as the reference classifies them and mesh to the
+    reference's unknown counts at both frequencies
+    (tests/golden/make_golden_multiregion_dispatch.py)."""
+    from make_golden_multiregion_dispatch import snapshots
+    bd = pkg("bor_dispatch")
+    mats = pkg("materials")
+    d = np.load(os.path.join(ROOT, "tests", "golden", "dispatch_multiregion.npz"))
+    for name, snap in snapshots().items():
+        pieces = bd.read_pieces(snap, 1.0)
+        kind = bd.layout(pieces)
+        assert kind == str(d[name + "_kind"]), name
+        diag = max(float(np.ptp(np.vstack([c.pts for c in pieces]), axis=0).max()), 1e-9)
+        plan = bd.region_plan(kind, pieces, max(1e-12, 1e-9 * diag), 1e-6 * diag)
+        lib = mats.MaterialLibrary.from_entries([], snap["dielectrics"], base_dir=ROOT)
+        got = [_unknowns(plan, bd, lib, fg, 1e-6 * diag) for fg in (0.3, 0.36)]
+        assert got == list(d[name + "_n_unknowns"]), (name, got)
+
+
+def test_multi_region_validation_errors():
+    from make_golden_multiregion_dispatch import snapshots
+    bd = pkg("bor_dispatch")
+    snaps = snapshots()
+    bad = dict(snaps["coated"])
+    bad["segments"] = [dict(bad["segments"][0]), dict(bad["segments"][1], properties=[4, 1, 0, 2, 0])]
+    with pytest.raises(ValueError):          # interface and core on different coating flags
+        bd.layout(bd.read_pieces(bad, 1.0))
+    bad = dict(snaps["layered"])
+    bad["segments"] = [bad["segments"][0], dict(bad["segments"][1], properties=[5, 1, 0, 2, 2]),
+                       bad["segments"][2]]
+    with pytest.raises(ValueError):          # a TYPE 5 pair needs two different flags
+        bd.layout(bd.read_pieces(bad, 1.0))
+    bad = dict(snaps["layered_n"])
+    bad["segments"] = [bad["segments"][0], dict(bad["segments"][1], properties=[5, 1, 0, 3, 4]),
+                       bad["segments"][2], bad["segments"][3]]
+    pieces = bd.read_pieces(bad, 1.0)
+    with pytest.raises(ValueError):          # the layer flags do not lead to the core's
+        bd.region_plan(bd.layout(pieces), pieces, 1e-9, 1e-6)

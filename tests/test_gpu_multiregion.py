@@ -1,4 +1,5 @@
-"""Multi-region PMCHWT (PEC cores under coating layers) on the HIP path
+"""Multi-region PMCHWT (PEC cores under full or partial coatings, layers and
+patches, and their .geo dispatch) on the HIP path
 against the reference's golden vectors
 (tests/golden/make_golden_multiregion.py), the FP64 oracle and the layered
 Mie series (run on an MI355X)."""
@@ -121,3 +122,57 @@ def test_coated_sphere_against_oracle_and_mie():
     mie_db = 10.0 * math.log10(mie.sigma_coated_pec_sphere(0.3, 0.4, eps, mu, 300e6))
     for key in ("sigma_vv", "sigma_hh"):
         assert np.max(np.abs(10.0 * np.log10(np.array(out[key])) - mie_db)) < 0.1, key
+
+
+def test_partial_coating_and_patch_match_reference():
+    """A coated cap (one junction), a coated band (two junctions) and a
+    second layer over part of a coating, end to end against the reference's
+    solve_bor_partial_coating / solve_bor_coating_patch
+    (tests/golden/make_golden_multiregion_dispatch.py)."""
+    bs = pkg("bor_solver")
+    d = load("partial_multiregion")
+    nm, f, th = int(d["m_max"]), float(d["freq_hz"]), d["thetas"]
+    for tag, bares in (("cap", [d["cap_bare"]]), ("band", [d["band_bare0"], d["band_bare1"]])):
+        print(f"[case] solve_bor_partial_coating {tag}", flush=True)
+        out = bs.solve_bor_partial_coating(d[tag + "_iface"], d[tag + "_covered"], bares, f, th, complex(d["eps"]),
+                                           complex(d["mu"]), n_modes=nm)
+        for key in ("amp_vv", "amp_hh"):
+            assert rel(np.array(out[key]), d[tag + "_" + key]) < 1e-9, (tag, key)
+        assert out["modes_used"] == int(d[tag + "_modes_used"])
+        assert out["n_unknowns"] == int(d[tag + "_n_unknowns"])
+        assert out["n_junctions"] == int(d[tag + "_n_junctions"])
+        assert out["linear_residual"] < 1e-12
+    with pytest.raises(NotImplementedError):
+        bs.solve_bor_partial_coating(d["cap_iface"], d["cap_covered"], [d["cap_bare"]], f, th, 2.0, 1.0,
+                                     bare_zs=[50.0], n_modes=nm)
+    d = load("patch_multiregion")
+    print("[case] solve_bor_coating_patch", flush=True)
+    out = bs.solve_bor_coating_patch(d["patch"], d["mid_cov"], d["mid_bare"], d["core"], float(d["freq_hz"]),
+                                     d["thetas"], complex(d["eps_inner"]), complex(d["mu_inner"]),
+                                     complex(d["eps_patch"]), complex(d["mu_patch"]), n_modes=int(d["m_max"]))
+    for key in ("amp_vv", "amp_hh"):
+        assert rel(np.array(out[key]), d[key]) < 1e-9, key
+    assert out["n_unknowns"] == int(d["n_unknowns"]) and out["n_junctions"] == int(d["n_junctions"])
+
+
+@pytest.mark.parametrize("name", ["coated", "partial", "layered", "patch", "layered_n"])
+def test_dispatch_multi_region_layouts_match_reference(name):
+    """.geo snapshots of the multi-region layouts through
+    solve_monostatic_rcs_bor at two frequencies against the reference's
+    dispatch (samples, modes used, unknowns, formulation label)."""
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden_multiregion_dispatch import snapshots, THETAS, NM
+    bd = pkg("bor_dispatch")
+    d = load("dispatch_multiregion")
+    print(f"[case] dispatch {name}", flush=True)
+    res = bd.solve_monostatic_rcs_bor(snapshots()[name], [0.3, 0.36], list(THETAS), "VV", geometry_units="meters",
+                                      n_modes=NM, workers=1)
+    amp = np.array([s["rcs_amp_real"] + 1j * s["rcs_amp_imag"] for s in res["samples"]])
+    assert rel(amp, d[name + "_amp"]) < 1e-9
+    assert np.max(np.abs(np.array([s["rcs_db"] for s in res["samples"]]) - d[name + "_db"])) < 1e-7
+    meta = res["metadata"]
+    assert [m["modes_used"] for m in meta["per_frequency"]] == list(d[name + "_modes_used"])
+    assert [m["n_unknowns"] for m in meta["per_frequency"]] == list(d[name + "_n_unknowns"])
+    assert meta["formulation"] == str(d[name + "_formulation"])
+    assert meta["geometry_kind"] == str(d[name + "_kind"])

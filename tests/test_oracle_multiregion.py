@@ -112,3 +112,28 @@ def test_cross_pair_classification():
     c = bo.Body(np.array([[0.0, 1.0], [0.5, 0.5], [0.9, 0.0]]))
     with pytest.raises(ValueError):
         bo.cross_pair_kinds(a, c)
+
+
+def test_partial_coating_and_patch_layouts_match_reference():
+    """The region system with the partial-coating layout (a coated cap with
+    one junction, a coated band with two) and with the coating-patch layout
+    gives the reference's dedicated solvers' far fields
+    (tests/golden/make_golden_multiregion_dispatch.py)."""
+    d = np.load(os.path.join(GOLD, "partial_multiregion.npz"))
+    eps, mu = complex(d["eps"]), complex(d["mu"])
+    for tag, bares in (("cap", [d["cap_bare"]]), ("band", [d["band_bare0"], d["band_bare1"]])):
+        surf, reg = bo.partial_coating_layout(d[tag + "_iface"], d[tag + "_covered"], bares, eps, mu)
+        out = bo.solve_multiregion(bo.MultiRegion(surf, reg, float(d["freq_hz"]), int(d["m_max"])), d["thetas"])
+        for key in ("amp_vv", "amp_hh"):
+            assert _rel(np.array(out[key]), d[tag + "_" + key]) < 1e-12, (tag, key)
+        assert out["modes_used"] == int(d[tag + "_modes_used"])
+        assert out["n_unknowns"] == int(d[tag + "_n_unknowns"])
+        assert out["n_junctions"] == int(d[tag + "_n_junctions"])
+    d = np.load(os.path.join(GOLD, "patch_multiregion.npz"))
+    surf, reg = bo.coating_patch_layout(d["patch"], d["mid_cov"], d["mid_bare"], d["core"],
+                                        complex(d["eps_inner"]), complex(d["mu_inner"]),
+                                        complex(d["eps_patch"]), complex(d["mu_patch"]))
+    out = bo.solve_multiregion(bo.MultiRegion(surf, reg, float(d["freq_hz"]), int(d["m_max"])), d["thetas"])
+    for key in ("amp_vv", "amp_hh"):
+        assert _rel(np.array(out[key]), d[key]) < 1e-12, key
+    assert out["n_unknowns"] == int(d["n_unknowns"]) and out["n_junctions"] == int(d["n_junctions"])
