@@ -30,11 +30,13 @@ import numpy as np
 
 from . import _lib
 
-DEFAULT = {_lib.GHOST_FP64: (64, 4), _lib.GHOST_MIXED: (64, 1)}
+DEFAULT = {_lib.GHOST_FP64: (64, 4), _lib.GHOST_MIXED: (64, 5)}
 CANDIDATES = {
     _lib.GHOST_FP64: tuple((nb, v) for nb in (32, 64) for v in (1, 2, 4)),
-    _lib.GHOST_MIXED: tuple((nb, v) for nb in (32, 64) for v in (1, 2, 3)),
+    # FP32 factors: the FP32 MFMA tile and the split-bf16 tile
+    _lib.GHOST_MIXED: tuple((nb, v) for nb in (32, 64) for v in (1, 5)),
 }
+VERSION = 2   # bumped when the candidate set changes: older cache entries are not reused
 _LOCK = threading.Lock()
 _MEM: Dict[str, Tuple[int, int]] = {}
 _cache_path = os.environ.get("GHOST_BOR_TUNE_CACHE",
@@ -60,7 +62,7 @@ def _bucket(v: int, lo: int = 6, hi: int = 12) -> int:
 
 def key(precision: int, n: int, nsys: int, nrhs: int) -> str:
     return (f"{_lib.device_name()}|p{precision}|n{_bucket(n)}|s{_bucket(nsys, 0, 8)}"
-            f"|c{_bucket(nrhs, 1, 10)}")
+            f"|c{_bucket(nrhs, 1, 10)}|v{VERSION}")
 
 
 def _load() -> Dict[str, Tuple[int, int]]:

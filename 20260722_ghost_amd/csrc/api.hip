@@ -394,8 +394,9 @@ struct ghost_bor {
   hipGraphExec_t gexec = nullptr;
   int refine_iters = 0;
   double probe_tol = 1e-5;
-  int gemm_variant = -1;             // see gemm_launch; -1: 1 for FP32, 4 for FP64 (measured at n ~ 2400)
-  int lu_super = 4;                  // panels per LU super-block (trailing update K = 64 x lu_super)
+  int gemm_variant = -1;             // see gemm_launch; -1: 5 for FP32, 4 for FP64 (measured at n ~ 2400)
+  int lu_super = 0;                  // panels per LU super-block (trailing update K = nb x lu_super);
+                                     // 0: 8 for FP32 factors, 4 for FP64
   std::vector<DevBuf> allocs;
   // optional per-launch profile
   bool profile = false;
@@ -1326,7 +1327,11 @@ static int build_schedule(ghost_bor* h) {
   // multiplied by L11^-1.  One trailing update with K = k1 - k0 then covers
   // rows and columns beyond the super-block: the trailing matrix is read and
   // written once per S panels instead of once per panel.
-  const int S = std::max(1, h->lu_super);
+  // deeper super-blocks pay off for the FP32 factors, whose split-bf16 GEMM
+  // is bound by the C-tile traffic of short-K updates (measured at n ~ 2400:
+  // S = 4 / 8 / 12 give 137.0 / 134.6 / 133.8 ms per step); the FP64 GEMM is
+  // MFMA-bound and S = 4..12 are within 2 %
+  const int S = h->lu_super > 0 ? h->lu_super : (h->precision == GHOST_FP64 ? 4 : 8);
   h->nparts = nsys >= 2 ? 2 : 1;
   for (int part = 0; part < 3; ++part) {
   if (part == 1 && h->nparts < 2) continue;
@@ -2485,16 +2490,24 @@ extern "C" int ghost_bor_set_solution(ghost_bor* h, int prob, int m, const doubl
 // Batched complex GEMM C -= A B with the handle's tile variant:
 // 0: 64 x 64 single-buffered; 1: 64 x 64, 2: 64 x 128, 3: 128 x 128 (FP32) /
 // 64 x 128 (FP64), double-buffered; 4: variant 1 with the C tile loaded
-// before the K loop.
+// before the K loop; 5 (FP32 only; FP64 runs 4): 64 x 64 on the bf16 MFMA
+// with three-way split operands (k_gemm_bf3).
 template <typename T, bool TA = false, bool TB = false>
 static void gemm_launch(ghost_bor* h, const GemmOp* ops, int nops, const int* tt, const void* A, const void* B,
                         void* C) {
   int v = h->gemm_variant;
-  if (v < 0) v = sizeof(T) == 8 ? 4 : 1;
+  if (v < 0) v = sizeof(T) == 8 ? 4 : 5;
   if (v == 3 && sizeof(T) == 8) v = 2;
+  if (v == 5 && sizeof(T) == 8) v = 4;
   const int sh = (v == 2) ? 1 : (v == 3 ? 2 : 0);
   if (tt[sh] <= 0) return;
   const dim3 grid((unsigned)tt[sh]);
+  if constexpr (sizeof(T) == 4) {
+    if (v == 5) {
+      hipLaunchKernelGGL((k_gemm_bf3<TA, TB>), grid, dim3(256), 0, h->stream, ops, nops, A, B, C);
+      return;
+    }
+  }
   switch (v) {
     case 0: hipLaunchKernelGGL((k_gemm_update<T, TA, TB>), grid, dim3(256), 0, h->stream, ops, nops, A, B, C); break;
     case 1: hipLaunchKernelGGL((k_gemm2<T, 2, 2, TA, TB>), grid, dim3(256), 0, h->stream, ops, nops, A, B, C); break;
@@ -2770,7 +2783,7 @@ extern "C" int ghost_bor_set_lu_super(ghost_bor* h, int panels) {
 
 extern "C" int ghost_bor_set_gemm_variant(ghost_bor* h, int variant) {
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
-  if (variant < -1 || variant > 4) return fail(GHOST_ERR_ARG, "GEMM variant must be -1..4");
+  if (variant < -1 || variant > 5) return fail(GHOST_ERR_ARG, "GEMM variant must be -1..5");
   h->gemm_variant = variant;
   return 0;
 }

@@ -356,6 +356,198 @@ __global__ __launch_bounds__(256) void k_gemm2(const GemmOp* ops, int nops, cons
       }
 }
 
+// ─────────────────────────────────────────────────────────────────────────────
+// FP32 complex GEMM on the bf16 MFMA: each FP32 operand is split exactly into
+// three bf16 parts (x = x0 + x1 + x2, 8 significant bits each), and the six
+// products x_i y_j with i + j <= 2 are accumulated in FP32 (the MFMA's own
+// accumulation rounds to nearest, checked on gfx950); the three dropped ones
+// stay below 2 x 2^-24 of the product, so the result carries FP32 accuracy.
+// A 32x32x16 block of real products (32768 flops) takes six
+// v_mfma_f32_32x32x16_bf16 of 32 cycles = 192 cycles, against 16
+// v_mfma_f32_16x16x4_f32 of 32 cycles = 512 cycles on the FP32 MFMA.
+// ─────────────────────────────────────────────────────────────────────────────
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// x = p0 + p1 + p2 exactly (x finite, normal range), two values (k, k + 1) at
+// a time, packed into one dword per part (k in the low half) as the MFMA
+// operand expects: p0 is x rounded to bf16, p1 the rounded remainder (at most
+// 16 significant bits before rounding), p2 what is left (at most 8, so exact).
+// Rounding to nearest lets p1 and p2 take either sign, so the three dropped
+// products (p1 q2, p2 q1, p2 q2) are unbiased; with truncated parts they all
+// carry the sign of the product, and their coherent sum moved the monostatic
+// RCS by 4.4e-3 dB (against 1.7e-3 dB rounded and 1.0e-3 dB for the FP32
+// MFMA; 1200-element ogive, |m| <= 70, measured).
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split3x2(float x0, float x1, unsigned& p0, unsigned& p1, unsigned& p2) {
+  const f2v x = {x0, x1};
+  const bf2v h0 = __builtin_convertvector(x, bf2v);
+  const f2v r = x - __builtin_convertvector(h0, f2v);
+  const bf2v h1 = __builtin_convertvector(r, bf2v);
+  const f2v t = r - __builtin_convertvector(h1, f2v);
+  const bf2v h2 = __builtin_convertvector(t, bf2v);
+  p0 = __builtin_bit_cast(unsigned, h0);
+  p1 = __builtin_bit_cast(unsigned, h1);
+  p2 = __builtin_bit_cast(unsigned, h2);
+}
+
+// One 32x32x16 block.  Lane l supplies A[l%32][8(l/32)+j] and B[8(l/32)+j][l%32]
+// (j = 0..7); accumulator element r holds D[8(r/4) + 4(l/32) + r%4][l%32]
+// (checked on gfx950 against a host product).
+__device__ __forceinline__ f16v mfma32(bf16x8 a, bf16x8 b, f16v c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// C -= A B, complex FP32, 64 x 64 tile per 256-thread block (2 x 2 wavefronts
+// of one 32 x 32 block each), K in chunks of 16 staged through one LDS buffer
+// (37 KB: four work-groups per CU) as the three bf16 parts of the real and
+// imaginary parts, with the next chunk's global loads in flight during the
+// current chunk's MFMAs.  (Two LDS buffers, 74 KB and two work-groups per CU,
+// measured slower: 36.6 against 35.1 ms per step at n ~ 2400; so were
+// 64 x 128 and 128 x 128 tiles, held to one work-group per CU.)  Same op list
+// and tile order as k_gemm2<float, 2, 2>.  Every thread stages four
+// consecutive k of one row of A and one column of B (one 8-byte LDS write per
+// part); when the operand's k is the strided index in memory the rows /
+// columns run over the lanes, so the global loads stay coalesced.  Chunks
+// inside the matrix take their loads from pointers advanced per chunk with
+// no bounds tests.  Three accumulators: Re A Re B, Im A Im B and the
+// imaginary part; Re C = first - second.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_gemm_bf3(const GemmOp* ops, int nops, const void* Abase,
+                                                  const void* Bbase, void* Cbase) {
+  typedef float2 C;
+  constexpr int BM = 64, BN = 64, KC = 16, KP = KC + 8;   // KP: plane row pitch (48 bytes)
+  const int bid = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const GemmOp op = ops[find_op(ops, nops, 0, bid)];
+  if (op.M <= 0 || op.N <= 0 || op.K <= 0) return;
+  const int tn = (op.N + BN - 1) / BN;
+  const int tiles = ((op.M + BM - 1) / BM) * tn;
+  const int tile = bid - op.t0[0];
+  if (tile < 0 || tile >= tiles) return;
+  const int tm = (op.M + BM - 1) / BM, per_group = 8 * tn;
+  const int g0 = (tile / per_group) * 8, gm = min(tm - g0, 8), t_in = tile % per_group;
+  const int m0 = (g0 + t_in % gm) * BM, n0 = (t_in / gm) * BN;
+  const C* A = (const C*)Abase + op.a;
+  const C* B = (const C*)Bbase + op.b;
+  C* Cm = (C*)Cbase + op.c;
+
+  // [part][re, im][row or col][k]: each lane's eight k of one row / column
+  // are one 16-byte read
+  __shared__ __attribute__((aligned(16))) __bf16 As[3][2][BM][KP];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[3][2][BN][KP];
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wy = wv >> 1, wx = wv & 1;
+  const int l32 = lane & 31, kh = lane >> 5;
+
+  // staging: row / column rc, k = k0 + 4 kq + q (q = 0..3)
+  const int a_rc = TA ? (tid & 63) : (tid >> 2), a_kq = TA ? (tid >> 6) : (tid & 3);
+  const int b_rc = TB ? (tid >> 2) : (tid & 63), b_kq = TB ? (tid & 3) : (tid >> 6);
+  const int gr = m0 + a_rc, gc = n0 + b_rc;
+  const bool rowin = gr < op.M, colin = gc < op.N;
+  // element (k0 + 4 kq + q) of this thread's row / column at pa[q * sa] (+ k0 * ka)
+  const size_t sa = TA ? (size_t)op.lda : 1, ka = TA ? (size_t)op.lda : 1;
+  const size_t sb = TB ? 1 : (size_t)op.ldb, kb = TB ? 1 : (size_t)op.ldb;
+  const C* pa = A + (TA ? (size_t)(4 * a_kq) * op.lda + (rowin ? gr : 0) : (size_t)(rowin ? gr : 0) * op.lda + 4 * a_kq);
+  const C* pb = B + (TB ? (size_t)(colin ? gc : 0) * op.ldb + 4 * b_kq : (size_t)(4 * b_kq) * op.ldb + (colin ? gc : 0));
+  C ra[4], rb[4];
+  auto gload = [&](int k0) {
+    if (k0 + KC <= op.K) {
+      const C* qa = pa + (size_t)k0 * ka;
+      const C* qb = pb + (size_t)k0 * kb;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ra[q] = qa[q * sa];
+        rb[q] = qb[q * sb];
+      }
+      if (!rowin) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { ra[q].x = 0; ra[q].y = 0; }
+      }
+      if (!colin) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { rb[q].x = 0; rb[q].y = 0; }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        C v;
+        v.x = 0; v.y = 0;
+        if (rowin && k0 + 4 * a_kq + q < op.K) v = pa[(size_t)k0 * ka + q * sa];
+        ra[q] = v;
+        C w;
+        w.x = 0; w.y = 0;
+        if (colin && k0 + 4 * b_kq + q < op.K) w = pb[(size_t)k0 * kb + q * sb];
+        rb[q] = w;
+      }
+    }
+  };
+  auto put = [&](__bf16* plane0, size_t pstride, size_t reim, const C* v) {
+    unsigned re[3][2], im[3][2];
+    split3x2(v[0].x, v[1].x, re[0][0], re[1][0], re[2][0]);
+    split3x2(v[2].x, v[3].x, re[0][1], re[1][1], re[2][1]);
+    split3x2(v[0].y, v[1].y, im[0][0], im[1][0], im[2][0]);
+    split3x2(v[2].y, v[3].y, im[0][1], im[1][1], im[2][1]);
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      *(uint2*)(plane0 + p * pstride) = make_uint2(re[p][0], re[p][1]);
+      *(uint2*)(plane0 + p * pstride + reim) = make_uint2(im[p][0], im[p][1]);
+    }
+  };
+  auto lstore = [&]() {
+    put(&As[0][0][a_rc][4 * a_kq], (size_t)2 * BM * KP, (size_t)BM * KP, ra);
+    put(&Bs[0][0][b_rc][4 * b_kq], (size_t)2 * BN * KP, (size_t)BN * KP, rb);
+  };
+
+  f16v pRR, pII, pIm;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) { pRR[r] = 0.f; pII[r] = 0.f; pIm[r] = 0.f; }
+  const int nk = (op.K + KC - 1) / KC;
+  const int arow = wy * 32 + l32, bcol = wx * 32 + l32, koff = 8 * kh;
+  gload(0);
+  for (int kc = 0; kc < nk; ++kc) {
+    lstore();
+    __syncthreads();
+    if (kc + 1 < nk) gload((kc + 1) * KC);
+    bf16x8 ar[3], ai[3], br[3], bi[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      ar[p] = *(const bf16x8*)&As[p][0][arow][koff];
+      ai[p] = *(const bf16x8*)&As[p][1][arow][koff];
+      br[p] = *(const bf16x8*)&Bs[p][0][bcol][koff];
+      bi[p] = *(const bf16x8*)&Bs[p][1][bcol][koff];
+    }
+    // the smallest terms first: (2,0) (1,1) (0,2), (1,0) (0,1), (0,0)
+#pragma unroll
+    for (int t = 5; t >= 0; --t) {
+      const int u = t == 5 ? 2 : (t == 4 ? 1 : (t == 2 ? 1 : 0));
+      const int v = t == 3 ? 2 : (t == 4 ? 1 : (t == 1 ? 1 : 0));
+      pRR = mfma32(ar[u], br[v], pRR);
+      pII = mfma32(ai[u], bi[v], pII);
+      pIm = mfma32(ar[u], bi[v], pIm);
+      pIm = mfma32(ai[u], br[v], pIm);
+    }
+    __syncthreads();   // every wave has read the chunk before the next is staged
+  }
+  const int gcol = n0 + bcol;
+  if (gcol < op.N) {
+    C* pc = Cm + (size_t)(m0 + wy * 32 + 4 * kh) * op.ldc + gcol;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int dr = 8 * (r >> 2) + (r & 3);
+      if (m0 + wy * 32 + 4 * kh + dr < op.M) {
+        C* p = pc + (size_t)dr * op.ldc;
+        C v = *p;
+        v.x -= pRR[r] - pII[r];
+        v.y -= pIm[r];
+        *p = v;
+      }
+    }
+  }
+}
+
 // Two-column triangular solves of the error probe with the FP32 factors, one
 // 1024-thread work-group per system (in place on the n x 2 block at x):
 // forward (UPPER = false) through the unit lower factor, y_b = T^-1 y_b with

@@ -25,7 +25,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MI355X_PEAK = {"fp64_mfma_tflops": 78.6, "fp64_vector_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
+MI355X_PEAK = {"fp64_mfma_tflops": 78.6, "fp64_vector_tflops": 78.6, "fp32_mfma_tflops": 157.3,
+               "bf16_mfma_tflops": 2500.0, "hbm_gbs": 8000.0}
+SPLIT_BF16_PRODUCTS = 6   # bf16 MFMA products per FP32 product in the split-bf16 GEMM (variant 5)
 
 
 def pkg(m):
@@ -52,7 +54,8 @@ def tuned_tiles(precision, nodes, thetas, m_max):
     return tiles
 
 
-def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profile=True, tiles=(64, -1)):
+def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profile=True, tiles=(64, -1),
+            lu_super=None):
     log(f"{precision}: build (m_max={m_max}, n_elems={len(nodes) - 1}, aspects={len(thetas)})")
     L = pkg("_lib")
     bs = pkg("bor_solver")
@@ -60,6 +63,8 @@ def run_gpu(nodes, freq, thetas, m_max, precision, steps, warmup, refine, profil
     _, ne, nbk, streaming = bs.discretisation(nodes, k, thetas, m_max, "cfie")
     h = L.Handle(L.GHOST_MIXED if precision == "mixed" else L.GHOST_FP64, nb=tiles[0], residual=False)
     h.set_gemm_variant(tiles[1])
+    if lu_super is not None:
+        h.set_lu_super(lu_super)
     h.add_problem(nodes, k, m_max, ne, nbk, 0.5, 0.5 * bs.ETA0, thetas)
     h.build()
     for i in range(warmup):
@@ -546,6 +551,10 @@ def main():
     ap.add_argument("--refine", type=int, default=-2,
                     help="FP64 refinement sweeps; negative = adaptive (error probe), up to |n|")
     ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--gemm-variant", type=int, default=None,
+                    help="GEMM tile variant of the headline path in place of the autotuner's")
+    ap.add_argument("--lu-super", type=int, default=None,
+                    help="LU panels per super-block of the headline path (default: the library's 4)")
     ap.add_argument("--precision", choices=("mixed", "fp64"), default="mixed",
                     help="path of the timed steps (fp64: for profiling the plain FP64 path)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -580,9 +589,12 @@ def main():
     solves = (2 * m_max + 1) * len(thetas)
 
     tiles_mx = tuned_tiles(args.precision, nodes, thetas, m_max)
+    if args.gemm_variant is not None:
+        tiles_mx = (tiles_mx[0], args.gemm_variant)
+        log(f"{args.precision}: GEMM variant set to {args.gemm_variant} by --gemm-variant")
     dt_mx, dF_mx, kt_mx, ph_mx, nxi = run_gpu(nodes, freq, thetas, m_max, args.precision, args.steps,
                                                args.warmup, args.refine if args.precision == "mixed" else 0,
-                                               tiles=tiles_mx)
+                                               tiles=tiles_mx, lu_super=args.lu_super)
     F_mx = pkg("bor_solver").truncate_modes(dF_mx, m_max, 0.0)[0]
     err_db = None
     fp64_rate = None
@@ -613,30 +625,45 @@ def main():
         achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12
         traffic = None
         fp32 = args.precision == "mixed"
+        split = fp32 and tiles_mx[1] == 5
         tpath = os.path.join(ROOT, "profiles", "ogive1200_m70_mixed_hbm_traffic.json")
         if args.config == 2 and fp32 and os.path.exists(tpath):
             with open(tpath) as f:
-                traffic = json.load(f)["per_launch"]["gemm"]["hbm_bytes"]
-        fp32 = args.precision == "mixed"
-        pk = MI355X_PEAK["fp32_mfma_tflops" if fp32 else "fp64_mfma_tflops"]
-        roof = {"kernel": ("k_gemm2<float,2,2> (complex FP32 MFMA 16x16x4, 64x64 tiles; LU updates and solves)"
-                           if fp32 else "k_gemm2<double,...> (complex FP64 MFMA 16x16x4; LU updates and solves)"),
-                "bound": "mfma", "achieved": round(achieved, 3), "peak": pk,
-                "unit": "TFLOP/s", "frac": round(achieved / pk, 4),
-                "traffic": traffic, "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
-                "launches": gemm_n, "avg_launch_ms": round(gemm_ms / gemm_n, 4),
-                "flops_per_launch": gemm_fl / gemm_n}
+                per = json.load(f)["per_launch"]
+            key = "gemm_bf3" if split else "gemm"
+            traffic = per[key]["hbm_bytes"] if key in per else None
+        if split:
+            # the pipe doing the work is the bf16 MFMA: each FP32 product is six bf16 products
+            issued = SPLIT_BF16_PRODUCTS * achieved
+            pk = MI355X_PEAK["bf16_mfma_tflops"]
+            roof = {"kernel": ("k_gemm_bf3 (complex FP32 GEMM on the bf16 MFMA 32x32x16: operands split into "
+                               "three bf16 parts, six products each; 64x64 tiles; LU updates and solves)"),
+                    "bound": "mfma", "achieved": round(issued, 3), "peak": pk, "unit": "TFLOP/s",
+                    "frac": round(issued / pk, 4),
+                    "achieved_note": "bf16 MFMA flops issued = 6 x the algorithmic complex FP32 GEMM flops",
+                    "fp32_equivalent": {"achieved": round(achieved, 3), "peak": MI355X_PEAK["fp32_mfma_tflops"],
+                                        "frac": round(achieved / MI355X_PEAK["fp32_mfma_tflops"], 4)}}
+        else:
+            pk = MI355X_PEAK["fp32_mfma_tflops" if fp32 else "fp64_mfma_tflops"]
+            roof = {"kernel": ("k_gemm2<float,2,2> (complex FP32 MFMA 16x16x4, 64x64 tiles; LU updates and solves)"
+                               if fp32 else "k_gemm2<double,...> (complex FP64 MFMA 16x16x4; LU updates and solves)"),
+                    "bound": "mfma", "achieved": round(achieved, 3), "peak": pk,
+                    "unit": "TFLOP/s", "frac": round(achieved / pk, 4)}
+        roof.update({"traffic": traffic,
+                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
+                     "launches": gemm_n, "avg_launch_ms": round(gemm_ms / gemm_n, 4),
+                     "flops_per_launch": gemm_fl / gemm_n})
     # the fill's two big kernels against their own bounds (live HIP-event time of the profiled step):
     # the far fill's MFMA flops issued against FP32 MFMA peak (it also issues the sample arithmetic on
     # the VALU, which does not co-issue with MFMA on gfx950), the near band's projection flops against
     # FP64 vector peak (its FP32 tail runs at the same issue rate with packed FP32 off)
     more = {}
-    for fam, peak_key, what in (("far_fill", "fp32_mfma_tflops", "k_far_fill (FP32 MFMA transform + VALU samples)"),
-                                ("near_modal", "fp64_vector_tflops", "k_near_modal (FP64 core / FP32 tail VALU projection)")):
-        ms_f, n_f, fl_f = kt_mx.get(fam, (0.0, 0, 0.0))
+    for fk, peak_key, what in (("far_fill", "fp32_mfma_tflops", "k_far_fill (FP32 MFMA transform + VALU samples)"),
+                               ("near_modal", "fp64_vector_tflops", "k_near_modal (FP64 core / FP32 tail VALU projection)")):
+        ms_f, n_f, fl_f = kt_mx.get(fk, (0.0, 0, 0.0))
         if ms_f > 0 and fl_f > 0:
             ach = fl_f / (ms_f * 1e-3) / 1e12
-            more[fam] = {"kernel": what, "bound": "mfma" if fam == "far_fill" else "valu",
+            more[fk] = {"kernel": what, "bound": "mfma" if fk == "far_fill" else "valu",
                          "achieved": round(ach, 3), "peak": MI355X_PEAK[peak_key], "unit": "TFLOP/s",
                          "frac": round(ach / MI355X_PEAK[peak_key], 4), "ms_per_step": round(ms_f, 3),
                          "flops_per_step": fl_f}

@@ -208,9 +208,12 @@ int ghost_bor_solve(ghost_bor* h, int refine_iters);
 /* Tile variant of the batched complex MFMA GEMM (trailing updates, solves,
  * residuals): 0 = 64x64 single-buffered, 1 = 64x64, 2 = 64x128, 3 = 128x128
  * (FP32; 64x128 for FP64) with double-buffered K chunks, 4 = variant 1 with
- * the C tile loaded before the K loop; -1 (default) = 1 for FP32, 4 for FP64. */
+ * the C tile loaded before the K loop; 5 (FP32 factors; FP64 handles run 4)
+ * = 64x64 on the bf16 MFMA with every FP32 operand split exactly into three
+ * bf16 parts (rounded to nearest) and the six leading cross products
+ * accumulated in FP32 (FP32 accuracy); -1 (default) = 5 for FP32, 4 for FP64. */
 int ghost_bor_set_gemm_variant(ghost_bor* h, int variant);
-/* Panels per LU super-block (before build; default 4): the trailing update
+/* Panels per LU super-block (before build; default 8 for FP32 factors, 4 for FP64): the trailing update
  * runs once per super-block with K = nb x panels, the panels and row blocks
  * inside it catch up through narrow GEMMs.  1 = plain right-looking LU. */
 int ghost_bor_set_lu_super(ghost_bor* h, int panels);

@@ -76,7 +76,7 @@ def _sphere120_handle(precision=0, nb=64, variant=None, lu_super=None):
 
 
 @pytest.mark.parametrize("lu_super", [1, 3, 4])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("nb", [16, 32, 64])
 @pytest.mark.parametrize("precision", [0, 1])
 def test_lu_tile_variants_on_sphere120(variant, nb, precision, lu_super):
