@@ -66,6 +66,7 @@ SIGNATURES = [
     ("ghost_bor_kernel_name", ctypes.c_char_p, [_ci]),
     ("ghost_bor_kernel_times", _ci, [_vp, _dp, _ip, _dp]),
     ("ghost_bor_adjoint", _ci, [_vp, _dp, _cd, _dp]),
+    ("ghost_bor_adjoint_analytic", _ci, [_vp, _dp, _dp]),
     ("ghost_copy_calibration", _ci, [_cd, _fp]),
     ("ghost_bor_add_dense", _ci, [_vp, _ci, _ci, _ip]),
     ("ghost_bor_set_dense", _ci, [_vp, _ci, _dp, _dp]),
@@ -363,19 +364,27 @@ class Handle:
         check(self.lib.ghost_bor_get_dF(self._h, int(prob), dptr(out)))
         return out[..., 0] + 1j * out[..., 1]
 
-    def adjoint(self, acoef, step: float) -> np.ndarray:
+    def adjoint(self, acoef, step: float = 0.0, method: str = "analytic") -> np.ndarray:
         """Gradient w.r.t. the radial coordinate of every node of every problem
         of dJ = Re sum_c a_c dF_c, with acoef [problems][n_rhs] complex in
         column order 2 * theta_index + (0 VV, 1 HH) and F the mode-summed
-        far-field amplitude.  Needs a solved FP64 handle; afterwards the
-        systems hold a perturbed fill (fill again before the next solve)."""
+        far-field amplitude.  Needs a solved FP64 handle.  method "analytic":
+        the fill's shape derivative in the kernels (the systems keep their
+        factors); "fd": central differences of the fill with half-width step
+        (0: 1e-6 of the shortest element), after which the systems hold a
+        perturbed fill (fill again before the next solve)."""
         a = np.ascontiguousarray(np.asarray(acoef, dtype=np.complex128).ravel())
         ncol = sum(2 * nth for _, nth in self.problems)
         if a.size != ncol:
             raise ValueError(f"acoef must have {ncol} entries, got {a.size}")
         ab = np.ascontiguousarray(np.column_stack([a.real, a.imag]).ravel())
         out = np.zeros(sum(self.nodes_per_problem), dtype=np.float64)
-        check(self.lib.ghost_bor_adjoint(self._h, dptr(ab), float(step), dptr(out)))
+        if method == "analytic":
+            check(self.lib.ghost_bor_adjoint_analytic(self._h, dptr(ab), dptr(out)))
+        elif method == "fd":
+            check(self.lib.ghost_bor_adjoint(self._h, dptr(ab), float(step), dptr(out)))
+        else:
+            raise ValueError(f"method must be 'analytic' or 'fd', got {method!r}")
         return out
 
     def num_systems(self) -> int:

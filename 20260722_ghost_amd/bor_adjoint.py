@@ -4,10 +4,12 @@
 of the PEC body whose node radii are ``spec.nodes[:, 0] + offsets``.  Its
 forward pass is the HIP path in FP64 (fill, batched LU, far field: the
 solve of bor_solver.py:820-1004 with every mode kept, mode_tol = 0); its
-backward pass is the library's adjoint (ghost_bor_adjoint): one transposed
-solve through the LU factors of the forward pass and FP64 central
-differences of the fill, right-hand side and far-field projection, so one
-backward costs about 12 fills whatever the number of nodes.
+backward pass is the library's adjoint (ghost_bor_adjoint_analytic): one
+transposed solve through the LU factors of the forward pass, then the
+fill's, right-hand side's and far-field projection's analytic shape
+derivative contracted with the adjoint weights in the kernels, whatever the
+number of nodes (``method="fd"``: FP64 central differences of 12 batched
+fills instead, the check path).
 
 The reference solver has no gradient; this is the differentiable front end
 of the same discretisation.  The azimuth grids (n_xi) are fixed by ``spec``
@@ -39,7 +41,8 @@ class RcsSpec:
     thetas_deg: np.ndarray
     m_max: int
     cfie_alpha: float = 0.5
-    step: float = 0.0             # adjoint difference half width [m]; 0: 1e-6 shortest element
+    step: float = 0.0             # "fd" half width [m]; 0: 1e-6 shortest element
+    method: str = "analytic"      # adjoint: "analytic" (dual-number fill derivative) or "fd"
     n_xi: tuple = field(default=(0, 0))
 
     def __post_init__(self):
@@ -90,7 +93,7 @@ class BorRcsDb(torch.autograd.Function):
         w = grad_out.detach().cpu().double().numpy()
         a = w * _DB * np.conj(F) / np.abs(F) ** 2          # dJ = Re sum a dF
         acoef = np.ascontiguousarray(a.T).ravel()           # column 2 theta_index + pol
-        g = h.adjoint(acoef, spec.step)
+        g = h.adjoint(acoef, spec.step, spec.method)
         h.close()
         ctx.handle = None
         return torch.from_numpy(g), None
@@ -108,6 +111,6 @@ def objective_and_gradient(spec: RcsSpec, weights=None):
     s = sigma_db(F)
     w = np.ones_like(s) if weights is None else np.asarray(weights, dtype=np.float64)
     a = w * _DB * np.conj(F) / np.abs(F) ** 2
-    g = h.adjoint(np.ascontiguousarray(a.T).ravel(), spec.step)
+    g = h.adjoint(np.ascontiguousarray(a.T).ravel(), spec.step, spec.method)
     h.close()
     return float(np.sum(w * s)), g

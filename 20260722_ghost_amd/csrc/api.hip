@@ -16,6 +16,7 @@
 #include "excite.hip"
 #include "lu.hip"
 #include "adjoint.hip"
+#include "dfill.hip"
 #include "../../include/ghost_bor.h"
 
 using namespace ghost;
@@ -365,6 +366,11 @@ struct ghost_bor {
   double2* d_acoef = nullptr;        // a' per problem column
   double2* d_ffpart = nullptr;       // per-element far-field partial sums of one problem
   double2* d_ffs = nullptr;          // per-element contraction with a'
+  double* d_adj_far = nullptr;       // analytic adjoint: far pair partials [Ne * Ne][4]
+  double* d_adj_near = nullptr;      // near batch partials [batches][4]
+  double* d_adj_ex = nullptr;        // excitation partials [Ne][ntheta][2]
+  double* d_adj_grad = nullptr;      // per-node gradient [Nn]
+  double* d_adj_gauss = nullptr;     // element Gauss rule: s [4] | weights [4]
   int* d_rowoff = nullptr;
   int triu_ops[3] = {0, 0, 0}, triu_n[3] = {0, 0, 0};
   int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
@@ -3404,6 +3410,18 @@ static int adjoint_alloc(ghost_bor* h) {
   HIPCHK(hipMemcpy(h->d_rowoff, ro.data(), ro.size() * sizeof(int), hipMemcpyHostToDevice));
   for (auto& p : h->probs)
     if ((rc = dalloc(h, &p.d_alt, (9 * (size_t)p.P + 2 * (size_t)p.Nn + 3 * (size_t)p.Ne) * 8))) return rc;
+  size_t far_n = 0, near_n = 0, ex_n = 0, nn_max = 0;
+  for (auto& p : h->probs) {
+    far_n = std::max(far_n, (size_t)p.Ne * p.Ne * 4);
+    near_n = std::max(near_n, p.batches.size() * 4);
+    ex_n = std::max(ex_n, (size_t)p.Ne * p.ntheta * 2);
+    nn_max = std::max(nn_max, (size_t)p.Nn);
+  }
+  if ((rc = dalloc(h, &h->d_adj_far, std::max<size_t>(far_n, 4) * 8))) return rc;
+  if ((rc = dalloc(h, &h->d_adj_near, std::max<size_t>(near_n, 4) * 8))) return rc;
+  if ((rc = dalloc(h, &h->d_adj_ex, std::max<size_t>(ex_n, 2) * 8))) return rc;
+  if ((rc = dalloc(h, &h->d_adj_grad, nn_max * 8))) return rc;
+  if ((rc = dalloc(h, &h->d_adj_gauss, 8 * 8))) return rc;
   h->adj_alloc = true;
   return 0;
 }
@@ -3414,8 +3432,11 @@ static bool adj_free(const Problem& p, int q, int r, double step) {
   return q >= 0 && q < p.Nn && q % 3 == r && p.nodes[2 * q] > 2.0 * step;
 }
 
-extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad) {
-  if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
+// Checks, then lambda = A^-T G through the forward LU factors and
+// Q = [-Lambda X^T | Lambda] (both adjoint forms share this part); coff:
+// each problem's first column in acoef.
+static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_t>& coff) {
+  if (!h || !acoef) return fail(GHOST_ERR_ARG, "null argument");
   if (!h->built) return fail(GHOST_ERR_STATE, "build first");
   for (auto& s : h->sys) {
     if (s.prob < 0) return fail(GHOST_ERR_STATE, "the adjoint covers handles of BoR problems only");
@@ -3428,25 +3449,10 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   if (!h->regions.empty() || !h->cross.empty())
     return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies in free space only (no multi-region sets)");
   if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
-  if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
   HIPCHK(hipSetDevice(h->device));
-  // half-width per problem: given, or 1e-6 of the shortest element (test and
-  // source sides move separately, so nearly coincident near-band point pairs,
-  // a few thousandths of an element apart, set the curvature scale)
-  std::vector<double> hstep(h->probs.size(), step);
-  for (size_t pi = 0; pi < h->probs.size(); ++pi)
-    if (step == 0.0) {
-      const Problem& p = h->probs[pi];
-      double lmin = INFINITY;
-      for (int e = 0; e < p.Ne; ++e)
-        lmin = std::min(lmin, std::hypot(p.nodes[2 * e + 2] - p.nodes[2 * e], p.nodes[2 * e + 3] - p.nodes[2 * e + 1]));
-      hstep[pi] = 1e-6 * lmin;
-    }
   int rc;
   if ((rc = adjoint_alloc(h))) return rc;
   const int nsys = (int)h->sys.size();
-  const bool prof = h->profile;
-  h->profile = false;
   size_t maxn = 0, maxr = 0, nrow = 0;
   for (auto& s : h->sys) {
     maxn = std::max(maxn, (size_t)s.n);
@@ -3460,7 +3466,7 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
                      h->d_piv, h->d_perm);
   LAUNCHCHK();
   // a' = a (-j k eta0 / 4 pi) per problem column; G = a' dS/dX
-  std::vector<size_t> coff(h->probs.size());
+  coff.assign(h->probs.size(), 0);
   std::vector<double2> ac;
   for (size_t pi = 0; pi < h->probs.size(); ++pi) {
     const Problem& p = h->probs[pi];
@@ -3491,6 +3497,36 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   gemm_launch<double, false, true>(h, h->d_gemm + h->adj_q_ops, nsys, h->adj_q_tt, h->d_lam2, h->d_Xs, h->d_Q);
   hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_toaug, h->d_lam2, h->d_Q);
   LAUNCHCHK();
+  return 0;
+}
+
+extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad) {
+  if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
+  if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
+  const bool prof = h->profile;
+  h->profile = false;
+  std::vector<size_t> coff;
+  int rc;
+  if ((rc = adjoint_lambda_q(h, acoef, coff))) { h->profile = prof; return rc; }
+  const int nsys = (int)h->sys.size();
+  size_t maxn = 0, nrow = 0;
+  for (auto& s : h->sys) {
+    maxn = std::max(maxn, (size_t)s.n);
+    nrow += (size_t)s.n;
+  }
+  HIPCHK(hipSetDevice(h->device));
+  // half-width per problem: given, or 1e-6 of the shortest element (test and
+  // source sides move separately, so nearly coincident near-band point pairs,
+  // a few thousandths of an element apart, set the curvature scale)
+  std::vector<double> hstep(h->probs.size(), step);
+  for (size_t pi = 0; pi < h->probs.size(); ++pi)
+    if (step == 0.0) {
+      const Problem& p = h->probs[pi];
+      double lmin = INFINITY;
+      for (int e = 0; e < p.Ne; ++e)
+        lmin = std::min(lmin, std::hypot(p.nodes[2 * e + 2] - p.nodes[2 * e], p.nodes[2 * e + 3] - p.nodes[2 * e + 1]));
+      hstep[pi] = 1e-6 * lmin;
+    }
   // perturbed fills: side 0 moves the test geometry (rows, right-hand sides and
   // far-field projection), side 1 the source geometry (columns); nodes of one
   // residue class mod 3 move together, whose row / column triples are disjoint
@@ -3568,6 +3604,116 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
       }
   std::copy(g.begin(), g.end(), grad);
   h->solved = false;   // the systems now hold the last perturbed fill
+  h->profile = prof;
+  return 0;
+}
+
+// Analytic form: the fill's shape derivative contracted with Q in the kernels
+// of dfill.hip (far pairs, near pairs with the mass term, right-hand sides
+// and far-field projection), gathered per node.  The systems keep the
+// forward factors (nothing is refilled).
+extern "C" int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, double* grad) {
+  if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
+  const bool prof = h->profile;
+  h->profile = false;
+  std::vector<size_t> coff;
+  int rc;
+  if ((rc = adjoint_lambda_q(h, acoef, coff))) { h->profile = prof; return rc; }
+  size_t goff = 0;
+  for (size_t pi = 0; pi < h->probs.size(); ++pi) {
+    Problem& p = h->probs[pi];
+    const SysInfo* sys = h->d_sys + p.sys0;
+    // the element Gauss rule (s, weight on [0, 1]) of the problem's point tables
+    {
+      double gq[8];
+      const int P = p.P;
+      for (int i = 0; i < 4; ++i) {
+        gq[i] = p.pts[6 * P + i];               // T1 = s
+        gq[4 + i] = p.pts[4 * P + i] / p.L[0];  // w / L
+      }
+      HIPCHK(hipMemcpyAsync(h->d_adj_gauss, gq, sizeof gq, hipMemcpyHostToDevice, h->stream));
+    }
+    const double ke = p.k * 376.730313668;
+    // far element pairs
+    {
+      AdjFarArgs fa;
+      fa.nodes = p.d_nodes;
+      fa.gs = h->d_adj_gauss;
+      fa.gwq = h->d_adj_gauss + 4;
+      fa.sys = sys;
+      fa.Q = h->d_Q;
+      fa.Ne = p.Ne; fa.M = p.M; fa.nE = p.nE; fa.nB = p.nB;
+      fa.k = p.k;
+      fa.ce_re = 0.0;
+      fa.ce_im = p.cE * ke * 2.0 * M_PI;
+      fa.cm = -2.0 * M_PI * p.cM;
+      fa.out = h->d_adj_far;
+      const size_t lds = (16 * (size_t)(p.M + 1) + (size_t)ADJ_CHUNK * 28) * sizeof(double2);
+      hipLaunchKernelGGL(k_adj_far, dim3((unsigned)((size_t)p.Ne * p.Ne)), dim3(256), lds, h->stream, fa);
+      LAUNCHCHK();
+    }
+    // near element pairs (and the mass term)
+    if (!p.batches.empty()) {
+      AdjNearArgs na;
+      na.nodes = p.d_nodes;
+      const int ncell = p.cell_off[2] + p.cell_cnt[2];
+      na.cell_s = p.d_cells;
+      na.cell_sp = p.d_cells + ncell;
+      na.cell_w = p.d_cells + 2 * ncell;
+      for (int i = 0; i < 8; ++i) { na.cell_off[i] = p.cell_off[i]; na.cell_cnt[i] = p.cell_cnt[i]; }
+      na.core_u = p.d_core;
+      na.core_w = p.d_core + 48;
+      na.tail_u = p.d_tail;
+      na.tail_w = p.d_tail + p.tail_w.size();
+      static_assert(sizeof(AdjNearPair) == sizeof(NearPairInfo), "near pair layouts");
+      na.pairs = reinterpret_cast<const AdjNearPair*>(p.d_pairs);
+      na.batches = p.d_batches;
+      na.sys = sys;
+      na.Q = h->d_Q;
+      na.gs = h->d_adj_gauss;
+      na.gwq = h->d_adj_gauss + 4;
+      na.Ne = p.Ne; na.M = p.M;
+      na.k = p.k;
+      na.ce_re = 0.0;
+      na.ce_im = p.cE * ke * 2.0 * M_PI;
+      na.cm = -2.0 * M_PI * p.cM;
+      na.cmass = M_PI * p.cM;
+      na.out = h->d_adj_near;
+      const size_t lds = 16 * (size_t)(p.M + 1) * sizeof(double2);
+      hipLaunchKernelGGL(k_adj_near, dim3((unsigned)p.batches.size()), dim3(256), lds, h->stream, na);
+      LAUNCHCHK();
+    }
+    // right-hand sides and far-field projection
+    {
+      AdjExciteArgs ea;
+      ea.nodes = p.d_nodes;
+      ea.gs = h->d_adj_gauss;
+      ea.gwq = h->d_adj_gauss + 4;
+      ea.sys = sys;
+      ea.Q = h->d_Q;
+      ea.X = h->d_Xs;
+      ea.acoef = h->d_acoef + coff[pi];
+      ea.J = p.d_J;
+      ea.theta = p.d_theta;
+      ea.Ne = p.Ne; ea.P = p.P; ea.M = p.M; ea.ntheta = p.ntheta;
+      ea.k = p.k; ea.cE = p.cE; ea.cM = p.cM;
+      ea.out = h->d_adj_ex;
+      hipLaunchKernelGGL(k_adj_excite, dim3((unsigned)((p.ntheta + 127) / 128), (unsigned)p.Ne), dim3(128), 0,
+                         h->stream, ea);
+      LAUNCHCHK();
+    }
+    double rmax = 0.0;
+    for (int q = 0; q < p.Nn; ++q) rmax = std::max(rmax, p.nodes[2 * q]);
+    hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)((p.Nn + 127) / 128)), dim3(128), 0, h->stream, h->d_adj_far,
+                       h->d_adj_near, p.d_batches, reinterpret_cast<const AdjNearPair*>(p.d_pairs),
+                       (int)p.batches.size(), h->d_adj_ex, p.Ne, p.ntheta, p.d_nodes,
+                       1e-12 * std::max(1.0, rmax), h->d_adj_grad);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(grad + goff, h->d_adj_grad, (size_t)p.Nn * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    goff += (size_t)p.Nn;
+  }
   h->profile = prof;
   return 0;
 }

@@ -295,6 +295,15 @@ int ghost_bor_kernel_times(ghost_bor* h, double* ms, int* count, double* flops);
  * nodes mod 3 at a time (12 batched fills).  Needs a GHOST_FP64 handle after
  * fill + solve; afterwards the systems hold a perturbed fill. */
 int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad);
+/* The same gradient with the fill differentiated analytically (no counterpart
+ * in the reference either): after the same transposed solve and Q, the
+ * kernels contract Q with dA/d rho, dB/d rho and the far-field projection's
+ * d/d rho carried by forward-mode dual numbers through the point positions,
+ * weights, tangents, azimuth samples and the near pairs' own sinh-graded
+ * rules, so the result is the exact derivative of the discrete operator
+ * (no step, no refill; the systems keep the forward factors).  grad is 0 for
+ * nodes on the axis.  Same handle requirements as ghost_bor_adjoint. */
+int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, double* grad);
 
 /* Right-hand sides only (Bessel table and excitation of every problem): the
  * device side of rhs_mode / rhs_mfie_mode (bor_solver.py:437-461, 625-645). */
