@@ -43,6 +43,8 @@ class RcsSpec:
     cfie_alpha: float = 0.5
     step: float = 0.0             # "fd" half width [m]; 0: 1e-6 shortest element
     method: str = "analytic"      # adjoint: "analytic" (dual-number fill derivative) or "fd"
+    precision: str = "fp64"       # "fp64", or "mixed": FP32 fill and LU under FP64 refinement
+                                  # (forward and adjoint solves; analytic adjoint only)
     n_xi: tuple = field(default=(0, 0))
 
     def __post_init__(self):
@@ -58,14 +60,18 @@ class RcsSpec:
 
 def solve_far_field(spec: RcsSpec, radii: np.ndarray):
     """Mode-summed far field F [2, n_theta] of the body with the given node
-    radii, and the solved FP64 handle (kept for the adjoint)."""
+    radii, and the solved handle (kept for the adjoint; FP64 or mixed as the
+    spec says, mode pairing off)."""
     nodes = spec.nodes.copy()
     nodes[:, 0] = radii
-    h = _lib.Handle(_lib.GHOST_FP64, nb=64, residual=False, pairing=False)
+    mixed = spec.precision == "mixed"
+    if spec.precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', got {spec.precision!r}")
+    h = _lib.Handle(_lib.GHOST_MIXED if mixed else _lib.GHOST_FP64, nb=64, residual=False, pairing=False)
     a = spec.cfie_alpha
     h.add_problem(nodes, spec.k, spec.m_max, spec.n_xi[0], spec.n_xi[1], a, (1.0 - a) * ETA0, spec.thetas_deg)
     h.build()
-    h.run(0)
+    h.run(-2 if mixed else 0)   # mixed: adaptive refinement (error probe, up to 2 sweeps)
     h.sync()
     F = h.dF(0).sum(axis=0)
     return F, h

@@ -75,6 +75,21 @@ __global__ void k_unpermute(const VecOp* ops, const int* perm, const double2* sr
   dst[op.d + (size_t)pr * op.ldd + c] = src[op.s + (size_t)r * op.lds + c];
 }
 
+// dst[perm row] (+)= (double) src[r] / scale[c]: the FP32 transposed solves'
+// output in the factor's row order, unscaled and put back in system order.
+__global__ void k_unpermute_widen(const VecOp* ops, const int* perm, const float2* src, const double* scale,
+                                  double2* dst, int add) {
+  const VecOp op = ops[blockIdx.z];
+  const int r = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= op.n || c >= op.N) return;
+  const int pr = perm[op.piv + r];
+  const float2 v = src[op.s + (size_t)r * op.lds + c];
+  const double f = 1.0 / scale[op.sc + c];
+  double2* p = dst + op.d + (size_t)pr * op.ldd + c;
+  if (add) { double2 o = *p; o.x += (double)v.x * f; o.y += (double)v.y * f; *p = o; }
+  else *p = make_double2((double)v.x * f, (double)v.y * f);
+}
+
 // rs[off + r] = sum_{j < n + nrhs} Q[r][j] A[r][j] for every row r of every system.
 __global__ __launch_bounds__(256) void k_adj_rowsum(const SysInfo* sys, const int* row_off, const double2* Q,
                                                     const double2* A, double2* rs) {

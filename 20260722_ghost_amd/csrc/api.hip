@@ -357,6 +357,7 @@ struct ghost_bor {
   // adjoint (FP64 handles): A^T lambda = G through the LU factors, Q = -Lambda X^T
   std::vector<Launch> adjT_fwd, adjT_back;
   int adj_q_ops = -1;
+  int adj_refine = 1;                // FP64 refinement sweeps of lambda on FP32 factors (mixed handles)
   bool adj_alloc = false;
   double2* d_Xs = nullptr;           // forward solution, x layout
   double2* d_lam = nullptr;          // G, then z = P lambda (x layout)
@@ -1529,7 +1530,7 @@ static int build_schedule(ghost_bor* h) {
     add_tri_solves(h->fwdY_sched, h->backY_sched, [&](int si) { return h->ss[si].x_off; },
                    [&](int si) { return h->ss[si].nrhs; });
   }
-  if (!mixed) {
+  {
     // transposed solves on the lambda block (n x nrhs at x_off): with P A = L U,
     // A^T lambda = G is U^T y = G (forward, diagonal blocks (U11^-1)^T = I - W_u^T),
     // then L^T z = y (backward, I - W_l^T), then lambda = P^T z
@@ -3442,7 +3443,6 @@ static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_
     if (s.prob < 0) return fail(GHOST_ERR_STATE, "the adjoint covers handles of BoR problems only");
     if (s.ncols != s.nrhs) return fail(GHOST_ERR_STATE, "the adjoint needs a handle built with mode pairing off");
   }
-  if (h->precision != GHOST_FP64) return fail(GHOST_ERR_STATE, "the adjoint needs an FP64 handle");
   for (auto& p : h->probs)
     if (p.has_zs || p.ki != 0.0 || p.eta_re != 376.730313668 || p.eta_im != 0.0)
       return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies in free space only");
@@ -3450,6 +3450,7 @@ static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_
     return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies in free space only (no multi-region sets)");
   if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
   HIPCHK(hipSetDevice(h->device));
+  const bool mixed = h->precision == GHOST_MIXED;
   int rc;
   if ((rc = adjoint_alloc(h))) return rc;
   const int nsys = (int)h->sys.size();
@@ -3460,8 +3461,10 @@ static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_
     nrow += (size_t)s.n;
   }
   const dim3 gv((unsigned)((maxr + 255) / 256), (unsigned)maxn, nsys);
-  // forward solution out of the augmented columns; the factor's row order
-  hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_aug, (const double2*)h->d_A, h->d_Xs);
+  // forward solution: out of the FP64 augmented columns, or the refined FP64
+  // currents of a mixed-precision solve; the factor's row order
+  if (mixed) hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_X64, h->d_Xs);
+  else hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_aug, (const double2*)h->d_A, h->d_Xs);
   hipLaunchKernelGGL(k_perm_from_pivots, dim3(nsys), dim3(256), 0, h->stream, h->d_vec + h->vec_all, nsys,
                      h->d_piv, h->d_perm);
   LAUNCHCHK();
@@ -3487,10 +3490,33 @@ static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_
     LAUNCHCHK();
   }
   // lambda = A^-T G through the LU factors
-  if ((rc = run_sched<double>(h, h->adjT_fwd, h->d_A, h->d_lam))) return rc;
-  if ((rc = run_sched<double>(h, h->adjT_back, h->d_A, h->d_lam))) return rc;
-  hipLaunchKernelGGL(k_unpermute, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_perm, h->d_lam, h->d_lam2);
-  LAUNCHCHK();
+  if (!mixed) {
+    if ((rc = run_sched<double>(h, h->adjT_fwd, h->d_A, h->d_lam))) return rc;
+    if ((rc = run_sched<double>(h, h->adjT_back, h->d_A, h->d_lam))) return rc;
+    hipLaunchKernelGGL(k_unpermute, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_perm, h->d_lam,
+                       h->d_lam2);
+    LAUNCHCHK();
+  } else {
+    // FP32 factors under FP64 refinement, as the forward solve: R = G - A64^T
+    // lambda in FP64 (the residual GEMM's ops with the operand transposed),
+    // column-scaled to FP32, z = (L^T)^-1 (U^T)^-1 R with the FP32 factors,
+    // lambda += P^T z; the first pass starts from lambda = 0 (R = G)
+    const dim3 gs((unsigned)((maxr + 63) / 64), 1, nsys);
+    for (int it = 0; it <= h->adj_refine; ++it) {
+      hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_lam, h->d_R64);
+      if (it > 0)
+        gemm_launch<double, true, false>(h, h->d_gemm + h->res_gemm, nsys, h->res_tt, h->d_A64, h->d_lam2, h->d_R64);
+      hipLaunchKernelGGL(k_col_scale, gs, dim3(64, 4), 0, h->stream, h->d_vec + h->vec_all, h->d_R64, h->d_rscale);
+      hipLaunchKernelGGL(k_scale_cast, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, nullptr, h->d_R64,
+                         h->d_rscale, h->d_Y32);
+      LAUNCHCHK();
+      if ((rc = run_sched<float>(h, h->adjT_fwd, h->d_A, h->d_Y32))) return rc;
+      if ((rc = run_sched<float>(h, h->adjT_back, h->d_A, h->d_Y32))) return rc;
+      hipLaunchKernelGGL(k_unpermute_widen, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_perm, h->d_Y32,
+                         h->d_rscale, h->d_lam2, it > 0 ? 1 : 0);
+      LAUNCHCHK();
+    }
+  }
   // Q = [-Lambda X^T | Lambda]
   hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (h->a_elems + 255) / 256)), dim3(256), 0,
                      h->stream, h->d_Q, h->a_elems);
@@ -3503,6 +3529,7 @@ static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_
 extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* grad) {
   if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
   if (!(step >= 0.0)) return fail(GHOST_ERR_ARG, "step must be >= 0 (0: automatic)");
+  if (h->precision != GHOST_FP64) return fail(GHOST_ERR_STATE, "the difference adjoint needs an FP64 handle");
   const bool prof = h->profile;
   h->profile = false;
   std::vector<size_t> coff;
