@@ -67,6 +67,9 @@ SIGNATURES = [
     ("ghost_bor_kernel_times", _ci, [_vp, _dp, _ip, _dp]),
     ("ghost_bor_adjoint", _ci, [_vp, _dp, _cd, _dp]),
     ("ghost_bor_adjoint_analytic", _ci, [_vp, _dp, _dp]),
+    ("ghost_bor_set_objective", _ci, [_vp, _dp]),
+    ("ghost_bor_capture_adjoint", _ci, [_vp, _ci]),
+    ("ghost_bor_get_gradient", _ci, [_vp, _dp]),
     ("ghost_copy_calibration", _ci, [_cd, _fp]),
     ("ghost_bor_add_dense", _ci, [_vp, _ci, _ci, _ip]),
     ("ghost_bor_set_dense", _ci, [_vp, _ci, _dp, _dp]),
@@ -385,6 +388,26 @@ class Handle:
             check(self.lib.ghost_bor_adjoint(self._h, dptr(ab), float(step), dptr(out)))
         else:
             raise ValueError(f"method must be 'analytic' or 'fd', got {method!r}")
+        return out
+
+    def set_objective(self, weights) -> None:
+        """J = sum_c w_c sigma_dBsm(c): weights [problems][n_rhs] real, column
+        order 2 * theta_index + (0 VV, 1 HH); used by capture_adjoint."""
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).ravel())
+        ncol = sum(2 * nth for _, nth in self.problems)
+        if w.size != ncol:
+            raise ValueError(f"weights must have {ncol} entries, got {w.size}")
+        check(self.lib.ghost_bor_set_objective(self._h, dptr(w)))
+
+    def capture_adjoint(self, refine_iters: int = 0) -> None:
+        """Capture fill + solve + far field + the objective's adjoint into one
+        graph (launch() runs it; dF() and gradient() read the results)."""
+        check(self.lib.ghost_bor_capture_adjoint(self._h, int(refine_iters)))
+
+    def gradient(self) -> np.ndarray:
+        """dJ/d rho_node of every problem's nodes from the last analytic adjoint."""
+        out = np.zeros(sum(self.nodes_per_problem), dtype=np.float64)
+        check(self.lib.ghost_bor_get_gradient(self._h, dptr(out)))
         return out
 
     def num_systems(self) -> int:

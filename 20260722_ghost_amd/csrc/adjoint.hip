@@ -75,6 +75,25 @@ __global__ void k_unpermute(const VecOp* ops, const int* perm, const double2* sr
   dst[op.d + (size_t)pr * op.ldd + c] = src[op.s + (size_t)r * op.lds + c];
 }
 
+// a'_c of the objective J = sum_c w_c 10 log10(4 pi |F_c|^2) on one problem:
+// F_c = sum over the problem's modes of dF[m][c] (fixed order), a_c = w_c
+// (20 / ln 10) conj(F_c) / |F_c|^2, a'_c = a_c j pr with pr = -k eta0 / (4 pi).
+__global__ void k_objective_acoef(const double2* dF, int nmodes, int nrhs, const double* w, double pr,
+                                  double2* out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  double fr = 0.0, fi = 0.0;
+  for (int m = 0; m < nmodes; ++m) {
+    const double2 v = dF[(size_t)m * nrhs + c];
+    fr += v.x;
+    fi += v.y;
+  }
+  const double n2 = fr * fr + fi * fi;
+  const double f = n2 > 0.0 ? w[c] * (20.0 / 2.302585092994045684) / n2 : 0.0;
+  const double ar = f * fr, ai = -f * fi;
+  out[c] = make_double2(-ai * pr, ar * pr);
+}
+
 // dst[perm row] (+)= (double) src[r] / scale[c]: the FP32 transposed solves'
 // output in the factor's row order, unscaled and put back in system order.
 __global__ void k_unpermute_widen(const VecOp* ops, const int* perm, const float2* src, const double* scale,

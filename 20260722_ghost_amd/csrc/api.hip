@@ -371,7 +371,9 @@ struct ghost_bor {
   double* d_adj_near = nullptr;      // near batch partials [batches][4]
   double* d_adj_ex = nullptr;        // excitation partials [Ne][ntheta][2]
   double* d_adj_grad = nullptr;      // per-node gradient [Nn]
-  double* d_adj_gauss = nullptr;     // element Gauss rule: s [4] | weights [4]
+  double* d_adj_gauss = nullptr;     // element Gauss rule per problem: s [4] | weights [4]
+  double* d_objw = nullptr;          // objective weights per problem column (captured adjoint)
+  bool has_objective = false;
   int* d_rowoff = nullptr;
   int triu_ops[3] = {0, 0, 0}, triu_n[3] = {0, 0, 0};
   int res_gemm = -1, res_gemm_aug = -1;  // residual GEMMs with X in X64 / in the augmented columns
@@ -3421,8 +3423,23 @@ static int adjoint_alloc(ghost_bor* h) {
   if ((rc = dalloc(h, &h->d_adj_far, std::max<size_t>(far_n, 4) * 8))) return rc;
   if ((rc = dalloc(h, &h->d_adj_near, std::max<size_t>(near_n, 4) * 8))) return rc;
   if ((rc = dalloc(h, &h->d_adj_ex, std::max<size_t>(ex_n, 2) * 8))) return rc;
-  if ((rc = dalloc(h, &h->d_adj_grad, nn_max * 8))) return rc;
-  if ((rc = dalloc(h, &h->d_adj_gauss, 8 * 8))) return rc;
+  size_t nn_all = 0;
+  for (auto& p : h->probs) nn_all += (size_t)p.Nn;
+  (void)nn_max;
+  if ((rc = dalloc(h, &h->d_adj_grad, std::max<size_t>(nn_all, 1) * 8))) return rc;
+  if ((rc = dalloc(h, &h->d_adj_gauss, 8 * std::max<size_t>(h->probs.size(), 1) * 8))) return rc;
+  {
+    // the element Gauss rule (s, weight on [0, 1]) of each problem's point tables
+    std::vector<double> gq(8 * h->probs.size());
+    for (size_t pi = 0; pi < h->probs.size(); ++pi) {
+      const Problem& p = h->probs[pi];
+      for (int i = 0; i < 4; ++i) {
+        gq[8 * pi + i] = p.pts[6 * (size_t)p.P + i];              // T1 = s
+        gq[8 * pi + 4 + i] = p.pts[4 * (size_t)p.P + i] / p.L[0]; // w / L
+      }
+    }
+    HIPCHK(hipMemcpy(h->d_adj_gauss, gq.data(), gq.size() * 8, hipMemcpyHostToDevice));
+  }
   h->adj_alloc = true;
   return 0;
 }
@@ -3433,11 +3450,10 @@ static bool adj_free(const Problem& p, int q, int r, double step) {
   return q >= 0 && q < p.Nn && q % 3 == r && p.nodes[2 * q] > 2.0 * step;
 }
 
-// Checks, then lambda = A^-T G through the forward LU factors and
-// Q = [-Lambda X^T | Lambda] (both adjoint forms share this part); coff:
-// each problem's first column in acoef.
-static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_t>& coff) {
-  if (!h || !acoef) return fail(GHOST_ERR_ARG, "null argument");
+// What every adjoint form needs of the handle (solved: checked by the
+// callers that run on an already solved handle).
+static int adjoint_checks(ghost_bor* h, bool need_solved) {
+  if (!h) return fail(GHOST_ERR_ARG, "null handle");
   if (!h->built) return fail(GHOST_ERR_STATE, "build first");
   for (auto& s : h->sys) {
     if (s.prob < 0) return fail(GHOST_ERR_STATE, "the adjoint covers handles of BoR problems only");
@@ -3448,11 +3464,35 @@ static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_
       return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies in free space only");
   if (!h->regions.empty() || !h->cross.empty())
     return fail(GHOST_ERR_STATE, "the adjoint covers PEC bodies in free space only (no multi-region sets)");
-  if (!h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
+  if (need_solved && !h->solved) return fail(GHOST_ERR_STATE, "fill and solve before the adjoint");
   HIPCHK(hipSetDevice(h->device));
-  const bool mixed = h->precision == GHOST_MIXED;
+  return 0;
+}
+
+// a' = a (-j k eta0 / 4 pi) per problem column, from the caller's a
+static int adjoint_host_acoef(ghost_bor* h, const double* acoef, std::vector<size_t>& coff) {
+  // a' = a (-j k eta0 / 4 pi) per problem column; G = a' dS/dX
+  coff.assign(h->probs.size(), 0);
+  std::vector<double2> ac;
+  for (size_t pi = 0; pi < h->probs.size(); ++pi) {
+    const Problem& p = h->probs[pi];
+    coff[pi] = ac.size();
+    const double pr = -p.k * 376.730313668 / (4.0 * M_PI);   // pref = j pr
+    for (int c = 0; c < p.nrhs; ++c) {
+      const double ar = acoef[2 * (coff[pi] + c)], ai = acoef[2 * (coff[pi] + c) + 1];
+      ac.push_back(make_double2(-ai * pr, ar * pr));       // (ar + j ai)(j pr)
+    }
+  }
+  HIPCHK(hipMemcpyAsync(h->d_acoef, ac.data(), ac.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+  return 0;
+}
+
+// lambda = A^-T G through the forward LU factors (G = a' dS/dX with a' in
+// d_acoef, coff: each problem's first column) and Q = [-Lambda X^T | Lambda],
+// shared by both adjoint forms; kernels only (capturable).
+static int adjoint_lambda_q(ghost_bor* h, const std::vector<size_t>& coff) {
   int rc;
-  if ((rc = adjoint_alloc(h))) return rc;
+  const bool mixed = h->precision == GHOST_MIXED;
   const int nsys = (int)h->sys.size();
   size_t maxn = 0, maxr = 0, nrow = 0;
   for (auto& s : h->sys) {
@@ -3468,19 +3508,6 @@ static int adjoint_lambda_q(ghost_bor* h, const double* acoef, std::vector<size_
   hipLaunchKernelGGL(k_perm_from_pivots, dim3(nsys), dim3(256), 0, h->stream, h->d_vec + h->vec_all, nsys,
                      h->d_piv, h->d_perm);
   LAUNCHCHK();
-  // a' = a (-j k eta0 / 4 pi) per problem column; G = a' dS/dX
-  coff.assign(h->probs.size(), 0);
-  std::vector<double2> ac;
-  for (size_t pi = 0; pi < h->probs.size(); ++pi) {
-    const Problem& p = h->probs[pi];
-    coff[pi] = ac.size();
-    const double pr = -p.k * 376.730313668 / (4.0 * M_PI);   // pref = j pr
-    for (int c = 0; c < p.nrhs; ++c) {
-      const double ar = acoef[2 * (coff[pi] + c)], ai = acoef[2 * (coff[pi] + c) + 1];
-      ac.push_back(make_double2(-ai * pr, ar * pr));       // (ar + j ai)(j pr)
-    }
-  }
-  HIPCHK(hipMemcpyAsync(h->d_acoef, ac.data(), ac.size() * sizeof(double2), hipMemcpyHostToDevice, h->stream));
   for (size_t pi = 0; pi < h->probs.size(); ++pi) {
     Problem& p = h->probs[pi];
     int mn = 0;
@@ -3534,7 +3561,11 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   h->profile = false;
   std::vector<size_t> coff;
   int rc;
-  if ((rc = adjoint_lambda_q(h, acoef, coff))) { h->profile = prof; return rc; }
+  if ((rc = adjoint_checks(h, true)) || (rc = adjoint_alloc(h)) || (rc = adjoint_host_acoef(h, acoef, coff)) ||
+      (rc = adjoint_lambda_q(h, coff))) {
+    h->profile = prof;
+    return rc;
+  }
   const int nsys = (int)h->sys.size();
   size_t maxn = 0, nrow = 0;
   for (auto& s : h->sys) {
@@ -3637,36 +3668,22 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
 
 // Analytic form: the fill's shape derivative contracted with Q in the kernels
 // of dfill.hip (far pairs, near pairs with the mass term, right-hand sides
-// and far-field projection), gathered per node.  The systems keep the
+// and far-field projection), gathered per node into d_adj_grad (every
+// problem's nodes in turn).  Kernels only (capturable); the systems keep the
 // forward factors (nothing is refilled).
-extern "C" int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, double* grad) {
-  if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
-  const bool prof = h->profile;
-  h->profile = false;
-  std::vector<size_t> coff;
-  int rc;
-  if ((rc = adjoint_lambda_q(h, acoef, coff))) { h->profile = prof; return rc; }
+static int adjoint_analytic_kernels(ghost_bor* h, const std::vector<size_t>& coff) {
   size_t goff = 0;
   for (size_t pi = 0; pi < h->probs.size(); ++pi) {
     Problem& p = h->probs[pi];
     const SysInfo* sys = h->d_sys + p.sys0;
-    // the element Gauss rule (s, weight on [0, 1]) of the problem's point tables
-    {
-      double gq[8];
-      const int P = p.P;
-      for (int i = 0; i < 4; ++i) {
-        gq[i] = p.pts[6 * P + i];               // T1 = s
-        gq[4 + i] = p.pts[4 * P + i] / p.L[0];  // w / L
-      }
-      HIPCHK(hipMemcpyAsync(h->d_adj_gauss, gq, sizeof gq, hipMemcpyHostToDevice, h->stream));
-    }
+    const double* gauss = h->d_adj_gauss + 8 * pi;   // s [4] | weights [4]
     const double ke = p.k * 376.730313668;
     // far element pairs
     {
       AdjFarArgs fa;
       fa.nodes = p.d_nodes;
-      fa.gs = h->d_adj_gauss;
-      fa.gwq = h->d_adj_gauss + 4;
+      fa.gs = gauss;
+      fa.gwq = gauss + 4;
       fa.sys = sys;
       fa.Q = h->d_Q;
       fa.Ne = p.Ne; fa.M = p.M; fa.nE = p.nE; fa.nB = p.nB;
@@ -3697,8 +3714,8 @@ extern "C" int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, dou
       na.batches = p.d_batches;
       na.sys = sys;
       na.Q = h->d_Q;
-      na.gs = h->d_adj_gauss;
-      na.gwq = h->d_adj_gauss + 4;
+      na.gs = gauss;
+      na.gwq = gauss + 4;
       na.Ne = p.Ne; na.M = p.M;
       na.k = p.k;
       na.ce_re = 0.0;
@@ -3714,8 +3731,8 @@ extern "C" int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, dou
     {
       AdjExciteArgs ea;
       ea.nodes = p.d_nodes;
-      ea.gs = h->d_adj_gauss;
-      ea.gwq = h->d_adj_gauss + 4;
+      ea.gs = gauss;
+      ea.gwq = gauss + 4;
       ea.sys = sys;
       ea.Q = h->d_Q;
       ea.X = h->d_Xs;
@@ -3734,14 +3751,103 @@ extern "C" int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, dou
     hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)((p.Nn + 127) / 128)), dim3(128), 0, h->stream, h->d_adj_far,
                        h->d_adj_near, p.d_batches, reinterpret_cast<const AdjNearPair*>(p.d_pairs),
                        (int)p.batches.size(), h->d_adj_ex, p.Ne, p.ntheta, p.d_nodes,
-                       1e-12 * std::max(1.0, rmax), h->d_adj_grad);
+                       1e-12 * std::max(1.0, rmax), h->d_adj_grad + goff);
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(grad + goff, h->d_adj_grad, (size_t)p.Nn * sizeof(double), hipMemcpyDeviceToHost,
-                          h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
     goff += (size_t)p.Nn;
   }
+  return 0;
+}
+
+extern "C" int ghost_bor_get_gradient(ghost_bor* h, double* grad);
+
+extern "C" int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, double* grad) {
+  if (!h || !acoef || !grad) return fail(GHOST_ERR_ARG, "null argument");
+  const bool prof = h->profile;
+  h->profile = false;
+  std::vector<size_t> coff;
+  int rc;
+  if ((rc = adjoint_checks(h, true)) || (rc = adjoint_alloc(h)) || (rc = adjoint_host_acoef(h, acoef, coff)) ||
+      (rc = adjoint_lambda_q(h, coff)) || (rc = adjoint_analytic_kernels(h, coff))) {
+    h->profile = prof;
+    return rc;
+  }
   h->profile = prof;
+  return ghost_bor_get_gradient(h, grad);
+}
+
+// Objective J = sum_c w_c sigma_dBsm(c) over every problem column for the
+// captured forward + adjoint: a_c = w_c (20 / ln 10) conj(F_c) / |F_c|^2 is
+// then formed on the device from the far field of the same launch.
+extern "C" int ghost_bor_set_objective(ghost_bor* h, const double* weights) {
+  if (!h || !weights) return fail(GHOST_ERR_ARG, "null argument");
+  if (!h->built) return fail(GHOST_ERR_STATE, "build first");
+  HIPCHK(hipSetDevice(h->device));
+  size_t ncol = 0;
+  for (auto& p : h->probs) ncol += (size_t)p.nrhs;
+  int rc;
+  if (!h->d_objw && (rc = dalloc(h, &h->d_objw, std::max<size_t>(ncol, 1) * sizeof(double)))) return rc;
+  HIPCHK(hipMemcpy(h->d_objw, weights, ncol * sizeof(double), hipMemcpyHostToDevice));
+  h->has_objective = true;
+  return 0;
+}
+
+// Capture fill + solve (refine_iters fixed sweeps) + far field + the
+// objective's a' + lambda, Q and the analytic gradient into the handle's
+// graph: ghost_bor_launch then runs the forward RCS and its gradient in one
+// graph launch; ghost_bor_get_dF and ghost_bor_get_gradient read the results.
+extern "C" int ghost_bor_capture_adjoint(ghost_bor* h, int refine_iters) {
+  int rc;
+  if ((rc = adjoint_checks(h, false))) return rc;
+  if (!h->has_objective) return fail(GHOST_ERR_STATE, "set the objective before capturing the adjoint");
+  if (refine_iters < 0) return fail(GHOST_ERR_ARG, "a captured solve takes a fixed number of refinement sweeps");
+  if ((rc = adjoint_alloc(h))) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (h->gexec) { hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { hipGraphDestroy(h->graph); h->graph = nullptr; }
+  std::vector<size_t> coff(h->probs.size());
+  {
+    size_t c = 0;
+    for (size_t pi = 0; pi < h->probs.size(); ++pi) { coff[pi] = c; c += (size_t)h->probs[pi].nrhs; }
+  }
+  const bool prof = h->profile;
+  h->profile = false;
+  h->capturing = true;
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  rc = run_fill(h);
+  if (!rc) rc = ghost_bor_solve(h, refine_iters);
+  if (!rc) rc = ghost_bor_farfield(h);
+  if (!rc) {
+    for (size_t pi = 0; pi < h->probs.size(); ++pi) {
+      const Problem& p = h->probs[pi];
+      hipLaunchKernelGGL(k_objective_acoef, dim3((unsigned)((p.nrhs + 127) / 128)), dim3(128), 0, h->stream,
+                         h->d_dF + p.dF_off, p.nmodes, p.nrhs, h->d_objw + coff[pi],
+                         -p.k * 376.730313668 / (4.0 * M_PI), h->d_acoef + coff[pi]);
+    }
+    if (hipGetLastError() != hipSuccess) rc = fail(GHOST_ERR_HIP, "objective kernel launch failed");
+  }
+  if (!rc) rc = adjoint_lambda_q(h, coff);
+  if (!rc) rc = adjoint_analytic_kernels(h, coff);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(h->stream, &g);
+  h->capturing = false;
+  h->profile = prof;
+  if (rc) { if (g) hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return fail(GHOST_ERR_HIP, "stream capture failed: %s", hipGetErrorString(e));
+  h->graph = g;
+  HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+  return 0;
+}
+
+// dJ/d rho of every node of every problem (problem order) from the last
+// analytic adjoint, eager or captured.
+extern "C" int ghost_bor_get_gradient(ghost_bor* h, double* grad) {
+  if (!h || !grad) return fail(GHOST_ERR_ARG, "null argument");
+  if (!h->adj_alloc) return fail(GHOST_ERR_STATE, "no adjoint has run on this handle");
+  HIPCHK(hipSetDevice(h->device));
+  size_t nn = 0;
+  for (auto& p : h->probs) nn += (size_t)p.Nn;
+  HIPCHK(hipMemcpyAsync(grad, h->d_adj_grad, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
 }
 

@@ -112,7 +112,8 @@ def frequency_sweep(n_freq: int = 64, seed: int = 7, n_theta: int = 181):
 
 
 def config(index: int):
-    """(name, nodes, freq_hz, thetas_deg, m_max) of a benchmark configuration."""
+    """(name, nodes, freq_hz, thetas_deg, m_max) of a benchmark configuration
+    (configuration 5, a batch of problems, is perturbed_batch)."""
     if index == 1:
         return ("pec-sphere-a0.5m-300MHz-120seg", sphere(0.5, 120), 3e8,
                 np.arange(0.0, 181.0, 1.0), 12)
@@ -125,3 +126,40 @@ def config(index: int):
         return ("cone-sphere-15deg-600seg", cone_sphere(600, 15.0, lam), C0 / lam,
                 np.arange(0.0, 181.0, 2.0), 40)
     raise ValueError(f"no configuration {index}")
+
+
+def fourier_perturbation(nodes: np.ndarray, coef: np.ndarray, amplitude: float, taper: float = 0.2) -> np.ndarray:
+    """Radial Fourier perturbation of a generatrix: d(t) = sum_k coef_k sin(k pi t)
+    over the normalised arc length t in [0, 1] (zero at both ends), scaled to
+    max |d| = amplitude, tapered by min(1, rho / taper) so thin ends stay off
+    the axis (rho + d >= rho (1 - amplitude / taper))."""
+    seg = np.hypot(*np.diff(nodes, axis=0).T)
+    t = np.concatenate([[0.0], np.cumsum(seg)]) / np.sum(seg)
+    k = np.arange(1, len(coef) + 1)
+    d = np.sin(np.pi * np.outer(t, k)) @ np.asarray(coef, dtype=np.float64)
+    d *= amplitude / np.max(np.abs(d))
+    out = nodes.copy()
+    out[:, 0] = nodes[:, 0] + d * np.minimum(1.0, nodes[:, 0] / taper)
+    out[0, 0] = nodes[0, 0]
+    out[-1, 0] = nodes[-1, 0]
+    return out
+
+
+def perturbed_batch(n_shapes: int = 32, n_coef: int = 16, amplitude: float = 0.05, seed: int = 5,
+                    freq_scales=(0.5, 1.0, 1.5), n_theta: int = 91, base_elems: int = 1200, base_m: int = 70):
+    """Config 5: n_shapes seeded Fourier radial perturbations (n_coef
+    coefficients c_k ~ U(-1, 1) / k, max offset = amplitude wavelengths) of the
+    ogive-cylinder-cone body (wavelength 1 m at f0 = c / 1 m), each at the
+    frequencies f0 x freq_scales with lambda / 10 meshing (600, 1200, 1800
+    elements) and m_max = base_m x scale; n_theta aspects x 2 pols.  Returns a
+    list of (shape index, nodes, freq_hz, thetas_deg, m_max)."""
+    rng = np.random.default_rng(seed)
+    coefs = [rng.uniform(-1.0, 1.0, n_coef) / np.arange(1, n_coef + 1) for _ in range(n_shapes)]
+    thetas = np.linspace(0.0, 180.0, n_theta)
+    bases = {sc: ogive_cylinder_cone(int(round(base_elems * sc)), 1.0) for sc in freq_scales}
+    out = []
+    for i, c in enumerate(coefs):
+        for sc in freq_scales:
+            nodes = fourier_perturbation(bases[sc], c, amplitude)
+            out.append((i, nodes, C0 * sc, thetas, int(round(base_m * sc))))
+    return out

@@ -274,6 +274,114 @@ def run_adjoint(steps):
     print(json.dumps(out), flush=True)
 
 
+def batch_bytes(nodes, m_max, n_theta):
+    """Device bytes of one problem in a mixed-precision handle with mode
+    pairing off and the adjoint's buffers: FP32 augmented systems, their FP64
+    copy, Q, the FP64 n x n_rhs blocks, far blocks and near point-pair values."""
+    Nn = len(nodes)
+    n, nsys, nrhs = 2 * Nn, 2 * m_max + 1, 2 * n_theta
+    ld = n + nrhs
+    zmodes = 16 * ((nsys + 15) // 16)
+    total = (nsys * n * ld * (8 + 16) + nsys * n * n * 16 + 8 * nsys * n * nrhs * 16
+             + (2 * Nn) ** 2 * zmodes * 8 + 1440 * (Nn - 1) * (m_max + 2) * 80)
+    return 1.1 * total
+
+
+def run_shape_batch(n_shapes, budget_gb):
+    """Config 5: forward RCS and the gradient of J = sum sigma_dBsm w.r.t.
+    every node radius for n_shapes Fourier perturbations of the ogive body at
+    three frequencies (600 / 1200 / 1800 elements, |m| <= 35 / 70 / 105), mixed
+    precision (FP32 fill and LU, one FP64 refinement sweep for the currents and
+    one for lambda), each batch of problems (grouped under the memory budget)
+    captured as ONE graph: fill + solve + far field + a' + transposed solve + Q
+    + analytic fill derivative.  Prints its own JSON line."""
+    wl = pkg("workloads")
+    bs = pkg("bor_solver")
+    L = pkg("_lib")
+    probs = wl.perturbed_batch(n_shapes)
+    specs = []
+    for (i, nodes, f, th, mm) in probs:
+        k = 2 * math.pi * f / bs.C0
+        _, ne, nb, _ = bs.discretisation(nodes, k, th, mm, "cfie")
+        specs.append((i, nodes, k, mm, ne, nb, th))
+    batches, cur, cur_b = [], [], 0.0
+    for sp in specs:
+        b = batch_bytes(sp[1], sp[3], len(sp[6]))
+        if cur and cur_b + b > budget_gb * 1e9:
+            batches.append(cur)
+            cur, cur_b = [], 0.0
+        cur.append(sp)
+        cur_b += b
+    if cur:
+        batches.append(cur)
+    solves = sum((2 * sp[3] + 1) * len(sp[6]) for sp in specs)
+    log(f"shape batch: {n_shapes} shapes x 3 frequencies = {len(specs)} problems, {solves} solves, "
+        f"{len(batches)} graph launches")
+    t_gpu, t_wall, rows = 0.0, time.perf_counter(), []
+    grads = []
+    for bi, batch in enumerate(batches):
+        t0 = time.perf_counter()
+        h = L.Handle(L.GHOST_MIXED, nb=64, residual=False, pairing=False)
+        for (_, nodes, k, mm, ne, nb, th) in batch:
+            h.add_problem(nodes, k, mm, ne, nb, 0.5, 0.5 * bs.ETA0, th)
+        h.build()
+        h.set_objective(np.ones(sum(2 * len(sp[6]) for sp in batch)))
+        h.capture_adjoint(1)
+        t1 = time.perf_counter()
+        h.launch()
+        h.sync()
+        ms = h.graph_ms()
+        g = h.gradient()
+        J = [float(np.sum(10 * np.log10(4 * math.pi * np.abs(h.dF(i).sum(axis=0)) ** 2))) for i in range(len(batch))]
+        h.close()
+        t_gpu += ms * 1e-3
+        off = 0
+        for (sp, j) in zip(batch, J):
+            grads.append((sp, j, g[off:off + len(sp[1])]))
+            off += len(sp[1])
+        rows.append({"problems": [f"shape{sp[0]}:{len(sp[1]) - 1}el" for sp in batch], "graph_ms": round(ms, 1),
+                     "build_capture_s": round(t1 - t0, 2)})
+        log(f"shape batch {bi + 1}/{len(batches)}: {len(batch)} problems, graph {ms:.1f} ms, "
+            f"build + capture {t1 - t0:.1f} s")
+    t_wall = time.perf_counter() - t_wall
+    # check: the smallest problem of shape 0 against its FP64 analytic gradient and
+    # central differences of the FP64 forward on two nodes
+    sp, J0, g0 = grads[0]
+    _, nodes, k, mm, ne, nb, th = sp
+    ad = pkg("bor_adjoint")
+    spec = ad.RcsSpec(nodes, k * bs.C0 / (2 * math.pi), th, mm, cfie_alpha=0.5, n_xi=(ne, nb))
+    J64, g64 = ad.objective_and_gradient(spec)
+    rel64 = float(np.max(np.abs(g0 - g64)) / np.max(np.abs(g64)))
+    lam = 2 * math.pi / k
+    fd_err = []
+    for q in (len(nodes) // 3, 2 * len(nodes) // 3):
+        vals = []
+        for sgn in (1.0, -1.0):
+            r = nodes[:, 0].copy()
+            r[q] += sgn * 1e-6 * lam
+            Fq, hq = ad.solve_far_field(spec, r)
+            hq.close()
+            vals.append(float(np.sum(ad.sigma_db(Fq))))
+        fd = (vals[0] - vals[1]) / (2e-6 * lam)
+        fd_err.append(abs(g0[q] - fd) / max(abs(fd), 1e-30))
+        log(f"shape batch check: node {q} mixed graph {g0[q]:.9e} fp64 {g64[q]:.9e} fd {fd:.9e}")
+    out = {
+        "metric": "forward RCS + gradient (J = sum sigma_dBsm) of perturbed shapes, batched graph launches",
+        "config": {"workload": f"ogive-1200seg-fourier16-perturbed-x{n_shapes}-3freq", "shapes": n_shapes,
+                   "frequencies": [0.5, 1.0, 1.5], "elements": [600, 1200, 1800], "m_max": [35, 70, 105],
+                   "aspects": len(specs[0][6]), "polarizations": 2, "precision": "mixed (FP32 fill/LU, FP64 "
+                   "refinement: 1 sweep currents, 1 sweep lambda)", "problems": len(specs),
+                   "graph_launches": len(batches), "budget_gb": budget_gb},
+        "gpu_s": round(t_gpu, 3), "wall_s": round(t_wall, 1),
+        "solves_per_s_gpu": round(solves / t_gpu, 1),
+        "gradients_per_s_gpu": round(len(specs) / t_gpu, 3),
+        "check_vs_fp64_max_rel": rel64, "check_vs_central_differences_max_rel": float(max(fd_err)),
+        "J_shape0_vs_fp64": [J0, J64],
+        "batches": rows,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_sweep(steps, warmup):
     """Config 3: 64 frequencies of one body in ONE batch (uneven systems of
     ~100-2000 unknowns), eagerly and as a replayed hipGraph, against the same
@@ -561,6 +669,9 @@ def main():
     ap.add_argument("--refine", type=int, default=-2,
                     help="FP64 refinement sweeps; negative = adaptive (error probe), up to |n|")
     ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--shapes", type=int, default=32, help="config 5: number of perturbed shapes")
+    ap.add_argument("--budget-gb", type=float, default=200.0,
+                    help="config 5: device memory per batch handle (problems are grouped under it)")
     ap.add_argument("--gemm-variant", type=int, default=None,
                     help="GEMM tile variant of the headline path in place of the autotuner's")
     ap.add_argument("--lu-super", type=int, default=None,
@@ -593,6 +704,9 @@ def main():
         return
     if args.config == 3:
         run_sweep(max(1, args.steps), args.warmup)
+        return
+    if args.config == 5:
+        run_shape_batch(args.shapes, args.budget_gb)
         return
     wl = pkg("workloads")
     name, nodes, freq, thetas, m_max = wl.config(args.config)

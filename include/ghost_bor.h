@@ -302,8 +302,21 @@ int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step, double* gr
  * weights, tangents, azimuth samples and the near pairs' own sinh-graded
  * rules, so the result is the exact derivative of the discrete operator
  * (no step, no refill; the systems keep the forward factors).  grad is 0 for
- * nodes on the axis.  Same handle requirements as ghost_bor_adjoint. */
+ * nodes on the axis.  Handles as ghost_bor_adjoint, or GHOST_MIXED: lambda
+ * then comes from the FP32 factors under one FP64 refinement sweep. */
 int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, double* grad);
+/* Forward RCS and its gradient in one graph launch (no counterpart in the
+ * reference).  set_objective: J = sum_c w_c sigma_dBsm(c) with real weights
+ * per problem column (column order as acoef).  capture_adjoint: records fill,
+ * solve with refine_iters fixed FP64 refinement sweeps (>= 0), far field, the
+ * objective's a_c = w_c (20 / ln 10) conj(F_c) / |F_c|^2 formed on the device,
+ * the transposed solve, Q and the analytic gradient into the handle's graph
+ * (replayed by ghost_bor_launch).  get_gradient: dJ/d rho of every node of
+ * every problem (problem order) after the last analytic adjoint, eager or
+ * captured.  FP64 or GHOST_MIXED handles with mode pairing off. */
+int ghost_bor_set_objective(ghost_bor* h, const double* weights);
+int ghost_bor_capture_adjoint(ghost_bor* h, int refine_iters);
+int ghost_bor_get_gradient(ghost_bor* h, double* grad);
 
 /* Right-hand sides only (Bessel table and excitation of every problem): the
  * device side of rhs_mode / rhs_mfie_mode (bor_solver.py:437-461, 625-645). */
