@@ -68,6 +68,7 @@ SIGNATURES = [
     ("ghost_bor_adjoint", _ci, [_vp, _dp, _cd, _dp]),
     ("ghost_bor_adjoint_analytic", _ci, [_vp, _dp, _dp]),
     ("ghost_bor_set_objective", _ci, [_vp, _dp]),
+    ("ghost_bor_set_nodes", _ci, [_vp, _ci, _dp]),
     ("ghost_bor_capture_adjoint", _ci, [_vp, _ci]),
     ("ghost_bor_get_gradient", _ci, [_vp, _dp]),
     ("ghost_copy_calibration", _ci, [_cd, _fp]),
@@ -389,6 +390,14 @@ class Handle:
         else:
             raise ValueError(f"method must be 'analytic' or 'fd', got {method!r}")
         return out
+
+    def set_nodes(self, prob: int, nodes) -> None:
+        """New (rho, z) nodes for a built problem (same count, same axis ends);
+        the build's quadrature rules and a captured graph stay valid."""
+        a = np.ascontiguousarray(np.asarray(nodes, dtype=np.float64))
+        if a.ndim != 2 or a.shape[1] != 2:
+            raise ValueError("nodes must be an (Nn, 2) array")
+        check(self.lib.ghost_bor_set_nodes(self._h, int(prob), dptr(a.ravel())))
 
     def set_objective(self, weights) -> None:
         """J = sum_c w_c sigma_dBsm(c): weights [problems][n_rhs] real, column

@@ -94,6 +94,15 @@ __global__ void k_objective_acoef(const double2* dF, int nmodes, int nrhs, const
   out[c] = make_double2(-ai * pr, ar * pr);
 }
 
+// dst32 = (float) src64 (an n x N block)
+__global__ void k_cast32(const VecOp* ops, const double2* src, float2* dst) {
+  const VecOp op = ops[blockIdx.z];
+  const int r = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= op.n || c >= op.N) return;
+  const double2 v = src[op.s + (size_t)r * op.lds + c];
+  dst[op.d + (size_t)r * op.ldd + c] = make_float2((float)v.x, (float)v.y);
+}
+
 // dst[perm row] (+)= (double) src[r] / scale[c]: the FP32 transposed solves'
 // output in the factor's row order, unscaled and put back in system order.
 __global__ void k_unpermute_widen(const VecOp* ops, const int* perm, const float2* src, const double* scale,

@@ -362,7 +362,9 @@ struct ghost_bor {
   double2* d_Xs = nullptr;           // forward solution, x layout
   double2* d_lam = nullptr;          // G, then z = P lambda (x layout)
   double2* d_lam2 = nullptr;         // lambda
-  double2* d_Q = nullptr;            // [-Lambda X^T | Lambda] in the augmented layout (FP64)
+  double2* d_Q = nullptr;            // [-Lambda X^T | Lambda] in the augmented layout (FP64; FP32 on
+                                     // mixed handles, where it is half the bytes)
+  float2* d_X32 = nullptr;           // mixed adjoint: FP32 copies of X and Lambda for the Q GEMM
   double2* d_rs = nullptr;           // row or column sums, per system at row_off
   double2* d_acoef = nullptr;        // a' per problem column
   double2* d_ffpart = nullptr;       // per-element far-field partial sums of one problem
@@ -633,6 +635,45 @@ extern "C" int ghost_bor_add_problem(ghost_bor* h, const double* nodes, int Nn, 
   }
   if (idx) *idx = (int)h->probs.size();
   h->probs.push_back(std::move(p));
+  return 0;
+}
+
+static int load_generatrix(Problem& p, const double* nodes, int Nn);
+static void geometry_tables(const double* nodes, int Ne, std::vector<double>& pts, std::vector<double>& elem);
+
+// New node coordinates for a built problem (same node count, the same ends
+// on the axis): the point, node and element tables are recomputed and copied
+// into the problem's device buffers, so a built handle, and a graph captured
+// on it, serve a family of shapes without reallocating.  The quadrature
+// rules chosen at build stay (n_xi, each near pair's tail order), as in the
+// adjoint's derivative.  Plain BoR problems only (no impedance, no medium, no
+// penetrable or multi-region surfaces).
+extern "C" int ghost_bor_set_nodes(ghost_bor* h, int prob, const double* nodes) {
+  if (!h || !nodes) return fail(GHOST_ERR_ARG, "null argument");
+  if (!h->built) return fail(GHOST_ERR_STATE, "build first");
+  if (prob < 0 || prob >= (int)h->probs.size()) return fail(GHOST_ERR_ARG, "bad problem index %d", prob);
+  Problem& p = h->probs[prob];
+  if (p.pen || p.standalone || p.has_zs || p.ki != 0.0 || p.sys0 < 0)
+    return fail(GHOST_ERR_ARG, "set_nodes covers plain conductor problems only");
+  Problem q;
+  q.k = p.k;
+  int rc;
+  if ((rc = load_generatrix(q, nodes, p.Nn))) return rc;
+  if (q.axis0 != p.axis0 || q.axis1 != p.axis1)
+    return fail(GHOST_ERR_ARG, "the generatrix ends must stay on (or off) the axis as built");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  std::vector<double> pts, elem;
+  geometry_tables(q.nodes.data(), q.Ne, pts, elem);
+  HIPCHK(hipMemcpy(p.d_pts, pts.data(), pts.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_nodes, q.nodes.data(), q.nodes.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p.d_elem, elem.data(), elem.size() * 8, hipMemcpyHostToDevice));
+  p.nodes = q.nodes;
+  p.pts = pts;
+  p.L = q.L;
+  p.trho = q.trho;
+  p.tz = q.tz;
+  h->solved = false;
   return 0;
 }
 
@@ -3402,7 +3443,9 @@ static int adjoint_alloc(ghost_bor* h) {
   if ((rc = dalloc(h, &h->d_Xs, h->x_elems * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &h->d_lam, h->x_elems * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &h->d_lam2, h->x_elems * sizeof(double2)))) return rc;
-  if ((rc = dalloc(h, &h->d_Q, h->a_elems * sizeof(double2)))) return rc;
+  const bool mixed = h->precision == GHOST_MIXED;
+  if ((rc = dalloc(h, &h->d_Q, h->a_elems * (mixed ? sizeof(float2) : sizeof(double2))))) return rc;
+  if (mixed && (rc = dalloc(h, &h->d_X32, h->x_elems * sizeof(float2)))) return rc;
   if ((rc = dalloc(h, &h->d_rs, nrow * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &h->d_acoef, ncol * sizeof(double2)))) return rc;
   if ((rc = dalloc(h, &h->d_ffpart, ffmax * sizeof(double2)))) return rc;
@@ -3545,10 +3588,24 @@ static int adjoint_lambda_q(ghost_bor* h, const std::vector<size_t>& coff) {
     }
   }
   // Q = [-Lambda X^T | Lambda]
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (h->a_elems + 255) / 256)), dim3(256), 0,
-                     h->stream, h->d_Q, h->a_elems);
-  gemm_launch<double, false, true>(h, h->d_gemm + h->adj_q_ops, nsys, h->adj_q_tt, h->d_lam2, h->d_Xs, h->d_Q);
-  hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_toaug, h->d_lam2, h->d_Q);
+  if (!mixed) {
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (h->a_elems + 255) / 256)), dim3(256), 0,
+                       h->stream, h->d_Q, h->a_elems);
+    gemm_launch<double, false, true>(h, h->d_gemm + h->adj_q_ops, nsys, h->adj_q_tt, h->d_lam2, h->d_Xs, h->d_Q);
+    hipLaunchKernelGGL(k_copy64, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_toaug, h->d_lam2, h->d_Q);
+  } else {
+    // FP32 Q (half the bytes of the FP64 systems it pairs with): Lambda and X
+    // cast to FP32, Q = -Lambda X^T on the FP32 GEMM, Lambda into the
+    // augmented columns
+    const size_t qz = (h->a_elems + 1) / 2;   // float2 pairs as double2
+    hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>(65535, (qz + 255) / 256)), dim3(256), 0, h->stream,
+                       h->d_Q, qz);
+    hipLaunchKernelGGL(k_cast32, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_lam2, h->d_Y32);
+    hipLaunchKernelGGL(k_cast32, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_all, h->d_Xs, h->d_X32);
+    gemm_launch<float, false, true>(h, h->d_gemm + h->adj_q_ops, nsys, h->adj_q_tt, h->d_Y32, h->d_X32,
+                                    (float2*)h->d_Q);
+    hipLaunchKernelGGL(k_cast32, gv, dim3(256), 0, h->stream, h->d_vec + h->vec_toaug, h->d_lam2, (float2*)h->d_Q);
+  }
   LAUNCHCHK();
   return 0;
 }
@@ -3666,6 +3723,11 @@ extern "C" int ghost_bor_adjoint(ghost_bor* h, const double* acoef, double step,
   return 0;
 }
 
+static QView adj_qview(ghost_bor* h) {
+  if (h->precision == GHOST_MIXED) return QView{nullptr, (const float2*)h->d_Q};
+  return QView{h->d_Q, nullptr};
+}
+
 // Analytic form: the fill's shape derivative contracted with Q in the kernels
 // of dfill.hip (far pairs, near pairs with the mass term, right-hand sides
 // and far-field projection), gathered per node into d_adj_grad (every
@@ -3685,7 +3747,7 @@ static int adjoint_analytic_kernels(ghost_bor* h, const std::vector<size_t>& cof
       fa.gs = gauss;
       fa.gwq = gauss + 4;
       fa.sys = sys;
-      fa.Q = h->d_Q;
+      fa.Q = adj_qview(h);
       fa.Ne = p.Ne; fa.M = p.M; fa.nE = p.nE; fa.nB = p.nB;
       fa.k = p.k;
       fa.ce_re = 0.0;
@@ -3713,7 +3775,7 @@ static int adjoint_analytic_kernels(ghost_bor* h, const std::vector<size_t>& cof
       na.pairs = reinterpret_cast<const AdjNearPair*>(p.d_pairs);
       na.batches = p.d_batches;
       na.sys = sys;
-      na.Q = h->d_Q;
+      na.Q = adj_qview(h);
       na.gs = gauss;
       na.gwq = gauss + 4;
       na.Ne = p.Ne; na.M = p.M;
@@ -3734,7 +3796,7 @@ static int adjoint_analytic_kernels(ghost_bor* h, const std::vector<size_t>& cof
       ea.gs = gauss;
       ea.gwq = gauss + 4;
       ea.sys = sys;
-      ea.Q = h->d_Q;
+      ea.Q = adj_qview(h);
       ea.X = h->d_Xs;
       ea.acoef = h->d_acoef + coff[pi];
       ea.J = p.d_J;
@@ -3748,7 +3810,7 @@ static int adjoint_analytic_kernels(ghost_bor* h, const std::vector<size_t>& cof
     }
     double rmax = 0.0;
     for (int q = 0; q < p.Nn; ++q) rmax = std::max(rmax, p.nodes[2 * q]);
-    hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)((p.Nn + 127) / 128)), dim3(128), 0, h->stream, h->d_adj_far,
+    hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)p.Nn), dim3(256), 0, h->stream, h->d_adj_far,
                        h->d_adj_near, p.d_batches, reinterpret_cast<const AdjNearPair*>(p.d_pairs),
                        (int)p.batches.size(), h->d_adj_ex, p.Ne, p.ntheta, p.d_nodes,
                        1e-12 * std::max(1.0, rmax), h->d_adj_grad + goff);

@@ -174,6 +174,18 @@ __device__ __forceinline__ DPoint dpoint(const DElem& E, double s) {
 // Q blocks of an element pair and their azimuth series
 // ─────────────────────────────────────────────────────────────────────────────
 
+// The adjoint weights Q in the augmented system layout: FP64 (FP64 handles)
+// or FP32 (mixed handles, whose Q comes from the FP32 GEMM).
+struct QView {
+  const double2* q64;
+  const float2* q32;
+  __device__ __forceinline__ double2 operator[](size_t i) const {
+    if (q64) return q64[i];
+    const float2 v = q32[i];
+    return make_double2((double)v.x, (double)v.y);
+  }
+};
+
 // Row of node a's basis of component c (0: J_t, 1: J_phi) in a system, or -1.
 __device__ __forceinline__ int sys_row(const SysInfo& s, int c, int a) {
   if (c == 0) { const int r = a - s.t0; return (r >= 0 && r < s.nt) ? r : -1; }
@@ -188,7 +200,7 @@ __device__ __forceinline__ int sys_row(const SysInfo& s, int c, int a) {
 // Then sum_m Q_m cos(m xi) = sum_m P+_m cos(m xi), sum_m Q_m sin(m xi) =
 // sum_m P-_m sin(m xi), sum_m m Q_m cos(m xi) = sum_m m P-_m cos(m xi) and
 // sum_m m^2 Q_m cos(m xi) = sum_m m^2 P+_m cos(m xi).
-__device__ void stage_q(const SysInfo* sys, const double2* Q, int M, int e, int f, double2* Pq) {
+__device__ void stage_q(const SysInfo* sys, QView Q, int M, int e, int f, double2* Pq) {
   const int mp1 = M + 1;
   for (int t = threadIdx.x; t < 16 * mp1; t += blockDim.x) {
     const int ent = t / mp1, m = t % mp1;
@@ -362,7 +374,7 @@ struct AdjFarArgs {
   const double* gs;      // Gauss points s on [0, 1] [4]
   const double* gwq;     // their weights [4]
   const SysInfo* sys;    // the problem's 2M+1 systems, mode m at index M + m
-  const double2* Q;
+  QView Q;
   int Ne, M, nE, nB;
   double k;
   double ce_re, ce_im;   // EFIE coefficient (cE j k eta0 2 pi)
@@ -411,11 +423,18 @@ __global__ __launch_bounds__(256) void k_adj_far(AdjFarArgs a) {
   double2* Pq = lds;                                   // [16][M+1]
   double2* Fx = lds + 16 * (a.M + 1);                  // [ADJ_CHUNK][7][4]
   stage_q(a.sys, a.Q, a.M, e, f, Pq);
-  const DElem Ee = delem(a.nodes, e, 0), Ef = delem(a.nodes, f, 2);
   const int tid = threadIdx.x;
-  const int pp = tid & 15, pi = pp >> 2, qi = pp & 3;
-  const DPoint p = dpoint(Ee, a.gs[pi]), q = dpoint(Ef, a.gs[qi]);
-  const DPair P = make_pair(Ee, Ef, p, q, a.gwq[pi] * a.gwq[qi]);
+  const int pp = tid & 15;
+  // the 16 point pairs' duals live in LDS (read per sample), which keeps
+  // them out of the registers the sample loops need
+  __shared__ DPair sP[16];
+  if (tid < 16) {
+    const DElem Ee = delem(a.nodes, e, 0), Ef = delem(a.nodes, f, 2);
+    const int pi = tid >> 2, qi = tid & 3;
+    const DPoint p = dpoint(Ee, a.gs[pi]), q = dpoint(Ef, a.gs[qi]);
+    sP[tid] = make_pair(Ee, Ef, p, q, a.gwq[pi] * a.gwq[qi]);
+  }
+  const DPair& P = sP[pp];
   const double k = a.k, ik2 = 1.0 / (k * k);
   C4 accE = cdz();
   D4 accK = dconst(0.0);
@@ -506,7 +525,7 @@ struct AdjNearArgs {
   const AdjNearPair* pairs;
   const int2* batches;    // (pair slot, first cell point) per work-group of 32 point pairs
   const SysInfo* sys;
-  const double2* Q;
+  QView Q;
   const double* gs;       // element Gauss rule (mass term)
   const double* gwq;
   int Ne, M;
@@ -566,20 +585,44 @@ __global__ __launch_bounds__(256) void k_adj_near(AdjNearArgs a) {
     const int cnt = a.cell_cnt[pr.kind];
     const double k = a.k, ik2 = 1.0 / (k * k);
     const DElem Ee = delem(a.nodes, pr.e, 0), Ef = delem(a.nodes, pr.f, 2);
-    if (q < cnt) {
+    // the 32 point pairs' duals and their azimuth rules in LDS (written by
+    // lane 0 of each slot, read by its 8 node lanes): held in registers across
+    // the node loop they would crowd out the series accumulators
+    struct DRule {
+      D4 d, aa, doa, vmax, xi0, span;
+      int ok;
+    };
+    __shared__ DPair sP[32];
+    __shared__ DRule sR[32];
+    if (cs == 0 && q < cnt) {
       const int qi = a.cell_off[pr.kind] + q;
       const DPoint p = dpoint(Ee, a.cell_s[qi]), qq = dpoint(Ef, a.cell_sp[qi]);
       const DPair P = make_pair(Ee, Ef, p, qq, a.cell_w[qi]);
+      sP[slot] = P;
       const D4 dr = P.rp - P.rq;
       const D4 d2 = dr * dr + P.dz * P.dz;
       const D4 rr4 = 4.0 * (P.rp * P.rq);
-      if (rr4.v > 1e-30) {   // axis pairs (closed form, order 0 only) do not occur off the axis
-        const D4 d = dsqrt(dmax(d2, 1e-300)), aa = dsqrt(dmax(rr4, 1e-300));
-        const D4 doa = d / aa;
-        const D4 s0 = dmin(20.0 * doa, 0.25);
-        const D4 vmax = dasinh(s0 / doa);
-        const D4 xi0 = 2.0 * dasin(dmin(s0, 1.0));
-        const D4 span = GH_PI - xi0;
+      DRule r;
+      r.ok = rr4.v > 1e-30;   // axis pairs (closed form, order 0 only) do not occur off the axis
+      r.d = dsqrt(dmax(d2, 1e-300));
+      r.aa = dsqrt(dmax(rr4, 1e-300));
+      r.doa = r.d / r.aa;
+      const D4 s0 = dmin(20.0 * r.doa, 0.25);
+      r.vmax = dasinh(s0 / r.doa);
+      r.xi0 = 2.0 * dasin(dmin(s0, 1.0));
+      r.span = GH_PI - r.xi0;
+      sR[slot] = r;
+    }
+    __syncthreads();
+    if (q < cnt && sR[slot].ok) {
+      const DPair& P = sP[slot];
+      {
+        const D4& d = sR[slot].d;
+        const D4& aa = sR[slot].aa;
+        const D4& doa = sR[slot].doa;
+        const D4& vmax = sR[slot].vmax;
+        const D4& xi0 = sR[slot].xi0;
+        const D4& span = sR[slot].span;
         const int nnodes = 48 + pr.n_tail;
         C4 accE = cdz();
         D4 accK = dconst(0.0);
@@ -627,6 +670,7 @@ __global__ __launch_bounds__(256) void k_adj_near(AdjNearArgs a) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) xd[i][j] = xv[i];
           }
+#pragma unroll 1
           for (int ab = 0; ab < 4; ++ab) {
             const int x = ab >> 1, y = ab & 1;
             double2 F[7], dF[7];
@@ -728,7 +772,7 @@ struct AdjExciteArgs {
   const double* gs;
   const double* gwq;
   const SysInfo* sys;     // the problem's systems (mode m at M + m)
-  const double2* Q;       // B part at columns [n, n + nrhs)
+  QView Q;                // B part at columns [n, n + nrhs)
   const double2* X;       // solutions: s.x_off + row * s.ncols + col
   const double2* acoef;   // a' per column (far-field prefactor included)
   const double* J;        // Bessel table [M+2][P][ntheta]
@@ -865,19 +909,19 @@ __global__ __launch_bounds__(128) void k_adj_excite(AdjExciteArgs a) {
 
 // grad[q] = the far pairs' partials of node q (test slot 0 of element q, slot
 // 1 of element q - 1; source slots 2 and 3 likewise), the near batches' and
-// the excitation's; zero for nodes on the axis.
-__global__ void k_adj_gather(const double* far, const double* nearb, const int2* batches, const AdjNearPair* pairs,
-                             int nbatch, const double* ex, int Ne, int ntheta, const double* nodes, double axis_tol,
-                             double* grad) {
-  const int qn = blockIdx.x * blockDim.x + threadIdx.x;
-  const int Nn = Ne + 1;
-  if (qn >= Nn) return;
+// the excitation's; zero for nodes on the axis.  One 256-thread work-group
+// per node: strided partial sums, then a fixed-order tree (deterministic).
+__global__ __launch_bounds__(256) void k_adj_gather(const double* far, const double* nearb, const int2* batches,
+                                                    const AdjNearPair* pairs, int nbatch, const double* ex, int Ne,
+                                                    int ntheta, const double* nodes, double axis_tol, double* grad) {
+  const int qn = blockIdx.x;
+  const int t = threadIdx.x;
   double g = 0.0;
-  for (int o = 0; o < Ne; ++o) {
+  for (int o = t; o < Ne; o += 256) {
     if (qn < Ne) g += far[((size_t)qn * Ne + o) * 4 + 0] + far[((size_t)o * Ne + qn) * 4 + 2];
     if (qn >= 1) g += far[((size_t)(qn - 1) * Ne + o) * 4 + 1] + far[((size_t)o * Ne + qn - 1) * 4 + 3];
   }
-  for (int b = 0; b < nbatch; ++b) {
+  for (int b = t; b < nbatch; b += 256) {
     const AdjNearPair pr = pairs[batches[b].x];
     if (pr.e < 0) continue;
     const double* v = nearb + (size_t)b * 4;
@@ -886,11 +930,18 @@ __global__ void k_adj_gather(const double* far, const double* nearb, const int2*
     if (pr.f == qn) g += v[2];
     if (pr.f + 1 == qn) g += v[3];
   }
-  for (int it = 0; it < ntheta; ++it) {
+  for (int it = t; it < ntheta; it += 256) {
     if (qn < Ne) g += ex[((size_t)qn * ntheta + it) * 2 + 0];
     if (qn >= 1) g += ex[((size_t)(qn - 1) * ntheta + it) * 2 + 1];
   }
-  grad[qn] = nodes[2 * qn] > axis_tol ? g : 0.0;
+  __shared__ double red[256];
+  red[t] = g;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) grad[qn] = nodes[2 * qn] > axis_tol ? red[0] : 0.0;
 }
 
 }  // namespace ghost

@@ -152,7 +152,7 @@ def perturbed_batch(n_shapes: int = 32, n_coef: int = 16, amplitude: float = 0.0
     ogive-cylinder-cone body (wavelength 1 m at f0 = c / 1 m), each at the
     frequencies f0 x freq_scales with lambda / 10 meshing (600, 1200, 1800
     elements) and m_max = base_m x scale; n_theta aspects x 2 pols.  Returns a
-    list of (shape index, nodes, freq_hz, thetas_deg, m_max)."""
+    list of (shape index, nodes, freq_hz, thetas_deg, m_max), shape-major."""
     rng = np.random.default_rng(seed)
     coefs = [rng.uniform(-1.0, 1.0, n_coef) / np.arange(1, n_coef + 1) for _ in range(n_shapes)]
     thetas = np.linspace(0.0, 180.0, n_theta)

@@ -277,12 +277,13 @@ def run_adjoint(steps):
 def batch_bytes(nodes, m_max, n_theta):
     """Device bytes of one problem in a mixed-precision handle with mode
     pairing off and the adjoint's buffers: FP32 augmented systems, their FP64
-    copy, Q, the FP64 n x n_rhs blocks, far blocks and near point-pair values."""
+    copy, the FP32 Q, the FP64 n x n_rhs blocks, far blocks and near
+    point-pair values."""
     Nn = len(nodes)
     n, nsys, nrhs = 2 * Nn, 2 * m_max + 1, 2 * n_theta
     ld = n + nrhs
     zmodes = 16 * ((nsys + 15) // 16)
-    total = (nsys * n * ld * (8 + 16) + nsys * n * n * 16 + 8 * nsys * n * nrhs * 16
+    total = (nsys * n * ld * (8 + 8) + nsys * n * n * 16 + 8 * nsys * n * nrhs * 16
              + (2 * Nn) ** 2 * zmodes * 8 + 1440 * (Nn - 1) * (m_max + 2) * 80)
     return 1.1 * total
 
@@ -292,67 +293,70 @@ def run_shape_batch(n_shapes, budget_gb):
     every node radius for n_shapes Fourier perturbations of the ogive body at
     three frequencies (600 / 1200 / 1800 elements, |m| <= 35 / 70 / 105), mixed
     precision (FP32 fill and LU, one FP64 refinement sweep for the currents and
-    one for lambda), each batch of problems (grouped under the memory budget)
-    captured as ONE graph: fill + solve + far field + a' + transposed solve + Q
-    + analytic fill derivative.  Prints its own JSON line."""
+    one for lambda).  Per frequency one handle holds as many shapes as fit the
+    memory budget and ONE captured graph (fill + solve + far field + a' +
+    transposed solve + Q + analytic fill derivative) serves all its shapes in
+    turn: ghost_bor_set_nodes swaps the geometry in place and every graph
+    launch returns the forward RCS and the gradients of a group of shapes.
+    The quadrature rules (n_xi, near tail orders) are those of the handle's
+    first shapes.  Prints its own JSON line."""
     wl = pkg("workloads")
     bs = pkg("bor_solver")
     L = pkg("_lib")
     probs = wl.perturbed_batch(n_shapes)
-    specs = []
-    for (i, nodes, f, th, mm) in probs:
+    scales = sorted({round(f / bs.C0, 6) for (_, _, f, _, _) in probs})
+    by_scale = {sc: [p for p in probs if round(p[2] / bs.C0, 6) == sc] for sc in scales}
+    solves = sum((2 * mm + 1) * len(th) for (_, _, _, th, mm) in probs)
+    t_gpu, t_wall, rows, n_launch = 0.0, time.perf_counter(), [], 0
+    results = {}
+    log(f"shape batch: {n_shapes} shapes x {len(scales)} frequencies = {len(probs)} problems, {solves} solves")
+    for sc in scales:
+        group = by_scale[sc]
+        _, nodes0, f, th, mm = group[0]
         k = 2 * math.pi * f / bs.C0
-        _, ne, nb, _ = bs.discretisation(nodes, k, th, mm, "cfie")
-        specs.append((i, nodes, k, mm, ne, nb, th))
-    batches, cur, cur_b = [], [], 0.0
-    for sp in specs:
-        b = batch_bytes(sp[1], sp[3], len(sp[6]))
-        if cur and cur_b + b > budget_gb * 1e9:
-            batches.append(cur)
-            cur, cur_b = [], 0.0
-        cur.append(sp)
-        cur_b += b
-    if cur:
-        batches.append(cur)
-    solves = sum((2 * sp[3] + 1) * len(sp[6]) for sp in specs)
-    log(f"shape batch: {n_shapes} shapes x 3 frequencies = {len(specs)} problems, {solves} solves, "
-        f"{len(batches)} graph launches")
-    t_gpu, t_wall, rows = 0.0, time.perf_counter(), []
-    grads = []
-    for bi, batch in enumerate(batches):
+        _, ne, nb, _ = bs.discretisation(nodes0, k, th, mm, "cfie")
+        slots = max(1, min(len(group), int(budget_gb * 1e9 // batch_bytes(nodes0, mm, len(th)))))
         t0 = time.perf_counter()
         h = L.Handle(L.GHOST_MIXED, nb=64, residual=False, pairing=False)
-        for (_, nodes, k, mm, ne, nb, th) in batch:
-            h.add_problem(nodes, k, mm, ne, nb, 0.5, 0.5 * bs.ETA0, th)
+        for j in range(slots):
+            h.add_problem(group[j][1], k, mm, ne, nb, 0.5, 0.5 * bs.ETA0, th)
         h.build()
-        h.set_objective(np.ones(sum(2 * len(sp[6]) for sp in batch)))
+        h.set_objective(np.ones(2 * len(th) * slots))
         h.capture_adjoint(1)
-        t1 = time.perf_counter()
-        h.launch()
-        h.sync()
-        ms = h.graph_ms()
-        g = h.gradient()
-        J = [float(np.sum(10 * np.log10(4 * math.pi * np.abs(h.dF(i).sum(axis=0)) ** 2))) for i in range(len(batch))]
+        t_cap = time.perf_counter() - t0
+        for c0 in range(0, len(group), slots):
+            chunk = group[c0:c0 + slots]
+            if c0 > 0:
+                for j, (_, nodes, _, _, _) in enumerate(chunk):
+                    h.set_nodes(j, nodes)
+            h.launch()
+            h.sync()
+            ms = h.graph_ms()
+            g = h.gradient()
+            n_launch += 1
+            t_gpu += ms * 1e-3
+            off = 0
+            for j, (si, nodes, _, _, _) in enumerate(chunk):
+                F = h.dF(j).sum(axis=0)
+                results[(si, sc)] = (float(np.sum(10 * np.log10(4 * math.pi * np.abs(F) ** 2))),
+                                     g[off:off + len(nodes)], (ne, nb))
+                off += len(nodes)
+            rows.append({"scale": sc, "elements": len(nodes0) - 1, "shapes": len(chunk), "graph_ms": round(ms, 1)})
+            log(f"shape batch: {sc} f0 ({len(nodes0) - 1} elements), shapes {c0}-{c0 + len(chunk) - 1}: "
+                f"graph {ms:.1f} ms")
         h.close()
-        t_gpu += ms * 1e-3
-        off = 0
-        for (sp, j) in zip(batch, J):
-            grads.append((sp, j, g[off:off + len(sp[1])]))
-            off += len(sp[1])
-        rows.append({"problems": [f"shape{sp[0]}:{len(sp[1]) - 1}el" for sp in batch], "graph_ms": round(ms, 1),
-                     "build_capture_s": round(t1 - t0, 2)})
-        log(f"shape batch {bi + 1}/{len(batches)}: {len(batch)} problems, graph {ms:.1f} ms, "
-            f"build + capture {t1 - t0:.1f} s")
+        log(f"shape batch: {sc} f0 handle of {slots} slots, build + capture {t_cap:.1f} s")
     t_wall = time.perf_counter() - t_wall
     # check: the smallest problem of shape 0 against its FP64 analytic gradient and
-    # central differences of the FP64 forward on two nodes
-    sp, J0, g0 = grads[0]
-    _, nodes, k, mm, ne, nb, th = sp
+    # central differences of the FP64 forward on two nodes (same quadrature rules)
+    sc0 = scales[0]
+    _, nodes, f, th, mm = by_scale[sc0][0]
+    J0, g0, (ne, nb) = results[(0, sc0)]
     ad = pkg("bor_adjoint")
-    spec = ad.RcsSpec(nodes, k * bs.C0 / (2 * math.pi), th, mm, cfie_alpha=0.5, n_xi=(ne, nb))
+    spec = ad.RcsSpec(nodes, f, th, mm, cfie_alpha=0.5, n_xi=(ne, nb))
     J64, g64 = ad.objective_and_gradient(spec)
     rel64 = float(np.max(np.abs(g0 - g64)) / np.max(np.abs(g64)))
-    lam = 2 * math.pi / k
+    lam = bs.C0 / f
     fd_err = []
     for q in (len(nodes) // 3, 2 * len(nodes) // 3):
         vals = []
@@ -368,16 +372,17 @@ def run_shape_batch(n_shapes, budget_gb):
     out = {
         "metric": "forward RCS + gradient (J = sum sigma_dBsm) of perturbed shapes, batched graph launches",
         "config": {"workload": f"ogive-1200seg-fourier16-perturbed-x{n_shapes}-3freq", "shapes": n_shapes,
-                   "frequencies": [0.5, 1.0, 1.5], "elements": [600, 1200, 1800], "m_max": [35, 70, 105],
-                   "aspects": len(specs[0][6]), "polarizations": 2, "precision": "mixed (FP32 fill/LU, FP64 "
-                   "refinement: 1 sweep currents, 1 sweep lambda)", "problems": len(specs),
-                   "graph_launches": len(batches), "budget_gb": budget_gb},
+                   "frequencies": scales, "elements": [len(by_scale[sc][0][1]) - 1 for sc in scales],
+                   "m_max": [by_scale[sc][0][4] for sc in scales], "aspects": len(probs[0][3]),
+                   "polarizations": 2,
+                   "precision": "mixed (FP32 fill/LU, FP64 refinement: 1 sweep currents, 1 sweep lambda)",
+                   "problems": len(probs), "graph_launches": n_launch, "budget_gb": budget_gb},
         "gpu_s": round(t_gpu, 3), "wall_s": round(t_wall, 1),
         "solves_per_s_gpu": round(solves / t_gpu, 1),
-        "gradients_per_s_gpu": round(len(specs) / t_gpu, 3),
+        "gradients_per_s_gpu": round(len(probs) / t_gpu, 3),
         "check_vs_fp64_max_rel": rel64, "check_vs_central_differences_max_rel": float(max(fd_err)),
         "J_shape0_vs_fp64": [J0, J64],
-        "batches": rows,
+        "launches": rows,
     }
     print(json.dumps(out), flush=True)
 

@@ -314,6 +314,13 @@ int ghost_bor_adjoint_analytic(ghost_bor* h, const double* acoef, double* grad);
  * (replayed by ghost_bor_launch).  get_gradient: dJ/d rho of every node of
  * every problem (problem order) after the last analytic adjoint, eager or
  * captured.  FP64 or GHOST_MIXED handles with mode pairing off. */
+/* New node coordinates [Nn][2] for built problem prob (same node count, the
+ * same ends on the axis; no counterpart in the reference): the point, node
+ * and element tables are recomputed in place, so one built handle, and a
+ * graph captured on it, serve a family of shapes without reallocating.  The
+ * build's quadrature rules stay (n_xi, the near pairs' tail orders).  Plain
+ * conductor problems only. */
+int ghost_bor_set_nodes(ghost_bor* h, int prob, const double* nodes);
 int ghost_bor_set_objective(ghost_bor* h, const double* weights);
 int ghost_bor_capture_adjoint(ghost_bor* h, int refine_iters);
 int ghost_bor_get_gradient(ghost_bor* h, double* grad);
