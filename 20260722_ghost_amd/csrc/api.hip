@@ -754,8 +754,8 @@ extern "C" int ghost_bor_set_interior(ghost_bor* h, int prob, double k_re, doubl
     return fail(GHOST_ERR_ARG, "far fill: interior azimuth tables of n_xi %d / %d exceed the 160 KiB of LDS",
                 n_xi_efie, n_xi_bracket);
   for (int m = -p.M; m <= p.M; ++m)
-    if (2 * h->sys[p.sys0 + m + p.M].n > 4096)
-      return fail(GHOST_ERR_ARG, "PMCHWT system size %d exceeds the panel limit 4096", 2 * h->sys[p.sys0 + m + p.M].n);
+    if (2 * h->sys[p.sys0 + m + p.M].n > 8192)
+      return fail(GHOST_ERR_ARG, "PMCHWT system size %d exceeds the panel limit 8192", 2 * h->sys[p.sys0 + m + p.M].n);
   // the bracket operators of both media at unit weight, no mass term
   p.zs.assign(p.Ne, make_double2(1.0, 0.0));
   p.has_zs = true;
@@ -1019,7 +1019,7 @@ extern "C" int ghost_bor_region_map(ghost_bor* h, int set, int m, int n_red, con
   if (set < 0 || set >= (int)h->regions.size()) return fail(GHOST_ERR_ARG, "bad region set");
   Region& r = h->regions[set];
   if (m < -r.M || m > r.M) return fail(GHOST_ERR_ARG, "mode outside range");
-  if (n_red < 1 || n_red > 4096) return fail(GHOST_ERR_ARG, "reduced size %d outside [1, 4096]", n_red);
+  if (n_red < 1 || n_red > 8192) return fail(GHOST_ERR_ARG, "reduced size %d outside [1, 8192]", n_red);
   if (!red_of_full || !coef) return fail(GHOST_ERR_ARG, "null map");
   std::vector<int> used(n_red, 0);
   for (int i = 0; i < r.n_full; ++i) {
@@ -1066,7 +1066,7 @@ extern "C" int ghost_bor_add_dense(ghost_bor* h, int n, int nrhs, int* sys_idx) 
   if (!h) return fail(GHOST_ERR_ARG, "null handle");
   if (h->built) return fail(GHOST_ERR_STATE, "add systems before build");
   if (n <= 0 || nrhs <= 0) return fail(GHOST_ERR_ARG, "n and nrhs must be positive");
-  if (n > 4096) return fail(GHOST_ERR_ARG, "system size %d exceeds the panel limit 4096", n);
+  if (n > 8192) return fail(GHOST_ERR_ARG, "system size %d exceeds the panel limit 8192", n);
   SysInfo s;
   memset(&s, 0, sizeof(s));
   s.prob = -1;
@@ -1300,7 +1300,7 @@ static int build_schedule(ghost_bor* h) {
   int maxn = 0;
   for (auto& s : h->ss) maxn = std::max(maxn, s.n);
   h->rpt = (maxn + 1023) / 1024;
-  if (h->rpt > 4) return fail(GHOST_ERR_ARG, "system size %d exceeds the panel limit 4096", maxn);
+  if (h->rpt > 8) return fail(GHOST_ERR_ARG, "system size %d exceeds the panel limit 8192", maxn);
   // pivot offsets
   int poff = 0;
   std::vector<int> pofs(nsys);
@@ -2578,7 +2578,13 @@ template <typename T> static int run_sched(ghost_bor* h, const std::vector<Launc
           case 1: hipLaunchKernelGGL((k_lu_panel<T, 1, 8>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
           case 2: hipLaunchKernelGGL((k_lu_panel<T, 2, sizeof(T) == 8 ? 4 : 8>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
           case 3: hipLaunchKernelGGL((k_lu_panel<T, 3, sizeof(T) == 8 ? 4 : 8>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
-          default: hipLaunchKernelGGL((k_lu_panel<T, 4, 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          case 4: hipLaunchKernelGGL((k_lu_panel<T, 4, 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          // 4097 .. 8192 rows: 5 .. 8 rows per thread with 4-column (FP32) or
+          // 2-column (FP64) register sub-panels, within 1024 threads' registers
+          case 5: hipLaunchKernelGGL((k_lu_panel<T, 5, sizeof(T) == 8 ? 2 : 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          case 6: hipLaunchKernelGGL((k_lu_panel<T, 6, sizeof(T) == 8 ? 2 : 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          case 7: hipLaunchKernelGGL((k_lu_panel<T, 7, sizeof(T) == 8 ? 2 : 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
+          default: hipLaunchKernelGGL((k_lu_panel<T, 8, sizeof(T) == 8 ? 2 : 4>), dim3(L.nops), dim3(1024), 0, h->stream, h->d_panel + L.ops, factor, h->d_piv, h->d_info); break;
         }
         break;
       case L_SWAP:

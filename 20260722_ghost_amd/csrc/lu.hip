@@ -1164,8 +1164,8 @@ __global__ __launch_bounds__(256) void k_perm_from_pivots(const VecOp* ops, int 
   const int s = blockIdx.x;
   if (s >= nsys) return;
   const VecOp op = ops[s];
-  if (op.n > 4096) return;   // the build rejects such systems
-  __shared__ int p[4096];
+  if (op.n > 8192) return;   // the build rejects such systems
+  __shared__ int p[8192];
   for (int i = threadIdx.x; i < op.n; i += blockDim.x) p[i] = i;
   __syncthreads();
   if (threadIdx.x == 0) {

@@ -177,7 +177,7 @@ int ghost_bor_region_block(ghost_bor* h, int set, int src, int idx, int which, d
                            int row_off, int col_off);
 /* Junction map of mode m (_MultiRegionBor.build_Q, bor_solver.py:1908-1986):
  * full index i -> reduced index red_of_full[i] (-1: dropped) with real
- * coefficient coef[i]; the mode's system is Q^T A Q (n_red <= 4096). */
+ * coefficient coef[i]; the mode's system is Q^T A Q (n_red <= 8192). */
 int ghost_bor_region_map(ghost_bor* h, int set, int m, int n_red, const int* red_of_full, const double* coef);
 /* A free-space operator surface bounding the exterior: its J rows start at
  * off_J of the full system and, off_M >= 0, its M / eta0 rows follow at
