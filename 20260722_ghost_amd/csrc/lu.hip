@@ -731,6 +731,79 @@ template <int SW> __device__ __forceinline__ void store_seg(double2* p, const do
   for (int c = 0; c < SW; ++c) p[c] = v[c];
 }
 
+// Block step of the two-level recursive panel: columns [k0, k0 + KB) are
+// factored (rows [k0, n) hold L, row-interchanged like every panel column),
+// columns [k0 + KB, k0 + KB + TW) are not yet touched by them.  U = L11^-1 A12
+// for rows [k0, k0 + KB) of the target columns (unit lower L11, one thread
+// per column, in LDS), then the rank-KB update A22 -= L21 U of rows
+// [k0 + KB, n) on the 16x16x4 MFMA: 16 wavefronts walk the 16 x 16 tiles, each
+// complex product four real MFMAs (Re A Re B, Im A Im B, and both cross terms
+// into one accumulator), K = KB in steps of 4.  One barrier-separated step of
+// the 1024-thread panel work-group; it replaces the VALU rank-SW updates of
+// the target columns by the factored block's sub-panels.
+template <typename T, int KB, int TW>
+__device__ void panel_block_update(typename CT<T>::c* A, int n, int ld, int k0, int tid) {
+  typedef typename CT<T>::c C;
+  typedef typename CT<T>::v4 V4;
+  __shared__ C Lb[KB][KB + 1];
+  __shared__ C Ub[KB][TW + 1];
+  for (int i = tid; i < KB * KB; i += 1024) {
+    const int r = i / KB, c = i % KB;
+    Lb[r][c] = A[(size_t)(k0 + r) * ld + k0 + c];
+  }
+  for (int i = tid; i < KB * TW; i += 1024) {
+    const int r = i / TW, c = i % TW;
+    Ub[r][c] = A[(size_t)(k0 + r) * ld + k0 + KB + c];
+  }
+  __syncthreads();
+  if (tid < TW) {
+    // forward substitution down column tid of U (L11 unit lower)
+    for (int r = 1; r < KB; ++r) {
+      C z = Ub[r][tid];
+      for (int q = 0; q < r; ++q) z = csub(z, cmul(Lb[r][q], Ub[q][tid]));
+      Ub[r][tid] = z;
+    }
+    for (int r = 0; r < KB; ++r) A[(size_t)(k0 + r) * ld + k0 + KB + tid] = Ub[r][tid];
+  }
+  __syncthreads();
+  const int lane = tid & 63, wv = tid >> 6;
+  const int rows = n - (k0 + KB);
+  constexpr int CT_ = TW / 16;
+  const int ntile = ((rows + 15) >> 4) * CT_;
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int t = wv; t < ntile; t += 16) {
+    const int r0 = k0 + KB + 16 * (t / CT_), cb = 16 * (t % CT_);
+    V4 arr, aii, aim;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) { arr[e] = (T)0; aii[e] = (T)0; aim[e] = (T)0; }
+    const int row = r0 + lr;
+    const C* lrow = A + (size_t)(row < n ? row : k0) * ld + k0;
+#pragma unroll
+    for (int kk = 0; kk < KB / 4; ++kk) {
+      const int q = 4 * kk + lk;
+      C a = lrow[q];
+      if (row >= n) { a.x = (T)0; a.y = (T)0; }
+      const C b = Ub[q][cb + lr];
+      arr = mfma4(a.x, b.x, arr);
+      aii = mfma4(a.y, b.y, aii);
+      aim = mfma4(a.x, b.y, aim);
+      aim = mfma4(a.y, b.x, aim);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int rr = r0 + mrow<T>(lk, e);
+      if (rr < n) {
+        C* pc = A + (size_t)rr * ld + k0 + KB + cb + lr;
+        C z = *pc;
+        z.x -= arr[e] - aii[e];
+        z.y -= aim[e];
+        *pc = z;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <typename T, int RPT, int SW>
 __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* base, int* pivots, int* info) {
   typedef typename CT<T>::c C;
@@ -886,8 +959,14 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
     }
     __syncthreads();
     // ── interchanges on the other panel columns [k, c0) and [c0 + ws, k + w) ──
-    const int nleft = c0 - k, nright = k + w - (c0 + ws);
-    if (tid < nleft + nright) {
+    const int nleft = c0 - k, nswap = k + w - (c0 + ws);
+    // full panels run as a two-level recursion over 16-column quarters: a
+    // sub-panel updates only its own quarter's columns; the next quarter (or
+    // the right half, after the left half) takes the factored block's rank-16
+    // (rank-32) update on the MFMA (panel_block_update, below)
+    const bool split = w == 64;
+    const int nright = split ? k + 16 * (j0 / 16 + 1) - (c0 + ws) : nswap;
+    if (tid < nleft + nswap) {
       const int cc = tid < nleft ? k + tid : c0 + ws + (tid - nleft);
       for (int jj = 0; jj < ws; ++jj) {
         const int r1 = c0 + jj, r2 = piv_s[jj];
@@ -946,6 +1025,9 @@ __global__ __launch_bounds__(1024) void k_lu_panel(const PanelOp* ops, void* bas
       }
       __syncthreads();
     }
+    if (split && j0 + ws == 16) panel_block_update<T, 16, 16>(A, n, ld, k, tid);        // quarter 0 -> 1
+    if (split && j0 + ws == 32) panel_block_update<T, 32, 32>(A, n, ld, k, tid);        // left half -> right half
+    if (split && j0 + ws == 48) panel_block_update<T, 16, 16>(A, n, ld, k + 32, tid);   // quarter 2 -> 3
   }
 }
 
