@@ -2890,7 +2890,7 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
     prof_end(h, K_FARFIELD, 0.0);
     LAUNCHCHK();
     const int tot = nv * p.nrhs;
-    hipLaunchKernelGGL(k_farfield_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream, h->d_partial, nch,
+    hipLaunchKernelGGL(k_farfield_reduce, dim3((tot + 63) / 64), dim3(256), 0, h->stream, h->d_partial, nch,
                        nv, p.nrhs, p.k, h->d_dF + p.dF_off + (size_t)mlo * p.nrhs, 0);
     LAUNCHCHK();
     if (mlo > 0) {
@@ -2922,7 +2922,7 @@ extern "C" int ghost_bor_farfield(ghost_bor* h) {
       prof_end(h, K_FARFIELD, 0.0);
       LAUNCHCHK();
       const int tot = r.nmodes * r.nrhs;
-      hipLaunchKernelGGL(k_farfield_reduce, dim3((tot + 255) / 256), dim3(256), 0, h->stream, h->d_partial, nch,
+      hipLaunchKernelGGL(k_farfield_reduce, dim3((tot + 63) / 64), dim3(256), 0, h->stream, h->d_partial, nch,
                          r.nmodes, r.nrhs, o.k, h->d_dF + r.dF_off, e > 0 ? 1 : 0);
       LAUNCHCHK();
     }

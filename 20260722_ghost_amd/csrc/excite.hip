@@ -267,6 +267,76 @@ __global__ __launch_bounds__(128) void k_rhs(ExciteArgs a, const SysInfo* sys, i
   }
 }
 
+// One Gauss point's far-field term w rho e^{j k cos(theta) z} x (projection of
+// the currents of its element's two nodes) for system s, aspect it, pol
+// (bor_solver.py:659-708; the far-field pairs (j_{m+1}, j_{m-1}) are reversed
+// relative to the excitation): node values t0v/t1v (J_t), f0v/f1v (J_phi)
+// and, for a PMCHWT pair, mt0/mt1/mf0/mf1 (M / eta0).
+__device__ __forceinline__ double2 ff_point_term(const ExciteArgs& a, const SysInfo& s, int p, int e, int it,
+                                                 int pol, double st, double ct, size_t stride, double2 t0v,
+                                                 double2 t1v, double2 f0v, double2 f1v, double2 mt0, double2 mt1,
+                                                 double2 mf0, double2 mf1) {
+  const PointSet& P = a.pts;
+  const double T0 = P.T0[p], T1 = P.T1[p];
+  const double2 Jt = make_double2(T0 * t0v.x + T1 * t1v.x, T0 * t0v.y + T1 * t1v.y);
+  const double2 Jf = make_double2(T0 * f0v.x + T1 * f1v.x, T0 * f0v.y + T1 * f1v.y);
+  const double* Jp = a.J + (size_t)p * a.ntheta + it;
+  const double2 jm = jn_val(Jp, abs(s.m), stride);
+  const double2 jm1 = jn_val(Jp, abs(s.m - 1), stride);
+  const double2 jp1 = jn_val(Jp, abs(s.m + 1), stride);
+  // Icos = pi (j_{m+1} + j_{m-1}),  Isin = -j pi (j_{m+1} - j_{m-1}),  I1 = 2 pi j_m
+  const double2 Icos = make_double2(GH_PI * (jp1.x + jm1.x), GH_PI * (jp1.y + jm1.y));
+  const double2 Isin = make_double2(GH_PI * (jp1.y - jm1.y), -GH_PI * (jp1.x - jm1.x));
+  const double2 I1 = make_double2(2.0 * GH_PI * jm.x, 2.0 * GH_PI * jm.y);
+  const double tr = P.trho[p], tz = P.tz[p];
+  double2 term;
+  if (pol == 0) {
+    // Jt (ct tr Icos - st tz I1) - Jf ct Isin
+    const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
+    term = make_double2(Jt.x * u.x - Jt.y * u.y - ct * (Jf.x * Isin.x - Jf.y * Isin.y),
+                        Jt.x * u.y + Jt.y * u.x - ct * (Jf.x * Isin.y + Jf.y * Isin.x));
+  } else {
+    // Jt tr Isin + Jf Icos
+    term = make_double2(tr * (Jt.x * Isin.x - Jt.y * Isin.y) + Jf.x * Icos.x - Jf.y * Icos.y,
+                        tr * (Jt.x * Isin.y + Jt.y * Isin.x) + Jf.x * Icos.y + Jf.y * Icos.x);
+  }
+  if (a.zs) {
+    // M_t = Z_s J_phi, M_phi = -Z_s J_t; with the common -j k eta0 / (4 pi):
+    // F_theta += (1/eta0) proj_phi(M) = (Z_s/eta0) (Jf tr Isin - Jt Icos),
+    // F_phi   -= (1/eta0) proj_theta(M) = -(Z_s/eta0) (Jf u + Jt ct Isin)
+    const double2 z = a.zs[e];
+    const double ie = 1.0 / 376.730313668;
+    double2 mt;
+    if (pol == 0) {
+      mt = make_double2(tr * (Jf.x * Isin.x - Jf.y * Isin.y) - (Jt.x * Icos.x - Jt.y * Icos.y),
+                        tr * (Jf.x * Isin.y + Jf.y * Isin.x) - (Jt.x * Icos.y + Jt.y * Icos.x));
+    } else {
+      const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
+      mt = make_double2(-(Jf.x * u.x - Jf.y * u.y) - ct * (Jt.x * Isin.x - Jt.y * Isin.y),
+                        -(Jf.x * u.y + Jf.y * u.x) - ct * (Jt.x * Isin.y + Jt.y * Isin.x));
+    }
+    term.x += ie * (z.x * mt.x - z.y * mt.y);
+    term.y += ie * (z.x * mt.y + z.y * mt.x);
+  }
+  if (s.pen) {
+    // an independent magnetic current M = eta0 M' (bor_solver.py:697-707):
+    // F_theta += proj_phi(M'), F_phi -= proj_theta(M') under the common
+    // -j k eta0 / (4 pi)
+    const double2 Mt = make_double2(T0 * mt0.x + T1 * mt1.x, T0 * mt0.y + T1 * mt1.y);
+    const double2 Mf = make_double2(T0 * mf0.x + T1 * mf1.x, T0 * mf0.y + T1 * mf1.y);
+    if (pol == 0) {
+      term.x += tr * (Mt.x * Isin.x - Mt.y * Isin.y) + Mf.x * Icos.x - Mf.y * Icos.y;
+      term.y += tr * (Mt.x * Isin.y + Mt.y * Isin.x) + Mf.x * Icos.y + Mf.y * Icos.x;
+    } else {
+      const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
+      term.x += -(Mt.x * u.x - Mt.y * u.y) + ct * (Mf.x * Isin.x - Mf.y * Isin.y);
+      term.y += -(Mt.x * u.y + Mt.y * u.x) + ct * (Mf.x * Isin.y + Mf.y * Isin.x);
+    }
+  }
+  const double2 E = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
+  return make_double2(term.x * E.x - term.y * E.y, term.x * E.y + term.y * E.x);
+}
+
 // Partial far-field sums over a chunk of Gauss points for every (system, column).
 // X is read from the mode's columns of the augmented system or of the n x
 // ncols solution block at element offset x_off.
@@ -327,65 +397,9 @@ __global__ __launch_bounds__(256) void k_farfield_partial(ExciteArgs a, const Sy
       if (s.pen) { mt1 = xt(e + 1, n1, ff); mf1 = xf(e + 1, n1, ft); }
       e_prev = e;
     }
-    const double T0 = P.T0[p], T1 = P.T1[p];
-    const double2 Jt = make_double2(T0 * t0v.x + T1 * t1v.x, T0 * t0v.y + T1 * t1v.y);
-    const double2 Jf = make_double2(T0 * f0v.x + T1 * f1v.x, T0 * f0v.y + T1 * f1v.y);
-    const double* Jp = a.J + (size_t)p * a.ntheta + it;
-    const double2 jm = jn_val(Jp, abs(s.m), stride);
-    const double2 jm1 = jn_val(Jp, abs(s.m - 1), stride);
-    const double2 jp1 = jn_val(Jp, abs(s.m + 1), stride);
-    // Icos = pi (j_{m+1} + j_{m-1}),  Isin = -j pi (j_{m+1} - j_{m-1}),  I1 = 2 pi j_m
-    const double2 Icos = make_double2(GH_PI * (jp1.x + jm1.x), GH_PI * (jp1.y + jm1.y));
-    const double2 Isin = make_double2(GH_PI * (jp1.y - jm1.y), -GH_PI * (jp1.x - jm1.x));
-    const double2 I1 = make_double2(2.0 * GH_PI * jm.x, 2.0 * GH_PI * jm.y);
-    const double tr = P.trho[p], tz = P.tz[p];
-    double2 term;
-    if (pol == 0) {
-      // Jt (ct tr Icos - st tz I1) - Jf ct Isin
-      const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
-      term = make_double2(Jt.x * u.x - Jt.y * u.y - ct * (Jf.x * Isin.x - Jf.y * Isin.y),
-                          Jt.x * u.y + Jt.y * u.x - ct * (Jf.x * Isin.y + Jf.y * Isin.x));
-    } else {
-      // Jt tr Isin + Jf Icos
-      term = make_double2(tr * (Jt.x * Isin.x - Jt.y * Isin.y) + Jf.x * Icos.x - Jf.y * Icos.y,
-                          tr * (Jt.x * Isin.y + Jt.y * Isin.x) + Jf.x * Icos.y + Jf.y * Icos.x);
-    }
-    if (a.zs) {
-      // M_t = Z_s J_phi, M_phi = -Z_s J_t; with the common -j k eta0 / (4 pi):
-      // F_theta += (1/eta0) proj_phi(M) = (Z_s/eta0) (Jf tr Isin - Jt Icos),
-      // F_phi   -= (1/eta0) proj_theta(M) = -(Z_s/eta0) (Jf u + Jt ct Isin)
-      const double2 z = a.zs[e];
-      const double ie = 1.0 / 376.730313668;
-      double2 mt;
-      if (pol == 0) {
-        mt = make_double2(tr * (Jf.x * Isin.x - Jf.y * Isin.y) - (Jt.x * Icos.x - Jt.y * Icos.y),
-                          tr * (Jf.x * Isin.y + Jf.y * Isin.x) - (Jt.x * Icos.y + Jt.y * Icos.x));
-      } else {
-        const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
-        mt = make_double2(-(Jf.x * u.x - Jf.y * u.y) - ct * (Jt.x * Isin.x - Jt.y * Isin.y),
-                          -(Jf.x * u.y + Jf.y * u.x) - ct * (Jt.x * Isin.y + Jt.y * Isin.x));
-      }
-      term.x += ie * (z.x * mt.x - z.y * mt.y);
-      term.y += ie * (z.x * mt.y + z.y * mt.x);
-    }
-    if (s.pen) {
-      // an independent magnetic current M = eta0 M' (bor_solver.py:697-707):
-      // F_theta += proj_phi(M'), F_phi -= proj_theta(M') under the common
-      // -j k eta0 / (4 pi)
-      const double2 Mt = make_double2(T0 * mt0.x + T1 * mt1.x, T0 * mt0.y + T1 * mt1.y);
-      const double2 Mf = make_double2(T0 * mf0.x + T1 * mf1.x, T0 * mf0.y + T1 * mf1.y);
-      if (pol == 0) {
-        term.x += tr * (Mt.x * Isin.x - Mt.y * Isin.y) + Mf.x * Icos.x - Mf.y * Icos.y;
-        term.y += tr * (Mt.x * Isin.y + Mt.y * Isin.x) + Mf.x * Icos.y + Mf.y * Icos.x;
-      } else {
-        const double2 u = make_double2(ct * tr * Icos.x - st * tz * I1.x, ct * tr * Icos.y - st * tz * I1.y);
-        term.x += -(Mt.x * u.x - Mt.y * u.y) + ct * (Mf.x * Isin.x - Mf.y * Isin.y);
-        term.y += -(Mt.x * u.y + Mt.y * u.x) + ct * (Mf.x * Isin.y + Mf.y * Isin.x);
-      }
-    }
-    const double2 E = a.E[(size_t)p * a.ntheta + it];   // w rho e^{j k cos(th) z}
-    acc.x += term.x * E.x - term.y * E.y;
-    acc.y += term.x * E.y + term.y * E.x;
+    const double2 v = ff_point_term(a, s, p, e, it, pol, st, ct, stride, t0v, t1v, f0v, f1v, mt0, mt1, mf0, mf1);
+    acc.x += v.x;
+    acc.y += v.y;
   }
   partial[((size_t)pc * nsys + si) * s.nrhs + col] = acc;
 }
@@ -399,17 +413,29 @@ __global__ void k_mirror_dF(double2* dF, int M, int nrhs) {
   dF[(size_t)(M - m) * nrhs + c] = dF[(size_t)(M + m) * nrhs + c];
 }
 
-// Sum the chunk partials in chunk order and apply -j k eta0 / (4 pi); acc:
-// add to dF (the next radiating surface of a multi-region body).
+// Sum the chunk partials and apply -j k eta0 / (4 pi); acc: add to dF (the
+// next radiating surface of a multi-region body).  A wavefront takes 16
+// (system, column) entries x 4 chunk phases (lane = phase x 16 + entry): each
+// lane sums the chunks of its phase in order, the four phases meet by two xor
+// shuffles (lanes e, e + 16, e + 32, e + 48): a fixed order, so the sum is
+// bitwise reproducible.
 __global__ void k_farfield_reduce(const double2* partial, int nchunk, int nsys, int nrhs, double k,
                                   double2* dF, int acc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nsys * nrhs) return;
+  const int lane = threadIdx.x & 63;
+  const int ph = lane >> 4;
+  const int t = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16 + (lane & 15);
+  const bool live = t < nsys * nrhs;
   double2 sum = make_double2(0.0, 0.0);
-  for (int c = 0; c < nchunk; ++c) {
-    const double2 v = partial[(size_t)c * nsys * nrhs + t];
-    sum.x += v.x; sum.y += v.y;
-  }
+  if (live)
+    for (int c = ph; c < nchunk; c += 4) {
+      const double2 v = partial[(size_t)c * nsys * nrhs + t];
+      sum.x += v.x; sum.y += v.y;
+    }
+  sum.x += __shfl_xor(sum.x, 16);
+  sum.y += __shfl_xor(sum.y, 16);
+  sum.x += __shfl_xor(sum.x, 32);
+  sum.y += __shfl_xor(sum.y, 32);
+  if (!live || ph != 0) return;
   const double pref = k * 376.730313668 / (4.0 * GH_PI);  // -j * pref
   double2 out = make_double2(pref * sum.y, -pref * sum.x);
   if (acc) { out.x += dF[t].x; out.y += dF[t].y; }
