@@ -597,6 +597,14 @@ __device__ __forceinline__ void cell_geom(const double* nodes, int e, double s, 
   z = z0 + s * (z1 - z0);
 }
 
+// the math calls of the near band's node values in either precision
+__device__ __forceinline__ void nsincos(double x, double& s, double& c) { sincos(x, &s, &c); }
+__device__ __forceinline__ void nsincos(float x, float& s, float& c) { sincosf(x, &s, &c); }
+__device__ __forceinline__ double nsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float nsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double nexp(double x) { return exp(x); }
+__device__ __forceinline__ float nexp(float x) { return expf(x); }
+
 // Accumulate one thread's C orders over one batch of 8 nodes: nvs = the
 // point pair's node values (12 per node), st0 / st1 = its chunk starts
 // e^{j c C xi} (re / im, [node * 9 + chunk]).  EFIE and even brackets
@@ -699,6 +707,86 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
 #pragma unroll
     for (int u = 0; u < 4; ++u) { kR[u][i] = 0.0; kI[u][i] = 0.0; }
   }
+  // Node values from (w, R, sin xi, cos xi): the kernel values in T, FP64
+  // over the sinh core and RT over the smooth tail (in FP32 the distance, the
+  // phase kR and the reciprocals use the single-precision instructions, at the
+  // level of the FP32 tail sums); xi, cos xi, sin xi and the chunk starts
+  // always in FP64.
+  auto emit = [&](auto wq, auto Rd, double sxd, double cxd, double* out) {
+    using T = decltype(wq);
+    const T sx = (T)sxd, cx = (T)cxd;
+    const double* g = pg[slot];
+    const T rp = (T)g[6], rq = (T)g[7], Rz = (T)g[8], trp = (T)g[9], tzp = (T)g[10], trq = (T)g[11], tzq = (T)g[12];
+    const bool axis = g[13] != 0.0;
+    const T tiny = sizeof(T) == 8 ? (T)1e-300 : (T)1e-30f;
+    Rd = Rd > tiny ? Rd : tiny;
+    T sk, ck;
+    nsincos((T)k * Rd, sk, ck);
+    const T inv = (T)1 / Rd;
+    // complex k = k + j ki: e^{-jkR} carries the decay e^{ki R}, and
+    // (1 + jkR) = (1 - ki R) + j k R
+    const T dec = a.ki != 0.0 ? nexp((T)a.ki * Rd) : (T)1, u = (T)1 - (T)a.ki * Rd;
+    // EFIE: 2 g w  (axis pairs use the closed form instead)
+    const T fpi = (T)(4.0 * GH_PI);
+    const T ge = axis ? (T)0 : (T)2 * wq * inv / fpi * dec;
+    out[0] = ge * ck;
+    out[1] = -ge * sk;
+    // brackets: 2 w p(R) b_uv(xi); a zero weight (the empty tail of an axis
+    // pair) stays zero
+    const T pre = wq == (T)0 ? (T)0 : (T)2 * wq * inv * inv * inv / fpi * dec;
+    const T kr = (T)k * Rd;
+    const T pr_ = pre * (u * ck + kr * sk), pi_ = pre * (kr * ck - u * sk);
+    const T Rx = rp - rq * cx, Ry = rq * sx;
+    const T Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
+    T btt, btf, bft, bff;
+    if (a.ibc) {
+      const T D = rp * cx - rq;
+      const T Wt_n = -trp * tzq * cx + tzp * trq, Wf_n = tzq * sx;
+      const T R_t = trq * D + tzq * Rz, R_f = rp * sx, R_n = -tzq * D + trq * Rz;
+      btt = Wt_n * R_t - Wt_t * R_n;
+      btf = Wt_n * R_f - Wt_f * R_n;
+      bft = Wf_n * R_t - Wf_t * R_n;
+      bff = Wf_n * R_f - cx * R_n;
+    } else {
+      const T WtR = trp * Rx + tzp * Rz, nR = -tzp * Rx + trp * Rz;
+      const T n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
+      btt = -(WtR * n_t - Wt_t * nR);
+      btf = -(WtR * n_f - Wt_f * nR);
+      bft = -(Ry * n_t - Wf_t * nR);
+      bff = -(Ry * n_f - cx * nR);
+    }
+    out[2] = btt * pr_; out[3] = btt * pi_;
+    out[4] = btf * pr_; out[5] = btf * pi_;
+    out[6] = bft * pr_; out[7] = bft * pi_;
+    out[8] = bff * pr_; out[9] = bff * pi_;
+    out[10] = cxd; out[11] = sxd;
+    // chunk starts e^{j c C xi}, c = 0..7, from e^{j C xi} = (e^{j xi})^C, in
+    // FP64 (a modulus error of e^{j xi} grows with the power)
+    double cC = 1.0, sC = 0.0;
+    {
+      double br = cxd, bi = sxd;
+#pragma unroll
+      for (int bit = 1; bit <= C; bit <<= 1) {
+        if (C & bit) {
+          const double t = cC * br - sC * bi;
+          sC = cC * bi + sC * br;
+          cC = t;
+        }
+        const double t = br * br - bi * bi;
+        bi = 2.0 * br * bi;
+        br = t;
+      }
+    }
+    double zr = 1.0, zi = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      st[0][slot * 72 + cs * 9 + c] = zr;
+      st[1][slot * 72 + cs * 9 + c] = zi;
+      const double t = zr * cC - zi * sC;
+      zi = zr * sC + zi * cC;
+      zr = t;
+    }
+  };
   auto node_values = [&](int v0) {
     // ── node value for node v0 + cs of this point pair ──
     const int vn = v0 + cs;
@@ -706,12 +794,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
     if (valid && vn < nnodes) {
       const double* g = pg[slot];
       const double vmax = g[0], d = g[1], aa = g[2], doa = g[3], xi0 = g[4], span = g[5];
-      const double rp = g[6], rq = g[7], Rz = g[8], trp = g[9], tzp = g[10], trq = g[11], tzq = g[12];
-      const bool axis = g[13] != 0.0;
       // xi enters through s = sin(xi/2) only: cos xi = 1 - 2 s^2 and
       // sin xi = 2 s cos(xi/2), with cos(xi/2) = sqrt(1 - s^2) on the core
       // (xi = 2 asin s, s <= 1), so neither asin nor a sincos of xi is needed
-      double wq, Rd, sx, cx;
       if (vn < 48) {
         const double vv = a.core_u[vn] * vmax, wv = a.core_w[vn] * vmax;
         const double em = expm1(vv), ev = em + 1.0;
@@ -719,83 +804,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
         double sv = doa * shv;
         sv = sv < 1.0 ? sv : 1.0;
         const double cv = sqrt(fmax(1.0 - sv * sv, 0.0));
-        wq = wv * 2.0 * doa * chv / fmax(cv, sqrt(1e-15));
-        Rd = sqrt(d * d + (aa * sv) * (aa * sv));
-        sx = 2.0 * sv * cv;
-        cx = 1.0 - 2.0 * sv * sv;
+        const double wq = wv * 2.0 * doa * chv / fmax(cv, sqrt(1e-15));
+        const double Rd = sqrt(d * d + (aa * sv) * (aa * sv));
+        emit(wq, Rd, 2.0 * sv * cv, 1.0 - 2.0 * sv * sv, out);
       } else {
         const int ti = pr.tail_off + (vn - 48);
         const double xi = xi0 + a.tail_u[ti] * span;
-        wq = a.tail_w[ti] * span;
+        const RT wq = (RT)(a.tail_w[ti] * span);
+        // sin / cos of xi/2 in FP64 in either case: an angle rounded to
+        // FP32 would put an error of n x 6e-8 on the phase of order n
         double sh, ch;
         sincos(0.5 * xi, &sh, &ch);
-        Rd = sqrt(d * d + (aa * sh) * (aa * sh));
-        sx = 2.0 * sh * ch;
-        cx = 1.0 - 2.0 * sh * sh;
-      }
-      Rd = Rd > 1e-300 ? Rd : 1e-300;
-      double sk, ck;
-      sincos(k * Rd, &sk, &ck);
-      const double inv = 1.0 / Rd;
-      // complex k = k + j ki: e^{-jkR} carries the decay e^{ki R}, and
-      // (1 + jkR) = (1 - ki R) + j k R
-      const double dec = a.ki != 0.0 ? exp(a.ki * Rd) : 1.0, u = 1.0 - a.ki * Rd;
-      // EFIE: 2 g w  (axis pairs use the closed form instead)
-      const double ge = axis ? 0.0 : 2.0 * wq * inv / (4.0 * GH_PI) * dec;
-      out[0] = ge * ck;
-      out[1] = -ge * sk;
-      // brackets: 2 w p(R) b_uv(xi)
-      const double pre = 2.0 * wq * inv * inv * inv / (4.0 * GH_PI) * dec;
-      const double kr = k * Rd;
-      const double pr_ = pre * (u * ck + kr * sk), pi_ = pre * (kr * ck - u * sk);
-      const double Rx = rp - rq * cx, Ry = rq * sx;
-      const double Wt_t = trp * trq * cx + tzp * tzq, Wt_f = trp * sx, Wf_t = -trq * sx;
-      double btt, btf, bft, bff;
-      if (a.ibc) {
-        const double D = rp * cx - rq;
-        const double Wt_n = -trp * tzq * cx + tzp * trq, Wf_n = tzq * sx;
-        const double R_t = trq * D + tzq * Rz, R_f = rp * sx, R_n = -tzq * D + trq * Rz;
-        btt = Wt_n * R_t - Wt_t * R_n;
-        btf = Wt_n * R_f - Wt_f * R_n;
-        bft = Wf_n * R_t - Wf_t * R_n;
-        bff = Wf_n * R_f - cx * R_n;
-      } else {
-        const double WtR = trp * Rx + tzp * Rz, nR = -tzp * Rx + trp * Rz;
-        const double n_t = -tzp * trq * cx + trp * tzq, n_f = -tzp * sx;
-        btt = -(WtR * n_t - Wt_t * nR);
-        btf = -(WtR * n_f - Wt_f * nR);
-        bft = -(Ry * n_t - Wf_t * nR);
-        bff = -(Ry * n_f - cx * nR);
-      }
-      out[2] = btt * pr_; out[3] = btt * pi_;
-      out[4] = btf * pr_; out[5] = btf * pi_;
-      out[6] = bft * pr_; out[7] = bft * pi_;
-      out[8] = bff * pr_; out[9] = bff * pi_;
-      out[10] = cx; out[11] = sx;
-      // chunk starts e^{j c C xi}, c = 0..7, from e^{j C xi} = (e^{j xi})^C
-      double cC = 1.0, sC = 0.0;
-      {
-        double br = cx, bi = sx;
-#pragma unroll
-        for (int bit = 1; bit <= C; bit <<= 1) {
-          if (C & bit) {
-            const double t = cC * br - sC * bi;
-            sC = cC * bi + sC * br;
-            cC = t;
-          }
-          const double t = br * br - bi * bi;
-          bi = 2.0 * br * bi;
-          br = t;
-        }
-      }
-      double zr = 1.0, zi = 0.0;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        st[0][slot * 72 + cs * 9 + c] = zr;
-        st[1][slot * 72 + cs * 9 + c] = zi;
-        const double t = zr * cC - zi * sC;
-        zi = zr * sC + zi * cC;
-        zr = t;
+        const RT dd = (RT)d, ad = (RT)aa, shr = (RT)sh;
+        const RT Rd = nsqrt(dd * dd + (ad * shr) * (ad * shr));
+        emit(wq, Rd, 2.0 * sh * ch, 1.0 - 2.0 * sh * sh, out);
       }
     } else {
 #pragma unroll
