@@ -1398,8 +1398,13 @@ static int build_schedule(ghost_bor* h) {
     }
     h->lu_sched[part].push_back(g);
   };
-  for (int b0 = 0; b0 < nblk_max; b0 += S) {
-    const int k0 = b0 * nb, nsb = std::min(S, nblk_max - b0);
+  // the second part's super-blocks start half a super-block later: its panel
+  // chains (latency-bound, one work-group per system) then fall on the first
+  // part's trailing updates and the other way round, where two parts in step
+  // would run their chains together and their trailing updates together
+  const int first_sb = (part == 1 && S >= 2) ? S / 2 : S;
+  for (int b0 = 0, sbw = first_sb; b0 < nblk_max; b0 += sbw, sbw = S) {
+    const int k0 = b0 * nb, nsb = std::min(sbw, nblk_max - b0);
     for (int sb = 0; sb < nsb; ++sb) {
       const int b = b0 + sb, k = b * nb;
       if (sb > 0)   // panel catch-up: A[k:n, k:k+w] -= L[k:n, k0:k] U[k0:k, k:k+w]
