@@ -597,11 +597,8 @@ __device__ __forceinline__ void cell_geom(const double* nodes, int e, double s, 
   z = z0 + s * (z1 - z0);
 }
 
-// the math calls of the near band's node values in either precision
-__device__ __forceinline__ void nsincos(double x, double& s, double& c) { sincos(x, &s, &c); }
-__device__ __forceinline__ void nsincos(float x, float& s, float& c) { sincosf(x, &s, &c); }
-__device__ __forceinline__ double nsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float nsqrt(float x) { return sqrtf(x); }
+// exp of the near band's node values in either precision (the decay of a
+// lossy medium); sqrt, sincos and reciprocals use the far fill's sample forms
 __device__ __forceinline__ double nexp(double x) { return exp(x); }
 __device__ __forceinline__ float nexp(float x) { return expf(x); }
 
@@ -709,9 +706,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
   }
   // Node values from (w, R, sin xi, cos xi): the kernel values in T, FP64
   // over the sinh core and RT over the smooth tail (in FP32 the distance, the
-  // phase kR and the reciprocals use the single-precision instructions, at the
-  // level of the FP32 tail sums); xi, cos xi, sin xi and the chunk starts
-  // always in FP64.
+  // phase kR and the reciprocal on the hardware sqrt, sin / cos and rcp, as
+  // in the far fill's samples, at the level of the FP32 tail sums); xi,
+  // cos xi, sin xi and the chunk starts always in FP64.
   auto emit = [&](auto wq, auto Rd, double sxd, double cxd, double* out) {
     using T = decltype(wq);
     const T sx = (T)sxd, cx = (T)cxd;
@@ -721,8 +718,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
     const T tiny = sizeof(T) == 8 ? (T)1e-300 : (T)1e-30f;
     Rd = Rd > tiny ? Rd : tiny;
     T sk, ck;
-    nsincos((T)k * Rd, sk, ck);
-    const T inv = (T)1 / Rd;
+    sincos_s((T)k * Rd, &sk, &ck);
+    const T inv = rcp_s(Rd);
     // complex k = k + j ki: e^{-jkR} carries the decay e^{ki R}, and
     // (1 + jkR) = (1 - ki R) + j k R
     const T dec = a.ki != 0.0 ? nexp((T)a.ki * Rd) : (T)1, u = (T)1 - (T)a.ki * Rd;
@@ -816,7 +813,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C <= 9 ? 2 
         double sh, ch;
         sincos(0.5 * xi, &sh, &ch);
         const RT dd = (RT)d, ad = (RT)aa, shr = (RT)sh;
-        const RT Rd = nsqrt(dd * dd + (ad * shr) * (ad * shr));
+        const RT Rd = sqrt_s(dd * dd + (ad * shr) * (ad * shr));
         emit(wq, Rd, 2.0 * sh * ch, 1.0 - 2.0 * sh * sh, out);
       }
     } else {
