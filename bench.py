@@ -741,61 +741,67 @@ def main():
         }
         fp64_rate = solves / dt_64
 
-    # dominant kernel of the headline path, roofline from live HIP-event timing:
-    # the GEMM kernel over all its uses in the step (LU updates, back
-    # substitution, refinement solves), the FP64 residual kernel aside
+    # rooflines from live HIP-event timing of the profiled step: the GEMM over all
+    # its uses in the step (LU updates, back substitution, refinement solves; the
+    # FP64 residual kernel aside), the far fill and the near band; "roofline" is
+    # the one of the family that takes the most time per step
     gl = [v for kk, v in kt_mx.items() if kk.startswith("gemm") and kk != "gemm_fp64_residual"]
     gemm_ms, gemm_n, gemm_fl = (sum(v[0] for v in gl), sum(v[1] for v in gl), sum(v[2] for v in gl))
     merged = {kk: v for kk, v in kt_mx.items() if not (kk.startswith("gemm") and kk != "gemm_fp64_residual")}
     merged["gemm (all uses)"] = (gemm_ms, gemm_n, gemm_fl)
     fam, (ms, cnt, fl) = max(merged.items(), key=lambda kv: kv[1][0])
-    roof = None
+    fp32 = args.precision == "mixed"
+    per = {}
+    tpath = os.path.join(ROOT, "profiles", "ogive1200_m70_mixed_hbm_traffic.json")
+    if args.config == 2 and fp32 and os.path.exists(tpath):
+        with open(tpath) as f:
+            per = json.load(f)["per_launch"]
+    traffic_unit = "bytes/launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)"
+    kinds = {}
     if gemm_n > 0 and gemm_ms > 0:
         achieved = gemm_fl / (gemm_ms * 1e-3) / 1e12
-        traffic = None
-        fp32 = args.precision == "mixed"
         split = fp32 and tiles_mx[1] == 5
-        tpath = os.path.join(ROOT, "profiles", "ogive1200_m70_mixed_hbm_traffic.json")
-        if args.config == 2 and fp32 and os.path.exists(tpath):
-            with open(tpath) as f:
-                per = json.load(f)["per_launch"]
-            key = "gemm_bf3" if split else "gemm"
-            traffic = per[key]["hbm_bytes"] if key in per else None
+        key = "gemm_bf3" if split else "gemm"
+        traffic = per[key]["hbm_bytes"] if key in per else None
         if split:
             # the pipe doing the work is the bf16 MFMA: each FP32 product is six bf16 products
             issued = SPLIT_BF16_PRODUCTS * achieved
             pk = MI355X_PEAK["bf16_mfma_tflops"]
-            roof = {"kernel": ("k_gemm_bf3 (complex FP32 GEMM on the bf16 MFMA 32x32x16: operands split into "
-                               "three bf16 parts, six products each; 64x64 tiles; LU updates and solves)"),
-                    "bound": "mfma", "achieved": round(issued, 3), "peak": pk, "unit": "TFLOP/s",
-                    "frac": round(issued / pk, 4),
-                    "achieved_note": "bf16 MFMA flops issued = 6 x the algorithmic complex FP32 GEMM flops",
-                    "fp32_equivalent": {"achieved": round(achieved, 3), "peak": MI355X_PEAK["fp32_mfma_tflops"],
-                                        "frac": round(achieved / MI355X_PEAK["fp32_mfma_tflops"], 4)}}
+            g = {"kernel": ("k_gemm_bf3 (complex FP32 GEMM on the bf16 MFMA 32x32x16: operands split into "
+                            "three bf16 parts, six products each; 64x64 tiles; LU updates and solves)"),
+                 "bound": "mfma", "achieved": round(issued, 3), "peak": pk, "unit": "TFLOP/s",
+                 "frac": round(issued / pk, 4),
+                 "achieved_note": "bf16 MFMA flops issued = 6 x the algorithmic complex FP32 GEMM flops",
+                 "fp32_equivalent": {"achieved": round(achieved, 3), "peak": MI355X_PEAK["fp32_mfma_tflops"],
+                                     "frac": round(achieved / MI355X_PEAK["fp32_mfma_tflops"], 4)}}
         else:
             pk = MI355X_PEAK["fp32_mfma_tflops" if fp32 else "fp64_mfma_tflops"]
-            roof = {"kernel": ("k_gemm2<float,2,2> (complex FP32 MFMA 16x16x4, 64x64 tiles; LU updates and solves)"
-                               if fp32 else "k_gemm2<double,...> (complex FP64 MFMA 16x16x4; LU updates and solves)"),
-                    "bound": "mfma", "achieved": round(achieved, 3), "peak": pk,
-                    "unit": "TFLOP/s", "frac": round(achieved / pk, 4)}
-        roof.update({"traffic": traffic,
-                     "traffic_unit": "bytes/launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0)",
-                     "launches": gemm_n, "avg_launch_ms": round(gemm_ms / gemm_n, 4),
-                     "flops_per_launch": gemm_fl / gemm_n})
-    # the fill's two big kernels against their own bounds (live HIP-event time of the profiled step):
-    # the far fill's MFMA flops issued against FP32 MFMA peak (it also issues the sample arithmetic on
-    # the VALU, which does not co-issue with MFMA on gfx950), the near band's projection flops against
-    # FP64 vector peak (its FP32 tail runs at the same issue rate with packed FP32 off)
-    more = {}
-    for fk, peak_key, what in (("far_fill", "fp32_mfma_tflops", "k_far_fill (FP32 MFMA transform + VALU samples)"),
+            g = {"kernel": ("k_gemm2<float,2,2> (complex FP32 MFMA 16x16x4, 64x64 tiles; LU updates and solves)"
+                            if fp32 else "k_gemm2<double,...> (complex FP64 MFMA 16x16x4; LU updates and solves)"),
+                 "bound": "mfma", "achieved": round(achieved, 3), "peak": pk,
+                 "unit": "TFLOP/s", "frac": round(achieved / pk, 4)}
+        g.update({"traffic": traffic, "traffic_unit": traffic_unit,
+                  "launches": gemm_n, "avg_launch_ms": round(gemm_ms / gemm_n, 4),
+                  "flops_per_launch": gemm_fl / gemm_n, "ms_per_step": round(gemm_ms, 3)})
+        kinds["gemm"] = g
+    # the far fill's MFMA flops issued against the FP32 MFMA peak (it also issues the sample
+    # arithmetic on the VALU, which does not co-issue with MFMA on gfx950), the near band's
+    # projection flops against the FP64 vector peak (its FP32 tail runs at the same issue rate
+    # with packed FP32 off)
+    for fk, peak_key, what in (("far_fill", "fp32_mfma_tflops" if fp32 else "fp64_mfma_tflops",
+                                "k_far_fill (MFMA azimuth transform + VALU samples)"),
                                ("near_modal", "fp64_vector_tflops", "k_near_modal (FP64 core / FP32 tail VALU projection)")):
         ms_f, n_f, fl_f = kt_mx.get(fk, (0.0, 0, 0.0))
-        if ms_f > 0 and fl_f > 0:
+        if ms_f > 0 and fl_f > 0 and n_f > 0:
             ach = fl_f / (ms_f * 1e-3) / 1e12
-            more[fk] = {"kernel": what, "bound": "mfma" if fk == "far_fill" else "valu",
+            kinds[fk] = {"kernel": what, "bound": "mfma" if fk == "far_fill" else "valu",
                          "achieved": round(ach, 3), "peak": MI355X_PEAK[peak_key], "unit": "TFLOP/s",
-                         "frac": round(ach / MI355X_PEAK[peak_key], 4), "ms_per_step": round(ms_f, 3),
-                         "flops_per_step": fl_f}
+                         "frac": round(ach / MI355X_PEAK[peak_key], 4),
+                         "traffic": per[fk]["hbm_bytes"] if fk in per else None, "traffic_unit": traffic_unit,
+                         "launches": n_f, "avg_launch_ms": round(ms_f / n_f, 4), "flops_per_launch": fl_f / n_f,
+                         "ms_per_step": round(ms_f, 3)}
+    dom = "gemm" if fam == "gemm (all uses)" else fam
+    roof = kinds.get(dom, kinds.get("gemm"))
     out = {
         "metric": "BOR monostatic solves/s (mode x freq x angle) at <=0.01 dB vs FP64; MFMA/HBM %",
         "value": round(solves / dt_mx, 2),
@@ -823,7 +829,7 @@ def main():
         "autotuned_lu_tiles": {args.precision: list(tiles_mx), "fp64": list(tiles_64) if tiles_64 else None,
                                "cache": pkg("autotune").cache_path()},
         "roofline": roof,
-        "roofline_fill_kernels": more,
+        "roofline_kernels": kinds,
         "cpu_baseline": None if args.no_cpu else cpu_baseline_timed(nodes, freq, thetas, m_max),
     }
     print(json.dumps(out), flush=True)
