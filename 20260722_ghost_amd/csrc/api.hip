@@ -3796,7 +3796,8 @@ static int adjoint_analytic_kernels(ghost_bor* h, const std::vector<size_t>& cof
       na.cm = -2.0 * M_PI * p.cM;
       na.cmass = M_PI * p.cM;
       na.out = h->d_adj_near;
-      const size_t lds = 16 * (size_t)(p.M + 1) * sizeof(double2);
+      // the 16 staged Q entry rows and the 19 contracted coefficient rows of one point pair
+      const size_t lds = 35 * (size_t)(p.M + 1) * sizeof(double2);
       hipLaunchKernelGGL(k_adj_near, dim3((unsigned)p.batches.size()), dim3(256), lds, h->stream, na);
       LAUNCHCHK();
     }

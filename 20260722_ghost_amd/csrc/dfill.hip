@@ -237,6 +237,9 @@ __device__ __forceinline__ void qseries(const double2* Pq, int M, int ab, double
   const double2* pff = Pq + (3 * 4 + ab) * (M + 1);
   double c = 1.0, s = 0.0, cp = cx, sp = -sx;   // cos / sin of m xi and (m - 1) xi
   const double tc = 2.0 * cx;
+  // two orders per step: the next order's LDS loads are in flight while this
+  // order's sums are formed
+#pragma unroll 2
   for (int m = 0; m <= M; ++m) {
     const double2 a = ptt[m], b = ptf[m], g = pft[m], d = pff[m];
     F[0].x += a.x * c; F[0].y += a.y * c;
@@ -536,69 +539,42 @@ struct AdjNearArgs {
   double* out;            // [n batches][4]
 };
 
-// Series values and xi-derivatives of one entry at a node: F[i] as in
-// qseries, dF[i] = dF[i] / dxi.
-__device__ __forceinline__ void qseries_d(const double2* Pq, int M, int ab, double cx, double sx, double2 (&F)[7],
-                                          double2 (&dF)[7]) {
-#pragma unroll
-  for (int i = 0; i < 7; ++i) { F[i] = make_double2(0.0, 0.0); dF[i] = F[i]; }
-  const double2* ptt = Pq + (0 * 4 + ab) * (M + 1);
-  const double2* ptf = Pq + (1 * 4 + ab) * (M + 1);
-  const double2* pft = Pq + (2 * 4 + ab) * (M + 1);
-  const double2* pff = Pq + (3 * 4 + ab) * (M + 1);
-  double c = 1.0, s = 0.0, cp = cx, sp = -sx;
-  const double tc = 2.0 * cx;
-  for (int m = 0; m <= M; ++m) {
-    const double2 a = ptt[m], b = ptf[m], g = pft[m], d = pff[m];
-    const double fm = (double)m;
-    const double mc = fm * c, ms = fm * s, m2c = fm * mc, m2s = fm * ms, m3s = fm * m2s;
-    // Ct = sum a cos, Ct' = -sum m a sin
-    F[0].x += a.x * c;   F[0].y += a.y * c;   dF[0].x -= a.x * ms;  dF[0].y -= a.y * ms;
-    // Sf = sum b sin, Sf' = Mf = sum m b cos, Mf' = -sum m^2 b sin
-    F[1].x += b.x * s;   F[1].y += b.y * s;   dF[1].x += b.x * mc;  dF[1].y += b.y * mc;
-    F[2].x += b.x * mc;  F[2].y += b.y * mc;  dF[2].x -= b.x * m2s; dF[2].y -= b.y * m2s;
-    F[3].x += g.x * s;   F[3].y += g.y * s;   dF[3].x += g.x * mc;  dF[3].y += g.y * mc;
-    F[4].x += g.x * mc;  F[4].y += g.y * mc;  dF[4].x -= g.x * m2s; dF[4].y -= g.y * m2s;
-    // Cf = sum d cos, Nf = sum m^2 d cos
-    F[5].x += d.x * c;   F[5].y += d.y * c;   dF[5].x -= d.x * ms;  dF[5].y -= d.y * ms;
-    F[6].x += d.x * m2c; F[6].y += d.y * m2c; dF[6].x -= d.x * m3s; dF[6].y -= d.y * m3s;
-    const double cn = tc * c - cp, sn = tc * s - sp;
-    cp = c; sp = s;
-    c = cn; s = sn;
-  }
-}
-
-// One 256-thread work-group per (near pair, 32 cell point pairs): thread =
-// (point-pair slot, node lane); lanes walk the pair's 48 core and n_tail tail
-// nodes with stride 8.
+// One 256-thread work-group per (near pair, up to 32 cell point pairs).  The
+// point pairs are taken one at a time.  For each, the series coefficients are
+// first contracted over the four basis entries ab with the pair's weights
+// (T_a T_b for the TT sums; value and four partials of D_a D_b, D_a T_b and
+// T_a D_b for the DD, DT and TD sums) into 19 arrays over m in LDS; then one
+// thread per azimuth node (the pair's 48 core and n_tail tail nodes) evaluates
+// the 28 series it needs from those arrays (values and xi slopes), instead of
+// four series evaluations per node, one per entry.
 __global__ __launch_bounds__(256) void k_adj_near(AdjNearArgs a) {
   const int2 bt = a.batches[blockIdx.x];
   const AdjNearPair pr = a.pairs[bt.x];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   extern __shared__ double2 lds[];
-  double2* Pq = lds;   // [16][M+1]
+  const int M1 = a.M + 1;
+  double2* Pq = lds;             // [16][M+1]
+  double2* Cq = lds + 16 * M1;   // [19][M+1], the current point pair's contracted coefficients
   if (pr.e >= 0) stage_q(a.sys, a.Q, a.M, pr.e, pr.f, Pq);
   __syncthreads();
+  const int tid = threadIdx.x;
   if (pr.e >= 0) {
-    const int slot = threadIdx.x >> 3, cs = threadIdx.x & 7;
-    const int q = bt.y + slot;
     const int cnt = a.cell_cnt[pr.kind];
+    const int npp = min(32, cnt - bt.y);
     const double k = a.k, ik2 = 1.0 / (k * k);
     const DElem Ee = delem(a.nodes, pr.e, 0), Ef = delem(a.nodes, pr.f, 2);
-    // the 32 point pairs' duals and their azimuth rules in LDS (written by
-    // lane 0 of each slot, read by its 8 node lanes): held in registers across
-    // the node loop they would crowd out the series accumulators
+    // the point pairs' duals and their azimuth rules, one thread each
     struct DRule {
       D4 d, aa, doa, vmax, xi0, span;
       int ok;
     };
     __shared__ DPair sP[32];
     __shared__ DRule sR[32];
-    if (cs == 0 && q < cnt) {
-      const int qi = a.cell_off[pr.kind] + q;
+    if (tid < npp) {
+      const int qi = a.cell_off[pr.kind] + bt.y + tid;
       const DPoint p = dpoint(Ee, a.cell_s[qi]), qq = dpoint(Ef, a.cell_sp[qi]);
       const DPair P = make_pair(Ee, Ef, p, qq, a.cell_w[qi]);
-      sP[slot] = P;
+      sP[tid] = P;
       const D4 dr = P.rp - P.rq;
       const D4 d2 = dr * dr + P.dz * P.dz;
       const D4 rr4 = 4.0 * (P.rp * P.rq);
@@ -611,148 +587,171 @@ __global__ __launch_bounds__(256) void k_adj_near(AdjNearArgs a) {
       r.vmax = dasinh(s0 / r.doa);
       r.xi0 = 2.0 * dasin(dmin(s0, 1.0));
       r.span = GH_PI - r.xi0;
-      sR[slot] = r;
+      sR[tid] = r;
     }
     __syncthreads();
-    if (q < cnt && sR[slot].ok) {
-      const DPair& P = sP[slot];
-      {
-        const D4& d = sR[slot].d;
-        const D4& aa = sR[slot].aa;
-        const D4& doa = sR[slot].doa;
-        const D4& vmax = sR[slot].vmax;
-        const D4& xi0 = sR[slot].xi0;
-        const D4& span = sR[slot].span;
-        const int nnodes = 48 + pr.n_tail;
-        C4 accE = cdz();
-        D4 accK = dconst(0.0);
-        for (int vn = cs; vn < nnodes; vn += 8) {
-          D4 wq, R, cx, sx;
-          if (vn < 48) {
-            const D4 vv = a.core_u[vn] * vmax, wv = a.core_w[vn] * vmax;
-            D4 sv = doa * dsinh(vv);
-            sv = dmin(sv, 1.0);
-            const D4 cv = dsqrt(dmax(1.0 - sv * sv, 0.0));
-            const D4 den = cv.v >= sqrt(1e-15) ? cv : dconst(sqrt(1e-15));
-            wq = (2.0 * wv) * doa * dcosh(vv) / den;
-            R = dsqrt(d * d + (aa * sv) * (aa * sv));
-            sx = 2.0 * (sv * cv);
-            cx = 1.0 - 2.0 * (sv * sv);
-          } else {
-            const int ti = pr.tail_off + (vn - 48);
-            const D4 xi = xi0 + a.tail_u[ti] * span;
-            wq = a.tail_w[ti] * span;
-            D4 sh, ch;
-            dsincos(0.5 * xi, &sh, &ch);
-            R = dsqrt(d * d + (aa * sh) * (aa * sh));
-            sx = 2.0 * (sh * ch);
-            cx = 1.0 - 2.0 * (sh * sh);
-          }
-          R = dmax(R, 1e-300);
-          // xi of the node and its partials: dxi = cos xi dsin xi - sin xi dcos xi
-          D4 dxi;
-          dxi.v = 0.0;
+    C4 accE = cdz();
+    D4 accK = dconst(0.0);
+    const int nnodes = 48 + pr.n_tail;
+    for (int sl = 0; sl < npp; ++sl) {
+      if (!sR[sl].ok) continue;   // the same for every thread: the barriers below stay matched
+      const DPair& P = sP[sl];
+      // contracted coefficients: arrays 0-3 the tt, tf, ft, ff blocks with T_a T_b;
+      // 4-8 tt with D_a D_b, 9-13 tf with D_a T_b, 14-18 ft with T_a D_b (value, partials)
+      for (int i = tid; i < 19 * M1; i += 256) {
+        const int arr = i / M1, m = i - arr * M1;
+        const double Tp[2] = {P.Tp0, P.Tp1}, Tq[2] = {P.Tq0, P.Tq1};
+        double2 z = make_double2(0.0, 0.0);
+        if (arr < 4) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) dxi.d[i] = cx.v * sx.d[i] - sx.v * cx.d[i];
-          // The series of each entry at xi (values F and xi slopes dF),
-          // contracted over the entries as they come: the T-weighted sums
-          // TT(F) are a constant plus a slope times dxi; the D-weighted sums
-          // DD(Ct), DT(Mf), TD(Mg) keep value, the four weight partials and
-          // the slope sum.
-          const double Tp[2] = {P.Tp0, P.Tp1}, Tq[2] = {P.Tq0, P.Tq1};
-          double2 ttv[5], ttd[5];          // Ct, Sf, Sg, Cf, Nf
-          double2 xv[3], xd[3][4], xs[3];  // DD(Ct), DT(Mf), TD(Mg)
+          for (int ab = 0; ab < 4; ++ab) z = c2add(z, c2scale(Pq[(arr * 4 + ab) * M1 + m], Tp[ab >> 1] * Tq[ab & 1]));
+        } else {
+          const int grp = (arr - 4) / 5, comp = (arr - 4) - 5 * grp;
 #pragma unroll
-          for (int i = 0; i < 5; ++i) { ttv[i] = make_double2(0, 0); ttd[i] = ttv[i]; }
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            xv[i] = make_double2(0, 0); xs[i] = xv[i];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) xd[i][j] = xv[i];
-          }
-#pragma unroll 1
           for (int ab = 0; ab < 4; ++ab) {
             const int x = ab >> 1, y = ab & 1;
-            double2 F[7], dF[7];
-            qseries_d(Pq, a.M, ab, cx.v, sx.v, F, dF);
-            const double tt = Tp[x] * Tq[y];
-            const int tix[5] = {0, 1, 3, 5, 6};
-#pragma unroll
-            for (int i = 0; i < 5; ++i) {
-              ttv[i] = c2add(ttv[i], c2scale(F[tix[i]], tt));
-              ttd[i] = c2add(ttd[i], c2scale(dF[tix[i]], tt));
-            }
-            const D4 W3[3] = {(x == 0 ? P.Dp0 : P.Dp1) * (y == 0 ? P.Dq0 : P.Dq1), Tq[y] * (x == 0 ? P.Dp0 : P.Dp1),
-                              Tp[x] * (y == 0 ? P.Dq0 : P.Dq1)};
-            const int xix[3] = {0, 2, 4};
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              const double2 f = F[xix[i]], df = dF[xix[i]];
-              xv[i] = c2add(xv[i], c2scale(f, W3[i].v));
-              xs[i] = c2add(xs[i], c2scale(df, W3[i].v));
-#pragma unroll
-              for (int j = 0; j < 4; ++j) xd[i][j] = c2add(xd[i][j], c2scale(f, W3[i].d[j]));
-            }
-          }
-          auto tdual = [&](int i) -> C4 {
-            return C4{dchain(dxi, ttv[i].x, ttd[i].x), dchain(dxi, ttv[i].y, ttd[i].y)};
-          };
-          auto xdual = [&](int i) -> C4 {
-            C4 r;
-            r.re.v = xv[i].x;
-            r.im.v = xv[i].y;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              r.re.d[j] = xd[i][j].x + xs[i].x * dxi.d[j];
-              r.im.d[j] = xd[i][j].y + xs[i].y * dxi.d[j];
-            }
-            return r;
-          };
-          const C4 TTc = tdual(0), TTs = tdual(1), TTg = tdual(2), TTf = tdual(3);
-          // EFIE: 2 wq g(R) Phi (Phi as in efie_phi, with dual series)
-          {
-            const C4 g = green(R, k);
-            const C4 TTn = tdual(4), DD = xdual(0), DT = xdual(1), TD = xdual(2);
-            const D4 A1 = cx * (P.tpr * P.tqr) + P.tpz * P.tqz;
-            C4 u = A1 * TTc + cx * TTf;
-            const C4 w = P.tpr * TTs - P.tqr * TTg;   // - j sx w
-            u.re = u.re + sx * w.im;
-            u.im = u.im - sx * w.re;
-            const C4 xx = DT - TD;
-            C4 v = DD + TTn;
-            v.re = v.re - xx.im;
-            v.im = v.im + xx.re;
-            const C4 phi = P.rr * u - (ik2 * P.ww) * v;
-            accE = accE + (2.0 * wq) * (g * phi);
-          }
-          // brackets: 2 wq rr p(R) [b_tt TT(Ct) - j b_tf TT(Sf) - j b_ft TT(Sg) + b_ff TT(Cf)]
-          if (a.cm != 0.0) {
-            D4 b[4], Rb;
-            brackets(P, cx, sx, b, &Rb);
-            const C4 pk = pkern(R, k);
-            const C4 tc = TTc, ts = TTs, tg = TTg, tf = TTf;
-            // -j ts = (ts.im, -ts.re)
-            const C4 U = b[0] * tc + b[1] * C4{ts.im, -ts.re} + b[2] * C4{tg.im, -tg.re} + b[3] * tf;
-            const C4 pu = pk * U;
-            accK = accK + ((2.0 * wq) * P.rr) * pu.re;
+            const D4& Dpx = x == 0 ? P.Dp0 : P.Dp1;
+            const D4& Dqy = y == 0 ? P.Dq0 : P.Dq1;
+            const D4 w = grp == 0 ? Dpx * Dqy : (grp == 1 ? Tq[y] * Dpx : Tp[x] * Dqy);
+            const double wc = comp == 0 ? w.v : w.d[comp - 1];
+            z = c2add(z, c2scale(Pq[(grp * 4 + ab) * M1 + m], wc));
           }
         }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) acc[i] = a.ce_re * accE.re.d[i] - a.ce_im * accE.im.d[i] + a.cm * accK.d[i];
+        Cq[i] = z;
       }
+      __syncthreads();
+      const D4& d = sR[sl].d;
+      const D4& aa = sR[sl].aa;
+      const D4& doa = sR[sl].doa;
+      const D4& vmax = sR[sl].vmax;
+      const D4& xi0 = sR[sl].xi0;
+      const D4& span = sR[sl].span;
+      for (int vn = tid; vn < nnodes; vn += 256) {
+        D4 wq, R, cx, sx;
+        if (vn < 48) {
+          const D4 vv = a.core_u[vn] * vmax, wv = a.core_w[vn] * vmax;
+          D4 sv = doa * dsinh(vv);
+          sv = dmin(sv, 1.0);
+          const D4 cv = dsqrt(dmax(1.0 - sv * sv, 0.0));
+          const D4 den = cv.v >= sqrt(1e-15) ? cv : dconst(sqrt(1e-15));
+          wq = (2.0 * wv) * doa * dcosh(vv) / den;
+          R = dsqrt(d * d + (aa * sv) * (aa * sv));
+          sx = 2.0 * (sv * cv);
+          cx = 1.0 - 2.0 * (sv * sv);
+        } else {
+          const int ti = pr.tail_off + (vn - 48);
+          const D4 xi = xi0 + a.tail_u[ti] * span;
+          wq = a.tail_w[ti] * span;
+          D4 sh, ch;
+          dsincos(0.5 * xi, &sh, &ch);
+          R = dsqrt(d * d + (aa * sh) * (aa * sh));
+          sx = 2.0 * (sh * ch);
+          cx = 1.0 - 2.0 * (sh * sh);
+        }
+        R = dmax(R, 1e-300);
+        // xi of the node and its partials: dxi = cos xi dsin xi - sin xi dcos xi
+        D4 dxi;
+        dxi.v = 0.0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dxi.d[i] = cx.v * sx.d[i] - sx.v * cx.d[i];
+        // the series at xi: TT sums of Ct, Sf, Sg, Cf, Nf (values and slopes),
+        // DD(Ct), DT(Mf), TD(Mg) (value and four weight partials, and the
+        // slope of the value)
+        const double2 z0 = make_double2(0.0, 0.0);
+        double2 tc = z0, tcs = z0, ts = z0, tss = z0, tg = z0, tgs = z0, tf = z0, tfs = z0, tn = z0, tns = z0;
+        double2 dd[5], dt[5], td[5], dds = z0, dts = z0, tds = z0;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) { dd[j] = z0; dt[j] = z0; td[j] = z0; }
+        double c = 1.0, s = 0.0, cp = cx.v, sp = -sx.v;
+        const double tcx = 2.0 * cx.v;
+        // two orders per step: the next order's 19 LDS loads are in flight
+        // while this order's sums are formed (at one wavefront per SIMD nothing
+        // else hides their latency)
+#pragma unroll 2
+        for (int m = 0; m <= a.M; ++m) {
+          const double fm = (double)m;
+          const double mc = fm * c, ms = fm * s, m2c = fm * mc, m2s = fm * ms, m3s = fm * m2s;
+          const double2* cm = Cq + m;
+          const double2 Att = cm[0], Atf = cm[M1], Aft = cm[2 * M1], Aff = cm[3 * M1];
+          tc = c2add(tc, c2scale(Att, c));    tcs = c2add(tcs, c2scale(Att, -ms));
+          ts = c2add(ts, c2scale(Atf, s));    tss = c2add(tss, c2scale(Atf, mc));
+          tg = c2add(tg, c2scale(Aft, s));    tgs = c2add(tgs, c2scale(Aft, mc));
+          tf = c2add(tf, c2scale(Aff, c));    tfs = c2add(tfs, c2scale(Aff, -ms));
+          tn = c2add(tn, c2scale(Aff, m2c));  tns = c2add(tns, c2scale(Aff, -m3s));
+#pragma unroll
+          for (int j = 0; j < 5; ++j) {
+            const double2 B = cm[(4 + j) * M1], Cc = cm[(9 + j) * M1], D = cm[(14 + j) * M1];
+            dd[j] = c2add(dd[j], c2scale(B, c));
+            dt[j] = c2add(dt[j], c2scale(Cc, mc));
+            td[j] = c2add(td[j], c2scale(D, mc));
+            if (j == 0) {
+              dds = c2add(dds, c2scale(B, -ms));
+              dts = c2add(dts, c2scale(Cc, -m2s));
+              tds = c2add(tds, c2scale(D, -m2s));
+            }
+          }
+          const double cn = tcx * c - cp, sn = tcx * s - sp;
+          cp = c; sp = s;
+          c = cn; s = sn;
+        }
+        auto tdual = [&](double2 v, double2 sv) -> C4 {
+          return C4{dchain(dxi, v.x, sv.x), dchain(dxi, v.y, sv.y)};
+        };
+        auto xdual = [&](const double2 (&x)[5], double2 xs) -> C4 {
+          C4 r;
+          r.re.v = x[0].x;
+          r.im.v = x[0].y;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            r.re.d[j] = x[j + 1].x + xs.x * dxi.d[j];
+            r.im.d[j] = x[j + 1].y + xs.y * dxi.d[j];
+          }
+          return r;
+        };
+        const C4 TTc = tdual(tc, tcs), TTs = tdual(ts, tss), TTg = tdual(tg, tgs), TTf = tdual(tf, tfs);
+        // EFIE: 2 wq g(R) Phi (Phi as in efie_phi, with dual series)
+        {
+          const C4 g = green(R, k);
+          const C4 TTn = tdual(tn, tns), DD = xdual(dd, dds), DT = xdual(dt, dts), TD = xdual(td, tds);
+          const D4 A1 = cx * (P.tpr * P.tqr) + P.tpz * P.tqz;
+          C4 u = A1 * TTc + cx * TTf;
+          const C4 w = P.tpr * TTs - P.tqr * TTg;   // - j sx w
+          u.re = u.re + sx * w.im;
+          u.im = u.im - sx * w.re;
+          const C4 xx = DT - TD;
+          C4 v = DD + TTn;
+          v.re = v.re - xx.im;
+          v.im = v.im + xx.re;
+          const C4 phi = P.rr * u - (ik2 * P.ww) * v;
+          accE = accE + (2.0 * wq) * (g * phi);
+        }
+        // brackets: 2 wq rr p(R) [b_tt TT(Ct) - j b_tf TT(Sf) - j b_ft TT(Sg) + b_ff TT(Cf)]
+        if (a.cm != 0.0) {
+          D4 b[4], Rb;
+          brackets(P, cx, sx, b, &Rb);
+          const C4 pk = pkern(R, k);
+          // -j ts = (ts.im, -ts.re)
+          const C4 U = b[0] * TTc + b[1] * C4{TTs.im, -TTs.re} + b[2] * C4{TTg.im, -TTg.re} + b[3] * TTf;
+          const C4 pu = pk * U;
+          accK = accK + ((2.0 * wq) * P.rr) * pu.re;
+        }
+      }
+      __syncthreads();   // every thread is done with Cq before the next pair's contraction
     }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = a.ce_re * accE.re.d[i] - a.ce_im * accE.im.d[i] + a.cm * accK.d[i];
     // mass term (1/2) cM M on the tt and ff blocks of a diagonal pair, once per pair
     // (bor_solver.py:389-397): pi cM sum_p w_p rho_p T_a T_b (Q_tt + Q_ff summed over m)
-    if (pr.e == pr.f && bt.y == 0 && threadIdx.x < 4 && a.cmass != 0.0) {
-      const DPoint p = dpoint(Ee, a.gs[threadIdx.x]);
-      const D4 wr = (a.gwq[threadIdx.x] * Ee.L) * p.rho;
+    if (pr.e == pr.f && bt.y == 0 && tid < 4 && a.cmass != 0.0) {
+      const DPoint p = dpoint(Ee, a.gs[tid]);
+      const D4 wr = (a.gwq[tid] * Ee.L) * p.rho;
       const double T[2] = {p.T0, p.T1};
       double2 qs = make_double2(0.0, 0.0);
       for (int ab = 0; ab < 4; ++ab) {
         double2 st = make_double2(0.0, 0.0);
         for (int m = 0; m <= a.M; ++m) {
-          st = c2add(st, Pq[(0 * 4 + ab) * (a.M + 1) + m]);
-          st = c2add(st, Pq[(3 * 4 + ab) * (a.M + 1) + m]);
+          st = c2add(st, Pq[(0 * 4 + ab) * M1 + m]);
+          st = c2add(st, Pq[(3 * 4 + ab) * M1 + m]);
         }
         qs = c2add(qs, c2scale(st, T[ab >> 1] * T[ab & 1]));
       }
