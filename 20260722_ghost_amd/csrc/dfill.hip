@@ -326,10 +326,8 @@ __device__ __forceinline__ C4 efie_phi(const DPair& P, const double2* Ct, const 
   return P.rr * u - (ik2 * P.ww) * v;
 }
 
-// e^{-jkR} / (4 pi R) as a complex dual of R
-__device__ __forceinline__ C4 green(const D4& R, double k) {
-  double s, c;
-  sincos(k * R.v, &s, &c);
+// e^{-jkR} / (4 pi R) as a complex dual of R (s, c = sin, cos of k R)
+__device__ __forceinline__ C4 green(const D4& R, double k, double s, double c) {
   const double i4 = 1.0 / (4.0 * GH_PI * R.v);
   const double gr = c * i4, gi = -s * i4;
   // dg/dR = g (-jk - 1/R)
@@ -337,19 +335,27 @@ __device__ __forceinline__ C4 green(const D4& R, double k) {
   const double pr = -gr * ir + gi * k, pi = -gi * ir - gr * k;
   return C4{dchain(R, gr, pr), dchain(R, gi, pi)};
 }
+__device__ __forceinline__ C4 green(const D4& R, double k) {
+  double s, c;
+  sincos(k * R.v, &s, &c);
+  return green(R, k, s, c);
+}
 
 // p(R) = (1 + jkR) e^{-jkR} / (4 pi R^3) as a complex dual of R;
 // p'(R) = [k^2 R e^{-jkR} - 3 (1 + jkR) e^{-jkR} / R] / (4 pi R^3)
-__device__ __forceinline__ C4 pkern(const D4& R, double k) {
+__device__ __forceinline__ C4 pkern(const D4& R, double k, double s, double c) {
   const double r = R.v;
-  double s, c;
-  sincos(k * r, &s, &c);
   const double i4 = 1.0 / (4.0 * GH_PI * r * r * r);
   // h = (1 + jkr)(c - j s) = (c + k r s) + j (k r c - s)
   const double hr = c + k * r * s, hi = k * r * c - s;
   const double pr = hr * i4, pi = hi * i4;
   const double dr = (k * k * r * c - 3.0 * hr / r) * i4, di = (-k * k * r * s - 3.0 * hi / r) * i4;
   return C4{dchain(R, pr, dr), dchain(R, pi, di)};
+}
+__device__ __forceinline__ C4 pkern(const D4& R, double k) {
+  double s, c;
+  sincos(k * R.v, &s, &c);
+  return pkern(R, k, s, c);
 }
 
 // The four MFIE brackets (without p(R)) at one xi (bor_kernels.py:273-306):
@@ -709,9 +715,11 @@ __global__ __launch_bounds__(256) void k_adj_near(AdjNearArgs a) {
           return r;
         };
         const C4 TTc = tdual(tc, tcs), TTs = tdual(ts, tss), TTg = tdual(tg, tgs), TTf = tdual(tf, tfs);
+        double skr, ckr;   // e^{-jkR} of the node, shared by g(R) and p(R)
+        sincos(k * R.v, &skr, &ckr);
         // EFIE: 2 wq g(R) Phi (Phi as in efie_phi, with dual series)
         {
-          const C4 g = green(R, k);
+          const C4 g = green(R, k, skr, ckr);
           const C4 TTn = tdual(tn, tns), DD = xdual(dd, dds), DT = xdual(dt, dts), TD = xdual(td, tds);
           const D4 A1 = cx * (P.tpr * P.tqr) + P.tpz * P.tqz;
           C4 u = A1 * TTc + cx * TTf;
@@ -729,7 +737,7 @@ __global__ __launch_bounds__(256) void k_adj_near(AdjNearArgs a) {
         if (a.cm != 0.0) {
           D4 b[4], Rb;
           brackets(P, cx, sx, b, &Rb);
-          const C4 pk = pkern(R, k);
+          const C4 pk = pkern(R, k, skr, ckr);
           // -j ts = (ts.im, -ts.re)
           const C4 U = b[0] * TTc + b[1] * C4{TTs.im, -TTs.re} + b[2] * C4{TTg.im, -TTg.re} + b[3] * TTf;
           const C4 pu = pk * U;
