@@ -470,6 +470,7 @@ __global__ __launch_bounds__(256) void k_adj_far(AdjFarArgs a) {
         }
       }
       __syncthreads();
+#pragma unroll 2
       for (int sl = tid >> 4; sl < ADJ_CHUNK; sl += 16) {
         const int js = j0 + sl;
         if (js >= J) break;
