@@ -2281,7 +2281,7 @@ static int fill_problem(ghost_bor* h, Problem& p, const Geom& tg, const Geom& sg
   aa.mlo = h->pairing ? p.M : 0;
   aa.zmodes = p.zmodes;
   aa.zoff = p.zoff;
-  dim3 g(2 * p.Nn, (2 * p.Nn + 63) / 64, (p.nmodes - aa.mlo + 31) / 32);
+  dim3 g(2 * p.Nn, (2 * p.Nn + 31) / 32, (p.nmodes - aa.mlo + 31) / 32);
   prof_begin(h);
   if (mixed) hipLaunchKernelGGL((k_assemble<float, float>), g, dim3(256), 0, h->stream, aa);
   else hipLaunchKernelGGL((k_assemble<double, double>), g, dim3(256), 0, h->stream, aa);

@@ -1062,21 +1062,19 @@ struct AsmArgs {
   double2* A64;              // optional FP64 copy of the n x n block (nullptr if unused)
 };
 
-// CW node-level columns x 32 modes per work-group: each mode row of the
-// tile is written as a run of CW entries
-template <typename Rf, typename Ra, int CW = 64>
+template <typename Rf, typename Ra>
 __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
   typedef typename CT<Rf>::c Cf;
   typedef typename CT<Ra>::c Ca;
-  __shared__ double2 tile[CW][33];
+  __shared__ double2 tile[32][33];
   const int i_row = blockIdx.x;            // node-level row in [0, 2Nn)
-  const int j0 = blockIdx.y * CW;          // node-level column chunk
+  const int j0 = blockIdx.y * 32;          // node-level column chunk
   const int m0 = a.mlo + blockIdx.z * 32;  // mode chunk (index into [0, nmodes))
   const int nmodes = 2 * a.M + 1;
   const size_t ncol = 2 * (size_t)a.Nn;
   const Cf* Z = (const Cf*)a.Zfar;
-  // load [CW cols][32 modes], modes fastest
-  for (int t = threadIdx.x; t < CW * 32; t += 256) {
+  // load [32 cols][32 modes], modes fastest
+  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
     const int jj = t >> 5, mm = t & 31;
     const int j = j0 + jj, mi = m0 + mm;
     double2 v = make_double2(0, 0);
@@ -1089,8 +1087,8 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
   __syncthreads();
   const int bi = i_row >= a.Nn ? 1 : 0;
   const int ii = i_row - bi * a.Nn;
-  for (int t = threadIdx.x; t < CW * 32; t += 256) {
-    const int mm = t / CW, jj = t % CW;
+  for (int t = threadIdx.x; t < 32 * 32; t += 256) {
+    const int mm = t >> 5, jj = t & 31;
     const int j = j0 + jj, mi = m0 + mm;
     if (j >= (int)ncol || mi >= nmodes) continue;
     const SysInfo s = a.sys[mi];
